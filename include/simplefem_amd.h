@@ -1,0 +1,202 @@
+/*
+ * simplefem_amd.h -- C-ABI of the MI355X (gfx950) sparse CG / SSOR solve path.
+ *
+ * The entry points replace FEEC++'s raw-pointer CSR solvers and the CSR
+ * operations they rely on.  Each declaration names the reference interface it
+ * stands in for (paths relative to the martinlicht/simplefem repository).
+ *
+ * Conventions
+ *   - Plain C types only: int, double, pointers and sizes.
+ *   - Every function returns an int status: SFA_OK (0) or a negative SFA_ERR_*;
+ *     sfa_last_error() gives the message of the calling thread's last failure.
+ *   - The reference returns the iteration count K from its solvers; here K and
+ *     the convergence data come back in an sfa_solve_info.
+ *   - Matrices are square CSR with int32 row offsets (N+1), int32 column
+ *     indices and double values, exactly the arrays MatrixCSR::getA/getC/getV
+ *     hand to the reference solvers (sparse/matcsr.hpp:89-93).
+ *   - "Host" entry points take host arrays, copy them to the GPU, solve there
+ *     and copy x and residual back: the semantics of the reference call.
+ *     The sfa_system_* handle API keeps the matrix and preconditioner
+ *     resident in device memory and can take device pointers.
+ */
+#ifndef SIMPLEFEM_AMD_H
+#define SIMPLEFEM_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SFA_API_VERSION 1
+
+/* status codes */
+#define SFA_OK                 0
+#define SFA_ERR_INVALID       -1  /* bad argument (sizes, pointers, CSR)       */
+#define SFA_ERR_HIP           -2  /* a HIP runtime call failed                 */
+#define SFA_ERR_NO_DEVICE     -3  /* no usable gfx950 device                   */
+#define SFA_ERR_OUT_OF_MEMORY -4  /* device or host allocation failed          */
+#define SFA_ERR_COLOURING     -5  /* F/B/R inconsistent, or a colour holds an  */
+                                  /* edge (the sweep would race)               */
+
+/* exit reasons of the CG loop; the reference logs these
+ * (solver/sparsesolver.cpp:601-604,629-642,713-714) */
+#define SFA_EXIT_CONVERGED 0  /* |z.r| < tolerance^2                          */
+#define SFA_EXIT_CAP       1  /* K reached N                                  */
+#define SFA_EXIT_BREAKDOWN 2  /* z.r or d.Ad non-finite or negative           */
+#define SFA_EXIT_SMALL     3  /* sqrt|d.Ad| < machine epsilon                 */
+
+/* solver variants (one per function of solver/sparsesolver.hpp:11-84) */
+#define SFA_METHOD_CG                  0
+#define SFA_METHOD_DIAGONAL            1
+#define SFA_METHOD_SSOR                2
+#define SFA_METHOD_SSOR_EISENSTAT      3
+#define SFA_METHOD_RAINBOW             4
+#define SFA_METHOD_EISENSTAT_RAINBOW   5
+
+typedef struct sfa_solve_info {
+    int    iterations;   /* K, what the reference solver returns             */
+    int    success;      /* 1 iff sqrt(z.r) < tolerance at exit ("SUCCESS")  */
+    int    exit_reason;  /* SFA_EXIT_*                                       */
+    int    reserved;
+    double zr;           /* final z.r (r.r for unpreconditioned CG)          */
+    double tolerance;    /* max(precision*||b||_2, sqrt(DBL_EPSILON))        */
+} sfa_solve_info;
+
+/* Message of the calling thread's last failed call ("" if none). */
+const char* sfa_last_error(void);
+
+/* SFA_API_VERSION of the built library. */
+int sfa_api_version(void);
+
+/* ---------------------------------------------------------------------------
+ * Drop-in solvers on host arrays.  Same arguments, same meaning and the same
+ * in/out contract for x (initial guess in, solution out) and residual (rows
+ * that a variant skips keep their input) as the reference.  print_modulo
+ * selects the reference's log lines (>= -1 begin/finish, >= 0 restart,
+ * >= 1 every k-th iteration, >= -2 breakdown/warning), printed to stdout.
+ * ------------------------------------------------------------------------- */
+
+/* ConjugateGradientSolverCSR, solver/sparsesolver.hpp:11-19 */
+int sfa_ConjugateGradientSolverCSR(
+    int N, double* x, const double* b,
+    const int* csrrows, const int* csrcolumns, const double* csrvalues,
+    double* residual, double precision, int print_modulo,
+    sfa_solve_info* info);
+
+/* ConjugateGradientSolverCSR_DiagonalPreconditioner, solver/sparsesolver.hpp:21-30 */
+int sfa_ConjugateGradientSolverCSR_DiagonalPreconditioner(
+    int N, double* x, const double* b,
+    const int* csrrows, const int* csrcolumns, const double* csrvalues,
+    double* residual, double precision, int print_modulo,
+    const double* precon, sfa_solve_info* info);
+
+/* ConjugateGradientSolverCSR_SSOR, solver/sparsesolver.hpp:33-43 */
+int sfa_ConjugateGradientSolverCSR_SSOR(
+    int N, double* x, const double* b,
+    const int* csrrows, const int* csrcolumns, const double* csrvalues,
+    double* residual, double precision, int print_modulo,
+    const double* diagonal, double omega, sfa_solve_info* info);
+
+/* ConjugateGradientSolverCSR_SSOR_Eisenstat, solver/sparsesolver.hpp:46-56 */
+int sfa_ConjugateGradientSolverCSR_SSOR_Eisenstat(
+    int N, double* x, const double* b,
+    const int* csrrows, const int* csrcolumns, const double* csrvalues,
+    double* residual, double precision, int print_modulo,
+    const double* diagonal, double omega, sfa_solve_info* info);
+
+/* ConjugateGradientSolverCSR_Rainbow, solver/sparsesolver.hpp:59-70 */
+int sfa_ConjugateGradientSolverCSR_Rainbow(
+    int N, double* x, const double* b,
+    const int* csrrows, const int* csrcolumns, const double* csrvalues,
+    double* residual, double precision, int print_modulo,
+    const double* diagonal, double omega,
+    int num_colors, const int* F, const int* B, const int* R,
+    sfa_solve_info* info);
+
+/* ConjugateGradientSolverCSR_Eisenstat_Rainbow, solver/sparsesolver.hpp:73-84 */
+int sfa_ConjugateGradientSolverCSR_Eisenstat_Rainbow(
+    int N, double* x, const double* b,
+    const int* csrrows, const int* csrcolumns, const double* csrvalues,
+    double* residual, double precision, int print_modulo,
+    const double* diagonal, double omega,
+    int num_colors, const int* F, const int* B, const int* R,
+    sfa_solve_info* info);
+
+/* ---------------------------------------------------------------------------
+ * CSR operations
+ * ------------------------------------------------------------------------- */
+
+/* MatrixCSR::apply, sparse/matcsr.hpp:73 / sparse/matcsr.cpp:216-244:
+ * dest = scaling * A * src, with dest zeroed first.  Host arrays; the product
+ * runs on the GPU. */
+int sfa_MatrixCSR_apply(int rows, int columns,
+                        const int* A, const int* C, const double* V,
+                        double* dest, const double* src, double scaling);
+
+/* MatrixCSR::getDiagonal, sparse/matcsr.hpp:82 / sparse/matcsr.cpp:266-286:
+ * diag[r] = sum of the entries of row r in column r (host). */
+int sfa_MatrixCSR_getDiagonal(int n, const int* A, const int* C,
+                              const double* V, double* diag);
+
+/* Rainbow::Rainbow, sparse/rainbow.hpp:15 / sparse/rainbow.cpp:14-130
+ * (do_shuffle = false): the reference's greedy colouring, bit for bit.
+ * F and R need n ints, B needs n+1 ints.  Host. */
+int sfa_Rainbow(int n, const int* A, const int* C,
+                int* F, int* B, int* R, int* num_colors);
+
+/* ---------------------------------------------------------------------------
+ * Device-resident systems.  No reference counterpart: the reference
+ * rebuilds nothing between calls because it keeps everything in host memory;
+ * these keep the matrix, the colour/level schedules and the Krylov
+ * workspace in HBM across solves.  sfa_system_create takes host CSR arrays
+ * plus the per-variant data of the matching drop-in solver (aux = precon
+ * for SFA_METHOD_DIAGONAL, diagonal otherwise; omega, num_colors/F/B/R
+ * only where the variant has them -- pass 0 / NULL elsewhere).
+ * ------------------------------------------------------------------------- */
+
+typedef struct sfa_system_s* sfa_system;
+
+int sfa_system_create(int method, int N,
+                      const int* csrrows, const int* csrcolumns,
+                      const double* csrvalues, const double* aux,
+                      double omega, int num_colors,
+                      const int* F, const int* B, const int* R,
+                      sfa_system* out);
+
+/* Solve with the system's matrix.  x, b, residual are N-vectors with the
+ * same meaning as in the drop-in call; device_pointers = 1 means they are
+ * device (HBM) addresses on the system's GPU, 0 that they are host arrays.
+ * Returns once the solve has finished. */
+int sfa_system_solve(sfa_system sys, double* x, const double* b,
+                     double* residual, double precision, int print_modulo,
+                     int device_pointers, sfa_solve_info* info);
+
+/* dest = scaling * A * src on the device (the system's SpMV kernel). */
+int sfa_system_apply(sfa_system sys, double* dest, const double* src,
+                     double scaling, int device_pointers);
+
+/* Run the solve on this HIP stream (a hipStream_t); NULL restores the
+ * system's own stream. */
+int sfa_system_set_stream(sfa_system sys, void* hip_stream);
+
+/* Plan facts: groups per sweep, stored entries, algorithmic bytes. */
+typedef struct sfa_system_stats {
+    int    n;                 /* rows                                        */
+    int    n_active;          /* rows the variant does not skip              */
+    long long nnz;            /* CSR entries                                 */
+    int    groups_forward;    /* colours / levels of the forward sweep       */
+    int    groups_backward;   /* colours / levels of the backward sweep      */
+    long long stored_entries; /* entries incl. SELL padding, all parts       */
+    double bytes_per_iteration; /* algorithmic HBM bytes of one CG step      */
+    double bytes_spmv;        /* algorithmic bytes of one SpMV launch        */
+    int    kernels_per_iteration;
+    int    reserved;
+} sfa_system_stats;
+
+int sfa_system_query(sfa_system sys, sfa_system_stats* stats);
+
+int sfa_system_destroy(sfa_system sys);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SIMPLEFEM_AMD_H */

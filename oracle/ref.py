@@ -1,0 +1,164 @@
+"""Calls into the reference's own compiled code (oracle/_ref/libfeecref.so).
+
+TEST INFRASTRUCTURE ONLY (checker and CPU baseline).  Each RefRunner owns
+one child process (oracle/_refworker.py) that loads the library with
+OMP_NUM_THREADS fixed, so the reference's start-up code (banner, flush-to-zero
+on every OpenMP thread, base/base.cpp:38-292) stays out of the calling
+process.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import pickle
+import struct
+import subprocess
+import sys
+from ctypes import POINTER, c_double, c_int
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+REF_LIB_PATH = os.path.join(HERE, "_ref", "libfeecref.so")
+
+_P_I = POINTER(c_int)
+_P_D = POINTER(c_double)
+_VARIANT = {"cg": 0, "jacobi": 1, "ssor": 2, "ssor_eisenstat": 3, "rainbow": 4,
+            "eisenstat_rainbow": 5}
+_lib = None
+
+
+def _init_library():
+    """Runs in the child process only."""
+    global _lib
+    _lib = ctypes.CDLL(REF_LIB_PATH)
+    _lib.feecref_desired_precision.restype = c_double
+    _lib.feecref_omp_threads.restype = c_int
+    _lib.feecref_rainbow.argtypes = [c_int, _P_I, _P_I, _P_D, _P_I, _P_I, _P_I]
+    _lib.feecref_rainbow.restype = c_int
+    _lib.feecref_csr_apply.argtypes = [c_int, _P_I, _P_I, _P_D, _P_D, _P_D, c_double]
+    _lib.feecref_csr_diagonal.argtypes = [c_int, _P_I, _P_I, _P_D, _P_D]
+    _lib.feecref_cg.argtypes = [c_int, c_int, _P_D, _P_D, _P_I, _P_I, _P_D, _P_D, c_double, c_int,
+                                _P_D, c_double, c_int, _P_I, _P_I, _P_I, _P_D]
+    _lib.feecref_cg.restype = c_int
+
+
+def _i(a):
+    return None if a is None else a.ctypes.data_as(_P_I)
+
+
+def _d(a):
+    return None if a is None else a.ctypes.data_as(_P_D)
+
+
+def _task(kind, kw):
+    L = _lib
+    if kind == "info":
+        return {"desired_precision": L.feecref_desired_precision(),
+                "threads": L.feecref_omp_threads()}
+    A = np.ascontiguousarray(kw["A"], dtype=np.int32)
+    C = np.ascontiguousarray(kw["C"], dtype=np.int32)
+    V = np.ascontiguousarray(kw["V"], dtype=np.float64)
+    n = A.shape[0] - 1
+    if kind == "rainbow":
+        F = np.empty(n, dtype=np.int32)
+        B = np.empty(n + 1, dtype=np.int32)
+        R = np.empty(n, dtype=np.int32)
+        nc = L.feecref_rainbow(n, _i(A), _i(C), _d(V), _i(F), _i(B), _i(R))
+        return nc, F, B[: nc + 1].copy(), R
+    if kind == "apply":
+        src = np.ascontiguousarray(kw["src"], dtype=np.float64)
+        dest = np.empty(n)
+        L.feecref_csr_apply(n, _i(A), _i(C), _d(V), _d(src), _d(dest), float(kw["scaling"]))
+        return dest
+    if kind == "diagonal":
+        d = np.empty(n)
+        L.feecref_csr_diagonal(n, _i(A), _i(C), _d(V), _d(d))
+        return d
+    if kind == "solve":
+        b = np.ascontiguousarray(kw["b"], dtype=np.float64)
+        x = np.array(kw["x0"], dtype=np.float64)
+        res = np.array(kw["residual0"], dtype=np.float64)
+        aux = kw.get("aux")
+        aux = None if aux is None else np.ascontiguousarray(aux, dtype=np.float64)
+        col = kw.get("colouring")
+        nc, F, B, R = (0, None, None, None) if col is None else col
+        F = None if F is None else np.ascontiguousarray(F, dtype=np.int32)
+        B = None if B is None else np.ascontiguousarray(B, dtype=np.int32)
+        R = None if R is None else np.ascontiguousarray(R, dtype=np.int32)
+        secs = c_double(0.0)
+        times = []
+        K = -1
+        for _ in range(kw.get("repeat", 1)):
+            xx, rr = x.copy(), res.copy()
+            K = L.feecref_cg(_VARIANT[kw["variant"]], n, _d(xx), _d(b), _i(A), _i(C), _d(V),
+                             _d(rr), float(kw["precision"]), int(kw["print_modulo"]), _d(aux),
+                             float(kw["omega"]), int(nc), _i(F), _i(B), _i(R), ctypes.byref(secs))
+            times.append(secs.value)
+        return xx, rr, K, times
+    raise ValueError(kind)
+
+
+class RefRunner:
+    """One child process running the reference library at `threads` threads."""
+
+    def __init__(self, threads: int = 1, timeout: float = 600.0):
+        if not os.path.exists(REF_LIB_PATH):
+            raise FileNotFoundError(REF_LIB_PATH)
+        self.threads = threads
+        self.timeout = timeout
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        self._proc = subprocess.Popen([sys.executable, os.path.join(HERE, "_refworker.py"),
+                                       str(threads)], stdin=subprocess.PIPE,
+                                      stdout=subprocess.PIPE, stderr=subprocess.DEVNULL,
+                                      env=env)
+
+    def _call(self, kind, **kw):
+        data = pickle.dumps((kind, kw), protocol=4)
+        self._proc.stdin.write(struct.pack("<Q", len(data)) + data)
+        self._proc.stdin.flush()
+        head = self._proc.stdout.read(8)
+        if len(head) < 8:
+            raise RuntimeError("reference worker exited (status %s)" % self._proc.poll())
+        (size,) = struct.unpack("<Q", head)
+        ok, out = pickle.loads(self._proc.stdout.read(size))
+        if not ok:
+            raise RuntimeError("reference worker: " + out)
+        return out
+
+    def info(self):
+        return self._call("info")
+
+    def rainbow(self, A, C, V):
+        return self._call("rainbow", A=A, C=C, V=V)
+
+    def csr_apply(self, A, C, V, src, scaling=1.0):
+        return self._call("apply", A=A, C=C, V=V, src=src, scaling=scaling)
+
+    def csr_diagonal(self, A, C, V):
+        return self._call("diagonal", A=A, C=C, V=V)
+
+    def solve(self, variant, A, C, V, b, x0=None, residual0=None, precision=1e-10,
+              print_modulo=-3, aux=None, omega=1.0, colouring=None, repeat=1):
+        """Returns (x, residual, K, [seconds per solve])."""
+        n = len(A) - 1
+        x0 = np.zeros(n) if x0 is None else x0
+        residual0 = np.zeros(n) if residual0 is None else residual0
+        return self._call("solve", variant=variant, A=A, C=C, V=V, b=b, x0=x0,
+                          residual0=residual0, precision=precision, print_modulo=print_modulo,
+                          aux=aux, omega=omega, colouring=colouring, repeat=repeat)
+
+    def close(self):
+        if self._proc.poll() is None:
+            self._proc.stdin.close()
+            try:
+                self._proc.wait(timeout=10)
+            except subprocess.TimeoutExpired:
+                self._proc.kill()
+                self._proc.wait()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

@@ -1,0 +1,324 @@
+"""ctypes binding of libsimplefem_amd.so and a Python mirror of the reference API.
+
+The functions below keep the names, argument order and in/out meaning of
+FEEC++'s raw-pointer CSR solvers (solver/sparsesolver.hpp:11-84): ``x`` and
+``residual`` are float64 numpy arrays updated in place and the return value is
+the iteration count K.  Where the reference would abort on a failed assert,
+these raise :class:`SfaError`.  Every call goes to the HIP library; nothing
+here computes a result on the CPU, and a missing library is an ImportError.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+from ctypes import POINTER, c_double, c_int, c_longlong, c_void_p
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsimplefem_amd.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "simplefem_amd.h")
+
+SFA_OK = 0
+ERRORS = {-1: "SFA_ERR_INVALID", -2: "SFA_ERR_HIP", -3: "SFA_ERR_NO_DEVICE",
+          -4: "SFA_ERR_OUT_OF_MEMORY", -5: "SFA_ERR_COLOURING"}
+EXIT_CONVERGED, EXIT_CAP, EXIT_BREAKDOWN, EXIT_SMALL = 0, 1, 2, 3
+METHOD_CG, METHOD_DIAGONAL, METHOD_SSOR, METHOD_SSOR_EISENSTAT, METHOD_RAINBOW, \
+    METHOD_EISENSTAT_RAINBOW = range(6)
+
+
+class SfaError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {message}")
+        self.code = code
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [("iterations", c_int), ("success", c_int), ("exit_reason", c_int),
+                ("reserved", c_int), ("zr", c_double), ("tolerance", c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+class SystemStats(ctypes.Structure):
+    _fields_ = [("n", c_int), ("n_active", c_int), ("nnz", c_longlong),
+                ("groups_forward", c_int), ("groups_backward", c_int),
+                ("stored_entries", c_longlong), ("bytes_per_iteration", c_double),
+                ("bytes_spmv", c_double), ("kernels_per_iteration", c_int),
+                ("reserved", c_int)]
+
+
+_P_I = POINTER(c_int)
+_P_D = POINTER(c_double)
+_SOLVER_ARGS = [c_int, _P_D, _P_D, _P_I, _P_I, _P_D, _P_D, c_double, c_int]
+_SIGNATURES = {
+    "sfa_last_error": ([], ctypes.c_char_p),
+    "sfa_api_version": ([], c_int),
+    "sfa_ConjugateGradientSolverCSR": (_SOLVER_ARGS + [POINTER(SolveInfo)], c_int),
+    "sfa_ConjugateGradientSolverCSR_DiagonalPreconditioner":
+        (_SOLVER_ARGS + [_P_D, POINTER(SolveInfo)], c_int),
+    "sfa_ConjugateGradientSolverCSR_SSOR": (_SOLVER_ARGS + [_P_D, c_double, POINTER(SolveInfo)], c_int),
+    "sfa_ConjugateGradientSolverCSR_SSOR_Eisenstat":
+        (_SOLVER_ARGS + [_P_D, c_double, POINTER(SolveInfo)], c_int),
+    "sfa_ConjugateGradientSolverCSR_Rainbow":
+        (_SOLVER_ARGS + [_P_D, c_double, c_int, _P_I, _P_I, _P_I, POINTER(SolveInfo)], c_int),
+    "sfa_ConjugateGradientSolverCSR_Eisenstat_Rainbow":
+        (_SOLVER_ARGS + [_P_D, c_double, c_int, _P_I, _P_I, _P_I, POINTER(SolveInfo)], c_int),
+    "sfa_MatrixCSR_apply": ([c_int, c_int, _P_I, _P_I, _P_D, _P_D, _P_D, c_double], c_int),
+    "sfa_MatrixCSR_getDiagonal": ([c_int, _P_I, _P_I, _P_D, _P_D], c_int),
+    "sfa_Rainbow": ([c_int, _P_I, _P_I, _P_I, _P_I, _P_I, _P_I], c_int),
+    "sfa_system_create": ([c_int, c_int, _P_I, _P_I, _P_D, _P_D, c_double, c_int, _P_I, _P_I,
+                           _P_I, POINTER(c_void_p)], c_int),
+    "sfa_system_solve": ([c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int, c_int,
+                          POINTER(SolveInfo)], c_int),
+    "sfa_system_apply": ([c_void_p, c_void_p, c_void_p, c_double, c_int], c_int),
+    "sfa_system_set_stream": ([c_void_p, c_void_p], c_int),
+    "sfa_system_query": ([c_void_p, POINTER(SystemStats)], c_int),
+    "sfa_system_destroy": ([c_void_p], c_int),
+}
+
+_lib = None
+
+
+def header_symbols(path: str = HEADER):
+    """Function names the C header declares."""
+    text = open(path).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(sfa_\w+)\s*\(", text)))
+
+
+def load(path: str = LIB_PATH):
+    """Loads the HIP library (raises ImportError when it has not been built)."""
+    global _lib
+    if _lib is not None and path == LIB_PATH:
+        return _lib
+    if not os.path.exists(path):
+        raise ImportError(f"{path} is missing: build it with `python -m simplefem_amd.build`")
+    lib = ctypes.CDLL(path)
+    for name, (args, res) in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    if path == LIB_PATH:
+        _lib = lib
+    return lib
+
+
+def _check(status: int):
+    if status != SFA_OK:
+        raise SfaError(status, load().sfa_last_error().decode())
+
+
+def _i(a):
+    return a.ctypes.data_as(_P_I) if a is not None else None
+
+
+def _d(a):
+    return a.ctypes.data_as(_P_D) if a is not None else None
+
+
+def _f64(a, name, n=None, writable=False):
+    if a is None:
+        return None
+    if not (isinstance(a, np.ndarray) and a.dtype == np.float64 and a.flags.c_contiguous):
+        if writable:
+            raise TypeError(f"{name} must be a C-contiguous float64 numpy array (updated in place)")
+        a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None and a.shape[0] < n:
+        raise ValueError(f"{name} has {a.shape[0]} entries, needs {n}")
+    return a
+
+
+def _i32(a, name):
+    if a is None:
+        return None
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _csr(csrrows, csrcolumns, csrvalues):
+    return _i32(csrrows, "csrrows"), _i32(csrcolumns, "csrcolumns"), _f64(csrvalues, "csrvalues")
+
+
+# --------------------------------------------------------------------------- solvers
+# Each returns K and fills `info` (a dict) when one is passed.
+
+def _solve(fn_name, N, x, b, A, C, V, residual, precision, print_modulo, extra, info):
+    lib = load()
+    x = _f64(x, "x", N, writable=True)
+    residual = _f64(residual, "residual", N, writable=True)
+    b = _f64(b, "b", N)
+    A, C, V = _csr(A, C, V)
+    out = SolveInfo()
+    _check(getattr(lib, fn_name)(N, _d(x), _d(b), _i(A), _i(C), _d(V), _d(residual),
+                                 float(precision), int(print_modulo), *extra, ctypes.byref(out)))
+    if info is not None:
+        info.update(out.as_dict())
+    return out.iterations
+
+
+def ConjugateGradientSolverCSR(N, x, b, csrrows, csrcolumns, csrvalues, residual, precision,
+                               print_modulo, info=None):
+    """solver/sparsesolver.hpp:11-19"""
+    return _solve("sfa_ConjugateGradientSolverCSR", N, x, b, csrrows, csrcolumns, csrvalues,
+                  residual, precision, print_modulo, [], info)
+
+
+def ConjugateGradientSolverCSR_DiagonalPreconditioner(N, x, b, csrrows, csrcolumns, csrvalues,
+                                                      residual, precision, print_modulo, precon,
+                                                      info=None):
+    """solver/sparsesolver.hpp:21-30"""
+    precon = _f64(precon, "precon", N)
+    return _solve("sfa_ConjugateGradientSolverCSR_DiagonalPreconditioner", N, x, b, csrrows,
+                  csrcolumns, csrvalues, residual, precision, print_modulo, [_d(precon)], info)
+
+
+def ConjugateGradientSolverCSR_SSOR(N, x, b, csrrows, csrcolumns, csrvalues, residual, precision,
+                                    print_modulo, diagonal, omega, info=None):
+    """solver/sparsesolver.hpp:33-43"""
+    diagonal = _f64(diagonal, "diagonal", N)
+    return _solve("sfa_ConjugateGradientSolverCSR_SSOR", N, x, b, csrrows, csrcolumns, csrvalues,
+                  residual, precision, print_modulo, [_d(diagonal), float(omega)], info)
+
+
+def ConjugateGradientSolverCSR_SSOR_Eisenstat(N, x, b, csrrows, csrcolumns, csrvalues, residual,
+                                              precision, print_modulo, diagonal, omega,
+                                              info=None):
+    """solver/sparsesolver.hpp:46-56"""
+    diagonal = _f64(diagonal, "diagonal", N)
+    return _solve("sfa_ConjugateGradientSolverCSR_SSOR_Eisenstat", N, x, b, csrrows, csrcolumns,
+                  csrvalues, residual, precision, print_modulo, [_d(diagonal), float(omega)], info)
+
+
+def ConjugateGradientSolverCSR_Rainbow(N, x, b, csrrows, csrcolumns, csrvalues, residual,
+                                       precision, print_modulo, diagonal, omega, num_colors, F, B,
+                                       R, info=None):
+    """solver/sparsesolver.hpp:59-70"""
+    diagonal = _f64(diagonal, "diagonal", N)
+    F, B, R = _i32(F, "F"), _i32(B, "B"), _i32(R, "R")
+    return _solve("sfa_ConjugateGradientSolverCSR_Rainbow", N, x, b, csrrows, csrcolumns,
+                  csrvalues, residual, precision, print_modulo,
+                  [_d(diagonal), float(omega), int(num_colors), _i(F), _i(B), _i(R)], info)
+
+
+def ConjugateGradientSolverCSR_Eisenstat_Rainbow(N, x, b, csrrows, csrcolumns, csrvalues,
+                                                 residual, precision, print_modulo, diagonal,
+                                                 omega, num_colors, F, B, R, info=None):
+    """solver/sparsesolver.hpp:73-84"""
+    diagonal = _f64(diagonal, "diagonal", N)
+    F, B, R = _i32(F, "F"), _i32(B, "B"), _i32(R, "R")
+    return _solve("sfa_ConjugateGradientSolverCSR_Eisenstat_Rainbow", N, x, b, csrrows,
+                  csrcolumns, csrvalues, residual, precision, print_modulo,
+                  [_d(diagonal), float(omega), int(num_colors), _i(F), _i(B), _i(R)], info)
+
+
+# --------------------------------------------------------------------------- CSR operations
+
+def csr_apply(rows, columns, A, C, V, src, scaling=1.0):
+    """MatrixCSR::apply (sparse/matcsr.cpp:216-244) on the GPU; returns dest."""
+    A, C, V = _csr(A, C, V)
+    src = _f64(src, "src", columns)
+    dest = np.empty(rows, dtype=np.float64)
+    _check(load().sfa_MatrixCSR_apply(rows, columns, _i(A), _i(C), _d(V), _d(dest), _d(src),
+                                      float(scaling)))
+    return dest
+
+
+def get_diagonal(n, A, C, V):
+    """MatrixCSR::getDiagonal (sparse/matcsr.cpp:266-286)."""
+    A, C, V = _csr(A, C, V)
+    diag = np.empty(n, dtype=np.float64)
+    _check(load().sfa_MatrixCSR_getDiagonal(n, _i(A), _i(C), _d(V), _d(diag)))
+    return diag
+
+
+class Rainbow:
+    """Rainbow::Rainbow (sparse/rainbow.hpp:13-31): num_rows, num_colors, F, B, R."""
+
+    def __init__(self, n, A, C):
+        A, C = _i32(A, "A"), _i32(C, "C")
+        self.num_rows = n
+        self.F = np.empty(n, dtype=np.int32)
+        B = np.empty(n + 1, dtype=np.int32)
+        self.R = np.empty(n, dtype=np.int32)
+        nc = c_int(0)
+        _check(load().sfa_Rainbow(n, _i(A), _i(C), _i(self.F), _i(B), _i(self.R), ctypes.byref(nc)))
+        self.num_colors = nc.value
+        self.B = B[: self.num_colors + 1].copy()
+
+
+# --------------------------------------------------------------------------- device systems
+
+class System:
+    """A matrix + preconditioner schedule resident in HBM (sfa_system_*).
+
+    ``solve`` accepts numpy arrays (host) or torch CUDA tensors (device
+    pointers, float64, contiguous).
+    """
+
+    def __init__(self, method, N, A, C, V, aux=None, omega=1.0, num_colors=0, F=None, B=None,
+                 R=None):
+        lib = load()
+        A, C, V = _csr(A, C, V)
+        aux = _f64(aux, "aux", N)
+        F, B, R = _i32(F, "F"), _i32(B, "B"), _i32(R, "R")
+        self._h = c_void_p()
+        self.N = N
+        _check(lib.sfa_system_create(method, N, _i(A), _i(C), _d(V), _d(aux), float(omega),
+                                     int(num_colors), _i(F), _i(B), _i(R), ctypes.byref(self._h)))
+
+    @staticmethod
+    def _ptr(a, N, writable):
+        if isinstance(a, np.ndarray):
+            a = _f64(a, "vector", N, writable)
+            return a, a.ctypes.data, 0
+        import torch  # device tensors only
+        if not (a.is_cuda and a.dtype == torch.float64 and a.is_contiguous() and a.numel() >= N):
+            raise TypeError("device vectors must be contiguous float64 CUDA tensors")
+        return a, a.data_ptr(), 1
+
+    def solve(self, x, b, residual, precision, print_modulo=-3, info=None):
+        x, px, dx = self._ptr(x, self.N, True)
+        b, pb, db = self._ptr(b, self.N, False)
+        residual, pr, dr = self._ptr(residual, self.N, True)
+        if not (dx == db == dr):
+            raise TypeError("x, b and residual must all be host arrays or all device tensors")
+        out = SolveInfo()
+        _check(load().sfa_system_solve(self._h, px, pb, pr, float(precision), int(print_modulo), dx,
+                                       ctypes.byref(out)))
+        if info is not None:
+            info.update(out.as_dict())
+        return out.iterations
+
+    def apply(self, dest, src, scaling=1.0):
+        dest, pd, dd = self._ptr(dest, self.N, True)
+        src, ps, ds = self._ptr(src, self.N, False)
+        if dd != ds:
+            raise TypeError("dest and src must both be host arrays or both device tensors")
+        _check(load().sfa_system_apply(self._h, pd, ps, float(scaling), dd))
+
+    def set_stream(self, stream_handle):
+        _check(load().sfa_system_set_stream(self._h, c_void_p(stream_handle)))
+
+    def stats(self):
+        st = SystemStats()
+        _check(load().sfa_system_query(self._h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_ if k != "reserved"}
+
+    def close(self):
+        if self._h:
+            load().sfa_system_destroy(self._h)
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()

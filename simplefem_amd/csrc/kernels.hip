@@ -1,0 +1,609 @@
+// kernels.hip -- gfx950 kernels of the CSR CG / SSOR / Rainbow-SSOR path.
+//
+// Roofline: every kernel here is HBM-bound (about 0.1 flop per byte); no MFMA.
+// Matrix parts are SELL-64: one 64-lane wavefront owns 64 rows, lane l reads
+// entry k of its row at off + 64*k + l, so each entry load of the wavefront
+// is one contiguous 512-byte (values) / 256-byte (columns) segment.  Each
+// row's sum runs over its entries in the reference's CSR order, with products
+// rounded before the add (built with -ffp-contract=off), so a row's value
+// matches the reference's loop bit for bit given the same inputs.
+//
+// Reductions (d.Ad, z.r) are deterministic: each block writes one partial,
+// the last block to finish (agent-scope ticket) sums the partials in block
+// order and runs the scalar part of the CG step on the device.
+#include "kernels.hpp"
+
+#include <cmath>
+
+#include "../../include/simplefem_amd.h"
+
+namespace sfa {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kSellBlock = 64;     // one wavefront = one slice per block
+constexpr int kVecBlock = 256;
+constexpr int kVecMaxBlocks = 2048;
+constexpr double kDesiredPrecision = 1.4901161193847656e-08;  // sqrt(DBL_EPSILON) = 2^-26
+constexpr double kMachineEpsilon = 2.220446049250313e-16;     // DBL_EPSILON
+
+__device__ inline double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+    return v;
+}
+
+// Block-wide deterministic reduction across the whole grid.  Returns true in
+// every thread of the last block; *total is then valid in all its threads.
+template <int BS>
+__device__ bool grid_reduce_last(double v, double* partials, unsigned* ticket, double* total)
+{
+    constexpr int NW = BS / kWave;
+    __shared__ double ws[NW];
+    __shared__ int is_last;
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    double s = wave_sum(v);
+    if (NW > 1) {
+        if (lane == 0) ws[wave] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.;
+            for (int w = 0; w < NW; w++) t += ws[w];
+            s = t;
+        }
+    }
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&partials[blockIdx.x], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        unsigned prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = prev == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    double acc = 0.;
+    for (unsigned i = threadIdx.x; i < gridDim.x; i += BS)
+        acc += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    acc = wave_sum(acc);
+    if (NW > 1) {
+        __syncthreads();
+        if (lane == 0) ws[wave] = acc;
+        __syncthreads();
+        double t = 0.;
+        for (int w = 0; w < NW; w++) t += ws[w];
+        acc = t;
+    }
+    *total = acc;
+    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+// base/base.hpp:229-237
+__device__ inline double ref_absolute(double x)
+{
+    if (x >= 0) return x;
+    if (x <= 0) return -x;
+    return x;
+}
+
+// Scalar step after d.Ad (solver/sparsesolver.cpp:629-645): breakdown tests
+// and alpha.  One thread.
+__device__ void after_dAd(SolveState* st, double dAd)
+{
+    st->dAd = dAd;
+    const double zr = st->zr;
+    const bool num_bad = !isfinite(zr) || zr < 0.;
+    const bool den_bad = !isfinite(dAd) || dAd < 0.;
+    const bool small = sqrt(ref_absolute(dAd)) < kMachineEpsilon;
+    if (num_bad || den_bad) {
+        st->done = 1;
+        st->exit_reason = SFA_EXIT_BREAKDOWN;
+    } else if (small && st->small_breaks) {
+        st->done = 1;
+        st->exit_reason = SFA_EXIT_SMALL;
+    } else {
+        st->alpha = zr / dAd;
+    }
+}
+
+// Scalar step after the new z.r (solver/sparsesolver.cpp:694-708 and the
+// loop head :504,601-604): beta, K++, then the loop's exit tests.  One thread.
+__device__ void after_zr_new(SolveState* st, double zr_new)
+{
+    st->beta = zr_new / st->zr;
+    st->zr = zr_new;
+    const int K = st->K + 1;
+    st->K = K;
+    st->hist[K] = zr_new;
+    if (K >= st->maxK) {
+        st->done = 1;
+        st->exit_reason = SFA_EXIT_CAP;
+    } else if (ref_absolute(zr_new) < st->tol2) {
+        st->done = 1;
+        st->exit_reason = SFA_EXIT_CONVERGED;
+    }
+}
+
+// z.r of the restart (K = 0) and the first loop-head test.
+__device__ void after_zr_restart(SolveState* st, double zr)
+{
+    st->zr = zr;
+    st->hist[0] = zr;
+    if (ref_absolute(zr) < st->tol2) {
+        st->done = 1;
+        st->exit_reason = SFA_EXIT_CONVERGED;
+    }
+}
+
+// acc (+|-)= sum_k v_k * vec[c_k] over one SELL row, entries in order.
+// Loads of a chunk are issued before its arithmetic so the gathers overlap.
+template <bool SUB>
+__device__ inline double sell_row(const SellView& A, int slice, int lane, const double* __restrict__ vec,
+                                  double acc)
+{
+    const int beg = A.off[slice] + lane;
+    const int end = A.off[slice + 1];
+    constexpr int U = 8;
+    int e = beg;
+    for (; e + (U - 1) * kWave < end; e += U * kWave) {
+        double v[U], g[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int c = __builtin_nontemporal_load(&A.cols[e + u * kWave]);
+            v[u] = __builtin_nontemporal_load(&A.vals[e + u * kWave]);
+            g[u] = vec[c];
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (SUB) acc -= v[u] * g[u];
+            else     acc += v[u] * g[u];
+        }
+    }
+    for (; e < end; e += kWave) {
+        const int c = __builtin_nontemporal_load(&A.cols[e]);
+        const double v = __builtin_nontemporal_load(&A.vals[e]);
+        if (SUB) acc -= v * vec[c];
+        else     acc += v * vec[c];
+    }
+    return acc;
+}
+
+__device__ inline int slice_row(const SellView& A, int slice, int lane)
+{
+    return A.rows[slice * kWave + lane];
+}
+
+// ------------------------------------------------------------------ vectors
+
+__global__ void k_fill(double* v, double value, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) v[i] = value;
+}
+
+__global__ void k_gather(double* dst, const double* src, const int* perm, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[perm[i]];
+}
+
+__global__ void k_scatter(double* dst, const double* src, const int* perm, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[perm[i]] = src[i];
+}
+
+// State reset and tolerance (solver/sparsesolver.cpp:469-472):
+// tol = max(precision * sqrt(sum b^2), desired_precision).
+__global__ void __launch_bounds__(kVecBlock) k_init_state(SolveState* st, const double* b, int n,
+                                                          double precision, double host_tol,
+                                                          int use_host_tol, int maxK,
+                                                          int small_breaks, double* partials)
+{
+    double s = 0.;
+    if (!use_host_tol)
+        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+            s += b[i] * b[i];
+    double total;
+    if (!grid_reduce_last<kVecBlock>(s, partials, &st->ticket, &total)) return;
+    if (threadIdx.x != 0) return;
+    double tol = host_tol;
+    if (!use_host_tol) {
+        tol = precision * sqrt(total);
+        // base/base.hpp:275-283 maximum()
+        if (!(tol >= kDesiredPrecision) && (tol <= kDesiredPrecision)) tol = kDesiredPrecision;
+    }
+    st->tol = tol;
+    st->tol2 = tol * tol;
+    st->zr = __builtin_nan("");
+    st->dAd = 0.;
+    st->alpha = 0.;
+    st->beta = 0.;
+    st->K = 0;
+    st->maxK = maxK;
+    st->done = 0;
+    st->exit_reason = SFA_EXIT_CAP;
+    st->small_breaks = small_breaks;
+}
+
+// ------------------------------------------------------------------ apply
+
+// MatrixCSR::apply (sparse/matcsr.cpp:216-244): dest = 0; dest += scaling*V*x.
+__global__ void __launch_bounds__(kSellBlock) k_apply(SellView A, int nslices, const double* src,
+                                                      double* dest, double scaling)
+{
+    const int slice = blockIdx.x;
+    const int lane = threadIdx.x;
+    if (slice >= nslices) return;
+    const int row = slice_row(A, slice, lane);
+    if (row < 0) return;
+    const int beg = A.off[slice] + lane, end = A.off[slice + 1];
+    double acc = 0.;
+    for (int e = beg; e < end; e += kWave) acc += scaling * A.vals[e] * src[A.cols[e]];
+    dest[row] = acc;
+}
+
+// ------------------------------------------------------------------ restart
+
+// solver/sparsesolver.cpp:80-98 (plain), :281-306 (Jacobi), :512-524 (SSOR),
+// :797-812 (Eisenstat, columns already filtered into the part).
+__global__ void __launch_bounds__(kSellBlock) k_residual(int mode, SellView A, int nslices,
+                                                         SolveState* st, const double* b,
+                                                         const double* x, double* out, double* d,
+                                                         double* z, const double* precon,
+                                                         double* partials)
+{
+    const int slice = blockIdx.x;
+    const int lane = threadIdx.x;
+    double part = 0.;
+    const int row = slice < nslices ? slice_row(A, slice, lane) : -1;
+    if (row >= 0) {
+        const double r = sell_row<true>(A, slice, lane, x, b[row]);
+        out[row] = r;
+        if (mode == RES_PLAIN) {
+            d[row] = r;
+            part = r * r;
+        } else if (mode == RES_JACOBI) {
+            const double zz = precon[row] * r;
+            z[row] = zz;
+            d[row] = zz;
+            part = zz * r;
+        }
+    }
+    if (mode == RES_ONLY) return;
+    double total;
+    if (!grid_reduce_last<kSellBlock>(part, partials, &st->ticket, &total)) return;
+    if (threadIdx.x == 0) after_zr_restart(st, total);
+}
+
+// solver/sparsesolver.cpp:570-583: d = z, z.r = sum z*r.
+__global__ void __launch_bounds__(kVecBlock) k_direction_init(SolveState* st, int n, const double* z,
+                                                              const double* r, double* d,
+                                                              double* partials)
+{
+    double part = 0.;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double zz = z[i];
+        d[i] = zz;
+        part += zz * r[i];
+    }
+    double total;
+    if (!grid_reduce_last<kVecBlock>(part, partials, &st->ticket, &total)) return;
+    if (threadIdx.x == 0) after_zr_restart(st, total);
+}
+
+// solver/sparsesolver.cpp:830-844: z = r*diag*(2-w)/w, d = z, z.r.
+__global__ void __launch_bounds__(kVecBlock) k_eisenstat_init(SolveState* st, int n, const double* r,
+                                                              const double* diag, double omega,
+                                                              double* z, double* d,
+                                                              double* partials)
+{
+    double part = 0.;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double rr = r[i];
+        const double zz = rr * diag[i] * (2. - omega) / omega;
+        z[i] = zz;
+        d[i] = zz;
+        part += zz * rr;
+    }
+    double total;
+    if (!grid_reduce_last<kVecBlock>(part, partials, &st->ticket, &total)) return;
+    if (threadIdx.x == 0) after_zr_restart(st, total);
+}
+
+// ------------------------------------------------------------------ sweeps
+
+// Forward sweep of one colour / level, solver/sparsesolver.cpp:528-541 (restart)
+// and :647-665, :1234-1261 (iteration, with the x and r updates).
+__global__ void __launch_bounds__(kSellBlock) k_forward(SellView L, int slice0, int nslices,
+                                                        const SolveState* st, int update_xr,
+                                                        double* x, double* r, const double* d,
+                                                        const double* q, double* m,
+                                                        const double* diag, double omega)
+{
+    if (st->done) return;
+    const int slice = slice0 + blockIdx.x;
+    const int lane = threadIdx.x;
+    if (blockIdx.x >= nslices) return;
+    const int row = slice_row(L, slice, lane);
+    if (row < 0) return;
+    double rr;
+    if (update_xr) {
+        const double alpha = st->alpha;
+        x[row] += alpha * d[row];
+        rr = r[row] - alpha * q[row];
+        r[row] = rr;
+    } else {
+        rr = r[row];
+    }
+    const double acc = sell_row<true>(L, slice, lane, m, rr);
+    m[row] = acc * omega / diag[row];
+}
+
+// Backward sweep of one colour / level, solver/sparsesolver.cpp:543-568
+// (restart) and :667-692, :1263-1303 (iteration): the scaling of m fused in,
+// z.r summed per group and over groups in launch order.
+__global__ void __launch_bounds__(kSellBlock) k_backward(SellView U, int slice0, int nslices,
+                                                         SolveState* st, int reduce, int group,
+                                                         int last_group, const double* r,
+                                                         const double* m, double* z,
+                                                         const double* diag, double omega,
+                                                         double c2, double* grp, double* partials)
+{
+    if (st->done) return;
+    const int slice = slice0 + blockIdx.x;
+    const int lane = threadIdx.x;
+    double part = 0.;
+    const int row = blockIdx.x < nslices ? slice_row(U, slice, lane) : -1;
+    if (row >= 0) {
+        const double dg = diag[row];
+        double t = m[row];
+        t *= dg;
+        t *= c2;
+        const double acc = sell_row<true>(U, slice, lane, z, t);
+        const double zz = acc * omega / dg;
+        z[row] = zz;
+        part = zz * r[row];
+    }
+    if (!reduce) return;
+    double total;
+    if (!grid_reduce_last<kSellBlock>(part, partials, &st->ticket, &total)) return;
+    if (threadIdx.x != 0) return;
+    grp[group] = total;
+    if (!last_group) return;
+    double zr_new = 0.;
+    for (int g = 0; g <= group; g++) zr_new += grp[g];
+    after_zr_new(st, zr_new);
+}
+
+// Eisenstat triangular solves (solver/sparsesolver.cpp:814-828, 876-908,
+// 1429-1455, 1501-1560): out = (in - [c3*diag*t] - T out) * w / diag.
+__global__ void __launch_bounds__(kSellBlock) k_eisenstat_solve(SellView T, int slice0, int nslices,
+                                                                const SolveState* st, int with_t,
+                                                                const double* in, const double* t,
+                                                                double* out, const double* diag,
+                                                                double omega, double c3)
+{
+    if (st->done) return;
+    const int slice = slice0 + blockIdx.x;
+    const int lane = threadIdx.x;
+    if (blockIdx.x >= nslices) return;
+    const int row = slice_row(T, slice, lane);
+    if (row < 0) return;
+    const double dg = diag[row];
+    double a = in[row];
+    if (with_t) a = a - c3 * dg * t[row];
+    const double acc = sell_row<true>(T, slice, lane, out, a);
+    out[row] = acc * omega / dg;
+}
+
+// ------------------------------------------------------------------ iteration
+
+// q = A d, d.Ad (solver/sparsesolver.cpp:611-626, 1197-1212), then the
+// breakdown tests and alpha.
+__global__ void __launch_bounds__(kSellBlock) k_spmv_dot(SellView A, int nslices, SolveState* st,
+                                                         const double* d, double* q,
+                                                         double* partials)
+{
+    if (st->done) return;
+    const int slice = blockIdx.x;
+    const int lane = threadIdx.x;
+    double part = 0.;
+    const int row = slice < nslices ? slice_row(A, slice, lane) : -1;
+    if (row >= 0) {
+        const double acc = sell_row<false>(A, slice, lane, d, 0.);
+        q[row] = acc;
+        part = d[row] * acc;
+    }
+    double total;
+    if (!grid_reduce_last<kSellBlock>(part, partials, &st->ticket, &total)) return;
+    if (threadIdx.x == 0) after_dAd(st, total);
+}
+
+// x += a dx; r -= a q; then z.r by mode (solver/sparsesolver.cpp:160-174,
+// 372-389, 952-968); beta and the loop-head tests.
+__global__ void __launch_bounds__(kVecBlock) k_update(int mode, SolveState* st, int n, double* x,
+                                                      double* r, const double* dx,
+                                                      const double* q, double* z,
+                                                      const double* dp, double omega,
+                                                      double* partials)
+{
+    if (st->done) return;
+    const double alpha = st->alpha;
+    double part = 0.;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        x[i] += alpha * dx[i];
+        const double rr = r[i] - alpha * q[i];
+        r[i] = rr;
+        if (mode == UPD_PLAIN) {
+            part += rr * rr;
+        } else if (mode == UPD_JACOBI) {
+            const double zz = dp[i] * rr;
+            z[i] = zz;
+            part += zz * rr;
+        } else {
+            const double zz = rr * (dp[i] * (2. - omega) / omega);
+            z[i] = zz;
+            part += zz * rr;
+        }
+    }
+    double total;
+    if (!grid_reduce_last<kVecBlock>(part, partials, &st->ticket, &total)) return;
+    if (threadIdx.x == 0) after_zr_new(st, total);
+}
+
+// solver/sparsesolver.cpp:912-922: q += t, d.Ad = sum q*d; breakdown tests.
+__global__ void __launch_bounds__(kVecBlock) k_eisenstat_combine(SolveState* st, int n, double* q,
+                                                                 const double* t, const double* d,
+                                                                 double* partials)
+{
+    if (st->done) return;
+    double part = 0.;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const double qq = q[i] + t[i];
+        q[i] = qq;
+        part += qq * d[i];
+    }
+    double total;
+    if (!grid_reduce_last<kVecBlock>(part, partials, &st->ticket, &total)) return;
+    if (threadIdx.x == 0) after_dAd(st, total);
+}
+
+// d = src + beta*d (solver/sparsesolver.cpp:698-706 and its siblings).
+__global__ void __launch_bounds__(kVecBlock) k_direction(const SolveState* st, int n,
+                                                         const double* src, double* d)
+{
+    if (st->done) return;
+    const double beta = st->beta;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = src[i] + beta * d[i];
+}
+
+int vec_blocks(int n)
+{
+    int b = (n + kVecBlock - 1) / kVecBlock;
+    if (b < 1) b = 1;
+    return b > kVecMaxBlocks ? kVecMaxBlocks : b;
+}
+
+int flat_blocks(int n)
+{
+    int b = (n + kVecBlock - 1) / kVecBlock;
+    return b < 1 ? 1 : b;
+}
+
+}  // namespace
+
+void launch_fill(hipStream_t s, double* v, double value, int n)
+{
+    if (n > 0) k_fill<<<flat_blocks(n), kVecBlock, 0, s>>>(v, value, n);
+}
+
+void launch_gather(hipStream_t s, double* dst, const double* src, const int* perm, int n)
+{
+    if (n > 0) k_gather<<<flat_blocks(n), kVecBlock, 0, s>>>(dst, src, perm, n);
+}
+
+void launch_scatter(hipStream_t s, double* dst, const double* src, const int* perm, int n)
+{
+    if (n > 0) k_scatter<<<flat_blocks(n), kVecBlock, 0, s>>>(dst, src, perm, n);
+}
+
+void launch_init_state(hipStream_t s, SolveState* st, const double* b, int n, double precision,
+                       double host_tol, int use_host_tol, int maxK, int small_breaks,
+                       double* partials)
+{
+    k_init_state<<<use_host_tol ? 1 : vec_blocks(n), kVecBlock, 0, s>>>(
+        st, b, n, precision, host_tol, use_host_tol, maxK, small_breaks, partials);
+}
+
+void launch_apply(hipStream_t s, SellView A, int nslices, const double* src, double* dest,
+                  double scaling)
+{
+    if (nslices > 0) k_apply<<<nslices, kSellBlock, 0, s>>>(A, nslices, src, dest, scaling);
+}
+
+void launch_residual(hipStream_t s, ResidualMode mode, SellView A, int nslices, SolveState* st,
+                     const double* b, const double* x, double* out, double* d, double* z,
+                     const double* precon, double* partials)
+{
+    const int grid = nslices > 0 ? nslices : 1;
+    k_residual<<<grid, kSellBlock, 0, s>>>((int)mode, A, nslices, st, b, x, out, d, z, precon,
+                                           partials);
+}
+
+void launch_direction_init(hipStream_t s, SolveState* st, int n_active, const double* z,
+                           const double* r, double* d, double* partials)
+{
+    k_direction_init<<<vec_blocks(n_active), kVecBlock, 0, s>>>(st, n_active, z, r, d, partials);
+}
+
+void launch_eisenstat_init(hipStream_t s, SolveState* st, int n_active, const double* r,
+                           const double* diag, double omega, double* z, double* d,
+                           double* partials)
+{
+    k_eisenstat_init<<<vec_blocks(n_active), kVecBlock, 0, s>>>(st, n_active, r, diag, omega, z, d,
+                                                                partials);
+}
+
+void launch_forward(hipStream_t s, SellView L, int slice0, int nslices, SolveState* st,
+                    int update_xr, double* x, double* r, const double* d, const double* q,
+                    double* m, const double* diag, double omega)
+{
+    if (nslices <= 0) return;
+    k_forward<<<nslices, kSellBlock, 0, s>>>(L, slice0, nslices, st, update_xr, x, r, d, q, m,
+                                             diag, omega);
+}
+
+void launch_backward(hipStream_t s, SellView U, int slice0, int nslices, SolveState* st,
+                     int reduce, int group, int last_group, const double* r, const double* m,
+                     double* z, const double* diag, double omega, double c2, double* grp,
+                     double* partials)
+{
+    // A group with no rows still runs one block when it has to publish its
+    // (zero) partial or finish the z.r sum.
+    const int grid = nslices > 0 ? nslices : (reduce ? 1 : 0);
+    if (grid == 0) return;
+    k_backward<<<grid, kSellBlock, 0, s>>>(U, slice0, nslices, st, reduce, group, last_group, r,
+                                           m, z, diag, omega, c2, grp, partials);
+}
+
+void launch_eisenstat_solve(hipStream_t s, SellView T, int slice0, int nslices, SolveState* st,
+                            int with_t, const double* in, const double* t, double* out,
+                            const double* diag, double omega, double c3)
+{
+    if (nslices <= 0) return;
+    k_eisenstat_solve<<<nslices, kSellBlock, 0, s>>>(T, slice0, nslices, st, with_t, in, t, out,
+                                                     diag, omega, c3);
+}
+
+void launch_spmv_dot(hipStream_t s, SellView A, int nslices, SolveState* st, const double* d,
+                     double* q, double* partials)
+{
+    const int grid = nslices > 0 ? nslices : 1;
+    k_spmv_dot<<<grid, kSellBlock, 0, s>>>(A, nslices, st, d, q, partials);
+}
+
+void launch_update(hipStream_t s, UpdateMode mode, SolveState* st, int n_active, double* x,
+                   double* r, const double* dx, const double* q, double* z,
+                   const double* diag_or_precon, double omega, double* partials)
+{
+    k_update<<<vec_blocks(n_active), kVecBlock, 0, s>>>((int)mode, st, n_active, x, r, dx, q, z,
+                                                        diag_or_precon, omega, partials);
+}
+
+void launch_eisenstat_combine(hipStream_t s, SolveState* st, int n_active, double* q,
+                              const double* t, const double* d, double* partials)
+{
+    k_eisenstat_combine<<<vec_blocks(n_active), kVecBlock, 0, s>>>(st, n_active, q, t, d,
+                                                                   partials);
+}
+
+void launch_direction(hipStream_t s, SolveState* st, int n_active, const double* src, double* d)
+{
+    if (n_active > 0)
+        k_direction<<<flat_blocks(n_active), kVecBlock, 0, s>>>(st, n_active, src, d);
+}
+
+}  // namespace sfa

@@ -1,0 +1,305 @@
+// plan.cpp -- device order, sweep schedules and SELL-64 parts (host code).
+#include "plan.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+#include "../../include/simplefem_amd.h"
+
+namespace sfa {
+
+std::string check_csr_rect(int rows, int columns, const int* A, const int* C)
+{
+    if (rows <= 0 || columns <= 0) return "matrix dimensions must be positive";
+    if (!A || !C) return "null CSR pointer";
+    if (A[0] != 0) return "csrrows[0] must be 0";
+    for (int i = 0; i < rows; i++)
+        if (A[i + 1] < A[i]) return "csrrows must be non-decreasing";
+    const int nnz = A[rows];
+    for (int k = 0; k < nnz; k++)
+        if (C[k] < 0 || C[k] >= columns) return "column index out of range";
+    return "";
+}
+
+std::string check_csr(int n, const int* A, const int* C)
+{
+    if (n <= 0) return "N must be positive";
+    return check_csr_rect(n, n, A, C);
+}
+
+// Greedy colouring with the reference's rule (sparse/rainbow.cpp:48-76): row r
+// takes the least colour no other column of its row carries, rows not yet
+// visited counting as colour 0.  Written as a stamp-array mex instead of the
+// reference's restart scan; both give that least colour.  B and R follow
+// sparse/rainbow.cpp:80-117 (R lists each colour's rows back to front).
+int rainbow_colour(int n, const int* A, const int* C, int* F, int* B, int* R)
+{
+    int num_colors = 1;
+    std::vector<int> stamp(16, -1);
+    for (int r = 0; r < n; r++) F[r] = 0;
+    for (int r = 0; r < n; r++) {
+        for (int k = A[r]; k < A[r + 1]; k++) {
+            const int j = C[k];
+            if (j == r) continue;
+            const int c = F[j];
+            if (c >= (int)stamp.size()) stamp.resize(2 * c + 2, -1);
+            stamp[c] = r;
+        }
+        int c = 0;
+        while (c < (int)stamp.size() && stamp[c] == r) c++;
+        F[r] = c;
+        if (c == num_colors) num_colors++;
+    }
+    std::vector<int> count(num_colors, 0);
+    for (int r = 0; r < n; r++) count[F[r]]++;
+    B[0] = 0;
+    for (int c = 0; c < num_colors; c++) B[c + 1] = B[c] + count[c];
+    for (int r = 0; r < n; r++) {
+        const int c = F[r];
+        R[B[c] + --count[c]] = r;
+    }
+    return num_colors;
+}
+
+namespace {
+
+// Appends the slices of one group of rows (device order) to a part.  keep(k)
+// decides whether CSR entry k of the row goes into this part.
+template <class Keep>
+void append_group(SellPart& part, const std::vector<int>& rows_dev, const std::vector<int>& perm,
+                  const std::vector<int>& iperm, int n, const int* A, const int* C,
+                  const double* V, Keep keep)
+{
+    const int count = (int)rows_dev.size();
+    std::vector<int> sel;
+    for (int s0 = 0; s0 < count; s0 += kSliceRows) {
+        const int s1 = std::min(count, s0 + kSliceRows);
+        int width = 0;
+        for (int i = s0; i < s1; i++) {
+            const int orow = perm[rows_dev[i]];
+            int w = 0;
+            for (int k = A[orow]; k < A[orow + 1]; k++) w += keep(orow, k) ? 1 : 0;
+            width = std::max(width, w);
+        }
+        const int base = part.slice_off.back();
+        part.cols.resize((size_t)base + (size_t)width * kSliceRows, n);
+        part.vals.resize((size_t)base + (size_t)width * kSliceRows, 0.0);
+        for (int lane = 0; lane < kSliceRows; lane++) {
+            const int i = s0 + lane;
+            if (i >= s1) {
+                part.slice_rows.push_back(-1);
+                continue;
+            }
+            const int p = rows_dev[i];
+            const int orow = perm[p];
+            part.slice_rows.push_back(p);
+            int w = 0;
+            for (int k = A[orow]; k < A[orow + 1]; k++) {
+                if (!keep(orow, k)) continue;
+                const size_t at = (size_t)base + (size_t)w * kSliceRows + lane;
+                part.cols[at] = iperm[C[k]];
+                part.vals[at] = V[k];
+                w++;
+            }
+            part.entries += w;
+            part.rows += 1;
+        }
+        part.slice_off.push_back(base + width * kSliceRows);
+        part.nslices++;
+    }
+}
+
+void start_part(SellPart& part)
+{
+    part = SellPart();
+    part.slice_off.push_back(0);
+    part.group_slice.push_back(0);
+}
+
+void close_group(SellPart& part) { part.group_slice.push_back(part.nslices); }
+
+// Dependency levels of a triangular sweep over the active rows: rows of one
+// level read no row of the same or a later level.  dep(orow, k) tells whether
+// CSR entry k of row orow is a dependency; rows are visited in `order`.
+template <class Dep>
+std::vector<int> sweep_levels(int n, const int* A, const int* C, const std::vector<char>& active,
+                              const std::vector<int>& order, Dep dep, int* nlevels)
+{
+    std::vector<int> level(n, -1);
+    int top = 0;
+    for (int orow : order) {
+        if (!active[orow]) continue;
+        int lv = 0;
+        for (int k = A[orow]; k < A[orow + 1]; k++) {
+            const int j = C[k];
+            if (j == orow || !active[j] || !dep(orow, k)) continue;
+            lv = std::max(lv, level[j] + 1);
+        }
+        level[orow] = lv;
+        top = std::max(top, lv + 1);
+    }
+    *nlevels = top;
+    return level;
+}
+
+}  // namespace
+
+void build_full_identity(SellPart& part, int rows, int columns, const int* A, const int* C,
+                         const double* V)
+{
+    std::vector<int> perm(std::max(rows, columns) + 1), list(rows);
+    for (int i = 0; i < (int)perm.size(); i++) perm[i] = i;
+    for (int i = 0; i < rows; i++) list[i] = i;
+    start_part(part);
+    append_group(part, list, perm, perm, columns, A, C, V, [](int, int) { return true; });
+    close_group(part);
+}
+
+std::string build_plan(Plan& plan, int method, int n, const int* A, const int* C, const double* V,
+                       const double* aux, double omega, int num_colors, const int* F,
+                       const int* B, const int* R, bool* colouring_error)
+{
+    *colouring_error = false;
+    if (method < SFA_METHOD_CG || method > SFA_METHOD_EISENSTAT_RAINBOW) return "unknown method";
+    std::string err = check_csr(n, A, C);
+    if (!err.empty()) return err;
+    if (!V) return "null csrvalues";
+    if (method != SFA_METHOD_CG && !aux) return "null diagonal/precon";
+    const bool rainbow = method == SFA_METHOD_RAINBOW || method == SFA_METHOD_EISENSTAT_RAINBOW;
+    const bool eisen = method == SFA_METHOD_SSOR_EISENSTAT || method == SFA_METHOD_EISENSTAT_RAINBOW;
+    const bool sweeps = method >= SFA_METHOD_SSOR;
+
+    if (rainbow) {
+        *colouring_error = true;
+        if (num_colors < 0 || !F || !B || !R) return "num_colors < 0 or null F/B/R";
+        if (B[0] != 0 || B[num_colors] != n) return "B must run from 0 to N";
+        for (int c = 0; c < num_colors; c++)
+            if (B[c + 1] < B[c]) return "B must be non-decreasing";
+        std::vector<char> seen(n, 0);
+        for (int c = 0; c < num_colors; c++)
+            for (int i = B[c]; i < B[c + 1]; i++) {
+                const int r = R[i];
+                if (r < 0 || r >= n || seen[r]) return "R must be a permutation of the rows";
+                if (F[r] != c) return "F disagrees with B/R";
+                seen[r] = 1;
+            }
+        *colouring_error = false;
+    }
+
+    plan = Plan();
+    plan.method = method;
+    plan.n = n;
+    plan.nnz = A[n];
+    plan.omega = omega;
+
+    std::vector<char> active(n, 1);
+    if (method != SFA_METHOD_CG)
+        for (int i = 0; i < n; i++) active[i] = aux[i] != 0. ? 1 : 0;  // NaN counts as active
+
+    // ---- device order and the sweep groups (as lists of original rows) ----
+    std::vector<std::vector<int>> fwd_groups, bwd_groups;  // original row ids, launch order
+    std::vector<int> order;                                // active rows in device order
+    if (rainbow) {
+        fwd_groups.assign(num_colors, {});
+        for (int r = 0; r < n; r++)
+            if (active[r]) fwd_groups[F[r]].push_back(r);
+        for (auto& g : fwd_groups) order.insert(order.end(), g.begin(), g.end());
+        bwd_groups.assign(fwd_groups.rbegin(), fwd_groups.rend());
+    } else if (sweeps) {
+        std::vector<int> asc(n), desc(n);
+        for (int i = 0; i < n; i++) asc[i] = i, desc[i] = n - 1 - i;
+        int nf = 0, nb = 0;
+        auto is_lower = [&](int orow, int k) { return C[k] < orow; };
+        auto is_upper = [&](int orow, int k) { return C[k] > orow; };
+        std::vector<int> lf = sweep_levels(n, A, C, active, asc, is_lower, &nf);
+        std::vector<int> lb = sweep_levels(n, A, C, active, desc, is_upper, &nb);
+        fwd_groups.assign(nf, {});
+        bwd_groups.assign(nb, {});
+        for (int r = 0; r < n; r++)
+            if (active[r]) {
+                fwd_groups[lf[r]].push_back(r);
+                bwd_groups[lb[r]].push_back(r);
+            }
+        for (auto& g : fwd_groups) order.insert(order.end(), g.begin(), g.end());
+    } else {
+        for (int r = 0; r < n; r++)
+            if (active[r]) order.push_back(r);
+    }
+    plan.n_active = (int)order.size();
+    plan.perm = order;
+    for (int r = 0; r < n; r++)
+        if (!active[r]) plan.perm.push_back(r);
+    std::vector<int> iperm(n + 1);
+    for (int p = 0; p < n; p++) iperm[plan.perm[p]] = p;
+    iperm[n] = n;
+
+    plan.diag.assign(n, 0.0);
+    if (aux)
+        for (int p = 0; p < n; p++) plan.diag[p] = aux[plan.perm[p]];
+
+    auto to_dev = [&](const std::vector<int>& orig) {
+        std::vector<int> dev(orig.size());
+        for (size_t i = 0; i < orig.size(); i++) dev[i] = iperm[orig[i]];
+        return dev;
+    };
+    auto guarded = [&](int k) { return aux[C[k]] != 0.; };
+
+    // ---- full part: SpMV of the iteration / residual of the restart ----
+    std::vector<int> active_dev(plan.n_active);
+    for (int p = 0; p < plan.n_active; p++) active_dev[p] = p;
+    start_part(plan.full);
+    if (eisen)  // solver/sparsesolver.cpp:806-807,1420-1421 skip columns with diagonal 0
+        append_group(plan.full, active_dev, plan.perm, iperm, n, A, C, V,
+                     [&](int, int k) { return guarded(k); });
+    else
+        append_group(plan.full, active_dev, plan.perm, iperm, n, A, C, V,
+                     [](int, int) { return true; });
+    close_group(plan.full);
+
+    if (!sweeps) return "";
+
+    // ---- triangular parts ----
+    auto lower_nat = [&](int orow, int k) { return C[k] < orow; };
+    auto upper_nat = [&](int orow, int k) { return C[k] > orow; };
+    auto lower_rb = [&](int orow, int k) { return F[C[k]] < F[orow]; };
+    auto upper_rb = [&](int orow, int k) { return F[C[k]] > F[orow]; };
+
+    start_part(plan.lower);
+    start_part(plan.upper);
+    for (const auto& g : fwd_groups) {
+        std::vector<int> dev = to_dev(g);
+        if (method == SFA_METHOD_SSOR)
+            append_group(plan.lower, dev, plan.perm, iperm, n, A, C, V, lower_nat);
+        else if (method == SFA_METHOD_SSOR_EISENSTAT)  // :824,903
+            append_group(plan.lower, dev, plan.perm, iperm, n, A, C, V,
+                         [&](int o, int k) { return lower_nat(o, k) && guarded(k); });
+        else
+            append_group(plan.lower, dev, plan.perm, iperm, n, A, C, V, lower_rb);
+        close_group(plan.lower);
+    }
+    for (const auto& g : bwd_groups) {
+        std::vector<int> dev = to_dev(g);
+        if (method == SFA_METHOD_SSOR)
+            append_group(plan.upper, dev, plan.perm, iperm, n, A, C, V, upper_nat);
+        else if (method == SFA_METHOD_SSOR_EISENSTAT)  // :886
+            append_group(plan.upper, dev, plan.perm, iperm, n, A, C, V,
+                         [&](int o, int k) { return upper_nat(o, k) && guarded(k); });
+        else
+            append_group(plan.upper, dev, plan.perm, iperm, n, A, C, V, upper_rb);
+        close_group(plan.upper);
+    }
+    if (method == SFA_METHOD_EISENSTAT_RAINBOW) {
+        // The restart's lower solve skips columns with diagonal 0 (:1448), the
+        // iteration's does not (:1550-1552).
+        plan.has_lower_restart = true;
+        start_part(plan.lower_restart);
+        for (const auto& g : fwd_groups) {
+            append_group(plan.lower_restart, to_dev(g), plan.perm, iperm, n, A, C, V,
+                         [&](int o, int k) { return lower_rb(o, k) && guarded(k); });
+            close_group(plan.lower_restart);
+        }
+    }
+    return "";
+}
+
+}  // namespace sfa

@@ -1,0 +1,784 @@
+// system.cpp -- device-resident CG systems and the C-ABI of libsimplefem_amd.
+//
+// A system owns, in HBM: the SELL-64 matrix parts of its plan, the device
+// order permutation, the diagonal, the Krylov vectors (n+1 entries each, the
+// last a constant 0), the SolveState and the reduction scratch.  A solve runs
+//   gather inputs -> state/tolerance -> restart kernels -> chunks of
+//   iterations replayed from a captured hipGraph until state.done -> scatter
+// and reads the state back once per chunk.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/simplefem_amd.h"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+namespace sfa {
+namespace {
+
+thread_local std::string g_last_error;
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define SFA_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            int code_ = e_ == hipErrorOutOfMemory ? SFA_ERR_OUT_OF_MEMORY : SFA_ERR_HIP; \
+            throw Error{code_, std::string(#call) + ": " + hipGetErrorString(e_)};     \
+        }                                                                              \
+    } while (0)
+
+template <class T>
+T* dev_alloc(size_t count)
+{
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    SFA_HIP(hipMalloc(&p, count * sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+void dev_free(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+void ensure_device()
+{
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0)
+        throw Error{SFA_ERR_NO_DEVICE, "no HIP device available"};
+}
+
+struct DevPart {
+    int* off = nullptr;
+    int* rows = nullptr;
+    int* cols = nullptr;
+    double* vals = nullptr;
+    int nslices = 0;
+    std::vector<int> group_slice;
+    long long stored = 0;
+    long long entries = 0;
+    long long nrows = 0;
+
+    void upload(const SellPart& p)
+    {
+        nslices = p.nslices;
+        group_slice = p.group_slice;
+        stored = (long long)p.vals.size();
+        entries = p.entries;
+        nrows = p.rows;
+        off = dev_alloc<int>(p.slice_off.size());
+        rows = dev_alloc<int>(p.slice_rows.size());
+        cols = dev_alloc<int>(p.cols.size());
+        vals = dev_alloc<double>(p.vals.size());
+        SFA_HIP(hipMemcpy(off, p.slice_off.data(), p.slice_off.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!p.slice_rows.empty())
+            SFA_HIP(hipMemcpy(rows, p.slice_rows.data(), p.slice_rows.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!p.cols.empty()) {
+            SFA_HIP(hipMemcpy(cols, p.cols.data(), p.cols.size() * sizeof(int), hipMemcpyHostToDevice));
+            SFA_HIP(hipMemcpy(vals, p.vals.data(), p.vals.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
+    void release()
+    {
+        dev_free(off);
+        dev_free(rows);
+        dev_free(cols);
+        dev_free(vals);
+        off = rows = cols = nullptr;
+        vals = nullptr;
+    }
+    SellView view() const { return SellView{off, rows, cols, vals}; }
+    int ngroups() const { return (int)group_slice.size() - 1; }
+    int group_begin(int g) const { return group_slice[g]; }
+    int group_count(int g) const { return group_slice[g + 1] - group_slice[g]; }
+};
+
+const char* method_title(int method)
+{
+    switch (method) {
+    case SFA_METHOD_CG: return "Conjugate Gradient CSR";
+    case SFA_METHOD_DIAGONAL: return "Conjugate Gradient CSR (Diag)";
+    case SFA_METHOD_SSOR: return "Conjugate Gradient CSR (SSOR)";
+    case SFA_METHOD_SSOR_EISENSTAT: return "Conjugate Gradient CSR (SSOR-Eisenstat)";
+    case SFA_METHOD_RAINBOW: return "Conjugate Gradient CSR (SSOR-Rainbow)";
+    default: return "Conjugate Gradient CSR (SSOR-Rainbow-Eisenstat)";
+    }
+}
+
+}  // namespace
+
+struct System {
+    int device = 0;
+    int method = 0;
+    int n = 0;
+    int n_active = 0;
+    long long nnz = 0;
+    double omega = 1.0;
+    bool eisen = false;
+    bool sweeps = false;
+    bool rainbow = false;
+
+    DevPart full, lower, upper, lower_restart;
+    bool has_lower_restart = false;
+    DevPart apply_part;  // all rows, identity order (sfa_system_apply), built lazily
+    bool has_apply_part = false;
+    std::vector<int> host_A, host_C;  // kept only to build apply_part later
+    std::vector<double> host_V;
+
+    int* perm = nullptr;
+    double* diag = nullptr;
+    double *x = nullptr, *r = nullptr, *b = nullptr, *d = nullptr, *z = nullptr, *q = nullptr,
+           *m = nullptr;
+    double* io = nullptr;  // n staging entries for host <-> device copies
+    double* io2 = nullptr;
+    double* io3 = nullptr;
+    SolveState* state = nullptr;
+    double* hist = nullptr;
+    double* partials = nullptr;
+    double* grp = nullptr;
+    SolveState* host_state = nullptr;  // pinned
+
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    hipGraphExec_t chunk_exec = nullptr;
+    int chunk_iters = 0;
+
+    ~System() { release(); }
+
+    void release()
+    {
+        if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
+        chunk_exec = nullptr;
+        full.release();
+        lower.release();
+        upper.release();
+        lower_restart.release();
+        apply_part.release();
+        for (void* p : {(void*)perm, (void*)diag, (void*)x, (void*)r, (void*)b, (void*)d, (void*)z,
+                        (void*)q, (void*)m, (void*)io, (void*)io2, (void*)io3, (void*)state,
+                        (void*)hist, (void*)partials, (void*)grp})
+            dev_free(p);
+        perm = nullptr;
+        diag = x = r = b = d = z = q = m = io = io2 = io3 = hist = partials = grp = nullptr;
+        state = nullptr;
+        if (host_state) (void)hipHostFree(host_state);
+        host_state = nullptr;
+        if (own_stream) (void)hipStreamDestroy(own_stream);
+        own_stream = stream = nullptr;
+    }
+
+    void create(int method_, int n_, const int* A, const int* C, const double* V, const double* aux,
+                double omega_, int nc, const int* F, const int* B, const int* R)
+    {
+        ensure_device();
+        SFA_HIP(hipGetDevice(&device));
+        Plan plan;
+        bool colouring_error = false;
+        std::string err = build_plan(plan, method_, n_, A, C, V, aux, omega_, nc, F, B, R,
+                                     &colouring_error);
+        if (!err.empty()) throw Error{colouring_error ? SFA_ERR_COLOURING : SFA_ERR_INVALID, err};
+        method = method_;
+        n = n_;
+        n_active = plan.n_active;
+        nnz = plan.nnz;
+        omega = omega_;
+        rainbow = method == SFA_METHOD_RAINBOW || method == SFA_METHOD_EISENSTAT_RAINBOW;
+        eisen = method == SFA_METHOD_SSOR_EISENSTAT || method == SFA_METHOD_EISENSTAT_RAINBOW;
+        sweeps = method >= SFA_METHOD_SSOR;
+        host_A.assign(A, A + n + 1);
+        host_C.assign(C, C + nnz);
+        host_V.assign(V, V + nnz);
+
+        SFA_HIP(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
+        stream = own_stream;
+
+        full.upload(plan.full);
+        if (sweeps) {
+            lower.upload(plan.lower);
+            upper.upload(plan.upper);
+        }
+        has_lower_restart = plan.has_lower_restart;
+        if (has_lower_restart) lower_restart.upload(plan.lower_restart);
+
+        perm = dev_alloc<int>(n);
+        SFA_HIP(hipMemcpy(perm, plan.perm.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        diag = dev_alloc<double>(n + 1);
+        std::vector<double> dg(plan.diag);
+        dg.push_back(0.0);
+        SFA_HIP(hipMemcpy(diag, dg.data(), (n + 1) * sizeof(double), hipMemcpyHostToDevice));
+
+        double** vecs[] = {&x, &r, &b, &d, &z, &q, &m};
+        for (double** v : vecs) {
+            *v = dev_alloc<double>(n + 1);
+            SFA_HIP(hipMemset(*v, 0, (n + 1) * sizeof(double)));
+        }
+        io = dev_alloc<double>(n);
+        io2 = dev_alloc<double>(n);
+        io3 = dev_alloc<double>(n);
+        state = dev_alloc<SolveState>(1);
+        hist = dev_alloc<double>((size_t)n + 1);
+        int max_blocks = 4096;
+        for (const DevPart* p : {&full, &lower, &upper, &lower_restart})
+            max_blocks = std::max(max_blocks, p->nslices + 1);
+        partials = dev_alloc<double>(max_blocks);
+        int ngr = 1;
+        if (sweeps) ngr = std::max({ngr, lower.ngroups(), upper.ngroups()});
+        grp = dev_alloc<double>(ngr);
+        SolveState init{};
+        init.hist = hist;
+        init.maxK = n;
+        SFA_HIP(hipMemcpy(state, &init, sizeof(SolveState), hipMemcpyHostToDevice));
+        SFA_HIP(hipHostMalloc((void**)&host_state, sizeof(SolveState), hipHostMallocDefault));
+    }
+
+    // ---------------------------------------------------------------- phases
+
+    double c2() const { return (2. - omega) / omega; }
+    double c3() const { return 2. / omega - 1.; }
+
+    void forward_sweep(hipStream_t s, const DevPart& L, int update_xr)
+    {
+        for (int g = 0; g < L.ngroups(); g++)
+            launch_forward(s, L.view(), L.group_begin(g), L.group_count(g), state, update_xr, x,
+                           r, d, q, m, diag, omega);
+    }
+
+    void backward_sweep(hipStream_t s, int reduce)
+    {
+        const int G = upper.ngroups();
+        for (int g = 0; g < G; g++)
+            launch_backward(s, upper.view(), upper.group_begin(g), upper.group_count(g), state,
+                            reduce, g, g == G - 1, r, m, z, diag, omega, c2(), grp, partials);
+    }
+
+    // Restart (K = 0) of each variant.
+    void restart(hipStream_t s)
+    {
+        const SellView F = full.view();
+        switch (method) {
+        case SFA_METHOD_CG:
+            launch_residual(s, RES_PLAIN, F, full.nslices, state, b, x, r, d, nullptr, nullptr,
+                            partials);
+            break;
+        case SFA_METHOD_DIAGONAL:
+            launch_residual(s, RES_JACOBI, F, full.nslices, state, b, x, r, d, z, diag, partials);
+            break;
+        case SFA_METHOD_SSOR:
+        case SFA_METHOD_RAINBOW:
+            launch_residual(s, RES_ONLY, F, full.nslices, state, b, x, r, nullptr, nullptr,
+                            nullptr, partials);
+            forward_sweep(s, lower, 0);
+            backward_sweep(s, 0);
+            launch_direction_init(s, state, n_active, z, r, d, partials);
+            break;
+        default: {  // Eisenstat variants: q = b - A x, then r = lower solve of q
+            launch_residual(s, RES_ONLY, F, full.nslices, state, b, x, q, nullptr, nullptr,
+                            nullptr, partials);
+            const DevPart& L = has_lower_restart ? lower_restart : lower;
+            for (int g = 0; g < L.ngroups(); g++)
+                launch_eisenstat_solve(s, L.view(), L.group_begin(g), L.group_count(g), state, 0,
+                                       q, nullptr, r, diag, omega, 0.);
+            launch_eisenstat_init(s, state, n_active, r, diag, omega, z, d, partials);
+            break;
+        }
+        }
+    }
+
+    // One CG iteration of each variant.
+    void iteration(hipStream_t s)
+    {
+        const SellView F = full.view();
+        switch (method) {
+        case SFA_METHOD_CG:
+            launch_spmv_dot(s, F, full.nslices, state, d, q, partials);
+            launch_update(s, UPD_PLAIN, state, n_active, x, r, d, q, nullptr, nullptr, omega,
+                          partials);
+            launch_direction(s, state, n_active, r, d);
+            break;
+        case SFA_METHOD_DIAGONAL:
+            launch_spmv_dot(s, F, full.nslices, state, d, q, partials);
+            launch_update(s, UPD_JACOBI, state, n_active, x, r, d, q, z, diag, omega, partials);
+            launch_direction(s, state, n_active, z, d);
+            break;
+        case SFA_METHOD_SSOR:
+        case SFA_METHOD_RAINBOW:
+            launch_spmv_dot(s, F, full.nslices, state, d, q, partials);
+            forward_sweep(s, lower, 1);
+            backward_sweep(s, 1);
+            launch_direction(s, state, n_active, z, d);
+            break;
+        default: {  // Eisenstat: t (in m) = upper solve of d, q = lower solve
+            for (int g = 0; g < upper.ngroups(); g++)
+                launch_eisenstat_solve(s, upper.view(), upper.group_begin(g), upper.group_count(g),
+                                       state, 0, d, nullptr, m, diag, omega, 0.);
+            for (int g = 0; g < lower.ngroups(); g++)
+                launch_eisenstat_solve(s, lower.view(), lower.group_begin(g), lower.group_count(g),
+                                       state, 1, d, m, q, diag, omega, c3());
+            launch_eisenstat_combine(s, state, n_active, q, m, d, partials);
+            launch_update(s, UPD_EISENSTAT, state, n_active, x, r, m, q, z, diag, omega, partials);
+            launch_direction(s, state, n_active, z, d);
+            break;
+        }
+        }
+    }
+
+    int kernels_per_iteration() const
+    {
+        switch (method) {
+        case SFA_METHOD_CG:
+        case SFA_METHOD_DIAGONAL: return 3;
+        case SFA_METHOD_SSOR:
+        case SFA_METHOD_RAINBOW: return 2 + lower.ngroups() + upper.ngroups();
+        default: return 3 + lower.ngroups() + upper.ngroups();
+        }
+    }
+
+    void build_chunk_graph(int iters)
+    {
+        if (chunk_exec && chunk_iters == iters) return;
+        if (chunk_exec) {
+            SFA_HIP(hipGraphExecDestroy(chunk_exec));
+            chunk_exec = nullptr;
+        }
+        hipGraph_t graph = nullptr;
+        SFA_HIP(hipStreamBeginCapture(own_stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < iters; i++) iteration(own_stream);
+        SFA_HIP(hipStreamEndCapture(own_stream, &graph));
+        hipError_t e = hipGraphInstantiate(&chunk_exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) throw Error{SFA_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)};
+        chunk_iters = iters;
+    }
+
+    void read_state()
+    {
+        SFA_HIP(hipMemcpyAsync(host_state, state, sizeof(SolveState), hipMemcpyDeviceToHost, stream));
+        SFA_HIP(hipStreamSynchronize(stream));
+    }
+
+    // Fills the scratch vectors the way the reference allocates them:
+    // zeros for SSOR / Rainbow (:493-494, :1056-1057), NaN otherwise.
+    void init_scratch(hipStream_t s)
+    {
+        const double init = (method == SFA_METHOD_SSOR || method == SFA_METHOD_RAINBOW) ? 0.0 : NAN;
+        for (double* v : {d, z, q, m}) launch_fill(s, v, init, n);
+    }
+
+    // ---------------------------------------------------------------- solve
+
+    void solve(double* ux, const double* ub, double* ures, double precision, int print_modulo,
+               int device_pointers, sfa_solve_info* info)
+    {
+        if (!(precision > 0)) throw Error{SFA_ERR_INVALID, "precision must be positive"};
+        SFA_HIP(hipSetDevice(device));
+        hipStream_t s = stream;
+        const double* src_x = ux;
+        const double* src_b = ub;
+        const double* src_r = ures;
+        double host_tol = 0.;
+        int use_host_tol = 0;
+        if (!device_pointers) {
+            // solver/sparsesolver.cpp:469-472 on the host, in row order
+            double t = 0.;
+            for (int i = 0; i < n; i++) t += ub[i] * ub[i];
+            t = precision * std::sqrt(t);
+            const double floor_ = 1.4901161193847656e-08;
+            if (!(t >= floor_) && t <= floor_) t = floor_;
+            host_tol = t;
+            use_host_tol = 1;
+            SFA_HIP(hipMemcpyAsync(io, ux, n * sizeof(double), hipMemcpyHostToDevice, s));
+            SFA_HIP(hipMemcpyAsync(io2, ub, n * sizeof(double), hipMemcpyHostToDevice, s));
+            SFA_HIP(hipMemcpyAsync(io3, ures, n * sizeof(double), hipMemcpyHostToDevice, s));
+            src_x = io;
+            src_b = io2;
+            src_r = io3;
+        }
+        launch_gather(s, x, src_x, perm, n);
+        launch_gather(s, b, src_b, perm, n);
+        launch_gather(s, r, src_r, perm, n);
+        init_scratch(s);
+        const int small_breaks = (method == SFA_METHOD_SSOR_EISENSTAT) ? (print_modulo >= -2) : 1;
+        launch_init_state(s, state, b, n, precision, host_tol, use_host_tol, n, small_breaks,
+                          partials);
+        restart(s);
+        read_state();
+
+        // Iterations: replay a captured chunk, predicting from the z.r history
+        // how many chunks to queue before the next look at the state.
+        const int G = 4;
+        build_chunk_graph(G);
+        std::vector<double> hist_host;
+        int queued = 4;
+        while (!host_state->done) {
+            for (int i = 0; i < queued; i++) SFA_HIP(hipGraphLaunch(chunk_exec, s));
+            read_state();
+            if (host_state->done) break;
+            queued = predict_chunks(G);
+        }
+
+        // results back to the caller's order
+        double* dst_x = device_pointers ? ux : io;
+        double* dst_r = device_pointers ? ures : io3;
+        launch_scatter(s, dst_x, x, perm, n);
+        launch_scatter(s, dst_r, r, perm, n);
+        if (!device_pointers) {
+            SFA_HIP(hipMemcpyAsync(ux, io, n * sizeof(double), hipMemcpyDeviceToHost, s));
+            SFA_HIP(hipMemcpyAsync(ures, io3, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        }
+        SFA_HIP(hipStreamSynchronize(s));
+
+        const SolveState& st = *host_state;
+        sfa_solve_info out{};
+        out.iterations = st.K;
+        out.zr = st.zr;
+        out.tolerance = st.tol;
+        out.exit_reason = st.exit_reason;
+        out.success = std::sqrt(st.zr) < st.tol ? 1 : 0;
+        if (info) *info = out;
+        if (print_modulo >= -2) print_log(print_modulo, st);
+    }
+
+    int predict_chunks(int G)
+    {
+        // Geometric model of the last few z.r values: remaining iterations
+        // until |z.r| < tol^2, capped to keep the look-ahead short.
+        const int K = host_state->K;
+        const double zr = host_state->zr, tol2 = host_state->tol2;
+        const int look = std::min(K, 8);
+        int want = 2 * G;
+        if (look >= 2 && zr > 0) {
+            double hk[2];
+            SFA_HIP(hipMemcpy(&hk[0], hist + (K - look), sizeof(double), hipMemcpyDeviceToHost));
+            hk[1] = zr;
+            if (hk[0] > zr && hk[0] > 0) {
+                const double rate = std::log(zr / hk[0]) / look;  // < 0
+                const double need = std::log(tol2 / zr) / rate;
+                if (need > 0 && std::isfinite(need)) want = (int)std::ceil(need) + 1;
+                else want = 1;
+            }
+        }
+        int chunks = (want + G - 1) / G;
+        return std::max(1, std::min(chunks, 16));
+    }
+
+    void print_log(int pm, const SolveState& st)
+    {
+        // Lines and formats of solver/sparsesolver.cpp:498-499,586-588,593-596,630-641,713-714.
+        const int K = st.K;
+        if (pm >= -1) std::printf("(%d/%d)     BEGIN: %s\n", 0, n, method_title(method));
+        std::vector<double> h(K + 1);
+        if (pm >= 0 && K + 1 > 0)
+            SFA_HIP(hipMemcpy(h.data(), hist, (K + 1) * sizeof(double), hipMemcpyDeviceToHost));
+        if (pm >= 0)
+            std::printf("(%d/%d)   RESTART: Residual norm is %.9le < %.9le\n", 0, n, std::sqrt(h[0]), st.tol);
+        if (pm >= 1) {
+            const int last = st.exit_reason == SFA_EXIT_CAP ? K - 1 : K;
+            for (int k = 0; k <= last; k += pm)
+                std::printf("(%d/%d)   INTERIM: Residual norm is %.9le < %.9le\n", k, n, std::sqrt(h[k]), st.tol);
+        }
+        if (st.exit_reason == SFA_EXIT_BREAKDOWN)
+            std::printf("(%d/%d) BREAKDOWN: Gradient energy is unreasonable with %.9le\n", K, n, st.dAd);
+        if (st.exit_reason == SFA_EXIT_SMALL) {
+            std::printf("(%d/%d)   WARNING: Residual norm is %.9le < %.9le\n", K, n, std::sqrt(st.zr), st.tol);
+            std::printf("(%d/%d)   WARNING: Gradient energy is small with %.9le\n", K, n, st.dAd);
+        }
+        if (pm >= -1)
+            std::printf("(%d/%d) %9s: Residual norm is %.9le < %.9le\n", K, n,
+                        std::sqrt(st.zr) < st.tol ? "SUCCESS" : "FAILED", std::sqrt(st.zr), st.tol);
+        std::fflush(stdout);
+    }
+
+    void apply(double* dest, const double* src, double scaling, int device_pointers)
+    {
+        SFA_HIP(hipSetDevice(device));
+        if (!has_apply_part) {
+            SellPart p;
+            build_full_identity(p, n, n, host_A.data(), host_C.data(), host_V.data());
+            apply_part.upload(p);
+            has_apply_part = true;
+        }
+        hipStream_t s = stream;
+        // source with a zero slot at n: use the x-sized buffer m
+        double* srcbuf = q;
+        if (device_pointers)
+            SFA_HIP(hipMemcpyAsync(srcbuf, src, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+        else
+            SFA_HIP(hipMemcpyAsync(srcbuf, src, n * sizeof(double), hipMemcpyHostToDevice, s));
+        launch_fill(s, srcbuf + n, 0.0, 1);
+        double* out = device_pointers ? dest : io;
+        launch_apply(s, apply_part.view(), apply_part.nslices, srcbuf, out, scaling);
+        if (!device_pointers)
+            SFA_HIP(hipMemcpyAsync(dest, io, n * sizeof(double), hipMemcpyDeviceToHost, s));
+        SFA_HIP(hipStreamSynchronize(s));
+    }
+
+    void query(sfa_system_stats* st) const
+    {
+        sfa_system_stats o{};
+        o.n = n;
+        o.n_active = n_active;
+        o.nnz = nnz;
+        o.groups_forward = sweeps ? lower.ngroups() : 0;
+        o.groups_backward = sweeps ? upper.ngroups() : 0;
+        o.stored_entries = full.stored + lower.stored + upper.stored;
+        // Algorithmic bytes: 12 B per stored-matrix entry read (8 value + 4
+        // column), the gathered vector entry counted once per row's vector
+        // stream, 4 B per row index read, 8 B per vector element moved.
+        const double na = n_active;
+        const double spmv = 12.0 * full.entries + 4.0 * na + 8.0 * 3 * na;  // d gather, d_i, q
+        double it = spmv;
+        if (method == SFA_METHOD_SSOR || method == SFA_METHOD_RAINBOW) {
+            it = spmv + 12.0 * lower.entries + 4.0 * na + 8.0 * 8 * na   // x,d,r,q,r,m,diag + m gather
+                 + 12.0 * upper.entries + 4.0 * na + 8.0 * 5 * na       // m,diag,z,r + z gather
+                 + 8.0 * 3 * na;                                        // direction
+        } else if (method == SFA_METHOD_CG || method == SFA_METHOD_DIAGONAL) {
+            it = spmv + 8.0 * 6 * na + 8.0 * 3 * na;
+        } else {
+            it = 12.0 * lower.entries + 12.0 * upper.entries + 8.0 * 20 * na;
+        }
+        o.bytes_spmv = spmv;
+        o.bytes_per_iteration = it;
+        o.kernels_per_iteration = kernels_per_iteration();
+        *st = o;
+    }
+};
+
+}  // namespace sfa
+
+using sfa::Error;
+using sfa::System;
+
+struct sfa_system_s {
+    System sys;
+};
+
+namespace {
+
+template <class Fn>
+int guarded(Fn&& fn)
+{
+    try {
+        fn();
+        sfa::g_last_error.clear();
+        return SFA_OK;
+    } catch (const Error& e) {
+        sfa::g_last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        sfa::g_last_error = "host allocation failed";
+        return SFA_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        sfa::g_last_error = e.what();
+        return SFA_ERR_INVALID;
+    }
+}
+
+int solve_host(int method, int N, double* x, const double* b, const int* A, const int* C,
+               const double* V, double* residual, double precision, int print_modulo,
+               const double* aux, double omega, int nc, const int* F, const int* B, const int* R,
+               sfa_solve_info* info)
+{
+    return guarded([&] {
+        if (!x || !b || !residual) throw Error{SFA_ERR_INVALID, "null x, b or residual"};
+        System sys;
+        sys.create(method, N, A, C, V, aux, omega, nc, F, B, R);
+        sys.solve(x, b, residual, precision, print_modulo, 0, info);
+    });
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* sfa_last_error(void) { return sfa::g_last_error.c_str(); }
+
+int sfa_api_version(void) { return SFA_API_VERSION; }
+
+int sfa_ConjugateGradientSolverCSR(int N, double* x, const double* b, const int* csrrows,
+                                   const int* csrcolumns, const double* csrvalues,
+                                   double* residual, double precision, int print_modulo,
+                                   sfa_solve_info* info)
+{
+    return solve_host(SFA_METHOD_CG, N, x, b, csrrows, csrcolumns, csrvalues, residual, precision,
+                      print_modulo, nullptr, 1.0, 0, nullptr, nullptr, nullptr, info);
+}
+
+int sfa_ConjugateGradientSolverCSR_DiagonalPreconditioner(
+    int N, double* x, const double* b, const int* csrrows, const int* csrcolumns,
+    const double* csrvalues, double* residual, double precision, int print_modulo,
+    const double* precon, sfa_solve_info* info)
+{
+    return solve_host(SFA_METHOD_DIAGONAL, N, x, b, csrrows, csrcolumns, csrvalues, residual,
+                      precision, print_modulo, precon, 1.0, 0, nullptr, nullptr, nullptr, info);
+}
+
+int sfa_ConjugateGradientSolverCSR_SSOR(int N, double* x, const double* b, const int* csrrows,
+                                        const int* csrcolumns, const double* csrvalues,
+                                        double* residual, double precision, int print_modulo,
+                                        const double* diagonal, double omega,
+                                        sfa_solve_info* info)
+{
+    return solve_host(SFA_METHOD_SSOR, N, x, b, csrrows, csrcolumns, csrvalues, residual,
+                      precision, print_modulo, diagonal, omega, 0, nullptr, nullptr, nullptr,
+                      info);
+}
+
+int sfa_ConjugateGradientSolverCSR_SSOR_Eisenstat(int N, double* x, const double* b,
+                                                  const int* csrrows, const int* csrcolumns,
+                                                  const double* csrvalues, double* residual,
+                                                  double precision, int print_modulo,
+                                                  const double* diagonal, double omega,
+                                                  sfa_solve_info* info)
+{
+    return solve_host(SFA_METHOD_SSOR_EISENSTAT, N, x, b, csrrows, csrcolumns, csrvalues,
+                      residual, precision, print_modulo, diagonal, omega, 0, nullptr, nullptr,
+                      nullptr, info);
+}
+
+int sfa_ConjugateGradientSolverCSR_Rainbow(int N, double* x, const double* b, const int* csrrows,
+                                           const int* csrcolumns, const double* csrvalues,
+                                           double* residual, double precision, int print_modulo,
+                                           const double* diagonal, double omega, int num_colors,
+                                           const int* F, const int* B, const int* R,
+                                           sfa_solve_info* info)
+{
+    return solve_host(SFA_METHOD_RAINBOW, N, x, b, csrrows, csrcolumns, csrvalues, residual,
+                      precision, print_modulo, diagonal, omega, num_colors, F, B, R, info);
+}
+
+int sfa_ConjugateGradientSolverCSR_Eisenstat_Rainbow(
+    int N, double* x, const double* b, const int* csrrows, const int* csrcolumns,
+    const double* csrvalues, double* residual, double precision, int print_modulo,
+    const double* diagonal, double omega, int num_colors, const int* F, const int* B,
+    const int* R, sfa_solve_info* info)
+{
+    return solve_host(SFA_METHOD_EISENSTAT_RAINBOW, N, x, b, csrrows, csrcolumns, csrvalues,
+                      residual, precision, print_modulo, diagonal, omega, num_colors, F, B, R,
+                      info);
+}
+
+int sfa_MatrixCSR_apply(int rows, int columns, const int* A, const int* C, const double* V,
+                        double* dest, const double* src, double scaling)
+{
+    return guarded([&] {
+        std::string err = sfa::check_csr_rect(rows, columns, A, C);
+        if (!err.empty()) throw Error{SFA_ERR_INVALID, err};
+        if (!V || !dest || !src) throw Error{SFA_ERR_INVALID, "null pointer"};
+        sfa::ensure_device();
+        sfa::SellPart p;
+        sfa::build_full_identity(p, rows, columns, A, C, V);
+        sfa::DevPart dp;
+        double *dsrc = nullptr, *ddst = nullptr;
+        try {
+            dp.upload(p);
+            dsrc = sfa::dev_alloc<double>(columns + 1);
+            ddst = sfa::dev_alloc<double>(rows);
+            SFA_HIP(hipMemcpy(dsrc, src, columns * sizeof(double), hipMemcpyHostToDevice));
+            SFA_HIP(hipMemset(dsrc + columns, 0, sizeof(double)));
+            sfa::launch_apply(nullptr, dp.view(), dp.nslices, dsrc, ddst, scaling);
+            SFA_HIP(hipGetLastError());
+            SFA_HIP(hipMemcpy(dest, ddst, rows * sizeof(double), hipMemcpyDeviceToHost));
+        } catch (...) {
+            dp.release();
+            sfa::dev_free(dsrc);
+            sfa::dev_free(ddst);
+            throw;
+        }
+        dp.release();
+        sfa::dev_free(dsrc);
+        sfa::dev_free(ddst);
+    });
+}
+
+int sfa_MatrixCSR_getDiagonal(int n, const int* A, const int* C, const double* V, double* diag)
+{
+    return guarded([&] {
+        std::string err = sfa::check_csr(n, A, C);
+        if (!err.empty()) throw Error{SFA_ERR_INVALID, err};
+        if (!V || !diag) throw Error{SFA_ERR_INVALID, "null pointer"};
+        for (int r = 0; r < n; r++) {
+            double s = 0.;
+            for (int k = A[r]; k < A[r + 1]; k++)
+                if (C[k] == r) s += V[k];
+            diag[r] = s;
+        }
+    });
+}
+
+int sfa_Rainbow(int n, const int* A, const int* C, int* F, int* B, int* R, int* num_colors)
+{
+    return guarded([&] {
+        std::string err = sfa::check_csr(n, A, C);
+        if (!err.empty()) throw Error{SFA_ERR_INVALID, err};
+        if (!F || !B || !R || !num_colors) throw Error{SFA_ERR_INVALID, "null pointer"};
+        *num_colors = sfa::rainbow_colour(n, A, C, F, B, R);
+    });
+}
+
+int sfa_system_create(int method, int N, const int* csrrows, const int* csrcolumns,
+                      const double* csrvalues, const double* aux, double omega, int num_colors,
+                      const int* F, const int* B, const int* R, sfa_system* out)
+{
+    return guarded([&] {
+        if (!out) throw Error{SFA_ERR_INVALID, "null output handle"};
+        *out = nullptr;
+        std::unique_ptr<sfa_system_s> h(new sfa_system_s());
+        h->sys.create(method, N, csrrows, csrcolumns, csrvalues, aux, omega, num_colors, F, B, R);
+        *out = h.release();
+    });
+}
+
+int sfa_system_solve(sfa_system sys, double* x, const double* b, double* residual,
+                     double precision, int print_modulo, int device_pointers,
+                     sfa_solve_info* info)
+{
+    return guarded([&] {
+        if (!sys || !x || !b || !residual) throw Error{SFA_ERR_INVALID, "null argument"};
+        sys->sys.solve(x, b, residual, precision, print_modulo, device_pointers, info);
+    });
+}
+
+int sfa_system_apply(sfa_system sys, double* dest, const double* src, double scaling,
+                     int device_pointers)
+{
+    return guarded([&] {
+        if (!sys || !dest || !src) throw Error{SFA_ERR_INVALID, "null argument"};
+        sys->sys.apply(dest, src, scaling, device_pointers);
+    });
+}
+
+int sfa_system_set_stream(sfa_system sys, void* hip_stream)
+{
+    return guarded([&] {
+        if (!sys) throw Error{SFA_ERR_INVALID, "null system"};
+        sys->sys.stream = hip_stream ? (hipStream_t)hip_stream : sys->sys.own_stream;
+    });
+}
+
+int sfa_system_query(sfa_system sys, sfa_system_stats* stats)
+{
+    return guarded([&] {
+        if (!sys || !stats) throw Error{SFA_ERR_INVALID, "null argument"};
+        sys->sys.query(stats);
+    });
+}
+
+int sfa_system_destroy(sfa_system sys)
+{
+    return guarded([&] { delete sys; });
+}
+
+}  // extern "C"

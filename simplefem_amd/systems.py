@@ -1,0 +1,279 @@
+"""Synthetic CSR systems of the kind FEEC++ hands to its CG solvers.
+
+* ``fd_laplacian_2d``      the 5-point finite-difference Laplacian of
+                           tests/solver/solvespd_csr.cpp:157-177 (h = 1/(N+1)).
+* ``poisson2d_p1``         P1 Lagrange stiffness on the unit square of
+                           mesh/examples2D.hpp:194-211 (two triangles sharing
+                           the (0,0)-(1,1) diagonal) after ``levels`` uniform
+                           refinements: a (2^L+1)^2 vertex grid.
+* ``poisson3d_p1_kuhn``    P1 Lagrange stiffness on n^3 cubes, each split into
+                           the six Kuhn tetrahedra of mesh/examples3D.hpp:14-37
+                           (all sharing the (0,0,0)-(1,1,1) diagonal).
+
+The FEM matrices keep the structural pattern of the assembly
+(fem/lagrangematrices.cpp:210-271 before exact zeros are dropped): every
+vertex pair that shares an element has an entry, entries that touch a
+Dirichlet (boundary) vertex are 0, and so is the diagonal of a Dirichlet row,
+which makes the SSOR-family solvers skip that row.  Element contributions are
+summed in exact integer arithmetic (units of 1/2 in 2D, h/6 in 3D) and
+rounded once, so couplings that vanish are exact zeros.
+
+Vertices are numbered lexicographically (x fastest).  The reference numbers
+the vertices of a refined mesh in refinement order instead; the matrices are
+the same up to that symmetric permutation.
+
+``rhs_uniform`` draws the seeded U[-1,1) right-hand side (SplitMix64 stream)
+on the active rows and puts 0 on Dirichlet rows, as the reference's RHS
+assembly does (tests/sullivan2D/poissondirichletlshaped.cpp:149-176).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+
+@dataclass
+class CSRSystem:
+    name: str
+    n: int
+    A: np.ndarray          # int32 row offsets (n+1)
+    C: np.ndarray          # int32 column indices
+    V: np.ndarray          # float64 values
+    dirichlet: np.ndarray  # bool per row
+
+    @property
+    def nnz(self) -> int:
+        return int(self.A[-1])
+
+    def diagonal(self) -> np.ndarray:
+        rows = np.repeat(np.arange(self.n), np.diff(self.A))
+        d = np.zeros(self.n)
+        on = self.C == rows
+        np.add.at(d, rows[on], self.V[on])
+        return d
+
+    def dense(self) -> np.ndarray:
+        M = np.zeros((self.n, self.n))
+        rows = np.repeat(np.arange(self.n), np.diff(self.A))
+        np.add.at(M, (rows, self.C), self.V)
+        return M
+
+
+def _coo_to_csr(n, rows, cols, vals):
+    """Sums duplicates, sorts columns inside each row."""
+    import scipy.sparse as sp
+
+    m = sp.coo_matrix((vals, (np.asarray(rows, dtype=np.int64), np.asarray(cols, dtype=np.int64))),
+                      shape=(n, n)).tocsr()
+    m.sum_duplicates()
+    m.sort_indices()
+    return m.indptr.astype(np.int32), m.indices.astype(np.int32), m.data
+
+
+def fd_laplacian_2d(N: int) -> CSRSystem:
+    """tests/solver/solvespd_csr.cpp:157-177: (4, -1, -1, -1, -1)/h^2 on an N x N grid."""
+    h = 1.0 / (N + 1)
+    h2 = h * h
+    rows, cols, vals = [], [], []
+    for e in range(N * N):
+        x, y = divmod(e, N)
+        rows.append(e); cols.append(e); vals.append(4.0 / h2)
+        if x != 0:
+            rows.append(e); cols.append((x - 1) * N + y); vals.append(-1.0 / h2)
+        if x != N - 1:
+            rows.append(e); cols.append((x + 1) * N + y); vals.append(-1.0 / h2)
+        if y != 0:
+            rows.append(e); cols.append(x * N + y - 1); vals.append(-1.0 / h2)
+        if y != N - 1:
+            rows.append(e); cols.append(x * N + y + 1); vals.append(-1.0 / h2)
+    A, C, V = _coo_to_csr(N * N, np.array(rows), np.array(cols), np.array(vals))
+    return CSRSystem(f"fd2d_N{N}", N * N, A, C, V, np.zeros(N * N, dtype=bool))
+
+
+def _assemble(n_vert, elems, kref_int, scale, dirichlet):
+    """elems: (E, nv) vertex ids of each element of one reference type;
+    kref_int: integer element matrix (nv, nv) in units of `scale`."""
+    nv = elems.shape[1]
+    rows = np.repeat(elems, nv, axis=1).ravel()
+    cols = np.tile(elems, (1, nv)).ravel()
+    vals = np.tile(kref_int.ravel(), elems.shape[0]).astype(np.int64)
+    return rows, cols, vals
+
+
+def _finish(name, n_vert, parts, scale, dirichlet):
+    rows = np.concatenate([p[0] for p in parts])
+    cols = np.concatenate([p[1] for p in parts])
+    vals = np.concatenate([p[2] for p in parts])
+    touch = dirichlet[rows] | dirichlet[cols]
+    vals = np.where(touch, 0, vals)
+    # integer sums (exact); explicit zeros stay in the pattern
+    A, C, Vint = _coo_to_csr(n_vert, rows, cols, vals.astype(np.int64))
+    V = Vint.astype(np.float64) * scale
+    return CSRSystem(name, n_vert, A, C, V, dirichlet)
+
+
+def _p1_element_matrix(coords):
+    """Integer (b_i.b_j) matrix of barycentric gradients times (d! * volume),
+    for integer vertex coordinates of a simplex of volume 1/d!."""
+    coords = np.asarray(coords, dtype=np.float64)
+    d = coords.shape[1]
+    T = (coords[1:] - coords[0]).T                 # d x d
+    Tinv = np.linalg.inv(T)
+    G = np.vstack([-Tinv.sum(axis=0), Tinv])       # gradients of barycentrics (rows)
+    K = G @ G.T
+    Ki = np.rint(K).astype(np.int64)
+    assert np.allclose(K, Ki), "reference simplex must have integer gradient products"
+    return Ki
+
+
+def _element_assembled_2d(levels: int) -> CSRSystem:
+    """Element-by-element P1 assembly on the refined unit square (slow; the
+    structured builder below is checked against it)."""
+    m = 2 ** levels
+    nx = m + 1
+    ix, iy = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
+    ix, iy = ix.ravel(), iy.ravel()
+    vid = lambda a, b: (b * nx + a)
+    v00, v10, v01, v11 = vid(ix, iy), vid(ix + 1, iy), vid(ix, iy + 1), vid(ix + 1, iy + 1)
+    # triangles {0,1,3} and {0,2,3} of mesh/examples2D.hpp:205-206
+    t1 = np.stack([v00, v10, v11], axis=1)
+    t2 = np.stack([v00, v01, v11], axis=1)
+    k1 = _p1_element_matrix([[0, 0], [1, 0], [1, 1]])  # units of 1/2 (area 1/2, K = G G^T / 2)
+    k2 = _p1_element_matrix([[0, 0], [0, 1], [1, 1]])
+    gx, gy = np.meshgrid(np.arange(nx), np.arange(nx), indexing="xy")
+    gx, gy = gx.ravel(), gy.ravel()
+    dirichlet = (gx == 0) | (gy == 0) | (gx == m) | (gy == m)
+    parts = [_assemble(nx * nx, t1, k1, 0.5, dirichlet), _assemble(nx * nx, t2, k2, 0.5, dirichlet)]
+    return _finish(f"poisson2d_p1_L{levels}", nx * nx, parts, 0.5, dirichlet)
+
+
+_KUHN = [(4, 6, 7), (4, 5, 7), (2, 6, 7), (2, 3, 7), (1, 5, 7), (1, 3, 7)]  # mesh/examples3D.hpp:27-32
+
+
+def _element_assembled_3d(n: int) -> CSRSystem:
+    """Element-by-element P1 assembly on n^3 Kuhn cubes (slow; the structured
+    builder below is checked against it)."""
+    nx = n + 1
+    h = 1.0 / n
+    ix, iy, iz = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    ix, iy, iz = ix.ravel(), iy.ravel(), iz.ravel()
+
+    def corner(bits):  # bit 2 = x, bit 1 = y, bit 0 = z (mesh/examples3D.hpp:18-25)
+        dx, dy, dz = (bits >> 2) & 1, (bits >> 1) & 1, bits & 1
+        return (ix + dx) + nx * ((iy + dy) + nx * (iz + dz))
+
+    parts = []
+    dirichlet = _boundary_3d(n)
+    for path in _KUHN:
+        bits = (0,) + path
+        coords = [[(bb >> 2) & 1, (bb >> 1) & 1, bb & 1] for bb in bits]
+        kint = _p1_element_matrix(coords)            # K = h * (G G^T) / 6
+        elems = np.stack([corner(bb) for bb in bits], axis=1)
+        parts.append(_assemble(nx ** 3, elems, kint, h / 6.0, dirichlet))
+    return _finish(f"poisson3d_p1_kuhn_{n}", nx ** 3, parts, h / 6.0, dirichlet)
+
+
+def _boundary_3d(n):
+    nx = n + 1
+    g = np.arange(nx)
+    gz, gy, gx = np.meshgrid(g, g, g, indexing="ij")  # vertex id = gx + nx*(gy + nx*gz)
+    return ((gx == 0) | (gy == 0) | (gz == 0) | (gx == n) | (gy == n) | (gz == n)).ravel()
+
+
+def _structured(name, dims, stencil_units, scale, dirichlet):
+    """CSR of a translation-invariant stencil on a vertex grid (x fastest),
+    keeping every in-grid neighbour; couplings with a Dirichlet end are 0."""
+    dims = tuple(dims)
+    nvert = int(np.prod(dims))
+    coords = np.stack(np.unravel_index(np.arange(nvert), dims[::-1])[::-1], axis=1)  # (x, y, z)
+    strides = np.cumprod((1,) + dims[:-1])
+    offs = sorted(stencil_units.items(), key=lambda kv: int(np.dot(kv[0], strides)))
+    delta = np.array([o for o, _ in offs])
+    units = np.array([u for _, u in offs], dtype=np.int64)
+    nb = coords[:, None, :] + delta[None, :, :]
+    inside = np.all((nb >= 0) & (nb < np.array(dims)), axis=2)
+    cols = (nb * strides).sum(axis=2)
+    A = np.zeros(nvert + 1, dtype=np.int64)
+    A[1:] = np.cumsum(inside.sum(axis=1))
+    C = cols[inside].astype(np.int32)
+    rows = np.repeat(np.arange(nvert), inside.sum(axis=1))
+    U = np.broadcast_to(units, inside.shape)[inside]
+    U = np.where(dirichlet[rows] | dirichlet[C], 0, U)
+    return CSRSystem(name, nvert, A.astype(np.int32), C, U.astype(np.float64) * scale, dirichlet)
+
+
+def _offsets_units(patch: CSRSystem, centre_xyz, dims, scale):
+    strides = np.cumprod((1,) + tuple(dims[:-1]))
+    centre = int(np.dot(centre_xyz, strides))
+    lo, hi = patch.A[centre], patch.A[centre + 1]
+    out = {}
+    for c, v in zip(patch.C[lo:hi], patch.V[lo:hi]):
+        xyz = np.array(np.unravel_index(int(c), tuple(dims)[::-1])[::-1])
+        out[tuple(int(t) for t in xyz - np.array(centre_xyz))] = int(round(v / scale))
+    return out
+
+
+def poisson2d_p1(levels: int) -> CSRSystem:
+    """P1 stiffness of UnitSquare2D_simple refined `levels` times: (2^L+1)^2 vertices."""
+    m = 2 ** levels
+    nx = m + 1
+    patch = _element_assembled_2d(2)                  # 5x5 vertices, interior stencil at (2,2)
+    units = _offsets_units(patch, (2, 2), (5, 5), 0.5)
+    gx, gy = np.meshgrid(np.arange(nx), np.arange(nx), indexing="xy")
+    dirichlet = ((gx == 0) | (gy == 0) | (gx == m) | (gy == m)).ravel()
+    return _structured(f"poisson2d_p1_L{levels}", (nx, nx), units, 0.5, dirichlet)
+
+
+def poisson3d_p1_kuhn(n: int) -> CSRSystem:
+    """P1 stiffness on n^3 cubes of six Kuhn tetrahedra each: (n+1)^3 vertices."""
+    nx = n + 1
+    patch = _element_assembled_3d(4)                  # 5^3 vertices, interior stencil at (2,2,2)
+    units = _offsets_units(patch, (2, 2, 2), (5, 5, 5), (1.0 / 4) / 6.0)
+    return _structured(f"poisson3d_p1_kuhn_{n}", (nx, nx, nx), units, (1.0 / n) / 6.0,
+                       _boundary_3d(n))
+
+
+def splitmix64(seed: int, count: int) -> np.ndarray:
+    """SplitMix64 outputs for counters seed+1 .. seed+count (uint64)."""
+    with np.errstate(over="ignore"):
+        z = (np.uint64(seed) + np.arange(1, count + 1, dtype=np.uint64)
+             * np.uint64(0x9E3779B97F4A7C15))
+        z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+        z = z ^ (z >> np.uint64(31))
+    return z
+
+
+def uniform_pm1(seed: int, count: int) -> np.ndarray:
+    """U[-1,1) doubles: 53 high bits of SplitMix64."""
+    u = (splitmix64(seed, count) >> np.uint64(11)).astype(np.float64) * (2.0 ** -53)
+    return 2.0 * u - 1.0
+
+
+def rhs_uniform(system: CSRSystem, seed: int) -> np.ndarray:
+    b = uniform_pm1(seed, system.n)
+    b[system.dirichlet] = 0.0
+    return b
+
+
+def random_spd(n: int, density: float, seed: int) -> CSRSystem:
+    """Random sparse symmetric diagonally dominant matrix (positive definite)."""
+    rng = np.random.default_rng(seed)
+    m = max(1, int(density * n * n / 2))
+    i = rng.integers(0, n, m)
+    j = rng.integers(0, n, m)
+    keep = i != j
+    i, j = i[keep], j[keep]
+    v = rng.uniform(-1, 1, i.shape[0])
+    rows = np.concatenate([i, j])
+    cols = np.concatenate([j, i])
+    vals = np.concatenate([v, v])
+    A0, C0, V0 = _coo_to_csr(n, rows, cols, vals)
+    rowsum = np.zeros(n)
+    np.add.at(rowsum, np.repeat(np.arange(n), np.diff(A0)), np.abs(V0))
+    rows = np.concatenate([np.repeat(np.arange(n), np.diff(A0)), np.arange(n)])
+    cols = np.concatenate([C0, np.arange(n)])
+    vals = np.concatenate([V0, rowsum + rng.uniform(0.5, 1.5, n)])
+    A, C, V = _coo_to_csr(n, rows, cols, vals)
+    return CSRSystem(f"random_spd_{n}_{seed}", n, A, C, V, np.zeros(n, dtype=bool))
