@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X sparse solve path.
+
+Workload: 3D P1 Lagrange Poisson on the unit cube, 64^3 cubes of six Kuhn
+tetrahedra each (274,625 vertex rows, 250,047 active, 4,018,753 CSR entries
+with the structural zeros of the assembly), FP64, Rainbow CG-SSOR
+(ConjugateGradientSolverCSR_Rainbow) with the reference's greedy colouring,
+omega = 1.0, precision 1e-10 (tolerance max(1e-10*||b||, 2^-26)), x0 = 0,
+seeded U[-1,1) right-hand side on the active rows.
+
+A step is one complete solve with the matrix, colouring and right-hand side
+already resident in HBM.  Prints one JSON line with solves/s, the roofline of
+the dominant kernel (per-launch HIP events in a profiled solve of the same
+system) and the CPU baseline (the reference's own solver, compiled from its
+sources, timed on this host on a bounded sample).
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "solves/s + HBM GB/s (3D P1 Poisson, rtol 1e-10); p50/p99 served latency"
+PEAK_HBM_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(system, b, d, colouring, precision, omega, budget_s=25.0):
+    """The reference's ConjugateGradientSolverCSR_Rainbow on the host cores."""
+    import oracle
+    from oracle.ref import RefRunner
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    if oracle.ref_available():
+        with RefRunner(cores) as ref:
+            t0 = time.time()
+            x, r, K, times = ref.solve("rainbow", system.A, system.C, system.V, b, aux=d,
+                                       omega=omega, colouring=colouring, precision=precision)
+            reps = max(0, min(4, int((budget_s - (time.time() - t0)) / max(times[0], 1e-3)) - 1))
+            if reps:
+                times += ref.solve("rainbow", system.A, system.C, system.V, b, aux=d,
+                                   omega=omega, colouring=colouring, precision=precision,
+                                   repeat=reps)[3]
+        med = float(np.median(times))
+        return {"value": 1.0 / med, "unit": "solves/s", "cores": cores, "kind": "reference",
+                "sample": f"{len(times)} full solves of the same system (K={K}), median "
+                          f"{med:.3f} s, OMP_NUM_THREADS={cores}"}
+    # no reference build: the oracle's C restatement, one thread
+    t0 = time.time()
+    x, r, info = oracle.solve("rainbow", system.A, system.C, system.V, b, aux=d, omega=omega,
+                              colouring=colouring, precision=precision)
+    dt = time.time() - t0
+    return {"value": 1.0 / dt, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"1 full solve (K={info['iterations']}), {dt:.3f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cube", type=int, default=64, help="cubes per axis")
+    ap.add_argument("--omega", type=float, default=1.0)
+    ap.add_argument("--precision", type=float, default=1e-10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-profile", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path "
+              f"(--gpus {args.gpus} requested).", file=sys.stderr)
+        return 2
+
+    from simplefem_amd import capi
+    from simplefem_amd import systems as S
+    from simplefem_amd.hipmem import DeviceArray, synchronize
+
+    t0 = time.time()
+    system = S.poisson3d_p1_kuhn(args.cube)
+    b = S.rhs_uniform(system, seed=2026)
+    d = capi.get_diagonal(system.n, system.A, system.C, system.V)
+    rb = capi.Rainbow(system.n, system.A, system.C)
+    colouring = (rb.num_colors, rb.F, rb.B, rb.R)
+    log(f"system n={system.n} nnz={system.nnz} colours={rb.num_colors} "
+        f"built in {time.time() - t0:.1f}s")
+
+    sysd = capi.System(capi.METHOD_RAINBOW, system.n, system.A, system.C, system.V, d,
+                       args.omega, *colouring)
+    stats = sysd.stats()
+    bt = DeviceArray.from_host(b)
+    x = DeviceArray(system.n)
+    r = DeviceArray(system.n)
+    info = {}
+
+    def step():
+        x.zero()
+        r.zero()
+        sysd.solve(x, bt, r, args.precision, info=info)
+
+    for i in range(args.warmup):
+        step()
+        log(f"warmup {i + 1}/{args.warmup}: K={info['iterations']}")
+    synchronize()
+    t1 = time.perf_counter()
+    iters = 0
+    for i in range(args.steps):
+        step()
+        iters += info["iterations"]
+    synchronize()
+    elapsed = time.perf_counter() - t1
+    log(f"timed {args.steps} solves in {elapsed:.3f}s, K={info['iterations']}")
+    solves_per_s = args.steps / elapsed
+    hbm_gbs = iters * stats["bytes_per_iteration"] / elapsed / 1e9
+
+    # residual check of the last solve against the tolerance it claims
+    xs = x.download()
+    res = b - capi.csr_apply(system.n, system.n, system.A, system.C, system.V, xs)
+    res[system.dirichlet] = 0.0
+    rel_res = float(np.linalg.norm(res) / np.linalg.norm(b))
+
+    roofline = None
+    prof = None
+    if not args.no_profile:
+        x.zero()
+        r.zero()
+        prof, pinfo = sysd.profile(x, bt, r, args.precision)
+        dom = max(prof, key=lambda k: prof[k]["ms"])
+        p = prof[dom]
+        achieved = p["bytes"] / (p["ms"] * 1e-3) / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
+                    "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                    "kernel": dom, "launches": p["launches"],
+                    "avg_launch_us": round(p["ms"] * 1e3 / max(p["launches"], 1), 3),
+                    "bytes_per_launch": round(p["bytes"] / max(p["launches"], 1))}
+        log("profile: " + ", ".join(f"{k} {v['launches']}x {v['ms']:.2f}ms" for k, v in
+                                     prof.items() if v["launches"]))
+
+    base = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        base = cpu_baseline(system, b, d, colouring, args.precision, args.omega)
+        log(f"cpu baseline {base['value']:.3f} solves/s ({base['kind']}, {base['cores']} cores)")
+
+    out = {
+        "metric": METRIC,
+        "value": round(solves_per_s, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"3D P1 Poisson, {args.cube}^3 Kuhn cubes, Rainbow CG-SSOR "
+                        f"omega={args.omega}, precision {args.precision:g}, x0=0",
+            "rows": system.n, "active_rows": stats["n_active"], "nnz": system.nnz,
+            "colours": rb.num_colors, "iterations": info["iterations"],
+            "kernels_per_iteration": stats["kernels_per_iteration"],
+            "bytes_per_iteration": stats["bytes_per_iteration"],
+            "true_relative_residual": rel_res,
+        },
+        "hbm_gbs_algorithmic": round(hbm_gbs, 1),
+        "roofline": roofline,
+        "cpu_baseline": base,
+    }
+    if prof is not None:
+        out["kernel_profile"] = {k: v for k, v in prof.items() if v["launches"]}
+    print(json.dumps(out), flush=True)
+    sysd.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

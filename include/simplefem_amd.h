@@ -194,6 +194,34 @@ typedef struct sfa_system_stats {
 
 int sfa_system_query(sfa_system sys, sfa_system_stats* stats);
 
+/* Kernel classes of a solve, for sfa_system_profile. */
+#define SFA_KERNEL_SPMV        0  /* q = A d and d.Ad                         */
+#define SFA_KERNEL_FORWARD     1  /* forward sweep of one colour / level      */
+#define SFA_KERNEL_BACKWARD    2  /* backward sweep of one colour / level     */
+#define SFA_KERNEL_DIRECTION   3  /* d = z + beta d                           */
+#define SFA_KERNEL_UPDATE      4  /* x, r update and z.r (CG, Jacobi, Eisenstat) */
+#define SFA_KERNEL_TRIANGULAR  5  /* Eisenstat triangular solve of one group  */
+#define SFA_KERNEL_COMBINE     6  /* Eisenstat q += t and d.Ad                */
+#define SFA_KERNEL_RESTART     7  /* K = 0 residual and set-up kernels        */
+#define SFA_KERNEL_CLASSES     8
+
+typedef struct sfa_kernel_profile {
+    int    launches;
+    int    reserved;
+    double milliseconds;  /* sum over launches of the HIP-event interval    */
+    double bytes;         /* algorithmic HBM bytes of those launches        */
+} sfa_kernel_profile;
+
+/* Runs one solve exactly as sfa_system_solve does, but launches every kernel
+ * on its own (no graph) between two HIP events on the system's stream, and
+ * returns per-class launch counts, summed event time and algorithmic bytes
+ * (out must hold SFA_KERNEL_CLASSES entries).  No reference counterpart;
+ * used to price each kernel against the HBM roofline. */
+int sfa_system_profile(sfa_system sys, double* x, const double* b,
+                       double* residual, double precision,
+                       int device_pointers, sfa_kernel_profile* out,
+                       sfa_solve_info* info);
+
 int sfa_system_destroy(sfa_system sys);
 
 #ifdef __cplusplus

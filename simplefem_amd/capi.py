@@ -50,6 +50,15 @@ class SystemStats(ctypes.Structure):
                 ("reserved", c_int)]
 
 
+class KernelProfile(ctypes.Structure):
+    _fields_ = [("launches", c_int), ("reserved", c_int), ("milliseconds", c_double),
+                ("bytes", c_double)]
+
+
+KERNEL_CLASSES = 8
+KERNEL_NAMES = ["spmv", "forward", "backward", "direction", "update", "triangular", "combine",
+                "restart"]
+
 _P_I = POINTER(c_int)
 _P_D = POINTER(c_double)
 _SOLVER_ARGS = [c_int, _P_D, _P_D, _P_I, _P_I, _P_D, _P_D, c_double, c_int]
@@ -76,6 +85,8 @@ _SIGNATURES = {
     "sfa_system_apply": ([c_void_p, c_void_p, c_void_p, c_double, c_int], c_int),
     "sfa_system_set_stream": ([c_void_p, c_void_p], c_int),
     "sfa_system_query": ([c_void_p, POINTER(SystemStats)], c_int),
+    "sfa_system_profile": ([c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int,
+                            POINTER(KernelProfile), POINTER(SolveInfo)], c_int),
     "sfa_system_destroy": ([c_void_p], c_int),
 }
 
@@ -253,8 +264,8 @@ class Rainbow:
 class System:
     """A matrix + preconditioner schedule resident in HBM (sfa_system_*).
 
-    ``solve`` accepts numpy arrays (host) or torch CUDA tensors (device
-    pointers, float64, contiguous).
+    ``solve`` accepts numpy arrays (host memory) or device buffers
+    (``simplefem_amd.hipmem.DeviceArray``, float64).
     """
 
     def __init__(self, method, N, A, C, V, aux=None, omega=1.0, num_colors=0, F=None, B=None,
@@ -273,10 +284,10 @@ class System:
         if isinstance(a, np.ndarray):
             a = _f64(a, "vector", N, writable)
             return a, a.ctypes.data, 0
-        import torch  # device tensors only
-        if not (a.is_cuda and a.dtype == torch.float64 and a.is_contiguous() and a.numel() >= N):
-            raise TypeError("device vectors must be contiguous float64 CUDA tensors")
-        return a, a.data_ptr(), 1
+        # a device buffer: anything with .address (simplefem_amd.hipmem.DeviceArray)
+        if getattr(a, "n", N) < N or getattr(a, "dtype", np.float64) != np.float64:
+            raise TypeError("device vectors must be float64 with at least N entries")
+        return a, a.address, 1
 
     def solve(self, x, b, residual, precision, print_modulo=-3, info=None):
         x, px, dx = self._ptr(x, self.N, True)
@@ -290,6 +301,21 @@ class System:
         if info is not None:
             info.update(out.as_dict())
         return out.iterations
+
+    def profile(self, x, b, residual, precision):
+        """One solve with every kernel timed by HIP events; per-class totals."""
+        x, px, dx = self._ptr(x, self.N, True)
+        b, pb, db = self._ptr(b, self.N, False)
+        residual, pr, dr = self._ptr(residual, self.N, True)
+        if not (dx == db == dr):
+            raise TypeError("x, b and residual must all be host arrays or all device buffers")
+        out = (KernelProfile * KERNEL_CLASSES)()
+        info = SolveInfo()
+        _check(load().sfa_system_profile(self._h, px, pb, pr, float(precision), dx, out,
+                                         ctypes.byref(info)))
+        prof = {KERNEL_NAMES[c]: {"launches": out[c].launches, "ms": out[c].milliseconds,
+                                  "bytes": out[c].bytes} for c in range(KERNEL_CLASSES)}
+        return prof, info.as_dict()
 
     def apply(self, dest, src, scaling=1.0):
         dest, pd, dd = self._ptr(dest, self.N, True)

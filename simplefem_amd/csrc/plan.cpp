@@ -71,7 +71,6 @@ void append_group(SellPart& part, const std::vector<int>& rows_dev, const std::v
                   const double* V, Keep keep)
 {
     const int count = (int)rows_dev.size();
-    std::vector<int> sel;
     for (int s0 = 0; s0 < count; s0 += kSliceRows) {
         const int s1 = std::min(count, s0 + kSliceRows);
         int width = 0;
@@ -116,7 +115,15 @@ void start_part(SellPart& part)
     part.group_slice.push_back(0);
 }
 
-void close_group(SellPart& part) { part.group_slice.push_back(part.nslices); }
+void close_group(SellPart& part)
+{
+    long long e = part.entries, r = part.rows;
+    for (long long v : part.group_entries) e -= v;
+    for (long long v : part.group_rows) r -= v;
+    part.group_entries.push_back(e);
+    part.group_rows.push_back(r);
+    part.group_slice.push_back(part.nslices);
+}
 
 // Dependency levels of a triangular sweep over the active rows: rows of one
 // level read no row of the same or a later level.  dep(orow, k) tells whether

@@ -32,6 +32,8 @@ struct SellPart {
     std::vector<int> cols;         // device-order column ids, n = zero slot
     std::vector<double> vals;
     std::vector<int> group_slice;  // ngroups+1 slice ranges, in launch order
+    std::vector<long long> group_entries;  // real entries per group
+    std::vector<long long> group_rows;     // real rows per group
     long long entries = 0;         // real (unpadded) entries
     long long rows = 0;            // real rows
     int ngroups() const { return (int)group_slice.size() - 1; }

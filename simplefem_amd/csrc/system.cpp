@@ -68,6 +68,7 @@ struct DevPart {
     double* vals = nullptr;
     int nslices = 0;
     std::vector<int> group_slice;
+    std::vector<long long> group_entries, group_rows;
     long long stored = 0;
     long long entries = 0;
     long long nrows = 0;
@@ -76,6 +77,8 @@ struct DevPart {
     {
         nslices = p.nslices;
         group_slice = p.group_slice;
+        group_entries = p.group_entries;
+        group_rows = p.group_rows;
         stored = (long long)p.vals.size();
         entries = p.entries;
         nrows = p.rows;
@@ -104,6 +107,36 @@ struct DevPart {
     int ngroups() const { return (int)group_slice.size() - 1; }
     int group_begin(int g) const { return group_slice[g]; }
     int group_count(int g) const { return group_slice[g + 1] - group_slice[g]; }
+    // Algorithmic bytes of one launch over group g: 12 B per entry (8 value +
+    // 4 column), 4 B per row id, `per_row` B of vector traffic per row.
+    double bytes(int g, double per_row) const
+    {
+        return 12.0 * group_entries[g] + (4.0 + per_row) * group_rows[g];
+    }
+    double bytes_all(double per_row) const
+    {
+        return 12.0 * entries + (4.0 + per_row) * nrows;
+    }
+};
+
+// Per-launch HIP events of a profiled solve.
+struct Profiler {
+    std::vector<hipEvent_t> pool;
+    std::vector<int> klass;
+    std::vector<double> bytes;
+    ~Profiler()
+    {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+    hipEvent_t event(size_t i)
+    {
+        while (pool.size() <= i) {
+            hipEvent_t e;
+            SFA_HIP(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[i];
+    }
 };
 
 const char* method_title(int method)
@@ -155,6 +188,23 @@ struct System {
     hipStream_t stream = nullptr;
     hipGraphExec_t chunk_exec = nullptr;
     int chunk_iters = 0;
+    Profiler* prof = nullptr;  // set only while sfa_system_profile runs
+
+    // Launches fn(); in a profiled solve, between two events on stream s.
+    template <class Fn>
+    void run(hipStream_t s, int klass, double bytes, Fn&& fn)
+    {
+        if (!prof) {
+            fn();
+            return;
+        }
+        const size_t i = prof->klass.size();
+        SFA_HIP(hipEventRecord(prof->event(2 * i), s));
+        fn();
+        SFA_HIP(hipEventRecord(prof->event(2 * i + 1), s));
+        prof->klass.push_back(klass);
+        prof->bytes.push_back(bytes);
+    }
 
     ~System() { release(); }
 
@@ -183,13 +233,14 @@ struct System {
     void create(int method_, int n_, const int* A, const int* C, const double* V, const double* aux,
                 double omega_, int nc, const int* F, const int* B, const int* R)
     {
-        ensure_device();
-        SFA_HIP(hipGetDevice(&device));
+        // Arguments are checked on the host before the device is touched.
         Plan plan;
         bool colouring_error = false;
         std::string err = build_plan(plan, method_, n_, A, C, V, aux, omega_, nc, F, B, R,
                                      &colouring_error);
         if (!err.empty()) throw Error{colouring_error ? SFA_ERR_COLOURING : SFA_ERR_INVALID, err};
+        ensure_device();
+        SFA_HIP(hipGetDevice(&device));
         method = method_;
         n = n_;
         n_active = plan.n_active;
@@ -251,49 +302,76 @@ struct System {
 
     void forward_sweep(hipStream_t s, const DevPart& L, int update_xr)
     {
+        // vector streams per row: x r/w, d, r r/w, q, diag, m (iteration);
+        // r, diag, m (restart)
+        const double per_row = update_xr ? 64.0 : 24.0;
         for (int g = 0; g < L.ngroups(); g++)
-            launch_forward(s, L.view(), L.group_begin(g), L.group_count(g), state, update_xr, x,
-                           r, d, q, m, diag, omega);
+            run(s, SFA_KERNEL_FORWARD, L.bytes(g, per_row), [&] {
+                launch_forward(s, L.view(), L.group_begin(g), L.group_count(g), state, update_xr,
+                               x, r, d, q, m, diag, omega);
+            });
     }
 
     void backward_sweep(hipStream_t s, int reduce)
     {
         const int G = upper.ngroups();
-        for (int g = 0; g < G; g++)
-            launch_backward(s, upper.view(), upper.group_begin(g), upper.group_count(g), state,
-                            reduce, g, g == G - 1, r, m, z, diag, omega, c2(), grp, partials);
+        for (int g = 0; g < G; g++)  // m, diag, z, r
+            run(s, SFA_KERNEL_BACKWARD, upper.bytes(g, 32.0), [&] {
+                launch_backward(s, upper.view(), upper.group_begin(g), upper.group_count(g), state,
+                                reduce, g, g == G - 1, r, m, z, diag, omega, c2(), grp, partials);
+            });
+    }
+
+    void eisenstat_sweep(hipStream_t s, const DevPart& T, int with_t, const double* in,
+                         const double* t, double* out)
+    {
+        const double per_row = with_t ? 32.0 : 24.0;  // in, diag, out [, t]
+        for (int g = 0; g < T.ngroups(); g++)
+            run(s, SFA_KERNEL_TRIANGULAR, T.bytes(g, per_row), [&] {
+                launch_eisenstat_solve(s, T.view(), T.group_begin(g), T.group_count(g), state,
+                                       with_t, in, t, out, diag, omega, c3());
+            });
     }
 
     // Restart (K = 0) of each variant.
     void restart(hipStream_t s)
     {
         const SellView F = full.view();
+        const double na = n_active;
         switch (method) {
         case SFA_METHOD_CG:
-            launch_residual(s, RES_PLAIN, F, full.nslices, state, b, x, r, d, nullptr, nullptr,
-                            partials);
+            run(s, SFA_KERNEL_RESTART, full.bytes_all(32.0), [&] {
+                launch_residual(s, RES_PLAIN, F, full.nslices, state, b, x, r, d, nullptr,
+                                nullptr, partials);
+            });
             break;
         case SFA_METHOD_DIAGONAL:
-            launch_residual(s, RES_JACOBI, F, full.nslices, state, b, x, r, d, z, diag, partials);
+            run(s, SFA_KERNEL_RESTART, full.bytes_all(48.0), [&] {
+                launch_residual(s, RES_JACOBI, F, full.nslices, state, b, x, r, d, z, diag,
+                                partials);
+            });
             break;
         case SFA_METHOD_SSOR:
         case SFA_METHOD_RAINBOW:
-            launch_residual(s, RES_ONLY, F, full.nslices, state, b, x, r, nullptr, nullptr,
-                            nullptr, partials);
+            run(s, SFA_KERNEL_RESTART, full.bytes_all(16.0), [&] {
+                launch_residual(s, RES_ONLY, F, full.nslices, state, b, x, r, nullptr, nullptr,
+                                nullptr, partials);
+            });
             forward_sweep(s, lower, 0);
             backward_sweep(s, 0);
-            launch_direction_init(s, state, n_active, z, r, d, partials);
+            run(s, SFA_KERNEL_RESTART, 24.0 * na,
+                [&] { launch_direction_init(s, state, n_active, z, r, d, partials); });
             break;
-        default: {  // Eisenstat variants: q = b - A x, then r = lower solve of q
-            launch_residual(s, RES_ONLY, F, full.nslices, state, b, x, q, nullptr, nullptr,
-                            nullptr, partials);
-            const DevPart& L = has_lower_restart ? lower_restart : lower;
-            for (int g = 0; g < L.ngroups(); g++)
-                launch_eisenstat_solve(s, L.view(), L.group_begin(g), L.group_count(g), state, 0,
-                                       q, nullptr, r, diag, omega, 0.);
-            launch_eisenstat_init(s, state, n_active, r, diag, omega, z, d, partials);
+        default:  // Eisenstat variants: q = b - A x, then r = lower solve of q
+            run(s, SFA_KERNEL_RESTART, full.bytes_all(16.0), [&] {
+                launch_residual(s, RES_ONLY, F, full.nslices, state, b, x, q, nullptr, nullptr,
+                                nullptr, partials);
+            });
+            eisenstat_sweep(s, has_lower_restart ? lower_restart : lower, 0, q, nullptr, r);
+            run(s, SFA_KERNEL_RESTART, 32.0 * na, [&] {
+                launch_eisenstat_init(s, state, n_active, r, diag, omega, z, d, partials);
+            });
             break;
-        }
         }
     }
 
@@ -301,37 +379,50 @@ struct System {
     void iteration(hipStream_t s)
     {
         const SellView F = full.view();
+        const double na = n_active;
+        const double spmv_bytes = full.bytes_all(16.0);  // d, q
         switch (method) {
         case SFA_METHOD_CG:
-            launch_spmv_dot(s, F, full.nslices, state, d, q, partials);
-            launch_update(s, UPD_PLAIN, state, n_active, x, r, d, q, nullptr, nullptr, omega,
-                          partials);
-            launch_direction(s, state, n_active, r, d);
+            run(s, SFA_KERNEL_SPMV, spmv_bytes,
+                [&] { launch_spmv_dot(s, F, full.nslices, state, d, q, partials); });
+            run(s, SFA_KERNEL_UPDATE, 48.0 * na, [&] {
+                launch_update(s, UPD_PLAIN, state, n_active, x, r, d, q, nullptr, nullptr, omega,
+                              partials);
+            });
+            run(s, SFA_KERNEL_DIRECTION, 24.0 * na,
+                [&] { launch_direction(s, state, n_active, r, d); });
             break;
         case SFA_METHOD_DIAGONAL:
-            launch_spmv_dot(s, F, full.nslices, state, d, q, partials);
-            launch_update(s, UPD_JACOBI, state, n_active, x, r, d, q, z, diag, omega, partials);
-            launch_direction(s, state, n_active, z, d);
+            run(s, SFA_KERNEL_SPMV, spmv_bytes,
+                [&] { launch_spmv_dot(s, F, full.nslices, state, d, q, partials); });
+            run(s, SFA_KERNEL_UPDATE, 64.0 * na, [&] {
+                launch_update(s, UPD_JACOBI, state, n_active, x, r, d, q, z, diag, omega,
+                              partials);
+            });
+            run(s, SFA_KERNEL_DIRECTION, 24.0 * na,
+                [&] { launch_direction(s, state, n_active, z, d); });
             break;
         case SFA_METHOD_SSOR:
         case SFA_METHOD_RAINBOW:
-            launch_spmv_dot(s, F, full.nslices, state, d, q, partials);
+            run(s, SFA_KERNEL_SPMV, spmv_bytes,
+                [&] { launch_spmv_dot(s, F, full.nslices, state, d, q, partials); });
             forward_sweep(s, lower, 1);
             backward_sweep(s, 1);
-            launch_direction(s, state, n_active, z, d);
+            run(s, SFA_KERNEL_DIRECTION, 24.0 * na,
+                [&] { launch_direction(s, state, n_active, z, d); });
             break;
-        default: {  // Eisenstat: t (in m) = upper solve of d, q = lower solve
-            for (int g = 0; g < upper.ngroups(); g++)
-                launch_eisenstat_solve(s, upper.view(), upper.group_begin(g), upper.group_count(g),
-                                       state, 0, d, nullptr, m, diag, omega, 0.);
-            for (int g = 0; g < lower.ngroups(); g++)
-                launch_eisenstat_solve(s, lower.view(), lower.group_begin(g), lower.group_count(g),
-                                       state, 1, d, m, q, diag, omega, c3());
-            launch_eisenstat_combine(s, state, n_active, q, m, d, partials);
-            launch_update(s, UPD_EISENSTAT, state, n_active, x, r, m, q, z, diag, omega, partials);
-            launch_direction(s, state, n_active, z, d);
+        default:  // Eisenstat: t (in m) = upper solve of d, q = lower solve
+            eisenstat_sweep(s, upper, 0, d, nullptr, m);
+            eisenstat_sweep(s, lower, 1, d, m, q);
+            run(s, SFA_KERNEL_COMBINE, 32.0 * na,
+                [&] { launch_eisenstat_combine(s, state, n_active, q, m, d, partials); });
+            run(s, SFA_KERNEL_UPDATE, 64.0 * na, [&] {
+                launch_update(s, UPD_EISENSTAT, state, n_active, x, r, m, q, z, diag, omega,
+                              partials);
+            });
+            run(s, SFA_KERNEL_DIRECTION, 24.0 * na,
+                [&] { launch_direction(s, state, n_active, z, d); });
             break;
-        }
         }
     }
 
@@ -379,8 +470,10 @@ struct System {
 
     // ---------------------------------------------------------------- solve
 
-    void solve(double* ux, const double* ub, double* ures, double precision, int print_modulo,
-               int device_pointers, sfa_solve_info* info)
+    // Copies the inputs into device order, resets the state and runs the
+    // restart.  Returns the stream the solve runs on.
+    hipStream_t begin_solve(double* ux, const double* ub, double* ures, double precision,
+                            int print_modulo, int device_pointers)
     {
         if (!(precision > 0)) throw Error{SFA_ERR_INVALID, "precision must be positive"};
         SFA_HIP(hipSetDevice(device));
@@ -415,21 +508,12 @@ struct System {
                           partials);
         restart(s);
         read_state();
+        return s;
+    }
 
-        // Iterations: replay a captured chunk, predicting from the z.r history
-        // how many chunks to queue before the next look at the state.
-        const int G = 4;
-        build_chunk_graph(G);
-        std::vector<double> hist_host;
-        int queued = 4;
-        while (!host_state->done) {
-            for (int i = 0; i < queued; i++) SFA_HIP(hipGraphLaunch(chunk_exec, s));
-            read_state();
-            if (host_state->done) break;
-            queued = predict_chunks(G);
-        }
-
-        // results back to the caller's order
+    void finish_solve(hipStream_t s, double* ux, double* ures, int print_modulo,
+                      int device_pointers, sfa_solve_info* info)
+    {
         double* dst_x = device_pointers ? ux : io;
         double* dst_r = device_pointers ? ures : io3;
         launch_scatter(s, dst_x, x, perm, n);
@@ -439,6 +523,7 @@ struct System {
             SFA_HIP(hipMemcpyAsync(ures, io3, n * sizeof(double), hipMemcpyDeviceToHost, s));
         }
         SFA_HIP(hipStreamSynchronize(s));
+        SFA_HIP(hipGetLastError());
 
         const SolveState& st = *host_state;
         sfa_solve_info out{};
@@ -449,6 +534,53 @@ struct System {
         out.success = std::sqrt(st.zr) < st.tol ? 1 : 0;
         if (info) *info = out;
         if (print_modulo >= -2) print_log(print_modulo, st);
+    }
+
+    void solve(double* ux, const double* ub, double* ures, double precision, int print_modulo,
+               int device_pointers, sfa_solve_info* info)
+    {
+        hipStream_t s = begin_solve(ux, ub, ures, precision, print_modulo, device_pointers);
+        // Iterations: replay a captured chunk, predicting from the z.r history
+        // how many chunks to queue before the next look at the state.
+        const int G = 4;
+        build_chunk_graph(G);
+        int queued = 4;
+        while (!host_state->done) {
+            for (int i = 0; i < queued; i++) SFA_HIP(hipGraphLaunch(chunk_exec, s));
+            read_state();
+            if (host_state->done) break;
+            queued = predict_chunks(G);
+        }
+        finish_solve(s, ux, ures, print_modulo, device_pointers, info);
+    }
+
+    // The same solve, every kernel launched on its own between two events.
+    void profile(double* ux, const double* ub, double* ures, double precision,
+                 int device_pointers, sfa_kernel_profile* out, sfa_solve_info* info)
+    {
+        Profiler p;
+        prof = &p;
+        try {
+            hipStream_t s = begin_solve(ux, ub, ures, precision, -3, device_pointers);
+            while (!host_state->done) {
+                iteration(s);
+                read_state();
+            }
+            finish_solve(s, ux, ures, -3, device_pointers, info);
+        } catch (...) {
+            prof = nullptr;
+            throw;
+        }
+        prof = nullptr;
+        for (int c = 0; c < SFA_KERNEL_CLASSES; c++) out[c] = sfa_kernel_profile{};
+        for (size_t i = 0; i < p.klass.size(); i++) {
+            float ms = 0.f;
+            SFA_HIP(hipEventElapsedTime(&ms, p.event(2 * i), p.event(2 * i + 1)));
+            sfa_kernel_profile& o = out[p.klass[i]];
+            o.launches += 1;
+            o.milliseconds += ms;
+            o.bytes += p.bytes[i];
+        }
     }
 
     int predict_chunks(int G)
@@ -534,21 +666,18 @@ struct System {
         o.groups_forward = sweeps ? lower.ngroups() : 0;
         o.groups_backward = sweeps ? upper.ngroups() : 0;
         o.stored_entries = full.stored + lower.stored + upper.stored;
-        // Algorithmic bytes: 12 B per stored-matrix entry read (8 value + 4
-        // column), the gathered vector entry counted once per row's vector
-        // stream, 4 B per row index read, 8 B per vector element moved.
+        // Algorithmic bytes, the same model the profiled launches use.
         const double na = n_active;
-        const double spmv = 12.0 * full.entries + 4.0 * na + 8.0 * 3 * na;  // d gather, d_i, q
-        double it = spmv;
-        if (method == SFA_METHOD_SSOR || method == SFA_METHOD_RAINBOW) {
-            it = spmv + 12.0 * lower.entries + 4.0 * na + 8.0 * 8 * na   // x,d,r,q,r,m,diag + m gather
-                 + 12.0 * upper.entries + 4.0 * na + 8.0 * 5 * na       // m,diag,z,r + z gather
-                 + 8.0 * 3 * na;                                        // direction
-        } else if (method == SFA_METHOD_CG || method == SFA_METHOD_DIAGONAL) {
-            it = spmv + 8.0 * 6 * na + 8.0 * 3 * na;
-        } else {
-            it = 12.0 * lower.entries + 12.0 * upper.entries + 8.0 * 20 * na;
-        }
+        const double spmv = full.bytes_all(16.0);
+        double it = 0.;
+        if (method == SFA_METHOD_SSOR || method == SFA_METHOD_RAINBOW)
+            it = spmv + lower.bytes_all(64.0) + upper.bytes_all(32.0) + 24.0 * na;
+        else if (method == SFA_METHOD_CG)
+            it = spmv + 48.0 * na + 24.0 * na;
+        else if (method == SFA_METHOD_DIAGONAL)
+            it = spmv + 64.0 * na + 24.0 * na;
+        else
+            it = upper.bytes_all(24.0) + lower.bytes_all(32.0) + (32.0 + 64.0 + 24.0) * na;
         o.bytes_spmv = spmv;
         o.bytes_per_iteration = it;
         o.kernels_per_iteration = kernels_per_iteration();
@@ -773,6 +902,16 @@ int sfa_system_query(sfa_system sys, sfa_system_stats* stats)
     return guarded([&] {
         if (!sys || !stats) throw Error{SFA_ERR_INVALID, "null argument"};
         sys->sys.query(stats);
+    });
+}
+
+int sfa_system_profile(sfa_system sys, double* x, const double* b, double* residual,
+                       double precision, int device_pointers, sfa_kernel_profile* out,
+                       sfa_solve_info* info)
+{
+    return guarded([&] {
+        if (!sys || !x || !b || !residual || !out) throw Error{SFA_ERR_INVALID, "null argument"};
+        sys->sys.profile(x, b, residual, precision, device_pointers, out, info);
     });
 }
 

@@ -1,0 +1,103 @@
+"""Writes the golden fixtures of tests/golden/ from the reference's own code.
+
+Run in a checkout where the reference sources were available when
+oracle/Makefile built oracle/_ref/libfeecref.so:
+
+    python tests/golden/make_golden.py
+
+Each fixture holds inputs that are cheap to regenerate (the matrix comes from
+simplefem_amd.systems by name, the right-hand side from a seed) and the
+outputs of the compiled reference at one OpenMP thread: colourings
+(F, B, R), iteration counts, final residual norms and, for the small
+systems, x and residual in full.  For the two full-size systems only
+SHA-256 digests of F/B/R, the colour count and the iteration counts are kept.
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle.ref import RefRunner  # noqa: E402
+from simplefem_amd import systems as S  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+VARIANTS = ("cg", "jacobi", "ssor", "ssor_eisenstat", "rainbow", "eisenstat_rainbow")
+
+SMALL = {
+    "fd2d_8": lambda: S.fd_laplacian_2d(8),
+    "p1_2d_L3": lambda: S.poisson2d_p1(3),
+    "p1_2d_L4": lambda: S.poisson2d_p1(4),
+    "kuhn_4": lambda: S.poisson3d_p1_kuhn(4),
+    "spd_120": lambda: S.random_spd(120, 0.05, 3),
+}
+FULL = {
+    "p1_2d_L7": lambda: S.poisson2d_p1(7),
+    "kuhn_64": lambda: S.poisson3d_p1_kuhn(64),
+}
+
+
+def aux_for(variant, d):
+    if variant == "cg":
+        return None
+    if variant == "jacobi":
+        return np.where(d != 0, 1.0 / np.where(d != 0, d, 1.0), 0.0)
+    return d
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.int32).tobytes()).hexdigest()
+
+
+def main():
+    with RefRunner(1) as ref:
+        meta = {"desired_precision": ref.info()["desired_precision"], "small": {}, "full": {}}
+        for name, make in SMALL.items():
+            s = make()
+            b = S.rhs_uniform(s, 11)
+            d = ref.csr_diagonal(s.A, s.C, s.V)
+            col = ref.rainbow(s.A, s.C, s.V)
+            arrays = {"F": col[1], "B": col[2], "R": col[3], "diag": d,
+                      "apply": ref.csr_apply(s.A, s.C, s.V, S.uniform_pm1(5, s.n), -0.37)}
+            ks = {}
+            for omega in (1.0, 1.2):
+                for v in VARIANTS:
+                    if v in ("cg", "jacobi") and omega != 1.0:
+                        continue
+                    x, r, K, _ = ref.solve(v, s.A, s.C, s.V, b, aux=aux_for(v, d), omega=omega,
+                                           colouring=col)
+                    tag = f"{v}_w{omega}"
+                    arrays["x_" + tag] = x
+                    arrays["res_" + tag] = r
+                    ks[tag] = K
+            np.savez_compressed(os.path.join(OUT, name + ".npz"), **arrays)
+            meta["small"][name] = {"n": s.n, "nnz": s.nnz, "num_colors": int(col[0]),
+                                   "rhs_seed": 11, "apply_seed": 5, "apply_scaling": -0.37,
+                                   "iterations": ks}
+            print(name, ks, flush=True)
+        for name, make in FULL.items():
+            s = make()
+            b = S.rhs_uniform(s, 2026)
+            d = ref.csr_diagonal(s.A, s.C, s.V)
+            col = ref.rainbow(s.A, s.C, s.V)
+            entry = {"n": s.n, "nnz": s.nnz, "num_colors": int(col[0]), "sha256_F": sha(col[1]),
+                     "sha256_B": sha(col[2]), "sha256_R": sha(col[3]),
+                     "colour_sizes": np.diff(col[2]).tolist(), "rhs_seed": 2026, "iterations": {}}
+            for v in ("rainbow", "ssor"):
+                for omega in (1.0, 1.2):
+                    x, r, K, t = ref.solve(v, s.A, s.C, s.V, b, aux=d, omega=omega,
+                                           colouring=col)
+                    entry["iterations"][f"{v}_w{omega}"] = K
+                    entry.setdefault("x_norm", {})[f"{v}_w{omega}"] = float(np.linalg.norm(x))
+                    print(name, v, omega, K, f"{t[0]:.2f}s", flush=True)
+            meta["full"][name] = entry
+    with open(os.path.join(OUT, "golden.json"), "w") as f:
+        json.dump(meta, f, indent=1, sort_keys=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,98 @@
+"""C-ABI library checks that need no GPU: it loads, exports every declared
+symbol, and its host-side logic (greedy colouring, diagonal extraction,
+argument validation) agrees with the oracle.  No kernel is launched here."""
+import ctypes
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from simplefem_amd import capi
+from simplefem_amd import systems as S
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def test_library_exports_every_header_symbol():
+    lib = capi.load()
+    names = capi.header_symbols()
+    assert len(names) >= 17
+    for name in names:
+        assert hasattr(lib, name), name
+    assert lib.sfa_api_version() == 1
+
+
+def test_python_signatures_cover_header():
+    assert set(capi.header_symbols()) == set(capi._SIGNATURES)
+
+
+@pytest.mark.parametrize("make", [lambda: S.poisson2d_p1(7), lambda: S.poisson3d_p1_kuhn(64),
+                                  lambda: S.random_spd(500, 0.01, 2), lambda: S.fd_laplacian_2d(20)])
+def test_rainbow_bit_exact(make):
+    s = make()
+    want = oracle.rainbow(s.n, s.A, s.C)
+    got = capi.Rainbow(s.n, s.A, s.C)
+    assert got.num_colors == want[0]
+    assert np.array_equal(got.F, want[1])
+    assert np.array_equal(got.B, want[2])
+    assert np.array_equal(got.R, want[3])
+
+
+def test_rainbow_matches_reference_digest():
+    meta = json.load(open(os.path.join(GOLDEN, "golden.json")))["full"]["kuhn_64"]
+    s = S.poisson3d_p1_kuhn(64)
+    rb = capi.Rainbow(s.n, s.A, s.C)
+    import hashlib
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a, dtype=np.int32).tobytes()).hexdigest()
+    assert rb.num_colors == meta["num_colors"]
+    assert (sha(rb.F), sha(rb.B), sha(rb.R)) == (meta["sha256_F"], meta["sha256_B"],
+                                                 meta["sha256_R"])
+
+
+def test_get_diagonal():
+    for s in (S.poisson3d_p1_kuhn(5), S.random_spd(200, 0.05, 1)):
+        assert np.array_equal(capi.get_diagonal(s.n, s.A, s.C, s.V),
+                              oracle.csr_diagonal(s.n, s.A, s.C, s.V))
+
+
+def test_invalid_csr_is_rejected_before_the_device():
+    n = 4
+    A = np.array([0, 2, 1, 3, 4], dtype=np.int32)  # decreasing offsets
+    C = np.array([0, 1, 2, 3], dtype=np.int32)
+    V = np.ones(4)
+    with pytest.raises(capi.SfaError) as e:
+        capi.ConjugateGradientSolverCSR(n, np.zeros(n), np.ones(n), A, C, V, np.zeros(n), 1e-10, -3)
+    assert e.value.code == -1
+    A = np.array([0, 1, 2, 3, 4], dtype=np.int32)
+    C = np.array([0, 1, 2, 7], dtype=np.int32)  # column out of range
+    with pytest.raises(capi.SfaError) as e:
+        capi.ConjugateGradientSolverCSR_SSOR(n, np.zeros(n), np.ones(n), A, C, V, np.zeros(n),
+                                             1e-10, -3, np.ones(n), 1.0)
+    assert e.value.code == -1
+    assert "column" in str(e.value)
+
+
+def test_inconsistent_colouring_is_rejected_before_the_device():
+    s = S.poisson2d_p1(3)
+    nc, F, B, R = oracle.rainbow(s.n, s.A, s.C)
+    F2 = F.copy()
+    F2[3] = (F2[3] + 1) % nc
+    n = s.n
+    with pytest.raises(capi.SfaError) as e:
+        capi.ConjugateGradientSolverCSR_Rainbow(n, np.zeros(n), np.ones(n), s.A, s.C, s.V,
+                                               np.zeros(n), 1e-10, -3, s.diagonal(), 1.0, nc, F2,
+                                               B, R)
+    assert e.value.code == -5
+
+
+def test_null_arguments():
+    lib = capi.load()
+    info = capi.SolveInfo()
+    rc = lib.sfa_ConjugateGradientSolverCSR(4, None, None, None, None, None, None, 1e-10, -3,
+                                            ctypes.byref(info))
+    assert rc == -1
+    assert lib.sfa_last_error().decode()
+    assert lib.sfa_system_destroy(None) == 0
+    assert lib.sfa_system_query(None, None) == -1

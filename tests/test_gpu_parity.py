@@ -1,0 +1,277 @@
+"""GPU parity of the HIP path against the oracle, through the C-ABI.
+
+Bars (stated per test):
+  * MatrixCSR::apply, restart residuals: bit-exact (same per-row order, no
+    fused multiply-add on either side);
+  * CG solves: K within max(1, 2% of K) of the oracle, exit reason equal,
+    x and residual within 1e-10 relative (||.||_inf against the oracle's),
+    NaN patterns equal where the reference breaks down.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from simplefem_amd import capi
+from simplefem_amd import systems as S
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+
+
+def _aux_for(variant, system):
+    d = system.diagonal()
+    if variant == "cg":
+        return None
+    if variant == "jacobi":
+        safe = np.where(d != 0, d, 1.0)
+        return np.where(d != 0, 1.0 / safe, 0.0)
+    return d
+
+
+def _gpu_solve(variant, system, b, aux, omega, col, x0=None, r0=None, precision=1e-10,
+               print_modulo=-3):
+    n = system.n
+    x = np.zeros(n) if x0 is None else x0.copy()
+    r = np.zeros(n) if r0 is None else r0.copy()
+    info = {}
+    args = (n, x, b, system.A, system.C, system.V, r, precision, print_modulo)
+    if variant == "cg":
+        capi.ConjugateGradientSolverCSR(*args, info=info)
+    elif variant == "jacobi":
+        capi.ConjugateGradientSolverCSR_DiagonalPreconditioner(*args, aux, info=info)
+    elif variant == "ssor":
+        capi.ConjugateGradientSolverCSR_SSOR(*args, aux, omega, info=info)
+    elif variant == "ssor_eisenstat":
+        capi.ConjugateGradientSolverCSR_SSOR_Eisenstat(*args, aux, omega, info=info)
+    elif variant == "rainbow":
+        capi.ConjugateGradientSolverCSR_Rainbow(*args, aux, omega, *col, info=info)
+    else:
+        capi.ConjugateGradientSolverCSR_Eisenstat_Rainbow(*args, aux, omega, *col, info=info)
+    return x, r, info
+
+
+def _close(a, b, rel=REL, scale=None):
+    """Same NaN pattern and |a-b|_inf <= rel * max(|b|_inf, scale, tiny)."""
+    na, nb = np.isnan(a), np.isnan(b)
+    assert np.array_equal(na, nb), "NaN patterns differ"
+    fa, fb = a[~na], b[~nb]
+    if fb.size == 0:
+        return
+    scale = max(np.abs(fb).max(), scale or 0.0, 1e-300)
+    err = np.abs(fa - fb).max() / scale
+    assert err <= rel, f"relative error {err:.3e} > {rel:.1e}"
+
+
+def _check_parity(variant, system, b, omega=1.0, precision=1e-10, x0=None, r0=None):
+    aux = _aux_for(variant, system)
+    col = oracle.rainbow(system.n, system.A, system.C)
+    xo, ro, io = oracle.solve(variant, system.A, system.C, system.V, b, x0=x0, residual0=r0,
+                              precision=precision, aux=aux, omega=omega, colouring=col)
+    xg, rg, ig = _gpu_solve(variant, system, b, aux, omega, col, x0=x0, r0=r0,
+                            precision=precision)
+    Ko, Kg = io["iterations"], ig["iterations"]
+    assert abs(Kg - Ko) <= max(1, int(0.02 * Ko)), (Ko, Kg)
+    assert ig["exit_reason"] == io["exit_reason"], (io, ig)
+    assert ig["success"] == io["success"]
+    # host pointers: the tolerance is summed in row order on both sides
+    assert ig["tolerance"] == io["tolerance"] or (np.isnan(ig["tolerance"])
+                                                  and np.isnan(io["tolerance"]))
+    _close(xg, xo)
+    # The final residual is a difference of O(||b||) terms: compare on that scale.
+    _close(rg, ro, scale=np.nanmax(np.abs(b)) if np.isfinite(b).any() else 1.0)
+    return io, ig
+
+
+SYSTEMS = {
+    "fd2d_12": lambda: S.fd_laplacian_2d(12),
+    "p1_2d_L4": lambda: S.poisson2d_p1(4),
+    "kuhn_6": lambda: S.poisson3d_p1_kuhn(6),
+    "spd_400": lambda: S.random_spd(400, 0.02, 7),
+}
+
+
+@pytest.mark.parametrize("variant", oracle.VARIANTS)
+@pytest.mark.parametrize("name", sorted(SYSTEMS))
+def test_solver_parity(variant, name):
+    system = SYSTEMS[name]()
+    b = S.rhs_uniform(system, seed=11)
+    print(f"[parity] {variant} on {name} n={system.n} nnz={system.nnz}", flush=True)
+    _check_parity(variant, system, b, omega=1.0)
+
+
+@pytest.mark.parametrize("variant", ["ssor", "rainbow", "ssor_eisenstat"])
+def test_omega_and_initial_guess(variant):
+    """solvespd_csr.cpp:530-566 uses omega = 0.9123456789; nonzero x0 and residual0."""
+    system = S.fd_laplacian_2d(16)
+    rng = np.random.default_rng(3)
+    b = rng.uniform(-1, 1, system.n)
+    x0 = rng.uniform(-1, 1, system.n)
+    r0 = rng.uniform(-1, 1, system.n)
+    _check_parity(variant, system, b, omega=0.9123456789, x0=x0, r0=r0)
+
+
+def test_apply_bitexact():
+    for system in (S.fd_laplacian_2d(9), S.poisson2d_p1(5), S.poisson3d_p1_kuhn(7),
+                   S.random_spd(257, 0.05, 1)):
+        src = S.uniform_pm1(5, system.n)
+        for scaling in (1.0, -0.37):
+            want = oracle.csr_apply(system.n, system.A, system.C, system.V, src, scaling)
+            got = capi.csr_apply(system.n, system.n, system.A, system.C, system.V, src, scaling)
+            assert np.array_equal(got, want), system.name
+
+
+def test_apply_rectangular_kat():
+    """tests/sparse/sparsematrix.cpp:51-67: [[1,2,3],[4,5,6]] x [13,17,19] = [104, 251]."""
+    A = np.array([0, 3, 6], dtype=np.int32)
+    C = np.array([0, 1, 2, 0, 1, 2], dtype=np.int32)
+    V = np.array([1., 2., 3., 4., 5., 6.])
+    got = capi.csr_apply(2, 3, A, C, V, np.array([13., 17., 19.]))
+    assert got.tolist() == [104.0, 251.0]
+
+
+def test_identity_kat():
+    """tests/sparse/sparsematrix.cpp:13-28: identity action is exact."""
+    n = 5
+    A = np.arange(n + 1, dtype=np.int32)
+    C = np.arange(n, dtype=np.int32)
+    V = np.ones(n)
+    vec = np.arange(n) * 1.2345
+    assert np.array_equal(capi.csr_apply(n, n, A, C, V, vec), vec)
+
+
+@pytest.mark.parametrize("variant", ["ssor", "rainbow", "ssor_eisenstat", "eisenstat_rainbow"])
+def test_diagonal_matrix_kat(variant):
+    """A = D: the SSOR preconditioner is omega(2-omega) D^-1, so K = 1 and x = b/d."""
+    n = 50
+    d = np.linspace(1.0, 3.0, n)
+    A = np.arange(n + 1, dtype=np.int32)
+    C = np.arange(n, dtype=np.int32)
+    system = S.CSRSystem("diag", n, A, C, d.copy(), np.zeros(n, dtype=bool))
+    b = S.uniform_pm1(9, n)
+    io, ig = _check_parity(variant, system, b, omega=1.3)
+    assert ig["iterations"] == 1
+
+
+def test_zero_rhs_and_single_row():
+    system = S.poisson2d_p1(3)
+    b = np.zeros(system.n)
+    for variant in oracle.VARIANTS:
+        io, ig = _check_parity(variant, system, b)
+    one = S.CSRSystem("one", 1, np.array([0, 1], dtype=np.int32), np.array([0], dtype=np.int32),
+                      np.array([2.0]), np.zeros(1, dtype=bool))
+    for variant in oracle.VARIANTS:
+        _check_parity(variant, one, np.array([0.7]))
+
+
+def test_breakdown_on_nan_rhs():
+    system = S.fd_laplacian_2d(6)
+    b = S.uniform_pm1(1, system.n)
+    b[5] = np.nan
+    for variant in oracle.VARIANTS:
+        io, ig = _check_parity(variant, system, b)
+        assert ig["exit_reason"] == capi.EXIT_BREAKDOWN
+
+
+def test_indefinite_breakdown():
+    """d.Ad < 0 on a negative definite matrix ends the loop with BREAKDOWN."""
+    system = S.fd_laplacian_2d(5)
+    neg = S.CSRSystem("neg", system.n, system.A, system.C, -system.V, system.dirichlet)
+    b = S.uniform_pm1(2, system.n)
+    io, ig = _check_parity("cg", neg, b)
+    assert ig["exit_reason"] == capi.EXIT_BREAKDOWN
+
+
+def _ill_conditioned(n, seed):
+    """Dense SPD matrix with eigenvalues log-spaced over 12 decades, as CSR."""
+    rng = np.random.default_rng(seed)
+    Q, _ = np.linalg.qr(rng.standard_normal((n, n)))
+    M = (Q * np.logspace(0, 12, n)) @ Q.T
+    M = 0.5 * (M + M.T)
+    A = np.arange(0, n * n + 1, n, dtype=np.int32)
+    C = np.tile(np.arange(n, dtype=np.int32), n)
+    return S.CSRSystem("illcond", n, A, C, M.ravel().copy(), np.zeros(n, dtype=bool))
+
+
+def test_iteration_cap():
+    """CG on an ill-conditioned system with a large right-hand side runs into
+    K = N (the loop bound solver/sparsesolver.cpp:75) on both sides."""
+    system = _ill_conditioned(40, 4)
+    b = 1e10 * S.uniform_pm1(3, system.n)
+    xo, ro, io = oracle.solve("cg", system.A, system.C, system.V, b, precision=1e-300)
+    xg, rg, ig = _gpu_solve("cg", system, b, None, 1.0, None, precision=1e-300)
+    assert io["iterations"] == system.n and io["exit_reason"] == capi.EXIT_CAP
+    assert ig["iterations"] == system.n and ig["exit_reason"] == capi.EXIT_CAP
+    assert ig["success"] == io["success"] == 0
+
+
+def test_colouring_is_validated():
+    system = S.poisson2d_p1(3)
+    nc, F, B, R = oracle.rainbow(system.n, system.A, system.C)
+    bad = R.copy()
+    bad[0] = bad[1]
+    n = system.n
+    with pytest.raises(capi.SfaError) as e:
+        capi.ConjugateGradientSolverCSR_Rainbow(n, np.zeros(n), np.ones(n), system.A, system.C,
+                                               system.V, np.zeros(n), 1e-10, -3,
+                                               system.diagonal(), 1.0, nc, F, B, bad)
+    assert e.value.code == -5
+
+
+def test_log_lines(capfd):
+    system = S.poisson2d_p1(4)
+    b = S.rhs_uniform(system, 1)
+    col = oracle.rainbow(system.n, system.A, system.C)
+    x, r, info = _gpu_solve("rainbow", system, b, system.diagonal(), 1.2, col, print_modulo=5)
+    out = capfd.readouterr().out.splitlines()
+    K = info["iterations"]
+    assert out[0].startswith(f"(0/{system.n})     BEGIN: Conjugate Gradient CSR (SSOR-Rainbow)")
+    assert out[1].startswith(f"(0/{system.n})   RESTART: Residual norm is ")
+    interim = [ln for ln in out if "INTERIM" in ln]
+    assert len(interim) == K // 5 + 1
+    assert out[-1].startswith(f"({K}/{system.n})   SUCCESS: Residual norm is ")
+
+
+def test_device_resident_system_matches_host_call():
+    from simplefem_amd.hipmem import DeviceArray
+
+    system = S.poisson3d_p1_kuhn(10)
+    b = S.rhs_uniform(system, 4)
+    d = system.diagonal()
+    col = oracle.rainbow(system.n, system.A, system.C)
+    xh, rh, ih = _gpu_solve("rainbow", system, b, d, 1.0, col)
+    with capi.System(capi.METHOD_RAINBOW, system.n, system.A, system.C, system.V, d, 1.0,
+                     *col) as sysd:
+        bt = DeviceArray.from_host(b)
+        x = DeviceArray(system.n)
+        r = DeviceArray(system.n)
+        info = {}
+        for _ in range(2):  # a second solve on the same system replays the same graph
+            x.zero()
+            r.zero()
+            sysd.solve(x, bt, r, 1e-10, info=info)
+            assert info["iterations"] == ih["iterations"]
+            _close(x.download(), xh, 1e-13)
+            _close(r.download(), rh, 1e-13, scale=np.abs(b).max())
+        q = DeviceArray(system.n)
+        sysd.apply(q, bt)
+        want = oracle.csr_apply(system.n, system.A, system.C, system.V, b)
+        assert np.array_equal(q.download(), want)
+        # the profiled solve gives the same answer as the graph-replayed one
+        x.zero()
+        r.zero()
+        prof, pinfo = sysd.profile(x, bt, r, 1e-10)
+        assert pinfo["iterations"] == ih["iterations"]
+        _close(x.download(), xh, 1e-13)
+        K = pinfo["iterations"]
+        st = sysd.stats()
+        assert prof["spmv"]["launches"] == K + 1 or prof["spmv"]["launches"] == K
+        assert prof["forward"]["launches"] >= K * st["groups_forward"]
+
+
+def test_full_size_3d_rainbow():
+    """The 64^3 Kuhn P1 Poisson system (250,047 active rows), rtol 1e-10, omega 1."""
+    system = S.poisson3d_p1_kuhn(64)
+    b = S.rhs_uniform(system, seed=2026)
+    io, ig = _check_parity("rainbow", system, b, omega=1.0)
+    print("[full] K oracle", io["iterations"], "K gpu", ig["iterations"], flush=True)
