@@ -35,6 +35,8 @@ SOURCES = [  # (source, compile as HIP)
     ("plan.cpp", False),
 ]
 CXX_SOURCES = ["dropin.cpp"]
+DROPIN_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_test.cpp")
+DROPIN_TEST_BIN = os.path.join(OBJ, "dropin_test")
 
 
 def _run(cmd):
@@ -73,6 +75,10 @@ def build(verbose: bool = False) -> str:
         _run(["g++", "-shared", "-o", tmp] + cxx_objs
              + ["-L" + HERE, "-lsimplefem_amd", "-Wl,-rpath,$ORIGIN"])
         os.replace(tmp, CXXLIB)
+        # a caller written like the reference's tests, against the drop-in headers
+        _run(["g++", "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include"), DROPIN_TEST_SRC,
+              "-o", DROPIN_TEST_BIN, "-L" + HERE, "-lsimplefem_amd_cxx", "-lsimplefem_amd",
+              "-Wl,-rpath,$ORIGIN/.."])
     if verbose:
         print("built", LIB)
     return LIB

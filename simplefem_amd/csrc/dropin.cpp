@@ -1,0 +1,673 @@
+// dropin.cpp -- C++ drop-in for FEEC++'s sparse solve path (libsimplefem_amd_cxx.so).
+//
+// FloatVector, MatrixCSR, Rainbow and the six ConjugateGradientSolverCSR*
+// functions with the reference's declarations (include/feec/...).  Host data
+// stays in host containers as in the reference; products and solves go to the
+// GPU through the C-ABI (include/simplefem_amd.h).  The C-ABI status of a call
+// is turned into the reference's failure behaviour: a message and abort().
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <numeric>
+#include <random>
+
+#include "../../include/feec/operators/floatvector.hpp"
+#include "../../include/feec/operators/linearoperator.hpp"
+#include "../../include/feec/solver/sparsesolver.hpp"
+#include "../../include/feec/sparse/matcsr.hpp"
+#include "../../include/feec/sparse/rainbow.hpp"
+#include "../../include/simplefem_amd.h"
+
+namespace {
+
+void require(int status, const char* what)
+{
+    if (status == SFA_OK) return;
+    std::fprintf(stderr, "%s failed (%d): %s\n", what, status, sfa_last_error());
+    std::abort();
+}
+
+void require_true(bool cond, const char* what)
+{
+    if (cond) return;
+    std::fprintf(stderr, "check failed: %s\n", what);
+    std::abort();
+}
+
+// Row-wise sort, duplicate merge and zero removal with the operations of
+// sparse/matcsr.cpp:585-687 (so duplicate sums round the same way).
+void sort_and_compress(std::vector<int>& A, std::vector<int>& C, std::vector<Float>& V)
+{
+    const int rows = (int)A.size() - 1;
+    std::vector<int> kept(rows);
+    for (int r = 0; r < rows; r++) {
+        for (int i = A[r]; i < A[r + 1]; i++)
+            for (int j = i + 1; j < A[r + 1]; j++) {
+                if (C[i] == C[j]) {
+                    V[i] += V[j];
+                    V[j] = 0.;
+                    continue;
+                }
+                if (C[i] > C[j]) {
+                    std::swap(C[i], C[j]);
+                    std::swap(V[i], V[j]);
+                }
+            }
+        for (int i = A[r]; i < A[r + 1]; i++)  // zeros to the end, order kept otherwise
+            for (int j = A[r] + 1; j < A[r + 1]; j++)
+                if (V[j - 1] == 0. && V[j] != 0.) {
+                    std::swap(C[j - 1], C[j]);
+                    std::swap(V[j - 1], V[j]);
+                }
+        int k = A[r];
+        while (k < A[r + 1] && V[k] != 0.) k++;
+        kept[r] = k - A[r];
+    }
+    int at = 0;
+    for (int r = 0; r < rows; r++)
+        for (int i = A[r]; i < A[r] + kept[r]; i++) {
+            C[at] = C[i];
+            V[at] = V[i];
+            at++;
+        }
+    A[0] = 0;
+    for (int r = 0; r < rows; r++) A[r + 1] = A[r] + kept[r];
+    C.resize(A[rows]);
+    V.resize(A[rows]);
+}
+
+}  // namespace
+
+// ============================================================== FloatVector
+
+FloatVector::FloatVector(int dim, Float initialvalue) : dimension(dim), pointer(nullptr)
+{
+    require_true(dim >= 0, "FloatVector dimension >= 0");
+    pointer = new Float[dim > 0 ? dim : 1];
+    std::fill_n(pointer, dim, initialvalue);
+}
+
+FloatVector::FloatVector(const FloatVector& src, Float scaling) : FloatVector(src.dimension)
+{
+    for (int i = 0; i < dimension; i++) pointer[i] = scaling * src.pointer[i];
+}
+
+FloatVector::FloatVector(const std::vector<Float>& v, Float scaling) : FloatVector((int)v.size())
+{
+    for (int i = 0; i < dimension; i++) pointer[i] = scaling * v[i];
+}
+
+FloatVector::FloatVector(const std::vector<int>& v, Float scaling) : FloatVector((int)v.size())
+{
+    for (int i = 0; i < dimension; i++) pointer[i] = scaling * v[i];
+}
+
+FloatVector::FloatVector(const std::initializer_list<Float>& l) : FloatVector((int)l.size())
+{
+    int i = 0;
+    for (Float f : l) pointer[i++] = f;
+}
+
+FloatVector::FloatVector(int dim, const std::function<Float(int)>& generator, Float scaling)
+    : FloatVector(dim)
+{
+    for (int i = 0; i < dim; i++) pointer[i] = scaling * generator(i);
+}
+
+FloatVector::FloatVector(const FloatVector& src) : FloatVector(src.dimension)
+{
+    std::copy_n(src.pointer, dimension, pointer);
+}
+
+FloatVector::FloatVector(FloatVector&& src) noexcept : dimension(src.dimension), pointer(src.pointer)
+{
+    src.dimension = 0;
+    src.pointer = nullptr;
+}
+
+FloatVector& FloatVector::operator=(const FloatVector& vec)
+{
+    require_true(dimension == vec.dimension, "FloatVector assignment of equal dimension");
+    if (this != &vec) std::copy_n(vec.pointer, dimension, pointer);
+    return *this;
+}
+
+FloatVector& FloatVector::operator=(FloatVector&& vec) noexcept
+{
+    if (this != &vec) {
+        delete[] pointer;
+        dimension = vec.dimension;
+        pointer = vec.pointer;
+        vec.dimension = 0;
+        vec.pointer = nullptr;
+    }
+    return *this;
+}
+
+FloatVector::~FloatVector() noexcept { delete[] pointer; }
+
+void FloatVector::check() const { require_true(dimension >= 0, "FloatVector::check"); }
+
+std::string FloatVector::text() const
+{
+    return "float vector of dimension: " + std::to_string(dimension);
+}
+
+std::string FloatVector::data_as_text(bool indexed, bool rowwise) const
+{
+    std::string s;
+    for (int i = 0; i < dimension; i++) {
+        if (indexed) s += std::to_string(i) + ": ";
+        s += std::to_string(pointer[i]);
+        s += rowwise ? " " : "\n";
+    }
+    return s;
+}
+
+Float FloatVector::setentry(int p, Float value) { return pointer[p] = value; }
+Float FloatVector::getentry(int p) const { return pointer[p]; }
+void FloatVector::setentries(Float v) { std::fill_n(pointer, dimension, v); }
+
+void FloatVector::random()
+{
+    static thread_local std::default_random_engine engine(0);
+    std::normal_distribution<Float> dist(0., 1.);
+    for (int i = 0; i < dimension; i++) pointer[i] = dist(engine);
+}
+
+void FloatVector::random_within_range(Float min, Float max)
+{
+    static thread_local std::default_random_engine engine(0);
+    std::uniform_real_distribution<Float> dist(min, max);
+    for (int i = 0; i < dimension; i++) pointer[i] = dist(engine);
+}
+
+void FloatVector::to_absolute()
+{
+    for (int i = 0; i < dimension; i++) pointer[i] = absolute(pointer[i]);
+}
+
+void FloatVector::zero() { std::fill_n(pointer, dimension, 0.); }
+
+FloatVector& FloatVector::normalize()
+{
+    const Float alpha = norm();
+    for (int i = 0; i < dimension; i++) pointer[i] = pointer[i] / alpha;
+    return *this;
+}
+
+FloatVector& FloatVector::scale(Float f)
+{
+    for (int i = 0; i < dimension; i++) pointer[i] *= f;
+    return *this;
+}
+
+FloatVector& FloatVector::scaleinverse(Float d)
+{
+    for (int i = 0; i < dimension; i++) pointer[i] /= d;
+    return *this;
+}
+
+FloatVector& FloatVector::shift(Float delta)
+{
+    for (int i = 0; i < dimension; i++) pointer[i] += delta;
+    return *this;
+}
+
+FloatVector& FloatVector::shiftnegative(Float delta) { return shift(-delta); }
+
+FloatVector& FloatVector::reciprocal()
+{
+    for (int i = 0; i < dimension; i++) pointer[i] = 1. / pointer[i];
+    return *this;
+}
+
+FloatVector FloatVector::getslice(int base, int len) const
+{
+    require_true(base >= 0 && len >= 0 && base + len <= dimension, "FloatVector::getslice bounds");
+    FloatVector ret(len);
+    std::copy_n(pointer + base, len, ret.pointer);
+    return ret;
+}
+
+void FloatVector::setslice(int base, int len, Float v)
+{
+    require_true(base >= 0 && len >= 0 && base + len <= dimension, "FloatVector::setslice bounds");
+    std::fill_n(pointer + base, len, v);
+}
+
+void FloatVector::setslice(int base, const FloatVector& src)
+{
+    require_true(base >= 0 && base + src.dimension <= dimension, "FloatVector::setslice bounds");
+    std::copy_n(src.pointer, src.dimension, pointer + base);
+}
+
+void FloatVector::addslice(int base, const FloatVector& summand, Float scaling)
+{
+    require_true(base >= 0 && base + summand.dimension <= dimension, "FloatVector::addslice bounds");
+    for (int i = 0; i < summand.dimension; i++) pointer[base + i] += scaling * summand.pointer[i];
+}
+
+void FloatVector::copydatafrom(const FloatVector& src) { copydatafrom(1., src); }
+
+void FloatVector::copydatafrom(Float scaling, const FloatVector& src)
+{
+    require_true(dimension == src.dimension, "copydatafrom of equal dimension");
+    for (int i = 0; i < dimension; i++) pointer[i] = scaling * src.pointer[i];
+}
+
+void FloatVector::generatedatafrom(const std::function<Float(int)>& g)
+{
+    for (int i = 0; i < dimension; i++) pointer[i] = g(i);
+}
+
+void FloatVector::adddatafrom(const FloatVector& s) { adddatafrom(1., 1., s); }
+void FloatVector::adddatafrom(Float ss, const FloatVector& s) { adddatafrom(1., ss, s); }
+
+// operators/floatvector.cpp:554-560: dest = sd * dest + ss * summand
+void FloatVector::adddatafrom(Float sd, Float ss, const FloatVector& s)
+{
+    require_true(dimension == s.dimension, "adddatafrom of equal dimension");
+    for (int i = 0; i < dimension; i++) pointer[i] = sd * pointer[i] + ss * s.pointer[i];
+}
+
+// long double accumulation (operators/floatvector.cpp:563-571)
+Float FloatVector::scalarproductwith(const FloatVector& right) const
+{
+    require_true(dimension == right.dimension, "scalar product of equal dimension");
+    long double ret = 0.;
+    for (int i = 0; i < dimension; i++) ret += pointer[i] * right.pointer[i];
+    return (Float)ret;
+}
+
+Float FloatVector::min() const
+{
+    Float ret = pointer[0];
+    for (int i = 1; i < dimension; i++) ret = (ret < pointer[i]) ? ret : pointer[i];
+    return ret;
+}
+
+Float FloatVector::average() const { return sum() / dimension; }
+
+Float FloatVector::sum() const
+{
+    long double ret = 0.;
+    for (int i = 0; i < dimension; i++) ret += pointer[i];
+    return (Float)ret;
+}
+
+Float FloatVector::norm() const { return std::sqrt(norm_sq()); }
+
+Float FloatVector::norm_sq() const
+{
+    long double ret = 0.;
+    for (int i = 0; i < dimension; i++) ret += pointer[i] * pointer[i];
+    return (Float)ret;
+}
+
+Float FloatVector::norm(const LinearOperator& op) const { return std::sqrt(norm_sq(op)); }
+
+Float FloatVector::norm_sq(const LinearOperator& op) const
+{
+    long double ret = (*this) * (op * (*this));
+    return (Float)ret;
+}
+
+Float FloatVector::maxnorm() const
+{
+    Float ret = 0.;
+    for (int i = 0; i < dimension; i++) ret = maximum(ret, absolute(pointer[i]));
+    return ret;
+}
+
+Float FloatVector::sumnorm() const
+{
+    long double ret = 0.;
+    for (int i = 0; i < dimension; i++) ret = ret + absolute(pointer[i]);
+    return (Float)ret;
+}
+
+bool FloatVector::is_finite() const
+{
+    for (int i = 0; i < dimension; i++)
+        if (!std::isfinite(pointer[i])) return false;
+    return true;
+}
+
+bool FloatVector::is_zero() const
+{
+    for (int i = 0; i < dimension; i++)
+        if (pointer[i] != 0.) return false;
+    return true;
+}
+
+bool FloatVector::is_positive() const
+{
+    for (int i = 0; i < dimension; i++)
+        if (!(pointer[i] > 0.)) return false;
+    return true;
+}
+
+bool FloatVector::is_negative() const
+{
+    for (int i = 0; i < dimension; i++)
+        if (!(pointer[i] < 0.)) return false;
+    return true;
+}
+
+bool FloatVector::is_nonnegative() const
+{
+    for (int i = 0; i < dimension; i++)
+        if (!(pointer[i] >= 0.)) return false;
+    return true;
+}
+
+bool FloatVector::is_nonpositive() const
+{
+    for (int i = 0; i < dimension; i++)
+        if (!(pointer[i] <= 0.)) return false;
+    return true;
+}
+
+bool FloatVector::is_numerically_small(Float threshold) const { return norm() < threshold; }
+
+bool FloatVector::is_equal_to(const FloatVector& l, const FloatVector& r)
+{
+    require_true(l.dimension == r.dimension, "comparison of equal dimension");
+    for (int i = 0; i < l.dimension; i++)
+        if (l.pointer[i] != r.pointer[i]) return false;
+    return true;
+}
+
+// ============================================================== MatrixCSR
+
+MatrixCSR::MatrixCSR(int rows, int columns, const std::vector<int>& A_, const std::vector<int>& C_,
+                     const std::vector<Float>& V_)
+    : LinearOperator(rows, columns), A(A_), C(C_), V(V_)
+{
+    check();
+}
+
+MatrixCSR::MatrixCSR(int rows, int columns, std::vector<int>&& A_, std::vector<int>&& C_,
+                     std::vector<Float>&& V_)
+    : LinearOperator(rows, columns), A(std::move(A_)), C(std::move(C_)), V(std::move(V_))
+{
+    check();
+}
+
+MatrixCSR::MatrixCSR(int rows, int columns)
+    : LinearOperator(rows, columns), A(rows + 1, 0), C(), V()
+{
+    check();
+}
+
+MatrixCSR::MatrixCSR(const MatrixCSR& m) : LinearOperator(m.getdimout(), m.getdimin()), A(m.A), C(m.C), V(m.V) {}
+
+MatrixCSR& MatrixCSR::operator=(const MatrixCSR& m)
+{
+    require_true(getdimin() == m.getdimin() && getdimout() == m.getdimout(), "MatrixCSR assignment");
+    if (this != &m) {
+        A = m.A;
+        C = m.C;
+        V = m.V;
+    }
+    return *this;
+}
+
+MatrixCSR::MatrixCSR(MatrixCSR&& m) noexcept
+    : LinearOperator(m.getdimout(), m.getdimin()), A(std::move(m.A)), C(std::move(m.C)), V(std::move(m.V))
+{
+}
+
+MatrixCSR& MatrixCSR::operator=(MatrixCSR&& m) noexcept
+{
+    if (this != &m) {
+        A = std::move(m.A);
+        C = std::move(m.C);
+        V = std::move(m.V);
+    }
+    return *this;
+}
+
+// sparse/matcsr.cpp:178-194
+void MatrixCSR::check() const
+{
+    require_true((int)A.size() == getdimout() + 1, "MatrixCSR: A has rows+1 entries");
+    require_true(C.size() == V.size(), "MatrixCSR: C and V have equal length");
+    for (int p = 1; p <= getdimout(); p++) require_true(A[p - 1] <= A[p], "MatrixCSR: A ascending");
+    require_true(A[getdimout()] == (int)V.size(), "MatrixCSR: A ends at nnz");
+    for (size_t i = 0; i < C.size(); i++)
+        require_true(0 <= C[i] && C[i] < getdimin() && std::isfinite(V[i]), "MatrixCSR: entries");
+}
+
+std::string MatrixCSR::text() const
+{
+    return "CSRMatrix " + std::to_string(getdimout()) + "x" + std::to_string(getdimin()) + " with " +
+           std::to_string(V.size()) + " entries";
+}
+
+void MatrixCSR::apply(FloatVector& dest, const FloatVector& add, Float scaling) const
+{
+    require_true(getdimin() == add.getdimension() && getdimout() == dest.getdimension(),
+                 "MatrixCSR::apply dimensions");
+    require_true(&dest != &add, "MatrixCSR::apply: dest and source differ");
+    if (getdimout() == 0) return;
+    if (getdimin() == 0) {
+        dest.zero();
+        return;
+    }
+    require(sfa_MatrixCSR_apply(getdimout(), getdimin(), A.data(), C.data(), V.data(), dest.raw(),
+                                add.raw(), scaling),
+            "MatrixCSR::apply");
+}
+
+void MatrixCSR::scale(Float s)
+{
+    for (auto& v : V) v *= s;
+}
+
+bool MatrixCSR::is_finite() const
+{
+    for (Float v : V)
+        if (!std::isfinite(v)) return false;
+    return true;
+}
+
+FloatVector MatrixCSR::getDiagonal() const
+{
+    require_true(getdimin() == getdimout(), "getDiagonal of a square matrix");
+    FloatVector ret(getdimin(), 0.);
+    if (getdimin() > 0)
+        require(sfa_MatrixCSR_getDiagonal(getdimin(), A.data(), C.data(), V.data(), ret.raw()),
+                "MatrixCSR::getDiagonal");
+    return ret;
+}
+
+// sparse/matcsr.cpp:291-350: counting transpose, then sort and compress.
+MatrixCSR MatrixCSR::getTranspose() const
+{
+    const int rows = getdimout(), cols = getdimin();
+    const int nnz = (int)V.size();
+    std::vector<int> B(cols + 1, 0), D(nnz, 0), Z(cols, 0);
+    std::vector<Float> W(nnz, 0.);
+    for (int i = 0; i < nnz; i++) Z[C[i]]++;
+    for (int c = 1; c <= cols; c++) B[c] = B[c - 1] + Z[c - 1];
+    for (int r = 0; r < rows; r++)
+        for (int i = A[r]; i < A[r + 1]; i++) {
+            const int c = C[i];
+            const int at = B[c] + (--Z[c]);
+            D[at] = r;
+            W[at] = V[i];
+        }
+    sort_and_compress(B, D, W);
+    return MatrixCSR(cols, rows, std::move(B), std::move(D), std::move(W));
+}
+
+int MatrixCSR::getnumberofzeroentries() const
+{
+    int ret = 0;
+    for (Float v : V)
+        if (v == 0.) ret++;
+    return ret;
+}
+
+int MatrixCSR::getmaxrowwidth() const
+{
+    int ret = 0;
+    for (int r = 0; r < getdimout(); r++) ret = std::max(ret, A[r + 1] - A[r]);
+    return ret;
+}
+
+Float MatrixCSR::eigenvalueupperbound() const
+{
+    Float ret = 0.;
+    for (int r = 0; r < getdimout(); r++) {
+        Float row = 0.;
+        for (int c = A[r]; c < A[r + 1]; c++) row += absolute(V[c]);
+        ret = maximum(ret, row);
+    }
+    return ret;
+}
+
+void MatrixCSR::compressentries() const
+{
+    auto& a = const_cast<std::vector<int>&>(A);
+    auto& c = const_cast<std::vector<int>&>(C);
+    auto& v = const_cast<std::vector<Float>&>(V);
+    sort_and_compress(a, c, v);
+}
+
+std::size_t MatrixCSR::memorysize() const
+{
+    return sizeof(*this) + A.size() * sizeof(int) + C.size() * sizeof(int) + V.size() * sizeof(Float);
+}
+
+// sparse/matcsr.cpp:707-765: concatenate the rows, then sort and compress.
+MatrixCSR MatrixCSRAddition(const MatrixCSR& m1, const MatrixCSR& m2, Float s1, Float s2)
+{
+    require_true(m1.getdimout() == m2.getdimout() && m1.getdimin() == m2.getdimin(),
+                 "MatrixCSRAddition dimensions");
+    const int rows = m1.getdimout();
+    const int* A1 = m1.getA();
+    const int* A2 = m2.getA();
+    std::vector<int> A(rows + 1, 0);
+    for (int r = 1; r <= rows; r++) A[r] = A[r - 1] + (A1[r] - A1[r - 1]) + (A2[r] - A2[r - 1]);
+    std::vector<int> C(A[rows]);
+    std::vector<Float> V(A[rows]);
+    for (int r = 0; r < rows; r++) {
+        int at = A[r];
+        for (int k = A1[r]; k < A1[r + 1]; k++, at++) C[at] = m1.getC()[k], V[at] = s1 * m1.getV()[k];
+        for (int k = A2[r]; k < A2[r + 1]; k++, at++) C[at] = m2.getC()[k], V[at] = s2 * m2.getV()[k];
+    }
+    sort_and_compress(A, C, V);
+    return MatrixCSR(rows, m1.getdimin(), std::move(A), std::move(C), std::move(V));
+}
+
+// ============================================================== Rainbow
+
+Rainbow::Rainbow(const MatrixCSR& mat, bool do_shuffle)
+    : num_rows(mat.getdimout()), num_colors(0), F(), B(), R()
+{
+    require_true(mat.is_square(), "Rainbow of a square matrix");
+    require_true(!do_shuffle, "Rainbow: the shuffled painting order is not supported");
+    const int n = num_rows;
+    F.assign(n, 0);
+    R.assign(n, 0);
+    std::vector<int> Bfull(n + 1, 0);
+    if (n > 0) require(sfa_Rainbow(n, mat.getA(), mat.getC(), F.data(), Bfull.data(), R.data(), &num_colors),
+                       "Rainbow");
+    else num_colors = 1;
+    B.assign(Bfull.begin(), Bfull.begin() + num_colors + 1);
+}
+
+// sparse/rainbow.cpp:133-152
+void Rainbow::check() const
+{
+    for (int c = 0; c < num_colors; c++)
+        for (int i = B[c]; i < B[c + 1]; i++) require_true(F[R[i]] == c, "Rainbow: F agrees with B/R");
+}
+
+std::string Rainbow::text() const { return "num_colors: " + std::to_string(num_colors); }
+
+// ============================================================== solvers
+
+int ConjugateGradientSolverCSR(const int N, Float* x, const Float* b, const int* csrrows,
+                               const int* csrcolumns, const Float* csrvalues, Float* residual,
+                               Float precision, int print_modulo)
+{
+    sfa_solve_info info{};
+    require(sfa_ConjugateGradientSolverCSR(N, x, b, csrrows, csrcolumns, csrvalues, residual,
+                                           precision, print_modulo, &info),
+            "ConjugateGradientSolverCSR");
+    return info.iterations;
+}
+
+int ConjugateGradientSolverCSR_DiagonalPreconditioner(const int N, Float* x, const Float* b,
+                                                      const int* csrrows, const int* csrcolumns,
+                                                      const Float* csrvalues, Float* residual,
+                                                      Float precision, int print_modulo,
+                                                      const Float* precon)
+{
+    sfa_solve_info info{};
+    require(sfa_ConjugateGradientSolverCSR_DiagonalPreconditioner(
+                N, x, b, csrrows, csrcolumns, csrvalues, residual, precision, print_modulo, precon,
+                &info),
+            "ConjugateGradientSolverCSR_DiagonalPreconditioner");
+    return info.iterations;
+}
+
+int ConjugateGradientSolverCSR_SSOR(const int N, Float* x, const Float* b, const int* csrrows,
+                                    const int* csrcolumns, const Float* csrvalues, Float* residual,
+                                    Float precision, int print_modulo, const Float* diagonal,
+                                    Float omega)
+{
+    sfa_solve_info info{};
+    require(sfa_ConjugateGradientSolverCSR_SSOR(N, x, b, csrrows, csrcolumns, csrvalues, residual,
+                                                precision, print_modulo, diagonal, omega, &info),
+            "ConjugateGradientSolverCSR_SSOR");
+    return info.iterations;
+}
+
+int ConjugateGradientSolverCSR_SSOR_Eisenstat(const int N, Float* x, const Float* b,
+                                              const int* csrrows, const int* csrcolumns,
+                                              const Float* csrvalues, Float* residual,
+                                              Float precision, int print_modulo,
+                                              const Float* diagonal, Float omega)
+{
+    sfa_solve_info info{};
+    require(sfa_ConjugateGradientSolverCSR_SSOR_Eisenstat(N, x, b, csrrows, csrcolumns, csrvalues,
+                                                          residual, precision, print_modulo,
+                                                          diagonal, omega, &info),
+            "ConjugateGradientSolverCSR_SSOR_Eisenstat");
+    return info.iterations;
+}
+
+int ConjugateGradientSolverCSR_Rainbow(const int N, Float* x, const Float* b, const int* csrrows,
+                                       const int* csrcolumns, const Float* csrvalues,
+                                       Float* residual, Float precision, int print_modulo,
+                                       const Float* diagonal, Float omega, int num_colors,
+                                       const int* F, const int* B, const int* R)
+{
+    sfa_solve_info info{};
+    require(sfa_ConjugateGradientSolverCSR_Rainbow(N, x, b, csrrows, csrcolumns, csrvalues,
+                                                   residual, precision, print_modulo, diagonal,
+                                                   omega, num_colors, F, B, R, &info),
+            "ConjugateGradientSolverCSR_Rainbow");
+    return info.iterations;
+}
+
+int ConjugateGradientSolverCSR_Eisenstat_Rainbow(const int N, Float* x, const Float* b,
+                                                 const int* csrrows, const int* csrcolumns,
+                                                 const Float* csrvalues, Float* residual,
+                                                 Float precision, int print_modulo,
+                                                 const Float* diagonal, Float omega,
+                                                 int num_colors, const int* F, const int* B,
+                                                 const int* R)
+{
+    sfa_solve_info info{};
+    require(sfa_ConjugateGradientSolverCSR_Eisenstat_Rainbow(
+                N, x, b, csrrows, csrcolumns, csrvalues, residual, precision, print_modulo,
+                diagonal, omega, num_colors, F, B, R, &info),
+            "ConjugateGradientSolverCSR_Eisenstat_Rainbow");
+    return info.iterations;
+}

@@ -275,3 +275,16 @@ def test_full_size_3d_rainbow():
     b = S.rhs_uniform(system, seed=2026)
     io, ig = _check_parity("rainbow", system, b, omega=1.0)
     print("[full] K oracle", io["iterations"], "K gpu", ig["iterations"], flush=True)
+
+
+def test_cpp_dropin_callers():
+    """A C++ caller of the drop-in headers, in the shape of the reference's
+    tests/solver/solvespd_csr.cpp: every variant converges on the GPU."""
+    import os
+    import subprocess
+
+    binary = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                          "simplefem_amd", "_build", "dropin_test")
+    proc = subprocess.run([binary, "gpu"], capture_output=True, text=True, timeout=240)
+    print(proc.stdout[-3000:])
+    assert proc.returncode == 0, proc.stdout[-3000:] + proc.stderr[-2000:]
