@@ -169,8 +169,15 @@ __device__ inline double sell_row(const SellView& A, int slice, int lane, const 
     return acc;
 }
 
-__device__ inline int slice_row(const SellView& A, int slice, int lane)
+// Device row of a lane: computed when the group's rows are consecutive
+// (row0 >= 0), read from the slice's row list otherwise.
+__device__ inline int group_row(const SellView& A, int slice, int local, int lane, int row0,
+                                int nrows)
 {
+    if (row0 >= 0) {
+        const int i = local * kWave + lane;
+        return i < nrows ? row0 + i : -1;
+    }
     return A.rows[slice * kWave + lane];
 }
 
@@ -230,13 +237,14 @@ __global__ void __launch_bounds__(kVecBlock) k_init_state(SolveState* st, const 
 // ------------------------------------------------------------------ apply
 
 // MatrixCSR::apply (sparse/matcsr.cpp:216-244): dest = 0; dest += scaling*V*x.
-__global__ void __launch_bounds__(kSellBlock) k_apply(SellView A, int nslices, const double* src,
-                                                      double* dest, double scaling)
+__global__ void __launch_bounds__(kSellBlock) k_apply(SellView A, int nslices, int row0, int nrows,
+                                                      const double* src, double* dest,
+                                                      double scaling)
 {
     const int slice = blockIdx.x;
     const int lane = threadIdx.x;
     if (slice >= nslices) return;
-    const int row = slice_row(A, slice, lane);
+    const int row = group_row(A, slice, slice, lane, row0, nrows);
     if (row < 0) return;
     const int beg = A.off[slice] + lane, end = A.off[slice + 1];
     double acc = 0.;
@@ -249,6 +257,7 @@ __global__ void __launch_bounds__(kSellBlock) k_apply(SellView A, int nslices, c
 // solver/sparsesolver.cpp:80-98 (plain), :281-306 (Jacobi), :512-524 (SSOR),
 // :797-812 (Eisenstat, columns already filtered into the part).
 __global__ void __launch_bounds__(kSellBlock) k_residual(int mode, SellView A, int nslices,
+                                                         int row0, int nrows,
                                                          SolveState* st, const double* b,
                                                          const double* x, double* out, double* d,
                                                          double* z, const double* precon,
@@ -257,7 +266,7 @@ __global__ void __launch_bounds__(kSellBlock) k_residual(int mode, SellView A, i
     const int slice = blockIdx.x;
     const int lane = threadIdx.x;
     double part = 0.;
-    const int row = slice < nslices ? slice_row(A, slice, lane) : -1;
+    const int row = slice < nslices ? group_row(A, slice, slice, lane, row0, nrows) : -1;
     if (row >= 0) {
         const double r = sell_row<true>(A, slice, lane, x, b[row]);
         out[row] = r;
@@ -317,6 +326,7 @@ __global__ void __launch_bounds__(kVecBlock) k_eisenstat_init(SolveState* st, in
 // Forward sweep of one colour / level, solver/sparsesolver.cpp:528-541 (restart)
 // and :647-665, :1234-1261 (iteration, with the x and r updates).
 __global__ void __launch_bounds__(kSellBlock) k_forward(SellView L, int slice0, int nslices,
+                                                        int row0, int nrows,
                                                         const SolveState* st, int update_xr,
                                                         double* x, double* r, const double* d,
                                                         const double* q, double* m,
@@ -326,7 +336,7 @@ __global__ void __launch_bounds__(kSellBlock) k_forward(SellView L, int slice0, 
     const int slice = slice0 + blockIdx.x;
     const int lane = threadIdx.x;
     if (blockIdx.x >= nslices) return;
-    const int row = slice_row(L, slice, lane);
+    const int row = group_row(L, slice, blockIdx.x, lane, row0, nrows);
     if (row < 0) return;
     double rr;
     if (update_xr) {
@@ -345,6 +355,7 @@ __global__ void __launch_bounds__(kSellBlock) k_forward(SellView L, int slice0, 
 // (restart) and :667-692, :1263-1303 (iteration): the scaling of m fused in,
 // z.r summed per group and over groups in launch order.
 __global__ void __launch_bounds__(kSellBlock) k_backward(SellView U, int slice0, int nslices,
+                                                         int row0, int nrows,
                                                          SolveState* st, int reduce, int group,
                                                          int last_group, const double* r,
                                                          const double* m, double* z,
@@ -355,7 +366,7 @@ __global__ void __launch_bounds__(kSellBlock) k_backward(SellView U, int slice0,
     const int slice = slice0 + blockIdx.x;
     const int lane = threadIdx.x;
     double part = 0.;
-    const int row = blockIdx.x < nslices ? slice_row(U, slice, lane) : -1;
+    const int row = blockIdx.x < nslices ? group_row(U, slice, blockIdx.x, lane, row0, nrows) : -1;
     if (row >= 0) {
         const double dg = diag[row];
         double t = m[row];
@@ -380,6 +391,7 @@ __global__ void __launch_bounds__(kSellBlock) k_backward(SellView U, int slice0,
 // Eisenstat triangular solves (solver/sparsesolver.cpp:814-828, 876-908,
 // 1429-1455, 1501-1560): out = (in - [c3*diag*t] - T out) * w / diag.
 __global__ void __launch_bounds__(kSellBlock) k_eisenstat_solve(SellView T, int slice0, int nslices,
+                                                                int row0, int nrows,
                                                                 const SolveState* st, int with_t,
                                                                 const double* in, const double* t,
                                                                 double* out, const double* diag,
@@ -389,7 +401,7 @@ __global__ void __launch_bounds__(kSellBlock) k_eisenstat_solve(SellView T, int 
     const int slice = slice0 + blockIdx.x;
     const int lane = threadIdx.x;
     if (blockIdx.x >= nslices) return;
-    const int row = slice_row(T, slice, lane);
+    const int row = group_row(T, slice, blockIdx.x, lane, row0, nrows);
     if (row < 0) return;
     const double dg = diag[row];
     double a = in[row];
@@ -402,7 +414,8 @@ __global__ void __launch_bounds__(kSellBlock) k_eisenstat_solve(SellView T, int 
 
 // q = A d, d.Ad (solver/sparsesolver.cpp:611-626, 1197-1212), then the
 // breakdown tests and alpha.
-__global__ void __launch_bounds__(kSellBlock) k_spmv_dot(SellView A, int nslices, SolveState* st,
+__global__ void __launch_bounds__(kSellBlock) k_spmv_dot(SellView A, int nslices, int row0,
+                                                         int nrows, SolveState* st,
                                                          const double* d, double* q,
                                                          double* partials)
 {
@@ -410,7 +423,7 @@ __global__ void __launch_bounds__(kSellBlock) k_spmv_dot(SellView A, int nslices
     const int slice = blockIdx.x;
     const int lane = threadIdx.x;
     double part = 0.;
-    const int row = slice < nslices ? slice_row(A, slice, lane) : -1;
+    const int row = slice < nslices ? group_row(A, slice, slice, lane, row0, nrows) : -1;
     if (row >= 0) {
         const double acc = sell_row<false>(A, slice, lane, d, 0.);
         q[row] = acc;
@@ -518,19 +531,20 @@ void launch_init_state(hipStream_t s, SolveState* st, const double* b, int n, do
         st, b, n, precision, host_tol, use_host_tol, maxK, small_breaks, partials);
 }
 
-void launch_apply(hipStream_t s, SellView A, int nslices, const double* src, double* dest,
-                  double scaling)
+void launch_apply(hipStream_t s, SellView A, int nslices, int row0, int nrows, const double* src,
+                  double* dest, double scaling)
 {
-    if (nslices > 0) k_apply<<<nslices, kSellBlock, 0, s>>>(A, nslices, src, dest, scaling);
+    if (nslices > 0)
+        k_apply<<<nslices, kSellBlock, 0, s>>>(A, nslices, row0, nrows, src, dest, scaling);
 }
 
-void launch_residual(hipStream_t s, ResidualMode mode, SellView A, int nslices, SolveState* st,
-                     const double* b, const double* x, double* out, double* d, double* z,
-                     const double* precon, double* partials)
+void launch_residual(hipStream_t s, ResidualMode mode, SellView A, int nslices, int row0,
+                     int nrows, SolveState* st, const double* b, const double* x, double* out,
+                     double* d, double* z, const double* precon, double* partials)
 {
     const int grid = nslices > 0 ? nslices : 1;
-    k_residual<<<grid, kSellBlock, 0, s>>>((int)mode, A, nslices, st, b, x, out, d, z, precon,
-                                           partials);
+    k_residual<<<grid, kSellBlock, 0, s>>>((int)mode, A, nslices, row0, nrows, st, b, x, out, d,
+                                           z, precon, partials);
 }
 
 void launch_direction_init(hipStream_t s, SolveState* st, int n_active, const double* z,
@@ -547,42 +561,43 @@ void launch_eisenstat_init(hipStream_t s, SolveState* st, int n_active, const do
                                                                 partials);
 }
 
-void launch_forward(hipStream_t s, SellView L, int slice0, int nslices, SolveState* st,
-                    int update_xr, double* x, double* r, const double* d, const double* q,
-                    double* m, const double* diag, double omega)
+void launch_forward(hipStream_t s, SellView L, int slice0, int nslices, int row0, int nrows,
+                    SolveState* st, int update_xr, double* x, double* r, const double* d,
+                    const double* q, double* m, const double* diag, double omega)
 {
     if (nslices <= 0) return;
-    k_forward<<<nslices, kSellBlock, 0, s>>>(L, slice0, nslices, st, update_xr, x, r, d, q, m,
-                                             diag, omega);
+    k_forward<<<nslices, kSellBlock, 0, s>>>(L, slice0, nslices, row0, nrows, st, update_xr, x, r,
+                                             d, q, m, diag, omega);
 }
 
-void launch_backward(hipStream_t s, SellView U, int slice0, int nslices, SolveState* st,
-                     int reduce, int group, int last_group, const double* r, const double* m,
-                     double* z, const double* diag, double omega, double c2, double* grp,
-                     double* partials)
+void launch_backward(hipStream_t s, SellView U, int slice0, int nslices, int row0, int nrows,
+                     SolveState* st, int reduce, int group, int last_group, const double* r,
+                     const double* m, double* z, const double* diag, double omega, double c2,
+                     double* grp, double* partials)
 {
     // A group with no rows still runs one block when it has to publish its
     // (zero) partial or finish the z.r sum.
     const int grid = nslices > 0 ? nslices : (reduce ? 1 : 0);
     if (grid == 0) return;
-    k_backward<<<grid, kSellBlock, 0, s>>>(U, slice0, nslices, st, reduce, group, last_group, r,
-                                           m, z, diag, omega, c2, grp, partials);
+    k_backward<<<grid, kSellBlock, 0, s>>>(U, slice0, nslices, row0, nrows, st, reduce, group,
+                                           last_group, r, m, z, diag, omega, c2, grp, partials);
 }
 
-void launch_eisenstat_solve(hipStream_t s, SellView T, int slice0, int nslices, SolveState* st,
-                            int with_t, const double* in, const double* t, double* out,
-                            const double* diag, double omega, double c3)
+void launch_eisenstat_solve(hipStream_t s, SellView T, int slice0, int nslices, int row0,
+                            int nrows, SolveState* st, int with_t, const double* in,
+                            const double* t, double* out, const double* diag, double omega,
+                            double c3)
 {
     if (nslices <= 0) return;
-    k_eisenstat_solve<<<nslices, kSellBlock, 0, s>>>(T, slice0, nslices, st, with_t, in, t, out,
-                                                     diag, omega, c3);
+    k_eisenstat_solve<<<nslices, kSellBlock, 0, s>>>(T, slice0, nslices, row0, nrows, st, with_t,
+                                                     in, t, out, diag, omega, c3);
 }
 
-void launch_spmv_dot(hipStream_t s, SellView A, int nslices, SolveState* st, const double* d,
-                     double* q, double* partials)
+void launch_spmv_dot(hipStream_t s, SellView A, int nslices, int row0, int nrows, SolveState* st,
+                     const double* d, double* q, double* partials)
 {
     const int grid = nslices > 0 ? nslices : 1;
-    k_spmv_dot<<<grid, kSellBlock, 0, s>>>(A, nslices, st, d, q, partials);
+    k_spmv_dot<<<grid, kSellBlock, 0, s>>>(A, nslices, row0, nrows, st, d, q, partials);
 }
 
 void launch_update(hipStream_t s, UpdateMode mode, SolveState* st, int n_active, double* x,

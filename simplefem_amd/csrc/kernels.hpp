@@ -44,17 +44,19 @@ void launch_init_state(hipStream_t s, SolveState* st, const double* b, int n, do
                        double* partials);
 
 // MatrixCSR::apply: dest[row] = sum (scaling*v)*src[col], over a part's rows.
-void launch_apply(hipStream_t s, SellView A, int nslices, const double* src, double* dest,
-                  double scaling);
+// (row0, nrows): a group whose rows are row0 .. row0+nrows-1 in slice order;
+// row0 = -1 makes the kernels read row ids from the part's row list.
+void launch_apply(hipStream_t s, SellView A, int nslices, int row0, int nrows, const double* src,
+                  double* dest, double scaling);
 
 // --- restart (K = 0) ---
 enum ResidualMode { RES_ONLY = 0, RES_PLAIN = 1, RES_JACOBI = 2 };
 // r = b - A x on the part's rows; RES_PLAIN also sets d = r and z.r = r.r,
 // RES_JACOBI z = p*r, d = z, z.r; both then test convergence.  With
 // out != r the result goes to out (the Eisenstat restart writes q).
-void launch_residual(hipStream_t s, ResidualMode mode, SellView A, int nslices, SolveState* st,
-                     const double* b, const double* x, double* out, double* d, double* z,
-                     const double* precon, double* partials);
+void launch_residual(hipStream_t s, ResidualMode mode, SellView A, int nslices, int row0,
+                     int nrows, SolveState* st, const double* b, const double* x, double* out,
+                     double* d, double* z, const double* precon, double* partials);
 // d = z, z.r, convergence test (SSOR family restart).
 void launch_direction_init(hipStream_t s, SolveState* st, int n_active, const double* z,
                            const double* r, double* d, double* partials);
@@ -65,23 +67,24 @@ void launch_eisenstat_init(hipStream_t s, SolveState* st, int n_active, const do
 
 // --- sweeps over one group (colour or level) of a triangular part ---
 // forward: [x += a d; r -= a q;]  m = (r - L m) * w / diag
-void launch_forward(hipStream_t s, SellView L, int slice0, int nslices, SolveState* st,
-                    int update_xr, double* x, double* r, const double* d, const double* q,
-                    double* m, const double* diag, double omega);
+void launch_forward(hipStream_t s, SellView L, int slice0, int nslices, int row0, int nrows,
+                    SolveState* st, int update_xr, double* x, double* r, const double* d,
+                    const double* q, double* m, const double* diag, double omega);
 // backward: t = m*diag*c2; z = (t - U z) * w / diag; [group z.r partial]
-void launch_backward(hipStream_t s, SellView U, int slice0, int nslices, SolveState* st,
-                     int reduce, int group, int last_group, const double* r, const double* m,
-                     double* z, const double* diag, double omega, double c2, double* grp,
-                     double* partials);
+void launch_backward(hipStream_t s, SellView U, int slice0, int nslices, int row0, int nrows,
+                     SolveState* st, int reduce, int group, int last_group, const double* r,
+                     const double* m, double* z, const double* diag, double omega, double c2,
+                     double* grp, double* partials);
 // Eisenstat: out = (in - [c3*diag*t] - T out) * w / diag  (T = L or U part)
-void launch_eisenstat_solve(hipStream_t s, SellView T, int slice0, int nslices, SolveState* st,
-                            int with_t, const double* in, const double* t, double* out,
-                            const double* diag, double omega, double c3);
+void launch_eisenstat_solve(hipStream_t s, SellView T, int slice0, int nslices, int row0,
+                            int nrows, SolveState* st, int with_t, const double* in,
+                            const double* t, double* out, const double* diag, double omega,
+                            double c3);
 
 // --- iteration ---
 // q = A d, d.Ad, breakdown tests, alpha
-void launch_spmv_dot(hipStream_t s, SellView A, int nslices, SolveState* st, const double* d,
-                     double* q, double* partials);
+void launch_spmv_dot(hipStream_t s, SellView A, int nslices, int row0, int nrows, SolveState* st,
+                     const double* d, double* q, double* partials);
 enum UpdateMode { UPD_PLAIN = 0, UPD_JACOBI = 1, UPD_EISENSTAT = 2 };
 // x += a*dx; r -= a*q; z.r per mode; beta; K++; convergence test
 void launch_update(hipStream_t s, UpdateMode mode, SolveState* st, int n_active, double* x,

@@ -71,6 +71,9 @@ void append_group(SellPart& part, const std::vector<int>& rows_dev, const std::v
                   const double* V, Keep keep)
 {
     const int count = (int)rows_dev.size();
+    bool consecutive = true;
+    for (int i = 1; i < count && consecutive; i++) consecutive = rows_dev[i] == rows_dev[0] + i;
+    part.pending_row0 = (count > 0 && consecutive) ? rows_dev[0] : (count == 0 ? 0 : -1);
     for (int s0 = 0; s0 < count; s0 += kSliceRows) {
         const int s1 = std::min(count, s0 + kSliceRows);
         int width = 0;
@@ -122,6 +125,8 @@ void close_group(SellPart& part)
     for (long long v : part.group_rows) r -= v;
     part.group_entries.push_back(e);
     part.group_rows.push_back(r);
+    part.group_row0.push_back(part.pending_row0);
+    part.pending_row0 = -1;
     part.group_slice.push_back(part.nslices);
 }
 

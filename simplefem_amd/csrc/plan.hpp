@@ -34,6 +34,10 @@ struct SellPart {
     std::vector<int> group_slice;  // ngroups+1 slice ranges, in launch order
     std::vector<long long> group_entries;  // real entries per group
     std::vector<long long> group_rows;     // real rows per group
+    // first device row of each group whose rows are consecutive in slice
+    // order (the kernels then compute row ids instead of loading them), else -1
+    std::vector<int> group_row0;
+    int pending_row0 = -1;
     long long entries = 0;         // real (unpadded) entries
     long long rows = 0;            // real rows
     int ngroups() const { return (int)group_slice.size() - 1; }

@@ -69,6 +69,7 @@ struct DevPart {
     int nslices = 0;
     std::vector<int> group_slice;
     std::vector<long long> group_entries, group_rows;
+    std::vector<int> group_row0;
     long long stored = 0;
     long long entries = 0;
     long long nrows = 0;
@@ -79,6 +80,7 @@ struct DevPart {
         group_slice = p.group_slice;
         group_entries = p.group_entries;
         group_rows = p.group_rows;
+        group_row0 = p.group_row0;
         stored = (long long)p.vals.size();
         entries = p.entries;
         nrows = p.rows;
@@ -107,6 +109,8 @@ struct DevPart {
     int ngroups() const { return (int)group_slice.size() - 1; }
     int group_begin(int g) const { return group_slice[g]; }
     int group_count(int g) const { return group_slice[g + 1] - group_slice[g]; }
+    int row0(int g) const { return group_row0[g]; }
+    int nrows_of(int g) const { return (int)group_rows[g]; }
     // Algorithmic bytes of one launch over group g: 12 B per entry (8 value +
     // 4 column), 4 B per row id, `per_row` B of vector traffic per row.
     double bytes(int g, double per_row) const
@@ -307,8 +311,8 @@ struct System {
         const double per_row = update_xr ? 64.0 : 24.0;
         for (int g = 0; g < L.ngroups(); g++)
             run(s, SFA_KERNEL_FORWARD, L.bytes(g, per_row), [&] {
-                launch_forward(s, L.view(), L.group_begin(g), L.group_count(g), state, update_xr,
-                               x, r, d, q, m, diag, omega);
+                launch_forward(s, L.view(), L.group_begin(g), L.group_count(g), L.row0(g),
+                               L.nrows_of(g), state, update_xr, x, r, d, q, m, diag, omega);
             });
     }
 
@@ -317,8 +321,9 @@ struct System {
         const int G = upper.ngroups();
         for (int g = 0; g < G; g++)  // m, diag, z, r
             run(s, SFA_KERNEL_BACKWARD, upper.bytes(g, 32.0), [&] {
-                launch_backward(s, upper.view(), upper.group_begin(g), upper.group_count(g), state,
-                                reduce, g, g == G - 1, r, m, z, diag, omega, c2(), grp, partials);
+                launch_backward(s, upper.view(), upper.group_begin(g), upper.group_count(g),
+                                upper.row0(g), upper.nrows_of(g), state, reduce, g, g == G - 1, r,
+                                m, z, diag, omega, c2(), grp, partials);
             });
     }
 
@@ -328,8 +333,9 @@ struct System {
         const double per_row = with_t ? 32.0 : 24.0;  // in, diag, out [, t]
         for (int g = 0; g < T.ngroups(); g++)
             run(s, SFA_KERNEL_TRIANGULAR, T.bytes(g, per_row), [&] {
-                launch_eisenstat_solve(s, T.view(), T.group_begin(g), T.group_count(g), state,
-                                       with_t, in, t, out, diag, omega, c3());
+                launch_eisenstat_solve(s, T.view(), T.group_begin(g), T.group_count(g), T.row0(g),
+                                       T.nrows_of(g), state, with_t, in, t, out, diag, omega,
+                                       c3());
             });
     }
 
@@ -341,20 +347,20 @@ struct System {
         switch (method) {
         case SFA_METHOD_CG:
             run(s, SFA_KERNEL_RESTART, full.bytes_all(32.0), [&] {
-                launch_residual(s, RES_PLAIN, F, full.nslices, state, b, x, r, d, nullptr,
+                launch_residual(s, RES_PLAIN, F, full.nslices, full.row0(0), full.nrows_of(0), state, b, x, r, d, nullptr,
                                 nullptr, partials);
             });
             break;
         case SFA_METHOD_DIAGONAL:
             run(s, SFA_KERNEL_RESTART, full.bytes_all(48.0), [&] {
-                launch_residual(s, RES_JACOBI, F, full.nslices, state, b, x, r, d, z, diag,
+                launch_residual(s, RES_JACOBI, F, full.nslices, full.row0(0), full.nrows_of(0), state, b, x, r, d, z, diag,
                                 partials);
             });
             break;
         case SFA_METHOD_SSOR:
         case SFA_METHOD_RAINBOW:
             run(s, SFA_KERNEL_RESTART, full.bytes_all(16.0), [&] {
-                launch_residual(s, RES_ONLY, F, full.nslices, state, b, x, r, nullptr, nullptr,
+                launch_residual(s, RES_ONLY, F, full.nslices, full.row0(0), full.nrows_of(0), state, b, x, r, nullptr, nullptr,
                                 nullptr, partials);
             });
             forward_sweep(s, lower, 0);
@@ -364,7 +370,7 @@ struct System {
             break;
         default:  // Eisenstat variants: q = b - A x, then r = lower solve of q
             run(s, SFA_KERNEL_RESTART, full.bytes_all(16.0), [&] {
-                launch_residual(s, RES_ONLY, F, full.nslices, state, b, x, q, nullptr, nullptr,
+                launch_residual(s, RES_ONLY, F, full.nslices, full.row0(0), full.nrows_of(0), state, b, x, q, nullptr, nullptr,
                                 nullptr, partials);
             });
             eisenstat_sweep(s, has_lower_restart ? lower_restart : lower, 0, q, nullptr, r);
@@ -384,7 +390,7 @@ struct System {
         switch (method) {
         case SFA_METHOD_CG:
             run(s, SFA_KERNEL_SPMV, spmv_bytes,
-                [&] { launch_spmv_dot(s, F, full.nslices, state, d, q, partials); });
+                [&] { launch_spmv_dot(s, F, full.nslices, full.row0(0), full.nrows_of(0), state, d, q, partials); });
             run(s, SFA_KERNEL_UPDATE, 48.0 * na, [&] {
                 launch_update(s, UPD_PLAIN, state, n_active, x, r, d, q, nullptr, nullptr, omega,
                               partials);
@@ -394,7 +400,7 @@ struct System {
             break;
         case SFA_METHOD_DIAGONAL:
             run(s, SFA_KERNEL_SPMV, spmv_bytes,
-                [&] { launch_spmv_dot(s, F, full.nslices, state, d, q, partials); });
+                [&] { launch_spmv_dot(s, F, full.nslices, full.row0(0), full.nrows_of(0), state, d, q, partials); });
             run(s, SFA_KERNEL_UPDATE, 64.0 * na, [&] {
                 launch_update(s, UPD_JACOBI, state, n_active, x, r, d, q, z, diag, omega,
                               partials);
@@ -405,7 +411,7 @@ struct System {
         case SFA_METHOD_SSOR:
         case SFA_METHOD_RAINBOW:
             run(s, SFA_KERNEL_SPMV, spmv_bytes,
-                [&] { launch_spmv_dot(s, F, full.nslices, state, d, q, partials); });
+                [&] { launch_spmv_dot(s, F, full.nslices, full.row0(0), full.nrows_of(0), state, d, q, partials); });
             forward_sweep(s, lower, 1);
             backward_sweep(s, 1);
             run(s, SFA_KERNEL_DIRECTION, 24.0 * na,
@@ -651,7 +657,8 @@ struct System {
             SFA_HIP(hipMemcpyAsync(srcbuf, src, n * sizeof(double), hipMemcpyHostToDevice, s));
         launch_fill(s, srcbuf + n, 0.0, 1);
         double* out = device_pointers ? dest : io;
-        launch_apply(s, apply_part.view(), apply_part.nslices, srcbuf, out, scaling);
+        launch_apply(s, apply_part.view(), apply_part.nslices, apply_part.row0(0), apply_part.nrows_of(0),
+                     srcbuf, out, scaling);
         if (!device_pointers)
             SFA_HIP(hipMemcpyAsync(dest, io, n * sizeof(double), hipMemcpyDeviceToHost, s));
         SFA_HIP(hipStreamSynchronize(s));
@@ -817,7 +824,8 @@ int sfa_MatrixCSR_apply(int rows, int columns, const int* A, const int* C, const
             ddst = sfa::dev_alloc<double>(rows);
             SFA_HIP(hipMemcpy(dsrc, src, columns * sizeof(double), hipMemcpyHostToDevice));
             SFA_HIP(hipMemset(dsrc + columns, 0, sizeof(double)));
-            sfa::launch_apply(nullptr, dp.view(), dp.nslices, dsrc, ddst, scaling);
+            sfa::launch_apply(nullptr, dp.view(), dp.nslices, dp.row0(0), dp.nrows_of(0), dsrc, ddst,
+                              scaling);
             SFA_HIP(hipGetLastError());
             SFA_HIP(hipMemcpy(dest, ddst, rows * sizeof(double), hipMemcpyDeviceToHost));
         } catch (...) {
