@@ -130,6 +130,10 @@ __device__ void after_zr_restart(SolveState* st, double zr)
 {
     st->zr = zr;
     st->hist[0] = zr;
+    if (st->rel2 > 0.) {  // inner solves: stop at a fixed reduction of z.r
+        st->tol2 = st->rel2 * ref_absolute(zr);
+        st->tol = sqrt(st->tol2);
+    }
     if (ref_absolute(zr) < st->tol2) {
         st->done = 1;
         st->exit_reason = SFA_EXIT_CONVERGED;
@@ -138,16 +142,16 @@ __device__ void after_zr_restart(SolveState* st, double zr)
 
 // acc (+|-)= sum_k v_k * vec[c_k] over one SELL row, entries in order.
 // Loads of a chunk are issued before its arithmetic so the gathers overlap.
-template <bool SUB>
-__device__ inline double sell_row(const SellView& A, int slice, int lane, const double* __restrict__ vec,
-                                  double acc)
+template <bool SUB, typename T>
+__device__ inline T sell_row(const SellViewT<T>& A, int slice, int lane, const T* __restrict__ vec,
+                             T acc)
 {
     const int beg = A.off[slice] + lane;
     const int end = A.off[slice + 1];
     constexpr int U = 8;
     int e = beg;
     for (; e + (U - 1) * kWave < end; e += U * kWave) {
-        double v[U], g[U];
+        T v[U], g[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
             const int c = __builtin_nontemporal_load(&A.cols[e + u * kWave]);
@@ -162,7 +166,7 @@ __device__ inline double sell_row(const SellView& A, int slice, int lane, const 
     }
     for (; e < end; e += kWave) {
         const int c = __builtin_nontemporal_load(&A.cols[e]);
-        const double v = __builtin_nontemporal_load(&A.vals[e]);
+        const T v = __builtin_nontemporal_load(&A.vals[e]);
         if (SUB) acc -= v * vec[c];
         else     acc += v * vec[c];
     }
@@ -171,7 +175,8 @@ __device__ inline double sell_row(const SellView& A, int slice, int lane, const 
 
 // Device row of a lane: computed when the group's rows are consecutive
 // (row0 >= 0), read from the slice's row list otherwise.
-__device__ inline int group_row(const SellView& A, int slice, int local, int lane, int row0,
+template <typename T>
+__device__ inline int group_row(const SellViewT<T>& A, int slice, int local, int lane, int row0,
                                 int nrows)
 {
     if (row0 >= 0) {
@@ -183,19 +188,22 @@ __device__ inline int group_row(const SellView& A, int slice, int local, int lan
 
 // ------------------------------------------------------------------ vectors
 
-__global__ void k_fill(double* v, double value, int n)
+template <typename T>
+__global__ void k_fill(T* v, T value, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) v[i] = value;
 }
 
-__global__ void k_gather(double* dst, const double* src, const int* perm, int n)
+template <typename T>
+__global__ void k_gather(T* dst, const T* src, const int* perm, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = src[perm[i]];
 }
 
-__global__ void k_scatter(double* dst, const double* src, const int* perm, int n)
+template <typename T>
+__global__ void k_scatter(T* dst, const T* src, const int* perm, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[perm[i]] = src[i];
@@ -203,15 +211,17 @@ __global__ void k_scatter(double* dst, const double* src, const int* perm, int n
 
 // State reset and tolerance (solver/sparsesolver.cpp:469-472):
 // tol = max(precision * sqrt(sum b^2), desired_precision).
-__global__ void __launch_bounds__(kVecBlock) k_init_state(SolveState* st, const double* b, int n,
+template <typename T>
+__global__ void __launch_bounds__(kVecBlock) k_init_state(SolveState* st, const T* b, int n,
                                                           double precision, double host_tol,
                                                           int use_host_tol, int maxK,
-                                                          int small_breaks, double* partials)
+                                                          int small_breaks, double rel2,
+                                                          double* partials)
 {
     double s = 0.;
     if (!use_host_tol)
         for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-            s += b[i] * b[i];
+            s += (double)b[i] * (double)b[i];
     double total;
     if (!grid_reduce_last<kVecBlock>(s, partials, &st->ticket, &total)) return;
     if (threadIdx.x != 0) return;
@@ -232,14 +242,17 @@ __global__ void __launch_bounds__(kVecBlock) k_init_state(SolveState* st, const 
     st->done = 0;
     st->exit_reason = SFA_EXIT_CAP;
     st->small_breaks = small_breaks;
+    st->rel2 = rel2;
+    st->result = 0.;
 }
 
 // ------------------------------------------------------------------ apply
 
 // MatrixCSR::apply (sparse/matcsr.cpp:216-244): dest = 0; dest += scaling*V*x.
-__global__ void __launch_bounds__(kSellBlock) k_apply(SellView A, int nslices, int row0, int nrows,
-                                                      const double* src, double* dest,
-                                                      double scaling)
+template <typename T>
+__global__ void __launch_bounds__(kSellBlock) k_apply(SellViewT<T> A, int nslices, int row0,
+                                                      int nrows, const T* src, T* dest,
+                                                      T scaling)
 {
     const int slice = blockIdx.x;
     const int lane = threadIdx.x;
@@ -247,7 +260,7 @@ __global__ void __launch_bounds__(kSellBlock) k_apply(SellView A, int nslices, i
     const int row = group_row(A, slice, slice, lane, row0, nrows);
     if (row < 0) return;
     const int beg = A.off[slice] + lane, end = A.off[slice + 1];
-    double acc = 0.;
+    T acc = 0;
     for (int e = beg; e < end; e += kWave) acc += scaling * A.vals[e] * src[A.cols[e]];
     dest[row] = acc;
 }
@@ -256,11 +269,12 @@ __global__ void __launch_bounds__(kSellBlock) k_apply(SellView A, int nslices, i
 
 // solver/sparsesolver.cpp:80-98 (plain), :281-306 (Jacobi), :512-524 (SSOR),
 // :797-812 (Eisenstat, columns already filtered into the part).
-__global__ void __launch_bounds__(kSellBlock) k_residual(int mode, SellView A, int nslices,
+template <typename T>
+__global__ void __launch_bounds__(kSellBlock) k_residual(int mode, SellViewT<T> A, int nslices,
                                                          int row0, int nrows,
-                                                         SolveState* st, const double* b,
-                                                         const double* x, double* out, double* d,
-                                                         double* z, const double* precon,
+                                                         SolveState* st, const T* b,
+                                                         const T* x, T* out, T* d,
+                                                         T* z, const T* precon,
                                                          double* partials)
 {
     const int slice = blockIdx.x;
@@ -268,13 +282,13 @@ __global__ void __launch_bounds__(kSellBlock) k_residual(int mode, SellView A, i
     double part = 0.;
     const int row = slice < nslices ? group_row(A, slice, slice, lane, row0, nrows) : -1;
     if (row >= 0) {
-        const double r = sell_row<true>(A, slice, lane, x, b[row]);
+        const T r = sell_row<true>(A, slice, lane, x, b[row]);
         out[row] = r;
         if (mode == RES_PLAIN) {
             d[row] = r;
             part = r * r;
         } else if (mode == RES_JACOBI) {
-            const double zz = precon[row] * r;
+            const T zz = precon[row] * r;
             z[row] = zz;
             d[row] = zz;
             part = zz * r;
@@ -287,15 +301,16 @@ __global__ void __launch_bounds__(kSellBlock) k_residual(int mode, SellView A, i
 }
 
 // solver/sparsesolver.cpp:570-583: d = z, z.r = sum z*r.
-__global__ void __launch_bounds__(kVecBlock) k_direction_init(SolveState* st, int n, const double* z,
-                                                              const double* r, double* d,
+template <typename T>
+__global__ void __launch_bounds__(kVecBlock) k_direction_init(SolveState* st, int n, const T* z,
+                                                              const T* r, T* d,
                                                               double* partials)
 {
     double part = 0.;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const double zz = z[i];
+        const T zz = z[i];
         d[i] = zz;
-        part += zz * r[i];
+        part += (double)(zz * r[i]);
     }
     double total;
     if (!grid_reduce_last<kVecBlock>(part, partials, &st->ticket, &total)) return;
@@ -303,18 +318,19 @@ __global__ void __launch_bounds__(kVecBlock) k_direction_init(SolveState* st, in
 }
 
 // solver/sparsesolver.cpp:830-844: z = r*diag*(2-w)/w, d = z, z.r.
-__global__ void __launch_bounds__(kVecBlock) k_eisenstat_init(SolveState* st, int n, const double* r,
-                                                              const double* diag, double omega,
-                                                              double* z, double* d,
+template <typename T>
+__global__ void __launch_bounds__(kVecBlock) k_eisenstat_init(SolveState* st, int n, const T* r,
+                                                              const T* diag, T omega,
+                                                              T* z, T* d,
                                                               double* partials)
 {
     double part = 0.;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const double rr = r[i];
-        const double zz = rr * diag[i] * (2. - omega) / omega;
+        const T rr = r[i];
+        const T zz = rr * diag[i] * (T(2) - omega) / omega;
         z[i] = zz;
         d[i] = zz;
-        part += zz * rr;
+        part += (double)(zz * rr);
     }
     double total;
     if (!grid_reduce_last<kVecBlock>(part, partials, &st->ticket, &total)) return;
@@ -325,12 +341,13 @@ __global__ void __launch_bounds__(kVecBlock) k_eisenstat_init(SolveState* st, in
 
 // Forward sweep of one colour / level, solver/sparsesolver.cpp:528-541 (restart)
 // and :647-665, :1234-1261 (iteration, with the x and r updates).
-__global__ void __launch_bounds__(kSellBlock) k_forward(SellView L, int slice0, int nslices,
+template <typename T>
+__global__ void __launch_bounds__(kSellBlock) k_forward(SellViewT<T> L, int slice0, int nslices,
                                                         int row0, int nrows,
                                                         const SolveState* st, int update_xr,
-                                                        double* x, double* r, const double* d,
-                                                        const double* q, double* m,
-                                                        const double* diag, double omega)
+                                                        T* x, T* r, const T* d,
+                                                        const T* q, T* m,
+                                                        const T* diag, T omega)
 {
     if (st->done) return;
     const int slice = slice0 + blockIdx.x;
@@ -338,29 +355,30 @@ __global__ void __launch_bounds__(kSellBlock) k_forward(SellView L, int slice0, 
     if (blockIdx.x >= nslices) return;
     const int row = group_row(L, slice, blockIdx.x, lane, row0, nrows);
     if (row < 0) return;
-    double rr;
+    T rr;
     if (update_xr) {
-        const double alpha = st->alpha;
+        const T alpha = (T)st->alpha;
         x[row] += alpha * d[row];
         rr = r[row] - alpha * q[row];
         r[row] = rr;
     } else {
         rr = r[row];
     }
-    const double acc = sell_row<true>(L, slice, lane, m, rr);
+    const T acc = sell_row<true>(L, slice, lane, m, rr);
     m[row] = acc * omega / diag[row];
 }
 
 // Backward sweep of one colour / level, solver/sparsesolver.cpp:543-568
 // (restart) and :667-692, :1263-1303 (iteration): the scaling of m fused in,
 // z.r summed per group and over groups in launch order.
-__global__ void __launch_bounds__(kSellBlock) k_backward(SellView U, int slice0, int nslices,
+template <typename T>
+__global__ void __launch_bounds__(kSellBlock) k_backward(SellViewT<T> U, int slice0, int nslices,
                                                          int row0, int nrows,
                                                          SolveState* st, int reduce, int group,
-                                                         int last_group, const double* r,
-                                                         const double* m, double* z,
-                                                         const double* diag, double omega,
-                                                         double c2, double* grp, double* partials)
+                                                         int last_group, const T* r,
+                                                         const T* m, T* z,
+                                                         const T* diag, T omega,
+                                                         T c2, double* grp, double* partials)
 {
     if (st->done) return;
     const int slice = slice0 + blockIdx.x;
@@ -368,14 +386,14 @@ __global__ void __launch_bounds__(kSellBlock) k_backward(SellView U, int slice0,
     double part = 0.;
     const int row = blockIdx.x < nslices ? group_row(U, slice, blockIdx.x, lane, row0, nrows) : -1;
     if (row >= 0) {
-        const double dg = diag[row];
-        double t = m[row];
+        const T dg = diag[row];
+        T t = m[row];
         t *= dg;
         t *= c2;
-        const double acc = sell_row<true>(U, slice, lane, z, t);
-        const double zz = acc * omega / dg;
+        const T acc = sell_row<true>(U, slice, lane, z, t);
+        const T zz = acc * omega / dg;
         z[row] = zz;
-        part = zz * r[row];
+        part = (double)(zz * r[row]);
     }
     if (!reduce) return;
     double total;
@@ -390,23 +408,24 @@ __global__ void __launch_bounds__(kSellBlock) k_backward(SellView U, int slice0,
 
 // Eisenstat triangular solves (solver/sparsesolver.cpp:814-828, 876-908,
 // 1429-1455, 1501-1560): out = (in - [c3*diag*t] - T out) * w / diag.
-__global__ void __launch_bounds__(kSellBlock) k_eisenstat_solve(SellView T, int slice0, int nslices,
-                                                                int row0, int nrows,
+template <typename T>
+__global__ void __launch_bounds__(kSellBlock) k_eisenstat_solve(SellViewT<T> P, int slice0,
+                                                                int nslices, int row0, int nrows,
                                                                 const SolveState* st, int with_t,
-                                                                const double* in, const double* t,
-                                                                double* out, const double* diag,
-                                                                double omega, double c3)
+                                                                const T* in, const T* t,
+                                                                T* out, const T* diag,
+                                                                T omega, T c3)
 {
     if (st->done) return;
     const int slice = slice0 + blockIdx.x;
     const int lane = threadIdx.x;
     if (blockIdx.x >= nslices) return;
-    const int row = group_row(T, slice, blockIdx.x, lane, row0, nrows);
+    const int row = group_row(P, slice, blockIdx.x, lane, row0, nrows);
     if (row < 0) return;
-    const double dg = diag[row];
-    double a = in[row];
+    const T dg = diag[row];
+    T a = in[row];
     if (with_t) a = a - c3 * dg * t[row];
-    const double acc = sell_row<true>(T, slice, lane, out, a);
+    const T acc = sell_row<true>(P, slice, lane, out, a);
     out[row] = acc * omega / dg;
 }
 
@@ -414,9 +433,10 @@ __global__ void __launch_bounds__(kSellBlock) k_eisenstat_solve(SellView T, int 
 
 // q = A d, d.Ad (solver/sparsesolver.cpp:611-626, 1197-1212), then the
 // breakdown tests and alpha.
-__global__ void __launch_bounds__(kSellBlock) k_spmv_dot(SellView A, int nslices, int row0,
+template <typename T>
+__global__ void __launch_bounds__(kSellBlock) k_spmv_dot(SellViewT<T> A, int nslices, int row0,
                                                          int nrows, SolveState* st,
-                                                         const double* d, double* q,
+                                                         const T* d, T* q,
                                                          double* partials)
 {
     if (st->done) return;
@@ -425,9 +445,9 @@ __global__ void __launch_bounds__(kSellBlock) k_spmv_dot(SellView A, int nslices
     double part = 0.;
     const int row = slice < nslices ? group_row(A, slice, slice, lane, row0, nrows) : -1;
     if (row >= 0) {
-        const double acc = sell_row<false>(A, slice, lane, d, 0.);
+        const T acc = sell_row<false>(A, slice, lane, d, T(0));
         q[row] = acc;
-        part = d[row] * acc;
+        part = (double)(d[row] * acc);
     }
     double total;
     if (!grid_reduce_last<kSellBlock>(part, partials, &st->ticket, &total)) return;
@@ -436,29 +456,30 @@ __global__ void __launch_bounds__(kSellBlock) k_spmv_dot(SellView A, int nslices
 
 // x += a dx; r -= a q; then z.r by mode (solver/sparsesolver.cpp:160-174,
 // 372-389, 952-968); beta and the loop-head tests.
-__global__ void __launch_bounds__(kVecBlock) k_update(int mode, SolveState* st, int n, double* x,
-                                                      double* r, const double* dx,
-                                                      const double* q, double* z,
-                                                      const double* dp, double omega,
+template <typename T>
+__global__ void __launch_bounds__(kVecBlock) k_update(int mode, SolveState* st, int n, T* x,
+                                                      T* r, const T* dx,
+                                                      const T* q, T* z,
+                                                      const T* dp, T omega,
                                                       double* partials)
 {
     if (st->done) return;
-    const double alpha = st->alpha;
+    const T alpha = (T)st->alpha;
     double part = 0.;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         x[i] += alpha * dx[i];
-        const double rr = r[i] - alpha * q[i];
+        const T rr = r[i] - alpha * q[i];
         r[i] = rr;
         if (mode == UPD_PLAIN) {
-            part += rr * rr;
+            part += (double)(rr * rr);
         } else if (mode == UPD_JACOBI) {
-            const double zz = dp[i] * rr;
+            const T zz = dp[i] * rr;
             z[i] = zz;
-            part += zz * rr;
+            part += (double)(zz * rr);
         } else {
-            const double zz = rr * (dp[i] * (2. - omega) / omega);
+            const T zz = rr * (dp[i] * (T(2) - omega) / omega);
             z[i] = zz;
-            part += zz * rr;
+            part += (double)(zz * rr);
         }
     }
     double total;
@@ -467,16 +488,17 @@ __global__ void __launch_bounds__(kVecBlock) k_update(int mode, SolveState* st, 
 }
 
 // solver/sparsesolver.cpp:912-922: q += t, d.Ad = sum q*d; breakdown tests.
-__global__ void __launch_bounds__(kVecBlock) k_eisenstat_combine(SolveState* st, int n, double* q,
-                                                                 const double* t, const double* d,
+template <typename T>
+__global__ void __launch_bounds__(kVecBlock) k_eisenstat_combine(SolveState* st, int n, T* q,
+                                                                 const T* t, const T* d,
                                                                  double* partials)
 {
     if (st->done) return;
     double part = 0.;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        const double qq = q[i] + t[i];
+        const T qq = q[i] + t[i];
         q[i] = qq;
-        part += qq * d[i];
+        part += (double)(qq * d[i]);
     }
     double total;
     if (!grid_reduce_last<kVecBlock>(part, partials, &st->ticket, &total)) return;
@@ -484,13 +506,51 @@ __global__ void __launch_bounds__(kVecBlock) k_eisenstat_combine(SolveState* st,
 }
 
 // d = src + beta*d (solver/sparsesolver.cpp:698-706 and its siblings).
+template <typename T>
 __global__ void __launch_bounds__(kVecBlock) k_direction(const SolveState* st, int n,
-                                                         const double* src, double* d)
+                                                         const T* src, T* d)
 {
     if (st->done) return;
-    const double beta = st->beta;
+    const T beta = (T)st->beta;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) d[i] = src[i] + beta * d[i];
+}
+
+// ------------------------------------------------------------------ mixed precision
+
+// out = b - A x in the part's rows, sum of squares into st->result.
+template <typename T>
+__global__ void __launch_bounds__(kSellBlock) k_residual_norm(SellViewT<T> A, int nslices, int row0,
+                                                              int nrows, SolveState* st,
+                                                              const T* b, const T* x, T* out,
+                                                              double* partials)
+{
+    const int slice = blockIdx.x;
+    const int lane = threadIdx.x;
+    double part = 0.;
+    const int row = slice < nslices ? group_row(A, slice, slice, lane, row0, nrows) : -1;
+    if (row >= 0) {
+        const T r = sell_row<true>(A, slice, lane, x, b[row]);
+        out[row] = r;
+        part = (double)(r * r);
+    }
+    double total;
+    if (!grid_reduce_last<kSellBlock>(part, partials, &st->ticket, &total)) return;
+    if (threadIdx.x == 0) st->result = total;
+}
+
+template <typename TD, typename TS>
+__global__ void k_convert(TD* dst, const TS* src, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (TD)src[i];
+}
+
+// x += (double)e over the first n rows
+__global__ void k_add_correction(double* x, const float* e, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] += (double)e[i];
 }
 
 int vec_blocks(int n)
@@ -508,117 +568,186 @@ int flat_blocks(int n)
 
 }  // namespace
 
-void launch_fill(hipStream_t s, double* v, double value, int n)
+template <typename T>
+void launch_fill(hipStream_t s, T* v, T value, int n)
 {
-    if (n > 0) k_fill<<<flat_blocks(n), kVecBlock, 0, s>>>(v, value, n);
+    if (n > 0) k_fill<T><<<flat_blocks(n), kVecBlock, 0, s>>>(v, value, n);
 }
 
-void launch_gather(hipStream_t s, double* dst, const double* src, const int* perm, int n)
+template <typename T>
+void launch_gather(hipStream_t s, T* dst, const T* src, const int* perm, int n)
 {
-    if (n > 0) k_gather<<<flat_blocks(n), kVecBlock, 0, s>>>(dst, src, perm, n);
+    if (n > 0) k_gather<T><<<flat_blocks(n), kVecBlock, 0, s>>>(dst, src, perm, n);
 }
 
-void launch_scatter(hipStream_t s, double* dst, const double* src, const int* perm, int n)
+template <typename T>
+void launch_scatter(hipStream_t s, T* dst, const T* src, const int* perm, int n)
 {
-    if (n > 0) k_scatter<<<flat_blocks(n), kVecBlock, 0, s>>>(dst, src, perm, n);
+    if (n > 0) k_scatter<T><<<flat_blocks(n), kVecBlock, 0, s>>>(dst, src, perm, n);
 }
 
-void launch_init_state(hipStream_t s, SolveState* st, const double* b, int n, double precision,
-                       double host_tol, int use_host_tol, int maxK, int small_breaks,
+template <typename T>
+void launch_init_state(hipStream_t s, SolveState* st, const T* b, int n, double precision,
+                       double host_tol, int use_host_tol, int maxK, int small_breaks, double rel2,
                        double* partials)
 {
-    k_init_state<<<use_host_tol ? 1 : vec_blocks(n), kVecBlock, 0, s>>>(
-        st, b, n, precision, host_tol, use_host_tol, maxK, small_breaks, partials);
+    k_init_state<T><<<use_host_tol ? 1 : vec_blocks(n), kVecBlock, 0, s>>>(
+        st, b, n, precision, host_tol, use_host_tol, maxK, small_breaks, rel2, partials);
 }
 
-void launch_apply(hipStream_t s, SellView A, int nslices, int row0, int nrows, const double* src,
-                  double* dest, double scaling)
+template <typename T>
+void launch_apply(hipStream_t s, SellViewT<T> A, int nslices, int row0, int nrows, const T* src,
+                  T* dest, T scaling)
 {
     if (nslices > 0)
-        k_apply<<<nslices, kSellBlock, 0, s>>>(A, nslices, row0, nrows, src, dest, scaling);
+        k_apply<T><<<nslices, kSellBlock, 0, s>>>(A, nslices, row0, nrows, src, dest, scaling);
 }
 
-void launch_residual(hipStream_t s, ResidualMode mode, SellView A, int nslices, int row0,
-                     int nrows, SolveState* st, const double* b, const double* x, double* out,
-                     double* d, double* z, const double* precon, double* partials)
+template <typename T>
+void launch_residual(hipStream_t s, ResidualMode mode, SellViewT<T> A, int nslices, int row0,
+                     int nrows, SolveState* st, const T* b, const T* x, T* out, T* d, T* z,
+                     const T* precon, double* partials)
 {
     const int grid = nslices > 0 ? nslices : 1;
-    k_residual<<<grid, kSellBlock, 0, s>>>((int)mode, A, nslices, row0, nrows, st, b, x, out, d,
-                                           z, precon, partials);
+    k_residual<T><<<grid, kSellBlock, 0, s>>>((int)mode, A, nslices, row0, nrows, st, b, x, out,
+                                              d, z, precon, partials);
 }
 
-void launch_direction_init(hipStream_t s, SolveState* st, int n_active, const double* z,
-                           const double* r, double* d, double* partials)
+template <typename T>
+void launch_residual_norm(hipStream_t s, SellViewT<T> A, int nslices, int row0, int nrows,
+                          SolveState* st, const T* b, const T* x, T* out, double* partials)
 {
-    k_direction_init<<<vec_blocks(n_active), kVecBlock, 0, s>>>(st, n_active, z, r, d, partials);
+    const int grid = nslices > 0 ? nslices : 1;
+    k_residual_norm<T><<<grid, kSellBlock, 0, s>>>(A, nslices, row0, nrows, st, b, x, out,
+                                                   partials);
 }
 
-void launch_eisenstat_init(hipStream_t s, SolveState* st, int n_active, const double* r,
-                           const double* diag, double omega, double* z, double* d,
-                           double* partials)
+template <typename T>
+void launch_direction_init(hipStream_t s, SolveState* st, int n_active, const T* z, const T* r,
+                           T* d, double* partials)
 {
-    k_eisenstat_init<<<vec_blocks(n_active), kVecBlock, 0, s>>>(st, n_active, r, diag, omega, z, d,
-                                                                partials);
+    k_direction_init<T><<<vec_blocks(n_active), kVecBlock, 0, s>>>(st, n_active, z, r, d,
+                                                                   partials);
 }
 
-void launch_forward(hipStream_t s, SellView L, int slice0, int nslices, int row0, int nrows,
-                    SolveState* st, int update_xr, double* x, double* r, const double* d,
-                    const double* q, double* m, const double* diag, double omega)
+template <typename T>
+void launch_eisenstat_init(hipStream_t s, SolveState* st, int n_active, const T* r,
+                           const T* diag, T omega, T* z, T* d, double* partials)
+{
+    k_eisenstat_init<T><<<vec_blocks(n_active), kVecBlock, 0, s>>>(st, n_active, r, diag, omega,
+                                                                   z, d, partials);
+}
+
+template <typename T>
+void launch_forward(hipStream_t s, SellViewT<T> L, int slice0, int nslices, int row0, int nrows,
+                    SolveState* st, int update_xr, T* x, T* r, const T* d, const T* q, T* m,
+                    const T* diag, T omega)
 {
     if (nslices <= 0) return;
-    k_forward<<<nslices, kSellBlock, 0, s>>>(L, slice0, nslices, row0, nrows, st, update_xr, x, r,
-                                             d, q, m, diag, omega);
+    k_forward<T><<<nslices, kSellBlock, 0, s>>>(L, slice0, nslices, row0, nrows, st, update_xr, x,
+                                                r, d, q, m, diag, omega);
 }
 
-void launch_backward(hipStream_t s, SellView U, int slice0, int nslices, int row0, int nrows,
-                     SolveState* st, int reduce, int group, int last_group, const double* r,
-                     const double* m, double* z, const double* diag, double omega, double c2,
-                     double* grp, double* partials)
+template <typename T>
+void launch_backward(hipStream_t s, SellViewT<T> U, int slice0, int nslices, int row0, int nrows,
+                     SolveState* st, int reduce, int group, int last_group, const T* r,
+                     const T* m, T* z, const T* diag, T omega, T c2, double* grp,
+                     double* partials)
 {
     // A group with no rows still runs one block when it has to publish its
     // (zero) partial or finish the z.r sum.
     const int grid = nslices > 0 ? nslices : (reduce ? 1 : 0);
     if (grid == 0) return;
-    k_backward<<<grid, kSellBlock, 0, s>>>(U, slice0, nslices, row0, nrows, st, reduce, group,
-                                           last_group, r, m, z, diag, omega, c2, grp, partials);
+    k_backward<T><<<grid, kSellBlock, 0, s>>>(U, slice0, nslices, row0, nrows, st, reduce, group,
+                                              last_group, r, m, z, diag, omega, c2, grp,
+                                              partials);
 }
 
-void launch_eisenstat_solve(hipStream_t s, SellView T, int slice0, int nslices, int row0,
-                            int nrows, SolveState* st, int with_t, const double* in,
-                            const double* t, double* out, const double* diag, double omega,
-                            double c3)
+template <typename T>
+void launch_eisenstat_solve(hipStream_t s, SellViewT<T> P, int slice0, int nslices, int row0,
+                            int nrows, SolveState* st, int with_t, const T* in, const T* t,
+                            T* out, const T* diag, T omega, T c3)
 {
     if (nslices <= 0) return;
-    k_eisenstat_solve<<<nslices, kSellBlock, 0, s>>>(T, slice0, nslices, row0, nrows, st, with_t,
-                                                     in, t, out, diag, omega, c3);
+    k_eisenstat_solve<T><<<nslices, kSellBlock, 0, s>>>(P, slice0, nslices, row0, nrows, st,
+                                                        with_t, in, t, out, diag, omega, c3);
 }
 
-void launch_spmv_dot(hipStream_t s, SellView A, int nslices, int row0, int nrows, SolveState* st,
-                     const double* d, double* q, double* partials)
+template <typename T>
+void launch_spmv_dot(hipStream_t s, SellViewT<T> A, int nslices, int row0, int nrows,
+                     SolveState* st, const T* d, T* q, double* partials)
 {
     const int grid = nslices > 0 ? nslices : 1;
-    k_spmv_dot<<<grid, kSellBlock, 0, s>>>(A, nslices, row0, nrows, st, d, q, partials);
+    k_spmv_dot<T><<<grid, kSellBlock, 0, s>>>(A, nslices, row0, nrows, st, d, q, partials);
 }
 
-void launch_update(hipStream_t s, UpdateMode mode, SolveState* st, int n_active, double* x,
-                   double* r, const double* dx, const double* q, double* z,
-                   const double* diag_or_precon, double omega, double* partials)
+template <typename T>
+void launch_update(hipStream_t s, UpdateMode mode, SolveState* st, int n_active, T* x, T* r,
+                   const T* dx, const T* q, T* z, const T* diag_or_precon, T omega,
+                   double* partials)
 {
-    k_update<<<vec_blocks(n_active), kVecBlock, 0, s>>>((int)mode, st, n_active, x, r, dx, q, z,
-                                                        diag_or_precon, omega, partials);
+    k_update<T><<<vec_blocks(n_active), kVecBlock, 0, s>>>((int)mode, st, n_active, x, r, dx, q,
+                                                           z, diag_or_precon, omega, partials);
 }
 
-void launch_eisenstat_combine(hipStream_t s, SolveState* st, int n_active, double* q,
-                              const double* t, const double* d, double* partials)
+template <typename T>
+void launch_eisenstat_combine(hipStream_t s, SolveState* st, int n_active, T* q, const T* t,
+                              const T* d, double* partials)
 {
-    k_eisenstat_combine<<<vec_blocks(n_active), kVecBlock, 0, s>>>(st, n_active, q, t, d,
-                                                                   partials);
+    k_eisenstat_combine<T><<<vec_blocks(n_active), kVecBlock, 0, s>>>(st, n_active, q, t, d,
+                                                                      partials);
 }
 
-void launch_direction(hipStream_t s, SolveState* st, int n_active, const double* src, double* d)
+template <typename T>
+void launch_direction(hipStream_t s, SolveState* st, int n_active, const T* src, T* d)
 {
     if (n_active > 0)
-        k_direction<<<flat_blocks(n_active), kVecBlock, 0, s>>>(st, n_active, src, d);
+        k_direction<T><<<flat_blocks(n_active), kVecBlock, 0, s>>>(st, n_active, src, d);
 }
+
+void launch_to_float(hipStream_t s, float* dst, const double* src, int n)
+{
+    if (n > 0) k_convert<float, double><<<flat_blocks(n), kVecBlock, 0, s>>>(dst, src, n);
+}
+
+void launch_add_correction(hipStream_t s, double* x, const float* e, int n)
+{
+    if (n > 0) k_add_correction<<<flat_blocks(n), kVecBlock, 0, s>>>(x, e, n);
+}
+
+#define SFA_INSTANTIATE(T)                                                                     \
+    template void launch_fill<T>(hipStream_t, T*, T, int);                                     \
+    template void launch_gather<T>(hipStream_t, T*, const T*, const int*, int);                \
+    template void launch_scatter<T>(hipStream_t, T*, const T*, const int*, int);               \
+    template void launch_init_state<T>(hipStream_t, SolveState*, const T*, int, double, double, \
+                                       int, int, int, double, double*);                        \
+    template void launch_apply<T>(hipStream_t, SellViewT<T>, int, int, int, const T*, T*, T);  \
+    template void launch_residual<T>(hipStream_t, ResidualMode, SellViewT<T>, int, int, int,   \
+                                     SolveState*, const T*, const T*, T*, T*, T*, const T*,    \
+                                     double*);                                                 \
+    template void launch_residual_norm<T>(hipStream_t, SellViewT<T>, int, int, int,            \
+                                          SolveState*, const T*, const T*, T*, double*);       \
+    template void launch_direction_init<T>(hipStream_t, SolveState*, int, const T*, const T*,  \
+                                           T*, double*);                                       \
+    template void launch_eisenstat_init<T>(hipStream_t, SolveState*, int, const T*, const T*,  \
+                                           T, T*, T*, double*);                                \
+    template void launch_forward<T>(hipStream_t, SellViewT<T>, int, int, int, int, SolveState*, \
+                                    int, T*, T*, const T*, const T*, T*, const T*, T);         \
+    template void launch_backward<T>(hipStream_t, SellViewT<T>, int, int, int, int,            \
+                                     SolveState*, int, int, int, const T*, const T*, T*,       \
+                                     const T*, T, T, double*, double*);                        \
+    template void launch_eisenstat_solve<T>(hipStream_t, SellViewT<T>, int, int, int, int,     \
+                                            SolveState*, int, const T*, const T*, T*,          \
+                                            const T*, T, T);                                   \
+    template void launch_spmv_dot<T>(hipStream_t, SellViewT<T>, int, int, int, SolveState*,    \
+                                     const T*, T*, double*);                                   \
+    template void launch_update<T>(hipStream_t, UpdateMode, SolveState*, int, T*, T*,          \
+                                   const T*, const T*, T*, const T*, T, double*);              \
+    template void launch_eisenstat_combine<T>(hipStream_t, SolveState*, int, T*, const T*,     \
+                                              const T*, double*);                              \
+    template void launch_direction<T>(hipStream_t, SolveState*, int, const T*, T*);
+
+SFA_INSTANTIATE(double)
+SFA_INSTANTIATE(float)
 
 }  // namespace sfa

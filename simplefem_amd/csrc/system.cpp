@@ -61,11 +61,12 @@ void ensure_device()
         throw Error{SFA_ERR_NO_DEVICE, "no HIP device available"};
 }
 
-struct DevPart {
+template <typename T>
+struct DevPartT {
     int* off = nullptr;
     int* rows = nullptr;
     int* cols = nullptr;
-    double* vals = nullptr;
+    T* vals = nullptr;
     int nslices = 0;
     std::vector<int> group_slice;
     std::vector<long long> group_entries, group_rows;
@@ -87,13 +88,14 @@ struct DevPart {
         off = dev_alloc<int>(p.slice_off.size());
         rows = dev_alloc<int>(p.slice_rows.size());
         cols = dev_alloc<int>(p.cols.size());
-        vals = dev_alloc<double>(p.vals.size());
+        vals = dev_alloc<T>(p.vals.size());
         SFA_HIP(hipMemcpy(off, p.slice_off.data(), p.slice_off.size() * sizeof(int), hipMemcpyHostToDevice));
         if (!p.slice_rows.empty())
             SFA_HIP(hipMemcpy(rows, p.slice_rows.data(), p.slice_rows.size() * sizeof(int), hipMemcpyHostToDevice));
         if (!p.cols.empty()) {
             SFA_HIP(hipMemcpy(cols, p.cols.data(), p.cols.size() * sizeof(int), hipMemcpyHostToDevice));
-            SFA_HIP(hipMemcpy(vals, p.vals.data(), p.vals.size() * sizeof(double), hipMemcpyHostToDevice));
+            std::vector<T> tv(p.vals.begin(), p.vals.end());
+            SFA_HIP(hipMemcpy(vals, tv.data(), tv.size() * sizeof(T), hipMemcpyHostToDevice));
         }
     }
     void release()
@@ -105,23 +107,25 @@ struct DevPart {
         off = rows = cols = nullptr;
         vals = nullptr;
     }
-    SellView view() const { return SellView{off, rows, cols, vals}; }
+    SellViewT<T> view() const { return SellViewT<T>{off, rows, cols, vals}; }
     int ngroups() const { return (int)group_slice.size() - 1; }
     int group_begin(int g) const { return group_slice[g]; }
     int group_count(int g) const { return group_slice[g + 1] - group_slice[g]; }
     int row0(int g) const { return group_row0[g]; }
     int nrows_of(int g) const { return (int)group_rows[g]; }
-    // Algorithmic bytes of one launch over group g: 12 B per entry (8 value +
-    // 4 column), 4 B per row id, `per_row` B of vector traffic per row.
+    // Algorithmic bytes of one launch over group g: (sizeof(T) + 4) B per
+    // entry (value + column), 4 B per row id, `per_row` B of vector traffic
+    // per row (callers give it for T = double and it scales with sizeof(T)).
     double bytes(int g, double per_row) const
     {
-        return 12.0 * group_entries[g] + (4.0 + per_row) * group_rows[g];
+        return (sizeof(T) + 4.0) * group_entries[g] + (4.0 + per_row * sizeof(T) / 8.0) * group_rows[g];
     }
     double bytes_all(double per_row) const
     {
-        return 12.0 * entries + (4.0 + per_row) * nrows;
+        return (sizeof(T) + 4.0) * entries + (4.0 + per_row * sizeof(T) / 8.0) * nrows;
     }
 };
+using DevPart = DevPartT<double>;
 
 // Per-launch HIP events of a profiled solve.
 struct Profiler {
@@ -157,7 +161,8 @@ const char* method_title(int method)
 
 }  // namespace
 
-struct System {
+template <typename T>
+struct SystemT {
     int device = 0;
     int method = 0;
     int n = 0;
@@ -168,20 +173,20 @@ struct System {
     bool sweeps = false;
     bool rainbow = false;
 
-    DevPart full, lower, upper, lower_restart;
+    DevPartT<T> full, lower, upper, lower_restart;
     bool has_lower_restart = false;
-    DevPart apply_part;  // all rows, identity order (sfa_system_apply), built lazily
+    DevPartT<T> apply_part;  // all rows, identity order (sfa_system_apply), built lazily
     bool has_apply_part = false;
     std::vector<int> host_A, host_C;  // kept only to build apply_part later
     std::vector<double> host_V;
 
     int* perm = nullptr;
-    double* diag = nullptr;
-    double *x = nullptr, *r = nullptr, *b = nullptr, *d = nullptr, *z = nullptr, *q = nullptr,
-           *m = nullptr;
-    double* io = nullptr;  // n staging entries for host <-> device copies
-    double* io2 = nullptr;
-    double* io3 = nullptr;
+    T* diag = nullptr;
+    T *x = nullptr, *r = nullptr, *b = nullptr, *d = nullptr, *z = nullptr, *q = nullptr,
+      *m = nullptr;
+    T* io = nullptr;  // n staging entries for host <-> device copies
+    T* io2 = nullptr;
+    T* io3 = nullptr;
     SolveState* state = nullptr;
     double* hist = nullptr;
     double* partials = nullptr;
@@ -210,7 +215,7 @@ struct System {
         prof->bytes.push_back(bytes);
     }
 
-    ~System() { release(); }
+    ~SystemT() { release(); }
 
     void release()
     {
@@ -226,7 +231,8 @@ struct System {
                         (void*)hist, (void*)partials, (void*)grp})
             dev_free(p);
         perm = nullptr;
-        diag = x = r = b = d = z = q = m = io = io2 = io3 = hist = partials = grp = nullptr;
+        diag = x = r = b = d = z = q = m = io = io2 = io3 = nullptr;
+        hist = partials = grp = nullptr;
         state = nullptr;
         if (host_state) (void)hipHostFree(host_state);
         host_state = nullptr;
@@ -243,6 +249,13 @@ struct System {
         std::string err = build_plan(plan, method_, n_, A, C, V, aux, omega_, nc, F, B, R,
                                      &colouring_error);
         if (!err.empty()) throw Error{colouring_error ? SFA_ERR_COLOURING : SFA_ERR_INVALID, err};
+        create_from_plan(plan, A, C, V);
+    }
+
+    void create_from_plan(const Plan& plan, const int* A, const int* C, const double* V)
+    {
+        const int method_ = plan.method, n_ = plan.n;
+        const double omega_ = plan.omega;
         ensure_device();
         SFA_HIP(hipGetDevice(&device));
         method = method_;
@@ -270,19 +283,19 @@ struct System {
 
         perm = dev_alloc<int>(n);
         SFA_HIP(hipMemcpy(perm, plan.perm.data(), n * sizeof(int), hipMemcpyHostToDevice));
-        diag = dev_alloc<double>(n + 1);
-        std::vector<double> dg(plan.diag);
-        dg.push_back(0.0);
-        SFA_HIP(hipMemcpy(diag, dg.data(), (n + 1) * sizeof(double), hipMemcpyHostToDevice));
+        diag = dev_alloc<T>(n + 1);
+        std::vector<T> dg(plan.diag.begin(), plan.diag.end());
+        dg.push_back(T(0));
+        SFA_HIP(hipMemcpy(diag, dg.data(), (n + 1) * sizeof(T), hipMemcpyHostToDevice));
 
-        double** vecs[] = {&x, &r, &b, &d, &z, &q, &m};
-        for (double** v : vecs) {
-            *v = dev_alloc<double>(n + 1);
-            SFA_HIP(hipMemset(*v, 0, (n + 1) * sizeof(double)));
+        T** vecs[] = {&x, &r, &b, &d, &z, &q, &m};
+        for (T** v : vecs) {
+            *v = dev_alloc<T>(n + 1);
+            SFA_HIP(hipMemset(*v, 0, (n + 1) * sizeof(T)));
         }
-        io = dev_alloc<double>(n);
-        io2 = dev_alloc<double>(n);
-        io3 = dev_alloc<double>(n);
+        io = dev_alloc<T>(n);
+        io2 = dev_alloc<T>(n);
+        io3 = dev_alloc<T>(n);
         state = dev_alloc<SolveState>(1);
         hist = dev_alloc<double>((size_t)n + 1);
         int max_blocks = 4096;
@@ -301,10 +314,11 @@ struct System {
 
     // ---------------------------------------------------------------- phases
 
-    double c2() const { return (2. - omega) / omega; }
-    double c3() const { return 2. / omega - 1.; }
+    T c2() const { return T((2. - omega) / omega); }
+    T c3() const { return T(2. / omega - 1.); }
+    T w() const { return T(omega); }
 
-    void forward_sweep(hipStream_t s, const DevPart& L, int update_xr)
+    void forward_sweep(hipStream_t s, const DevPartT<T>& L, int update_xr)
     {
         // vector streams per row: x r/w, d, r r/w, q, diag, m (iteration);
         // r, diag, m (restart)
@@ -312,7 +326,7 @@ struct System {
         for (int g = 0; g < L.ngroups(); g++)
             run(s, SFA_KERNEL_FORWARD, L.bytes(g, per_row), [&] {
                 launch_forward(s, L.view(), L.group_begin(g), L.group_count(g), L.row0(g),
-                               L.nrows_of(g), state, update_xr, x, r, d, q, m, diag, omega);
+                               L.nrows_of(g), state, update_xr, x, r, d, q, m, diag, w());
             });
     }
 
@@ -323,18 +337,18 @@ struct System {
             run(s, SFA_KERNEL_BACKWARD, upper.bytes(g, 32.0), [&] {
                 launch_backward(s, upper.view(), upper.group_begin(g), upper.group_count(g),
                                 upper.row0(g), upper.nrows_of(g), state, reduce, g, g == G - 1, r,
-                                m, z, diag, omega, c2(), grp, partials);
+                                m, z, diag, w(), c2(), grp, partials);
             });
     }
 
-    void eisenstat_sweep(hipStream_t s, const DevPart& T, int with_t, const double* in,
-                         const double* t, double* out)
+    void eisenstat_sweep(hipStream_t s, const DevPartT<T>& P, int with_t, const T* in,
+                         const T* t, T* out)
     {
         const double per_row = with_t ? 32.0 : 24.0;  // in, diag, out [, t]
-        for (int g = 0; g < T.ngroups(); g++)
-            run(s, SFA_KERNEL_TRIANGULAR, T.bytes(g, per_row), [&] {
-                launch_eisenstat_solve(s, T.view(), T.group_begin(g), T.group_count(g), T.row0(g),
-                                       T.nrows_of(g), state, with_t, in, t, out, diag, omega,
+        for (int g = 0; g < P.ngroups(); g++)
+            run(s, SFA_KERNEL_TRIANGULAR, P.bytes(g, per_row), [&] {
+                launch_eisenstat_solve(s, P.view(), P.group_begin(g), P.group_count(g), P.row0(g),
+                                       P.nrows_of(g), state, with_t, in, t, out, diag, w(),
                                        c3());
             });
     }
@@ -342,13 +356,13 @@ struct System {
     // Restart (K = 0) of each variant.
     void restart(hipStream_t s)
     {
-        const SellView F = full.view();
+        const SellViewT<T> F = full.view();
         const double na = n_active;
         switch (method) {
         case SFA_METHOD_CG:
             run(s, SFA_KERNEL_RESTART, full.bytes_all(32.0), [&] {
-                launch_residual(s, RES_PLAIN, F, full.nslices, full.row0(0), full.nrows_of(0), state, b, x, r, d, nullptr,
-                                nullptr, partials);
+                launch_residual(s, RES_PLAIN, F, full.nslices, full.row0(0), full.nrows_of(0), state, b, x, r, d, (T*)nullptr,
+                                (const T*)nullptr, partials);
             });
             break;
         case SFA_METHOD_DIAGONAL:
@@ -360,8 +374,8 @@ struct System {
         case SFA_METHOD_SSOR:
         case SFA_METHOD_RAINBOW:
             run(s, SFA_KERNEL_RESTART, full.bytes_all(16.0), [&] {
-                launch_residual(s, RES_ONLY, F, full.nslices, full.row0(0), full.nrows_of(0), state, b, x, r, nullptr, nullptr,
-                                nullptr, partials);
+                launch_residual(s, RES_ONLY, F, full.nslices, full.row0(0), full.nrows_of(0), state, b, x, r, (T*)nullptr, (T*)nullptr,
+                                (const T*)nullptr, partials);
             });
             forward_sweep(s, lower, 0);
             backward_sweep(s, 0);
@@ -370,12 +384,12 @@ struct System {
             break;
         default:  // Eisenstat variants: q = b - A x, then r = lower solve of q
             run(s, SFA_KERNEL_RESTART, full.bytes_all(16.0), [&] {
-                launch_residual(s, RES_ONLY, F, full.nslices, full.row0(0), full.nrows_of(0), state, b, x, q, nullptr, nullptr,
-                                nullptr, partials);
+                launch_residual(s, RES_ONLY, F, full.nslices, full.row0(0), full.nrows_of(0), state, b, x, q, (T*)nullptr, (T*)nullptr,
+                                (const T*)nullptr, partials);
             });
-            eisenstat_sweep(s, has_lower_restart ? lower_restart : lower, 0, q, nullptr, r);
+            eisenstat_sweep(s, has_lower_restart ? lower_restart : lower, 0, q, (const T*)nullptr, r);
             run(s, SFA_KERNEL_RESTART, 32.0 * na, [&] {
-                launch_eisenstat_init(s, state, n_active, r, diag, omega, z, d, partials);
+                launch_eisenstat_init(s, state, n_active, r, diag, w(), z, d, partials);
             });
             break;
         }
@@ -384,7 +398,7 @@ struct System {
     // One CG iteration of each variant.
     void iteration(hipStream_t s)
     {
-        const SellView F = full.view();
+        const SellViewT<T> F = full.view();
         const double na = n_active;
         const double spmv_bytes = full.bytes_all(16.0);  // d, q
         switch (method) {
@@ -392,8 +406,8 @@ struct System {
             run(s, SFA_KERNEL_SPMV, spmv_bytes,
                 [&] { launch_spmv_dot(s, F, full.nslices, full.row0(0), full.nrows_of(0), state, d, q, partials); });
             run(s, SFA_KERNEL_UPDATE, 48.0 * na, [&] {
-                launch_update(s, UPD_PLAIN, state, n_active, x, r, d, q, nullptr, nullptr, omega,
-                              partials);
+                launch_update(s, UPD_PLAIN, state, n_active, x, r, d, q, (T*)nullptr,
+                              (const T*)nullptr, w(), partials);
             });
             run(s, SFA_KERNEL_DIRECTION, 24.0 * na,
                 [&] { launch_direction(s, state, n_active, r, d); });
@@ -402,7 +416,7 @@ struct System {
             run(s, SFA_KERNEL_SPMV, spmv_bytes,
                 [&] { launch_spmv_dot(s, F, full.nslices, full.row0(0), full.nrows_of(0), state, d, q, partials); });
             run(s, SFA_KERNEL_UPDATE, 64.0 * na, [&] {
-                launch_update(s, UPD_JACOBI, state, n_active, x, r, d, q, z, diag, omega,
+                launch_update(s, UPD_JACOBI, state, n_active, x, r, d, q, z, diag, w(),
                               partials);
             });
             run(s, SFA_KERNEL_DIRECTION, 24.0 * na,
@@ -418,12 +432,12 @@ struct System {
                 [&] { launch_direction(s, state, n_active, z, d); });
             break;
         default:  // Eisenstat: t (in m) = upper solve of d, q = lower solve
-            eisenstat_sweep(s, upper, 0, d, nullptr, m);
+            eisenstat_sweep(s, upper, 0, d, (const T*)nullptr, m);
             eisenstat_sweep(s, lower, 1, d, m, q);
             run(s, SFA_KERNEL_COMBINE, 32.0 * na,
                 [&] { launch_eisenstat_combine(s, state, n_active, q, m, d, partials); });
             run(s, SFA_KERNEL_UPDATE, 64.0 * na, [&] {
-                launch_update(s, UPD_EISENSTAT, state, n_active, x, r, m, q, z, diag, omega,
+                launch_update(s, UPD_EISENSTAT, state, n_active, x, r, m, q, z, diag, w(),
                               partials);
             });
             run(s, SFA_KERNEL_DIRECTION, 24.0 * na,
@@ -470,37 +484,37 @@ struct System {
     // zeros for SSOR / Rainbow (:493-494, :1056-1057), NaN otherwise.
     void init_scratch(hipStream_t s)
     {
-        const double init = (method == SFA_METHOD_SSOR || method == SFA_METHOD_RAINBOW) ? 0.0 : NAN;
-        for (double* v : {d, z, q, m}) launch_fill(s, v, init, n);
+        const T init = (method == SFA_METHOD_SSOR || method == SFA_METHOD_RAINBOW) ? T(0) : T(NAN);
+        for (T* v : {d, z, q, m}) launch_fill(s, v, init, n);
     }
 
     // ---------------------------------------------------------------- solve
 
     // Copies the inputs into device order, resets the state and runs the
     // restart.  Returns the stream the solve runs on.
-    hipStream_t begin_solve(double* ux, const double* ub, double* ures, double precision,
-                            int print_modulo, int device_pointers)
+    hipStream_t begin_solve(T* ux, const T* ub, T* ures, double precision, int print_modulo,
+                            int device_pointers, double rel2 = 0.)
     {
         if (!(precision > 0)) throw Error{SFA_ERR_INVALID, "precision must be positive"};
         SFA_HIP(hipSetDevice(device));
         hipStream_t s = stream;
-        const double* src_x = ux;
-        const double* src_b = ub;
-        const double* src_r = ures;
+        const T* src_x = ux;
+        const T* src_b = ub;
+        const T* src_r = ures;
         double host_tol = 0.;
         int use_host_tol = 0;
         if (!device_pointers) {
             // solver/sparsesolver.cpp:469-472 on the host, in row order
             double t = 0.;
-            for (int i = 0; i < n; i++) t += ub[i] * ub[i];
+            for (int i = 0; i < n; i++) t += (double)ub[i] * (double)ub[i];
             t = precision * std::sqrt(t);
             const double floor_ = 1.4901161193847656e-08;
             if (!(t >= floor_) && t <= floor_) t = floor_;
             host_tol = t;
             use_host_tol = 1;
-            SFA_HIP(hipMemcpyAsync(io, ux, n * sizeof(double), hipMemcpyHostToDevice, s));
-            SFA_HIP(hipMemcpyAsync(io2, ub, n * sizeof(double), hipMemcpyHostToDevice, s));
-            SFA_HIP(hipMemcpyAsync(io3, ures, n * sizeof(double), hipMemcpyHostToDevice, s));
+            SFA_HIP(hipMemcpyAsync(io, ux, n * sizeof(T), hipMemcpyHostToDevice, s));
+            SFA_HIP(hipMemcpyAsync(io2, ub, n * sizeof(T), hipMemcpyHostToDevice, s));
+            SFA_HIP(hipMemcpyAsync(io3, ures, n * sizeof(T), hipMemcpyHostToDevice, s));
             src_x = io;
             src_b = io2;
             src_r = io3;
@@ -511,22 +525,22 @@ struct System {
         init_scratch(s);
         const int small_breaks = (method == SFA_METHOD_SSOR_EISENSTAT) ? (print_modulo >= -2) : 1;
         launch_init_state(s, state, b, n, precision, host_tol, use_host_tol, n, small_breaks,
-                          partials);
+                          rel2, partials);
         restart(s);
         read_state();
         return s;
     }
 
-    void finish_solve(hipStream_t s, double* ux, double* ures, int print_modulo,
-                      int device_pointers, sfa_solve_info* info)
+    void finish_solve(hipStream_t s, T* ux, T* ures, int print_modulo, int device_pointers,
+                      sfa_solve_info* info)
     {
-        double* dst_x = device_pointers ? ux : io;
-        double* dst_r = device_pointers ? ures : io3;
+        T* dst_x = device_pointers ? ux : io;
+        T* dst_r = device_pointers ? ures : io3;
         launch_scatter(s, dst_x, x, perm, n);
         launch_scatter(s, dst_r, r, perm, n);
         if (!device_pointers) {
-            SFA_HIP(hipMemcpyAsync(ux, io, n * sizeof(double), hipMemcpyDeviceToHost, s));
-            SFA_HIP(hipMemcpyAsync(ures, io3, n * sizeof(double), hipMemcpyDeviceToHost, s));
+            SFA_HIP(hipMemcpyAsync(ux, io, n * sizeof(T), hipMemcpyDeviceToHost, s));
+            SFA_HIP(hipMemcpyAsync(ures, io3, n * sizeof(T), hipMemcpyDeviceToHost, s));
         }
         SFA_HIP(hipStreamSynchronize(s));
         SFA_HIP(hipGetLastError());
@@ -542,10 +556,17 @@ struct System {
         if (print_modulo >= -2) print_log(print_modulo, st);
     }
 
-    void solve(double* ux, const double* ub, double* ures, double precision, int print_modulo,
+    void solve(T* ux, const T* ub, T* ures, double precision, int print_modulo,
                int device_pointers, sfa_solve_info* info)
     {
         hipStream_t s = begin_solve(ux, ub, ures, precision, print_modulo, device_pointers);
+        iterate(s);
+        finish_solve(s, ux, ures, print_modulo, device_pointers, info);
+    }
+
+    // Runs iterations until the state says the loop has ended.
+    void iterate(hipStream_t s)
+    {
         // Iterations: replay a captured chunk, predicting from the z.r history
         // how many chunks to queue before the next look at the state.
         const int G = 4;
@@ -557,12 +578,11 @@ struct System {
             if (host_state->done) break;
             queued = predict_chunks(G);
         }
-        finish_solve(s, ux, ures, print_modulo, device_pointers, info);
     }
 
     // The same solve, every kernel launched on its own between two events.
-    void profile(double* ux, const double* ub, double* ures, double precision,
-                 int device_pointers, sfa_kernel_profile* out, sfa_solve_info* info)
+    void profile(T* ux, const T* ub, T* ures, double precision, int device_pointers,
+                 sfa_kernel_profile* out, sfa_solve_info* info)
     {
         Profiler p;
         prof = &p;
@@ -639,7 +659,7 @@ struct System {
         std::fflush(stdout);
     }
 
-    void apply(double* dest, const double* src, double scaling, int device_pointers)
+    void apply(T* dest, const T* src, double scaling, int device_pointers)
     {
         SFA_HIP(hipSetDevice(device));
         if (!has_apply_part) {
@@ -650,17 +670,17 @@ struct System {
         }
         hipStream_t s = stream;
         // source with a zero slot at n: use the x-sized buffer m
-        double* srcbuf = q;
+        T* srcbuf = q;
         if (device_pointers)
-            SFA_HIP(hipMemcpyAsync(srcbuf, src, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+            SFA_HIP(hipMemcpyAsync(srcbuf, src, n * sizeof(T), hipMemcpyDeviceToDevice, s));
         else
-            SFA_HIP(hipMemcpyAsync(srcbuf, src, n * sizeof(double), hipMemcpyHostToDevice, s));
-        launch_fill(s, srcbuf + n, 0.0, 1);
-        double* out = device_pointers ? dest : io;
+            SFA_HIP(hipMemcpyAsync(srcbuf, src, n * sizeof(T), hipMemcpyHostToDevice, s));
+        launch_fill(s, srcbuf + n, T(0), 1);
+        T* out = device_pointers ? dest : io;
         launch_apply(s, apply_part.view(), apply_part.nslices, apply_part.row0(0), apply_part.nrows_of(0),
-                     srcbuf, out, scaling);
+                     srcbuf, out, T(scaling));
         if (!device_pointers)
-            SFA_HIP(hipMemcpyAsync(dest, io, n * sizeof(double), hipMemcpyDeviceToHost, s));
+            SFA_HIP(hipMemcpyAsync(dest, io, n * sizeof(T), hipMemcpyDeviceToHost, s));
         SFA_HIP(hipStreamSynchronize(s));
     }
 
@@ -691,6 +711,8 @@ struct System {
         *st = o;
     }
 };
+
+using System = SystemT<double>;
 
 }  // namespace sfa
 
