@@ -44,6 +44,10 @@ extern "C" {
 #define SFA_EXIT_BREAKDOWN 2  /* z.r or d.Ad non-finite or negative           */
 #define SFA_EXIT_SMALL     3  /* sqrt|d.Ad| < machine epsilon                 */
 
+/* OR into the method of sfa_system_create: also keep an FP32 copy of the
+ * matrix and schedules for sfa_system_solve_mixed */
+#define SFA_FLAG_FP32_INNER 0x100
+
 /* solver variants (one per function of solver/sparsesolver.hpp:11-84) */
 #define SFA_METHOD_CG                  0
 #define SFA_METHOD_DIAGONAL            1
@@ -169,6 +173,27 @@ int sfa_system_create(int method, int N,
 int sfa_system_solve(sfa_system sys, double* x, const double* b,
                      double* residual, double precision, int print_modulo,
                      int device_pointers, sfa_solve_info* info);
+
+/* Mixed precision.  No reference counterpart (the reference's single-precision
+ * build, common.compile.mk:162-172, makes everything FP32).  FP64 iterative
+ * refinement: r = b - A x in FP64, the system's variant solves A e = r in
+ * FP32 (FP32 matrix, vectors and sweeps; FP64 dot products) until its
+ * preconditioned residual has dropped by inner_rtol, x += e in FP64; stops
+ * when ||b - A x||_2 <= max(rtol*||b||_2, sqrt(DBL_EPSILON)) or after
+ * max_outer corrections.  Needs a system created with SFA_FLAG_FP32_INNER;
+ * rows the variant skips keep their x. */
+typedef struct sfa_refine_info {
+    int    outer;             /* FP64 corrections applied                    */
+    int    inner_iterations;  /* FP32 CG iterations, summed                  */
+    int    success;           /* final FP64 residual met the target          */
+    int    reserved;
+    double residual_norm;     /* ||b - A x||_2 of the returned x (FP64)      */
+    double relative_residual; /* residual_norm / ||b||_2                     */
+} sfa_refine_info;
+
+int sfa_system_solve_mixed(sfa_system sys, double* x, const double* b,
+                           double rtol, double inner_rtol, int max_outer,
+                           int device_pointers, sfa_refine_info* info);
 
 /* dest = scaling * A * src on the device (the system's SpMV kernel). */
 int sfa_system_apply(sfa_system sys, double* dest, const double* src,

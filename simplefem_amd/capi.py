@@ -50,6 +50,17 @@ class SystemStats(ctypes.Structure):
                 ("reserved", c_int)]
 
 
+class RefineInfo(ctypes.Structure):
+    _fields_ = [("outer", c_int), ("inner_iterations", c_int), ("success", c_int),
+                ("reserved", c_int), ("residual_norm", c_double), ("relative_residual", c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+FLAG_FP32_INNER = 0x100
+
+
 class KernelProfile(ctypes.Structure):
     _fields_ = [("launches", c_int), ("reserved", c_int), ("milliseconds", c_double),
                 ("bytes", c_double)]
@@ -83,6 +94,8 @@ _SIGNATURES = {
     "sfa_system_solve": ([c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int, c_int,
                           POINTER(SolveInfo)], c_int),
     "sfa_system_apply": ([c_void_p, c_void_p, c_void_p, c_double, c_int], c_int),
+    "sfa_system_solve_mixed": ([c_void_p, c_void_p, c_void_p, c_double, c_double, c_int, c_int,
+                                POINTER(RefineInfo)], c_int),
     "sfa_system_set_stream": ([c_void_p, c_void_p], c_int),
     "sfa_system_query": ([c_void_p, POINTER(SystemStats)], c_int),
     "sfa_system_profile": ([c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int,
@@ -269,8 +282,10 @@ class System:
     """
 
     def __init__(self, method, N, A, C, V, aux=None, omega=1.0, num_colors=0, F=None, B=None,
-                 R=None):
+                 R=None, fp32_inner=False):
         lib = load()
+        if fp32_inner:
+            method |= FLAG_FP32_INNER
         A, C, V = _csr(A, C, V)
         aux = _f64(aux, "aux", N)
         F, B, R = _i32(F, "F"), _i32(B, "B"), _i32(R, "R")
@@ -301,6 +316,19 @@ class System:
         if info is not None:
             info.update(out.as_dict())
         return out.iterations
+
+    def solve_mixed(self, x, b, rtol, inner_rtol=1e-4, max_outer=20):
+        """FP64 refinement around FP32 inner solves (system created with
+        fp32_inner=True).  Returns a dict (outer, inner_iterations, success,
+        residual_norm, relative_residual)."""
+        x, px, dx = self._ptr(x, self.N, True)
+        b, pb, db = self._ptr(b, self.N, False)
+        if dx != db:
+            raise TypeError("x and b must both be host arrays or both device buffers")
+        out = RefineInfo()
+        _check(load().sfa_system_solve_mixed(self._h, px, pb, float(rtol), float(inner_rtol),
+                                             int(max_outer), dx, ctypes.byref(out)))
+        return out.as_dict()
 
     def profile(self, x, b, residual, precision):
         """One solve with every kernel timed by HIP events; per-class totals."""

@@ -198,6 +198,8 @@ struct SystemT {
     hipGraphExec_t chunk_exec = nullptr;
     int chunk_iters = 0;
     Profiler* prof = nullptr;  // set only while sfa_system_profile runs
+    // FP32 copy of the plan for the mixed-precision solve (T = double only)
+    std::unique_ptr<SystemT<float>> inner;
 
     // Launches fn(); in a profiled solve, between two events on stream s.
     template <class Fn>
@@ -219,6 +221,7 @@ struct SystemT {
 
     void release()
     {
+        inner.reset();
         if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
         chunk_exec = nullptr;
         full.release();
@@ -241,7 +244,8 @@ struct SystemT {
     }
 
     void create(int method_, int n_, const int* A, const int* C, const double* V, const double* aux,
-                double omega_, int nc, const int* F, const int* B, const int* R)
+                double omega_, int nc, const int* F, const int* B, const int* R,
+                bool with_fp32 = false)
     {
         // Arguments are checked on the host before the device is touched.
         Plan plan;
@@ -250,6 +254,11 @@ struct SystemT {
                                      &colouring_error);
         if (!err.empty()) throw Error{colouring_error ? SFA_ERR_COLOURING : SFA_ERR_INVALID, err};
         create_from_plan(plan, A, C, V);
+        if (with_fp32) {
+            inner.reset(new SystemT<float>());
+            inner->create_from_plan(plan, A, C, V);
+            inner->stream = stream;
+        }
     }
 
     void create_from_plan(const Plan& plan, const int* A, const int* C, const double* V)
@@ -299,7 +308,7 @@ struct SystemT {
         state = dev_alloc<SolveState>(1);
         hist = dev_alloc<double>((size_t)n + 1);
         int max_blocks = 4096;
-        for (const DevPart* p : {&full, &lower, &upper, &lower_restart})
+        for (const DevPartT<T>* p : {&full, &lower, &upper, &lower_restart})
             max_blocks = std::max(max_blocks, p->nslices + 1);
         partials = dev_alloc<double>(max_blocks);
         int ngr = 1;
@@ -522,13 +531,20 @@ struct SystemT {
         launch_gather(s, x, src_x, perm, n);
         launch_gather(s, b, src_b, perm, n);
         launch_gather(s, r, src_r, perm, n);
+        start(s, precision, print_modulo, host_tol, use_host_tol, rel2);
+        return s;
+    }
+
+    // State, scratch and restart for x, b, r already in device order.
+    void start(hipStream_t s, double precision, int print_modulo, double host_tol,
+               int use_host_tol, double rel2)
+    {
         init_scratch(s);
         const int small_breaks = (method == SFA_METHOD_SSOR_EISENSTAT) ? (print_modulo >= -2) : 1;
         launch_init_state(s, state, b, n, precision, host_tol, use_host_tol, n, small_breaks,
                           rel2, partials);
         restart(s);
         read_state();
-        return s;
     }
 
     void finish_solve(hipStream_t s, T* ux, T* ures, int print_modulo, int device_pointers,
@@ -682,6 +698,75 @@ struct SystemT {
         if (!device_pointers)
             SFA_HIP(hipMemcpyAsync(dest, io, n * sizeof(T), hipMemcpyDeviceToHost, s));
         SFA_HIP(hipStreamSynchronize(s));
+    }
+
+    // FP64 iterative refinement around FP32 inner solves of the same variant:
+    // r = b - A x (FP64) -> e = A^-1 r in FP32 to a reduction inner_rtol of
+    // the preconditioned residual -> x += e (FP64), until ||b - A x||_2 <=
+    // max(rtol * ||b||_2, 2^-26) or max_outer corrections.
+    void solve_mixed(T* ux, const T* ub, double rtol, double inner_rtol, int max_outer,
+                     int device_pointers, sfa_refine_info* info)
+    {
+        if (!inner) throw Error{SFA_ERR_INVALID, "system was created without SFA_FLAG_FP32_INNER"};
+        if (!(rtol > 0) || !(inner_rtol > 0) || !(inner_rtol < 1) || max_outer < 0)
+            throw Error{SFA_ERR_INVALID, "need rtol > 0, 0 < inner_rtol < 1, max_outer >= 0"};
+        SFA_HIP(hipSetDevice(device));
+        hipStream_t s = stream;
+        inner->stream = s;
+        const T* src_x = ux;
+        const T* src_b = ub;
+        double bnorm2 = 0.;
+        if (!device_pointers) {
+            for (int i = 0; i < n; i++) bnorm2 += (double)ub[i] * (double)ub[i];
+            SFA_HIP(hipMemcpyAsync(io, ux, n * sizeof(T), hipMemcpyHostToDevice, s));
+            SFA_HIP(hipMemcpyAsync(io2, ub, n * sizeof(T), hipMemcpyHostToDevice, s));
+            src_x = io;
+            src_b = io2;
+        }
+        launch_gather(s, x, src_x, perm, n);
+        launch_gather(s, b, src_b, perm, n);
+        if (device_pointers) {  // ||b|| on the device (k_init_state with precision 1)
+            launch_init_state(s, state, b, n, 1.0, 0., 0, n, 1, 0., partials);
+            read_state();
+            bnorm2 = host_state->tol * host_state->tol;
+        }
+        double target = rtol * std::sqrt(bnorm2);
+        const double floor_ = 1.4901161193847656e-08;
+        if (!(target >= floor_)) target = floor_;
+        int outer = 0;
+        long long inner_total = 0;
+        double res = 0.;
+        for (;;) {
+            launch_residual_norm(s, full.view(), full.nslices, full.row0(0), full.nrows_of(0),
+                                 state, b, x, r, partials);
+            read_state();
+            res = std::sqrt(host_state->result);
+            if (!(res > target) || outer >= max_outer) break;
+            launch_to_float(s, inner->b, r, n_active);
+            launch_fill(s, inner->x, 0.0f, n);
+            launch_fill(s, inner->r, 0.0f, n);
+            inner->start(s, 1.0, -3, 0., 0, inner_rtol * inner_rtol);
+            inner->iterate(s);
+            if (inner->host_state->exit_reason == SFA_EXIT_BREAKDOWN) break;
+            inner_total += inner->host_state->K;
+            launch_add_correction(s, x, inner->x, n_active);
+            outer++;
+        }
+        T* dst_x = device_pointers ? ux : io;
+        launch_scatter(s, dst_x, x, perm, n);
+        if (!device_pointers)
+            SFA_HIP(hipMemcpyAsync(ux, io, n * sizeof(T), hipMemcpyDeviceToHost, s));
+        SFA_HIP(hipStreamSynchronize(s));
+        SFA_HIP(hipGetLastError());
+        if (info) {
+            sfa_refine_info o{};
+            o.outer = outer;
+            o.inner_iterations = (int)inner_total;
+            o.success = res <= target ? 1 : 0;
+            o.residual_norm = res;
+            o.relative_residual = bnorm2 > 0 ? res / std::sqrt(bnorm2) : res;
+            *info = o;
+        }
     }
 
     void query(sfa_system_stats* st) const
@@ -895,7 +980,9 @@ int sfa_system_create(int method, int N, const int* csrrows, const int* csrcolum
         if (!out) throw Error{SFA_ERR_INVALID, "null output handle"};
         *out = nullptr;
         std::unique_ptr<sfa_system_s> h(new sfa_system_s());
-        h->sys.create(method, N, csrrows, csrcolumns, csrvalues, aux, omega, num_colors, F, B, R);
+        const bool fp32 = (method & SFA_FLAG_FP32_INNER) != 0;
+        h->sys.create(method & ~SFA_FLAG_FP32_INNER, N, csrrows, csrcolumns, csrvalues, aux, omega,
+                      num_colors, F, B, R, fp32);
         *out = h.release();
     });
 }
@@ -907,6 +994,16 @@ int sfa_system_solve(sfa_system sys, double* x, const double* b, double* residua
     return guarded([&] {
         if (!sys || !x || !b || !residual) throw Error{SFA_ERR_INVALID, "null argument"};
         sys->sys.solve(x, b, residual, precision, print_modulo, device_pointers, info);
+    });
+}
+
+int sfa_system_solve_mixed(sfa_system sys, double* x, const double* b, double rtol,
+                           double inner_rtol, int max_outer, int device_pointers,
+                           sfa_refine_info* info)
+{
+    return guarded([&] {
+        if (!sys || !x || !b) throw Error{SFA_ERR_INVALID, "null argument"};
+        sys->sys.solve_mixed(x, b, rtol, inner_rtol, max_outer, device_pointers, info);
     });
 }
 

@@ -288,3 +288,32 @@ def test_cpp_dropin_callers():
     proc = subprocess.run([binary, "gpu"], capture_output=True, text=True, timeout=240)
     print(proc.stdout[-3000:])
     assert proc.returncode == 0, proc.stdout[-3000:] + proc.stderr[-2000:]
+
+
+@pytest.mark.parametrize("variant,method", [("rainbow", capi.METHOD_RAINBOW),
+                                            ("ssor", capi.METHOD_SSOR)])
+def test_mixed_precision_refinement(variant, method):
+    """FP32 inner solves + FP64 refinement reach ||b-Ax|| <= 1e-10 ||b|| in FP64
+    and agree with the FP64 solve of the same system to the accuracy that
+    residual allows (relative error bound 1e-10 * condition estimate)."""
+    system = S.poisson3d_p1_kuhn(16)
+    b = S.rhs_uniform(system, 8)
+    d = system.diagonal()
+    col = oracle.rainbow(system.n, system.A, system.C)
+    colour_args = col if variant == "rainbow" else (0, None, None, None)
+    with capi.System(method, system.n, system.A, system.C, system.V, d, 1.0, *colour_args,
+                     fp32_inner=True) as sysm:
+        x = np.zeros(system.n)
+        info = sysm.solve_mixed(x, b, 1e-10, inner_rtol=1e-4)
+        assert info["success"] == 1 and info["relative_residual"] <= 1e-10, info
+        assert info["outer"] >= 1
+        res = b - oracle.csr_apply(system.n, system.A, system.C, system.V, x)
+        res[system.dirichlet] = 0.0
+        assert np.linalg.norm(res) <= 1.0000001e-10 * np.linalg.norm(b)
+        assert np.array_equal(x[system.dirichlet], np.zeros(system.dirichlet.sum()))
+        x64, r64, i64 = oracle.solve(variant, system.A, system.C, system.V, b, aux=d,
+                                     colouring=col, precision=1e-12)
+        err = np.abs(x - x64).max() / np.abs(x64).max()
+        assert err <= 1e-8, err
+        print(f"[mixed] {variant}: outer {info['outer']} inner {info['inner_iterations']} "
+              f"rel.res {info['relative_residual']:.2e} err vs FP64 {err:.2e}")
