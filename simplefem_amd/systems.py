@@ -277,3 +277,104 @@ def random_spd(n: int, density: float, seed: int) -> CSRSystem:
     vals = np.concatenate([V0, rowsum + rng.uniform(0.5, 1.5, n)])
     A, C, V = _coo_to_csr(n, rows, cols, vals)
     return CSRSystem(f"random_spd_{n}_{seed}", n, A, C, V, np.zeros(n, dtype=bool))
+
+
+def _csr_from_int_coo(n, rows, cols, vals_list):
+    """COO with several integer value arrays -> CSR (sorted columns), exact
+    integer duplicate sums; returns A, C and one summed array per input."""
+    key = rows.astype(np.int64) * n + cols.astype(np.int64)
+    order = np.argsort(key, kind="stable")
+    key = key[order]
+    start = np.concatenate([[True], key[1:] != key[:-1]])
+    idx = np.nonzero(start)[0]
+    uniq = key[idx]
+    sums = [np.add.reduceat(v[order].astype(np.int64), idx) for v in vals_list]
+    r = uniq // n
+    A = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(A, r + 1, 1)
+    return np.cumsum(A).astype(np.int32), (uniq % n).astype(np.int32), sums
+
+
+def _whitney1_element(coords):
+    """Integer curl-curl and mass matrices of the six Whitney 1-forms of a
+    unit Kuhn tetrahedron (volume 1/6), edge (i, j), i < j local:
+      curl-curl: K_ef = vol*4*(g_i x g_j).(g_k x g_l)  = (2/3) * Kint
+      mass:      M_ef = vol/20 * sum (1+d_ac) g_b.g_d ... = (1/120) * Mint
+    with g = gradients of the barycentric coordinates."""
+    coords = np.asarray(coords, dtype=np.float64)
+    T = (coords[1:] - coords[0]).T
+    Tinv = np.linalg.inv(T)
+    g = np.vstack([-Tinv.sum(axis=0), Tinv])
+    edges = [(i, j) for i in range(4) for j in range(i + 1, 4)]
+    K = np.zeros((6, 6))
+    M = np.zeros((6, 6))
+    dl = lambda a, b: 1.0 if a == b else 0.0
+    for e, (i, j) in enumerate(edges):
+        for f, (k, l) in enumerate(edges):
+            K[e, f] = 4.0 * np.dot(np.cross(g[i], g[j]), np.cross(g[k], g[l])) / 6.0
+            M[e, f] = ((1 + dl(i, k)) * g[j] @ g[l] - (1 + dl(i, l)) * g[j] @ g[k]
+                       - (1 + dl(j, k)) * g[i] @ g[l] + (1 + dl(j, l)) * g[i] @ g[k]) / 120.0
+    Kint = np.rint(K * 1.5).astype(np.int64)
+    Mint = np.rint(M * 120.0).astype(np.int64)
+    assert np.allclose(K * 1.5, Kint) and np.allclose(M * 120.0, Mint)
+    return edges, Kint, Mint
+
+
+def hcurl3d_whitney_kuhn(n: int, curl_weight: float = 1.0, mass_weight: float = 1.0,
+                         boundary: bool = True) -> CSRSystem:
+    """curl-curl + mass of lowest-order Whitney 1-forms (edge elements) on n^3
+    Kuhn cubes, edges oriented from the lower to the higher vertex number;
+    edges lying in the boundary carry the tangential Dirichlet condition
+    (their rows and columns are 0, structure kept).  For n = 48: 795,024 edges."""
+    nx = n + 1
+    h = 1.0 / n
+    ix, iy, iz = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    ix, iy, iz = ix.ravel(), iy.ravel(), iz.ravel()
+
+    def corner(bits):
+        dx, dy, dz = (bits >> 2) & 1, (bits >> 1) & 1, bits & 1
+        return (ix + dx) + nx * ((iy + dy) + nx * (iz + dz))
+
+    tet_edges, tet_K, tet_M = [], [], []
+    for path in _KUHN:
+        bits = (0,) + path
+        coords = [[(bb >> 2) & 1, (bb >> 1) & 1, bb & 1] for bb in bits]
+        edges, Kint, Mint = _whitney1_element(coords)
+        verts = np.stack([corner(bb) for bb in bits], axis=1)  # (cubes, 4), increasing ids
+        ga = np.stack([verts[:, i] for i, j in edges], axis=1)
+        gb = np.stack([verts[:, j] for i, j in edges], axis=1)
+        assert (ga < gb).all()  # a Kuhn path visits vertices in increasing id order
+        tet_edges.append(ga.astype(np.int64) * nx ** 3 + gb)
+        tet_K.append(Kint)
+        tet_M.append(Mint)
+    allkeys = np.concatenate([t.ravel() for t in tet_edges])
+    ekeys, inverse = np.unique(allkeys, return_inverse=True)
+    ne = ekeys.shape[0]
+    rows, cols, kv, mv = [], [], [], []
+    off = 0
+    for t in range(len(_KUHN)):
+        cnt = tet_edges[t].size
+        eid = inverse[off:off + cnt].reshape(-1, 6)
+        off += cnt
+        rows.append(np.repeat(eid, 6, axis=1).ravel())
+        cols.append(np.tile(eid, (1, 6)).ravel())
+        kv.append(np.tile(tet_K[t].ravel(), eid.shape[0]))
+        mv.append(np.tile(tet_M[t].ravel(), eid.shape[0]))
+    rows, cols = np.concatenate(rows), np.concatenate(cols)
+    kv, mv = np.concatenate(kv), np.concatenate(mv)
+    # boundary edges: both ends on one boundary plane
+    a, b = ekeys // nx ** 3, ekeys % nx ** 3
+    ca = np.stack([a % nx, (a // nx) % nx, a // nx ** 2], axis=1)
+    cb = np.stack([b % nx, (b // nx) % nx, b // nx ** 2], axis=1)
+    dirichlet = (((ca == 0) & (cb == 0)) | ((ca == n) & (cb == n))).any(axis=1)
+    if not boundary:
+        dirichlet = np.zeros(ne, dtype=bool)
+    touch = dirichlet[rows] | dirichlet[cols]
+    kv = np.where(touch, 0, kv)
+    mv = np.where(touch, 0, mv)
+    A, C, (Ks, Ms) = _csr_from_int_coo(ne, rows, cols, [kv, mv])
+    V = (curl_weight * (2.0 / (3.0 * h))) * Ks.astype(np.float64) \
+        + (mass_weight * (h / 120.0)) * Ms.astype(np.float64)
+    system = CSRSystem(f"hcurl3d_whitney_kuhn_{n}", ne, A, C, V, dirichlet)
+    system.edges = np.stack([a, b], axis=1)  # vertex pair of each edge
+    return system

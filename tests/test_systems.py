@@ -1,0 +1,100 @@
+"""The synthetic systems: structure and values fixed by the mathematics."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from simplefem_amd import systems as S
+
+
+def _csr(s):
+    return sp.csr_matrix((s.V, s.C, s.A), shape=(s.n, s.n))
+
+
+@pytest.mark.parametrize("L", range(1, 6))
+def test_p1_2d_structured_equals_element_assembly(L):
+    a, b = S._element_assembled_2d(L), S.poisson2d_p1(L)
+    assert np.array_equal(a.A, b.A) and np.array_equal(a.C, b.C) and np.array_equal(a.V, b.V)
+
+
+@pytest.mark.parametrize("n", range(1, 7))
+def test_p1_3d_structured_equals_element_assembly(n):
+    a, b = S._element_assembled_3d(n), S.poisson3d_p1_kuhn(n)
+    assert np.array_equal(a.A, b.A) and np.array_equal(a.C, b.C) and np.array_equal(a.V, b.V)
+
+
+def test_config_sizes():
+    """SURVEY §8 counts: N = (2^L+1)^2, nnz = N + 2E (every element edge kept)."""
+    s2 = S.poisson2d_p1(7)
+    assert (s2.n, s2.nnz, int((~s2.dirichlet).sum())) == (16641, 115457, 16129)
+    s3 = S.poisson3d_p1_kuhn(64)
+    assert (s3.n, s3.nnz, int((~s3.dirichlet).sum())) == (274625, 4018753, 250047)
+
+
+def test_p1_kuhn_stencil_is_seven_point_laplacian():
+    """On a uniform Kuhn mesh the P1 stiffness equals h * (7-point Laplacian):
+    face- and body-diagonal couplings are exact zeros."""
+    n = 6
+    s = S.poisson3d_p1_kuhn(n)
+    h = 1.0 / n
+    M = _csr(s).toarray()
+    i = (n // 2) + (n + 1) * ((n // 2) + (n + 1) * (n // 2))
+    row = M[i]
+    assert row[i] == 6 * h
+    for off in (1, n + 1, (n + 1) ** 2):
+        assert row[i + off] == -h and row[i - off] == -h
+    assert np.count_nonzero(row) == 7
+    assert s.A[i + 1] - s.A[i] == 15  # but 15 structural entries
+
+
+def test_whitney_de_rham_identities():
+    """Whitney 1-forms: curl-curl annihilates discrete gradients, and the
+    edge mass matrix pulled back by the gradient is the P1 stiffness."""
+    n = 3
+    curl = S.hcurl3d_whitney_kuhn(n, curl_weight=1.0, mass_weight=0.0, boundary=False)
+    mass = S.hcurl3d_whitney_kuhn(n, curl_weight=0.0, mass_weight=1.0, boundary=False)
+    p1 = S._element_assembled_3d(n)
+    nv = p1.n
+    e = curl.edges
+    G = sp.csr_matrix((np.concatenate([-np.ones(len(e)), np.ones(len(e))]),
+                       (np.concatenate([np.arange(len(e))] * 2), np.concatenate([e[:, 0], e[:, 1]]))),
+                      shape=(len(e), nv))
+    # P1 without boundary conditions: assemble again with no Dirichlet vertices
+    p1_free = S._finish("free", nv, [S._assemble(nv, *args) for args in _kuhn_parts(n)],
+                        (1.0 / n) / 6.0, np.zeros(nv, dtype=bool))
+    KG = (_csr(curl) @ G).toarray()
+    assert np.abs(KG).max() <= 1e-12 * np.abs(_csr(curl).toarray()).max()
+    GtMG = (G.T @ _csr(mass) @ G).toarray()
+    assert np.allclose(GtMG, _csr(p1_free).toarray(), rtol=0, atol=1e-14)
+
+
+def _kuhn_parts(n):
+    nx = n + 1
+    ix, iy, iz = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    ix, iy, iz = ix.ravel(), iy.ravel(), iz.ravel()
+    out = []
+    for path in S._KUHN:
+        bits = (0,) + path
+        coords = [[(bb >> 2) & 1, (bb >> 1) & 1, bb & 1] for bb in bits]
+        elems = np.stack([(ix + ((bb >> 2) & 1)) + nx * ((iy + ((bb >> 1) & 1)) + nx * (iz + (bb & 1)))
+                          for bb in bits], axis=1)
+        out.append((elems, S._p1_element_matrix(coords), (1.0 / n) / 6.0,
+                    np.zeros(nx ** 3, dtype=bool)))
+    return out
+
+
+def test_whitney_system_is_spd_on_interior_edges():
+    s = S.hcurl3d_whitney_kuhn(3)
+    M = _csr(s).toarray()
+    assert np.array_equal(M, M.T)
+    act = ~s.dirichlet
+    assert np.linalg.eigvalsh(M[np.ix_(act, act)]).min() > 0
+    assert np.all(M[s.dirichlet] == 0)
+
+
+def test_rhs_is_seeded_and_zero_on_dirichlet_rows():
+    s = S.poisson2d_p1(4)
+    b1, b2 = S.rhs_uniform(s, 3), S.rhs_uniform(s, 3)
+    assert np.array_equal(b1, b2)
+    assert np.all(b1[s.dirichlet] == 0) and np.all(np.abs(b1[~s.dirichlet]) <= 1)
+    # SplitMix64 known answer (seed 0, first output)
+    assert int(S.splitmix64(0, 1)[0]) == 0xE220A8397B1DCDAF
