@@ -249,6 +249,69 @@ int sfa_system_profile(sfa_system sys, double* x, const double* b,
 
 int sfa_system_destroy(sfa_system sys);
 
+/* ---------------------------------------------------------------------------
+ * Batched and served solves.  No reference counterpart (the reference solves
+ * one system per call).  A batch holds up to max_slots (<= 1024) solves of
+ * Rainbow CG-SSOR (the iteration of ConjugateGradientSolverCSR_Rainbow,
+ * solver/sparsesolver.cpp:1002-1335) over any number of device-resident
+ * matrices; one persistent kernel advances every running solve a phase per
+ * step, so solves of any size share each launch.  Solves can be submitted and
+ * collected between sfa_batch_run calls (continuous batching).
+ *   SFA_BATCH_FP64:  FP64 throughout; stopping rule of the reference
+ *                    (sqrt(z.r) < max(precision*||b||, 2^-26)).
+ *   SFA_BATCH_MIXED: FP32 matrix/vectors/sweeps inside FP64 iterative
+ *                    refinement, stopping on ||b - A x||_2 <=
+ *                    max(precision*||b||, 2^-26) in FP64.
+ * ------------------------------------------------------------------------- */
+#define SFA_BATCH_FP64  0
+#define SFA_BATCH_MIXED 1
+
+typedef struct sfa_batch_s* sfa_batch;
+
+typedef struct sfa_batch_info {
+    int    iterations;     /* CG iterations (FP32 inner iterations, summed, for MIXED) */
+    int    outer;          /* FP64 corrections (MIXED)                       */
+    int    success;        /* the stopping rule was met                      */
+    int    exit_reason;    /* SFA_EXIT_* of the last CG loop                 */
+    double zr;             /* final z.r of the last CG loop                  */
+    double residual_norm;  /* sqrt(z.r) (FP64) or ||b - A x||_2 (MIXED)      */
+} sfa_batch_info;
+
+int sfa_batch_create(int precision_mode, int max_slots, sfa_batch* out);
+
+/* Uploads a matrix with its Rainbow colouring (arguments of
+ * ConjugateGradientSolverCSR_Rainbow); returns its id. */
+int sfa_batch_add_matrix(sfa_batch b, int N, const int* csrrows,
+                         const int* csrcolumns, const double* csrvalues,
+                         const double* diagonal, double omega, int num_colors,
+                         const int* F, const int* B, const int* R, int* matrix_id);
+
+/* Frees a matrix's device memory; no slot may still use it. */
+int sfa_batch_remove_matrix(sfa_batch b, int matrix_id);
+
+/* Queues a solve of matrix_id for the host right-hand side rhs (N entries,
+ * original row order) from x0 (NULL: 0); returns its slot.  inner_rtol and
+ * max_outer apply to SFA_BATCH_MIXED only. */
+int sfa_batch_submit(sfa_batch b, int matrix_id, const double* rhs,
+                     const double* x0, double precision, double inner_rtol,
+                     int max_outer, int* slot);
+
+/* Runs at most max_steps steps (0: until no solve is running); *running
+ * receives the number of solves still running. */
+int sfa_batch_run(sfa_batch b, int max_steps, int* running);
+
+/* Lists finished, not yet collected slots. */
+int sfa_batch_finished(sfa_batch b, int* slots, int capacity, int* count);
+
+/* Copies a finished solve's x (original row order; NULL to skip) and frees
+ * its slot. */
+int sfa_batch_collect(sfa_batch b, int slot, double* x, sfa_batch_info* info);
+
+/* Device memory held by matrices and slots. */
+int sfa_batch_memory(sfa_batch b, long long* bytes);
+
+int sfa_batch_destroy(sfa_batch b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -59,6 +59,15 @@ class RefineInfo(ctypes.Structure):
 
 
 FLAG_FP32_INNER = 0x100
+BATCH_FP64, BATCH_MIXED = 0, 1
+
+
+class BatchInfo(ctypes.Structure):
+    _fields_ = [("iterations", c_int), ("outer", c_int), ("success", c_int),
+                ("exit_reason", c_int), ("zr", c_double), ("residual_norm", c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class KernelProfile(ctypes.Structure):
@@ -101,6 +110,17 @@ _SIGNATURES = {
     "sfa_system_profile": ([c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int,
                             POINTER(KernelProfile), POINTER(SolveInfo)], c_int),
     "sfa_system_destroy": ([c_void_p], c_int),
+    "sfa_batch_create": ([c_int, c_int, POINTER(c_void_p)], c_int),
+    "sfa_batch_add_matrix": ([c_void_p, c_int, _P_I, _P_I, _P_D, _P_D, c_double, c_int, _P_I,
+                              _P_I, _P_I, POINTER(c_int)], c_int),
+    "sfa_batch_remove_matrix": ([c_void_p, c_int], c_int),
+    "sfa_batch_submit": ([c_void_p, c_int, _P_D, _P_D, c_double, c_double, c_int,
+                          POINTER(c_int)], c_int),
+    "sfa_batch_run": ([c_void_p, c_int, POINTER(c_int)], c_int),
+    "sfa_batch_finished": ([c_void_p, _P_I, c_int, POINTER(c_int)], c_int),
+    "sfa_batch_collect": ([c_void_p, c_int, _P_D, POINTER(BatchInfo)], c_int),
+    "sfa_batch_memory": ([c_void_p, POINTER(c_longlong)], c_int),
+    "sfa_batch_destroy": ([c_void_p], c_int),
 }
 
 _lib = None
@@ -363,6 +383,81 @@ class System:
     def close(self):
         if self._h:
             load().sfa_system_destroy(self._h)
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class Batch:
+    """Many Rainbow CG-SSOR solves advanced together by one persistent kernel
+    (sfa_batch_*): add matrices, submit right-hand sides, run, collect."""
+
+    def __init__(self, mixed=False, max_slots=256):
+        self._h = c_void_p()
+        self.mixed = mixed
+        _check(load().sfa_batch_create(BATCH_MIXED if mixed else BATCH_FP64, int(max_slots),
+                                       ctypes.byref(self._h)))
+        self.n_of = {}
+
+    def add_matrix(self, N, A, C, V, diagonal, omega, num_colors, F, B, R):
+        A, C, V = _csr(A, C, V)
+        diagonal = _f64(diagonal, "diagonal", N)
+        F, B, R = _i32(F, "F"), _i32(B, "B"), _i32(R, "R")
+        mid = c_int(-1)
+        _check(load().sfa_batch_add_matrix(self._h, N, _i(A), _i(C), _d(V), _d(diagonal),
+                                           float(omega), int(num_colors), _i(F), _i(B), _i(R),
+                                           ctypes.byref(mid)))
+        self.n_of[mid.value] = N
+        return mid.value
+
+    def remove_matrix(self, mid):
+        _check(load().sfa_batch_remove_matrix(self._h, int(mid)))
+        self.n_of.pop(mid, None)
+
+    def submit(self, mid, b, x0=None, precision=1e-10, inner_rtol=1e-4, max_outer=30):
+        N = self.n_of[mid]
+        b = _f64(b, "b", N)
+        x0 = _f64(x0, "x0", N)
+        slot = c_int(-1)
+        _check(load().sfa_batch_submit(self._h, int(mid), _d(b), _d(x0), float(precision),
+                                       float(inner_rtol), int(max_outer), ctypes.byref(slot)))
+        return slot.value
+
+    def run(self, max_steps=0):
+        running = c_int(0)
+        _check(load().sfa_batch_run(self._h, int(max_steps), ctypes.byref(running)))
+        return running.value
+
+    def finished(self, capacity=1024):
+        buf = np.empty(capacity, dtype=np.int32)
+        cnt = c_int(0)
+        _check(load().sfa_batch_finished(self._h, _i(buf), capacity, ctypes.byref(cnt)))
+        return buf[: cnt.value].tolist()
+
+    def collect(self, slot, N):
+        x = np.empty(N)
+        info = BatchInfo()
+        _check(load().sfa_batch_collect(self._h, int(slot), _d(x), ctypes.byref(info)))
+        return x, info.as_dict()
+
+    def memory(self):
+        b = c_longlong(0)
+        _check(load().sfa_batch_memory(self._h, ctypes.byref(b)))
+        return b.value
+
+    def close(self):
+        if self._h:
+            load().sfa_batch_destroy(self._h)
             self._h = c_void_p()
 
     def __del__(self):

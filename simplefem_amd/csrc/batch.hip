@@ -1,0 +1,1033 @@
+// batch.hip -- persistent multi-system Rainbow CG-SSOR engine.
+//
+// Many independent systems (each a matrix + one right-hand side, a "slot")
+// advance together.  Every slot runs a small program of phases:
+//
+//   restart:   RESID  FWD_R[0..Gf)  BWD_R[0..Gb)  DINIT
+//   iteration: SPMV   FWD[0..Gf)    BWD[0..Gb)    DIR          (repeat)
+//   mixed:     RES64 -> CONVERT -> (restart, iterations in FP32) -> CORRECT -> RES64 ...
+//
+// the same phases as the single-system kernels, with the same per-row
+// arithmetic.  A "step" executes the current phase of every active slot; a
+// phase is split into work units of 256 rows (four SELL-64 slices, one per
+// wavefront of a 256-thread block).  One persistent kernel runs the steps:
+// blocks claim units from the step's queue (agent-scope atomic), the last
+// unit of a slot's phase does the slot's scalar step (alpha / beta / exit
+// tests, deterministic partial sums in unit order) and advances its program,
+// and the last unit of the step builds the next step's queue and publishes
+// the step counter with release semantics; idle blocks wait for it with
+// acquire loads.  A block only waits while other running blocks finish
+// claimed units, so the grid makes progress whether or not all blocks are
+// resident.  The kernel returns when the step counter reaches the host's
+// limit, when no slot is active, or when a wait exceeds its spin bound (then
+// sched->error is set): every wave reaches one of these exits.
+//
+// Between launches the host admits new slots and collects finished ones.  A
+// new slot joins the queue the next time one is built (at the end of the
+// current step); only when no slot is running does a one-block kernel build
+// the current step's queue, so no unit already executed is ever scheduled
+// again.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/simplefem_amd.h"
+#include "device_common.hpp"
+#include "host_common.hpp"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+namespace sfa {
+namespace {
+
+using namespace dev;
+
+constexpr int kUnitBlock = 256;             // threads per block = rows per unit
+constexpr int kUnitSlices = kUnitBlock / 64;
+constexpr int kMaxSlots = 1024;
+constexpr unsigned kSpinLimit = 1u << 26;   // ~seconds of s_sleep; a bound, never reached
+
+enum Phase : int {
+    PH_RESID = 0, PH_FWD_R, PH_BWD_R, PH_DINIT,
+    PH_SPMV, PH_FWD, PH_BWD, PH_DIR,
+    PH_RES64, PH_CONVERT, PH_CORRECT,
+    PH_DONE
+};
+
+template <typename T>
+struct BMat {
+    SellViewT<T> full, lower, upper;
+    SellViewT<double> full64;       // FP64 copy of `full` (mixed precision)
+    const int* fslice;              // gf+1 slice offsets of the forward groups
+    const int* frow0;               // first row of each forward group (-1: row list)
+    const int* frows;               // rows per forward group
+    const int* bslice;
+    const int* brow0;
+    const int* brows;
+    const T* diag;
+    int gf, gb;
+    int n, n_active;
+    int full_slices, full_row0;
+    T omega, c2;
+};
+
+template <typename T>
+struct BSlot {
+    int mat;
+    int kind, group;
+    int units;
+    unsigned ticket;
+    int mixed;
+    int K, maxK, exit_reason, outer, max_outer, inner_total;
+    double zr, dAd, alpha, beta, tol2, rel2, target64, res64;
+    T *x, *r, *b, *d, *z, *q, *m;
+    double *x64, *b64, *r64;
+    double* partials;
+    double* grp;
+};
+
+struct BSched {
+    unsigned step;
+    unsigned limit;
+    int stop;
+    int error;
+    int nslots;
+    int pad;
+    int W[2];
+    unsigned claim[2];
+    unsigned done[2];
+    int start[2][kMaxSlots + 1];
+};
+
+template <typename T>
+struct BArgs {
+    BMat<T>* mats;
+    BSlot<T>* slots;
+    BSched* sched;
+};
+
+__device__ inline int cdiv(int a, int b) { return (a + b - 1) / b; }
+
+// Units of a phase; reduction phases always get at least one unit.
+template <typename T>
+__device__ int phase_units(const BMat<T>& M, int kind, int g)
+{
+    switch (kind) {
+    case PH_RESID: return cdiv(M.full_slices, kUnitSlices);
+    case PH_SPMV: return max(1, cdiv(M.full_slices, kUnitSlices));
+    case PH_RES64: return max(1, cdiv(M.full_slices, kUnitSlices));
+    case PH_FWD_R:
+    case PH_FWD: return cdiv(M.fslice[g + 1] - M.fslice[g], kUnitSlices);
+    case PH_BWD_R: return cdiv(M.bslice[g + 1] - M.bslice[g], kUnitSlices);
+    case PH_BWD: return max(1, cdiv(M.bslice[g + 1] - M.bslice[g], kUnitSlices));
+    case PH_DINIT: return max(1, cdiv(M.n_active, kUnitBlock));
+    case PH_DIR:
+    case PH_CONVERT:
+    case PH_CORRECT: return cdiv(M.n_active, kUnitBlock);
+    default: return 0;
+    }
+}
+
+// End of the T solve: mixed slots apply their correction, others stop.
+template <typename T>
+__device__ void finish_solve(BSlot<T>& S)
+{
+    const bool failed = S.exit_reason == SFA_EXIT_BREAKDOWN || S.exit_reason == SFA_EXIT_SMALL;
+    if (S.mixed && !failed) {
+        S.kind = PH_CORRECT;
+        S.group = 0;
+    } else {
+        S.kind = PH_DONE;
+    }
+}
+
+// Program order after a phase without a decision to make.
+template <typename T>
+__device__ void advance(BSlot<T>& S, const BMat<T>& M)
+{
+    const int g = S.group;
+    switch (S.kind) {
+    case PH_RESID:
+        if (M.gf > 0) S.kind = PH_FWD_R, S.group = 0;
+        else if (M.gb > 0) S.kind = PH_BWD_R, S.group = 0;
+        else S.kind = PH_DINIT;
+        break;
+    case PH_FWD_R:
+        if (g + 1 < M.gf) S.group = g + 1;
+        else if (M.gb > 0) S.kind = PH_BWD_R, S.group = 0;
+        else S.kind = PH_DINIT;
+        break;
+    case PH_BWD_R:
+        if (g + 1 < M.gb) S.group = g + 1;
+        else S.kind = PH_DINIT, S.group = 0;
+        break;
+    case PH_FWD:
+        if (g + 1 < M.gf) S.group = g + 1;
+        else S.kind = PH_BWD, S.group = 0;
+        break;
+    case PH_DIR: S.kind = PH_SPMV, S.group = 0; break;
+    case PH_CONVERT: S.kind = PH_RESID, S.group = 0; break;
+    case PH_CORRECT: S.kind = PH_RES64, S.group = 0; break;
+    default: break;
+    }
+}
+
+// Scalar step of a slot after its phase's last unit; `total` is the phase's
+// reduction (unit partials summed in unit order).
+template <typename T>
+__device__ void phase_complete(BSlot<T>& S, const BMat<T>& M, double total)
+{
+    switch (S.kind) {
+    case PH_DINIT:  // solver/sparsesolver.cpp:1156-1190
+        S.zr = total;
+        if (S.rel2 > 0.) S.tol2 = S.rel2 * ref_absolute(total);
+        if (ref_absolute(total) < S.tol2) {
+            S.exit_reason = SFA_EXIT_CONVERGED;
+            finish_solve(S);
+        } else {
+            S.kind = PH_SPMV, S.group = 0;
+        }
+        return;
+    case PH_SPMV: {  // :1215-1231
+        S.dAd = total;
+        const bool num_bad = !isfinite(S.zr) || S.zr < 0.;
+        const bool den_bad = !isfinite(total) || total < 0.;
+        const bool small = sqrt(ref_absolute(total)) < 2.220446049250313e-16;
+        if (num_bad || den_bad) {
+            S.exit_reason = SFA_EXIT_BREAKDOWN;
+            finish_solve(S);
+        } else if (small) {
+            S.exit_reason = SFA_EXIT_SMALL;
+            finish_solve(S);
+        } else {
+            S.alpha = S.zr / total;
+            if (M.gf > 0) S.kind = PH_FWD, S.group = 0;
+            else S.kind = PH_BWD, S.group = 0;
+        }
+        return;
+    }
+    case PH_BWD: {  // :1263-1307, summed per colour then over colours
+        S.grp[S.group] = total;
+        if (S.group + 1 < M.gb) {
+            S.group++;
+            return;
+        }
+        double zr_new = 0.;
+        for (int g = 0; g < M.gb; g++) zr_new += S.grp[g];
+        S.beta = zr_new / S.zr;
+        S.zr = zr_new;
+        S.K++;
+        if (S.mixed) S.inner_total++;
+        if (S.K >= S.maxK) {
+            S.exit_reason = SFA_EXIT_CAP;
+            finish_solve(S);
+        } else if (ref_absolute(zr_new) < S.tol2) {
+            S.exit_reason = SFA_EXIT_CONVERGED;
+            finish_solve(S);
+        } else {
+            S.kind = PH_DIR, S.group = 0;
+        }
+        return;
+    }
+    case PH_RES64:
+        S.res64 = sqrt(total);
+        if (!(S.res64 > S.target64) || S.outer >= S.max_outer) {
+            S.kind = PH_DONE;
+        } else {
+            S.kind = PH_CONVERT, S.group = 0;
+        }
+        return;
+    case PH_CONVERT:  // a new inner solve starts from x = 0
+        S.K = 0;
+        S.exit_reason = SFA_EXIT_CAP;
+        advance(S, M);
+        return;
+    case PH_CORRECT:
+        S.outer++;
+        advance(S, M);
+        return;
+    default:
+        advance(S, M);
+        return;
+    }
+}
+
+// One work unit: 256 rows of the slot's current phase.  Returns the block's
+// partial sum for reduction phases (valid in thread 0).
+template <typename T>
+__device__ double run_unit(BSlot<T>& S, const BMat<T>& M, int kind, int g, int j)
+{
+    const int lane = threadIdx.x & 63;
+    const int w = threadIdx.x >> 6;
+    double part = 0.;
+    switch (kind) {
+    case PH_RESID:
+    case PH_SPMV: {
+        const int local = j * kUnitSlices + w;
+        if (local < M.full_slices) {
+            const int row = group_row(M.full, local, local, lane, M.full_row0, M.n_active);
+            if (row >= 0) {
+                if (kind == PH_RESID) {
+                    S.r[row] = sell_row<true>(M.full, local, lane, S.x, S.b[row]);
+                } else {
+                    const T acc = sell_row<false>(M.full, local, lane, S.d, T(0));
+                    S.q[row] = acc;
+                    part = (double)(S.d[row] * acc);
+                }
+            }
+        }
+        break;
+    }
+    case PH_RES64: {
+        const int local = j * kUnitSlices + w;
+        if (local < M.full_slices) {
+            const int row = group_row(M.full64, local, local, lane, M.full_row0, M.n_active);
+            if (row >= 0) {
+                const double r = sell_row<true>(M.full64, local, lane, S.x64, S.b64[row]);
+                S.r64[row] = r;
+                part = r * r;
+            }
+        }
+        break;
+    }
+    case PH_FWD_R:
+    case PH_FWD: {
+        const int local = j * kUnitSlices + w;
+        const int slice = M.fslice[g] + local;
+        if (slice < M.fslice[g + 1]) {
+            const int row = group_row(M.lower, slice, local, lane, M.frow0[g], M.frows[g]);
+            if (row >= 0) {
+                T rr;
+                if (kind == PH_FWD) {
+                    const T alpha = (T)S.alpha;
+                    S.x[row] += alpha * S.d[row];
+                    rr = S.r[row] - alpha * S.q[row];
+                    S.r[row] = rr;
+                } else {
+                    rr = S.r[row];
+                }
+                const T acc = sell_row<true>(M.lower, slice, lane, S.m, rr);
+                S.m[row] = acc * M.omega / M.diag[row];
+            }
+        }
+        break;
+    }
+    case PH_BWD_R:
+    case PH_BWD: {
+        const int local = j * kUnitSlices + w;
+        const int slice = M.bslice[g] + local;
+        if (slice < M.bslice[g + 1]) {
+            const int row = group_row(M.upper, slice, local, lane, M.brow0[g], M.brows[g]);
+            if (row >= 0) {
+                const T dg = M.diag[row];
+                T t = S.m[row];
+                t *= dg;
+                t *= M.c2;
+                const T acc = sell_row<true>(M.upper, slice, lane, S.z, t);
+                const T zz = acc * M.omega / dg;
+                S.z[row] = zz;
+                if (kind == PH_BWD) part = (double)(zz * S.r[row]);
+            }
+        }
+        break;
+    }
+    case PH_DINIT: {
+        const int i = j * kUnitBlock + threadIdx.x;
+        if (i < M.n_active) {
+            const T zz = S.z[i];
+            S.d[i] = zz;
+            part = (double)(zz * S.r[i]);
+        }
+        break;
+    }
+    case PH_DIR: {
+        const int i = j * kUnitBlock + threadIdx.x;
+        if (i < M.n_active) S.d[i] = S.z[i] + (T)S.beta * S.d[i];
+        break;
+    }
+    case PH_CONVERT: {
+        const int i = j * kUnitBlock + threadIdx.x;
+        if (i < M.n_active) {
+            S.b[i] = (T)S.r64[i];
+            S.x[i] = T(0);
+            S.r[i] = T(0);
+        }
+        break;
+    }
+    case PH_CORRECT: {
+        const int i = j * kUnitBlock + threadIdx.x;
+        if (i < M.n_active) S.x64[i] += (double)S.x[i];
+        break;
+    }
+    default: break;
+    }
+    return part;
+}
+
+__device__ inline double block_sum(double v, double* ws)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    v = wave_sum(v);
+    __syncthreads();
+    if (lane == 0) ws[w] = v;
+    __syncthreads();
+    double t = 0.;
+    for (int i = 0; i < kUnitBlock / 64; i++) t += ws[i];
+    return t;
+}
+
+// Builds the work queue of step parity p from the slots' program counters,
+// skipping phases that have no units and no decision (one block).
+template <typename T>
+__device__ void build_schedule(const BArgs<T>& a, int p)
+{
+    __shared__ int counts[kUnitBlock];
+    BSched* sc = a.sched;
+    const int ns = sc->nslots;
+    const int per = (ns + kUnitBlock - 1) / kUnitBlock;
+    const int s0 = threadIdx.x * per;
+    int local = 0;
+    for (int s = s0; s < min(ns, s0 + per); s++) {
+        BSlot<T>& S = a.slots[s];
+        int units = 0;
+        if (S.kind != PH_DONE && S.mat >= 0) {
+            const BMat<T>& M = a.mats[S.mat];
+            for (int guard = 0; guard < 64; guard++) {
+                units = phase_units(M, S.kind, S.group);
+                if (units > 0 || S.kind == PH_DONE) break;
+                advance(S, M);  // a phase with no rows and nothing to decide
+            }
+        }
+        S.units = units;
+        S.ticket = 0;
+        local += units;
+        sc->start[p][s + 1] = local;  // slot-local prefix, fixed below
+    }
+    counts[threadIdx.x] = local;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int acc = 0;
+        for (int t = 0; t < kUnitBlock; t++) {
+            const int c = counts[t];
+            counts[t] = acc;
+            acc += c;
+        }
+        sc->W[p] = acc;
+        sc->claim[p] = 0;
+        sc->done[p] = 0;
+        sc->start[p][0] = 0;
+    }
+    __syncthreads();
+    const int base = counts[threadIdx.x];
+    for (int s = s0; s < min(ns, s0 + per); s++) sc->start[p][s + 1] += base;
+    // every thread's queue entries become visible with the step counter
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kUnitBlock) k_batch_schedule(BArgs<T> a)
+{
+    const int p = (int)(a.sched->step & 1u);
+    build_schedule(a, p);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kUnitBlock) k_batch_run(BArgs<T> a)
+{
+    __shared__ int sh_cmd, sh_slot, sh_unit, sh_slot_last, sh_step_last;
+    __shared__ unsigned sh_step;
+    __shared__ double ws[kUnitBlock / 64];
+    BSched* sc = a.sched;
+    for (;;) {
+        if (threadIdx.x == 0) {
+            int cmd = 0;  // 0 work, 1 retry, 2 exit
+            const unsigned step = __hip_atomic_load(&sc->step, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+            const int p = (int)(step & 1u);
+            if (step >= sc->limit || sc->stop || sc->error || sc->W[p] == 0) {
+                cmd = 2;
+            } else {
+                const unsigned u = __hip_atomic_fetch_add(&sc->claim[p], 1u, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT);
+                if (u >= (unsigned)sc->W[p]) {
+                    unsigned spins = 0;
+                    while (__hip_atomic_load(&sc->step, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == step) {
+                        __builtin_amdgcn_s_sleep(2);
+                        if (++spins > kSpinLimit) {
+                            __hip_atomic_store(&sc->error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                            break;
+                        }
+                    }
+                    cmd = 1;
+                } else {
+                    int lo = 0, hi = sc->nslots;  // largest s with start[s] <= u
+                    while (hi - lo > 1) {
+                        const int mid = (lo + hi) >> 1;
+                        if ((unsigned)sc->start[p][mid] <= u) lo = mid;
+                        else hi = mid;
+                    }
+                    sh_slot = lo;
+                    sh_unit = (int)(u - (unsigned)sc->start[p][lo]);
+                    sh_step = step;
+                }
+            }
+            sh_cmd = cmd;
+        }
+        __syncthreads();
+        const int cmd = sh_cmd;
+        if (cmd == 2) return;
+        if (cmd == 1) {
+            __syncthreads();
+            continue;
+        }
+        const int s = sh_slot, j = sh_unit;
+        const unsigned step = sh_step;
+        const int p = (int)(step & 1u);
+        BSlot<T>& S = a.slots[s];
+        const BMat<T>& M = a.mats[S.mat];
+        const int kind = S.kind, g = S.group;
+        const double part = block_sum(run_unit(S, M, kind, g, j), ws);
+        // slot phase bookkeeping
+        if (threadIdx.x == 0) {
+            S.partials[j] = part;
+            const unsigned t = __hip_atomic_fetch_add(&S.ticket, 1u, __ATOMIC_ACQ_REL,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+            sh_slot_last = (t + 1 == (unsigned)S.units);
+        }
+        __syncthreads();
+        if (sh_slot_last) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            double acc = 0.;
+            for (int i = threadIdx.x; i < S.units; i += kUnitBlock) acc += S.partials[i];
+            const double total = block_sum(acc, ws);
+            if (threadIdx.x == 0) phase_complete(S, M, total);
+        }
+        // step bookkeeping
+        if (threadIdx.x == 0) {
+            const unsigned d = __hip_atomic_fetch_add(&sc->done[p], 1u, __ATOMIC_ACQ_REL,
+                                                      __HIP_MEMORY_SCOPE_AGENT);
+            sh_step_last = (d + 1 == (unsigned)sc->W[p]);
+        }
+        __syncthreads();
+        if (sh_step_last) {
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            build_schedule(a, p ^ 1);
+            // An increment, not a store of step + 1: a block that read the step
+            // counter long before its claim may be finishing a later step of the
+            // same parity, whose number it does not know.
+            if (threadIdx.x == 0)
+                __hip_atomic_fetch_add(&sc->step, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        __syncthreads();
+    }
+}
+
+// ====================================================================== host
+
+constexpr int kRunBlocks = 1024;  // persistent grid: 4 blocks of 256 threads per CU
+
+template <typename T>
+struct HostMat {
+    bool live = false;
+    int n = 0, n_active = 0;
+    long long bytes = 0;
+    std::vector<int> perm;
+    DevPartT<T> full, lower, upper;
+    DevPartT<double> full64;
+    int *fslice = nullptr, *frow0 = nullptr, *frows = nullptr;
+    int *bslice = nullptr, *brow0 = nullptr, *brows = nullptr;
+    T* diag = nullptr;
+    int max_units = 1, gb = 0;
+    int slots_in_use = 0;
+
+    void release()
+    {
+        full.release();
+        lower.release();
+        upper.release();
+        full64.release();
+        for (void* p : {(void*)fslice, (void*)frow0, (void*)frows, (void*)bslice, (void*)brow0,
+                        (void*)brows, (void*)diag})
+            dev_free(p);
+        fslice = frow0 = frows = bslice = brow0 = brows = nullptr;
+        diag = nullptr;
+        live = false;
+    }
+};
+
+// Vectors of one slot (device order, n+1 entries each).
+template <typename T>
+struct SlotVectors {
+    T *x = nullptr, *r = nullptr, *b = nullptr, *d = nullptr, *z = nullptr, *q = nullptr,
+      *m = nullptr;
+    double *x64 = nullptr, *b64 = nullptr, *r64 = nullptr, *partials = nullptr, *grp = nullptr;
+    long long bytes = 0;
+    void release()
+    {
+        for (void* p : {(void*)x, (void*)r, (void*)b, (void*)d, (void*)z, (void*)q, (void*)m,
+                        (void*)x64, (void*)b64, (void*)r64, (void*)partials, (void*)grp})
+            dev_free(p);
+        *this = SlotVectors();
+    }
+};
+
+int* upload_ints(const std::vector<int>& v)
+{
+    int* p = dev_alloc<int>(v.size());
+    if (!v.empty()) SFA_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
+    return p;
+}
+
+template <typename T>
+struct Engine {
+    bool mixed = false;
+    int max_slots = 0;
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<HostMat<T>> mats;
+    std::vector<BSlot<T>> slots;              // host mirror
+    std::vector<SlotVectors<T>> vecs;
+    std::vector<int> slot_used;
+    BMat<T>* d_mats = nullptr;
+    int d_mats_cap = 0;
+    BSlot<T>* d_slots = nullptr;
+    BSched* d_sched = nullptr;
+    BSched* h_sched = nullptr;                // pinned
+    unsigned step = 0;
+
+    ~Engine() { release(); }
+
+    void release()
+    {
+        if (stream) (void)hipStreamSynchronize(stream);
+        for (auto& v : vecs) v.release();
+        for (auto& m : mats) m.release();
+        dev_free(d_mats);
+        dev_free(d_slots);
+        dev_free(d_sched);
+        if (h_sched) (void)hipHostFree(h_sched);
+        if (stream) (void)hipStreamDestroy(stream);
+        d_mats = nullptr;
+        d_slots = nullptr;
+        d_sched = nullptr;
+        h_sched = nullptr;
+        stream = nullptr;
+    }
+
+    void create(int mixed_, int max_slots_)
+    {
+        if (max_slots_ <= 0 || max_slots_ > kMaxSlots)
+            throw Error{SFA_ERR_INVALID, "max_slots must be in 1..1024"};
+        ensure_device();
+        SFA_HIP(hipGetDevice(&device));
+        mixed = mixed_ != 0;
+        max_slots = max_slots_;
+        SFA_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        slots.assign(max_slots, BSlot<T>{});
+        for (auto& sl : slots) {
+            sl.mat = -1;
+            sl.kind = PH_DONE;
+        }
+        vecs.assign(max_slots, SlotVectors<T>{});
+        slot_used.assign(max_slots, 0);
+        d_slots = dev_alloc<BSlot<T>>(max_slots);
+        SFA_HIP(hipMemcpy(d_slots, slots.data(), sizeof(BSlot<T>) * max_slots, hipMemcpyHostToDevice));
+        d_sched = dev_alloc<BSched>(1);
+        SFA_HIP(hipHostMalloc((void**)&h_sched, sizeof(BSched), hipHostMallocDefault));
+        std::memset(h_sched, 0, sizeof(BSched));
+        h_sched->nslots = max_slots;
+        SFA_HIP(hipMemcpy(d_sched, h_sched, sizeof(BSched), hipMemcpyHostToDevice));
+    }
+
+    BArgs<T> args() const { return BArgs<T>{d_mats, d_slots, d_sched}; }
+
+    int add_matrix(int n, const int* A, const int* C, const double* V, const double* diagonal,
+                   double omega, int nc, const int* F, const int* B, const int* R)
+    {
+        Plan plan;
+        bool colouring_error = false;
+        std::string err = build_plan(plan, SFA_METHOD_RAINBOW, n, A, C, V, diagonal, omega, nc, F,
+                                     B, R, &colouring_error);
+        if (!err.empty()) throw Error{colouring_error ? SFA_ERR_COLOURING : SFA_ERR_INVALID, err};
+        SFA_HIP(hipSetDevice(device));
+        int id = -1;
+        for (size_t i = 0; i < mats.size(); i++)
+            if (!mats[i].live) id = (int)i;
+        if (id < 0) {
+            id = (int)mats.size();
+            mats.emplace_back();
+        }
+        HostMat<T>& M = mats[id];
+        M = HostMat<T>();
+        M.live = true;
+        M.n = n;
+        M.n_active = plan.n_active;
+        M.perm = plan.perm;
+        M.full.upload(plan.full);
+        M.lower.upload(plan.lower);
+        M.upper.upload(plan.upper);
+        if (mixed) M.full64.upload(plan.full);
+        M.fslice = upload_ints(plan.lower.group_slice);
+        M.frow0 = upload_ints(plan.lower.group_row0);
+        std::vector<int> fr(plan.lower.group_rows.begin(), plan.lower.group_rows.end());
+        M.frows = upload_ints(fr);
+        M.bslice = upload_ints(plan.upper.group_slice);
+        M.brow0 = upload_ints(plan.upper.group_row0);
+        std::vector<int> br(plan.upper.group_rows.begin(), plan.upper.group_rows.end());
+        M.brows = upload_ints(br);
+        M.diag = dev_alloc<T>(n + 1);
+        std::vector<T> dg(plan.diag.begin(), plan.diag.end());
+        dg.push_back(T(0));
+        SFA_HIP(hipMemcpy(M.diag, dg.data(), dg.size() * sizeof(T), hipMemcpyHostToDevice));
+        M.gb = plan.upper.ngroups();
+        int mu = std::max(1, (plan.full.nslices + kUnitSlices - 1) / kUnitSlices);
+        mu = std::max(mu, (plan.n_active + kUnitBlock - 1) / kUnitBlock);
+        for (const SellPart* part : {&plan.lower, &plan.upper})
+            for (int g = 0; g < part->ngroups(); g++)
+                mu = std::max(mu, (part->group_slice[g + 1] - part->group_slice[g] +
+                                   kUnitSlices - 1) / kUnitSlices);
+        M.max_units = mu;
+        M.bytes = (long long)(plan.full.vals.size() + plan.lower.vals.size() + plan.upper.vals.size()) *
+                      (sizeof(T) + 4) + (long long)(n + 1) * sizeof(T) +
+                  (mixed ? (long long)plan.full.vals.size() * 12 : 0);
+
+        BMat<T> bm{};
+        bm.full = M.full.view();
+        bm.lower = M.lower.view();
+        bm.upper = M.upper.view();
+        bm.full64 = mixed ? M.full64.view() : SellViewT<double>{nullptr, nullptr, nullptr, nullptr};
+        bm.fslice = M.fslice;
+        bm.frow0 = M.frow0;
+        bm.frows = M.frows;
+        bm.bslice = M.bslice;
+        bm.brow0 = M.brow0;
+        bm.brows = M.brows;
+        bm.diag = M.diag;
+        bm.gf = plan.lower.ngroups();
+        bm.gb = plan.upper.ngroups();
+        bm.n = n;
+        bm.n_active = plan.n_active;
+        bm.full_slices = plan.full.nslices;
+        bm.full_row0 = plan.full.group_row0.empty() ? 0 : plan.full.group_row0[0];
+        bm.omega = T(omega);
+        bm.c2 = T((2. - omega) / omega);
+        if (id >= d_mats_cap) {  // grow the device table (no kernel is running)
+            const int cap = std::max(16, 2 * d_mats_cap);
+            BMat<T>* nm = dev_alloc<BMat<T>>(cap);
+            if (d_mats) {
+                SFA_HIP(hipMemcpy(nm, d_mats, sizeof(BMat<T>) * d_mats_cap, hipMemcpyDeviceToDevice));
+                dev_free(d_mats);
+            }
+            d_mats = nm;
+            d_mats_cap = cap;
+        }
+        SFA_HIP(hipMemcpy(d_mats + id, &bm, sizeof(BMat<T>), hipMemcpyHostToDevice));
+        return id;
+    }
+
+    void remove_matrix(int id)
+    {
+        if (id < 0 || id >= (int)mats.size() || !mats[id].live)
+            throw Error{SFA_ERR_INVALID, "no such matrix"};
+        if (mats[id].slots_in_use > 0)
+            throw Error{SFA_ERR_INVALID, "matrix still has slots; collect them first"};
+        SFA_HIP(hipStreamSynchronize(stream));
+        mats[id].release();
+    }
+
+    long long bytes_in_use() const
+    {
+        long long b = 0;
+        for (const auto& m : mats)
+            if (m.live) b += m.bytes;
+        for (const auto& v : vecs) b += v.bytes;
+        return b;
+    }
+
+    int submit(int id, const double* bvec, const double* x0, double precision, double inner_rtol,
+               int max_outer)
+    {
+        if (id < 0 || id >= (int)mats.size() || !mats[id].live)
+            throw Error{SFA_ERR_INVALID, "no such matrix"};
+        if (!bvec) throw Error{SFA_ERR_INVALID, "null right-hand side"};
+        if (!(precision > 0)) throw Error{SFA_ERR_INVALID, "precision must be positive"};
+        if (mixed && !(inner_rtol > 0 && inner_rtol < 1))
+            throw Error{SFA_ERR_INVALID, "inner_rtol must be in (0, 1)"};
+        int s = -1;
+        for (int i = 0; i < max_slots; i++)
+            if (!slot_used[i]) {
+                s = i;
+                break;
+            }
+        if (s < 0) throw Error{SFA_ERR_INVALID, "no free slot"};
+        HostMat<T>& M = mats[id];
+        const int n = M.n;
+        SFA_HIP(hipSetDevice(device));
+        SlotVectors<T>& v = vecs[s];
+        v.release();
+        T** tv[] = {&v.x, &v.r, &v.b, &v.d, &v.z, &v.q, &v.m};
+        for (T** p : tv) *p = dev_alloc<T>(n + 1);
+        v.bytes = 7LL * (n + 1) * sizeof(T);
+        if (mixed) {
+            v.x64 = dev_alloc<double>(n + 1);
+            v.b64 = dev_alloc<double>(n + 1);
+            v.r64 = dev_alloc<double>(n + 1);
+            v.bytes += 3LL * (n + 1) * 8;
+        }
+        v.partials = dev_alloc<double>(M.max_units);
+        v.grp = dev_alloc<double>(std::max(1, M.gb));
+        for (T* p : {v.x, v.r, v.b, v.d, v.z, v.q, v.m})
+            SFA_HIP(hipMemsetAsync(p, 0, (n + 1) * sizeof(T), stream));
+        // inputs in device order; tolerance as the reference computes it
+        std::vector<double> bd(n), xd(n, 0.0);
+        double bb = 0.;
+        for (int i = 0; i < n; i++) bb += bvec[i] * bvec[i];
+        for (int p = 0; p < n; p++) bd[p] = bvec[M.perm[p]];
+        if (x0)
+            for (int p = 0; p < n; p++) xd[p] = x0[M.perm[p]];
+        double tol = precision * std::sqrt(bb);
+        const double floor_ = 1.4901161193847656e-08;
+        if (!(tol >= floor_) && tol <= floor_) tol = floor_;
+        BSlot<T> sl{};
+        sl.mat = id;
+        sl.mixed = mixed ? 1 : 0;
+        sl.maxK = n;
+        sl.exit_reason = SFA_EXIT_CAP;
+        sl.max_outer = max_outer;
+        sl.x = v.x; sl.r = v.r; sl.b = v.b; sl.d = v.d; sl.z = v.z; sl.q = v.q; sl.m = v.m;
+        sl.x64 = v.x64; sl.b64 = v.b64; sl.r64 = v.r64;
+        sl.partials = v.partials;
+        sl.grp = v.grp;
+        if (mixed) {
+            SFA_HIP(hipMemsetAsync(v.r64, 0, (n + 1) * sizeof(double), stream));
+            SFA_HIP(hipMemcpyAsync(v.b64, bd.data(), n * sizeof(double), hipMemcpyHostToDevice, stream));
+            SFA_HIP(hipMemcpyAsync(v.x64, xd.data(), n * sizeof(double), hipMemcpyHostToDevice, stream));
+            SFA_HIP(hipMemsetAsync(v.x64 + n, 0, sizeof(double), stream));
+            sl.kind = PH_RES64;
+            sl.target64 = tol;
+            sl.rel2 = inner_rtol * inner_rtol;
+        } else {
+            std::vector<T> bt(bd.begin(), bd.end()), xt(xd.begin(), xd.end());
+            SFA_HIP(hipMemcpyAsync(v.b, bt.data(), n * sizeof(T), hipMemcpyHostToDevice, stream));
+            SFA_HIP(hipMemcpyAsync(v.x, xt.data(), n * sizeof(T), hipMemcpyHostToDevice, stream));
+            sl.kind = PH_RESID;
+            sl.tol2 = tol * tol;
+        }
+        SFA_HIP(hipMemcpyAsync(d_slots + s, &sl, sizeof(BSlot<T>), hipMemcpyHostToDevice, stream));
+        SFA_HIP(hipStreamSynchronize(stream));  // host vectors go out of scope
+        slots[s] = sl;
+        slot_used[s] = 1;
+        M.slots_in_use++;
+        return s;
+    }
+
+    void read_sched()
+    {
+        SFA_HIP(hipMemcpyAsync(h_sched, d_sched, sizeof(BSched), hipMemcpyDeviceToHost, stream));
+        SFA_HIP(hipStreamSynchronize(stream));
+        step = h_sched->step;
+    }
+
+    // Runs the persistent kernel for at most max_steps steps (0: until idle).
+    // Returns the number of slots still running.
+    int run(int max_steps)
+    {
+        SFA_HIP(hipSetDevice(device));
+        read_sched();
+        if (h_sched->error) throw Error{SFA_ERR_HIP, "batch engine stopped on a wait bound"};
+        const int p = (int)(step & 1u);
+        if (h_sched->W[p] == 0) {  // idle: nothing has run in this step, build its queue here
+            k_batch_schedule<T><<<1, kUnitBlock, 0, stream>>>(args());
+            SFA_HIP(hipGetLastError());
+        }
+        const unsigned limit = max_steps > 0 ? step + (unsigned)max_steps : 0xFFFFFFF0u;
+        SFA_HIP(hipMemcpyAsync(&d_sched->limit, &limit, sizeof(unsigned), hipMemcpyHostToDevice,
+                               stream));
+        k_batch_run<T><<<kRunBlocks, kUnitBlock, 0, stream>>>(args());
+        SFA_HIP(hipGetLastError());
+        read_sched();
+        if (h_sched->error) throw Error{SFA_ERR_HIP, "batch engine stopped on a wait bound"};
+        return running();
+    }
+
+    int running()
+    {
+        SFA_HIP(hipMemcpyAsync(slots.data(), d_slots, sizeof(BSlot<T>) * max_slots,
+                               hipMemcpyDeviceToHost, stream));
+        SFA_HIP(hipStreamSynchronize(stream));
+        int n = 0;
+        for (int s = 0; s < max_slots; s++)
+            if (slot_used[s] && slots[s].kind != PH_DONE) n++;
+        return n;
+    }
+
+    int finished(int* out, int cap)
+    {
+        running();
+        int n = 0;
+        for (int s = 0; s < max_slots && n < cap; s++)
+            if (slot_used[s] && slots[s].kind == PH_DONE) out[n++] = s;
+        return n;
+    }
+
+    void collect(int s, double* x, sfa_batch_info* info)
+    {
+        if (s < 0 || s >= max_slots || !slot_used[s]) throw Error{SFA_ERR_INVALID, "no such slot"};
+        SFA_HIP(hipMemcpyAsync(&slots[s], d_slots + s, sizeof(BSlot<T>), hipMemcpyDeviceToHost, stream));
+        SFA_HIP(hipStreamSynchronize(stream));
+        const BSlot<T>& sl = slots[s];
+        if (sl.kind != PH_DONE) throw Error{SFA_ERR_INVALID, "slot has not finished"};
+        HostMat<T>& M = mats[sl.mat];
+        const int n = M.n;
+        if (x) {
+            std::vector<double> xd(n);
+            if (mixed) {
+                SFA_HIP(hipMemcpy(xd.data(), sl.x64, n * sizeof(double), hipMemcpyDeviceToHost));
+            } else {
+                std::vector<T> xt(n);
+                SFA_HIP(hipMemcpy(xt.data(), sl.x, n * sizeof(T), hipMemcpyDeviceToHost));
+                for (int i = 0; i < n; i++) xd[i] = (double)xt[i];
+            }
+            for (int p = 0; p < n; p++) x[M.perm[p]] = xd[p];
+        }
+        if (info) {
+            sfa_batch_info o{};
+            o.iterations = mixed ? sl.inner_total : sl.K;
+            o.outer = sl.outer;
+            o.exit_reason = sl.exit_reason;
+            o.zr = sl.zr;
+            o.residual_norm = mixed ? sl.res64 : std::sqrt(sl.zr);
+            o.success = mixed ? (sl.res64 <= sl.target64 ? 1 : 0)
+                              : (std::sqrt(sl.zr) < std::sqrt(sl.tol2) ? 1 : 0);
+            *info = o;
+        }
+        BSlot<T> cleared{};
+        cleared.mat = -1;
+        cleared.kind = PH_DONE;
+        SFA_HIP(hipMemcpy(d_slots + s, &cleared, sizeof(BSlot<T>), hipMemcpyHostToDevice));
+        slots[s] = cleared;
+        slot_used[s] = 0;
+        vecs[s].release();
+        M.slots_in_use--;
+    }
+};
+
+}  // namespace
+
+struct BatchHandle {
+    std::unique_ptr<Engine<double>> e64;
+    std::unique_ptr<Engine<float>> e32;
+    template <class Fn>
+    void visit(Fn&& fn)
+    {
+        if (e64) fn(*e64);
+        else fn(*e32);
+    }
+};
+
+}  // namespace sfa
+
+struct sfa_batch_s {
+    sfa::BatchHandle h;
+};
+
+extern "C" {
+
+int sfa_batch_create(int precision_mode, int max_slots, sfa_batch* out)
+{
+    return sfa::guarded([&] {
+        if (!out) throw sfa::Error{SFA_ERR_INVALID, "null output handle"};
+        *out = nullptr;
+        std::unique_ptr<sfa_batch_s> b(new sfa_batch_s());
+        if (precision_mode == SFA_BATCH_FP64) {
+            b->h.e64.reset(new sfa::Engine<double>());
+            b->h.e64->create(0, max_slots);
+        } else if (precision_mode == SFA_BATCH_MIXED) {
+            b->h.e32.reset(new sfa::Engine<float>());
+            b->h.e32->create(1, max_slots);
+        } else {
+            throw sfa::Error{SFA_ERR_INVALID, "precision_mode must be SFA_BATCH_FP64 or SFA_BATCH_MIXED"};
+        }
+        *out = b.release();
+    });
+}
+
+int sfa_batch_add_matrix(sfa_batch b, int N, const int* csrrows, const int* csrcolumns,
+                         const double* csrvalues, const double* diagonal, double omega,
+                         int num_colors, const int* F, const int* B, const int* R, int* matrix_id)
+{
+    return sfa::guarded([&] {
+        if (!b || !matrix_id) throw sfa::Error{SFA_ERR_INVALID, "null argument"};
+        b->h.visit([&](auto& e) {
+            *matrix_id = e.add_matrix(N, csrrows, csrcolumns, csrvalues, diagonal, omega,
+                                      num_colors, F, B, R);
+        });
+    });
+}
+
+int sfa_batch_remove_matrix(sfa_batch b, int matrix_id)
+{
+    return sfa::guarded([&] {
+        if (!b) throw sfa::Error{SFA_ERR_INVALID, "null batch"};
+        b->h.visit([&](auto& e) { e.remove_matrix(matrix_id); });
+    });
+}
+
+int sfa_batch_submit(sfa_batch b, int matrix_id, const double* rhs, const double* x0,
+                     double precision, double inner_rtol, int max_outer, int* slot)
+{
+    return sfa::guarded([&] {
+        if (!b || !slot) throw sfa::Error{SFA_ERR_INVALID, "null argument"};
+        b->h.visit([&](auto& e) {
+            *slot = e.submit(matrix_id, rhs, x0, precision, inner_rtol, max_outer);
+        });
+    });
+}
+
+int sfa_batch_run(sfa_batch b, int max_steps, int* running)
+{
+    return sfa::guarded([&] {
+        if (!b) throw sfa::Error{SFA_ERR_INVALID, "null batch"};
+        b->h.visit([&](auto& e) {
+            const int r = e.run(max_steps);
+            if (running) *running = r;
+        });
+    });
+}
+
+int sfa_batch_finished(sfa_batch b, int* slots, int capacity, int* count)
+{
+    return sfa::guarded([&] {
+        if (!b || !count || (capacity > 0 && !slots)) throw sfa::Error{SFA_ERR_INVALID, "null argument"};
+        b->h.visit([&](auto& e) { *count = e.finished(slots, capacity); });
+    });
+}
+
+int sfa_batch_collect(sfa_batch b, int slot, double* x, sfa_batch_info* info)
+{
+    return sfa::guarded([&] {
+        if (!b) throw sfa::Error{SFA_ERR_INVALID, "null batch"};
+        b->h.visit([&](auto& e) { e.collect(slot, x, info); });
+    });
+}
+
+int sfa_batch_memory(sfa_batch b, long long* bytes)
+{
+    return sfa::guarded([&] {
+        if (!b || !bytes) throw sfa::Error{SFA_ERR_INVALID, "null argument"};
+        b->h.visit([&](auto& e) { *bytes = e.bytes_in_use(); });
+    });
+}
+
+int sfa_batch_destroy(sfa_batch b)
+{
+    return sfa::guarded([&] { delete b; });
+}
+
+}  // extern "C"
+

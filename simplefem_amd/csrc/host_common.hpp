@@ -1,0 +1,163 @@
+// host_common.hpp -- host-side helpers shared by the library's sources:
+// error type and HIP checks, device allocation, SELL parts in device memory.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/simplefem_amd.h"
+#include "kernels.hpp"
+#include "plan.hpp"
+
+namespace sfa {
+
+// message of the calling thread's last failed C-ABI call
+std::string& last_error();
+
+
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define SFA_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess) {                                                        \
+            int code_ = e_ == hipErrorOutOfMemory ? SFA_ERR_OUT_OF_MEMORY : SFA_ERR_HIP; \
+            throw Error{code_, std::string(#call) + ": " + hipGetErrorString(e_)};     \
+        }                                                                              \
+    } while (0)
+
+template <class T>
+T* dev_alloc(size_t count)
+{
+    void* p = nullptr;
+    if (count == 0) count = 1;
+    SFA_HIP(hipMalloc(&p, count * sizeof(T)));
+    return static_cast<T*>(p);
+}
+
+inline void dev_free(void* p)
+{
+    if (p) (void)hipFree(p);
+}
+
+inline void ensure_device()
+{
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess || count == 0)
+        throw Error{SFA_ERR_NO_DEVICE, "no HIP device available"};
+}
+
+template <typename T>
+struct DevPartT {
+    int* off = nullptr;
+    int* rows = nullptr;
+    int* cols = nullptr;
+    T* vals = nullptr;
+    int nslices = 0;
+    std::vector<int> group_slice;
+    std::vector<long long> group_entries, group_rows;
+    std::vector<int> group_row0;
+    long long stored = 0;
+    long long entries = 0;
+    long long nrows = 0;
+
+    void upload(const SellPart& p)
+    {
+        nslices = p.nslices;
+        group_slice = p.group_slice;
+        group_entries = p.group_entries;
+        group_rows = p.group_rows;
+        group_row0 = p.group_row0;
+        stored = (long long)p.vals.size();
+        entries = p.entries;
+        nrows = p.rows;
+        off = dev_alloc<int>(p.slice_off.size());
+        rows = dev_alloc<int>(p.slice_rows.size());
+        cols = dev_alloc<int>(p.cols.size());
+        vals = dev_alloc<T>(p.vals.size());
+        SFA_HIP(hipMemcpy(off, p.slice_off.data(), p.slice_off.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!p.slice_rows.empty())
+            SFA_HIP(hipMemcpy(rows, p.slice_rows.data(), p.slice_rows.size() * sizeof(int), hipMemcpyHostToDevice));
+        if (!p.cols.empty()) {
+            SFA_HIP(hipMemcpy(cols, p.cols.data(), p.cols.size() * sizeof(int), hipMemcpyHostToDevice));
+            std::vector<T> tv(p.vals.begin(), p.vals.end());
+            SFA_HIP(hipMemcpy(vals, tv.data(), tv.size() * sizeof(T), hipMemcpyHostToDevice));
+        }
+    }
+    void release()
+    {
+        dev_free(off);
+        dev_free(rows);
+        dev_free(cols);
+        dev_free(vals);
+        off = rows = cols = nullptr;
+        vals = nullptr;
+    }
+    SellViewT<T> view() const { return SellViewT<T>{off, rows, cols, vals}; }
+    int ngroups() const { return (int)group_slice.size() - 1; }
+    int group_begin(int g) const { return group_slice[g]; }
+    int group_count(int g) const { return group_slice[g + 1] - group_slice[g]; }
+    int row0(int g) const { return group_row0[g]; }
+    int nrows_of(int g) const { return (int)group_rows[g]; }
+    // Algorithmic bytes of one launch over group g: (sizeof(T) + 4) B per
+    // entry (value + column), 4 B per row id, `per_row` B of vector traffic
+    // per row (callers give it for T = double and it scales with sizeof(T)).
+    double bytes(int g, double per_row) const
+    {
+        return (sizeof(T) + 4.0) * group_entries[g] + (4.0 + per_row * sizeof(T) / 8.0) * group_rows[g];
+    }
+    double bytes_all(double per_row) const
+    {
+        return (sizeof(T) + 4.0) * entries + (4.0 + per_row * sizeof(T) / 8.0) * nrows;
+    }
+};
+using DevPart = DevPartT<double>;
+
+// Per-launch HIP events of a profiled solve.
+struct Profiler {
+    std::vector<hipEvent_t> pool;
+    std::vector<int> klass;
+    std::vector<double> bytes;
+    ~Profiler()
+    {
+        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+    }
+    hipEvent_t event(size_t i)
+    {
+        while (pool.size() <= i) {
+            hipEvent_t e;
+            SFA_HIP(hipEventCreate(&e));
+            pool.push_back(e);
+        }
+        return pool[i];
+    }
+};
+
+
+// Runs fn, turning exceptions into the C-ABI status and message.
+template <class Fn>
+int guarded(Fn&& fn)
+{
+    try {
+        fn();
+        last_error().clear();
+        return SFA_OK;
+    } catch (const Error& e) {
+        last_error() = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        last_error() = "host allocation failed";
+        return SFA_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception& e) {
+        last_error() = e.what();
+        return SFA_ERR_INVALID;
+    }
+}
+
+}  // namespace sfa

@@ -17,135 +17,19 @@
 #include <vector>
 
 #include "../../include/simplefem_amd.h"
+#include "host_common.hpp"
 #include "kernels.hpp"
 #include "plan.hpp"
 
 namespace sfa {
+
+std::string& last_error()
+{
+    thread_local std::string msg;
+    return msg;
+}
+
 namespace {
-
-thread_local std::string g_last_error;
-
-struct Error {
-    int code;
-    std::string msg;
-};
-
-#define SFA_HIP(call)                                                                  \
-    do {                                                                               \
-        hipError_t e_ = (call);                                                        \
-        if (e_ != hipSuccess) {                                                        \
-            int code_ = e_ == hipErrorOutOfMemory ? SFA_ERR_OUT_OF_MEMORY : SFA_ERR_HIP; \
-            throw Error{code_, std::string(#call) + ": " + hipGetErrorString(e_)};     \
-        }                                                                              \
-    } while (0)
-
-template <class T>
-T* dev_alloc(size_t count)
-{
-    void* p = nullptr;
-    if (count == 0) count = 1;
-    SFA_HIP(hipMalloc(&p, count * sizeof(T)));
-    return static_cast<T*>(p);
-}
-
-void dev_free(void* p)
-{
-    if (p) (void)hipFree(p);
-}
-
-void ensure_device()
-{
-    int count = 0;
-    hipError_t e = hipGetDeviceCount(&count);
-    if (e != hipSuccess || count == 0)
-        throw Error{SFA_ERR_NO_DEVICE, "no HIP device available"};
-}
-
-template <typename T>
-struct DevPartT {
-    int* off = nullptr;
-    int* rows = nullptr;
-    int* cols = nullptr;
-    T* vals = nullptr;
-    int nslices = 0;
-    std::vector<int> group_slice;
-    std::vector<long long> group_entries, group_rows;
-    std::vector<int> group_row0;
-    long long stored = 0;
-    long long entries = 0;
-    long long nrows = 0;
-
-    void upload(const SellPart& p)
-    {
-        nslices = p.nslices;
-        group_slice = p.group_slice;
-        group_entries = p.group_entries;
-        group_rows = p.group_rows;
-        group_row0 = p.group_row0;
-        stored = (long long)p.vals.size();
-        entries = p.entries;
-        nrows = p.rows;
-        off = dev_alloc<int>(p.slice_off.size());
-        rows = dev_alloc<int>(p.slice_rows.size());
-        cols = dev_alloc<int>(p.cols.size());
-        vals = dev_alloc<T>(p.vals.size());
-        SFA_HIP(hipMemcpy(off, p.slice_off.data(), p.slice_off.size() * sizeof(int), hipMemcpyHostToDevice));
-        if (!p.slice_rows.empty())
-            SFA_HIP(hipMemcpy(rows, p.slice_rows.data(), p.slice_rows.size() * sizeof(int), hipMemcpyHostToDevice));
-        if (!p.cols.empty()) {
-            SFA_HIP(hipMemcpy(cols, p.cols.data(), p.cols.size() * sizeof(int), hipMemcpyHostToDevice));
-            std::vector<T> tv(p.vals.begin(), p.vals.end());
-            SFA_HIP(hipMemcpy(vals, tv.data(), tv.size() * sizeof(T), hipMemcpyHostToDevice));
-        }
-    }
-    void release()
-    {
-        dev_free(off);
-        dev_free(rows);
-        dev_free(cols);
-        dev_free(vals);
-        off = rows = cols = nullptr;
-        vals = nullptr;
-    }
-    SellViewT<T> view() const { return SellViewT<T>{off, rows, cols, vals}; }
-    int ngroups() const { return (int)group_slice.size() - 1; }
-    int group_begin(int g) const { return group_slice[g]; }
-    int group_count(int g) const { return group_slice[g + 1] - group_slice[g]; }
-    int row0(int g) const { return group_row0[g]; }
-    int nrows_of(int g) const { return (int)group_rows[g]; }
-    // Algorithmic bytes of one launch over group g: (sizeof(T) + 4) B per
-    // entry (value + column), 4 B per row id, `per_row` B of vector traffic
-    // per row (callers give it for T = double and it scales with sizeof(T)).
-    double bytes(int g, double per_row) const
-    {
-        return (sizeof(T) + 4.0) * group_entries[g] + (4.0 + per_row * sizeof(T) / 8.0) * group_rows[g];
-    }
-    double bytes_all(double per_row) const
-    {
-        return (sizeof(T) + 4.0) * entries + (4.0 + per_row * sizeof(T) / 8.0) * nrows;
-    }
-};
-using DevPart = DevPartT<double>;
-
-// Per-launch HIP events of a profiled solve.
-struct Profiler {
-    std::vector<hipEvent_t> pool;
-    std::vector<int> klass;
-    std::vector<double> bytes;
-    ~Profiler()
-    {
-        for (hipEvent_t e : pool) (void)hipEventDestroy(e);
-    }
-    hipEvent_t event(size_t i)
-    {
-        while (pool.size() <= i) {
-            hipEvent_t e;
-            SFA_HIP(hipEventCreate(&e));
-            pool.push_back(e);
-        }
-        return pool[i];
-    }
-};
 
 const char* method_title(int method)
 {
@@ -810,31 +694,12 @@ struct sfa_system_s {
 
 namespace {
 
-template <class Fn>
-int guarded(Fn&& fn)
-{
-    try {
-        fn();
-        sfa::g_last_error.clear();
-        return SFA_OK;
-    } catch (const Error& e) {
-        sfa::g_last_error = e.msg;
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        sfa::g_last_error = "host allocation failed";
-        return SFA_ERR_OUT_OF_MEMORY;
-    } catch (const std::exception& e) {
-        sfa::g_last_error = e.what();
-        return SFA_ERR_INVALID;
-    }
-}
-
 int solve_host(int method, int N, double* x, const double* b, const int* A, const int* C,
                const double* V, double* residual, double precision, int print_modulo,
                const double* aux, double omega, int nc, const int* F, const int* B, const int* R,
                sfa_solve_info* info)
 {
-    return guarded([&] {
+    return sfa::guarded([&] {
         if (!x || !b || !residual) throw Error{SFA_ERR_INVALID, "null x, b or residual"};
         System sys;
         sys.create(method, N, A, C, V, aux, omega, nc, F, B, R);
@@ -846,7 +711,7 @@ int solve_host(int method, int N, double* x, const double* b, const int* A, cons
 
 extern "C" {
 
-const char* sfa_last_error(void) { return sfa::g_last_error.c_str(); }
+const char* sfa_last_error(void) { return sfa::last_error().c_str(); }
 
 int sfa_api_version(void) { return SFA_API_VERSION; }
 
@@ -916,7 +781,7 @@ int sfa_ConjugateGradientSolverCSR_Eisenstat_Rainbow(
 int sfa_MatrixCSR_apply(int rows, int columns, const int* A, const int* C, const double* V,
                         double* dest, const double* src, double scaling)
 {
-    return guarded([&] {
+    return sfa::guarded([&] {
         std::string err = sfa::check_csr_rect(rows, columns, A, C);
         if (!err.empty()) throw Error{SFA_ERR_INVALID, err};
         if (!V || !dest || !src) throw Error{SFA_ERR_INVALID, "null pointer"};
@@ -949,7 +814,7 @@ int sfa_MatrixCSR_apply(int rows, int columns, const int* A, const int* C, const
 
 int sfa_MatrixCSR_getDiagonal(int n, const int* A, const int* C, const double* V, double* diag)
 {
-    return guarded([&] {
+    return sfa::guarded([&] {
         std::string err = sfa::check_csr(n, A, C);
         if (!err.empty()) throw Error{SFA_ERR_INVALID, err};
         if (!V || !diag) throw Error{SFA_ERR_INVALID, "null pointer"};
@@ -964,7 +829,7 @@ int sfa_MatrixCSR_getDiagonal(int n, const int* A, const int* C, const double* V
 
 int sfa_Rainbow(int n, const int* A, const int* C, int* F, int* B, int* R, int* num_colors)
 {
-    return guarded([&] {
+    return sfa::guarded([&] {
         std::string err = sfa::check_csr(n, A, C);
         if (!err.empty()) throw Error{SFA_ERR_INVALID, err};
         if (!F || !B || !R || !num_colors) throw Error{SFA_ERR_INVALID, "null pointer"};
@@ -976,7 +841,7 @@ int sfa_system_create(int method, int N, const int* csrrows, const int* csrcolum
                       const double* csrvalues, const double* aux, double omega, int num_colors,
                       const int* F, const int* B, const int* R, sfa_system* out)
 {
-    return guarded([&] {
+    return sfa::guarded([&] {
         if (!out) throw Error{SFA_ERR_INVALID, "null output handle"};
         *out = nullptr;
         std::unique_ptr<sfa_system_s> h(new sfa_system_s());
@@ -991,7 +856,7 @@ int sfa_system_solve(sfa_system sys, double* x, const double* b, double* residua
                      double precision, int print_modulo, int device_pointers,
                      sfa_solve_info* info)
 {
-    return guarded([&] {
+    return sfa::guarded([&] {
         if (!sys || !x || !b || !residual) throw Error{SFA_ERR_INVALID, "null argument"};
         sys->sys.solve(x, b, residual, precision, print_modulo, device_pointers, info);
     });
@@ -1001,7 +866,7 @@ int sfa_system_solve_mixed(sfa_system sys, double* x, const double* b, double rt
                            double inner_rtol, int max_outer, int device_pointers,
                            sfa_refine_info* info)
 {
-    return guarded([&] {
+    return sfa::guarded([&] {
         if (!sys || !x || !b) throw Error{SFA_ERR_INVALID, "null argument"};
         sys->sys.solve_mixed(x, b, rtol, inner_rtol, max_outer, device_pointers, info);
     });
@@ -1010,7 +875,7 @@ int sfa_system_solve_mixed(sfa_system sys, double* x, const double* b, double rt
 int sfa_system_apply(sfa_system sys, double* dest, const double* src, double scaling,
                      int device_pointers)
 {
-    return guarded([&] {
+    return sfa::guarded([&] {
         if (!sys || !dest || !src) throw Error{SFA_ERR_INVALID, "null argument"};
         sys->sys.apply(dest, src, scaling, device_pointers);
     });
@@ -1018,7 +883,7 @@ int sfa_system_apply(sfa_system sys, double* dest, const double* src, double sca
 
 int sfa_system_set_stream(sfa_system sys, void* hip_stream)
 {
-    return guarded([&] {
+    return sfa::guarded([&] {
         if (!sys) throw Error{SFA_ERR_INVALID, "null system"};
         sys->sys.stream = hip_stream ? (hipStream_t)hip_stream : sys->sys.own_stream;
     });
@@ -1026,7 +891,7 @@ int sfa_system_set_stream(sfa_system sys, void* hip_stream)
 
 int sfa_system_query(sfa_system sys, sfa_system_stats* stats)
 {
-    return guarded([&] {
+    return sfa::guarded([&] {
         if (!sys || !stats) throw Error{SFA_ERR_INVALID, "null argument"};
         sys->sys.query(stats);
     });
@@ -1036,7 +901,7 @@ int sfa_system_profile(sfa_system sys, double* x, const double* b, double* resid
                        double precision, int device_pointers, sfa_kernel_profile* out,
                        sfa_solve_info* info)
 {
-    return guarded([&] {
+    return sfa::guarded([&] {
         if (!sys || !x || !b || !residual || !out) throw Error{SFA_ERR_INVALID, "null argument"};
         sys->sys.profile(x, b, residual, precision, device_pointers, out, info);
     });
@@ -1044,7 +909,7 @@ int sfa_system_profile(sfa_system sys, double* x, const double* b, double* resid
 
 int sfa_system_destroy(sfa_system sys)
 {
-    return guarded([&] { delete sys; });
+    return sfa::guarded([&] { delete sys; });
 }
 
 }  // extern "C"

@@ -1,0 +1,123 @@
+"""The persistent batch engine (sfa_batch_*): many systems per launch.
+
+Bars: FP64 batch solves match the oracle's ConjugateGradientSolverCSR_Rainbow
+to the single-system bar (K within max(1, 2%), x within 1e-10 relative);
+a solve's result is bit-identical alone and in any mix (its reductions never
+see other slots); mixed-precision solves reach ||b - A x|| <= 1e-10 ||b||.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from simplefem_amd import capi
+from simplefem_amd import systems as S
+
+pytestmark = pytest.mark.gpu
+
+
+def _systems():
+    return [S.poisson2d_p1(3), S.poisson2d_p1(5), S.poisson3d_p1_kuhn(6), S.fd_laplacian_2d(12),
+            S.poisson2d_p1(4), S.random_spd(300, 0.02, 5)]
+
+
+def _add(batch, s, omega=1.0):
+    col = oracle.rainbow(s.n, s.A, s.C)
+    return batch.add_matrix(s.n, s.A, s.C, s.V, s.diagonal(), omega, *col), col
+
+
+def _run_all(batch, slots, steps=0):
+    print(f"[batch] run with {len(slots)} slots", flush=True)
+    while batch.run(steps) > 0:
+        pass
+    done = set(batch.finished())
+    assert done == set(slots), (done, slots)
+
+
+def test_batch_fp64_matches_oracle():
+    systems = _systems()
+    with capi.Batch(mixed=False, max_slots=64) as batch:
+        jobs = []
+        for k, s in enumerate(systems):
+            mid, col = _add(batch, s)
+            for seed in (1, 2):
+                b = S.rhs_uniform(s, 100 * k + seed)
+                jobs.append((batch.submit(mid, b), s, b, col))
+        _run_all(batch, [j[0] for j in jobs])
+        for slot, s, b, col in jobs:
+            x, info = batch.collect(slot, s.n)
+            xo, ro, io = oracle.solve("rainbow", s.A, s.C, s.V, b, aux=s.diagonal(), colouring=col)
+            Ko = io["iterations"]
+            assert abs(info["iterations"] - Ko) <= max(1, int(0.02 * Ko)), (s.name, info, io)
+            assert info["exit_reason"] == io["exit_reason"]
+            assert np.abs(x - xo).max() <= 1e-10 * max(np.abs(xo).max(), 1e-300), s.name
+
+
+def test_batch_result_is_the_same_alone_and_in_the_mix():
+    systems = _systems()
+    target = systems[2]
+    bt = S.rhs_uniform(target, 77)
+    with capi.Batch(max_slots=8) as alone:
+        mid, _ = _add(alone, target)
+        slot = alone.submit(mid, bt)
+        _run_all(alone, [slot])
+        x_alone, info_alone = alone.collect(slot, target.n)
+    with capi.Batch(max_slots=64) as mix:
+        slots = []
+        for k, s in enumerate(systems):
+            mid, _ = _add(mix, s)
+            for seed in range(3):
+                slots.append(mix.submit(mid, S.rhs_uniform(s, 5 + seed + 10 * k)))
+        mid_t, _ = _add(mix, target)
+        st = mix.submit(mid_t, bt)
+        _run_all(mix, slots + [st])
+        x_mix, info_mix = mix.collect(st, target.n)
+    assert info_mix["iterations"] == info_alone["iterations"]
+    assert np.array_equal(x_mix, x_alone)
+
+
+def test_continuous_admission():
+    """Solves submitted while others are running finish with the same result
+    as when they run alone."""
+    systems = _systems()
+    with capi.Batch(max_slots=32) as batch:
+        first = []
+        for k, s in enumerate(systems[:3]):
+            mid, _ = _add(batch, s)
+            first.append((batch.submit(mid, S.rhs_uniform(s, k)), s, mid, k))
+        batch.run(15)  # partly done
+        late = []
+        for k, s in enumerate(systems[3:]):
+            mid, _ = _add(batch, s)
+            late.append((batch.submit(mid, S.rhs_uniform(s, 50 + k)), s, mid, 50 + k))
+        while batch.run(0) > 0:
+            pass
+        results = {}
+        for slot, s, mid, seed in first + late:
+            results[(s.name, seed)] = batch.collect(slot, s.n)
+    for s, seed in [(systems[1], 1), (systems[4], 51)]:
+        with capi.Batch(max_slots=4) as alone:
+            mid, _ = _add(alone, s)
+            slot = alone.submit(mid, S.rhs_uniform(s, seed))
+            _run_all(alone, [slot])
+            x, info = alone.collect(slot, s.n)
+        xm, im = results[(s.name, seed)]
+        assert np.array_equal(x, xm) and info["iterations"] == im["iterations"]
+
+
+def test_batch_mixed_precision_reaches_fp64_residual():
+    systems = [S.poisson2d_p1(5), S.poisson3d_p1_kuhn(8), S.fd_laplacian_2d(20)]
+    with capi.Batch(mixed=True, max_slots=16) as batch:
+        jobs = []
+        for k, s in enumerate(systems):
+            mid, col = _add(batch, s)
+            for seed in (1, 2, 3, 4):
+                b = S.rhs_uniform(s, seed + 10 * k)
+                jobs.append((batch.submit(mid, b, precision=1e-10, inner_rtol=1e-4), s, b))
+        _run_all(batch, [j[0] for j in jobs])
+        for slot, s, b in jobs:
+            x, info = batch.collect(slot, s.n)
+            assert info["success"] == 1, (s.name, info)
+            res = b - oracle.csr_apply(s.n, s.A, s.C, s.V, x)
+            res[s.diagonal() == 0] = 0.0
+            assert np.linalg.norm(res) <= 1.0000001e-10 * np.linalg.norm(b), (s.name, info)
+            assert info["outer"] >= 1
