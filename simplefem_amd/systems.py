@@ -378,3 +378,128 @@ def hcurl3d_whitney_kuhn(n: int, curl_weight: float = 1.0, mass_weight: float = 
     system = CSRSystem(f"hcurl3d_whitney_kuhn_{n}", ne, A, C, V, dirichlet)
     system.edges = np.stack([a, b], axis=1)  # vertex pair of each edge
     return system
+
+
+# --------------------------------------------------------------------------- P_r Lagrange
+
+def _poly_mul(p, q):
+    out = {}
+    for ea, ca in p.items():
+        for eb, cb in q.items():
+            e = tuple(x + y for x, y in zip(ea, eb))
+            out[e] = out.get(e, 0.0) + ca * cb
+    return out
+
+
+def _lagrange_barycentric(d, r):
+    """Principal-lattice Lagrange basis of degree r on a d-simplex as
+    polynomials in the d+1 barycentric coordinates:
+    phi_a = prod_i prod_{k<a_i} (r*lam_i - k)/(k+1),  |a| = r."""
+    import itertools
+
+    alphas = [a for a in itertools.product(range(r + 1), repeat=d + 1) if sum(a) == r]
+    basis = []
+    for a in alphas:
+        p = {tuple([0] * (d + 1)): 1.0}
+        for i in range(d + 1):
+            for k in range(a[i]):
+                e1 = tuple(1 if j == i else 0 for j in range(d + 1))
+                p = _poly_mul(p, {e1: r / (k + 1.0), tuple([0] * (d + 1)): -k / (k + 1.0)})
+        basis.append({e: c for e, c in p.items() if c != 0.0})
+    return alphas, basis
+
+
+def _pr_element_matrix(coords, r):
+    """Stiffness of the degree-r Lagrange basis on one simplex with integer
+    vertex coordinates: K_ij = int grad phi_i . grad phi_j, exact monomial
+    integration int lam^b = d! vol b! / (|b| + d)!."""
+    from math import factorial
+
+    coords = np.asarray(coords, dtype=np.float64)
+    d = coords.shape[1]
+    T = (coords[1:] - coords[0]).T
+    vol = abs(np.linalg.det(T)) / factorial(d)
+    Tinv = np.linalg.inv(T)
+    G = np.vstack([-Tinv.sum(axis=0), Tinv])  # grad lam_i
+    alphas, basis = _lagrange_barycentric(d, r)
+
+    def deriv(p, i):
+        out = {}
+        for e, c in p.items():
+            if e[i] > 0:
+                e2 = tuple(x - (1 if j == i else 0) for j, x in enumerate(e))
+                out[e2] = out.get(e2, 0.0) + c * e[i]
+        return out
+
+    def integral(p):
+        tot = 0.0
+        for e, c in p.items():
+            num = np.prod([factorial(x) for x in e])
+            tot += c * factorial(d) * vol * num / factorial(sum(e) + d)
+        return tot
+
+    dphi = [[deriv(p, i) for i in range(d + 1)] for p in basis]
+    nb = len(basis)
+    K = np.zeros((nb, nb))
+    GG = G @ G.T
+    for a in range(nb):
+        for b in range(a, nb):
+            s = 0.0
+            for i in range(d + 1):
+                for j in range(d + 1):
+                    if GG[i, j] != 0.0 and dphi[a][i] and dphi[b][j]:
+                        s += GG[i, j] * integral(_poly_mul(dphi[a][i], dphi[b][j]))
+            K[a, b] = K[b, a] = s
+    return alphas, K
+
+
+_PR_CACHE = {}
+
+
+def lagrange_pr(dim: int, n: int, r: int) -> CSRSystem:
+    """Degree-r Lagrange stiffness on the unit square (two triangles per cell,
+    mesh/examples2D.hpp:194-211) or unit cube (six Kuhn tetrahedra per cell,
+    mesh/examples3D.hpp:14-37) with n cells per axis: (r*n+1)^dim nodes on the
+    principal lattice, numbered lexicographically; boundary nodes Dirichlet.
+    For r = 1 this is poisson2d_p1 / poisson3d_p1_kuhn up to value rounding."""
+    cache_key = (dim, n, r)
+    if cache_key in _PR_CACHE:
+        return _PR_CACHE[cache_key]
+    m = r * n + 1
+    if dim == 2:
+        simplices = [[(0, 0), (1, 0), (1, 1)], [(0, 0), (0, 1), (1, 1)]]
+    else:
+        simplices = [[((bb >> 2) & 1, (bb >> 1) & 1, bb & 1) for bb in (0,) + path] for path in _KUHN]
+    cells = np.stack(np.meshgrid(*[np.arange(n)] * dim, indexing="ij"), axis=-1).reshape(-1, dim)
+    strides = np.array([m ** k for k in range(dim)])
+    rows, cols, vals = [], [], []
+    for simp in simplices:
+        alphas, K = _pr_element_matrix(simp, r)
+        V = np.array(simp)
+        local = np.array([r * V[0] + sum(a[i] * (V[i] - V[0]) for i in range(1, dim + 1))
+                          for a in alphas])  # node offsets in the fine lattice of one cell
+        lin_cells = (r * cells * strides).sum(axis=1)                # (cells,)
+        lin_local = (local * strides).sum(axis=1)                    # (nb,)
+        nodes = lin_cells[:, None] + lin_local[None, :]              # (cells, nb)
+        nb = len(alphas)
+        rows.append(np.repeat(nodes, nb, axis=1).ravel())
+        cols.append(np.tile(nodes, (1, nb)).ravel())
+        vals.append(np.tile(K.ravel(), nodes.shape[0]))
+    rows, cols, vals = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+    grid = np.stack(np.unravel_index(np.arange(m ** dim), (m,) * dim)[::-1], axis=1)
+    dirichlet = ((grid == 0) | (grid == m - 1)).any(axis=1)
+    vals = np.where(dirichlet[rows] | dirichlet[cols], 0.0, vals) * (1.0 / n) ** (dim - 2)
+    N = m ** dim
+    key = rows.astype(np.int64) * N + cols
+    order = np.argsort(key, kind="stable")
+    key = key[order]
+    first = np.concatenate([[True], key[1:] != key[:-1]])
+    idx = np.nonzero(first)[0]
+    V = np.add.reduceat(vals[order], idx)
+    uniq = key[idx]
+    A = np.zeros(N + 1, dtype=np.int64)
+    A[1:] = np.bincount(uniq // N, minlength=N)
+    out = CSRSystem(f"lagrange{dim}d_n{n}_r{r}", N, np.cumsum(A).astype(np.int32),
+                    (uniq % N).astype(np.int32), V, dirichlet)
+    _PR_CACHE[cache_key] = out
+    return out
