@@ -845,9 +845,9 @@ struct Engine {
             k_batch_schedule<T><<<1, kUnitBlock, 0, stream>>>(args());
             SFA_HIP(hipGetLastError());
         }
-        const unsigned limit = max_steps > 0 ? step + (unsigned)max_steps : 0xFFFFFFF0u;
-        SFA_HIP(hipMemcpyAsync(&d_sched->limit, &limit, sizeof(unsigned), hipMemcpyHostToDevice,
-                               stream));
+        h_sched->limit = max_steps > 0 ? step + (unsigned)max_steps : 0xFFFFFFF0u;
+        SFA_HIP(hipMemcpyAsync(&d_sched->limit, &h_sched->limit, sizeof(unsigned),
+                               hipMemcpyHostToDevice, stream));
         k_batch_run<T><<<kRunBlocks, kUnitBlock, 0, stream>>>(args());
         SFA_HIP(hipGetLastError());
         read_sched();

@@ -98,3 +98,47 @@ def test_rhs_is_seeded_and_zero_on_dirichlet_rows():
     assert np.all(b1[s.dirichlet] == 0) and np.all(np.abs(b1[~s.dirichlet]) <= 1)
     # SplitMix64 known answer (seed 0, first output)
     assert int(S.splitmix64(0, 1)[0]) == 0xE220A8397B1DCDAF
+
+
+@pytest.mark.parametrize("dim,n,r", [(2, 3, 2), (2, 3, 3), (3, 2, 2), (3, 1, 3)])
+def test_lagrange_pr_reproduces_linears(dim, n, r):
+    """Degree-r Lagrange stiffness: K * (affine function) vanishes on interior
+    rows (the discrete Laplacian of an affine function is zero), the matrix is
+    symmetric and positive definite on interior nodes, and r = 1 is P1."""
+    s = S.lagrange_pr(dim, n, r)
+    M = _csr(s).toarray()
+    assert np.allclose(M, M.T, atol=1e-13)
+    m = r * n + 1
+    grid = np.stack(np.unravel_index(np.arange(m ** dim), (m,) * dim)[::-1], axis=1) / (r * n)
+    # without boundary conditions: assemble the same pattern with no Dirichlet nodes
+    act = ~s.dirichlet
+    assert np.linalg.eigvalsh(M[np.ix_(act, act)]).min() > 0
+    f = 0.3 + grid @ np.arange(1, dim + 1)  # affine
+    # rows touching the boundary were cut; use interior rows whose neighbours are interior
+    interior = act.copy()
+    for i in np.nonzero(act)[0]:
+        if s.dirichlet[s.C[s.A[i]:s.A[i + 1]]].any():
+            interior[i] = False
+    if interior.any():
+        assert np.abs((M @ f)[interior]).max() <= 1e-10 * np.abs(M).max()
+
+
+def test_lagrange_p1_matches_structured_generators():
+    a, b = S.lagrange_pr(2, 8, 1), S.poisson2d_p1(3)
+    assert np.array_equal(a.A, b.A) and np.array_equal(a.C, b.C) and np.array_equal(a.V, b.V)
+    a, b = S.lagrange_pr(3, 4, 1), S.poisson3d_p1_kuhn(4)
+    assert np.array_equal(a.A, b.A) and np.array_equal(a.C, b.C)
+    assert np.abs(a.V - b.V).max() <= 1e-15
+
+
+def test_workloads_are_seeded():
+    from simplefem_amd import workloads as W
+
+    j1 = W.batched_systems(count=6, nrhs=2, seed=3, levels=(2, 3))
+    j2 = W.batched_systems(count=6, nrhs=2, seed=3, levels=(2, 3))
+    assert [j.system.n for j in j1] == [j.system.n for j in j2]
+    assert all(np.array_equal(a.rhs[1], b.rhs[1]) for a, b in zip(j1, j2))
+    req = W.served_requests(count=8, rate=10.0, seed=1, dofs=(1e2, 3e3))
+    t = [j.arrival for j in req]
+    assert t == sorted(t) and t[0] > 0
+    assert all(100 <= j.system.n <= 20000 for j in req)
