@@ -23,48 +23,54 @@ __device__ inline double wave_sum(double v)
     return v;
 }
 
-// Block-wide deterministic reduction across the whole grid.  Returns true in
-// every thread of the last block; *total is then valid in all its threads.
+// Stores the block's sum of v at *out (thread 0, plain store).  Every thread
+// of the block must call it.  Waves are added in wave order.
 template <int BS>
-__device__ inline bool grid_reduce_last(double v, double* partials, unsigned* ticket, double* total)
+__device__ inline void block_partial(double v, double* out)
 {
     constexpr int NW = BS / kWave;
-    __shared__ double ws[NW];
-    __shared__ int is_last;
-    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-    double s = wave_sum(v);
-    if (NW > 1) {
-        if (lane == 0) ws[wave] = s;
+    const double s = wave_sum(v);
+    if constexpr (NW == 1) {
+        if (threadIdx.x == 0) *out = s;
+    } else {
+        __shared__ double ws[NW];
+        if (threadIdx.x % kWave == 0) ws[threadIdx.x / kWave] = s;
         __syncthreads();
         if (threadIdx.x == 0) {
             double t = 0.;
             for (int w = 0; w < NW; w++) t += ws[w];
-            s = t;
+            *out = t;
         }
     }
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(&partials[blockIdx.x], s, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        unsigned prev = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = prev == gridDim.x - 1;
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    double acc = 0.;
-    for (unsigned i = threadIdx.x; i < gridDim.x; i += BS)
-        acc += __hip_atomic_load(&partials[i], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    acc = wave_sum(acc);
-    if (NW > 1) {
+}
+
+// Sum of v[0..count) in a fixed order (lane l adds v[l], v[l+64], ..., then
+// the xor butterfly), formed by the calling wavefront; every lane gets the
+// same bits, and so does every block that sums the same values.
+__device__ inline double wave_fold(const double* v, int count)
+{
+    const int lane = threadIdx.x % kWave;
+    double a = 0.;
+    for (int i = lane; i < count; i += kWave) a += v[i];
+    return wave_sum(a);
+}
+
+// wave_fold for a block of BS threads: the first wavefront sums, the block
+// shares the result.  Every thread of the block must call it.
+template <int BS>
+__device__ inline double block_fold(const double* v, int count)
+{
+    if constexpr (BS == kWave) {
+        return wave_fold(v, count);
+    } else {
+        __shared__ double sh;
+        if (threadIdx.x < kWave) {
+            const double t = wave_fold(v, count);
+            if (threadIdx.x == 0) sh = t;
+        }
         __syncthreads();
-        if (lane == 0) ws[wave] = acc;
-        __syncthreads();
-        double t = 0.;
-        for (int w = 0; w < NW; w++) t += ws[w];
-        acc = t;
+        return sh;
     }
-    *total = acc;
-    if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return true;
 }
 
 // base/base.hpp:229-237
@@ -73,59 +79,6 @@ __device__ inline double ref_absolute(double x)
     if (x >= 0) return x;
     if (x <= 0) return -x;
     return x;
-}
-
-// Scalar step after d.Ad (solver/sparsesolver.cpp:629-645): breakdown tests
-// and alpha.  One thread.
-__device__ inline void after_dAd(SolveState* st, double dAd)
-{
-    st->dAd = dAd;
-    const double zr = st->zr;
-    const bool num_bad = !isfinite(zr) || zr < 0.;
-    const bool den_bad = !isfinite(dAd) || dAd < 0.;
-    const bool small = sqrt(ref_absolute(dAd)) < kMachineEpsilon;
-    if (num_bad || den_bad) {
-        st->done = 1;
-        st->exit_reason = SFA_EXIT_BREAKDOWN;
-    } else if (small && st->small_breaks) {
-        st->done = 1;
-        st->exit_reason = SFA_EXIT_SMALL;
-    } else {
-        st->alpha = zr / dAd;
-    }
-}
-
-// Scalar step after the new z.r (solver/sparsesolver.cpp:694-708 and the
-// loop head :504,601-604): beta, K++, then the loop's exit tests.  One thread.
-__device__ inline void after_zr_new(SolveState* st, double zr_new)
-{
-    st->beta = zr_new / st->zr;
-    st->zr = zr_new;
-    const int K = st->K + 1;
-    st->K = K;
-    st->hist[K] = zr_new;
-    if (K >= st->maxK) {
-        st->done = 1;
-        st->exit_reason = SFA_EXIT_CAP;
-    } else if (ref_absolute(zr_new) < st->tol2) {
-        st->done = 1;
-        st->exit_reason = SFA_EXIT_CONVERGED;
-    }
-}
-
-// z.r of the restart (K = 0) and the first loop-head test.
-__device__ inline void after_zr_restart(SolveState* st, double zr)
-{
-    st->zr = zr;
-    st->hist[0] = zr;
-    if (st->rel2 > 0.) {  // inner solves: stop at a fixed reduction of z.r
-        st->tol2 = st->rel2 * ref_absolute(zr);
-        st->tol = sqrt(st->tol2);
-    }
-    if (ref_absolute(zr) < st->tol2) {
-        st->done = 1;
-        st->exit_reason = SFA_EXIT_CONVERGED;
-    }
 }
 
 // acc (+|-)= sum_k v_k * vec[c_k] over one SELL row, entries in order.
@@ -157,6 +110,59 @@ __device__ inline T sell_row(const SellViewT<T>& A, int slice, int lane, const T
         const T v = __builtin_nontemporal_load(&A.vals[e]);
         if (SUB) acc -= v * vec[c];
         else     acc += v * vec[c];
+    }
+    return acc;
+}
+
+// The first U entries of a lane's SELL row, loaded ahead of whatever the
+// kernel reads next so that the loads overlap.  All lanes of a slice hold
+// the same number of entries (the slice is padded to its longest row), so
+// the guards below never split a wavefront.
+template <typename T, int U = 8>
+struct SellChunk {
+    int e = 0, end = 0;
+    int c[U];
+    T v[U];
+};
+
+template <typename T, int U>
+__device__ inline void sell_prefetch(const SellViewT<T>& A, int slice, int lane, SellChunk<T, U>& R)
+{
+    R.e = A.off[slice] + lane;
+    R.end = A.off[slice + 1];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int e = R.e + u * kWave;
+        if (e < R.end) {
+            R.c[u] = __builtin_nontemporal_load(&A.cols[e]);
+            R.v[u] = __builtin_nontemporal_load(&A.vals[e]);
+        } else {
+            R.c[u] = 0;
+            R.v[u] = T(0);
+        }
+    }
+}
+
+// acc (+|-)= sum_k v_k * get(c_k) over the row of a prefetched chunk, entries
+// in order; entries past the chunk are loaded here.
+template <bool SUB, typename T, int U, class Get>
+__device__ inline T sell_finish(const SellViewT<T>& A, const SellChunk<T, U>& R, Get get, T acc)
+{
+    T g[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) g[u] = R.e + u * kWave < R.end ? get(R.c[u]) : T(0);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        if (R.e + u * kWave < R.end) {
+            if (SUB) acc -= R.v[u] * g[u];
+            else     acc += R.v[u] * g[u];
+        }
+    }
+    for (int e = R.e + U * kWave; e < R.end; e += kWave) {
+        const int c = __builtin_nontemporal_load(&A.cols[e]);
+        const T v = __builtin_nontemporal_load(&A.vals[e]);
+        if (SUB) acc -= v * get(c);
+        else     acc += v * get(c);
     }
     return acc;
 }

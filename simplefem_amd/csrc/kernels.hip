@@ -8,9 +8,13 @@
 // rounded before the add (built with -ffp-contract=off), so a row's value
 // matches the reference's loop bit for bit given the same inputs.
 //
-// Reductions (d.Ad, z.r) are deterministic: each block writes one partial,
-// the last block to finish (agent-scope ticket) sums the partials in block
-// order and runs the scalar part of the CG step on the device.
+// Reductions (d.Ad, z.r) are deterministic and deferred: each block stores
+// one partial, and the next kernel on the stream sums them in a fixed order
+// (wave_fold) in every block that needs the total.  The scalar steps of the
+// recurrence (alpha, beta, K, the exit tests) are computed identically by
+// every such block; thread 0 of block 0 records them in the state for the
+// kernels that follow and for the host.  Sweep kernels issue their matrix
+// and row loads before they read the state, so the loads overlap it.
 #include "kernels.hpp"
 
 #include <cmath>
@@ -20,9 +24,114 @@
 
 namespace sfa {
 
+int spmv_blocks(int nslices)
+{
+    const int b = (nslices + kSpmvWaves - 1) / kSpmvWaves;
+    return b < 1 ? 1 : (b > kSpmvMaxBlocks ? kSpmvMaxBlocks : b);
+}
+
+int vec_reduce_blocks(int n)
+{
+    const int b = (n + dev::kVecBlock - 1) / dev::kVecBlock;
+    return b < 1 ? 1 : (b > kVecReduceBlocks ? kVecReduceBlocks : b);
+}
+
 namespace {
 
 using namespace dev;
+
+constexpr int kSpmvBlock = kSpmvWaves * kWave;
+
+// ------------------------------------------------------------------ scalar steps
+
+// z.r of the restart and the first loop-head test (solver/sparsesolver.cpp:
+// 586-596 and siblings).  One thread.
+__device__ inline void restart_step(SolveState* st, double zr)
+{
+    st->zr = zr;
+    st->hist[0] = zr;
+    st->zrp[0] = zr;
+    st->Kp[0] = 0;
+    st->fold[0] = 0;
+    st->K = 0;
+    if (st->rel2 > 0.) {  // inner solves: stop at a fixed reduction of z.r
+        st->tol2 = st->rel2 * ref_absolute(zr);
+        st->tol = sqrt(st->tol2);
+    }
+    if (ref_absolute(zr) < st->tol2) {
+        st->exit_reason = SFA_EXIT_CONVERGED;
+        st->done = 1;
+    }
+}
+
+// Opens the iteration of parity p.  With fold, zr_new is the z.r of the new
+// iterate: beta = zr_new / zr, K+1, then the loop's exit tests
+// (solver/sparsesolver.cpp:694-708 and the loop head :504,601-604).  Every
+// calling block computes the same values; thread 0 of block 0 records them.
+// Returns true when the loop has ended.
+__device__ inline bool open_step(SolveState* st, int p, int fold, double zr_old, int K,
+                                 double zr_new, double* beta_out)
+{
+    bool stop = false;
+    int reason = SFA_EXIT_CAP;
+    int Kn = K;
+    double zr = zr_old;
+    double beta = 0.;
+    if (fold) {
+        beta = zr_new / zr_old;
+        zr = zr_new;
+        Kn = K + 1;
+        if (Kn >= st->maxK) {
+            stop = true;
+            reason = SFA_EXIT_CAP;
+        } else if (ref_absolute(zr_new) < st->tol2) {
+            stop = true;
+            reason = SFA_EXIT_CONVERGED;
+        }
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->Kp[1 - p] = Kn;
+        st->fold[1 - p] = 1;
+        st->zrp[1 - p] = zr;
+        st->K = Kn;
+        st->zr = zr;
+        if (fold) {
+            st->hist[Kn] = zr;
+            st->beta = beta;
+        }
+        if (stop) {
+            st->exit_reason = reason;
+            st->done = 1;
+        }
+    }
+    *beta_out = beta;
+    return stop;
+}
+
+// After d.Ad (solver/sparsesolver.cpp:629-645): breakdown tests and
+// alpha = z.r / d.Ad.  Every calling block computes the same values; thread 0
+// of block 0 records them.  Returns true when the loop has ended.
+__device__ inline bool dad_step(SolveState* st, double zr, double dAd, double* alpha_out)
+{
+    const bool num_bad = !isfinite(zr) || zr < 0.;
+    const bool den_bad = !isfinite(dAd) || dAd < 0.;
+    const bool small = sqrt(ref_absolute(dAd)) < kMachineEpsilon;
+    int reason = -1;
+    if (num_bad || den_bad) reason = SFA_EXIT_BREAKDOWN;
+    else if (small && st->small_breaks) reason = SFA_EXIT_SMALL;
+    const double alpha = zr / dAd;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->dAd = dAd;
+        if (reason >= 0) {
+            st->exit_reason = reason;
+            st->done = 1;
+        } else {
+            st->alpha = alpha;
+        }
+    }
+    *alpha_out = alpha;
+    return reason >= 0;
+}
 
 // ------------------------------------------------------------------ vectors
 
@@ -47,21 +156,23 @@ __global__ void k_scatter(T* dst, const T* src, const int* perm, int n)
     if (i < n) dst[perm[i]] = src[i];
 }
 
-// State reset and tolerance (solver/sparsesolver.cpp:469-472):
-// tol = max(precision * sqrt(sum b^2), desired_precision).
+// Partials of sum b^2 (solver/sparsesolver.cpp:469-472).
 template <typename T>
-__global__ void __launch_bounds__(kVecBlock) k_init_state(SolveState* st, const T* b, int n,
-                                                          double precision, double host_tol,
-                                                          int use_host_tol, int maxK,
-                                                          int small_breaks, double rel2,
-                                                          double* partials)
+__global__ void __launch_bounds__(kVecBlock) k_sumsq(const T* b, int n, double* partials)
 {
     double s = 0.;
-    if (!use_host_tol)
-        for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
-            s += (double)b[i] * (double)b[i];
-    double total;
-    if (!grid_reduce_last<kVecBlock>(s, partials, &st->ticket, &total)) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        s += (double)b[i] * (double)b[i];
+    block_partial<kVecBlock>(s, &partials[blockIdx.x]);
+}
+
+// State reset and tolerance: tol = max(precision * sqrt(sum b^2), desired_precision).
+__global__ void __launch_bounds__(kWave) k_init_state(SolveState* st, const double* partials,
+                                                      int np, double precision, double host_tol,
+                                                      int use_host_tol, int maxK,
+                                                      int small_breaks, double rel2)
+{
+    const double total = use_host_tol ? 0. : wave_fold(partials, np);
     if (threadIdx.x != 0) return;
     double tol = host_tol;
     if (!use_host_tol) {
@@ -71,17 +182,35 @@ __global__ void __launch_bounds__(kVecBlock) k_init_state(SolveState* st, const 
     }
     st->tol = tol;
     st->tol2 = tol * tol;
-    st->zr = __builtin_nan("");
-    st->dAd = 0.;
+    st->rel2 = rel2;
+    st->maxK = maxK;
+    st->small_breaks = small_breaks;
+    st->Kp[0] = st->Kp[1] = 0;
+    st->fold[0] = st->fold[1] = 0;
+    st->zrp[0] = st->zrp[1] = __builtin_nan("");
     st->alpha = 0.;
     st->beta = 0.;
+    st->zr = __builtin_nan("");
+    st->dAd = 0.;
+    st->result = 0.;
     st->K = 0;
-    st->maxK = maxK;
     st->done = 0;
     st->exit_reason = SFA_EXIT_CAP;
-    st->small_breaks = small_breaks;
-    st->rel2 = rel2;
-    st->result = 0.;
+}
+
+// Restart's scalar step from the restart kernel's partials.
+__global__ void __launch_bounds__(kWave) k_restart_finish(SolveState* st, const double* partials,
+                                                          int np)
+{
+    const double total = wave_fold(partials, np);
+    if (threadIdx.x == 0) restart_step(st, total);
+}
+
+__global__ void __launch_bounds__(kWave) k_result_finish(SolveState* st, const double* partials,
+                                                         int np)
+{
+    const double total = wave_fold(partials, np);
+    if (threadIdx.x == 0) st->result = total;
 }
 
 // ------------------------------------------------------------------ apply
@@ -109,11 +238,9 @@ __global__ void __launch_bounds__(kSellBlock) k_apply(SellViewT<T> A, int nslice
 // :797-812 (Eisenstat, columns already filtered into the part).
 template <typename T>
 __global__ void __launch_bounds__(kSellBlock) k_residual(int mode, SellViewT<T> A, int nslices,
-                                                         int row0, int nrows,
-                                                         SolveState* st, const T* b,
-                                                         const T* x, T* out, T* d,
-                                                         T* z, const T* precon,
-                                                         double* partials)
+                                                         int row0, int nrows, const T* b,
+                                                         const T* x, T* out, T* d, T* z,
+                                                         const T* precon, double* partials)
 {
     const int slice = blockIdx.x;
     const int lane = threadIdx.x;
@@ -132,16 +259,12 @@ __global__ void __launch_bounds__(kSellBlock) k_residual(int mode, SellViewT<T> 
             part = zz * r;
         }
     }
-    if (mode == RES_ONLY) return;
-    double total;
-    if (!grid_reduce_last<kSellBlock>(part, partials, &st->ticket, &total)) return;
-    if (threadIdx.x == 0) after_zr_restart(st, total);
+    if (mode != RES_ONLY) block_partial<kSellBlock>(part, &partials[blockIdx.x]);
 }
 
-// solver/sparsesolver.cpp:570-583: d = z, z.r = sum z*r.
+// solver/sparsesolver.cpp:570-583: d = z, partials of z.r.
 template <typename T>
-__global__ void __launch_bounds__(kVecBlock) k_direction_init(SolveState* st, int n, const T* z,
-                                                              const T* r, T* d,
+__global__ void __launch_bounds__(kVecBlock) k_direction_init(int n, const T* z, const T* r, T* d,
                                                               double* partials)
 {
     double part = 0.;
@@ -150,16 +273,13 @@ __global__ void __launch_bounds__(kVecBlock) k_direction_init(SolveState* st, in
         d[i] = zz;
         part += (double)(zz * r[i]);
     }
-    double total;
-    if (!grid_reduce_last<kVecBlock>(part, partials, &st->ticket, &total)) return;
-    if (threadIdx.x == 0) after_zr_restart(st, total);
+    block_partial<kVecBlock>(part, &partials[blockIdx.x]);
 }
 
-// solver/sparsesolver.cpp:830-844: z = r*diag*(2-w)/w, d = z, z.r.
+// solver/sparsesolver.cpp:830-844: z = r*diag*(2-w)/w, d = z, partials of z.r.
 template <typename T>
-__global__ void __launch_bounds__(kVecBlock) k_eisenstat_init(SolveState* st, int n, const T* r,
-                                                              const T* diag, T omega,
-                                                              T* z, T* d,
+__global__ void __launch_bounds__(kVecBlock) k_eisenstat_init(int n, const T* r, const T* diag,
+                                                              T omega, T* z, T* d,
                                                               double* partials)
 {
     double part = 0.;
@@ -170,82 +290,108 @@ __global__ void __launch_bounds__(kVecBlock) k_eisenstat_init(SolveState* st, in
         d[i] = zz;
         part += (double)(zz * rr);
     }
-    double total;
-    if (!grid_reduce_last<kVecBlock>(part, partials, &st->ticket, &total)) return;
-    if (threadIdx.x == 0) after_zr_restart(st, total);
+    block_partial<kVecBlock>(part, &partials[blockIdx.x]);
 }
 
 // ------------------------------------------------------------------ sweeps
 
 // Forward sweep of one colour / level, solver/sparsesolver.cpp:528-541 (restart)
-// and :647-665, :1234-1261 (iteration, with the x and r updates).
+// and :629-665, :1234-1261 (iteration: d.Ad step in the first group, then the
+// x and r updates of the group's rows).
 template <typename T>
 __global__ void __launch_bounds__(kSellBlock) k_forward(SellViewT<T> L, int slice0, int nslices,
-                                                        int row0, int nrows,
-                                                        const SolveState* st, int update_xr,
-                                                        T* x, T* r, const T* d,
-                                                        const T* q, T* m,
-                                                        const T* diag, T omega)
+                                                        int row0, int nrows, SolveState* st,
+                                                        int mode, int p, T* x, T* r,
+                                                        const T* d, const T* q, T* m,
+                                                        const T* diag, T omega,
+                                                        const double* spmv_partials, int nspmv)
 {
-    if (st->done) return;
-    const int slice = slice0 + blockIdx.x;
     const int lane = threadIdx.x;
-    if (blockIdx.x >= nslices) return;
-    const int row = group_row(L, slice, blockIdx.x, lane, row0, nrows);
-    if (row < 0) return;
-    T rr;
-    if (update_xr) {
-        const T alpha = (T)st->alpha;
-        x[row] += alpha * d[row];
-        rr = r[row] - alpha * q[row];
-        r[row] = rr;
-    } else {
-        rr = r[row];
+    const int slice = slice0 + blockIdx.x;
+    SellChunk<T> R;
+    int row = -1;
+    if ((int)blockIdx.x < nslices) {
+        sell_prefetch(L, slice, lane, R);
+        row = group_row(L, slice, blockIdx.x, lane, row0, nrows);
     }
-    const T acc = sell_row<true>(L, slice, lane, m, rr);
-    m[row] = acc * omega / diag[row];
+    T rr = T(0), dg = T(1), xx = T(0), dd = T(0), qq = T(0);
+    if (row >= 0) {
+        rr = r[row];
+        dg = diag[row];
+        if (mode != FWD_RESTART) {
+            xx = x[row];
+            dd = d[row];
+            qq = q[row];
+        }
+    }
+    if (st->done) return;
+    T alpha = T(0);
+    if (mode == FWD_FIRST) {
+        const double dAd = wave_fold(spmv_partials, nspmv);
+        double a;
+        if (dad_step(st, st->zrp[1 - p], dAd, &a)) return;
+        alpha = (T)a;
+    } else if (mode == FWD_UPDATE) {
+        alpha = (T)st->alpha;
+    }
+    if (row < 0) return;
+    if (mode != FWD_RESTART) {
+        x[row] = xx + alpha * dd;
+        rr = rr - alpha * qq;
+        r[row] = rr;
+    }
+    const T acc = sell_finish<true>(L, R, [&](int c) { return m[c]; }, rr);
+    m[row] = acc * omega / dg;
 }
 
 // Backward sweep of one colour / level, solver/sparsesolver.cpp:543-568
-// (restart) and :667-692, :1263-1303 (iteration): the scaling of m fused in,
-// z.r summed per group and over groups in launch order.
+// (restart) and :667-692, :1263-1303 (iteration): the scaling of m fused in.
+// With reduce, the z.r partial of each block goes to gpart[slice], and block
+// 0 sums the previous group's partials into *prev_sum, so z.r is summed per
+// group and over the groups in launch order.
 template <typename T>
 __global__ void __launch_bounds__(kSellBlock) k_backward(SellViewT<T> U, int slice0, int nslices,
-                                                         int row0, int nrows,
-                                                         SolveState* st, int reduce, int group,
-                                                         int last_group, const T* r,
-                                                         const T* m, T* z,
-                                                         const T* diag, T omega,
-                                                         T c2, double* grp, double* partials)
+                                                         int row0, int nrows, SolveState* st,
+                                                         int reduce, const T* r, const T* m,
+                                                         T* z, const T* diag, T omega, T c2,
+                                                         double* gpart, int prev0, int prevn,
+                                                         double* prev_sum)
 {
-    if (st->done) return;
-    const int slice = slice0 + blockIdx.x;
     const int lane = threadIdx.x;
-    double part = 0.;
-    const int row = blockIdx.x < nslices ? group_row(U, slice, blockIdx.x, lane, row0, nrows) : -1;
+    const int slice = slice0 + blockIdx.x;
+    const bool has = (int)blockIdx.x < nslices;
+    SellChunk<T> R;
+    int row = -1;
+    if (has) {
+        sell_prefetch(U, slice, lane, R);
+        row = group_row(U, slice, blockIdx.x, lane, row0, nrows);
+    }
+    T dg = T(1), mm = T(0), rr = T(0);
     if (row >= 0) {
-        const T dg = diag[row];
-        T t = m[row];
+        dg = diag[row];
+        mm = m[row];
+        if (reduce) rr = r[row];
+    }
+    if (st->done) return;
+    if (reduce && prev_sum && blockIdx.x == 0) {
+        const double t = wave_fold(gpart + prev0, prevn);
+        if (lane == 0) *prev_sum = t;
+    }
+    double part = 0.;
+    if (row >= 0) {
+        T t = mm;
         t *= dg;
         t *= c2;
-        const T acc = sell_row<true>(U, slice, lane, z, t);
+        const T acc = sell_finish<true>(U, R, [&](int c) { return z[c]; }, t);
         const T zz = acc * omega / dg;
         z[row] = zz;
-        part = (double)(zz * r[row]);
+        part = (double)(zz * rr);
     }
-    if (!reduce) return;
-    double total;
-    if (!grid_reduce_last<kSellBlock>(part, partials, &st->ticket, &total)) return;
-    if (threadIdx.x != 0) return;
-    grp[group] = total;
-    if (!last_group) return;
-    double zr_new = 0.;
-    for (int g = 0; g <= group; g++) zr_new += grp[g];
-    after_zr_new(st, zr_new);
+    if (reduce && has) block_partial<kSellBlock>(part, &gpart[slice]);
 }
 
 // Eisenstat triangular solves (solver/sparsesolver.cpp:814-828, 876-908,
-// 1429-1455, 1501-1560): out = (in - [c3*diag*t] - T out) * w / diag.
+// 1429-1455, 1501-1560): out = (in - [c3*diag*t] - P out) * w / diag.
 template <typename T>
 __global__ void __launch_bounds__(kSellBlock) k_eisenstat_solve(SellViewT<T> P, int slice0,
                                                                 int nslices, int row0, int nrows,
@@ -254,55 +400,144 @@ __global__ void __launch_bounds__(kSellBlock) k_eisenstat_solve(SellViewT<T> P, 
                                                                 T* out, const T* diag,
                                                                 T omega, T c3)
 {
-    if (st->done) return;
     const int slice = slice0 + blockIdx.x;
     const int lane = threadIdx.x;
-    if (blockIdx.x >= nslices) return;
+    if ((int)blockIdx.x >= nslices) return;
+    SellChunk<T> R;
+    sell_prefetch(P, slice, lane, R);
     const int row = group_row(P, slice, blockIdx.x, lane, row0, nrows);
+    T dg = T(1), a = T(0), tt = T(0);
+    if (row >= 0) {
+        dg = diag[row];
+        a = in[row];
+        if (with_t) tt = t[row];
+    }
+    if (st->done) return;
     if (row < 0) return;
-    const T dg = diag[row];
-    T a = in[row];
-    if (with_t) a = a - c3 * dg * t[row];
-    const T acc = sell_row<true>(P, slice, lane, out, a);
+    if (with_t) a = a - c3 * dg * tt;
+    const T acc = sell_finish<true>(P, R, [&](int c) { return out[c]; }, a);
+    out[row] = acc * omega / dg;
+}
+
+// Eisenstat upper solve (solver/sparsesolver.cpp:876-892, 1429-1455) with the
+// previous iteration's direction update (:960-966) fused in.  The first
+// group opens the iteration from the update kernel's z.r partials.
+template <typename T>
+__global__ void __launch_bounds__(kSellBlock) k_eisenstat_upper(SellViewT<T> P, int slice0,
+                                                                int nslices, int row0, int nrows,
+                                                                SolveState* st, int first, int p,
+                                                                const T* z, const T* dold,
+                                                                T* dnew, T* out, const T* diag,
+                                                                T omega, const double* fpart,
+                                                                int nfold)
+{
+    const int lane = threadIdx.x;
+    const int slice = slice0 + blockIdx.x;
+    SellChunk<T> R;
+    int row = -1;
+    if ((int)blockIdx.x < nslices) {
+        sell_prefetch(P, slice, lane, R);
+        row = group_row(P, slice, blockIdx.x, lane, row0, nrows);
+    }
+    T dg = T(1), zz = T(0), dd = T(0);
+    if (row >= 0) {
+        dg = diag[row];
+        zz = z[row];
+        dd = dold[row];
+    }
+    if (st->done) return;
+    const int fold = st->fold[p];
+    T beta;
+    if (first) {
+        const double zr_new = fold ? wave_fold(fpart, nfold) : 0.;
+        double b;
+        if (open_step(st, p, fold, st->zrp[p], st->Kp[p], zr_new, &b)) return;
+        beta = (T)b;
+    } else {
+        beta = (T)st->beta;
+    }
+    if (row < 0) return;
+    const T dn = fold ? zz + beta * dd : dd;
+    dnew[row] = dn;
+    const T acc = sell_finish<true>(P, R, [&](int c) { return out[c]; }, dn);
     out[row] = acc * omega / dg;
 }
 
 // ------------------------------------------------------------------ iteration
 
-// q = A d, d.Ad (solver/sparsesolver.cpp:611-626, 1197-1212), then the
-// breakdown tests and alpha.
+// Opens the iteration (z.r of the previous sweep, beta, K, exit tests), then
+// dn = src + beta*dold (solver/sparsesolver.cpp:698-706 and its siblings),
+// q = A dn and the d.Ad partials (:611-626, 1197-1212).  Each wavefront owns
+// slices blockIdx.x*4 + wave + k*4*gridDim.x; neighbours' dn are formed from
+// src and dold where they are gathered, with the same arithmetic as the
+// owner's, so every copy has the same bits.
 template <typename T>
-__global__ void __launch_bounds__(kSellBlock) k_spmv_dot(SellViewT<T> A, int nslices, int row0,
-                                                         int nrows, SolveState* st,
-                                                         const T* d, T* q,
-                                                         double* partials)
+__global__ void __launch_bounds__(kSpmvBlock) k_spmv_dir(SellViewT<T> A, int nslices, int row0,
+                                                         int nrows, int n_active,
+                                                         SolveState* st, int p,
+                                                         const T* src, const T* dold, T* dnew,
+                                                         T* q, double* spart,
+                                                         const double* fpart, int nfold,
+                                                         const double* gsum, int ngroups)
 {
-    if (st->done) return;
-    const int slice = blockIdx.x;
-    const int lane = threadIdx.x;
-    double part = 0.;
-    const int row = slice < nslices ? group_row(A, slice, slice, lane, row0, nrows) : -1;
-    if (row >= 0) {
-        const T acc = sell_row<false>(A, slice, lane, d, T(0));
-        q[row] = acc;
-        part = (double)(d[row] * acc);
+    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
+    const int first_slice = blockIdx.x * kSpmvWaves + wave;
+    const int stride = gridDim.x * kSpmvWaves;
+    SellChunk<T> R;
+    if (first_slice < nslices) sell_prefetch(A, first_slice, lane, R);
+    const int fold = st->fold[p];
+    const int K = st->Kp[p];
+    const double zr_old = st->zrp[p];
+    // Block 0 may set done while other blocks start; the vote keeps the
+    // waves of a block together (a set flag means the loop has ended).
+    if (__syncthreads_or(st->done)) return;
+    double zr_new = 0.;
+    if (fold) {
+        __shared__ double sh;
+        if (wave == 0) {
+            double t = 0.;
+            for (int g = 0; g < ngroups; g++) t += gsum[g];
+            t += wave_fold(fpart, nfold);
+            if (lane == 0) sh = t;
+        }
+        __syncthreads();
+        zr_new = sh;
     }
-    double total;
-    if (!grid_reduce_last<kSellBlock>(part, partials, &st->ticket, &total)) return;
-    if (threadIdx.x == 0) after_dAd(st, total);
+    double beta_d;
+    if (open_step(st, p, fold, zr_old, K, zr_new, &beta_d)) return;
+    const T beta = (T)beta_d;
+    // rows past n_active (inactive rows, the zero slot) keep their d
+    auto dn = [&](int c) -> T { return fold && c < n_active ? src[c] + beta * dold[c] : dold[c]; };
+    double part = 0.;
+    for (int slice = first_slice; slice < nslices; slice += stride) {
+        if (slice != first_slice) sell_prefetch(A, slice, lane, R);
+        const int row = group_row(A, slice, slice, lane, row0, nrows);
+        const T acc = sell_finish<false>(A, R, dn, T(0));
+        if (row >= 0) {
+            const T dr = dn(row);
+            dnew[row] = dr;
+            q[row] = acc;
+            part += (double)(dr * acc);
+        }
+    }
+    block_partial<kSpmvBlock>(part, &spart[blockIdx.x]);
 }
 
-// x += a dx; r -= a q; then z.r by mode (solver/sparsesolver.cpp:160-174,
-// 372-389, 952-968); beta and the loop-head tests.
+// d.Ad step from dpart, then x += a dx; r -= a q; z.r by mode
+// (solver/sparsesolver.cpp:160-174, 372-389, 952-968) into zpart.
 template <typename T>
-__global__ void __launch_bounds__(kVecBlock) k_update(int mode, SolveState* st, int n, T* x,
-                                                      T* r, const T* dx,
-                                                      const T* q, T* z,
+__global__ void __launch_bounds__(kVecBlock) k_update(int mode, SolveState* st, int p, int n, T* x,
+                                                      T* r, const T* dx, const T* q, T* z,
                                                       const T* dp, T omega,
-                                                      double* partials)
+                                                      const double* dpart, int ndad,
+                                                      double* zpart)
 {
-    if (st->done) return;
-    const T alpha = (T)st->alpha;
+    if (__syncthreads_or(st->done)) return;  // see k_spmv_dir
+    const double zr = st->zrp[1 - p];
+    const double dAd = block_fold<kVecBlock>(dpart, ndad);
+    double a;
+    if (dad_step(st, zr, dAd, &a)) return;
+    const T alpha = (T)a;
     double part = 0.;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         x[i] += alpha * dx[i];
@@ -320,14 +555,12 @@ __global__ void __launch_bounds__(kVecBlock) k_update(int mode, SolveState* st, 
             part += (double)(zz * rr);
         }
     }
-    double total;
-    if (!grid_reduce_last<kVecBlock>(part, partials, &st->ticket, &total)) return;
-    if (threadIdx.x == 0) after_zr_new(st, total);
+    block_partial<kVecBlock>(part, &zpart[blockIdx.x]);
 }
 
-// solver/sparsesolver.cpp:912-922: q += t, d.Ad = sum q*d; breakdown tests.
+// solver/sparsesolver.cpp:912-922: q += t, partials of d.Ad = sum q*d.
 template <typename T>
-__global__ void __launch_bounds__(kVecBlock) k_eisenstat_combine(SolveState* st, int n, T* q,
+__global__ void __launch_bounds__(kVecBlock) k_eisenstat_combine(const SolveState* st, int n, T* q,
                                                                  const T* t, const T* d,
                                                                  double* partials)
 {
@@ -338,30 +571,16 @@ __global__ void __launch_bounds__(kVecBlock) k_eisenstat_combine(SolveState* st,
         q[i] = qq;
         part += (double)(qq * d[i]);
     }
-    double total;
-    if (!grid_reduce_last<kVecBlock>(part, partials, &st->ticket, &total)) return;
-    if (threadIdx.x == 0) after_dAd(st, total);
-}
-
-// d = src + beta*d (solver/sparsesolver.cpp:698-706 and its siblings).
-template <typename T>
-__global__ void __launch_bounds__(kVecBlock) k_direction(const SolveState* st, int n,
-                                                         const T* src, T* d)
-{
-    if (st->done) return;
-    const T beta = (T)st->beta;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) d[i] = src[i] + beta * d[i];
+    block_partial<kVecBlock>(part, &partials[blockIdx.x]);
 }
 
 // ------------------------------------------------------------------ mixed precision
 
-// out = b - A x in the part's rows, sum of squares into st->result.
+// out = b - A x in the part's rows, partials of the sum of squares.
 template <typename T>
 __global__ void __launch_bounds__(kSellBlock) k_residual_norm(SellViewT<T> A, int nslices, int row0,
-                                                              int nrows, SolveState* st,
-                                                              const T* b, const T* x, T* out,
-                                                              double* partials)
+                                                              int nrows, const T* b, const T* x,
+                                                              T* out, double* partials)
 {
     const int slice = blockIdx.x;
     const int lane = threadIdx.x;
@@ -372,9 +591,7 @@ __global__ void __launch_bounds__(kSellBlock) k_residual_norm(SellViewT<T> A, in
         out[row] = r;
         part = (double)(r * r);
     }
-    double total;
-    if (!grid_reduce_last<kSellBlock>(part, partials, &st->ticket, &total)) return;
-    if (threadIdx.x == 0) st->result = total;
+    block_partial<kSellBlock>(part, &partials[blockIdx.x]);
 }
 
 template <typename TD, typename TS>
@@ -389,13 +606,6 @@ __global__ void k_add_correction(double* x, const float* e, int n)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) x[i] += (double)e[i];
-}
-
-int vec_blocks(int n)
-{
-    int b = (n + kVecBlock - 1) / kVecBlock;
-    if (b < 1) b = 1;
-    return b > kVecMaxBlocks ? kVecMaxBlocks : b;
 }
 
 int flat_blocks(int n)
@@ -429,8 +639,10 @@ void launch_init_state(hipStream_t s, SolveState* st, const T* b, int n, double 
                        double host_tol, int use_host_tol, int maxK, int small_breaks, double rel2,
                        double* partials)
 {
-    k_init_state<T><<<use_host_tol ? 1 : vec_blocks(n), kVecBlock, 0, s>>>(
-        st, b, n, precision, host_tol, use_host_tol, maxK, small_breaks, rel2, partials);
+    const int grid = vec_reduce_blocks(n);
+    if (!use_host_tol) k_sumsq<T><<<grid, kVecBlock, 0, s>>>(b, n, partials);
+    k_init_state<<<1, kWave, 0, s>>>(st, partials, grid, precision, host_tol, use_host_tol, maxK,
+                                     small_breaks, rel2);
 }
 
 template <typename T>
@@ -447,8 +659,9 @@ void launch_residual(hipStream_t s, ResidualMode mode, SellViewT<T> A, int nslic
                      const T* precon, double* partials)
 {
     const int grid = nslices > 0 ? nslices : 1;
-    k_residual<T><<<grid, kSellBlock, 0, s>>>((int)mode, A, nslices, row0, nrows, st, b, x, out,
-                                              d, z, precon, partials);
+    k_residual<T><<<grid, kSellBlock, 0, s>>>((int)mode, A, nslices, row0, nrows, b, x, out, d, z,
+                                              precon, partials);
+    if (mode != RES_ONLY) k_restart_finish<<<1, kWave, 0, s>>>(st, partials, grid);
 }
 
 template <typename T>
@@ -456,49 +669,54 @@ void launch_residual_norm(hipStream_t s, SellViewT<T> A, int nslices, int row0, 
                           SolveState* st, const T* b, const T* x, T* out, double* partials)
 {
     const int grid = nslices > 0 ? nslices : 1;
-    k_residual_norm<T><<<grid, kSellBlock, 0, s>>>(A, nslices, row0, nrows, st, b, x, out,
-                                                   partials);
+    k_residual_norm<T><<<grid, kSellBlock, 0, s>>>(A, nslices, row0, nrows, b, x, out, partials);
+    k_result_finish<<<1, kWave, 0, s>>>(st, partials, grid);
 }
 
 template <typename T>
 void launch_direction_init(hipStream_t s, SolveState* st, int n_active, const T* z, const T* r,
                            T* d, double* partials)
 {
-    k_direction_init<T><<<vec_blocks(n_active), kVecBlock, 0, s>>>(st, n_active, z, r, d,
-                                                                   partials);
+    const int grid = vec_reduce_blocks(n_active);
+    k_direction_init<T><<<grid, kVecBlock, 0, s>>>(n_active, z, r, d, partials);
+    k_restart_finish<<<1, kWave, 0, s>>>(st, partials, grid);
 }
 
 template <typename T>
 void launch_eisenstat_init(hipStream_t s, SolveState* st, int n_active, const T* r,
                            const T* diag, T omega, T* z, T* d, double* partials)
 {
-    k_eisenstat_init<T><<<vec_blocks(n_active), kVecBlock, 0, s>>>(st, n_active, r, diag, omega,
-                                                                   z, d, partials);
+    const int grid = vec_reduce_blocks(n_active);
+    k_eisenstat_init<T><<<grid, kVecBlock, 0, s>>>(n_active, r, diag, omega, z, d, partials);
+    k_restart_finish<<<1, kWave, 0, s>>>(st, partials, grid);
 }
 
 template <typename T>
 void launch_forward(hipStream_t s, SellViewT<T> L, int slice0, int nslices, int row0, int nrows,
-                    SolveState* st, int update_xr, T* x, T* r, const T* d, const T* q, T* m,
-                    const T* diag, T omega)
+                    SolveState* st, int mode, int p, T* x, T* r, const T* d, const T* q, T* m,
+                    const T* diag, T omega, const double* spmv_partials, int nspmv)
 {
-    if (nslices <= 0) return;
-    k_forward<T><<<nslices, kSellBlock, 0, s>>>(L, slice0, nslices, row0, nrows, st, update_xr, x,
-                                                r, d, q, m, diag, omega);
+    // The first group of an iteration runs one block even when empty: it
+    // forms alpha for the groups after it.
+    const int grid = nslices > 0 ? nslices : (mode == FWD_FIRST ? 1 : 0);
+    if (grid == 0) return;
+    k_forward<T><<<grid, kSellBlock, 0, s>>>(L, slice0, nslices, row0, nrows, st, mode, p, x, r, d,
+                                             q, m, diag, omega, spmv_partials, nspmv);
 }
 
 template <typename T>
 void launch_backward(hipStream_t s, SellViewT<T> U, int slice0, int nslices, int row0, int nrows,
-                     SolveState* st, int reduce, int group, int last_group, const T* r,
-                     const T* m, T* z, const T* diag, T omega, T c2, double* grp,
-                     double* partials)
+                     SolveState* st, int reduce, const T* r, const T* m, T* z, const T* diag,
+                     T omega, T c2, double* group_partials, int prev0, int prevn,
+                     double* prev_sum)
 {
-    // A group with no rows still runs one block when it has to publish its
-    // (zero) partial or finish the z.r sum.
-    const int grid = nslices > 0 ? nslices : (reduce ? 1 : 0);
+    // An empty group still runs one block when it has to sum the previous
+    // group's partials.
+    const int grid = nslices > 0 ? nslices : (reduce && prev_sum ? 1 : 0);
     if (grid == 0) return;
-    k_backward<T><<<grid, kSellBlock, 0, s>>>(U, slice0, nslices, row0, nrows, st, reduce, group,
-                                              last_group, r, m, z, diag, omega, c2, grp,
-                                              partials);
+    k_backward<T><<<grid, kSellBlock, 0, s>>>(U, slice0, nslices, row0, nrows, st, reduce, r, m, z,
+                                              diag, omega, c2, group_partials, prev0, prevn,
+                                              prev_sum);
 }
 
 template <typename T>
@@ -512,35 +730,47 @@ void launch_eisenstat_solve(hipStream_t s, SellViewT<T> P, int slice0, int nslic
 }
 
 template <typename T>
-void launch_spmv_dot(hipStream_t s, SellViewT<T> A, int nslices, int row0, int nrows,
-                     SolveState* st, const T* d, T* q, double* partials)
+void launch_eisenstat_upper(hipStream_t s, SellViewT<T> P, int slice0, int nslices, int row0,
+                            int nrows, SolveState* st, int first, int p, const T* z,
+                            const T* dold, T* dnew, T* out, const T* diag, T omega,
+                            const double* fold_partials, int nfold)
 {
-    const int grid = nslices > 0 ? nslices : 1;
-    k_spmv_dot<T><<<grid, kSellBlock, 0, s>>>(A, nslices, row0, nrows, st, d, q, partials);
+    const int grid = nslices > 0 ? nslices : (first ? 1 : 0);
+    if (grid == 0) return;
+    k_eisenstat_upper<T><<<grid, kSellBlock, 0, s>>>(P, slice0, nslices, row0, nrows, st, first, p,
+                                                     z, dold, dnew, out, diag, omega,
+                                                     fold_partials, nfold);
 }
 
 template <typename T>
-void launch_update(hipStream_t s, UpdateMode mode, SolveState* st, int n_active, T* x, T* r,
-                   const T* dx, const T* q, T* z, const T* diag_or_precon, T omega,
-                   double* partials)
+void launch_spmv_dir(hipStream_t s, SellViewT<T> A, int nslices, int row0, int nrows,
+                     int n_active, SolveState* st, int p, const T* src, const T* dold, T* dnew,
+                     T* q, double* spmv_partials, const double* fold_partials, int nfold,
+                     const double* group_sums, int ngroups)
 {
-    k_update<T><<<vec_blocks(n_active), kVecBlock, 0, s>>>((int)mode, st, n_active, x, r, dx, q,
-                                                           z, diag_or_precon, omega, partials);
+    k_spmv_dir<T><<<spmv_blocks(nslices), kSpmvBlock, 0, s>>>(A, nslices, row0, nrows, n_active,
+                                                              st, p, src,
+                                                              dold, dnew, q, spmv_partials,
+                                                              fold_partials, nfold, group_sums,
+                                                              ngroups);
+}
+
+template <typename T>
+void launch_update(hipStream_t s, UpdateMode mode, SolveState* st, int p, int n_active, T* x,
+                   T* r, const T* dx, const T* q, T* z, const T* diag_or_precon, T omega,
+                   const double* dad_partials, int ndad, double* zr_partials)
+{
+    k_update<T><<<vec_reduce_blocks(n_active), kVecBlock, 0, s>>>(
+        (int)mode, st, p, n_active, x, r, dx, q, z, diag_or_precon, omega, dad_partials, ndad,
+        zr_partials);
 }
 
 template <typename T>
 void launch_eisenstat_combine(hipStream_t s, SolveState* st, int n_active, T* q, const T* t,
                               const T* d, double* partials)
 {
-    k_eisenstat_combine<T><<<vec_blocks(n_active), kVecBlock, 0, s>>>(st, n_active, q, t, d,
-                                                                      partials);
-}
-
-template <typename T>
-void launch_direction(hipStream_t s, SolveState* st, int n_active, const T* src, T* d)
-{
-    if (n_active > 0)
-        k_direction<T><<<flat_blocks(n_active), kVecBlock, 0, s>>>(st, n_active, src, d);
+    k_eisenstat_combine<T><<<vec_reduce_blocks(n_active), kVecBlock, 0, s>>>(st, n_active, q, t, d,
+                                                                            partials);
 }
 
 void launch_to_float(hipStream_t s, float* dst, const double* src, int n)
@@ -570,20 +800,25 @@ void launch_add_correction(hipStream_t s, double* x, const float* e, int n)
     template void launch_eisenstat_init<T>(hipStream_t, SolveState*, int, const T*, const T*,  \
                                            T, T*, T*, double*);                                \
     template void launch_forward<T>(hipStream_t, SellViewT<T>, int, int, int, int, SolveState*, \
-                                    int, T*, T*, const T*, const T*, T*, const T*, T);         \
+                                    int, int, T*, T*, const T*, const T*, T*, const T*, T,     \
+                                    const double*, int);                                       \
     template void launch_backward<T>(hipStream_t, SellViewT<T>, int, int, int, int,            \
-                                     SolveState*, int, int, int, const T*, const T*, T*,       \
-                                     const T*, T, T, double*, double*);                        \
+                                     SolveState*, int, const T*, const T*, T*, const T*, T, T, \
+                                     double*, int, int, double*);                              \
     template void launch_eisenstat_solve<T>(hipStream_t, SellViewT<T>, int, int, int, int,     \
                                             SolveState*, int, const T*, const T*, T*,          \
                                             const T*, T, T);                                   \
-    template void launch_spmv_dot<T>(hipStream_t, SellViewT<T>, int, int, int, SolveState*,    \
-                                     const T*, T*, double*);                                   \
-    template void launch_update<T>(hipStream_t, UpdateMode, SolveState*, int, T*, T*,          \
-                                   const T*, const T*, T*, const T*, T, double*);              \
+    template void launch_eisenstat_upper<T>(hipStream_t, SellViewT<T>, int, int, int, int,     \
+                                            SolveState*, int, int, const T*, const T*, T*, T*, \
+                                            const T*, T, const double*, int);                  \
+    template void launch_spmv_dir<T>(hipStream_t, SellViewT<T>, int, int, int, int,            \
+                                     SolveState*, int, const T*, const T*, T*, T*, double*,    \
+                                     const double*, int, const double*, int);                  \
+    template void launch_update<T>(hipStream_t, UpdateMode, SolveState*, int, int, T*, T*,     \
+                                   const T*, const T*, T*, const T*, T, const double*, int,    \
+                                   double*);                                                   \
     template void launch_eisenstat_combine<T>(hipStream_t, SolveState*, int, T*, const T*,     \
-                                              const T*, double*);                              \
-    template void launch_direction<T>(hipStream_t, SolveState*, int, const T*, T*);
+                                              const T*, double*);
 
 SFA_INSTANTIATE(double)
 SFA_INSTANTIATE(float)

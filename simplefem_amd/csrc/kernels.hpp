@@ -6,6 +6,16 @@
 // SolveState; every kernel of an iteration first reads state->done and
 // returns at once when the solve has finished, so a captured chunk of
 // iterations can be replayed past convergence without changing the result.
+//
+// Reductions are deferred across a kernel boundary: the producing kernel
+// stores one partial sum per block (plain stores), and the kernel that runs
+// next on the stream -- which sees them because kernels on one stream are
+// ordered -- sums them in a fixed order in every block that needs the total
+// and runs the scalar part of the CG step there.  No block waits for another
+// and no grid-wide atomics or cache-maintenance fences are needed.  Values
+// that a kernel both reads and updates (iteration count, z.r, the direction
+// d) are double-buffered by the parity p of the iteration: the iteration of
+// parity p reads slot p and writes slot 1-p.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -22,27 +32,44 @@ struct SellViewT {
 using SellView = SellViewT<double>;
 
 struct SolveState {
-    double zr;          // z.r of the current iterate
-    double dAd;
-    double alpha;
-    double beta;
+    // fixed during the iteration loop
     double tol;         // max(precision*||b||, sqrt(eps))
     double tol2;        // tol*tol
     double rel2;        // > 0: tol2 = rel2 * |z.r of the restart| (inner solves)
-    double result;      // output of k_residual_norm
-    int K;              // iterations done
     int maxK;           // N
+    int small_breaks;   // 0: a small d.Ad does not end the loop (natural Eisenstat, quiet)
+    // double-buffered by iteration parity
+    int Kp[2];          // iterations done when the iteration of parity p starts
+    int fold[2];        // 1: that iteration first folds the z.r of the previous one
+    double zrp[2];      // z.r when that iteration's d.Ad step runs
+    // written by one thread of the kernel that forms them, read by later kernels
+    double alpha;
+    double beta;
+    // host-facing record
+    double zr;          // z.r of the current iterate
+    double dAd;
+    double result;      // output of the residual norm
+    int K;              // iterations done
     int done;           // 1 once the loop has ended
     int exit_reason;    // SFA_EXIT_*
-    int small_breaks;   // 0: a small d.Ad does not end the loop (natural Eisenstat, quiet)
-    unsigned ticket;    // last-block counter of the running reduction
+    int pad_;
     double* hist;       // z.r at the top of iteration k, k = 0..K
 };
+
+// Block sizes and grid limits of the reducing kernels (their partial counts).
+constexpr int kSpmvWaves = 4;          // slices per block of the SpMV kernel
+constexpr int kSpmvMaxBlocks = 1024;
+constexpr int kVecReduceBlocks = 512;  // vector kernels with a reduction
+
+int spmv_blocks(int nslices);
+int vec_reduce_blocks(int n);
 
 // Vector kernels
 template <typename T> void launch_fill(hipStream_t s, T* v, T value, int n);
 template <typename T> void launch_gather(hipStream_t s, T* dst, const T* src, const int* perm, int n);
 template <typename T> void launch_scatter(hipStream_t s, T* dst, const T* src, const int* perm, int n);
+// Resets the state for a solve: tol = max(precision*||b||, 2^-26) (or host_tol),
+// Kp = {0, 0}, fold = {0, 0}.
 template <typename T>
 void launch_init_state(hipStream_t s, SolveState* st, const T* b, int n, double precision,
                        double host_tol, int use_host_tol, int maxK, int small_breaks, double rel2,
@@ -58,8 +85,8 @@ void launch_apply(hipStream_t s, SellViewT<T> A, int nslices, int row0, int nrow
 // --- restart (K = 0) ---
 enum ResidualMode { RES_ONLY = 0, RES_PLAIN = 1, RES_JACOBI = 2 };
 // r = b - A x on the part's rows; RES_PLAIN also sets d = r and z.r = r.r,
-// RES_JACOBI z = p*r, d = z, z.r; both then test convergence.  With
-// out != r the result goes to out (the Eisenstat restart writes q).
+// RES_JACOBI z = p*r, d = z, z.r; both then run the restart's scalar step
+// (z.r, loop-head test).  With out != r the result goes to out.
 template <typename T>
 void launch_residual(hipStream_t s, ResidualMode mode, SellViewT<T> A, int nslices, int row0,
                      int nrows, SolveState* st, const T* b, const T* x, T* out, T* d, T* z,
@@ -68,51 +95,74 @@ void launch_residual(hipStream_t s, ResidualMode mode, SellViewT<T> A, int nslic
 template <typename T>
 void launch_residual_norm(hipStream_t s, SellViewT<T> A, int nslices, int row0, int nrows,
                           SolveState* st, const T* b, const T* x, T* out, double* partials);
-// d = z, z.r, convergence test (SSOR family restart).
+// d = z, z.r, restart scalar step (SSOR family).
 template <typename T>
 void launch_direction_init(hipStream_t s, SolveState* st, int n_active, const T* z, const T* r,
                            T* d, double* partials);
-// Eisenstat restart: z = r*diag*(2-w)/w, d = z, z.r, convergence test.
+// Eisenstat restart: z = r*diag*(2-w)/w, d = z, z.r, restart scalar step.
 template <typename T>
 void launch_eisenstat_init(hipStream_t s, SolveState* st, int n_active, const T* r,
                            const T* diag, T omega, T* z, T* d, double* partials);
 
 // --- sweeps over one group (colour or level) of a triangular part ---
-// forward: [x += a d; r -= a q;]  m = (r - L m) * w / diag
+// mode FWD_RESTART: m = (r - L m) * w / diag.
+// mode FWD_UPDATE:  x += a d; r -= a q; then the same, a = st->alpha.
+// mode FWD_FIRST:   as FWD_UPDATE, but first forms d.Ad from the SpMV's
+//                   partials (spmv_partials, nspmv), runs the breakdown tests
+//                   and computes a (first group of the iteration, parity p).
+enum ForwardMode { FWD_RESTART = 0, FWD_UPDATE = 1, FWD_FIRST = 2 };
 template <typename T>
 void launch_forward(hipStream_t s, SellViewT<T> L, int slice0, int nslices, int row0, int nrows,
-                    SolveState* st, int update_xr, T* x, T* r, const T* d, const T* q, T* m,
-                    const T* diag, T omega);
-// backward: t = m*diag*c2; z = (t - U z) * w / diag; [group z.r partial]
+                    SolveState* st, int mode, int p, T* x, T* r, const T* d, const T* q, T* m,
+                    const T* diag, T omega, const double* spmv_partials, int nspmv);
+// backward: t = m*diag*c2; z = (t - U z) * w / diag.  With reduce, each
+// block stores its z.r partial at group_partials[slice0 + block], and block 0
+// sums the previous group's partials (prev0, prevn) into *prev_sum.
 template <typename T>
 void launch_backward(hipStream_t s, SellViewT<T> U, int slice0, int nslices, int row0, int nrows,
-                     SolveState* st, int reduce, int group, int last_group, const T* r,
-                     const T* m, T* z, const T* diag, T omega, T c2, double* grp,
-                     double* partials);
+                     SolveState* st, int reduce, const T* r, const T* m, T* z, const T* diag,
+                     T omega, T c2, double* group_partials, int prev0, int prevn,
+                     double* prev_sum);
 // Eisenstat: out = (in - [c3*diag*t] - P out) * w / diag  (P = L or U part)
 template <typename T>
 void launch_eisenstat_solve(hipStream_t s, SellViewT<T> P, int slice0, int nslices, int row0,
                             int nrows, SolveState* st, int with_t, const T* in, const T* t,
                             T* out, const T* diag, T omega, T c3);
+// Eisenstat upper solve with the direction update of the previous
+// iteration fused in: dn = fold ? z + beta*dold : dold on the group's rows,
+// dnew = dn, out = (dn - U out) * w / diag.  first: this group opens the
+// iteration and forms z.r, beta, K from the update's partials (fold_partials,
+// nfold); later groups read beta from the state.
+template <typename T>
+void launch_eisenstat_upper(hipStream_t s, SellViewT<T> P, int slice0, int nslices, int row0,
+                            int nrows, SolveState* st, int first, int p, const T* z,
+                            const T* dold, T* dnew, T* out, const T* diag, T omega,
+                            const double* fold_partials, int nfold);
 
 // --- iteration ---
-// q = A d, d.Ad, breakdown tests, alpha
+// Opens an iteration of parity p for the SpMV-based variants: when
+// state.fold[p], forms the new z.r from the previous iteration's partials
+// (sum of group_sums[0..ngroups) then of fold_partials[0..nfold)), beta, K
+// and the loop-head tests; then dn = fold ? src + beta*dold : dold on the
+// active rows (dold elsewhere), dnew = dn, q = A dn and the d.Ad partials
+// (spmv_partials, one per block).
 template <typename T>
-void launch_spmv_dot(hipStream_t s, SellViewT<T> A, int nslices, int row0, int nrows,
-                     SolveState* st, const T* d, T* q, double* partials);
+void launch_spmv_dir(hipStream_t s, SellViewT<T> A, int nslices, int row0, int nrows,
+                     int n_active, SolveState* st, int p, const T* src, const T* dold, T* dnew,
+                     T* q,
+                     double* spmv_partials, const double* fold_partials, int nfold,
+                     const double* group_sums, int ngroups);
 enum UpdateMode { UPD_PLAIN = 0, UPD_JACOBI = 1, UPD_EISENSTAT = 2 };
-// x += a*dx; r -= a*q; z.r per mode; beta; K++; convergence test
+// Forms d.Ad from dad_partials (ndad), breakdown tests, alpha; then
+// x += a*dx; r -= a*q; z.r per mode into zr_partials (one per block).
 template <typename T>
-void launch_update(hipStream_t s, UpdateMode mode, SolveState* st, int n_active, T* x, T* r,
-                   const T* dx, const T* q, T* z, const T* diag_or_precon, T omega,
-                   double* partials);
-// Eisenstat: q += t, d.Ad, breakdown tests, alpha
+void launch_update(hipStream_t s, UpdateMode mode, SolveState* st, int p, int n_active, T* x,
+                   T* r, const T* dx, const T* q, T* z, const T* diag_or_precon, T omega,
+                   const double* dad_partials, int ndad, double* zr_partials);
+// Eisenstat: q += t, d.Ad partials (one per block).
 template <typename T>
 void launch_eisenstat_combine(hipStream_t s, SolveState* st, int n_active, T* q, const T* t,
                               const T* d, double* partials);
-// d = src + beta*d
-template <typename T>
-void launch_direction(hipStream_t s, SolveState* st, int n_active, const T* src, T* d);
 
 // mixed precision helpers
 void launch_to_float(hipStream_t s, float* dst, const double* src, int n);

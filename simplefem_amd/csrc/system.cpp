@@ -66,14 +66,19 @@ struct SystemT {
 
     int* perm = nullptr;
     T* diag = nullptr;
-    T *x = nullptr, *r = nullptr, *b = nullptr, *d = nullptr, *z = nullptr, *q = nullptr,
-      *m = nullptr;
+    // d and d1: the search direction, double-buffered by iteration parity
+    T *x = nullptr, *r = nullptr, *b = nullptr, *d = nullptr, *d1 = nullptr, *z = nullptr,
+      *q = nullptr, *m = nullptr;
     T* io = nullptr;  // n staging entries for host <-> device copies
     T* io2 = nullptr;
     T* io3 = nullptr;
     SolveState* state = nullptr;
     double* hist = nullptr;
-    double* partials = nullptr;
+    // block partials of the deferred reductions: SpMV / combine / tolerance
+    // (ps), update / restart (pu), backward groups by slice (pb); per-group z.r
+    double* ps = nullptr;
+    double* pu = nullptr;
+    double* pb = nullptr;
     double* grp = nullptr;
     SolveState* host_state = nullptr;  // pinned
 
@@ -113,13 +118,13 @@ struct SystemT {
         upper.release();
         lower_restart.release();
         apply_part.release();
-        for (void* p : {(void*)perm, (void*)diag, (void*)x, (void*)r, (void*)b, (void*)d, (void*)z,
-                        (void*)q, (void*)m, (void*)io, (void*)io2, (void*)io3, (void*)state,
-                        (void*)hist, (void*)partials, (void*)grp})
+        for (void* p : {(void*)perm, (void*)diag, (void*)x, (void*)r, (void*)b, (void*)d, (void*)d1,
+                        (void*)z, (void*)q, (void*)m, (void*)io, (void*)io2, (void*)io3,
+                        (void*)state, (void*)hist, (void*)ps, (void*)pu, (void*)pb, (void*)grp})
             dev_free(p);
         perm = nullptr;
-        diag = x = r = b = d = z = q = m = io = io2 = io3 = nullptr;
-        hist = partials = grp = nullptr;
+        diag = x = r = b = d = d1 = z = q = m = io = io2 = io3 = nullptr;
+        hist = ps = pu = pb = grp = nullptr;
         state = nullptr;
         if (host_state) (void)hipHostFree(host_state);
         host_state = nullptr;
@@ -181,7 +186,7 @@ struct SystemT {
         dg.push_back(T(0));
         SFA_HIP(hipMemcpy(diag, dg.data(), (n + 1) * sizeof(T), hipMemcpyHostToDevice));
 
-        T** vecs[] = {&x, &r, &b, &d, &z, &q, &m};
+        T** vecs[] = {&x, &r, &b, &d, &d1, &z, &q, &m};
         for (T** v : vecs) {
             *v = dev_alloc<T>(n + 1);
             SFA_HIP(hipMemset(*v, 0, (n + 1) * sizeof(T)));
@@ -191,10 +196,9 @@ struct SystemT {
         io3 = dev_alloc<T>(n);
         state = dev_alloc<SolveState>(1);
         hist = dev_alloc<double>((size_t)n + 1);
-        int max_blocks = 4096;
-        for (const DevPartT<T>* p : {&full, &lower, &upper, &lower_restart})
-            max_blocks = std::max(max_blocks, p->nslices + 1);
-        partials = dev_alloc<double>(max_blocks);
+        ps = dev_alloc<double>(std::max(spmv_blocks(full.nslices), kVecReduceBlocks));
+        pu = dev_alloc<double>(std::max({full.nslices, kVecReduceBlocks, 1}));
+        pb = dev_alloc<double>(std::max(sweeps ? upper.nslices : 0, 1));
         int ngr = 1;
         if (sweeps) ngr = std::max({ngr, lower.ngroups(), upper.ngroups()});
         grp = dev_alloc<double>(ngr);
@@ -211,27 +215,36 @@ struct SystemT {
     T c3() const { return T(2. / omega - 1.); }
     T w() const { return T(omega); }
 
-    void forward_sweep(hipStream_t s, const DevPartT<T>& L, int update_xr)
+    // mode FWD_RESTART / FWD_UPDATE; the first group of an iteration runs as
+    // FWD_FIRST (alpha from the SpMV's partials).
+    void forward_sweep(hipStream_t s, const DevPartT<T>& L, int mode, int p, const T* dd)
     {
         // vector streams per row: x r/w, d, r r/w, q, diag, m (iteration);
         // r, diag, m (restart)
-        const double per_row = update_xr ? 64.0 : 24.0;
-        for (int g = 0; g < L.ngroups(); g++)
+        const double per_row = mode == FWD_RESTART ? 24.0 : 64.0;
+        const int nspmv = spmv_blocks(full.nslices);
+        for (int g = 0; g < L.ngroups(); g++) {
+            const int gm = (mode != FWD_RESTART && g == 0) ? FWD_FIRST : mode;
             run(s, SFA_KERNEL_FORWARD, L.bytes(g, per_row), [&] {
                 launch_forward(s, L.view(), L.group_begin(g), L.group_count(g), L.row0(g),
-                               L.nrows_of(g), state, update_xr, x, r, d, q, m, diag, w());
+                               L.nrows_of(g), state, gm, p, x, r, dd, q, m, diag, w(), ps, nspmv);
             });
+        }
     }
 
     void backward_sweep(hipStream_t s, int reduce)
     {
         const int G = upper.ngroups();
-        for (int g = 0; g < G; g++)  // m, diag, z, r
+        for (int g = 0; g < G; g++) {  // m, diag, z, r
+            const bool fold_prev = reduce && g > 0;
             run(s, SFA_KERNEL_BACKWARD, upper.bytes(g, 32.0), [&] {
                 launch_backward(s, upper.view(), upper.group_begin(g), upper.group_count(g),
-                                upper.row0(g), upper.nrows_of(g), state, reduce, g, g == G - 1, r,
-                                m, z, diag, w(), c2(), grp, partials);
+                                upper.row0(g), upper.nrows_of(g), state, reduce, r, m, z, diag,
+                                w(), c2(), pb, fold_prev ? upper.group_begin(g - 1) : 0,
+                                fold_prev ? upper.group_count(g - 1) : 0,
+                                fold_prev ? grp + (g - 1) : (double*)nullptr);
             });
+        }
     }
 
     void eisenstat_sweep(hipStream_t s, const DevPartT<T>& P, int with_t, const T* in,
@@ -246,7 +259,7 @@ struct SystemT {
             });
     }
 
-    // Restart (K = 0) of each variant.
+    // Restart (K = 0) of each variant; the direction goes to d (parity 0).
     void restart(hipStream_t s)
     {
         const SellViewT<T> F = full.view();
@@ -254,87 +267,98 @@ struct SystemT {
         switch (method) {
         case SFA_METHOD_CG:
             run(s, SFA_KERNEL_RESTART, full.bytes_all(32.0), [&] {
-                launch_residual(s, RES_PLAIN, F, full.nslices, full.row0(0), full.nrows_of(0), state, b, x, r, d, (T*)nullptr,
-                                (const T*)nullptr, partials);
+                launch_residual(s, RES_PLAIN, F, full.nslices, full.row0(0), full.nrows_of(0),
+                                state, b, x, r, d, (T*)nullptr, (const T*)nullptr, pu);
             });
             break;
         case SFA_METHOD_DIAGONAL:
             run(s, SFA_KERNEL_RESTART, full.bytes_all(48.0), [&] {
-                launch_residual(s, RES_JACOBI, F, full.nslices, full.row0(0), full.nrows_of(0), state, b, x, r, d, z, diag,
-                                partials);
+                launch_residual(s, RES_JACOBI, F, full.nslices, full.row0(0), full.nrows_of(0),
+                                state, b, x, r, d, z, diag, pu);
             });
             break;
         case SFA_METHOD_SSOR:
         case SFA_METHOD_RAINBOW:
             run(s, SFA_KERNEL_RESTART, full.bytes_all(16.0), [&] {
-                launch_residual(s, RES_ONLY, F, full.nslices, full.row0(0), full.nrows_of(0), state, b, x, r, (T*)nullptr, (T*)nullptr,
-                                (const T*)nullptr, partials);
+                launch_residual(s, RES_ONLY, F, full.nslices, full.row0(0), full.nrows_of(0),
+                                state, b, x, r, (T*)nullptr, (T*)nullptr, (const T*)nullptr, pu);
             });
-            forward_sweep(s, lower, 0);
+            forward_sweep(s, lower, FWD_RESTART, 0, d);
             backward_sweep(s, 0);
             run(s, SFA_KERNEL_RESTART, 24.0 * na,
-                [&] { launch_direction_init(s, state, n_active, z, r, d, partials); });
+                [&] { launch_direction_init(s, state, n_active, z, r, d, pu); });
             break;
         default:  // Eisenstat variants: q = b - A x, then r = lower solve of q
             run(s, SFA_KERNEL_RESTART, full.bytes_all(16.0), [&] {
-                launch_residual(s, RES_ONLY, F, full.nslices, full.row0(0), full.nrows_of(0), state, b, x, q, (T*)nullptr, (T*)nullptr,
-                                (const T*)nullptr, partials);
+                launch_residual(s, RES_ONLY, F, full.nslices, full.row0(0), full.nrows_of(0),
+                                state, b, x, q, (T*)nullptr, (T*)nullptr, (const T*)nullptr, pu);
             });
             eisenstat_sweep(s, has_lower_restart ? lower_restart : lower, 0, q, (const T*)nullptr, r);
             run(s, SFA_KERNEL_RESTART, 32.0 * na, [&] {
-                launch_eisenstat_init(s, state, n_active, r, diag, w(), z, d, partials);
+                launch_eisenstat_init(s, state, n_active, r, diag, w(), z, d, pu);
             });
             break;
         }
     }
 
-    // One CG iteration of each variant.
-    void iteration(hipStream_t s)
+    // One CG iteration of parity p: reads the direction of slot p, writes
+    // slot 1-p.  The z.r of the previous iteration is folded in by this
+    // iteration's first kernel.
+    void iteration(hipStream_t s, int p)
     {
         const SellViewT<T> F = full.view();
         const double na = n_active;
-        const double spmv_bytes = full.bytes_all(16.0);  // d, q
+        T* dold = p ? d1 : d;
+        T* dnew = p ? d : d1;
+        const double spmv_bytes = full.bytes_all(32.0);  // src, dold, dnew, q
+        const int nspmv = spmv_blocks(full.nslices);
+        const int nvec = vec_reduce_blocks(n_active);
         switch (method) {
         case SFA_METHOD_CG:
-            run(s, SFA_KERNEL_SPMV, spmv_bytes,
-                [&] { launch_spmv_dot(s, F, full.nslices, full.row0(0), full.nrows_of(0), state, d, q, partials); });
-            run(s, SFA_KERNEL_UPDATE, 48.0 * na, [&] {
-                launch_update(s, UPD_PLAIN, state, n_active, x, r, d, q, (T*)nullptr,
-                              (const T*)nullptr, w(), partials);
+        case SFA_METHOD_DIAGONAL: {
+            const bool jac = method == SFA_METHOD_DIAGONAL;
+            run(s, SFA_KERNEL_SPMV, spmv_bytes, [&] {
+                launch_spmv_dir(s, F, full.nslices, full.row0(0), full.nrows_of(0), n_active,
+                                state, p, jac ? z : r, dold, dnew, q, ps, pu, nvec,
+                                (const double*)nullptr, 0);
             });
-            run(s, SFA_KERNEL_DIRECTION, 24.0 * na,
-                [&] { launch_direction(s, state, n_active, r, d); });
-            break;
-        case SFA_METHOD_DIAGONAL:
-            run(s, SFA_KERNEL_SPMV, spmv_bytes,
-                [&] { launch_spmv_dot(s, F, full.nslices, full.row0(0), full.nrows_of(0), state, d, q, partials); });
-            run(s, SFA_KERNEL_UPDATE, 64.0 * na, [&] {
-                launch_update(s, UPD_JACOBI, state, n_active, x, r, d, q, z, diag, w(),
-                              partials);
+            run(s, SFA_KERNEL_UPDATE, (jac ? 64.0 : 48.0) * na, [&] {
+                launch_update(s, jac ? UPD_JACOBI : UPD_PLAIN, state, p, n_active, x, r, dnew, q,
+                              jac ? z : (T*)nullptr, jac ? diag : (const T*)nullptr, w(), ps,
+                              nspmv, pu);
             });
-            run(s, SFA_KERNEL_DIRECTION, 24.0 * na,
-                [&] { launch_direction(s, state, n_active, z, d); });
             break;
+        }
         case SFA_METHOD_SSOR:
-        case SFA_METHOD_RAINBOW:
-            run(s, SFA_KERNEL_SPMV, spmv_bytes,
-                [&] { launch_spmv_dot(s, F, full.nslices, full.row0(0), full.nrows_of(0), state, d, q, partials); });
-            forward_sweep(s, lower, 1);
-            backward_sweep(s, 1);
-            run(s, SFA_KERNEL_DIRECTION, 24.0 * na,
-                [&] { launch_direction(s, state, n_active, z, d); });
-            break;
-        default:  // Eisenstat: t (in m) = upper solve of d, q = lower solve
-            eisenstat_sweep(s, upper, 0, d, (const T*)nullptr, m);
-            eisenstat_sweep(s, lower, 1, d, m, q);
-            run(s, SFA_KERNEL_COMBINE, 32.0 * na,
-                [&] { launch_eisenstat_combine(s, state, n_active, q, m, d, partials); });
-            run(s, SFA_KERNEL_UPDATE, 64.0 * na, [&] {
-                launch_update(s, UPD_EISENSTAT, state, n_active, x, r, m, q, z, diag, w(),
-                              partials);
+        case SFA_METHOD_RAINBOW: {
+            // z.r of the last backward group comes from its block partials,
+            // the other groups' sums from grp
+            const int G = upper.ngroups();
+            run(s, SFA_KERNEL_SPMV, spmv_bytes, [&] {
+                launch_spmv_dir(s, F, full.nslices, full.row0(0), full.nrows_of(0), n_active,
+                                state, p, z, dold, dnew, q, ps,
+                                G > 0 ? pb + upper.group_begin(G - 1) : pb,
+                                G > 0 ? upper.group_count(G - 1) : 0, grp, G > 0 ? G - 1 : 0);
             });
-            run(s, SFA_KERNEL_DIRECTION, 24.0 * na,
-                [&] { launch_direction(s, state, n_active, z, d); });
+            forward_sweep(s, lower, FWD_UPDATE, p, dnew);
+            backward_sweep(s, 1);
+            break;
+        }
+        default:  // Eisenstat: t (in m) = upper solve of d, q = lower solve
+            for (int g = 0; g < upper.ngroups(); g++)  // z, dold, dnew, diag, m
+                run(s, SFA_KERNEL_TRIANGULAR, upper.bytes(g, 40.0), [&] {
+                    launch_eisenstat_upper(s, upper.view(), upper.group_begin(g),
+                                           upper.group_count(g), upper.row0(g), upper.nrows_of(g),
+                                           state, g == 0, p, z, dold, dnew, m, diag, w(), pu,
+                                           nvec);
+                });
+            eisenstat_sweep(s, lower, 1, dnew, m, q);
+            run(s, SFA_KERNEL_COMBINE, 32.0 * na,
+                [&] { launch_eisenstat_combine(s, state, n_active, q, m, dnew, ps); });
+            run(s, SFA_KERNEL_UPDATE, 64.0 * na, [&] {
+                launch_update(s, UPD_EISENSTAT, state, p, n_active, x, r, m, q, z, diag, w(), ps,
+                              nvec, pu);
+            });
             break;
         }
     }
@@ -343,10 +367,10 @@ struct SystemT {
     {
         switch (method) {
         case SFA_METHOD_CG:
-        case SFA_METHOD_DIAGONAL: return 3;
+        case SFA_METHOD_DIAGONAL: return 2;
         case SFA_METHOD_SSOR:
-        case SFA_METHOD_RAINBOW: return 2 + lower.ngroups() + upper.ngroups();
-        default: return 3 + lower.ngroups() + upper.ngroups();
+        case SFA_METHOD_RAINBOW: return 1 + lower.ngroups() + upper.ngroups();
+        default: return 2 + lower.ngroups() + upper.ngroups();
         }
     }
 
@@ -359,7 +383,7 @@ struct SystemT {
         }
         hipGraph_t graph = nullptr;
         SFA_HIP(hipStreamBeginCapture(own_stream, hipStreamCaptureModeThreadLocal));
-        for (int i = 0; i < iters; i++) iteration(own_stream);
+        for (int i = 0; i < iters; i++) iteration(own_stream, i & 1);
         SFA_HIP(hipStreamEndCapture(own_stream, &graph));
         hipError_t e = hipGraphInstantiate(&chunk_exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
@@ -378,7 +402,7 @@ struct SystemT {
     void init_scratch(hipStream_t s)
     {
         const T init = (method == SFA_METHOD_SSOR || method == SFA_METHOD_RAINBOW) ? T(0) : T(NAN);
-        for (T* v : {d, z, q, m}) launch_fill(s, v, init, n);
+        for (T* v : {d, d1, z, q, m}) launch_fill(s, v, init, n);
     }
 
     // ---------------------------------------------------------------- solve
@@ -426,7 +450,7 @@ struct SystemT {
         init_scratch(s);
         const int small_breaks = (method == SFA_METHOD_SSOR_EISENSTAT) ? (print_modulo >= -2) : 1;
         launch_init_state(s, state, b, n, precision, host_tol, use_host_tol, n, small_breaks,
-                          rel2, partials);
+                          rel2, ps);
         restart(s);
         read_state();
     }
@@ -488,8 +512,8 @@ struct SystemT {
         prof = &p;
         try {
             hipStream_t s = begin_solve(ux, ub, ures, precision, -3, device_pointers);
-            while (!host_state->done) {
-                iteration(s);
+            for (int p = 0; !host_state->done; p ^= 1) {
+                iteration(s, p);
                 read_state();
             }
             finish_solve(s, ux, ures, -3, device_pointers, info);
@@ -610,7 +634,7 @@ struct SystemT {
         launch_gather(s, x, src_x, perm, n);
         launch_gather(s, b, src_b, perm, n);
         if (device_pointers) {  // ||b|| on the device (k_init_state with precision 1)
-            launch_init_state(s, state, b, n, 1.0, 0., 0, n, 1, 0., partials);
+            launch_init_state(s, state, b, n, 1.0, 0., 0, n, 1, 0., ps);
             read_state();
             bnorm2 = host_state->tol * host_state->tol;
         }
@@ -622,7 +646,7 @@ struct SystemT {
         double res = 0.;
         for (;;) {
             launch_residual_norm(s, full.view(), full.nslices, full.row0(0), full.nrows_of(0),
-                                 state, b, x, r, partials);
+                                 state, b, x, r, pu);
             read_state();
             res = std::sqrt(host_state->result);
             if (!(res > target) || outer >= max_outer) break;
@@ -664,16 +688,16 @@ struct SystemT {
         o.stored_entries = full.stored + lower.stored + upper.stored;
         // Algorithmic bytes, the same model the profiled launches use.
         const double na = n_active;
-        const double spmv = full.bytes_all(16.0);
+        const double spmv = full.bytes_all(32.0);
         double it = 0.;
         if (method == SFA_METHOD_SSOR || method == SFA_METHOD_RAINBOW)
-            it = spmv + lower.bytes_all(64.0) + upper.bytes_all(32.0) + 24.0 * na;
+            it = spmv + lower.bytes_all(64.0) + upper.bytes_all(32.0);
         else if (method == SFA_METHOD_CG)
-            it = spmv + 48.0 * na + 24.0 * na;
+            it = spmv + 48.0 * na;
         else if (method == SFA_METHOD_DIAGONAL)
-            it = spmv + 64.0 * na + 24.0 * na;
+            it = spmv + 64.0 * na;
         else
-            it = upper.bytes_all(24.0) + lower.bytes_all(32.0) + (32.0 + 64.0 + 24.0) * na;
+            it = upper.bytes_all(40.0) + lower.bytes_all(32.0) + (32.0 + 64.0) * na;
         o.bytes_spmv = spmv;
         o.bytes_per_iteration = it;
         o.kernels_per_iteration = kernels_per_iteration();
