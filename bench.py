@@ -28,6 +28,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "solves/s + HBM GB/s (3D P1 Poisson, rtol 1e-10); p50/p99 served latency"
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per launch of each kernel class on the default workload (64^3,
+# omega 1): rocprofv3 FETCH_SIZE and WRITE_SIZE runs, each scaled by what the
+# counter reads for a device-to-device copy of known size (bench.py
+# --copy-calibration); commands and per-launch figures are kept in
+# profiles/rainbow_cg_ssor_64cube_fp64_hbm_traffic.md.
+MEASURED_TRAFFIC = {"spmv": 80487884, "forward": 7232842, "backward": 5695753}
 
 
 def log(msg):
@@ -67,6 +73,20 @@ def cpu_baseline(system, b, d, colouring, precision, omega, budget_s=25.0):
             "sample": f"1 full solve (K={info['iterations']}), {dt:.3f} s"}
 
 
+def copy_calibration(mib):
+    """A plain copy of known size: its counters read (bytes read, bytes written)."""
+    from simplefem_amd.hipmem import DeviceArray, synchronize
+
+    n = int(mib * 2 ** 20) // 8
+    src = DeviceArray.from_host(np.arange(n, dtype=np.float64))
+    dst = DeviceArray(n)
+    for _ in range(5):
+        dst.copy_from(src)
+    synchronize()
+    print(json.dumps({"copy_calibration_bytes": n * 8, "copies": 5}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,11 +104,16 @@ def main():
     ap.add_argument("--systems", type=int, default=256, help="batched: number of systems")
     ap.add_argument("--requests", type=int, default=48, help="served: number of requests")
     ap.add_argument("--rate", type=float, default=20.0, help="served: arrivals per second")
+    ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="MIB",
+                    help="only copy MIB MiB device to device 5 times (hipMemcpy), to scale "
+                         "the FETCH_SIZE / WRITE_SIZE counters of a kernel of known traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path "
               f"(--gpus {args.gpus} requested).", file=sys.stderr)
         return 2
+    if args.copy_calibration > 0:
+        return copy_calibration(args.copy_calibration)
     if args.config != "poisson3d":
         from simplefem_amd import forms_bench
         out = forms_bench.run(args)
@@ -151,8 +176,10 @@ def main():
         dom = max(prof, key=lambda k: prof[k]["ms"])
         p = prof[dom]
         achieved = p["bytes"] / (p["ms"] * 1e-3) / 1e9
+        default_workload = args.cube == 64 and args.omega == 1.0
+        traffic = MEASURED_TRAFFIC.get(dom) if default_workload else None
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                    "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                    "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
                     "kernel": dom, "launches": p["launches"],
                     "avg_launch_us": round(p["ms"] * 1e3 / max(p["launches"], 1), 3),
                     "bytes_per_launch": round(p["bytes"] / max(p["launches"], 1))}

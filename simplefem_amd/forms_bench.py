@@ -183,7 +183,11 @@ def batched(args):
 
 def served(args, budget=200e9):
     t0 = time.time()
-    jobs = W.served_requests(args.requests, args.rate, dofs=(1e3, 2e6))
+    def progress(k, job):
+        if k % 8 == 7:
+            log(f"generated {k + 1} requests ({time.time() - t0:.1f}s, last n={job.system.n})")
+
+    jobs = W.served_requests(args.requests, args.rate, dofs=(1e3, 2e6), progress=progress)
     log(f"{len(jobs)} requests generated ({time.time() - t0:.1f}s)")
     cols = {}
     for j in jobs:

@@ -88,6 +88,12 @@ class DeviceArray:
     def zero(self):
         check(hip().hipMemset(self.ptr, 0, self.nbytes))
 
+    def copy_from(self, other: "DeviceArray"):
+        """Device-to-device copy of other's first nbytes (hipMemcpy)."""
+        if other.nbytes < self.nbytes:
+            raise ValueError("source is smaller than the destination")
+        check(hip().hipMemcpy(self.ptr, other.ptr, self.nbytes, D2D))
+
     @property
     def address(self) -> int:
         return self.ptr.value

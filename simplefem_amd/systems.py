@@ -470,36 +470,48 @@ def lagrange_pr(dim: int, n: int, r: int) -> CSRSystem:
         simplices = [[(0, 0), (1, 0), (1, 1)], [(0, 0), (0, 1), (1, 1)]]
     else:
         simplices = [[((bb >> 2) & 1, (bb >> 1) & 1, bb & 1) for bb in (0,) + path] for path in _KUHN]
-    cells = np.stack(np.meshgrid(*[np.arange(n)] * dim, indexing="ij"), axis=-1).reshape(-1, dim)
     strides = np.array([m ** k for k in range(dim)])
-    rows, cols, vals = [], [], []
+    N = m ** dim
+    grid = np.stack(np.unravel_index(np.arange(N), (m,) * dim)[::-1], axis=1)
+    dirichlet = ((grid == 0) | (grid == m - 1)).any(axis=1)
+    elems = []
     for simp in simplices:
         alphas, K = _pr_element_matrix(simp, r)
-        V = np.array(simp)
-        local = np.array([r * V[0] + sum(a[i] * (V[i] - V[0]) for i in range(1, dim + 1))
-                          for a in alphas])  # node offsets in the fine lattice of one cell
-        lin_cells = (r * cells * strides).sum(axis=1)                # (cells,)
-        lin_local = (local * strides).sum(axis=1)                    # (nb,)
-        nodes = lin_cells[:, None] + lin_local[None, :]              # (cells, nb)
-        nb = len(alphas)
-        rows.append(np.repeat(nodes, nb, axis=1).ravel())
-        cols.append(np.tile(nodes, (1, nb)).ravel())
-        vals.append(np.tile(K.ravel(), nodes.shape[0]))
-    rows, cols, vals = np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
-    grid = np.stack(np.unravel_index(np.arange(m ** dim), (m,) * dim)[::-1], axis=1)
-    dirichlet = ((grid == 0) | (grid == m - 1)).any(axis=1)
-    vals = np.where(dirichlet[rows] | dirichlet[cols], 0.0, vals) * (1.0 / n) ** (dim - 2)
-    N = m ** dim
-    key = rows.astype(np.int64) * N + cols
-    order = np.argsort(key, kind="stable")
-    key = key[order]
-    first = np.concatenate([[True], key[1:] != key[:-1]])
-    idx = np.nonzero(first)[0]
-    V = np.add.reduceat(vals[order], idx)
-    uniq = key[idx]
+        V0 = np.array(simp)
+        local = np.array([r * V0[0] + sum(a[i] * (V0[i] - V0[0]) for i in range(1, dim + 1))
+                          for a in alphas])  # lattice coordinates of the nodes of one cell
+        elems.append((local, (local * strides).sum(axis=1), K))
+    # Every entry couples two nodes of one simplex, so col - row is one of a
+    # few lattice offsets: accumulate into an (offset, node) array.  For one
+    # simplex type and one local pair (a, b), node a of every cell is a
+    # strided sub-lattice and the offset is fixed, so each update is a
+    # strided slice; contributions to an entry add in (simplex type, a) order.
+    offsets = np.unique(np.concatenate([(ll[None, :] - ll[:, None]).ravel() for _, ll, _ in elems]))
+    u = len(offsets)
+    W = np.zeros((u,) + (m,) * dim)      # axes: offset, coordinate dim-1 .. 0
+    P = np.zeros((u,) + (m,) * dim, dtype=bool)
+    D = dirichlet.reshape((m,) * dim)
+    scale = (1.0 / n) ** (dim - 2)
+
+    def sub(c):  # node with cell-local lattice coordinates c, in every cell
+        return tuple(slice(c[i], c[i] + r * n, r) for i in reversed(range(dim)))
+
+    for local, ll, K in elems:
+        for a in range(len(ll)):
+            sa = sub(local[a])
+            da = D[sa]
+            ks = np.searchsorted(offsets, ll - ll[a])
+            for b in range(len(ll)):
+                W[(ks[b],) + sa] += np.where(da | D[sub(local[b])], 0.0, K[a, b]) * scale
+                P[(ks[b],) + sa] = True
+    P = np.ascontiguousarray(P.reshape(u, N).T)                       # (node, offset)
+    row, k = np.nonzero(P)                                            # row-major, offsets ascending
+    V = W.reshape(-1)[k * N + row]
+    del W
     A = np.zeros(N + 1, dtype=np.int64)
-    A[1:] = np.bincount(uniq // N, minlength=N)
+    A[1:] = P.sum(axis=1)
+    del P
     out = CSRSystem(f"lagrange{dim}d_n{n}_r{r}", N, np.cumsum(A).astype(np.int32),
-                    (uniq % N).astype(np.int32), V, dirichlet)
+                    (row + offsets[k]).astype(np.int32), V, dirichlet)
     _PR_CACHE[cache_key] = out
     return out

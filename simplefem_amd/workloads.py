@@ -40,8 +40,10 @@ def batched_systems(count=256, nrhs=4, seed=2026, levels=(4, 8), degrees=(1, 3))
     return jobs
 
 
-def served_requests(count=64, rate=50.0, seed=7, dofs=(1e3, 2e6), degrees=(1, 3)):
-    """count requests with exponential inter-arrival times (rate per second)."""
+def served_requests(count=64, rate=50.0, seed=7, dofs=(1e3, 2e6), degrees=(1, 3),
+                    progress=None):
+    """count requests with exponential inter-arrival times (rate per second).
+    progress(k, job), if given, is called after request k is generated."""
     rng = _rng(seed)
     t = 0.0
     jobs = []
@@ -53,4 +55,6 @@ def served_requests(count=64, rate=50.0, seed=7, dofs=(1e3, 2e6), degrees=(1, 3)
         n = max(2, int(round((target ** (1.0 / dim) - 1.0) / r)))
         s = S.lagrange_pr(dim, n, r)
         jobs.append(Job(s, [S.rhs_uniform(s, seed * 7919 + k)], arrival=t))
+        if progress is not None:
+            progress(k, jobs[-1])
     return jobs
