@@ -103,6 +103,8 @@ def main():
                          "run the mixed-precision, batched and served forms")
     ap.add_argument("--systems", type=int, default=256, help="batched: number of systems")
     ap.add_argument("--requests", type=int, default=48, help="served: number of requests")
+    ap.add_argument("--chunk-steps", type=int, default=2000,
+                    help="batched: engine steps per persistent launch between progress checks")
     ap.add_argument("--rate", type=float, default=20.0, help="served: arrivals per second")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="MIB",
                     help="only copy MIB MiB device to device 5 times (hipMemcpy), to scale "

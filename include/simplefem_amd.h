@@ -254,8 +254,9 @@ int sfa_system_destroy(sfa_system sys);
  * one system per call).  A batch holds up to max_slots (<= 1024) solves of
  * Rainbow CG-SSOR (the iteration of ConjugateGradientSolverCSR_Rainbow,
  * solver/sparsesolver.cpp:1002-1335) over any number of device-resident
- * matrices; one persistent kernel advances every running solve a phase per
- * step, so solves of any size share each launch.  Solves can be submitted and
+ * matrices; each step advances every running solve by one phase in one set of
+ * three kernel launches (a captured graph replays 32 steps at a time), so
+ * solves of any size share each launch.  Solves can be submitted and
  * collected between sfa_batch_run calls (continuous batching).
  *   SFA_BATCH_FP64:  FP64 throughout; stopping rule of the reference
  *                    (sqrt(z.r) < max(precision*||b||, 2^-26)).
@@ -296,8 +297,9 @@ int sfa_batch_submit(sfa_batch b, int matrix_id, const double* rhs,
                      const double* x0, double precision, double inner_rtol,
                      int max_outer, int* slot);
 
-/* Runs at most max_steps steps (0: until no solve is running); *running
- * receives the number of solves still running. */
+/* Runs max_steps steps rounded down to whole graph replays of 32 steps, at
+ * least one replay (0: until no solve is running); *running receives the
+ * number of solves still running. */
 int sfa_batch_run(sfa_batch b, int max_steps, int* running);
 
 /* Lists finished, not yet collected slots. */

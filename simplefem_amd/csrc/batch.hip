@@ -10,23 +10,23 @@
 // the same phases as the single-system kernels, with the same per-row
 // arithmetic.  A "step" executes the current phase of every active slot; a
 // phase is split into work units of 256 rows (four SELL-64 slices, one per
-// wavefront of a 256-thread block).  One persistent kernel runs the steps:
-// blocks claim units from the step's queue (agent-scope atomic), the last
-// unit of a slot's phase does the slot's scalar step (alpha / beta / exit
-// tests, deterministic partial sums in unit order) and advances its program,
-// and the last unit of the step builds the next step's queue and publishes
-// the step counter with release semantics; idle blocks wait for it with
-// acquire loads.  A block only waits while other running blocks finish
-// claimed units, so the grid makes progress whether or not all blocks are
-// resident.  The kernel returns when the step counter reaches the host's
-// limit, when no slot is active, or when a wait exceeds its spin bound (then
-// sched->error is set): every wave reaches one of these exits.
+// wavefront of a 256-thread block).  A step is three kernels:
 //
-// Between launches the host admits new slots and collects finished ones.  A
-// new slot joins the queue the next time one is built (at the end of the
-// current step); only when no slot is running does a one-block kernel build
-// the current step's queue, so no unit already executed is ever scheduled
-// again.
+//   k_batch_step    every unit of the step's schedule (grid-stride); each
+//                   unit stores its partial sum in its slot's partials[j];
+//   k_batch_fold    one block per slot: sums the slot's unit partials in unit
+//                   order, runs the slot's scalar step (alpha / beta / exit
+//                   tests) and advances its program, then counts the units
+//                   of its next phase;
+//   k_batch_prefix  one block: prefix sums of the unit counts -> the next
+//                   step's schedule, and the step counter.
+//
+// Kernel boundaries order the three, so no kernel waits on another block:
+// there are no atomics, fences or spin loops.  The host replays a captured
+// graph of a fixed number of steps; a step with no scheduled units is a
+// no-op, so replays past the end change nothing.  Between replays the host
+// admits new slots and collects finished ones: a new slot is counted by the
+// next k_batch_fold and joins the schedule that k_batch_prefix builds.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -50,7 +50,6 @@ using namespace dev;
 constexpr int kUnitBlock = 256;             // threads per block = rows per unit
 constexpr int kUnitSlices = kUnitBlock / 64;
 constexpr int kMaxSlots = 1024;
-constexpr unsigned kSpinLimit = 1u << 26;   // ~seconds of s_sleep; a bound, never reached
 
 enum Phase : int {
     PH_RESID = 0, PH_FWD_R, PH_BWD_R, PH_DINIT,
@@ -80,8 +79,7 @@ template <typename T>
 struct BSlot {
     int mat;
     int kind, group;
-    int units;
-    unsigned ticket;
+    int units;          // units of the phase in the step being scheduled
     int mixed;
     int K, maxK, exit_reason, outer, max_outer, inner_total;
     double zr, dAd, alpha, beta, tol2, rel2, target64, res64;
@@ -92,16 +90,12 @@ struct BSlot {
 };
 
 struct BSched {
-    unsigned step;
-    unsigned limit;
-    int stop;
-    int error;
+    unsigned step;      // steps completed
     int nslots;
+    int W[2];           // units of the step of parity p
+    int active;         // slots with units in the next step
     int pad;
-    int W[2];
-    unsigned claim[2];
-    unsigned done[2];
-    int start[2][kMaxSlots + 1];
+    int start[2][kMaxSlots + 1];  // unit offsets of the slots in the step of parity p
 };
 
 template <typename T>
@@ -381,155 +375,113 @@ __device__ inline double block_sum(double v, double* ws)
     return t;
 }
 
-// Builds the work queue of step parity p from the slots' program counters,
-// skipping phases that have no units and no decision (one block).
+// Units of the slot's current phase, skipping phases that have no units and
+// no decision to make.  One thread.
 template <typename T>
-__device__ void build_schedule(const BArgs<T>& a, int p)
+__device__ int next_units(BSlot<T>& S, const BMat<T>& M)
 {
-    __shared__ int counts[kUnitBlock];
-    BSched* sc = a.sched;
-    const int ns = sc->nslots;
-    const int per = (ns + kUnitBlock - 1) / kUnitBlock;
-    const int s0 = threadIdx.x * per;
-    int local = 0;
-    for (int s = s0; s < min(ns, s0 + per); s++) {
-        BSlot<T>& S = a.slots[s];
-        int units = 0;
-        if (S.kind != PH_DONE && S.mat >= 0) {
-            const BMat<T>& M = a.mats[S.mat];
-            for (int guard = 0; guard < 64; guard++) {
-                units = phase_units(M, S.kind, S.group);
-                if (units > 0 || S.kind == PH_DONE) break;
-                advance(S, M);  // a phase with no rows and nothing to decide
-            }
-        }
-        S.units = units;
-        S.ticket = 0;
-        local += units;
-        sc->start[p][s + 1] = local;  // slot-local prefix, fixed below
+    int units = 0;
+    for (int guard = 0; guard < 64; guard++) {
+        units = phase_units(M, S.kind, S.group);
+        if (units > 0 || S.kind == PH_DONE) break;
+        advance(S, M);  // a phase with no rows and nothing to decide
     }
-    counts[threadIdx.x] = local;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        int acc = 0;
-        for (int t = 0; t < kUnitBlock; t++) {
-            const int c = counts[t];
-            counts[t] = acc;
-            acc += c;
-        }
-        sc->W[p] = acc;
-        sc->claim[p] = 0;
-        sc->done[p] = 0;
-        sc->start[p][0] = 0;
-    }
-    __syncthreads();
-    const int base = counts[threadIdx.x];
-    for (int s = s0; s < min(ns, s0 + per); s++) sc->start[p][s + 1] += base;
-    // every thread's queue entries become visible with the step counter
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
+    return S.kind == PH_DONE ? 0 : units;
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kUnitBlock) k_batch_schedule(BArgs<T> a)
+__global__ void __launch_bounds__(kUnitBlock) k_batch_step(BArgs<T> a)
 {
-    const int p = (int)(a.sched->step & 1u);
-    build_schedule(a, p);
-}
-
-template <typename T>
-__global__ void __launch_bounds__(kUnitBlock) k_batch_run(BArgs<T> a)
-{
-    __shared__ int sh_cmd, sh_slot, sh_unit, sh_slot_last, sh_step_last;
-    __shared__ unsigned sh_step;
     __shared__ double ws[kUnitBlock / 64];
+    const BSched* sc = a.sched;
+    const int p = (int)(sc->step & 1u);
+    const int W = sc->W[p];
+    const int ns = sc->nslots;
+    const int* start = sc->start[p];
+    for (int u = blockIdx.x; u < W; u += gridDim.x) {
+        int lo = 0, hi = ns;  // the slot s with start[s] <= u < start[s+1]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (start[mid] <= u) lo = mid;
+            else hi = mid;
+        }
+        BSlot<T>& S = a.slots[lo];
+        const int j = u - start[lo];
+        const double part = block_sum(run_unit(S, a.mats[S.mat], S.kind, S.group, j), ws);
+        if (threadIdx.x == 0) S.partials[j] = part;
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kUnitBlock) k_batch_fold(BArgs<T> a)
+{
+    __shared__ double ws[kUnitBlock / 64];
+    const BSched* sc = a.sched;
+    const int s = blockIdx.x;
+    if (s >= sc->nslots) return;
+    BSlot<T>& S = a.slots[s];
+    if (S.mat < 0 || S.kind == PH_DONE) {
+        if (threadIdx.x == 0) S.units = 0;
+        return;
+    }
+    const BMat<T>& M = a.mats[S.mat];
+    const int p = (int)(sc->step & 1u);
+    const int ran = sc->start[p][s + 1] - sc->start[p][s];
+    if (ran > 0) {  // the slot's phase ran in this step
+        double acc = 0.;
+        for (int i = threadIdx.x; i < ran; i += kUnitBlock) acc += S.partials[i];
+        const double total = block_sum(acc, ws);
+        if (threadIdx.x == 0) phase_complete(S, M, total);
+    }
+    if (threadIdx.x == 0) S.units = next_units(S, M);
+}
+
+constexpr int kPrefixBlock = kMaxSlots;
+
+template <typename T>
+__global__ void __launch_bounds__(kPrefixBlock) k_batch_prefix(BArgs<T> a)
+{
+    __shared__ int wsum[kPrefixBlock / 64];
+    __shared__ int active;
     BSched* sc = a.sched;
-    for (;;) {
-        if (threadIdx.x == 0) {
-            int cmd = 0;  // 0 work, 1 retry, 2 exit
-            const unsigned step = __hip_atomic_load(&sc->step, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-            const int p = (int)(step & 1u);
-            if (step >= sc->limit || sc->stop || sc->error || sc->W[p] == 0) {
-                cmd = 2;
-            } else {
-                const unsigned u = __hip_atomic_fetch_add(&sc->claim[p], 1u, __ATOMIC_RELAXED,
-                                                          __HIP_MEMORY_SCOPE_AGENT);
-                if (u >= (unsigned)sc->W[p]) {
-                    unsigned spins = 0;
-                    while (__hip_atomic_load(&sc->step, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == step) {
-                        __builtin_amdgcn_s_sleep(2);
-                        if (++spins > kSpinLimit) {
-                            __hip_atomic_store(&sc->error, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                            break;
-                        }
-                    }
-                    cmd = 1;
-                } else {
-                    int lo = 0, hi = sc->nslots;  // largest s with start[s] <= u
-                    while (hi - lo > 1) {
-                        const int mid = (lo + hi) >> 1;
-                        if ((unsigned)sc->start[p][mid] <= u) lo = mid;
-                        else hi = mid;
-                    }
-                    sh_slot = lo;
-                    sh_unit = (int)(u - (unsigned)sc->start[p][lo]);
-                    sh_step = step;
-                }
-            }
-            sh_cmd = cmd;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int ns = sc->nslots;
+    const unsigned step = sc->step;
+    const int q = (int)((step + 1u) & 1u);
+    const int u = t < ns ? a.slots[t].units : 0;
+    if (t == 0) active = 0;
+    int v = u;  // inclusive scan: wavefront, then wavefront totals
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int w = __shfl_up(v, o, 64);
+        if (lane >= o) v += w;
+    }
+    if (lane == 63) wsum[wave] = v;
+    __syncthreads();
+    if (u > 0) atomicAdd(&active, 1);
+    if (t == 0) {
+        int acc = 0;
+        for (int w = 0; w < kPrefixBlock / 64; w++) {
+            const int x = wsum[w];
+            wsum[w] = acc;
+            acc += x;
         }
-        __syncthreads();
-        const int cmd = sh_cmd;
-        if (cmd == 2) return;
-        if (cmd == 1) {
-            __syncthreads();
-            continue;
-        }
-        const int s = sh_slot, j = sh_unit;
-        const unsigned step = sh_step;
-        const int p = (int)(step & 1u);
-        BSlot<T>& S = a.slots[s];
-        const BMat<T>& M = a.mats[S.mat];
-        const int kind = S.kind, g = S.group;
-        const double part = block_sum(run_unit(S, M, kind, g, j), ws);
-        // slot phase bookkeeping
-        if (threadIdx.x == 0) {
-            S.partials[j] = part;
-            const unsigned t = __hip_atomic_fetch_add(&S.ticket, 1u, __ATOMIC_ACQ_REL,
-                                                      __HIP_MEMORY_SCOPE_AGENT);
-            sh_slot_last = (t + 1 == (unsigned)S.units);
-        }
-        __syncthreads();
-        if (sh_slot_last) {
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            double acc = 0.;
-            for (int i = threadIdx.x; i < S.units; i += kUnitBlock) acc += S.partials[i];
-            const double total = block_sum(acc, ws);
-            if (threadIdx.x == 0) phase_complete(S, M, total);
-        }
-        // step bookkeeping
-        if (threadIdx.x == 0) {
-            const unsigned d = __hip_atomic_fetch_add(&sc->done[p], 1u, __ATOMIC_ACQ_REL,
-                                                      __HIP_MEMORY_SCOPE_AGENT);
-            sh_step_last = (d + 1 == (unsigned)sc->W[p]);
-        }
-        __syncthreads();
-        if (sh_step_last) {
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            build_schedule(a, p ^ 1);
-            // An increment, not a store of step + 1: a block that read the step
-            // counter long before its claim may be finishing a later step of the
-            // same parity, whose number it does not know.
-            if (threadIdx.x == 0)
-                __hip_atomic_fetch_add(&sc->step, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        }
-        __syncthreads();
+    }
+    __syncthreads();
+    v += wsum[wave];
+    if (t < ns) sc->start[q][t + 1] = v;
+    if (t == ns - 1) sc->W[q] = v;
+    if (t == 0) {
+        sc->start[q][0] = 0;
+        sc->active = active;
+        sc->step = step + 1u;
     }
 }
 
 // ====================================================================== host
 
-constexpr int kRunBlocks = 1024;  // persistent grid: 4 blocks of 256 threads per CU
+constexpr int kStepBlocks = 2048;  // grid of k_batch_step: 8 blocks of 256 threads per CU
+constexpr int kGraphSteps = 32;    // steps per replay of the captured graph
 
 template <typename T>
 struct HostMat {
@@ -599,12 +551,16 @@ struct Engine {
     BSched* d_sched = nullptr;
     BSched* h_sched = nullptr;                // pinned
     unsigned step = 0;
+    hipGraphExec_t graph = nullptr;
+    BMat<T>* graph_mats = nullptr;            // d_mats the graph was captured with
 
     ~Engine() { release(); }
 
     void release()
     {
         if (stream) (void)hipStreamSynchronize(stream);
+        if (graph) (void)hipGraphExecDestroy(graph);
+        graph = nullptr;
         for (auto& v : vecs) v.release();
         for (auto& m : mats) m.release();
         dev_free(d_mats);
@@ -833,25 +789,49 @@ struct Engine {
         step = h_sched->step;
     }
 
-    // Runs the persistent kernel for at most max_steps steps (0: until idle).
-    // Returns the number of slots still running.
+    // Captures kGraphSteps steps (k_batch_step, k_batch_fold, k_batch_prefix)
+    // into a graph; rebuilt when the device matrix table has moved.
+    void build_graph()
+    {
+        if (graph && graph_mats == d_mats) return;
+        if (graph) SFA_HIP(hipGraphExecDestroy(graph));
+        graph = nullptr;
+        hipGraph_t g = nullptr;
+        const BArgs<T> a = args();
+        SFA_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        for (int i = 0; i < kGraphSteps; i++) {
+            k_batch_step<T><<<kStepBlocks, kUnitBlock, 0, stream>>>(a);
+            k_batch_fold<T><<<max_slots, kUnitBlock, 0, stream>>>(a);
+            k_batch_prefix<T><<<1, kPrefixBlock, 0, stream>>>(a);
+        }
+        SFA_HIP(hipStreamEndCapture(stream, &g));
+        hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            graph = nullptr;
+            throw Error{SFA_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)};
+        }
+        graph_mats = d_mats;
+    }
+
+    // Runs about max_steps steps (whole graph replays, at least one; 0: until
+    // no solve is running).  Returns the number of solves still running.
     int run(int max_steps)
     {
         SFA_HIP(hipSetDevice(device));
-        read_sched();
-        if (h_sched->error) throw Error{SFA_ERR_HIP, "batch engine stopped on a wait bound"};
-        const int p = (int)(step & 1u);
-        if (h_sched->W[p] == 0) {  // idle: nothing has run in this step, build its queue here
-            k_batch_schedule<T><<<1, kUnitBlock, 0, stream>>>(args());
-            SFA_HIP(hipGetLastError());
+        if (!d_mats) return running();
+        build_graph();
+        if (max_steps > 0) {
+            const int reps = std::max(1, max_steps / kGraphSteps);
+            for (int i = 0; i < reps; i++) SFA_HIP(hipGraphLaunch(graph, stream));
+        } else {
+            // the first replay also schedules slots admitted since the last one
+            do {
+                SFA_HIP(hipGraphLaunch(graph, stream));
+                read_sched();
+            } while (h_sched->active > 0);
         }
-        h_sched->limit = max_steps > 0 ? step + (unsigned)max_steps : 0xFFFFFFF0u;
-        SFA_HIP(hipMemcpyAsync(&d_sched->limit, &h_sched->limit, sizeof(unsigned),
-                               hipMemcpyHostToDevice, stream));
-        k_batch_run<T><<<kRunBlocks, kUnitBlock, 0, stream>>>(args());
-        SFA_HIP(hipGetLastError());
         read_sched();
-        if (h_sched->error) throw Error{SFA_ERR_HIP, "batch engine stopped on a wait bound"};
         return running();
     }
 

@@ -122,8 +122,16 @@ def batched(args):
                 for b in j.rhs:
                     slots.append((eng.submit(mid, b, precision=1e-10, inner_rtol=1e-4), j, b))
             t_sub = time.perf_counter() - t1
-            while eng.run(0) > 0:
-                pass
+            last = t1
+            while True:
+                left = eng.run(args.chunk_steps)
+                if left == 0:
+                    break
+                now = time.perf_counter()
+                if now - last > 5.0:
+                    log(f"{'mixed' if mixed else 'fp64'} engine: {left} of {len(slots)} solves "
+                        f"running after {now - t1:.1f}s")
+                    last = now
             synchronize()
             elapsed = time.perf_counter() - t1
             worst = 0.0
@@ -199,6 +207,7 @@ def served(args, budget=200e9):
         pending = list(jobs)
         inflight = {}
         start = time.perf_counter()
+        last_log = 0.0
         done = 0
         while pending or inflight:
             now = time.perf_counter() - start
@@ -216,6 +225,9 @@ def served(args, budget=200e9):
                 pending.pop(0)
             if inflight:
                 eng.run(400)
+                if time.perf_counter() - start - last_log > 5.0:
+                    last_log = time.perf_counter() - start
+                    log(f"served {done} of {len(jobs)} after {last_log:.1f}s, {len(inflight)} in flight")
             elif pending:
                 time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - start)))
             for slot in eng.finished():
