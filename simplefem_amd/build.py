@@ -52,8 +52,8 @@ def _compile(src, hip):
     path = os.path.join(CSRC, src)
     if hip:
         cmd = [HIPCC, "-x", "hip"] + COMMON + DEVICE + ["-c", path, "-o", out]
-    else:
-        cmd = ["g++"] + COMMON + ["-c", path, "-o", out]
+    else:  # host-only sources; the plan builder runs its passes with OpenMP
+        cmd = ["g++"] + COMMON + ["-fopenmp", "-c", path, "-o", out]
     _run(cmd)
     return out
 
@@ -65,7 +65,7 @@ def build(verbose: bool = False) -> str:
         objs = list(pool.map(lambda s: _compile(*s), SOURCES))
     tmp = LIB + ".tmp"
     _run([HIPCC, "-shared", "--offload-arch=" + ARCH, "-o", tmp] + objs
-         + ["-Wl,-soname,libsimplefem_amd.so"])
+         + ["-Wl,-soname,libsimplefem_amd.so", "-lgomp"])
     os.replace(tmp, LIB)
     cxx_objs = []
     for src in CXX_SOURCES:

@@ -64,7 +64,9 @@ int rainbow_colour(int n, const int* A, const int* C, int* F, int* B, int* R)
 namespace {
 
 // Appends the slices of one group of rows (device order) to a part.  keep(k)
-// decides whether CSR entry k of the row goes into this part.
+// decides whether CSR entry k of the row goes into this part.  Two passes
+// over the group's slices, each parallel over slices: the widths, then (after
+// their prefix sum fixes every slice's offset) the entries.
 template <class Keep>
 void append_group(SellPart& part, const std::vector<int>& rows_dev, const std::vector<int>& perm,
                   const std::vector<int>& iperm, int n, const int* A, const int* C,
@@ -74,41 +76,53 @@ void append_group(SellPart& part, const std::vector<int>& rows_dev, const std::v
     bool consecutive = true;
     for (int i = 1; i < count && consecutive; i++) consecutive = rows_dev[i] == rows_dev[0] + i;
     part.pending_row0 = (count > 0 && consecutive) ? rows_dev[0] : (count == 0 ? 0 : -1);
-    for (int s0 = 0; s0 < count; s0 += kSliceRows) {
-        const int s1 = std::min(count, s0 + kSliceRows);
-        int width = 0;
-        for (int i = s0; i < s1; i++) {
+    const int nsl = (count + kSliceRows - 1) / kSliceRows;
+    if (nsl == 0) return;
+    std::vector<int> width(nsl, 0);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int sl = 0; sl < nsl; sl++) {
+        const int s1 = std::min(count, (sl + 1) * kSliceRows);
+        int wmax = 0;
+        for (int i = sl * kSliceRows; i < s1; i++) {
             const int orow = perm[rows_dev[i]];
             int w = 0;
             for (int k = A[orow]; k < A[orow + 1]; k++) w += keep(orow, k) ? 1 : 0;
-            width = std::max(width, w);
+            wmax = std::max(wmax, w);
         }
-        const int base = part.slice_off.back();
-        part.cols.resize((size_t)base + (size_t)width * kSliceRows, n);
-        part.vals.resize((size_t)base + (size_t)width * kSliceRows, 0.0);
-        for (int lane = 0; lane < kSliceRows; lane++) {
-            const int i = s0 + lane;
-            if (i >= s1) {
-                part.slice_rows.push_back(-1);
-                continue;
-            }
+        width[sl] = wmax;
+    }
+    const int first = part.nslices;
+    for (int sl = 0; sl < nsl; sl++)
+        part.slice_off.push_back(part.slice_off.back() + width[sl] * kSliceRows);
+    const size_t total = (size_t)part.slice_off.back();
+    part.cols.resize(total, n);
+    part.vals.resize(total, 0.0);
+    const size_t row_base = part.slice_rows.size();
+    part.slice_rows.resize(row_base + (size_t)nsl * kSliceRows, -1);
+    long long entries = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : entries)
+    for (int sl = 0; sl < nsl; sl++) {
+        const size_t base = (size_t)part.slice_off[first + sl];
+        const int s0 = sl * kSliceRows, s1 = std::min(count, s0 + kSliceRows);
+        for (int i = s0; i < s1; i++) {
+            const int lane = i - s0;
             const int p = rows_dev[i];
             const int orow = perm[p];
-            part.slice_rows.push_back(p);
+            part.slice_rows[row_base + (size_t)sl * kSliceRows + lane] = p;
             int w = 0;
             for (int k = A[orow]; k < A[orow + 1]; k++) {
                 if (!keep(orow, k)) continue;
-                const size_t at = (size_t)base + (size_t)w * kSliceRows + lane;
+                const size_t at = base + (size_t)w * kSliceRows + lane;
                 part.cols[at] = iperm[C[k]];
                 part.vals[at] = V[k];
                 w++;
             }
-            part.entries += w;
-            part.rows += 1;
+            entries += w;
         }
-        part.slice_off.push_back(base + width * kSliceRows);
-        part.nslices++;
     }
+    part.entries += entries;
+    part.rows += count;
+    part.nslices += nsl;
 }
 
 void start_part(SellPart& part)
