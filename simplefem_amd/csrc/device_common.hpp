@@ -116,28 +116,32 @@ __device__ inline T sell_row(const SellViewT<T>& A, int slice, int lane, const T
 
 // The first U entries of a lane's SELL row, loaded ahead of whatever the
 // kernel reads next so that the loads overlap.  All lanes of a slice hold
-// the same number of entries (the slice is padded to its longest row), so
-// the guards below never split a wavefront.
-template <typename T, int U = 8>
+// the same number of entries w (the slice is padded to its longest row), and
+// w comes from the slice offsets, which are wave-uniform, so the guards below
+// are scalar branches that never split a wavefront.
+template <typename T, int U = 16>
 struct SellChunk {
-    int e = 0, end = 0;
-    int c[U];
+    int e = 0;  // the lane's first entry
+    int w = 0;  // entries per lane
+    // unsigned: widening an index for the address is then free and does not
+    // make the wavefront wait for the column load
+    unsigned c[U];
     T v[U];
 };
 
 template <typename T, int U>
 __device__ inline void sell_prefetch(const SellViewT<T>& A, int slice, int lane, SellChunk<T, U>& R)
 {
-    R.e = A.off[slice] + lane;
-    R.end = A.off[slice + 1];
+    const int o0 = A.off[slice];
+    R.w = (A.off[slice + 1] - o0) / kWave;
+    R.e = o0 + lane;
 #pragma unroll
     for (int u = 0; u < U; u++) {
-        const int e = R.e + u * kWave;
-        if (e < R.end) {
-            R.c[u] = __builtin_nontemporal_load(&A.cols[e]);
-            R.v[u] = __builtin_nontemporal_load(&A.vals[e]);
+        if (u < R.w) {
+            R.c[u] = (unsigned)__builtin_nontemporal_load(&A.cols[R.e + u * kWave]);
+            R.v[u] = __builtin_nontemporal_load(&A.vals[R.e + u * kWave]);
         } else {
-            R.c[u] = 0;
+            R.c[u] = 0u;
             R.v[u] = T(0);
         }
     }
@@ -150,21 +154,51 @@ __device__ inline T sell_finish(const SellViewT<T>& A, const SellChunk<T, U>& R,
 {
     T g[U];
 #pragma unroll
-    for (int u = 0; u < U; u++) g[u] = R.e + u * kWave < R.end ? get(R.c[u]) : T(0);
+    for (int u = 0; u < U; u++) g[u] = u < R.w ? get(R.c[u]) : T(0);
 #pragma unroll
     for (int u = 0; u < U; u++) {
-        if (R.e + u * kWave < R.end) {
+        if (u < R.w) {
             if (SUB) acc -= R.v[u] * g[u];
             else     acc += R.v[u] * g[u];
         }
     }
-    for (int e = R.e + U * kWave; e < R.end; e += kWave) {
-        const int c = __builtin_nontemporal_load(&A.cols[e]);
-        const T v = __builtin_nontemporal_load(&A.vals[e]);
-        if (SUB) acc -= v * get(c);
-        else     acc += v * get(c);
+    // longer rows: further chunks of 8, loads issued before the arithmetic
+    constexpr int V = 8;
+    for (int k0 = U; k0 < R.w; k0 += V) {
+        unsigned c[V];
+        T v[V], h[V];
+#pragma unroll
+        for (int u = 0; u < V; u++) {
+            if (k0 + u < R.w) {
+                c[u] = (unsigned)__builtin_nontemporal_load(&A.cols[R.e + (k0 + u) * kWave]);
+                v[u] = __builtin_nontemporal_load(&A.vals[R.e + (k0 + u) * kWave]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < V; u++) h[u] = k0 + u < R.w ? get(c[u]) : T(0);
+#pragma unroll
+        for (int u = 0; u < V; u++) {
+            if (k0 + u < R.w) {
+                if (SUB) acc -= v[u] * h[u];
+                else     acc += v[u] * h[u];
+            }
+        }
     }
     return acc;
+}
+
+// Workgroups are dispatched to the GPU's 8 XCDs round robin (workgroup i on
+// XCD i % 8), and each XCD has its own L2.  xcd_block maps workgroup i of n
+// to a logical block so that every XCD gets a contiguous range of logical
+// blocks: rows that a wavefront gathers are then mostly rows of its own
+// XCD's range, already in that L2.  A bijection of [0, n), so only which
+// hardware runs a block changes, never a result.
+constexpr int kXcds = 8;
+__device__ inline int xcd_block(int i, int n)
+{
+    const int per = n / kXcds, rem = n % kXcds;
+    const int x = i % kXcds, k = i / kXcds;
+    return x * per + (x < rem ? x : rem) + k;
 }
 
 // Device row of a lane: computed when the group's rows are consecutive

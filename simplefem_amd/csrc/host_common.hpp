@@ -66,10 +66,12 @@ struct DevPartT {
     long long stored = 0;
     long long entries = 0;
     long long nrows = 0;
+    int zero_slot = 0;
 
     void upload(const SellPart& p)
     {
         nslices = p.nslices;
+        zero_slot = p.zero_slot;
         group_slice = p.group_slice;
         group_entries = p.group_entries;
         group_rows = p.group_rows;
@@ -99,7 +101,7 @@ struct DevPartT {
         off = rows = cols = nullptr;
         vals = nullptr;
     }
-    SellViewT<T> view() const { return SellViewT<T>{off, rows, cols, vals}; }
+    SellViewT<T> view() const { return SellViewT<T>{off, rows, cols, vals, zero_slot}; }
     int ngroups() const { return (int)group_slice.size() - 1; }
     int group_begin(int g) const { return group_slice[g]; }
     int group_count(int g) const { return group_slice[g + 1] - group_slice[g]; }

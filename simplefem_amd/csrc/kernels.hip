@@ -297,33 +297,35 @@ __global__ void __launch_bounds__(kVecBlock) k_eisenstat_init(int n, const T* r,
 
 // Forward sweep of one colour / level, solver/sparsesolver.cpp:528-541 (restart)
 // and :629-665, :1234-1261 (iteration: d.Ad step in the first group, then the
-// x and r updates of the group's rows).
+// x and r updates of the group's rows).  With zb set, the group is also the
+// first group of the backward sweep and has no upper entries (the last colour
+// of a Rainbow schedule): its backward rows z = (m*diag*c2)*w/diag (:667-692)
+// are formed here, with their z.r partials at zpart[block] when zpart is set.
 template <typename T>
 __global__ void __launch_bounds__(kSellBlock) k_forward(SellViewT<T> L, int slice0, int nslices,
                                                         int row0, int nrows, SolveState* st,
                                                         int mode, int p, T* x, T* r,
                                                         const T* d, const T* q, T* m,
                                                         const T* diag, T omega,
-                                                        const double* spmv_partials, int nspmv)
+                                                        const double* spmv_partials, int nspmv,
+                                                        T* zb, T c2, double* zpart)
 {
     const int lane = threadIdx.x;
-    const int slice = slice0 + blockIdx.x;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int slice = slice0 + b;
+    const bool has = b < nslices;
+    const int row = has ? group_row(L, slice, b, lane, row0, nrows) : -1;
+    const int ri = row >= 0 ? row : L.zero;  // padding lanes read the zero slot
+    T rr = r[ri];
+    const T dg = diag[ri];
+    T xx = T(0), dd = T(0), qq = T(0);
+    if (mode != FWD_RESTART) {
+        xx = x[ri];
+        dd = d[ri];
+        qq = q[ri];
+    }
     SellChunk<T> R;
-    int row = -1;
-    if ((int)blockIdx.x < nslices) {
-        sell_prefetch(L, slice, lane, R);
-        row = group_row(L, slice, blockIdx.x, lane, row0, nrows);
-    }
-    T rr = T(0), dg = T(1), xx = T(0), dd = T(0), qq = T(0);
-    if (row >= 0) {
-        rr = r[row];
-        dg = diag[row];
-        if (mode != FWD_RESTART) {
-            xx = x[row];
-            dd = d[row];
-            qq = q[row];
-        }
-    }
+    if (has) sell_prefetch(L, slice, lane, R);
     if (st->done) return;
     T alpha = T(0);
     if (mode == FWD_FIRST) {
@@ -334,21 +336,33 @@ __global__ void __launch_bounds__(kSellBlock) k_forward(SellViewT<T> L, int slic
     } else if (mode == FWD_UPDATE) {
         alpha = (T)st->alpha;
     }
-    if (row < 0) return;
-    if (mode != FWD_RESTART) {
-        x[row] = xx + alpha * dd;
-        rr = rr - alpha * qq;
-        r[row] = rr;
+    double part = 0.;
+    if (row >= 0) {
+        if (mode != FWD_RESTART) {
+            x[row] = xx + alpha * dd;
+            rr = rr - alpha * qq;
+            r[row] = rr;
+        }
+        const T acc = sell_finish<true>(L, R, [&](unsigned c) { return m[c]; }, rr);
+        const T mm = acc * omega / dg;
+        m[row] = mm;
+        if (zb) {
+            T t = mm;
+            t *= dg;
+            t *= c2;
+            const T zz = t * omega / dg;
+            zb[row] = zz;
+            part = (double)(zz * rr);
+        }
     }
-    const T acc = sell_finish<true>(L, R, [&](int c) { return m[c]; }, rr);
-    m[row] = acc * omega / dg;
+    if (zpart && has) block_partial<kSellBlock>(part, &zpart[b]);
 }
 
 // Backward sweep of one colour / level, solver/sparsesolver.cpp:543-568
 // (restart) and :667-692, :1263-1303 (iteration): the scaling of m fused in.
-// With reduce, the z.r partial of each block goes to gpart[slice], and block
-// 0 sums the previous group's partials into *prev_sum, so z.r is summed per
-// group and over the groups in launch order.
+// With reduce, the z.r partial of each block goes to gpart[slice], and one
+// extra block (logical block nslices) sums the previous group's partials into
+// *prev_sum, so z.r is summed per group and over the groups in launch order.
 template <typename T>
 __global__ void __launch_bounds__(kSellBlock) k_backward(SellViewT<T> U, int slice0, int nslices,
                                                          int row0, int nrows, SolveState* st,
@@ -358,31 +372,28 @@ __global__ void __launch_bounds__(kSellBlock) k_backward(SellViewT<T> U, int sli
                                                          double* prev_sum)
 {
     const int lane = threadIdx.x;
-    const int slice = slice0 + blockIdx.x;
-    const bool has = (int)blockIdx.x < nslices;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int slice = slice0 + b;
+    const bool has = b < nslices;
+    const int row = has ? group_row(U, slice, b, lane, row0, nrows) : -1;
+    const int ri = row >= 0 ? row : U.zero;  // padding lanes read the zero slot
+    const T dg = diag[ri];
+    const T mm = m[ri];
+    const T rr = reduce ? r[ri] : T(0);
     SellChunk<T> R;
-    int row = -1;
-    if (has) {
-        sell_prefetch(U, slice, lane, R);
-        row = group_row(U, slice, blockIdx.x, lane, row0, nrows);
-    }
-    T dg = T(1), mm = T(0), rr = T(0);
-    if (row >= 0) {
-        dg = diag[row];
-        mm = m[row];
-        if (reduce) rr = r[row];
-    }
+    if (has) sell_prefetch(U, slice, lane, R);
     if (st->done) return;
-    if (reduce && prev_sum && blockIdx.x == 0) {
+    if (reduce && prev_sum && b == nslices) {
         const double t = wave_fold(gpart + prev0, prevn);
         if (lane == 0) *prev_sum = t;
+        return;
     }
     double part = 0.;
     if (row >= 0) {
         T t = mm;
         t *= dg;
         t *= c2;
-        const T acc = sell_finish<true>(U, R, [&](int c) { return z[c]; }, t);
+        const T acc = sell_finish<true>(U, R, [&](unsigned c) { return z[c]; }, t);
         const T zz = acc * omega / dg;
         z[row] = zz;
         part = (double)(zz * rr);
@@ -400,22 +411,21 @@ __global__ void __launch_bounds__(kSellBlock) k_eisenstat_solve(SellViewT<T> P, 
                                                                 T* out, const T* diag,
                                                                 T omega, T c3)
 {
-    const int slice = slice0 + blockIdx.x;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int slice = slice0 + b;
     const int lane = threadIdx.x;
-    if ((int)blockIdx.x >= nslices) return;
+    if (b >= nslices) return;
+    const int row = group_row(P, slice, b, lane, row0, nrows);
+    const int ri = row >= 0 ? row : P.zero;  // padding lanes read the zero slot
+    const T dg = diag[ri];
+    T a = in[ri];
+    const T tt = with_t ? t[ri] : T(0);
     SellChunk<T> R;
     sell_prefetch(P, slice, lane, R);
-    const int row = group_row(P, slice, blockIdx.x, lane, row0, nrows);
-    T dg = T(1), a = T(0), tt = T(0);
-    if (row >= 0) {
-        dg = diag[row];
-        a = in[row];
-        if (with_t) tt = t[row];
-    }
     if (st->done) return;
     if (row < 0) return;
     if (with_t) a = a - c3 * dg * tt;
-    const T acc = sell_finish<true>(P, R, [&](int c) { return out[c]; }, a);
+    const T acc = sell_finish<true>(P, R, [&](unsigned c) { return out[c]; }, a);
     out[row] = acc * omega / dg;
 }
 
@@ -432,19 +442,16 @@ __global__ void __launch_bounds__(kSellBlock) k_eisenstat_upper(SellViewT<T> P, 
                                                                 int nfold)
 {
     const int lane = threadIdx.x;
-    const int slice = slice0 + blockIdx.x;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    const int slice = slice0 + b;
+    const bool has = b < nslices;
+    const int row = has ? group_row(P, slice, b, lane, row0, nrows) : -1;
+    const int ri = row >= 0 ? row : P.zero;  // padding lanes read the zero slot
+    const T dg = diag[ri];
+    const T zz = z[ri];
+    const T dd = dold[ri];
     SellChunk<T> R;
-    int row = -1;
-    if ((int)blockIdx.x < nslices) {
-        sell_prefetch(P, slice, lane, R);
-        row = group_row(P, slice, blockIdx.x, lane, row0, nrows);
-    }
-    T dg = T(1), zz = T(0), dd = T(0);
-    if (row >= 0) {
-        dg = diag[row];
-        zz = z[row];
-        dd = dold[row];
-    }
+    if (has) sell_prefetch(P, slice, lane, R);
     if (st->done) return;
     const int fold = st->fold[p];
     T beta;
@@ -459,7 +466,7 @@ __global__ void __launch_bounds__(kSellBlock) k_eisenstat_upper(SellViewT<T> P, 
     if (row < 0) return;
     const T dn = fold ? zz + beta * dd : dd;
     dnew[row] = dn;
-    const T acc = sell_finish<true>(P, R, [&](int c) { return out[c]; }, dn);
+    const T acc = sell_finish<true>(P, R, [&](unsigned c) { return out[c]; }, dn);
     out[row] = acc * omega / dg;
 }
 
@@ -468,7 +475,8 @@ __global__ void __launch_bounds__(kSellBlock) k_eisenstat_upper(SellViewT<T> P, 
 // Opens the iteration (z.r of the previous sweep, beta, K, exit tests), then
 // dn = src + beta*dold (solver/sparsesolver.cpp:698-706 and its siblings),
 // q = A dn and the d.Ad partials (:611-626, 1197-1212).  Each wavefront owns
-// slices blockIdx.x*4 + wave + k*4*gridDim.x; neighbours' dn are formed from
+// slices b*4 + wave + k*4*gridDim.x (b: the block's logical index, see
+// xcd_block); neighbours' dn are formed from
 // src and dold where they are gathered, with the same arithmetic as the
 // owner's, so every copy has the same bits.
 template <typename T>
@@ -480,11 +488,23 @@ __global__ void __launch_bounds__(kSpmvBlock) k_spmv_dir(SellViewT<T> A, int nsl
                                                          const double* fpart, int nfold,
                                                          const double* gsum, int ngroups)
 {
-    const int lane = threadIdx.x % kWave, wave = threadIdx.x / kWave;
-    const int first_slice = blockIdx.x * kSpmvWaves + wave;
+    // the wavefront index is uniform: readfirstlane lets the slice offsets be
+    // scalar loads
+    const int lane = threadIdx.x % kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int first_slice = xcd_block(blockIdx.x, gridDim.x) * kSpmvWaves + wave;
     const int stride = gridDim.x * kSpmvWaves;
-    SellChunk<T> R;
-    if (first_slice < nslices) sell_prefetch(A, first_slice, lane, R);
+    // the first slice's own-row values and entries, loaded ahead of the state
+    int row = -1;
+    T s_own = T(0), d_own = T(0);
+    SellChunk<T, 8> R;  // 8: keeps every block of the grid resident
+    if (first_slice < nslices) {
+        row = group_row(A, first_slice, first_slice, lane, row0, nrows);
+        const int ri = row >= 0 ? row : A.zero;
+        s_own = src[ri];
+        d_own = dold[ri];
+        sell_prefetch(A, first_slice, lane, R);
+    }
     const int fold = st->fold[p];
     const int K = st->Kp[p];
     const double zr_old = st->zrp[p];
@@ -507,14 +527,21 @@ __global__ void __launch_bounds__(kSpmvBlock) k_spmv_dir(SellViewT<T> A, int nsl
     if (open_step(st, p, fold, zr_old, K, zr_new, &beta_d)) return;
     const T beta = (T)beta_d;
     // rows past n_active (inactive rows, the zero slot) keep their d
-    auto dn = [&](int c) -> T { return fold && c < n_active ? src[c] + beta * dold[c] : dold[c]; };
+    auto dn = [&](unsigned c) -> T {
+        return fold && c < (unsigned)n_active ? src[c] + beta * dold[c] : dold[c];
+    };
     double part = 0.;
     for (int slice = first_slice; slice < nslices; slice += stride) {
-        if (slice != first_slice) sell_prefetch(A, slice, lane, R);
-        const int row = group_row(A, slice, slice, lane, row0, nrows);
+        if (slice != first_slice) {
+            row = group_row(A, slice, slice, lane, row0, nrows);
+            const int ri = row >= 0 ? row : A.zero;
+            s_own = src[ri];
+            d_own = dold[ri];
+            sell_prefetch(A, slice, lane, R);
+        }
         const T acc = sell_finish<false>(A, R, dn, T(0));
         if (row >= 0) {
-            const T dr = dn(row);
+            const T dr = fold && row < n_active ? s_own + beta * d_own : d_own;  // = dn(row)
             dnew[row] = dr;
             q[row] = acc;
             part += (double)(dr * acc);
@@ -694,14 +721,16 @@ void launch_eisenstat_init(hipStream_t s, SolveState* st, int n_active, const T*
 template <typename T>
 void launch_forward(hipStream_t s, SellViewT<T> L, int slice0, int nslices, int row0, int nrows,
                     SolveState* st, int mode, int p, T* x, T* r, const T* d, const T* q, T* m,
-                    const T* diag, T omega, const double* spmv_partials, int nspmv)
+                    const T* diag, T omega, const double* spmv_partials, int nspmv, T* zb, T c2,
+                    double* zpart)
 {
     // The first group of an iteration runs one block even when empty: it
     // forms alpha for the groups after it.
     const int grid = nslices > 0 ? nslices : (mode == FWD_FIRST ? 1 : 0);
     if (grid == 0) return;
     k_forward<T><<<grid, kSellBlock, 0, s>>>(L, slice0, nslices, row0, nrows, st, mode, p, x, r, d,
-                                             q, m, diag, omega, spmv_partials, nspmv);
+                                             q, m, diag, omega, spmv_partials, nspmv, zb, c2,
+                                             zpart);
 }
 
 template <typename T>
@@ -710,9 +739,8 @@ void launch_backward(hipStream_t s, SellViewT<T> U, int slice0, int nslices, int
                      T omega, T c2, double* group_partials, int prev0, int prevn,
                      double* prev_sum)
 {
-    // An empty group still runs one block when it has to sum the previous
-    // group's partials.
-    const int grid = nslices > 0 ? nslices : (reduce && prev_sum ? 1 : 0);
+    // one more block sums the previous group's partials
+    const int grid = nslices + (reduce && prev_sum ? 1 : 0);
     if (grid == 0) return;
     k_backward<T><<<grid, kSellBlock, 0, s>>>(U, slice0, nslices, row0, nrows, st, reduce, r, m, z,
                                               diag, omega, c2, group_partials, prev0, prevn,
@@ -801,7 +829,7 @@ void launch_add_correction(hipStream_t s, double* x, const float* e, int n)
                                            T, T*, T*, double*);                                \
     template void launch_forward<T>(hipStream_t, SellViewT<T>, int, int, int, int, SolveState*, \
                                     int, int, T*, T*, const T*, const T*, T*, const T*, T,     \
-                                    const double*, int);                                       \
+                                    const double*, int, T*, T, double*);                       \
     template void launch_backward<T>(hipStream_t, SellViewT<T>, int, int, int, int,            \
                                      SolveState*, int, const T*, const T*, T*, const T*, T, T, \
                                      double*, int, int, double*);                              \

@@ -28,6 +28,7 @@ struct SellViewT {
     const int* rows;     // nslices*64, -1 = padding lane
     const int* cols;
     const T* vals;
+    int zero;            // the vectors' zero slot (n): padding lanes read there
 };
 using SellView = SellViewT<double>;
 
@@ -110,11 +111,15 @@ void launch_eisenstat_init(hipStream_t s, SolveState* st, int n_active, const T*
 // mode FWD_FIRST:   as FWD_UPDATE, but first forms d.Ad from the SpMV's
 //                   partials (spmv_partials, nspmv), runs the breakdown tests
 //                   and computes a (first group of the iteration, parity p).
+// zb != null: the group is also the backward sweep's first group and has no
+// upper entries; its z = (m*diag*c2)*w/diag goes to zb, with z.r partials at
+// zpart[block] when zpart != null.
 enum ForwardMode { FWD_RESTART = 0, FWD_UPDATE = 1, FWD_FIRST = 2 };
 template <typename T>
 void launch_forward(hipStream_t s, SellViewT<T> L, int slice0, int nslices, int row0, int nrows,
                     SolveState* st, int mode, int p, T* x, T* r, const T* d, const T* q, T* m,
-                    const T* diag, T omega, const double* spmv_partials, int nspmv);
+                    const T* diag, T omega, const double* spmv_partials, int nspmv, T* zb, T c2,
+                    double* zpart);
 // backward: t = m*diag*c2; z = (t - U z) * w / diag.  With reduce, each
 // block stores its z.r partial at group_partials[slice0 + block], and block 0
 // sums the previous group's partials (prev0, prevn) into *prev_sum.

@@ -73,6 +73,7 @@ void append_group(SellPart& part, const std::vector<int>& rows_dev, const std::v
                   const double* V, Keep keep)
 {
     const int count = (int)rows_dev.size();
+    part.zero_slot = n;
     bool consecutive = true;
     for (int i = 1; i < count && consecutive; i++) consecutive = rows_dev[i] == rows_dev[0] + i;
     part.pending_row0 = (count > 0 && consecutive) ? rows_dev[0] : (count == 0 ? 0 : -1);

@@ -38,6 +38,7 @@ struct SellPart {
     // order (the kernels then compute row ids instead of loading them), else -1
     std::vector<int> group_row0;
     int pending_row0 = -1;
+    int zero_slot = 0;             // padding column = the vectors' zero slot
     long long entries = 0;         // real (unpadded) entries
     long long rows = 0;            // real rows
     int ngroups() const { return (int)group_slice.size() - 1; }

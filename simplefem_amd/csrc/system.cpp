@@ -56,6 +56,7 @@ struct SystemT {
     bool eisen = false;
     bool sweeps = false;
     bool rainbow = false;
+    bool fuse_first_backward = false;  // see forward_sweep
 
     DevPartT<T> full, lower, upper, lower_restart;
     bool has_lower_restart = false;
@@ -177,6 +178,13 @@ struct SystemT {
             upper.upload(plan.upper);
         }
         has_lower_restart = plan.has_lower_restart;
+        if (sweeps && !eisen && lower.ngroups() > 0 && upper.ngroups() > 0) {
+            const int gl = lower.ngroups() - 1;
+            fuse_first_backward = lower.row0(gl) >= 0 && upper.row0(0) == lower.row0(gl) &&
+                                  upper.nrows_of(0) == lower.nrows_of(gl) &&
+                                  upper.group_count(0) == lower.group_count(gl) &&
+                                  upper.group_entries[0] == 0;
+        }
         if (has_lower_restart) lower_restart.upload(plan.lower_restart);
 
         perm = dev_alloc<int>(n);
@@ -217,17 +225,23 @@ struct SystemT {
 
     // mode FWD_RESTART / FWD_UPDATE; the first group of an iteration runs as
     // FWD_FIRST (alpha from the SpMV's partials).
+    // The last forward group also runs the first backward group when that
+    // group has the same rows and no upper entries (fuse_first_backward).
     void forward_sweep(hipStream_t s, const DevPartT<T>& L, int mode, int p, const T* dd)
     {
         // vector streams per row: x r/w, d, r r/w, q, diag, m (iteration);
-        // r, diag, m (restart)
+        // r, diag, m (restart); + z when fused
         const double per_row = mode == FWD_RESTART ? 24.0 : 64.0;
         const int nspmv = spmv_blocks(full.nslices);
-        for (int g = 0; g < L.ngroups(); g++) {
+        const int G = L.ngroups();
+        for (int g = 0; g < G; g++) {
             const int gm = (mode != FWD_RESTART && g == 0) ? FWD_FIRST : mode;
-            run(s, SFA_KERNEL_FORWARD, L.bytes(g, per_row), [&] {
+            const bool fuse = fuse_first_backward && g == G - 1;
+            double* zpart = fuse && mode != FWD_RESTART ? pb + upper.group_begin(0) : nullptr;
+            run(s, SFA_KERNEL_FORWARD, L.bytes(g, per_row + (fuse ? 8.0 : 0.0)), [&] {
                 launch_forward(s, L.view(), L.group_begin(g), L.group_count(g), L.row0(g),
-                               L.nrows_of(g), state, gm, p, x, r, dd, q, m, diag, w(), ps, nspmv);
+                               L.nrows_of(g), state, gm, p, x, r, dd, q, m, diag, w(), ps, nspmv,
+                               fuse ? z : (T*)nullptr, c2(), zpart);
             });
         }
     }
@@ -235,7 +249,7 @@ struct SystemT {
     void backward_sweep(hipStream_t s, int reduce)
     {
         const int G = upper.ngroups();
-        for (int g = 0; g < G; g++) {  // m, diag, z, r
+        for (int g = fuse_first_backward ? 1 : 0; g < G; g++) {  // m, diag, z, r
             const bool fold_prev = reduce && g > 0;
             run(s, SFA_KERNEL_BACKWARD, upper.bytes(g, 32.0), [&] {
                 launch_backward(s, upper.view(), upper.group_begin(g), upper.group_count(g),
@@ -369,7 +383,8 @@ struct SystemT {
         case SFA_METHOD_CG:
         case SFA_METHOD_DIAGONAL: return 2;
         case SFA_METHOD_SSOR:
-        case SFA_METHOD_RAINBOW: return 1 + lower.ngroups() + upper.ngroups();
+        case SFA_METHOD_RAINBOW:
+            return 1 + lower.ngroups() + upper.ngroups() - (fuse_first_backward ? 1 : 0);
         default: return 2 + lower.ngroups() + upper.ngroups();
         }
     }
