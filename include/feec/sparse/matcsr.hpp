@@ -10,9 +10,14 @@
 #include "../base/include.hpp"
 #include "../operators/floatvector.hpp"
 #include "../operators/linearoperator.hpp"
+#include "sparsematrix.hpp"
 
 class MatrixCSR : public LinearOperator {
 public:
+    // from a list of entries (sparse/matcsr.cpp:43-89): an unsorted list is
+    // first sorted and compressed in place (SparseMatrix::sortandcompressentries);
+    // a sorted one is taken as it is, duplicates and zeros included
+    explicit MatrixCSR(const SparseMatrix& matrix);
     explicit MatrixCSR(int rows, int columns, const std::vector<int>& A, const std::vector<int>& C,
                        const std::vector<Float>& V);
     explicit MatrixCSR(int rows, int columns, std::vector<int>&& A, std::vector<int>&& C,
@@ -56,6 +61,16 @@ private:
 };
 
 MatrixCSR MatrixCSRAddition(const MatrixCSR& mat1, const MatrixCSR& mat2, Float s1, Float s2);
+// mat1 * mat2: all products v1*v2, then sorted and compressed (sparse/matcsr.cpp:767-849)
+MatrixCSR MatrixCSRMultiplication(const MatrixCSR& mat1, const MatrixCSR& mat2);
+// mat1 * mat2 over nonzero pairs only, summed per output column in loop
+// order, then compressed (sparse/matcsr.cpp:857-962)
+MatrixCSR MatrixCSRMultiplication_reduced(const MatrixCSR& mat1, const MatrixCSR& mat2);
+// factor^T * center * factor without compression: every product
+// (v_j * v_i) * v_k as its own entry (sparse/matcsr.cpp:1067-1178)
+MatrixCSR Conjugation(const MatrixCSR& center, const MatrixCSR& factor);
+
+inline MatrixCSR operator&(const MatrixCSR& m1, const MatrixCSR& m2) { return MatrixCSRMultiplication(m1, m2); }
 
 inline MatrixCSR operator+(const MatrixCSR& m1, const MatrixCSR& m2) { return MatrixCSRAddition(m1, m2, 1., 1.); }
 inline MatrixCSR operator-(const MatrixCSR& m1, const MatrixCSR& m2) { return MatrixCSRAddition(m1, m2, 1., -1.); }

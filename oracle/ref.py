@@ -41,6 +41,12 @@ def _init_library():
     _lib.feecref_cg.argtypes = [c_int, c_int, _P_D, _P_D, _P_I, _P_I, _P_D, _P_D, c_double, c_int,
                                 _P_D, c_double, c_int, _P_I, _P_I, _P_I, _P_D]
     _lib.feecref_cg.restype = c_int
+    _lib.feecref_coo_to_csr.argtypes = [c_int, c_int, c_int, _P_I, _P_I, _P_D]
+    _lib.feecref_coo_to_csr.restype = c_int
+    _lib.feecref_csr_op.argtypes = [c_int, c_int, c_int, _P_I, _P_I, _P_D, c_int, c_int, _P_I,
+                                    _P_I, _P_D, c_double, c_double]
+    _lib.feecref_csr_op.restype = c_int
+    _lib.feecref_result.argtypes = [_P_I, _P_I, _P_I, _P_D]
 
 
 def _i(a):
@@ -51,11 +57,38 @@ def _d(a):
     return None if a is None else a.ctypes.data_as(_P_D)
 
 
+def _result(nnz, rows):
+    dims = np.zeros(2, dtype=np.int32)
+    A = np.empty(rows + 1, dtype=np.int32)
+    C = np.empty(max(nnz, 1), dtype=np.int32)
+    V = np.empty(max(nnz, 1))
+    _lib.feecref_result(_i(dims), _i(A), _i(C), _d(V))
+    return int(dims[0]), int(dims[1]), A, C[:nnz].copy(), V[:nnz].copy()
+
+
 def _task(kind, kw):
     L = _lib
     if kind == "info":
         return {"desired_precision": L.feecref_desired_precision(),
                 "threads": L.feecref_omp_threads()}
+    if kind == "coo_to_csr":
+        er = np.ascontiguousarray(kw["rows_of"], dtype=np.int32)
+        ec = np.ascontiguousarray(kw["cols_of"], dtype=np.int32)
+        ev = np.ascontiguousarray(kw["vals_of"], dtype=np.float64)
+        nnz = L.feecref_coo_to_csr(int(kw["rows"]), int(kw["cols"]), len(ev), _i(er), _i(ec),
+                                   _d(ev))
+        return _result(nnz, int(kw["rows"]))
+    if kind == "csr_op":
+        m1, m2 = kw["m1"], kw["m2"] if kw.get("m2") is not None else kw["m1"]
+        a1 = [np.ascontiguousarray(m1[k], dtype=t) for k, t in ((2, np.int32), (3, np.int32), (4, np.float64))]
+        a2 = [np.ascontiguousarray(m2[k], dtype=t) for k, t in ((2, np.int32), (3, np.int32), (4, np.float64))]
+        nnz = L.feecref_csr_op(int(kw["op"]), int(m1[0]), int(m1[1]), _i(a1[0]), _i(a1[1]),
+                               _d(a1[2]), int(m2[0]), int(m2[1]), _i(a2[0]), _i(a2[1]), _d(a2[2]),
+                               float(kw.get("s1", 1.0)), float(kw.get("s2", 1.0)))
+        if nnz < 0:
+            raise ValueError("unknown csr op")
+        out_rows = {2: m2[1], 3: m1[1]}.get(int(kw["op"]), m1[0])
+        return _result(nnz, int(out_rows))
     A = np.ascontiguousarray(kw["A"], dtype=np.int32)
     C = np.ascontiguousarray(kw["C"], dtype=np.int32)
     V = np.ascontiguousarray(kw["V"], dtype=np.float64)
@@ -128,6 +161,18 @@ class RefRunner:
 
     def info(self):
         return self._call("info")
+
+    def coo_to_csr(self, rows, cols, rows_of, cols_of, vals_of):
+        """MatrixCSR(SparseMatrix(rows, cols, entries)): (rows, cols, A, C, V)."""
+        return self._call("coo_to_csr", rows=rows, cols=cols, rows_of=rows_of, cols_of=cols_of,
+                          vals_of=vals_of)
+
+    def csr_op(self, op, m1, m2=None, s1=1.0, s2=1.0):
+        """op in mul, mul_reduced, conjugation, transpose, add, compress; a
+        matrix is (rows, cols, A, C, V).  Returns (rows, cols, A, C, V)."""
+        ops = {"mul": 0, "mul_reduced": 1, "conjugation": 2, "transpose": 3, "add": 4,
+               "compress": 5}
+        return self._call("csr_op", op=ops[op], m1=m1, m2=m2, s1=s1, s2=s2)
 
     def rainbow(self, A, C, V):
         return self._call("rainbow", A=A, C=C, V=V)

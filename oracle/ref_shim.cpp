@@ -18,6 +18,7 @@
 #include "solver/sparsesolver.hpp"
 #include "sparse/matcsr.hpp"
 #include "sparse/rainbow.hpp"
+#include "sparse/sparsematrix.hpp"
 
 namespace {
 
@@ -33,9 +34,75 @@ MatrixCSR make_csr(int n, const int* A, const int* C, const double* V)
                      std::vector<Float>(V, V + nnz));
 }
 
+// The result of the last feecref_coo_to_csr / feecref_csr_op call, until
+// feecref_result copies it out.
+struct Result {
+    int rows = 0, cols = 0;
+    std::vector<int> A, C;
+    std::vector<double> V;
+} g_result;
+
+int keep(const MatrixCSR& m)
+{
+    g_result.rows = m.getdimout();
+    g_result.cols = m.getdimin();
+    const int nnz = m.getA()[m.getdimout()];
+    g_result.A.assign(m.getA(), m.getA() + m.getdimout() + 1);
+    g_result.C.assign(m.getC(), m.getC() + nnz);
+    g_result.V.assign(m.getV(), m.getV() + nnz);
+    return nnz;
+}
+
+MatrixCSR make_rect(int rows, int cols, const int* A, const int* C, const double* V)
+{
+    const int nnz = A[rows];
+    return MatrixCSR(rows, cols, std::vector<int>(A, A + rows + 1), std::vector<int>(C, C + nnz),
+                     std::vector<Float>(V, V + nnz));
+}
+
 }  // namespace
 
 extern "C" {
+
+// MatrixCSR(const SparseMatrix&) (sparse/matcsr.cpp:43-89) of a list of entries.
+int feecref_coo_to_csr(int rows, int cols, int n, const int* er, const int* ec, const double* ev)
+{
+    std::vector<SparseMatrix::MatrixEntry> list(n);
+    for (int i = 0; i < n; i++) {
+        list[i].row = er[i];
+        list[i].column = ec[i];
+        list[i].value = ev[i];
+    }
+    SparseMatrix sm(rows, cols, list);
+    return keep(MatrixCSR(sm));
+}
+
+// 0 MatrixCSRMultiplication, 1 MatrixCSRMultiplication_reduced,
+// 2 Conjugation(center = m1, factor = m2), 3 m1.getTranspose(),
+// 4 MatrixCSRAddition(m1, m2, s1, s2), 5 m1.compressentries().
+int feecref_csr_op(int op, int r1, int c1, const int* A1, const int* C1, const double* V1, int r2,
+                   int c2, const int* A2, const int* C2, const double* V2, double s1, double s2)
+{
+    MatrixCSR m1 = make_rect(r1, c1, A1, C1, V1);
+    switch (op) {
+    case 0: return keep(MatrixCSRMultiplication(m1, make_rect(r2, c2, A2, C2, V2)));
+    case 1: return keep(MatrixCSRMultiplication_reduced(m1, make_rect(r2, c2, A2, C2, V2)));
+    case 2: return keep(Conjugation(m1, make_rect(r2, c2, A2, C2, V2)));
+    case 3: return keep(m1.getTranspose());
+    case 4: return keep(MatrixCSRAddition(m1, make_rect(r2, c2, A2, C2, V2), s1, s2));
+    case 5: m1.compressentries(); return keep(m1);
+    default: return -1;
+    }
+}
+
+void feecref_result(int* dims, int* A, int* C, double* V)
+{
+    dims[0] = g_result.rows;
+    dims[1] = g_result.cols;
+    std::memcpy(A, g_result.A.data(), sizeof(int) * g_result.A.size());
+    std::memcpy(C, g_result.C.data(), sizeof(int) * g_result.C.size());
+    std::memcpy(V, g_result.V.data(), sizeof(double) * g_result.V.size());
+}
 
 double feecref_desired_precision() { return desired_precision; }
 

@@ -38,6 +38,8 @@ SOURCES = [  # (source, compile as HIP)
 CXX_SOURCES = ["dropin.cpp"]
 DROPIN_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_test.cpp")
 DROPIN_TEST_BIN = os.path.join(OBJ, "dropin_test")
+SPARSE_OPS_SRC = os.path.join(ROOT, "tests", "cpp", "sparse_ops_test.cpp")
+SPARSE_OPS_BIN = os.path.join(OBJ, "sparse_ops_test")
 
 
 def _run(cmd):
@@ -77,9 +79,10 @@ def build(verbose: bool = False) -> str:
              + ["-L" + HERE, "-lsimplefem_amd", "-Wl,-rpath,$ORIGIN"])
         os.replace(tmp, CXXLIB)
         # a caller written like the reference's tests, against the drop-in headers
-        _run(["g++", "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include"), DROPIN_TEST_SRC,
-              "-o", DROPIN_TEST_BIN, "-L" + HERE, "-lsimplefem_amd_cxx", "-lsimplefem_amd",
-              "-Wl,-rpath,$ORIGIN/.."])
+        for src, binary in ((DROPIN_TEST_SRC, DROPIN_TEST_BIN), (SPARSE_OPS_SRC, SPARSE_OPS_BIN)):
+            _run(["g++", "-std=c++17", "-O2", "-I" + os.path.join(ROOT, "include"), src,
+                  "-o", binary, "-L" + HERE, "-lsimplefem_amd_cxx", "-lsimplefem_amd",
+                  "-Wl,-rpath,$ORIGIN/.."])
     if verbose:
         print("built", LIB)
     return LIB

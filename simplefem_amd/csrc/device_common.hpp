@@ -129,11 +129,20 @@ struct SellChunk {
     T v[U];
 };
 
+// Slice `local` of a group whose slices all have gw entries per lane starts
+// at gbase + local*64*gw; with gw < 0 the offsets are read from the part.
 template <typename T, int U>
-__device__ inline void sell_prefetch(const SellViewT<T>& A, int slice, int lane, SellChunk<T, U>& R)
+__device__ inline void sell_prefetch(const SellViewT<T>& A, int slice, int local, int gbase,
+                                     int gw, int lane, SellChunk<T, U>& R)
 {
-    const int o0 = A.off[slice];
-    R.w = (A.off[slice + 1] - o0) / kWave;
+    int o0;
+    if (gw >= 0) {
+        o0 = gbase + local * (gw * kWave);
+        R.w = gw;
+    } else {
+        o0 = A.off[slice];
+        R.w = (A.off[slice + 1] - o0) / kWave;
+    }
     R.e = o0 + lane;
 #pragma unroll
     for (int u = 0; u < U; u++) {

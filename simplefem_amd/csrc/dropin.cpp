@@ -16,6 +16,9 @@
 #include "../../include/feec/solver/sparsesolver.hpp"
 #include "../../include/feec/sparse/matcsr.hpp"
 #include "../../include/feec/sparse/rainbow.hpp"
+#include "../../include/feec/sparse/sparsematrix.hpp"
+
+#include <set>
 #include "../../include/simplefem_amd.h"
 
 namespace {
@@ -561,6 +564,253 @@ MatrixCSR MatrixCSRAddition(const MatrixCSR& m1, const MatrixCSR& m2, Float s1, 
     }
     sort_and_compress(A, C, V);
     return MatrixCSR(rows, m1.getdimin(), std::move(A), std::move(C), std::move(V));
+}
+
+// mat1 * mat2 (sparse/matcsr.cpp:767-849): row r collects, for each entry
+// (r, c1) of mat1 and each entry (c1, c2) of mat2 in order, the product
+// v1 * v2 at column c2; sort_and_compress then merges and drops zeros.
+MatrixCSR MatrixCSRMultiplication(const MatrixCSR& m1, const MatrixCSR& m2)
+{
+    require_true(m1.getdimin() == m2.getdimout(), "MatrixCSRMultiplication dimensions");
+    const int rows = m1.getdimout();
+    const int *A1 = m1.getA(), *C1 = m1.getC(), *A2 = m2.getA(), *C2 = m2.getC();
+    const Float *V1 = m1.getV(), *V2 = m2.getV();
+    std::vector<int> A(rows + 1, 0);
+    for (int r = 0; r < rows; r++) {
+        int count = 0;
+        for (int i = A1[r]; i < A1[r + 1]; i++) count += A2[C1[i] + 1] - A2[C1[i]];
+        A[r + 1] = A[r] + count;
+    }
+    std::vector<int> C(A[rows]);
+    std::vector<Float> V(A[rows]);
+    for (int r = 0; r < rows; r++) {
+        int at = A[r];
+        for (int i = A1[r]; i < A1[r + 1]; i++)
+            for (int k = A2[C1[i]]; k < A2[C1[i] + 1]; k++, at++) {
+                C[at] = C2[k];
+                V[at] = V1[i] * V2[k];
+            }
+    }
+    sort_and_compress(A, C, V);
+    return MatrixCSR(rows, m2.getdimin(), std::move(A), std::move(C), std::move(V));
+}
+
+// sparse/matcsr.cpp:857-962: the output pattern of row r is the set of
+// columns c2 reached through a pair of nonzero values; each output value
+// starts at 0 and adds v1 * v2 over those pairs in (c1, c2) loop order.
+MatrixCSR MatrixCSRMultiplication_reduced(const MatrixCSR& m1, const MatrixCSR& m2)
+{
+    require_true(m1.getdimin() == m2.getdimout(), "MatrixCSRMultiplication_reduced dimensions");
+    const int rows = m1.getdimout();
+    const int *A1 = m1.getA(), *C1 = m1.getC(), *A2 = m2.getA(), *C2 = m2.getC();
+    const Float *V1 = m1.getV(), *V2 = m2.getV();
+    std::vector<int> A(rows + 1, 0);
+    std::vector<int> C;
+    std::vector<Float> V;
+    std::vector<int> at(m2.getdimin(), -1);  // output position of a column in the current row
+    for (int r = 0; r < rows; r++) {
+        std::set<int> cols;
+        for (int i = A1[r]; i < A1[r + 1]; i++)
+            for (int k = A2[C1[i]]; k < A2[C1[i] + 1]; k++)
+                if (V1[i] != 0. && V2[k] != 0.) cols.insert(C2[k]);
+        const int base = (int)C.size();
+        for (int c : cols) {
+            at[c] = (int)C.size();
+            C.push_back(c);
+            V.push_back(0.);
+        }
+        for (int i = A1[r]; i < A1[r + 1]; i++)
+            for (int k = A2[C1[i]]; k < A2[C1[i] + 1]; k++)
+                if (V1[i] != 0. && V2[k] != 0.) V[at[C2[k]]] += V1[i] * V2[k];
+        A[r + 1] = (int)C.size();
+        for (int p = base; p < (int)C.size(); p++) at[C[p]] = -1;
+    }
+    sort_and_compress(A, C, V);
+    return MatrixCSR(rows, m2.getdimin(), std::move(A), std::move(C), std::move(V));
+}
+
+// sparse/matcsr.cpp:1067-1178: for each row r of center and factor, each
+// entry (r, ci) of center, (r, cj) of factor and (ci, ck) of factor gives the
+// entry (cj, ck) with value (vj * vi) * vk; entries of an output row are
+// placed from the end of its range backwards, in that loop order, and are
+// neither sorted nor merged.
+MatrixCSR Conjugation(const MatrixCSR& center, const MatrixCSR& factor)
+{
+    require_true(center.is_square() && center.getdimin() == factor.getdimout(),
+                 "Conjugation dimensions");
+    const int N = center.getdimout();
+    const int L = factor.getdimin();
+    const int *cA = center.getA(), *cC = center.getC(), *fA = factor.getA(), *fC = factor.getC();
+    const Float *cV = center.getV(), *fV = factor.getV();
+    std::vector<int> left(L, 0);  // entries still to place in each output row
+    for (int r = 0; r < N; r++) {
+        int per = 0;
+        for (int i = cA[r]; i < cA[r + 1]; i++) per += fA[cC[i] + 1] - fA[cC[i]];
+        for (int j = fA[r]; j < fA[r + 1]; j++) left[fC[j]] += per;
+    }
+    std::vector<int> A(L + 1, 0);
+    for (int c = 0; c < L; c++) A[c + 1] = A[c] + left[c];
+    std::vector<int> C(A[L], -1);
+    std::vector<Float> V(A[L], notanumber);
+    for (int r = 0; r < N; r++)
+        for (int i = cA[r]; i < cA[r + 1]; i++)
+            for (int j = fA[r]; j < fA[r + 1]; j++) {
+                const int cj = fC[j];
+                const Float vjvi = fV[j] * cV[i];
+                const int ci = cC[i];
+                for (int k = fA[ci]; k < fA[ci + 1]; k++) {
+                    const int p = A[cj] + (--left[cj]);
+                    C[p] = fC[k];
+                    V[p] = vjvi * fV[k];
+                }
+            }
+    return MatrixCSR(L, L, std::move(A), std::move(C), std::move(V));
+}
+
+// ============================================================== SparseMatrix
+
+SparseMatrix::SparseMatrix(int dimout, int dimin, int numentries,
+                           const std::function<MatrixEntry(int)>& generator)
+    : LinearOperator(dimout, dimin), entries()
+{
+    require_true(numentries >= 0, "SparseMatrix: numentries >= 0");
+    entries.reserve(numentries);
+    for (int i = 0; i < numentries; i++) appendentry(generator(i));
+}
+
+SparseMatrix::SparseMatrix(int dimout, int dimin, const std::vector<MatrixEntry>& list)
+    : LinearOperator(dimout, dimin), entries()
+{
+    for (const MatrixEntry& e : list) appendentry(e);
+}
+
+SparseMatrix::SparseMatrix(int dimout, int dimin, const std::initializer_list<MatrixEntry>& list)
+    : LinearOperator(dimout, dimin), entries()
+{
+    for (const MatrixEntry& e : list) appendentry(e);
+}
+
+void SparseMatrix::check() const
+{
+    for (const MatrixEntry& e : entries)
+        require_true(0 <= e.row && e.row < getdimout() && 0 <= e.column && e.column < getdimin(),
+                     "SparseMatrix: entry in range");
+}
+
+std::string SparseMatrix::text() const
+{
+    std::string ret = "SparseMatrix " + std::to_string(getdimout()) + "x" + std::to_string(getdimin());
+    for (const MatrixEntry& e : entries)
+        ret += "\n" + std::to_string(e.row) + " " + std::to_string(e.column) + " : " +
+               std::to_string(e.value);
+    return ret;
+}
+
+void SparseMatrix::apply(FloatVector& dest, const FloatVector& add, Float scaling) const
+{
+    require_true(getdimin() == add.getdimension() && getdimout() == dest.getdimension(),
+                 "SparseMatrix::apply dimensions");
+    dest.zero();
+    for (const MatrixEntry& e : entries) dest[e.row] += scaling * e.value * add[e.column];
+}
+
+void SparseMatrix::scale(Float s)
+{
+    for (MatrixEntry& e : entries) e.value *= s;
+}
+
+bool SparseMatrix::is_finite() const
+{
+    for (const MatrixEntry& e : entries)
+        if (!std::isfinite(e.value)) return false;
+    return true;
+}
+
+bool SparseMatrix::is_sorted(MatrixEntrySorting manner) const
+{
+    const bool rowwise = manner == MatrixEntrySorting::rowwise;
+    for (size_t i = 1; i < entries.size(); i++) {
+        const MatrixEntry& a = entries[i - 1];
+        const MatrixEntry& b = entries[i];
+        const int a1 = rowwise ? a.row : a.column, a2 = rowwise ? a.column : a.row;
+        const int b1 = rowwise ? b.row : b.column, b2 = rowwise ? b.column : b.row;
+        if (a1 > b1 || (a1 == b1 && a2 > b2)) return false;
+    }
+    return true;
+}
+
+// The standard library's (unstable) sort with the strict (major, minor)
+// order: entries with equal keys end up in the order this sort leaves them,
+// which the compression then sums in -- the same sort over the same input as
+// the reference's, so the same sums.
+const SparseMatrix& SparseMatrix::sortentries(MatrixEntrySorting manner) const
+{
+    if (manner == MatrixEntrySorting::rowwise)
+        std::sort(entries.begin(), entries.end(), [](const MatrixEntry& a, const MatrixEntry& b) {
+            return a.row < b.row || (a.row == b.row && a.column < b.column);
+        });
+    else
+        std::sort(entries.begin(), entries.end(), [](const MatrixEntry& a, const MatrixEntry& b) {
+            return a.column < b.column || (a.column == b.column && a.row < b.row);
+        });
+    return *this;
+}
+
+const SparseMatrix& SparseMatrix::sortandcompressentries(MatrixEntrySorting manner) const
+{
+    entries.erase(std::remove_if(entries.begin(), entries.end(),
+                                 [](const MatrixEntry& e) { return e.value == 0.; }),
+                  entries.end());
+    sortentries(manner);
+    if (entries.empty()) {
+        // the reference's compaction always keeps dest+1 entries: an empty
+        // list becomes one value-initialised entry (0, 0, 0.)
+        // (sparse/sparsematrix.cpp:438-458)
+        if (getdimout() > 0 && getdimin() > 0) entries.resize(1, MatrixEntry{0, 0, 0.});
+        return *this;
+    }
+    size_t last = 0;  // the entry that equal keys are added into
+    for (size_t i = 1; i < entries.size(); i++) {
+        if (entries[i].row == entries[last].row && entries[i].column == entries[last].column)
+            entries[last].value += entries[i].value;
+        else
+            entries[++last] = entries[i];
+    }
+    entries.resize(last + 1);
+    return *this;
+}
+
+void SparseMatrix::setentry(int i, MatrixEntry entry)
+{
+    require_true(0 <= i && i < (int)entries.size(), "SparseMatrix::setentry index");
+    entries[i] = entry;
+}
+
+void SparseMatrix::appendentry(MatrixEntry entry)
+{
+    require_true(0 <= entry.row && entry.row < getdimout() && 0 <= entry.column &&
+                     entry.column < getdimin(),
+                 "SparseMatrix: entry in range");
+    entries.push_back(entry);
+}
+
+// sparse/matcsr.cpp:43-89
+MatrixCSR::MatrixCSR(const SparseMatrix& matrix)
+    : LinearOperator(matrix.getdimout(), matrix.getdimin()), A(), C(), V()
+{
+    matrix.check();
+    if (!matrix.is_sorted()) matrix.sortandcompressentries();
+    const auto& list = matrix.getentries();
+    const int rows = matrix.getdimout();
+    A.assign(rows + 1, 0);
+    C.resize(list.size());
+    V.resize(list.size());
+    for (size_t i = 0; i < list.size(); i++) {
+        A[list[i].row + 1]++;
+        C[i] = list[i].column;
+        V[i] = list[i].value;
+    }
+    for (int r = 1; r <= rows; r++) A[r] += A[r - 1];
 }
 
 // ============================================================== Rainbow

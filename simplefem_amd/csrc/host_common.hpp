@@ -63,6 +63,8 @@ struct DevPartT {
     std::vector<int> group_slice;
     std::vector<long long> group_entries, group_rows;
     std::vector<int> group_row0;
+    std::vector<int> group_width;  // uniform entries per lane, or -1
+    std::vector<int> group_base;   // first entry of each group
     long long stored = 0;
     long long entries = 0;
     long long nrows = 0;
@@ -76,6 +78,10 @@ struct DevPartT {
         group_entries = p.group_entries;
         group_rows = p.group_rows;
         group_row0 = p.group_row0;
+        group_width = p.group_width;
+        group_base.clear();
+        for (size_t g = 0; g + 1 < p.group_slice.size(); g++)
+            group_base.push_back(p.slice_off[p.group_slice[g]]);
         stored = (long long)p.vals.size();
         entries = p.entries;
         nrows = p.rows;
@@ -106,6 +112,8 @@ struct DevPartT {
     int group_begin(int g) const { return group_slice[g]; }
     int group_count(int g) const { return group_slice[g + 1] - group_slice[g]; }
     int row0(int g) const { return group_row0[g]; }
+    int width(int g) const { return group_width[g]; }
+    int base(int g) const { return group_base[g]; }
     int nrows_of(int g) const { return (int)group_rows[g]; }
     // Algorithmic bytes of one launch over group g: (sizeof(T) + 4) B per
     // entry (value + column), 4 B per row id, `per_row` B of vector traffic

@@ -308,7 +308,8 @@ __global__ void __launch_bounds__(kSellBlock) k_forward(SellViewT<T> L, int slic
                                                         const T* d, const T* q, T* m,
                                                         const T* diag, T omega,
                                                         const double* spmv_partials, int nspmv,
-                                                        T* zb, T c2, double* zpart)
+                                                        T* zb, T c2, double* zpart, int gbase,
+                                                        int gw)
 {
     const int lane = threadIdx.x;
     const int b = xcd_block(blockIdx.x, gridDim.x);
@@ -325,7 +326,7 @@ __global__ void __launch_bounds__(kSellBlock) k_forward(SellViewT<T> L, int slic
         qq = q[ri];
     }
     SellChunk<T> R;
-    if (has) sell_prefetch(L, slice, lane, R);
+    if (has) sell_prefetch(L, slice, b, gbase, gw, lane, R);
     if (st->done) return;
     T alpha = T(0);
     if (mode == FWD_FIRST) {
@@ -369,7 +370,7 @@ __global__ void __launch_bounds__(kSellBlock) k_backward(SellViewT<T> U, int sli
                                                          int reduce, const T* r, const T* m,
                                                          T* z, const T* diag, T omega, T c2,
                                                          double* gpart, int prev0, int prevn,
-                                                         double* prev_sum)
+                                                         double* prev_sum, int gbase, int gw)
 {
     const int lane = threadIdx.x;
     const int b = xcd_block(blockIdx.x, gridDim.x);
@@ -381,7 +382,7 @@ __global__ void __launch_bounds__(kSellBlock) k_backward(SellViewT<T> U, int sli
     const T mm = m[ri];
     const T rr = reduce ? r[ri] : T(0);
     SellChunk<T> R;
-    if (has) sell_prefetch(U, slice, lane, R);
+    if (has) sell_prefetch(U, slice, b, gbase, gw, lane, R);
     if (st->done) return;
     if (reduce && prev_sum && b == nslices) {
         const double t = wave_fold(gpart + prev0, prevn);
@@ -409,7 +410,7 @@ __global__ void __launch_bounds__(kSellBlock) k_eisenstat_solve(SellViewT<T> P, 
                                                                 const SolveState* st, int with_t,
                                                                 const T* in, const T* t,
                                                                 T* out, const T* diag,
-                                                                T omega, T c3)
+                                                                T omega, T c3, int gbase, int gw)
 {
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int slice = slice0 + b;
@@ -421,7 +422,7 @@ __global__ void __launch_bounds__(kSellBlock) k_eisenstat_solve(SellViewT<T> P, 
     T a = in[ri];
     const T tt = with_t ? t[ri] : T(0);
     SellChunk<T> R;
-    sell_prefetch(P, slice, lane, R);
+    sell_prefetch(P, slice, b, gbase, gw, lane, R);
     if (st->done) return;
     if (row < 0) return;
     if (with_t) a = a - c3 * dg * tt;
@@ -439,7 +440,7 @@ __global__ void __launch_bounds__(kSellBlock) k_eisenstat_upper(SellViewT<T> P, 
                                                                 const T* z, const T* dold,
                                                                 T* dnew, T* out, const T* diag,
                                                                 T omega, const double* fpart,
-                                                                int nfold)
+                                                                int nfold, int gbase, int gw)
 {
     const int lane = threadIdx.x;
     const int b = xcd_block(blockIdx.x, gridDim.x);
@@ -451,7 +452,7 @@ __global__ void __launch_bounds__(kSellBlock) k_eisenstat_upper(SellViewT<T> P, 
     const T zz = z[ri];
     const T dd = dold[ri];
     SellChunk<T> R;
-    if (has) sell_prefetch(P, slice, lane, R);
+    if (has) sell_prefetch(P, slice, b, gbase, gw, lane, R);
     if (st->done) return;
     const int fold = st->fold[p];
     T beta;
@@ -486,7 +487,8 @@ __global__ void __launch_bounds__(kSpmvBlock) k_spmv_dir(SellViewT<T> A, int nsl
                                                          const T* src, const T* dold, T* dnew,
                                                          T* q, double* spart,
                                                          const double* fpart, int nfold,
-                                                         const double* gsum, int ngroups)
+                                                         const double* gsum, int ngroups,
+                                                         int gw)
 {
     // the wavefront index is uniform: readfirstlane lets the slice offsets be
     // scalar loads
@@ -503,7 +505,7 @@ __global__ void __launch_bounds__(kSpmvBlock) k_spmv_dir(SellViewT<T> A, int nsl
         const int ri = row >= 0 ? row : A.zero;
         s_own = src[ri];
         d_own = dold[ri];
-        sell_prefetch(A, first_slice, lane, R);
+        sell_prefetch(A, first_slice, first_slice, 0, gw, lane, R);
     }
     const int fold = st->fold[p];
     const int K = st->Kp[p];
@@ -537,7 +539,7 @@ __global__ void __launch_bounds__(kSpmvBlock) k_spmv_dir(SellViewT<T> A, int nsl
             const int ri = row >= 0 ? row : A.zero;
             s_own = src[ri];
             d_own = dold[ri];
-            sell_prefetch(A, slice, lane, R);
+            sell_prefetch(A, slice, slice, 0, gw, lane, R);
         }
         const T acc = sell_finish<false>(A, R, dn, T(0));
         if (row >= 0) {
@@ -722,7 +724,7 @@ template <typename T>
 void launch_forward(hipStream_t s, SellViewT<T> L, int slice0, int nslices, int row0, int nrows,
                     SolveState* st, int mode, int p, T* x, T* r, const T* d, const T* q, T* m,
                     const T* diag, T omega, const double* spmv_partials, int nspmv, T* zb, T c2,
-                    double* zpart)
+                    double* zpart, int gbase, int gw)
 {
     // The first group of an iteration runs one block even when empty: it
     // forms alpha for the groups after it.
@@ -730,57 +732,58 @@ void launch_forward(hipStream_t s, SellViewT<T> L, int slice0, int nslices, int 
     if (grid == 0) return;
     k_forward<T><<<grid, kSellBlock, 0, s>>>(L, slice0, nslices, row0, nrows, st, mode, p, x, r, d,
                                              q, m, diag, omega, spmv_partials, nspmv, zb, c2,
-                                             zpart);
+                                             zpart, gbase, gw);
 }
 
 template <typename T>
 void launch_backward(hipStream_t s, SellViewT<T> U, int slice0, int nslices, int row0, int nrows,
                      SolveState* st, int reduce, const T* r, const T* m, T* z, const T* diag,
                      T omega, T c2, double* group_partials, int prev0, int prevn,
-                     double* prev_sum)
+                     double* prev_sum, int gbase, int gw)
 {
     // one more block sums the previous group's partials
     const int grid = nslices + (reduce && prev_sum ? 1 : 0);
     if (grid == 0) return;
     k_backward<T><<<grid, kSellBlock, 0, s>>>(U, slice0, nslices, row0, nrows, st, reduce, r, m, z,
                                               diag, omega, c2, group_partials, prev0, prevn,
-                                              prev_sum);
+                                              prev_sum, gbase, gw);
 }
 
 template <typename T>
 void launch_eisenstat_solve(hipStream_t s, SellViewT<T> P, int slice0, int nslices, int row0,
                             int nrows, SolveState* st, int with_t, const T* in, const T* t,
-                            T* out, const T* diag, T omega, T c3)
+                            T* out, const T* diag, T omega, T c3, int gbase, int gw)
 {
     if (nslices <= 0) return;
     k_eisenstat_solve<T><<<nslices, kSellBlock, 0, s>>>(P, slice0, nslices, row0, nrows, st,
-                                                        with_t, in, t, out, diag, omega, c3);
+                                                        with_t, in, t, out, diag, omega, c3,
+                                                        gbase, gw);
 }
 
 template <typename T>
 void launch_eisenstat_upper(hipStream_t s, SellViewT<T> P, int slice0, int nslices, int row0,
                             int nrows, SolveState* st, int first, int p, const T* z,
                             const T* dold, T* dnew, T* out, const T* diag, T omega,
-                            const double* fold_partials, int nfold)
+                            const double* fold_partials, int nfold, int gbase, int gw)
 {
     const int grid = nslices > 0 ? nslices : (first ? 1 : 0);
     if (grid == 0) return;
     k_eisenstat_upper<T><<<grid, kSellBlock, 0, s>>>(P, slice0, nslices, row0, nrows, st, first, p,
                                                      z, dold, dnew, out, diag, omega,
-                                                     fold_partials, nfold);
+                                                     fold_partials, nfold, gbase, gw);
 }
 
 template <typename T>
 void launch_spmv_dir(hipStream_t s, SellViewT<T> A, int nslices, int row0, int nrows,
                      int n_active, SolveState* st, int p, const T* src, const T* dold, T* dnew,
                      T* q, double* spmv_partials, const double* fold_partials, int nfold,
-                     const double* group_sums, int ngroups)
+                     const double* group_sums, int ngroups, int gw)
 {
     k_spmv_dir<T><<<spmv_blocks(nslices), kSpmvBlock, 0, s>>>(A, nslices, row0, nrows, n_active,
                                                               st, p, src,
                                                               dold, dnew, q, spmv_partials,
                                                               fold_partials, nfold, group_sums,
-                                                              ngroups);
+                                                              ngroups, gw);
 }
 
 template <typename T>
@@ -829,19 +832,19 @@ void launch_add_correction(hipStream_t s, double* x, const float* e, int n)
                                            T, T*, T*, double*);                                \
     template void launch_forward<T>(hipStream_t, SellViewT<T>, int, int, int, int, SolveState*, \
                                     int, int, T*, T*, const T*, const T*, T*, const T*, T,     \
-                                    const double*, int, T*, T, double*);                       \
+                                    const double*, int, T*, T, double*, int, int);             \
     template void launch_backward<T>(hipStream_t, SellViewT<T>, int, int, int, int,            \
                                      SolveState*, int, const T*, const T*, T*, const T*, T, T, \
-                                     double*, int, int, double*);                              \
+                                     double*, int, int, double*, int, int);                    \
     template void launch_eisenstat_solve<T>(hipStream_t, SellViewT<T>, int, int, int, int,     \
                                             SolveState*, int, const T*, const T*, T*,          \
-                                            const T*, T, T);                                   \
+                                            const T*, T, T, int, int);                         \
     template void launch_eisenstat_upper<T>(hipStream_t, SellViewT<T>, int, int, int, int,     \
                                             SolveState*, int, int, const T*, const T*, T*, T*, \
-                                            const T*, T, const double*, int);                  \
+                                            const T*, T, const double*, int, int, int);        \
     template void launch_spmv_dir<T>(hipStream_t, SellViewT<T>, int, int, int, int,            \
                                      SolveState*, int, const T*, const T*, T*, T*, double*,    \
-                                     const double*, int, const double*, int);                  \
+                                     const double*, int, const double*, int, int);             \
     template void launch_update<T>(hipStream_t, UpdateMode, SolveState*, int, int, T*, T*,     \
                                    const T*, const T*, T*, const T*, T, const double*, int,    \
                                    double*);                                                   \

@@ -106,6 +106,9 @@ void launch_eisenstat_init(hipStream_t s, SolveState* st, int n_active, const T*
                            const T* diag, T omega, T* z, T* d, double* partials);
 
 // --- sweeps over one group (colour or level) of a triangular part ---
+// (gbase, gw): the group's first entry and, when all its slices have the same
+// number of entries per lane, that number (the kernels then compute slice
+// offsets instead of loading them); gw = -1 otherwise.
 // mode FWD_RESTART: m = (r - L m) * w / diag.
 // mode FWD_UPDATE:  x += a d; r -= a q; then the same, a = st->alpha.
 // mode FWD_FIRST:   as FWD_UPDATE, but first forms d.Ad from the SpMV's
@@ -119,7 +122,7 @@ template <typename T>
 void launch_forward(hipStream_t s, SellViewT<T> L, int slice0, int nslices, int row0, int nrows,
                     SolveState* st, int mode, int p, T* x, T* r, const T* d, const T* q, T* m,
                     const T* diag, T omega, const double* spmv_partials, int nspmv, T* zb, T c2,
-                    double* zpart);
+                    double* zpart, int gbase, int gw);
 // backward: t = m*diag*c2; z = (t - U z) * w / diag.  With reduce, each
 // block stores its z.r partial at group_partials[slice0 + block], and block 0
 // sums the previous group's partials (prev0, prevn) into *prev_sum.
@@ -127,12 +130,12 @@ template <typename T>
 void launch_backward(hipStream_t s, SellViewT<T> U, int slice0, int nslices, int row0, int nrows,
                      SolveState* st, int reduce, const T* r, const T* m, T* z, const T* diag,
                      T omega, T c2, double* group_partials, int prev0, int prevn,
-                     double* prev_sum);
+                     double* prev_sum, int gbase, int gw);
 // Eisenstat: out = (in - [c3*diag*t] - P out) * w / diag  (P = L or U part)
 template <typename T>
 void launch_eisenstat_solve(hipStream_t s, SellViewT<T> P, int slice0, int nslices, int row0,
                             int nrows, SolveState* st, int with_t, const T* in, const T* t,
-                            T* out, const T* diag, T omega, T c3);
+                            T* out, const T* diag, T omega, T c3, int gbase, int gw);
 // Eisenstat upper solve with the direction update of the previous
 // iteration fused in: dn = fold ? z + beta*dold : dold on the group's rows,
 // dnew = dn, out = (dn - U out) * w / diag.  first: this group opens the
@@ -142,7 +145,7 @@ template <typename T>
 void launch_eisenstat_upper(hipStream_t s, SellViewT<T> P, int slice0, int nslices, int row0,
                             int nrows, SolveState* st, int first, int p, const T* z,
                             const T* dold, T* dnew, T* out, const T* diag, T omega,
-                            const double* fold_partials, int nfold);
+                            const double* fold_partials, int nfold, int gbase, int gw);
 
 // --- iteration ---
 // Opens an iteration of parity p for the SpMV-based variants: when
@@ -154,9 +157,8 @@ void launch_eisenstat_upper(hipStream_t s, SellViewT<T> P, int slice0, int nslic
 template <typename T>
 void launch_spmv_dir(hipStream_t s, SellViewT<T> A, int nslices, int row0, int nrows,
                      int n_active, SolveState* st, int p, const T* src, const T* dold, T* dnew,
-                     T* q,
-                     double* spmv_partials, const double* fold_partials, int nfold,
-                     const double* group_sums, int ngroups);
+                     T* q, double* spmv_partials, const double* fold_partials, int nfold,
+                     const double* group_sums, int ngroups, int gw);
 enum UpdateMode { UPD_PLAIN = 0, UPD_JACOBI = 1, UPD_EISENSTAT = 2 };
 // Forms d.Ad from dad_partials (ndad), breakdown tests, alpha; then
 // x += a*dx; r -= a*q; z.r per mode into zr_partials (one per block).

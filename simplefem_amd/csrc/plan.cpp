@@ -78,6 +78,7 @@ void append_group(SellPart& part, const std::vector<int>& rows_dev, const std::v
     for (int i = 1; i < count && consecutive; i++) consecutive = rows_dev[i] == rows_dev[0] + i;
     part.pending_row0 = (count > 0 && consecutive) ? rows_dev[0] : (count == 0 ? 0 : -1);
     const int nsl = (count + kSliceRows - 1) / kSliceRows;
+    part.pending_width = 0;
     if (nsl == 0) return;
     std::vector<int> width(nsl, 0);
 #pragma omp parallel for schedule(dynamic, 64)
@@ -91,6 +92,16 @@ void append_group(SellPart& part, const std::vector<int>& rows_dev, const std::v
             wmax = std::max(wmax, w);
         }
         width[sl] = wmax;
+    }
+    // One width for the whole group when that pads at most 10% more entries.
+    long long wsum = 0;
+    int wmax = 0;
+    for (int w : width) wsum += w, wmax = std::max(wmax, w);
+    if (((long long)wmax * nsl - wsum) * 10 <= wsum) {
+        for (int& w : width) w = wmax;
+        part.pending_width = wmax;
+    } else {
+        part.pending_width = -1;
     }
     const int first = part.nslices;
     for (int sl = 0; sl < nsl; sl++)
@@ -141,7 +152,9 @@ void close_group(SellPart& part)
     part.group_entries.push_back(e);
     part.group_rows.push_back(r);
     part.group_row0.push_back(part.pending_row0);
+    part.group_width.push_back(part.pending_width);
     part.pending_row0 = -1;
+    part.pending_width = -1;
     part.group_slice.push_back(part.nslices);
 }
 

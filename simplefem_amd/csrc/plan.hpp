@@ -37,7 +37,11 @@ struct SellPart {
     // first device row of each group whose rows are consecutive in slice
     // order (the kernels then compute row ids instead of loading them), else -1
     std::vector<int> group_row0;
+    // entries per lane of every slice of a group when they are all padded to
+    // the same width (the kernels then compute slice offsets), else -1
+    std::vector<int> group_width;
     int pending_row0 = -1;
+    int pending_width = -1;
     int zero_slot = 0;             // padding column = the vectors' zero slot
     long long entries = 0;         // real (unpadded) entries
     long long rows = 0;            // real rows

@@ -241,7 +241,7 @@ struct SystemT {
             run(s, SFA_KERNEL_FORWARD, L.bytes(g, per_row + (fuse ? 8.0 : 0.0)), [&] {
                 launch_forward(s, L.view(), L.group_begin(g), L.group_count(g), L.row0(g),
                                L.nrows_of(g), state, gm, p, x, r, dd, q, m, diag, w(), ps, nspmv,
-                               fuse ? z : (T*)nullptr, c2(), zpart);
+                               fuse ? z : (T*)nullptr, c2(), zpart, L.base(g), L.width(g));
             });
         }
     }
@@ -256,7 +256,8 @@ struct SystemT {
                                 upper.row0(g), upper.nrows_of(g), state, reduce, r, m, z, diag,
                                 w(), c2(), pb, fold_prev ? upper.group_begin(g - 1) : 0,
                                 fold_prev ? upper.group_count(g - 1) : 0,
-                                fold_prev ? grp + (g - 1) : (double*)nullptr);
+                                fold_prev ? grp + (g - 1) : (double*)nullptr, upper.base(g),
+                                upper.width(g));
             });
         }
     }
@@ -269,7 +270,7 @@ struct SystemT {
             run(s, SFA_KERNEL_TRIANGULAR, P.bytes(g, per_row), [&] {
                 launch_eisenstat_solve(s, P.view(), P.group_begin(g), P.group_count(g), P.row0(g),
                                        P.nrows_of(g), state, with_t, in, t, out, diag, w(),
-                                       c3());
+                                       c3(), P.base(g), P.width(g));
             });
     }
 
@@ -334,7 +335,7 @@ struct SystemT {
             run(s, SFA_KERNEL_SPMV, spmv_bytes, [&] {
                 launch_spmv_dir(s, F, full.nslices, full.row0(0), full.nrows_of(0), n_active,
                                 state, p, jac ? z : r, dold, dnew, q, ps, pu, nvec,
-                                (const double*)nullptr, 0);
+                                (const double*)nullptr, 0, full.width(0));
             });
             run(s, SFA_KERNEL_UPDATE, (jac ? 64.0 : 48.0) * na, [&] {
                 launch_update(s, jac ? UPD_JACOBI : UPD_PLAIN, state, p, n_active, x, r, dnew, q,
@@ -352,7 +353,8 @@ struct SystemT {
                 launch_spmv_dir(s, F, full.nslices, full.row0(0), full.nrows_of(0), n_active,
                                 state, p, z, dold, dnew, q, ps,
                                 G > 0 ? pb + upper.group_begin(G - 1) : pb,
-                                G > 0 ? upper.group_count(G - 1) : 0, grp, G > 0 ? G - 1 : 0);
+                                G > 0 ? upper.group_count(G - 1) : 0, grp, G > 0 ? G - 1 : 0,
+                                full.width(0));
             });
             forward_sweep(s, lower, FWD_UPDATE, p, dnew);
             backward_sweep(s, 1);
@@ -364,7 +366,7 @@ struct SystemT {
                     launch_eisenstat_upper(s, upper.view(), upper.group_begin(g),
                                            upper.group_count(g), upper.row0(g), upper.nrows_of(g),
                                            state, g == 0, p, z, dold, dnew, m, diag, w(), pu,
-                                           nvec);
+                                           nvec, upper.base(g), upper.width(g));
                 });
             eisenstat_sweep(s, lower, 1, dnew, m, q);
             run(s, SFA_KERNEL_COMBINE, 32.0 * na,
