@@ -106,6 +106,17 @@ def main():
     ap.add_argument("--chunk-steps", type=int, default=2000,
                     help="batched: engine steps per persistent launch between progress checks")
     ap.add_argument("--rate", type=float, default=20.0, help="served: arrivals per second")
+    ap.add_argument("--inner-rtol", type=float, default=1e-4,
+                    help="mixed precision: reduction of the preconditioned residual per FP32 "
+                         "inner solve")
+    ap.add_argument("--route-rows", type=int, default=100000,
+                    help="served: requests with at least this many rows get their own System")
+    ap.add_argument("--dedicated-workers", type=int, default=6,
+                    help="served: concurrent dedicated Systems (0: everything in the batch)")
+    ap.add_argument("--no-served-baseline", action="store_true",
+                    help="served: skip the plain FP64 pass over the same stream")
+    ap.add_argument("--prep-workers", type=int, default=8,
+                    help="served: host threads that colour and plan arriving matrices")
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="MIB",
                     help="only copy MIB MiB device to device 5 times (hipMemcpy), to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters of a kernel of known traffic")

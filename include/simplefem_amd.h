@@ -287,6 +287,19 @@ int sfa_batch_add_matrix(sfa_batch b, int N, const int* csrrows,
                          const double* diagonal, double omega, int num_colors,
                          const int* F, const int* B, const int* R, int* matrix_id);
 
+/* The host half of sfa_batch_add_matrix: builds the matrix's device-order
+ * plan (permutation, colour groups, SELL parts) without touching the device
+ * or any batch, so any thread can run it while a batch runs; the plan is then
+ * uploaded with sfa_batch_add_prepared (on the thread that drives the batch)
+ * and freed with sfa_prepared_destroy. */
+typedef struct sfa_prepared_s* sfa_prepared;
+int sfa_batch_prepare_matrix(int N, const int* csrrows, const int* csrcolumns,
+                             const double* csrvalues, const double* diagonal, double omega,
+                             int num_colors, const int* F, const int* B, const int* R,
+                             sfa_prepared* out);
+int sfa_batch_add_prepared(sfa_batch b, sfa_prepared prepared, int* matrix_id);
+int sfa_prepared_destroy(sfa_prepared prepared);
+
 /* Frees a matrix's device memory; no slot may still use it. */
 int sfa_batch_remove_matrix(sfa_batch b, int matrix_id);
 

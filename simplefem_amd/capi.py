@@ -62,6 +62,33 @@ FLAG_FP32_INNER = 0x100
 BATCH_FP64, BATCH_MIXED = 0, 1
 
 
+class PreparedMatrix:
+    """A matrix's device-order plan built on the host (sfa_batch_prepare_matrix):
+    no device or batch is touched, so any thread may build one (the call
+    releases the GIL) while another drives a batch; Batch.add_prepared uploads it."""
+
+    def __init__(self, N, A, C, V, diagonal, omega, num_colors, F, B, R):
+        A, C, V = _csr(A, C, V)
+        diagonal = _f64(diagonal, "diagonal", N)
+        F, B, R = _i32(F, "F"), _i32(B, "B"), _i32(R, "R")
+        self.N = N
+        self._h = c_void_p()
+        _check(load().sfa_batch_prepare_matrix(N, _i(A), _i(C), _d(V), _d(diagonal),
+                                               float(omega), int(num_colors), _i(F), _i(B),
+                                               _i(R), ctypes.byref(self._h)))
+
+    def close(self):
+        if self._h:
+            load().sfa_prepared_destroy(self._h)
+            self._h = c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class BatchInfo(ctypes.Structure):
     _fields_ = [("iterations", c_int), ("outer", c_int), ("success", c_int),
                 ("exit_reason", c_int), ("zr", c_double), ("residual_norm", c_double)]
@@ -114,6 +141,10 @@ _SIGNATURES = {
     "sfa_batch_add_matrix": ([c_void_p, c_int, _P_I, _P_I, _P_D, _P_D, c_double, c_int, _P_I,
                               _P_I, _P_I, POINTER(c_int)], c_int),
     "sfa_batch_remove_matrix": ([c_void_p, c_int], c_int),
+    "sfa_batch_prepare_matrix": ([c_int, _P_I, _P_I, _P_D, _P_D, c_double, c_int, _P_I, _P_I,
+                                  _P_I, POINTER(c_void_p)], c_int),
+    "sfa_batch_add_prepared": ([c_void_p, c_void_p, POINTER(c_int)], c_int),
+    "sfa_prepared_destroy": ([c_void_p], c_int),
     "sfa_batch_submit": ([c_void_p, c_int, _P_D, _P_D, c_double, c_double, c_int,
                           POINTER(c_int)], c_int),
     "sfa_batch_run": ([c_void_p, c_int, POINTER(c_int)], c_int),
@@ -418,6 +449,13 @@ class Batch:
                                            float(omega), int(num_colors), _i(F), _i(B), _i(R),
                                            ctypes.byref(mid)))
         self.n_of[mid.value] = N
+        return mid.value
+
+    def add_prepared(self, prepared):
+        """Uploads a PreparedMatrix; returns its matrix id."""
+        mid = c_int(-1)
+        _check(load().sfa_batch_add_prepared(self._h, prepared._h, ctypes.byref(mid)))
+        self.n_of[mid.value] = prepared.N
         return mid.value
 
     def remove_matrix(self, mid):

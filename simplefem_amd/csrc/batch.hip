@@ -40,6 +40,7 @@
 #include "device_common.hpp"
 #include "host_common.hpp"
 #include "kernels.hpp"
+#include "launch.hpp"
 #include "plan.hpp"
 
 namespace sfa {
@@ -610,6 +611,14 @@ struct Engine {
         std::string err = build_plan(plan, SFA_METHOD_RAINBOW, n, A, C, V, diagonal, omega, nc, F,
                                      B, R, &colouring_error);
         if (!err.empty()) throw Error{colouring_error ? SFA_ERR_COLOURING : SFA_ERR_INVALID, err};
+        return add_plan(plan);
+    }
+
+    // Uploads a plan built on the host and registers it; returns its id.
+    int add_plan(const Plan& plan)
+    {
+        const int n = plan.n;
+        const double omega = plan.omega;
         SFA_HIP(hipSetDevice(device));
         int id = -1;
         for (size_t i = 0; i < mats.size(); i++)
@@ -789,28 +798,25 @@ struct Engine {
         step = h_sched->step;
     }
 
-    // Captures kGraphSteps steps (k_batch_step, k_batch_fold, k_batch_prefix)
-    // into a graph; rebuilt when the device matrix table has moved.
+    // Records kGraphSteps steps (k_batch_step, k_batch_fold, k_batch_prefix)
+    // into a graph, node by node; rebuilt when the device matrix table has moved.
     void build_graph()
     {
         if (graph && graph_mats == d_mats) return;
         if (graph) SFA_HIP(hipGraphExecDestroy(graph));
         graph = nullptr;
-        hipGraph_t g = nullptr;
         const BArgs<T> a = args();
-        SFA_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        GraphRecorder rec;
         for (int i = 0; i < kGraphSteps; i++) {
-            k_batch_step<T><<<kStepBlocks, kUnitBlock, 0, stream>>>(a);
-            k_batch_fold<T><<<max_slots, kUnitBlock, 0, stream>>>(a);
-            k_batch_prefix<T><<<1, kPrefixBlock, 0, stream>>>(a);
+            launch(k_batch_step<T>, dim3(kStepBlocks), dim3(kUnitBlock), stream, a);
+            launch(k_batch_fold<T>, dim3(max_slots), dim3(kUnitBlock), stream, a);
+            launch(k_batch_prefix<T>, dim3(1), dim3(kPrefixBlock), stream, a);
         }
-        SFA_HIP(hipStreamEndCapture(stream, &g));
-        hipError_t e = hipGraphInstantiate(&graph, g, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(g);
-        if (e != hipSuccess) {
-            graph = nullptr;
-            throw Error{SFA_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)};
-        }
+        hipGraphExec_t exec = nullptr;
+        const hipError_t e = rec.instantiate(&exec);
+        if (e != hipSuccess)
+            throw Error{SFA_ERR_HIP, std::string("batch step graph: ") + hipGetErrorString(e)};
+        graph = exec;
         graph_mats = d_mats;
     }
 
@@ -916,6 +922,10 @@ struct sfa_batch_s {
     sfa::BatchHandle h;
 };
 
+struct sfa_prepared_s {
+    sfa::Plan plan;
+};
+
 extern "C" {
 
 int sfa_batch_create(int precision_mode, int max_slots, sfa_batch* out)
@@ -948,6 +958,38 @@ int sfa_batch_add_matrix(sfa_batch b, int N, const int* csrrows, const int* csrc
                                       num_colors, F, B, R);
         });
     });
+}
+
+int sfa_batch_prepare_matrix(int N, const int* csrrows, const int* csrcolumns,
+                             const double* csrvalues, const double* diagonal, double omega,
+                             int num_colors, const int* F, const int* B, const int* R,
+                             sfa_prepared* out)
+{
+    return sfa::guarded([&] {
+        if (!out) throw sfa::Error{SFA_ERR_INVALID, "null output handle"};
+        *out = nullptr;
+        std::unique_ptr<sfa_prepared_s> p(new sfa_prepared_s());
+        bool colouring_error = false;
+        std::string err = sfa::build_plan(p->plan, SFA_METHOD_RAINBOW, N, csrrows, csrcolumns,
+                                          csrvalues, diagonal, omega, num_colors, F, B, R,
+                                          &colouring_error);
+        if (!err.empty())
+            throw sfa::Error{colouring_error ? SFA_ERR_COLOURING : SFA_ERR_INVALID, err};
+        *out = p.release();
+    });
+}
+
+int sfa_batch_add_prepared(sfa_batch b, sfa_prepared prepared, int* matrix_id)
+{
+    return sfa::guarded([&] {
+        if (!b || !prepared || !matrix_id) throw sfa::Error{SFA_ERR_INVALID, "null argument"};
+        b->h.visit([&](auto& e) { *matrix_id = e.add_plan(prepared->plan); });
+    });
+}
+
+int sfa_prepared_destroy(sfa_prepared prepared)
+{
+    return sfa::guarded([&] { delete prepared; });
 }
 
 int sfa_batch_remove_matrix(sfa_batch b, int matrix_id)

@@ -21,8 +21,15 @@
 
 #include "../../include/simplefem_amd.h"
 #include "device_common.hpp"
+#include "launch.hpp"
 
 namespace sfa {
+
+GraphRecorder*& current_recorder()
+{
+    thread_local GraphRecorder* rec = nullptr;
+    return rec;
+}
 
 int spmv_blocks(int nslices)
 {
@@ -648,19 +655,19 @@ int flat_blocks(int n)
 template <typename T>
 void launch_fill(hipStream_t s, T* v, T value, int n)
 {
-    if (n > 0) k_fill<T><<<flat_blocks(n), kVecBlock, 0, s>>>(v, value, n);
+    if (n > 0) launch(k_fill<T>, dim3(flat_blocks(n)), dim3(kVecBlock), s, v, value, n);
 }
 
 template <typename T>
 void launch_gather(hipStream_t s, T* dst, const T* src, const int* perm, int n)
 {
-    if (n > 0) k_gather<T><<<flat_blocks(n), kVecBlock, 0, s>>>(dst, src, perm, n);
+    if (n > 0) launch(k_gather<T>, dim3(flat_blocks(n)), dim3(kVecBlock), s, dst, src, perm, n);
 }
 
 template <typename T>
 void launch_scatter(hipStream_t s, T* dst, const T* src, const int* perm, int n)
 {
-    if (n > 0) k_scatter<T><<<flat_blocks(n), kVecBlock, 0, s>>>(dst, src, perm, n);
+    if (n > 0) launch(k_scatter<T>, dim3(flat_blocks(n)), dim3(kVecBlock), s, dst, src, perm, n);
 }
 
 template <typename T>
@@ -669,8 +676,8 @@ void launch_init_state(hipStream_t s, SolveState* st, const T* b, int n, double 
                        double* partials)
 {
     const int grid = vec_reduce_blocks(n);
-    if (!use_host_tol) k_sumsq<T><<<grid, kVecBlock, 0, s>>>(b, n, partials);
-    k_init_state<<<1, kWave, 0, s>>>(st, partials, grid, precision, host_tol, use_host_tol, maxK,
+    if (!use_host_tol) launch(k_sumsq<T>, dim3(grid), dim3(kVecBlock), s, b, n, partials);
+    launch(k_init_state, dim3(1), dim3(kWave), s, st, partials, grid, precision, host_tol, use_host_tol, maxK,
                                      small_breaks, rel2);
 }
 
@@ -679,7 +686,7 @@ void launch_apply(hipStream_t s, SellViewT<T> A, int nslices, int row0, int nrow
                   T* dest, T scaling)
 {
     if (nslices > 0)
-        k_apply<T><<<nslices, kSellBlock, 0, s>>>(A, nslices, row0, nrows, src, dest, scaling);
+        launch(k_apply<T>, dim3(nslices), dim3(kSellBlock), s, A, nslices, row0, nrows, src, dest, scaling);
 }
 
 template <typename T>
@@ -688,9 +695,9 @@ void launch_residual(hipStream_t s, ResidualMode mode, SellViewT<T> A, int nslic
                      const T* precon, double* partials)
 {
     const int grid = nslices > 0 ? nslices : 1;
-    k_residual<T><<<grid, kSellBlock, 0, s>>>((int)mode, A, nslices, row0, nrows, b, x, out, d, z,
+    launch(k_residual<T>, dim3(grid), dim3(kSellBlock), s, (int)mode, A, nslices, row0, nrows, b, x, out, d, z,
                                               precon, partials);
-    if (mode != RES_ONLY) k_restart_finish<<<1, kWave, 0, s>>>(st, partials, grid);
+    if (mode != RES_ONLY) launch(k_restart_finish, dim3(1), dim3(kWave), s, st, partials, grid);
 }
 
 template <typename T>
@@ -698,8 +705,8 @@ void launch_residual_norm(hipStream_t s, SellViewT<T> A, int nslices, int row0, 
                           SolveState* st, const T* b, const T* x, T* out, double* partials)
 {
     const int grid = nslices > 0 ? nslices : 1;
-    k_residual_norm<T><<<grid, kSellBlock, 0, s>>>(A, nslices, row0, nrows, b, x, out, partials);
-    k_result_finish<<<1, kWave, 0, s>>>(st, partials, grid);
+    launch(k_residual_norm<T>, dim3(grid), dim3(kSellBlock), s, A, nslices, row0, nrows, b, x, out, partials);
+    launch(k_result_finish, dim3(1), dim3(kWave), s, st, partials, grid);
 }
 
 template <typename T>
@@ -707,8 +714,8 @@ void launch_direction_init(hipStream_t s, SolveState* st, int n_active, const T*
                            T* d, double* partials)
 {
     const int grid = vec_reduce_blocks(n_active);
-    k_direction_init<T><<<grid, kVecBlock, 0, s>>>(n_active, z, r, d, partials);
-    k_restart_finish<<<1, kWave, 0, s>>>(st, partials, grid);
+    launch(k_direction_init<T>, dim3(grid), dim3(kVecBlock), s, n_active, z, r, d, partials);
+    launch(k_restart_finish, dim3(1), dim3(kWave), s, st, partials, grid);
 }
 
 template <typename T>
@@ -716,8 +723,8 @@ void launch_eisenstat_init(hipStream_t s, SolveState* st, int n_active, const T*
                            const T* diag, T omega, T* z, T* d, double* partials)
 {
     const int grid = vec_reduce_blocks(n_active);
-    k_eisenstat_init<T><<<grid, kVecBlock, 0, s>>>(n_active, r, diag, omega, z, d, partials);
-    k_restart_finish<<<1, kWave, 0, s>>>(st, partials, grid);
+    launch(k_eisenstat_init<T>, dim3(grid), dim3(kVecBlock), s, n_active, r, diag, omega, z, d, partials);
+    launch(k_restart_finish, dim3(1), dim3(kWave), s, st, partials, grid);
 }
 
 template <typename T>
@@ -730,7 +737,7 @@ void launch_forward(hipStream_t s, SellViewT<T> L, int slice0, int nslices, int 
     // forms alpha for the groups after it.
     const int grid = nslices > 0 ? nslices : (mode == FWD_FIRST ? 1 : 0);
     if (grid == 0) return;
-    k_forward<T><<<grid, kSellBlock, 0, s>>>(L, slice0, nslices, row0, nrows, st, mode, p, x, r, d,
+    launch(k_forward<T>, dim3(grid), dim3(kSellBlock), s, L, slice0, nslices, row0, nrows, st, mode, p, x, r, d,
                                              q, m, diag, omega, spmv_partials, nspmv, zb, c2,
                                              zpart, gbase, gw);
 }
@@ -744,7 +751,7 @@ void launch_backward(hipStream_t s, SellViewT<T> U, int slice0, int nslices, int
     // one more block sums the previous group's partials
     const int grid = nslices + (reduce && prev_sum ? 1 : 0);
     if (grid == 0) return;
-    k_backward<T><<<grid, kSellBlock, 0, s>>>(U, slice0, nslices, row0, nrows, st, reduce, r, m, z,
+    launch(k_backward<T>, dim3(grid), dim3(kSellBlock), s, U, slice0, nslices, row0, nrows, st, reduce, r, m, z,
                                               diag, omega, c2, group_partials, prev0, prevn,
                                               prev_sum, gbase, gw);
 }
@@ -755,7 +762,7 @@ void launch_eisenstat_solve(hipStream_t s, SellViewT<T> P, int slice0, int nslic
                             T* out, const T* diag, T omega, T c3, int gbase, int gw)
 {
     if (nslices <= 0) return;
-    k_eisenstat_solve<T><<<nslices, kSellBlock, 0, s>>>(P, slice0, nslices, row0, nrows, st,
+    launch(k_eisenstat_solve<T>, dim3(nslices), dim3(kSellBlock), s, P, slice0, nslices, row0, nrows, st,
                                                         with_t, in, t, out, diag, omega, c3,
                                                         gbase, gw);
 }
@@ -768,7 +775,7 @@ void launch_eisenstat_upper(hipStream_t s, SellViewT<T> P, int slice0, int nslic
 {
     const int grid = nslices > 0 ? nslices : (first ? 1 : 0);
     if (grid == 0) return;
-    k_eisenstat_upper<T><<<grid, kSellBlock, 0, s>>>(P, slice0, nslices, row0, nrows, st, first, p,
+    launch(k_eisenstat_upper<T>, dim3(grid), dim3(kSellBlock), s, P, slice0, nslices, row0, nrows, st, first, p,
                                                      z, dold, dnew, out, diag, omega,
                                                      fold_partials, nfold, gbase, gw);
 }
@@ -779,7 +786,7 @@ void launch_spmv_dir(hipStream_t s, SellViewT<T> A, int nslices, int row0, int n
                      T* q, double* spmv_partials, const double* fold_partials, int nfold,
                      const double* group_sums, int ngroups, int gw)
 {
-    k_spmv_dir<T><<<spmv_blocks(nslices), kSpmvBlock, 0, s>>>(A, nslices, row0, nrows, n_active,
+    launch(k_spmv_dir<T>, dim3(spmv_blocks(nslices)), dim3(kSpmvBlock), s, A, nslices, row0, nrows, n_active,
                                                               st, p, src,
                                                               dold, dnew, q, spmv_partials,
                                                               fold_partials, nfold, group_sums,
@@ -791,7 +798,7 @@ void launch_update(hipStream_t s, UpdateMode mode, SolveState* st, int p, int n_
                    T* r, const T* dx, const T* q, T* z, const T* diag_or_precon, T omega,
                    const double* dad_partials, int ndad, double* zr_partials)
 {
-    k_update<T><<<vec_reduce_blocks(n_active), kVecBlock, 0, s>>>(
+    launch(k_update<T>, dim3(vec_reduce_blocks(n_active)), dim3(kVecBlock), s, 
         (int)mode, st, p, n_active, x, r, dx, q, z, diag_or_precon, omega, dad_partials, ndad,
         zr_partials);
 }
@@ -800,18 +807,18 @@ template <typename T>
 void launch_eisenstat_combine(hipStream_t s, SolveState* st, int n_active, T* q, const T* t,
                               const T* d, double* partials)
 {
-    k_eisenstat_combine<T><<<vec_reduce_blocks(n_active), kVecBlock, 0, s>>>(st, n_active, q, t, d,
+    launch(k_eisenstat_combine<T>, dim3(vec_reduce_blocks(n_active)), dim3(kVecBlock), s, st, n_active, q, t, d,
                                                                             partials);
 }
 
 void launch_to_float(hipStream_t s, float* dst, const double* src, int n)
 {
-    if (n > 0) k_convert<float, double><<<flat_blocks(n), kVecBlock, 0, s>>>(dst, src, n);
+    if (n > 0) launch(k_convert<float, double>, dim3(flat_blocks(n)), dim3(kVecBlock), s, dst, src, n);
 }
 
 void launch_add_correction(hipStream_t s, double* x, const float* e, int n)
 {
-    if (n > 0) k_add_correction<<<flat_blocks(n), kVecBlock, 0, s>>>(x, e, n);
+    if (n > 0) launch(k_add_correction, dim3(flat_blocks(n)), dim3(kVecBlock), s, x, e, n);
 }
 
 #define SFA_INSTANTIATE(T)                                                                     \

@@ -19,6 +19,7 @@
 #include "../../include/simplefem_amd.h"
 #include "host_common.hpp"
 #include "kernels.hpp"
+#include "launch.hpp"
 #include "plan.hpp"
 
 namespace sfa {
@@ -398,13 +399,15 @@ struct SystemT {
             SFA_HIP(hipGraphExecDestroy(chunk_exec));
             chunk_exec = nullptr;
         }
-        hipGraph_t graph = nullptr;
-        SFA_HIP(hipStreamBeginCapture(own_stream, hipStreamCaptureModeThreadLocal));
+        // Recorded node by node (no stream capture), so other threads may use
+        // the runtime meanwhile.
+        GraphRecorder rec;
         for (int i = 0; i < iters; i++) iteration(own_stream, i & 1);
-        SFA_HIP(hipStreamEndCapture(own_stream, &graph));
-        hipError_t e = hipGraphInstantiate(&chunk_exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) throw Error{SFA_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e)};
+        const hipError_t e = rec.instantiate(&chunk_exec);
+        if (e != hipSuccess) {
+            chunk_exec = nullptr;
+            throw Error{SFA_ERR_HIP, std::string("iteration graph: ") + hipGetErrorString(e)};
+        }
         chunk_iters = iters;
     }
 
@@ -560,7 +563,9 @@ struct SystemT {
         int want = 2 * G;
         if (look >= 2 && zr > 0) {
             double hk[2];
-            SFA_HIP(hipMemcpy(&hk[0], hist + (K - look), sizeof(double), hipMemcpyDeviceToHost));
+            SFA_HIP(hipMemcpyAsync(&hk[0], hist + (K - look), sizeof(double),
+                                   hipMemcpyDeviceToHost, stream));
+            SFA_HIP(hipStreamSynchronize(stream));
             hk[1] = zr;
             if (hk[0] > zr && hk[0] > 0) {
                 const double rate = std::log(zr / hk[0]) / look;  // < 0
@@ -579,8 +584,11 @@ struct SystemT {
         const int K = st.K;
         if (pm >= -1) std::printf("(%d/%d)     BEGIN: %s\n", 0, n, method_title(method));
         std::vector<double> h(K + 1);
-        if (pm >= 0 && K + 1 > 0)
-            SFA_HIP(hipMemcpy(h.data(), hist, (K + 1) * sizeof(double), hipMemcpyDeviceToHost));
+        if (pm >= 0 && K + 1 > 0) {
+            SFA_HIP(hipMemcpyAsync(h.data(), hist, (K + 1) * sizeof(double),
+                                   hipMemcpyDeviceToHost, stream));
+            SFA_HIP(hipStreamSynchronize(stream));
+        }
         if (pm >= 0)
             std::printf("(%d/%d)   RESTART: Residual norm is %.9le < %.9le\n", 0, n, std::sqrt(h[0]), st.tol);
         if (pm >= 1) {

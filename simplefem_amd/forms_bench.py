@@ -65,7 +65,7 @@ def hcurl_mixed(args):
 
     def mixed():
         x.zero()
-        return sysd.solve_mixed(x, bt, 1e-10, inner_rtol=1e-4, max_outer=30)
+        return sysd.solve_mixed(x, bt, 1e-10, inner_rtol=args.inner_rtol, max_outer=30)
 
     fp64()
     synchronize()
@@ -120,7 +120,7 @@ def batched(args):
             slots = []
             for j, mid in zip(jobs, mids):
                 for b in j.rhs:
-                    slots.append((eng.submit(mid, b, precision=1e-10, inner_rtol=1e-4), j, b))
+                    slots.append((eng.submit(mid, b, precision=1e-10, inner_rtol=args.inner_rtol), j, b))
             t_sub = time.perf_counter() - t1
             last = t1
             while True:
@@ -189,61 +189,141 @@ def batched(args):
                  "f32+f64")
 
 
+def _prepare(job):
+    """Colouring and device-order plan of a request's matrix (host only)."""
+    s = job.system
+    rb = capi.Rainbow(s.n, s.A, s.C)
+    return capi.PreparedMatrix(s.n, s.A, s.C, s.V, s.diagonal(), 1.0, rb.num_colors, rb.F, rb.B,
+                               rb.R)
+
+
+def _served_fp64_one_at_a_time(jobs):
+    """The plain FP64 path on the same stream: requests in arrival order, one
+    device-resident System each (colouring, plan, upload, solve)."""
+    lat = []
+    start = time.perf_counter()
+    last = start
+    for k, j in enumerate(jobs):
+        now = time.perf_counter() - start
+        if j.arrival > now:
+            time.sleep(j.arrival - now)
+        s = j.system
+        rb = capi.Rainbow(s.n, s.A, s.C)
+        with capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, s.diagonal(), 1.0,
+                         rb.num_colors, rb.F, rb.B, rb.R) as sysd:
+            bt = DeviceArray.from_host(j.rhs[0])
+            x = DeviceArray(s.n)
+            r = DeviceArray(s.n)
+            sysd.solve(x, bt, r, 1e-10)
+            synchronize()
+        lat.append(time.perf_counter() - start - j.arrival)
+        if time.perf_counter() - last > 5.0:
+            last = time.perf_counter()
+            log(f"fp64 one at a time: {k + 1} of {len(jobs)} after {last - start:.1f}s")
+    return time.perf_counter() - start, np.array(lat)
+
+
+def _solve_dedicated(job, inner_rtol, start):
+    """A large request on its own device-resident System and stream: colouring,
+    plan, upload and the mixed-precision solve, on a worker thread while the
+    batch engine runs.  Returns the completion time (seconds after start)."""
+    s = job.system
+    rb = capi.Rainbow(s.n, s.A, s.C)
+    with capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, s.diagonal(), 1.0, rb.num_colors,
+                     rb.F, rb.B, rb.R, fp32_inner=True) as sysd:
+        bt = DeviceArray.from_host(job.rhs[0])
+        x = DeviceArray(s.n)
+        sysd.solve_mixed(x, bt, 1e-10, inner_rtol=inner_rtol, max_outer=30)  # returns when done
+        bt.free()
+        x.free()
+    return time.perf_counter() - start
+
+
 def served(args, budget=200e9):
+    from concurrent.futures import ThreadPoolExecutor
+
     t0 = time.time()
+
     def progress(k, job):
         if k % 8 == 7:
             log(f"generated {k + 1} requests ({time.time() - t0:.1f}s, last n={job.system.n})")
 
     jobs = W.served_requests(args.requests, args.rate, dofs=(1e3, 2e6), progress=progress)
     log(f"{len(jobs)} requests generated ({time.time() - t0:.1f}s)")
-    cols = {}
-    for j in jobs:
-        key = id(j.system)
-        if key not in cols:
-            cols[key] = capi.Rainbow(j.system.n, j.system.A, j.system.C)
     lat = []
-    with capi.Batch(mixed=True, max_slots=256) as eng:
-        pending = list(jobs)
+    dedicated = []  # (job, future) of requests routed to their own System
+    with capi.Batch(mixed=True, max_slots=256) as eng, \
+            ThreadPoolExecutor(max_workers=args.prep_workers) as pool, \
+            ThreadPoolExecutor(max_workers=max(1, args.dedicated_workers)) as big_pool:
+        arrivals = list(jobs)  # not yet arrived
+        preparing = []         # (job, future), arrival order
         inflight = {}
         start = time.perf_counter()
         last_log = 0.0
         done = 0
-        while pending or inflight:
+        while arrivals or preparing or inflight:
             now = time.perf_counter() - start
-            while pending and pending[0].arrival <= now:
-                j = pending[0]
+            while arrivals and arrivals[0].arrival <= now:  # a request is seen once it arrives
+                j = arrivals.pop(0)
+                if args.dedicated_workers > 0 and j.system.n >= args.route_rows:
+                    dedicated.append((j, big_pool.submit(_solve_dedicated, j, args.inner_rtol,
+                                                         start)))
+                else:
+                    preparing.append((j, pool.submit(_prepare, j)))
+            while preparing and preparing[0][1].done():  # admit in arrival order, within budget
+                j, fut = preparing[0]
                 s = j.system
                 need = 40.0 * s.nnz + 80.0 * s.n
                 if inflight and eng.memory() + need > budget:
                     break
-                rb = cols[id(s)]
-                mid = eng.add_matrix(s.n, s.A, s.C, s.V, s.diagonal(), 1.0, rb.num_colors, rb.F,
-                                     rb.B, rb.R)
-                slot = eng.submit(mid, j.rhs[0], precision=1e-10, inner_rtol=1e-4)
+                prep = fut.result()
+                mid = eng.add_prepared(prep)
+                prep.close()
+                slot = eng.submit(mid, j.rhs[0], precision=1e-10, inner_rtol=args.inner_rtol)
                 inflight[slot] = (j, mid)
-                pending.pop(0)
+                preparing.pop(0)
             if inflight:
-                eng.run(400)
+                eng.run(128)
                 if time.perf_counter() - start - last_log > 5.0:
                     last_log = time.perf_counter() - start
-                    log(f"served {done} of {len(jobs)} after {last_log:.1f}s, {len(inflight)} in flight")
-            elif pending:
-                time.sleep(max(0.0, pending[0].arrival - (time.perf_counter() - start)))
+                    log(f"served {done} of {len(jobs)} after {last_log:.1f}s, "
+                        f"{len(inflight)} in flight, {len(preparing)} being prepared")
+            elif preparing:
+                time.sleep(0.0005)
+            elif arrivals:
+                time.sleep(max(0.0, arrivals[0].arrival - (time.perf_counter() - start)))
             for slot in eng.finished():
                 j, mid = inflight.pop(slot)
                 x, info = eng.collect(slot, j.system.n)
                 lat.append(time.perf_counter() - start - j.arrival)
                 eng.remove_matrix(mid)
                 done += 1
+        for j, fut in dedicated:  # the batch is drained; wait for the large requests
+            lat.append(fut.result() - j.arrival)
+            done += 1
         total = time.perf_counter() - start
     lat = np.array(lat)
+    log(f"served {done} in {total:.2f}s")
+    if args.no_served_baseline:
+        total64, lat64 = float("nan"), np.array([np.nan])
+    else:
+        log("plain FP64 path on the same stream ...")
+        total64, lat64 = _served_fp64_one_at_a_time(jobs)
     cfg = {"workload": f"{len(jobs)} requests, Poisson arrivals at {args.rate}/s, 2D/3D, "
                        "log-uniform 1e3..2e6 unknowns, Lagrange degree 1..3, one RHS each, "
-                       "mixed precision to rtol 1e-10, 200 GB HBM admission budget",
+                       "mixed precision to rtol 1e-10, 200 GB HBM admission budget; colouring "
+                       "and plan built on host worker threads after arrival; requests of at "
+                       f"least {args.route_rows} rows on their own System and stream",
            "requests": len(jobs), "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
            "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
-           "makespan_s": total}
+           "makespan_s": total, "prep_workers": args.prep_workers,
+           "dedicated": {"rows_at_least": args.route_rows, "workers": args.dedicated_workers,
+                         "requests": len(dedicated)},
+           "fp64_one_at_a_time": {"solves_per_s": len(jobs) / total64,
+                                  "p50_latency_ms": float(np.percentile(lat64, 50) * 1e3),
+                                  "p99_latency_ms": float(np.percentile(lat64, 99) * 1e3),
+                                  "makespan_s": total64},
+           "speedup_over_fp64_one_at_a_time": (done / total) / (len(jobs) / total64)}
     return _base(args, round(done / total, 3), "solves/s", round(total / done * 1e3, 3), cfg,
                  "f32+f64")
 

@@ -123,3 +123,31 @@ def test_batch_mixed_precision_reaches_fp64_residual():
             target = max(1e-10 * np.linalg.norm(b), 2.0 ** -26)
             assert np.linalg.norm(res) <= 1.0000001 * target, (s.name, info)
             assert info["outer"] >= 1
+
+
+def test_prepared_matrix_from_another_thread():
+    """A plan built on a worker thread (sfa_batch_prepare_matrix) while the
+    batch runs gives the same solve, bit for bit, as sfa_batch_add_matrix."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    systems = _systems()[:4]
+    cols = [oracle.rainbow(s.n, s.A, s.C) for s in systems]
+    with capi.Batch(max_slots=16) as batch, ThreadPoolExecutor(2) as pool:
+        futs = [pool.submit(capi.PreparedMatrix, s.n, s.A, s.C, s.V, s.diagonal(), 1.0, *c)
+                for s, c in zip(systems, cols)]
+        direct = []
+        for k, s in enumerate(systems):  # the batch runs while the plans are built
+            mid, _ = _add(batch, s)
+            direct.append(batch.submit(mid, S.rhs_uniform(s, 40 + k)))
+            batch.run(32)
+        prepared = []
+        for k, (s, f) in enumerate(zip(systems, futs)):
+            p = f.result()
+            mid = batch.add_prepared(p)
+            p.close()
+            prepared.append(batch.submit(mid, S.rhs_uniform(s, 40 + k)))
+        _run_all(batch, direct + prepared)
+        for s, a, b in zip(systems, direct, prepared):
+            xa, ia = batch.collect(a, s.n)
+            xb, ib = batch.collect(b, s.n)
+            assert np.array_equal(xa, xb) and ia["iterations"] == ib["iterations"], s.name

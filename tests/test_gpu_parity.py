@@ -277,6 +277,17 @@ def test_full_size_3d_rainbow():
     print("[full] K oracle", io["iterations"], "K gpu", ig["iterations"], flush=True)
 
 
+@pytest.mark.parametrize("omega", [1.0, 1.2])
+def test_full_size_2d_ssor(omega):
+    """The 129^2 P1 Poisson system (16,129 active rows) with natural-order SSOR,
+    the reference's own CPU configuration: ~250 dependency levels per sweep."""
+    system = S.poisson2d_p1(7)
+    b = S.rhs_uniform(system, seed=2026)
+    io, ig = _check_parity("ssor", system, b, omega=omega)
+    print("[full 2d] omega", omega, "K oracle", io["iterations"], "K gpu", ig["iterations"],
+          flush=True)
+
+
 def test_cpp_dropin_callers():
     """A C++ caller of the drop-in headers, in the shape of the reference's
     tests/solver/solvespd_csr.cpp: every variant converges on the GPU."""
