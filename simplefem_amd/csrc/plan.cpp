@@ -285,8 +285,15 @@ std::string build_plan(Plan& plan, int method, int n, const int* A, const int* C
     auto guarded = [&](int k) { return aux[C[k]] != 0.; };
 
     // ---- full part: SpMV of the iteration / residual of the restart ----
-    std::vector<int> active_dev(plan.n_active);
-    for (int p = 0; p < plan.n_active; p++) active_dev[p] = p;
+    // Its rows are independent, so they are listed in original order, not in
+    // device order: a range of slices then covers a piece of the mesh, whose
+    // neighbours lie in a few short runs of each colour's rows (device order
+    // groups rows by colour), and a GPU die that runs a contiguous range of
+    // slices gathers them from its own cache instead of from all of each vector.
+    std::vector<int> active_dev;
+    active_dev.reserve(plan.n_active);
+    for (int r = 0; r < n; r++)
+        if (active[r]) active_dev.push_back(iperm[r]);
     start_part(plan.full);
     if (eisen)  // solver/sparsesolver.cpp:806-807,1420-1421 skip columns with diagonal 0
         append_group(plan.full, active_dev, plan.perm, iperm, n, A, C, V,
