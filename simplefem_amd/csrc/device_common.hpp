@@ -50,8 +50,19 @@ __device__ inline void block_partial(double v, double* out)
 __device__ inline double wave_fold(const double* v, int count)
 {
     const int lane = threadIdx.x % kWave;
+    constexpr int B = 16;  // loads in flight per lane before they are added
     double a = 0.;
-    for (int i = lane; i < count; i += kWave) a += v[i];
+    for (int i0 = lane; i0 < count; i0 += B * kWave) {
+        double t[B];
+#pragma unroll
+        for (int u = 0; u < B; u++) {
+            const int i = i0 + u * kWave;
+            t[u] = i < count ? v[i] : 0.;
+        }
+#pragma unroll
+        for (int u = 0; u < B; u++)
+            if (i0 + u * kWave < count) a += t[u];
+    }
     return wave_sum(a);
 }
 

@@ -308,6 +308,11 @@ __global__ void __launch_bounds__(kVecBlock) k_eisenstat_init(int n, const T* r,
 // first group of the backward sweep and has no upper entries (the last colour
 // of a Rainbow schedule): its backward rows z = (m*diag*c2)*w/diag (:667-692)
 // are formed here, with their z.r partials at zpart[block] when zpart is set.
+// With m0_nrows > 0 the kernel also runs the sweep's first group, rows m0_row0
+// .. m0_row0+m0_nrows-1, which has no lower entries: extra blocks write its m
+// = (r - a q) * w / diag (its x and r are updated later, by its backward
+// kernel), and this group's rows form those m values themselves, with the
+// same arithmetic, where they gather them.
 template <typename T>
 __global__ void __launch_bounds__(kSellBlock) k_forward(SellViewT<T> L, int slice0, int nslices,
                                                         int row0, int nrows, SolveState* st,
@@ -316,13 +321,18 @@ __global__ void __launch_bounds__(kSellBlock) k_forward(SellViewT<T> L, int slic
                                                         const T* diag, T omega,
                                                         const double* spmv_partials, int nspmv,
                                                         T* zb, T c2, double* zpart, int gbase,
-                                                        int gw)
+                                                        int gw, int m0_row0, int m0_nrows)
 {
     const int lane = threadIdx.x;
     const int b = xcd_block(blockIdx.x, gridDim.x);
     const int slice = slice0 + b;
     const bool has = b < nslices;
-    const int row = has ? group_row(L, slice, b, lane, row0, nrows) : -1;
+    const bool first_group = b >= nslices && m0_nrows > 0;  // a block of the merged group
+    int row = has ? group_row(L, slice, b, lane, row0, nrows) : -1;
+    if (first_group) {
+        const int i = (b - nslices) * kWave + lane;
+        row = i < m0_nrows ? m0_row0 + i : -1;
+    }
     const int ri = row >= 0 ? row : L.zero;  // padding lanes read the zero slot
     T rr = r[ri];
     const T dg = diag[ri];
@@ -344,6 +354,18 @@ __global__ void __launch_bounds__(kSellBlock) k_forward(SellViewT<T> L, int slic
     } else if (mode == FWD_UPDATE) {
         alpha = (T)st->alpha;
     }
+    if (first_group) {  // no lower entries: m = (r - a q) * w / diag
+        if (row >= 0) m[row] = (mode != FWD_RESTART ? rr - alpha * qq : rr) * omega / dg;
+        return;
+    }
+    // m of a row of the merged first group, formed as its own block forms it
+    auto get_m = [&](unsigned c) -> T {
+        if (c - (unsigned)m0_row0 < (unsigned)m0_nrows) {
+            const T rc = r[c];
+            return (mode != FWD_RESTART ? rc - alpha * q[c] : rc) * omega / diag[c];
+        }
+        return m[c];
+    };
     double part = 0.;
     if (row >= 0) {
         if (mode != FWD_RESTART) {
@@ -351,7 +373,7 @@ __global__ void __launch_bounds__(kSellBlock) k_forward(SellViewT<T> L, int slic
             rr = rr - alpha * qq;
             r[row] = rr;
         }
-        const T acc = sell_finish<true>(L, R, [&](unsigned c) { return m[c]; }, rr);
+        const T acc = sell_finish<true>(L, R, get_m, rr);
         const T mm = acc * omega / dg;
         m[row] = mm;
         if (zb) {
@@ -371,13 +393,18 @@ __global__ void __launch_bounds__(kSellBlock) k_forward(SellViewT<T> L, int slic
 // With reduce, the z.r partial of each block goes to gpart[slice], and one
 // extra block (logical block nslices) sums the previous group's partials into
 // *prev_sum, so z.r is summed per group and over the groups in launch order.
+// With defer_update the group's x and r updates of the iteration (x += a d,
+// r -= a q) are made here, first (its forward rows were run merged into the
+// next group's kernel, which leaves them).
 template <typename T>
 __global__ void __launch_bounds__(kSellBlock) k_backward(SellViewT<T> U, int slice0, int nslices,
                                                          int row0, int nrows, SolveState* st,
-                                                         int reduce, const T* r, const T* m,
+                                                         int reduce, T* r, const T* m,
                                                          T* z, const T* diag, T omega, T c2,
                                                          double* gpart, int prev0, int prevn,
-                                                         double* prev_sum, int gbase, int gw)
+                                                         double* prev_sum, int gbase, int gw,
+                                                         int defer_update, T* x, const T* d,
+                                                         const T* q)
 {
     const int lane = threadIdx.x;
     const int b = xcd_block(blockIdx.x, gridDim.x);
@@ -387,10 +414,22 @@ __global__ void __launch_bounds__(kSellBlock) k_backward(SellViewT<T> U, int sli
     const int ri = row >= 0 ? row : U.zero;  // padding lanes read the zero slot
     const T dg = diag[ri];
     const T mm = m[ri];
-    const T rr = reduce ? r[ri] : T(0);
+    T rr = reduce ? r[ri] : T(0);
+    T xx = T(0), dd = T(0), qq = T(0);
+    if (defer_update) {
+        xx = x[ri];
+        dd = d[ri];
+        qq = q[ri];
+    }
     SellChunk<T> R;
     if (has) sell_prefetch(U, slice, b, gbase, gw, lane, R);
     if (st->done) return;
+    if (defer_update && row >= 0) {
+        const T alpha = (T)st->alpha;
+        x[row] = xx + alpha * dd;
+        rr = rr - alpha * qq;
+        r[row] = rr;
+    }
     if (reduce && prev_sum && b == nslices) {
         const double t = wave_fold(gpart + prev0, prevn);
         if (lane == 0) *prev_sum = t;
@@ -731,29 +770,32 @@ template <typename T>
 void launch_forward(hipStream_t s, SellViewT<T> L, int slice0, int nslices, int row0, int nrows,
                     SolveState* st, int mode, int p, T* x, T* r, const T* d, const T* q, T* m,
                     const T* diag, T omega, const double* spmv_partials, int nspmv, T* zb, T c2,
-                    double* zpart, int gbase, int gw)
+                    double* zpart, int gbase, int gw, int m0_row0, int m0_nrows)
 {
     // The first group of an iteration runs one block even when empty: it
     // forms alpha for the groups after it.
-    const int grid = nslices > 0 ? nslices : (mode == FWD_FIRST ? 1 : 0);
+    const int extra = (m0_nrows + kWave - 1) / kWave;  // blocks of a merged first group
+    int grid = nslices + extra;
+    if (grid == 0 && mode == FWD_FIRST) grid = 1;
     if (grid == 0) return;
     launch(k_forward<T>, dim3(grid), dim3(kSellBlock), s, L, slice0, nslices, row0, nrows, st, mode, p, x, r, d,
                                              q, m, diag, omega, spmv_partials, nspmv, zb, c2,
-                                             zpart, gbase, gw);
+                                             zpart, gbase, gw, m0_row0, m0_nrows);
 }
 
 template <typename T>
 void launch_backward(hipStream_t s, SellViewT<T> U, int slice0, int nslices, int row0, int nrows,
-                     SolveState* st, int reduce, const T* r, const T* m, T* z, const T* diag,
+                     SolveState* st, int reduce, T* r, const T* m, T* z, const T* diag,
                      T omega, T c2, double* group_partials, int prev0, int prevn,
-                     double* prev_sum, int gbase, int gw)
+                     double* prev_sum, int gbase, int gw, int defer_update, T* x, const T* d,
+                     const T* q)
 {
     // one more block sums the previous group's partials
     const int grid = nslices + (reduce && prev_sum ? 1 : 0);
     if (grid == 0) return;
     launch(k_backward<T>, dim3(grid), dim3(kSellBlock), s, U, slice0, nslices, row0, nrows, st, reduce, r, m, z,
                                               diag, omega, c2, group_partials, prev0, prevn,
-                                              prev_sum, gbase, gw);
+                                              prev_sum, gbase, gw, defer_update, x, d, q);
 }
 
 template <typename T>
@@ -839,10 +881,11 @@ void launch_add_correction(hipStream_t s, double* x, const float* e, int n)
                                            T, T*, T*, double*);                                \
     template void launch_forward<T>(hipStream_t, SellViewT<T>, int, int, int, int, SolveState*, \
                                     int, int, T*, T*, const T*, const T*, T*, const T*, T,     \
-                                    const double*, int, T*, T, double*, int, int);             \
+                                    const double*, int, T*, T, double*, int, int, int, int);   \
     template void launch_backward<T>(hipStream_t, SellViewT<T>, int, int, int, int,            \
-                                     SolveState*, int, const T*, const T*, T*, const T*, T, T, \
-                                     double*, int, int, double*, int, int);                    \
+                                     SolveState*, int, T*, const T*, T*, const T*, T, T,       \
+                                     double*, int, int, double*, int, int, int, T*, const T*,  \
+                                     const T*);                                                \
     template void launch_eisenstat_solve<T>(hipStream_t, SellViewT<T>, int, int, int, int,     \
                                             SolveState*, int, const T*, const T*, T*,          \
                                             const T*, T, T, int, int);                         \

@@ -116,21 +116,24 @@ void launch_eisenstat_init(hipStream_t s, SolveState* st, int n_active, const T*
 //                   and computes a (first group of the iteration, parity p).
 // zb != null: the group is also the backward sweep's first group and has no
 // upper entries; its z = (m*diag*c2)*w/diag goes to zb, with z.r partials at
-// zpart[block] when zpart != null.
+// zpart[block] when zpart != null.  m0_nrows > 0: the kernel also runs the
+// sweep's first group (rows m0_row0.., no lower entries), writing only its m;
+// that group's x and r updates are left to its backward kernel (defer_update).
 enum ForwardMode { FWD_RESTART = 0, FWD_UPDATE = 1, FWD_FIRST = 2 };
 template <typename T>
 void launch_forward(hipStream_t s, SellViewT<T> L, int slice0, int nslices, int row0, int nrows,
                     SolveState* st, int mode, int p, T* x, T* r, const T* d, const T* q, T* m,
                     const T* diag, T omega, const double* spmv_partials, int nspmv, T* zb, T c2,
-                    double* zpart, int gbase, int gw);
+                    double* zpart, int gbase, int gw, int m0_row0, int m0_nrows);
 // backward: t = m*diag*c2; z = (t - U z) * w / diag.  With reduce, each
 // block stores its z.r partial at group_partials[slice0 + block], and block 0
 // sums the previous group's partials (prev0, prevn) into *prev_sum.
 template <typename T>
 void launch_backward(hipStream_t s, SellViewT<T> U, int slice0, int nslices, int row0, int nrows,
-                     SolveState* st, int reduce, const T* r, const T* m, T* z, const T* diag,
+                     SolveState* st, int reduce, T* r, const T* m, T* z, const T* diag,
                      T omega, T c2, double* group_partials, int prev0, int prevn,
-                     double* prev_sum, int gbase, int gw);
+                     double* prev_sum, int gbase, int gw, int defer_update, T* x, const T* d,
+                     const T* q);
 // Eisenstat: out = (in - [c3*diag*t] - P out) * w / diag  (P = L or U part)
 template <typename T>
 void launch_eisenstat_solve(hipStream_t s, SellViewT<T> P, int slice0, int nslices, int row0,

@@ -58,6 +58,7 @@ struct SystemT {
     bool sweeps = false;
     bool rainbow = false;
     bool fuse_first_backward = false;  // see forward_sweep
+    bool merge_first_forward = false;  // see forward_sweep
 
     DevPartT<T> full, lower, upper, lower_restart;
     bool has_lower_restart = false;
@@ -185,6 +186,14 @@ struct SystemT {
                                   upper.nrows_of(0) == lower.nrows_of(gl) &&
                                   upper.group_count(0) == lower.group_count(gl) &&
                                   upper.group_entries[0] == 0;
+            // The first forward group (no lower entries) runs inside the second
+            // one's kernel; its x, r updates move to its own backward kernel,
+            // the last one, which must cover the same rows.
+            const int gu = upper.ngroups() - 1;
+            merge_first_forward = rainbow && lower.ngroups() >= 3 && lower.group_entries[0] == 0 &&
+                                  lower.row0(0) >= 0 && upper.row0(gu) == lower.row0(0) &&
+                                  upper.nrows_of(gu) == lower.nrows_of(0) &&
+                                  upper.group_count(gu) == lower.group_count(0);
         }
         if (has_lower_restart) lower_restart.upload(plan.lower_restart);
 
@@ -227,7 +236,9 @@ struct SystemT {
     // mode FWD_RESTART / FWD_UPDATE; the first group of an iteration runs as
     // FWD_FIRST (alpha from the SpMV's partials).
     // The last forward group also runs the first backward group when that
-    // group has the same rows and no upper entries (fuse_first_backward).
+    // group has the same rows and no upper entries (fuse_first_backward); the
+    // first forward group runs inside the second one's kernel
+    // (merge_first_forward).
     void forward_sweep(hipStream_t s, const DevPartT<T>& L, int mode, int p, const T* dd)
     {
         // vector streams per row: x r/w, d, r r/w, q, diag, m (iteration);
@@ -235,30 +246,38 @@ struct SystemT {
         const double per_row = mode == FWD_RESTART ? 24.0 : 64.0;
         const int nspmv = spmv_blocks(full.nslices);
         const int G = L.ngroups();
-        for (int g = 0; g < G; g++) {
-            const int gm = (mode != FWD_RESTART && g == 0) ? FWD_FIRST : mode;
+        const int g0 = merge_first_forward ? 1 : 0;
+        for (int g = g0; g < G; g++) {
+            const int gm = (mode != FWD_RESTART && g == g0) ? FWD_FIRST : mode;
             const bool fuse = fuse_first_backward && g == G - 1;
+            const bool merged = merge_first_forward && g == 1;
             double* zpart = fuse && mode != FWD_RESTART ? pb + upper.group_begin(0) : nullptr;
-            run(s, SFA_KERNEL_FORWARD, L.bytes(g, per_row + (fuse ? 8.0 : 0.0)), [&] {
+            const double bytes = L.bytes(g, per_row + (fuse ? 8.0 : 0.0)) +
+                                 (merged ? L.bytes(0, mode == FWD_RESTART ? 24.0 : 32.0) : 0.0);
+            run(s, SFA_KERNEL_FORWARD, bytes, [&] {
                 launch_forward(s, L.view(), L.group_begin(g), L.group_count(g), L.row0(g),
                                L.nrows_of(g), state, gm, p, x, r, dd, q, m, diag, w(), ps, nspmv,
-                               fuse ? z : (T*)nullptr, c2(), zpart, L.base(g), L.width(g));
+                               fuse ? z : (T*)nullptr, c2(), zpart, L.base(g), L.width(g),
+                               merged ? L.row0(0) : 0, merged ? L.nrows_of(0) : 0);
             });
         }
     }
 
-    void backward_sweep(hipStream_t s, int reduce)
+    // dd: the iteration's direction (for the deferred x update of a merged
+    // first forward group, made by the last backward kernel).
+    void backward_sweep(hipStream_t s, int reduce, const T* dd)
     {
         const int G = upper.ngroups();
         for (int g = fuse_first_backward ? 1 : 0; g < G; g++) {  // m, diag, z, r
             const bool fold_prev = reduce && g > 0;
-            run(s, SFA_KERNEL_BACKWARD, upper.bytes(g, 32.0), [&] {
+            const bool defer = merge_first_forward && reduce && g == G - 1;
+            run(s, SFA_KERNEL_BACKWARD, upper.bytes(g, defer ? 64.0 : 32.0), [&] {
                 launch_backward(s, upper.view(), upper.group_begin(g), upper.group_count(g),
                                 upper.row0(g), upper.nrows_of(g), state, reduce, r, m, z, diag,
                                 w(), c2(), pb, fold_prev ? upper.group_begin(g - 1) : 0,
                                 fold_prev ? upper.group_count(g - 1) : 0,
                                 fold_prev ? grp + (g - 1) : (double*)nullptr, upper.base(g),
-                                upper.width(g));
+                                upper.width(g), defer ? 1 : 0, x, dd, q);
             });
         }
     }
@@ -300,7 +319,7 @@ struct SystemT {
                                 state, b, x, r, (T*)nullptr, (T*)nullptr, (const T*)nullptr, pu);
             });
             forward_sweep(s, lower, FWD_RESTART, 0, d);
-            backward_sweep(s, 0);
+            backward_sweep(s, 0, d);
             run(s, SFA_KERNEL_RESTART, 24.0 * na,
                 [&] { launch_direction_init(s, state, n_active, z, r, d, pu); });
             break;
@@ -358,7 +377,7 @@ struct SystemT {
                                 full.width(0));
             });
             forward_sweep(s, lower, FWD_UPDATE, p, dnew);
-            backward_sweep(s, 1);
+            backward_sweep(s, 1, dnew);
             break;
         }
         default:  // Eisenstat: t (in m) = upper solve of d, q = lower solve
@@ -387,7 +406,8 @@ struct SystemT {
         case SFA_METHOD_DIAGONAL: return 2;
         case SFA_METHOD_SSOR:
         case SFA_METHOD_RAINBOW:
-            return 1 + lower.ngroups() + upper.ngroups() - (fuse_first_backward ? 1 : 0);
+            return 1 + lower.ngroups() + upper.ngroups() - (fuse_first_backward ? 1 : 0) -
+                   (merge_first_forward ? 1 : 0);
         default: return 2 + lower.ngroups() + upper.ngroups();
         }
     }
