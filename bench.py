@@ -33,7 +33,7 @@ PEAK_HBM_GBS = 8000.0
 # counter reads for a device-to-device copy of known size (bench.py
 # --copy-calibration); commands and per-launch figures are kept in
 # profiles/rainbow_cg_ssor_64cube_fp64_hbm_traffic.md.
-MEASURED_TRAFFIC = {"spmv": 80487884, "forward": 7232842, "backward": 5695753}
+MEASURED_TRAFFIC = {"spmv": 55312525, "forward": 6585245, "backward": 5002750}
 
 
 def log(msg):
