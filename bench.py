@@ -97,6 +97,8 @@ def main():
     ap.add_argument("--precision", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--spmv-lanes", type=int, default=0, choices=[0, 1, 2, 4, 8],
+                    help="SpMV lanes per row; 0: autotune on the system (default)")
     ap.add_argument("--config", default="poisson3d",
                     choices=["poisson3d", "hcurl-mixed", "batched", "served"],
                     help="poisson3d: the headline FP64 Rainbow solve (default); the others "
@@ -148,6 +150,14 @@ def main():
 
     sysd = capi.System(capi.METHOD_RAINBOW, system.n, system.A, system.C, system.V, d,
                        args.omega, *colouring)
+    if args.spmv_lanes:
+        sysd.set_spmv(args.spmv_lanes)
+        tuning = {"lanes_per_row": args.spmv_lanes, "tuned": False}
+    else:
+        rep = sysd.tune_spmv()
+        tuning = {"lanes_per_row": sysd.stats()["spmv_lanes_per_row"], "tuned": True,
+                  "buckets": rep}
+        log(f"spmv tuning: {rep}")
     stats = sysd.stats()
     bt = DeviceArray.from_host(b)
     x = DeviceArray(system.n)
@@ -225,6 +235,7 @@ def main():
             "colours": rb.num_colors, "iterations": info["iterations"],
             "kernels_per_iteration": stats["kernels_per_iteration"],
             "bytes_per_iteration": stats["bytes_per_iteration"],
+            "spmv_mapping": tuning,
             "true_relative_residual": rel_res,
         },
         "hbm_gbs_algorithmic": round(hbm_gbs, 1),

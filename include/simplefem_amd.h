@@ -214,7 +214,7 @@ typedef struct sfa_system_stats {
     double bytes_per_iteration; /* algorithmic HBM bytes of one CG step      */
     double bytes_spmv;        /* algorithmic bytes of one SpMV launch        */
     int    kernels_per_iteration;
-    int    reserved;
+    int    spmv_lanes_per_row; /* SpMV mapping in use (sfa_system_tune_spmv) */
 } sfa_system_stats;
 
 int sfa_system_query(sfa_system sys, sfa_system_stats* stats);
@@ -246,6 +246,37 @@ int sfa_system_profile(sfa_system sys, double* x, const double* b,
                        double* residual, double precision,
                        int device_pointers, sfa_kernel_profile* out,
                        sfa_solve_info* info);
+
+/* SpMV mapping.  Matrix parts are SELL-64 slices (64 rows, entries stored
+ * column-major); a mapping is the number of lanes that share a row, 1, 2, 4
+ * or 8: with K lanes per row a wavefront covers 64/K rows of a slice and
+ * lane t of a row reads the row's entries t, t+K, t+2K, ...  Every mapping
+ * adds a row's products in the row's entry order, so all give the same
+ * q = A d bit for bit; only the block partials of d.Ad follow the mapping.
+ * Row-length buckets: the widest row of a slice has <= 8, <= 16, <= 32 or
+ * more entries.  No reference counterpart (the reference's SpMV is a serial
+ * row loop, sparse/matcsr.cpp:216-244). */
+#define SFA_SPMV_BUCKETS  4
+#define SFA_SPMV_MAPPINGS 4   /* candidates: 1, 2, 4, 8 lanes per row */
+typedef struct sfa_spmv_tuning {
+    int       lanes_per_row[SFA_SPMV_BUCKETS];  /* chosen mapping; 0: no rows   */
+    long long rows[SFA_SPMV_BUCKETS];           /* rows per bucket              */
+    /* mean time of one launch per candidate, in microseconds (0: not timed) */
+    double    us[SFA_SPMV_BUCKETS][SFA_SPMV_MAPPINGS];
+} sfa_spmv_tuning;
+
+/* Autotunes the system's iteration SpMV: times every candidate mapping on
+ * the system's own SpMV kernel (on its device, its matrix, its stream) and
+ * installs the fastest.  The SpMV of one system is one launch and runs one
+ * mapping, so the candidates are timed over the whole matrix and reported
+ * under the bucket that holds most of its entries; every non-empty bucket
+ * reports the same choice.  fp32_inner != 0 tunes the FP32 inner system of
+ * a mixed-precision system instead.  Eisenstat variants have no SpMV: the
+ * call then reports lanes_per_row = 1 and times nothing. */
+int sfa_system_tune_spmv(sfa_system sys, int fp32_inner, sfa_spmv_tuning* out);
+/* Installs a mapping (1, 2, 4 or 8 lanes per row) without timing, for the
+ * system and its FP32 inner system. */
+int sfa_system_set_spmv(sfa_system sys, int lanes_per_row);
 
 int sfa_system_destroy(sfa_system sys);
 
@@ -324,6 +355,18 @@ int sfa_batch_collect(sfa_batch b, int slot, double* x, sfa_batch_info* info);
 
 /* Device memory held by matrices and slots. */
 int sfa_batch_memory(sfa_batch b, long long* bytes);
+
+/* Autotunes the batch's SpMV mapping per row-length bucket: for each bucket
+ * that holds matrices (a matrix's bucket is the one holding most of its
+ * entries), times the SpMV phase of all those matrices under every candidate
+ * mapping with the step kernel's SpMV code and grid, and installs the
+ * fastest for the bucket; matrices added later take their bucket's choice.
+ * Only between solves (SFA_ERR_INVALID while solves are running).  No
+ * reference counterpart. */
+int sfa_batch_tune_spmv(sfa_batch b, sfa_spmv_tuning* out);
+/* Installs lanes per row (1, 2, 4, 8) per bucket without timing; between
+ * solves only. */
+int sfa_batch_set_spmv(sfa_batch b, const int lanes_per_row[SFA_SPMV_BUCKETS]);
 
 int sfa_batch_destroy(sfa_batch b);
 

@@ -47,7 +47,7 @@ class SystemStats(ctypes.Structure):
                 ("groups_forward", c_int), ("groups_backward", c_int),
                 ("stored_entries", c_longlong), ("bytes_per_iteration", c_double),
                 ("bytes_spmv", c_double), ("kernels_per_iteration", c_int),
-                ("reserved", c_int)]
+                ("spmv_lanes_per_row", c_int)]
 
 
 class RefineInfo(ctypes.Structure):
@@ -56,6 +56,27 @@ class RefineInfo(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
+SPMV_BUCKETS = 4
+SPMV_MAPPINGS = (1, 2, 4, 8)
+SPMV_BUCKET_NAMES = ("<=8", "<=16", "<=32", ">32")
+
+
+class SpmvTuning(ctypes.Structure):
+    _fields_ = [("lanes_per_row", c_int * SPMV_BUCKETS), ("rows", c_longlong * SPMV_BUCKETS),
+                ("us", (c_double * len(SPMV_MAPPINGS)) * SPMV_BUCKETS)]
+
+    def as_dict(self):
+        """{bucket: {"rows", "lanes_per_row", "us": {lanes: microseconds}}} for non-empty buckets."""
+        out = {}
+        for b, name in enumerate(SPMV_BUCKET_NAMES):
+            if self.rows[b] == 0:
+                continue
+            out[name] = {"rows": int(self.rows[b]), "lanes_per_row": int(self.lanes_per_row[b]),
+                         "us": {k: round(self.us[b][i], 3) for i, k in enumerate(SPMV_MAPPINGS)
+                                if self.us[b][i] > 0}}
+        return out
 
 
 FLAG_FP32_INNER = 0x100
@@ -136,6 +157,8 @@ _SIGNATURES = {
     "sfa_system_query": ([c_void_p, POINTER(SystemStats)], c_int),
     "sfa_system_profile": ([c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int,
                             POINTER(KernelProfile), POINTER(SolveInfo)], c_int),
+    "sfa_system_tune_spmv": ([c_void_p, c_int, POINTER(SpmvTuning)], c_int),
+    "sfa_system_set_spmv": ([c_void_p, c_int], c_int),
     "sfa_system_destroy": ([c_void_p], c_int),
     "sfa_batch_create": ([c_int, c_int, POINTER(c_void_p)], c_int),
     "sfa_batch_add_matrix": ([c_void_p, c_int, _P_I, _P_I, _P_D, _P_D, c_double, c_int, _P_I,
@@ -151,6 +174,8 @@ _SIGNATURES = {
     "sfa_batch_finished": ([c_void_p, _P_I, c_int, POINTER(c_int)], c_int),
     "sfa_batch_collect": ([c_void_p, c_int, _P_D, POINTER(BatchInfo)], c_int),
     "sfa_batch_memory": ([c_void_p, POINTER(c_longlong)], c_int),
+    "sfa_batch_tune_spmv": ([c_void_p, POINTER(SpmvTuning)], c_int),
+    "sfa_batch_set_spmv": ([c_void_p, _P_I], c_int),
     "sfa_batch_destroy": ([c_void_p], c_int),
 }
 
@@ -406,6 +431,16 @@ class System:
     def set_stream(self, stream_handle):
         _check(load().sfa_system_set_stream(self._h, c_void_p(stream_handle)))
 
+    def tune_spmv(self, fp32_inner=False):
+        """Times every SpMV mapping (lanes per row) on this system and installs
+        the fastest; returns the per-bucket report (SpmvTuning.as_dict)."""
+        t = SpmvTuning()
+        _check(load().sfa_system_tune_spmv(self._h, 1 if fp32_inner else 0, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_spmv(self, lanes_per_row):
+        _check(load().sfa_system_set_spmv(self._h, int(lanes_per_row)))
+
     def stats(self):
         st = SystemStats()
         _check(load().sfa_system_query(self._h, ctypes.byref(st)))
@@ -487,6 +522,18 @@ class Batch:
         info = BatchInfo()
         _check(load().sfa_batch_collect(self._h, int(slot), _d(x), ctypes.byref(info)))
         return x, info.as_dict()
+
+    def tune_spmv(self):
+        """Per-bucket SpMV autotuning over the batch's matrices (between solves)."""
+        t = SpmvTuning()
+        _check(load().sfa_batch_tune_spmv(self._h, ctypes.byref(t)))
+        return t.as_dict()
+
+    def set_spmv(self, lanes_per_row):
+        lanes = np.ascontiguousarray(lanes_per_row, dtype=np.int32)
+        if lanes.shape != (SPMV_BUCKETS,):
+            raise ValueError("one mapping per bucket (%d)" % SPMV_BUCKETS)
+        _check(load().sfa_batch_set_spmv(self._h, _i(lanes)))
 
     def memory(self):
         b = c_longlong(0)

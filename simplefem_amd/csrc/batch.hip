@@ -30,6 +30,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -73,6 +74,7 @@ struct BMat {
     int gf, gb;
     int n, n_active;
     int full_slices, full_row0;
+    int spmv_lanes;                 // lanes per row of the iteration's SpMV (1, 2, 4, 8)
     T omega, c2;
 };
 
@@ -114,7 +116,7 @@ __device__ int phase_units(const BMat<T>& M, int kind, int g)
 {
     switch (kind) {
     case PH_RESID: return cdiv(M.full_slices, kUnitSlices);
-    case PH_SPMV: return max(1, cdiv(M.full_slices, kUnitSlices));
+    case PH_SPMV: return max(1, cdiv(M.full_slices * M.spmv_lanes, kUnitSlices));
     case PH_RES64: return max(1, cdiv(M.full_slices, kUnitSlices));
     case PH_FWD_R:
     case PH_FWD: return cdiv(M.fslice[g + 1] - M.fslice[g], kUnitSlices);
@@ -252,6 +254,35 @@ __device__ void phase_complete(BSlot<T>& S, const BMat<T>& M, double total)
     }
 }
 
+// q = A d over the rows of SpMV task `task` of a matrix whose SpMV runs K
+// lanes per row (task i: slice i/K, part i%K of 64/K rows); returns the
+// lane's share of d.Ad (the row's first lane holds the row's product).
+template <int K, typename T>
+__device__ double spmv_task(const BMat<T>& M, const T* __restrict__ d, T* __restrict__ q,
+                            int task, int lane)
+{
+    const int slice = task / K;
+    if (slice >= M.full_slices) return 0.;
+    const int row = group_row_at(M.full, slice, slice, sellk_slice_row<K>(task % K, lane),
+                                 M.full_row0, M.n_active);
+    if (row < 0) return 0.;  // every lane of a row agrees
+    const T acc = sellk_row<K, false>(M.full, slice, task % K, lane, d, T(0));
+    if (lane % K != 0) return 0.;
+    q[row] = acc;
+    return (double)(d[row] * acc);
+}
+
+template <typename T>
+__device__ double spmv_task_any(const BMat<T>& M, const T* d, T* q, int task, int lane)
+{
+    switch (M.spmv_lanes) {
+    case 2: return spmv_task<2>(M, d, q, task, lane);
+    case 4: return spmv_task<4>(M, d, q, task, lane);
+    case 8: return spmv_task<8>(M, d, q, task, lane);
+    default: return spmv_task<1>(M, d, q, task, lane);
+    }
+}
+
 // One work unit: 256 rows of the slot's current phase.  Returns the block's
 // partial sum for reduction phases (valid in thread 0).
 template <typename T>
@@ -261,23 +292,17 @@ __device__ double run_unit(BSlot<T>& S, const BMat<T>& M, int kind, int g, int j
     const int w = threadIdx.x >> 6;
     double part = 0.;
     switch (kind) {
-    case PH_RESID:
-    case PH_SPMV: {
+    case PH_RESID: {
         const int local = j * kUnitSlices + w;
         if (local < M.full_slices) {
             const int row = group_row(M.full, local, local, lane, M.full_row0, M.n_active);
-            if (row >= 0) {
-                if (kind == PH_RESID) {
-                    S.r[row] = sell_row<true>(M.full, local, lane, S.x, S.b[row]);
-                } else {
-                    const T acc = sell_row<false>(M.full, local, lane, S.d, T(0));
-                    S.q[row] = acc;
-                    part = (double)(S.d[row] * acc);
-                }
-            }
+            if (row >= 0) S.r[row] = sell_row<true>(M.full, local, lane, S.x, S.b[row]);
         }
         break;
     }
+    case PH_SPMV:
+        part = spmv_task_any(M, S.d, S.q, j * kUnitSlices + w, lane);
+        break;
     case PH_RES64: {
         const int local = j * kUnitSlices + w;
         if (local < M.full_slices) {
@@ -479,6 +504,32 @@ __global__ void __launch_bounds__(kPrefixBlock) k_batch_prefix(BArgs<T> a)
     }
 }
 
+// SpMV autotuning probe: the PH_SPMV units of the matrices mats[0..nm) with
+// K lanes per row, grid-stride as k_batch_step runs them (unit u belongs to
+// the matrix m with ustart[m] <= u < ustart[m+1]); d and q are scratch
+// vectors of at least n+1 entries for every listed matrix.
+template <typename T, int K>
+__global__ void __launch_bounds__(kUnitBlock) k_batch_spmv_probe(const BMat<T>* mats,
+                                                                 const int* ustart, int nm,
+                                                                 const T* d, T* q,
+                                                                 double* partials)
+{
+    __shared__ double ws[kUnitBlock / 64];
+    const int W = ustart[nm];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    for (int u = blockIdx.x; u < W; u += gridDim.x) {
+        int lo = 0, hi = nm;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (ustart[mid] <= u) lo = mid;
+            else hi = mid;
+        }
+        const int j = u - ustart[lo];
+        const double part = block_sum(spmv_task<K>(mats[lo], d, q, j * kUnitSlices + w, lane), ws);
+        if (threadIdx.x == 0) partials[u] = part;
+    }
+}
+
 // ====================================================================== host
 
 constexpr int kStepBlocks = 2048;  // grid of k_batch_step: 8 blocks of 256 threads per CU
@@ -497,6 +548,8 @@ struct HostMat {
     T* diag = nullptr;
     int max_units = 1, gb = 0;
     int slots_in_use = 0;
+    int bucket = 0;   // SpMV row-length bucket of the matrix (its full part's main one)
+    BMat<T> bm{};     // host copy of the device table entry
 
     void release()
     {
@@ -554,6 +607,7 @@ struct Engine {
     unsigned step = 0;
     hipGraphExec_t graph = nullptr;
     BMat<T>* graph_mats = nullptr;            // d_mats the graph was captured with
+    int spmv_table[kSpmvBuckets] = {1, 1, 1, 1};  // lanes per row of the SpMV per bucket
 
     ~Engine() { release(); }
 
@@ -656,6 +710,8 @@ struct Engine {
             for (int g = 0; g < part->ngroups(); g++)
                 mu = std::max(mu, (part->group_slice[g + 1] - part->group_slice[g] +
                                    kUnitSlices - 1) / kUnitSlices);
+        M.bucket = M.full.main_bucket();
+        mu = std::max(mu, (plan.full.nslices * 8 + kUnitSlices - 1) / kUnitSlices);  // any mapping
         M.max_units = mu;
         M.bytes = (long long)(plan.full.vals.size() + plan.lower.vals.size() + plan.upper.vals.size()) *
                       (sizeof(T) + 4) + (long long)(n + 1) * sizeof(T) +
@@ -679,8 +735,10 @@ struct Engine {
         bm.n_active = plan.n_active;
         bm.full_slices = plan.full.nslices;
         bm.full_row0 = plan.full.group_row0.empty() ? 0 : plan.full.group_row0[0];
+        bm.spmv_lanes = spmv_table[M.bucket];
         bm.omega = T(omega);
         bm.c2 = T((2. - omega) / omega);
+        M.bm = bm;
         if (id >= d_mats_cap) {  // grow the device table (no kernel is running)
             const int cap = std::max(16, 2 * d_mats_cap);
             BMat<T>* nm = dev_alloc<BMat<T>>(cap);
@@ -693,6 +751,134 @@ struct Engine {
         }
         SFA_HIP(hipMemcpy(d_mats + id, &bm, sizeof(BMat<T>), hipMemcpyHostToDevice));
         return id;
+    }
+
+    bool any_running()
+    {
+        running();
+        for (int s = 0; s < max_slots; s++)
+            if (slot_used[s] && slots[s].kind != PH_DONE) return true;
+        return false;
+    }
+
+    // Installs lanes per row for each bucket in every matrix (the step kernel
+    // reads them from the device table at each SpMV phase).  Only between
+    // solves: a phase's unit count is fixed when the phase is scheduled.
+    void set_spmv(const int* lanes)
+    {
+        for (int b = 0; b < kSpmvBuckets; b++)
+            if (lanes[b] != 1 && lanes[b] != 2 && lanes[b] != 4 && lanes[b] != 8)
+                throw Error{SFA_ERR_INVALID, "lanes per row must be 1, 2, 4 or 8"};
+        SFA_HIP(hipSetDevice(device));
+        if (any_running()) throw Error{SFA_ERR_INVALID, "solves are running; set the SpMV mapping between solves"};
+        for (int b = 0; b < kSpmvBuckets; b++) spmv_table[b] = lanes[b];
+        for (size_t id = 0; id < mats.size(); id++) {
+            HostMat<T>& M = mats[id];
+            if (!M.live) continue;
+            M.bm.spmv_lanes = spmv_table[M.bucket];
+            SFA_HIP(hipMemcpy(d_mats + id, &M.bm, sizeof(BMat<T>), hipMemcpyHostToDevice));
+        }
+    }
+
+    // For each bucket that holds matrices: times the SpMV phase of all its
+    // matrices under each mapping (k_batch_spmv_probe, the step kernel's SpMV
+    // code and grid) and installs the fastest.
+    void tune_spmv(sfa_spmv_tuning* out)
+    {
+        SFA_HIP(hipSetDevice(device));
+        if (any_running()) throw Error{SFA_ERR_INVALID, "solves are running; tune between solves"};
+        sfa_spmv_tuning t{};
+        std::vector<int> chosen(spmv_table, spmv_table + kSpmvBuckets);
+        std::vector<std::vector<int>> ids(kSpmvBuckets);
+        int nmax = 1;
+        long long umax = 1;
+        for (size_t id = 0; id < mats.size(); id++) {
+            const HostMat<T>& M = mats[id];
+            if (!M.live) continue;
+            ids[M.bucket].push_back((int)id);
+            nmax = std::max(nmax, M.n + 1);
+            for (int b = 0; b < kSpmvBuckets; b++) t.rows[b] += M.full.bucket_rows[b];
+        }
+        for (int b = 0; b < kSpmvBuckets; b++) {
+            long long u = 0;
+            for (int id : ids[b]) u += (mats[id].full.nslices * 8LL + kUnitSlices - 1) / kUnitSlices + 1;
+            umax = std::max(umax, u);
+        }
+        if (umax > INT_MAX / 2) throw Error{SFA_ERR_INVALID, "too many SpMV units to tune"};
+        T* vd = dev_alloc<T>(nmax);
+        T* vq = dev_alloc<T>(nmax);
+        double* part = dev_alloc<double>(umax);
+        BMat<T>* dm = nullptr;
+        int* dstart = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto cleanup = [&] {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            for (void* v : {(void*)vd, (void*)vq, (void*)part, (void*)dm, (void*)dstart}) dev_free(v);
+        };
+        try {
+            launch_fill<T>(stream, vd, T(1), nmax);
+            SFA_HIP(hipEventCreate(&e0));
+            SFA_HIP(hipEventCreate(&e1));
+            const int cand[SFA_SPMV_MAPPINGS] = {1, 2, 4, 8};
+            for (int b = 0; b < kSpmvBuckets; b++) {
+                const int nm = (int)ids[b].size();
+                if (nm == 0) continue;
+                std::vector<BMat<T>> hm;
+                for (int id : ids[b]) hm.push_back(mats[id].bm);
+                dev_free(dm);
+                dev_free(dstart);
+                dm = nullptr;
+                dstart = nullptr;
+                dm = dev_alloc<BMat<T>>(nm);
+                dstart = dev_alloc<int>(nm + 1);
+                SFA_HIP(hipMemcpy(dm, hm.data(), sizeof(BMat<T>) * nm, hipMemcpyHostToDevice));
+                double best_us = 0.;
+                for (int i = 0; i < SFA_SPMV_MAPPINGS; i++) {
+                    const int K = cand[i];
+                    std::vector<int> us(nm + 1, 0);
+                    for (int m = 0; m < nm; m++)
+                        us[m + 1] = us[m] + std::max(1, (hm[m].full_slices * K + kUnitSlices - 1) / kUnitSlices);
+                    SFA_HIP(hipMemcpy(dstart, us.data(), sizeof(int) * (nm + 1), hipMemcpyHostToDevice));
+                    auto once = [&] {
+                        switch (K) {
+                        case 1: launch(k_batch_spmv_probe<T, 1>, dim3(kStepBlocks), dim3(kUnitBlock), stream, (const BMat<T>*)dm, (const int*)dstart, nm, (const T*)vd, vq, part); break;
+                        case 2: launch(k_batch_spmv_probe<T, 2>, dim3(kStepBlocks), dim3(kUnitBlock), stream, (const BMat<T>*)dm, (const int*)dstart, nm, (const T*)vd, vq, part); break;
+                        case 4: launch(k_batch_spmv_probe<T, 4>, dim3(kStepBlocks), dim3(kUnitBlock), stream, (const BMat<T>*)dm, (const int*)dstart, nm, (const T*)vd, vq, part); break;
+                        default: launch(k_batch_spmv_probe<T, 8>, dim3(kStepBlocks), dim3(kUnitBlock), stream, (const BMat<T>*)dm, (const int*)dstart, nm, (const T*)vd, vq, part); break;
+                        }
+                    };
+                    for (int w = 0; w < 3; w++) once();
+                    constexpr int kReps = 10;
+                    float ms_min = 0.f;
+                    for (int rep = 0; rep < 3; rep++) {
+                        SFA_HIP(hipEventRecord(e0, stream));
+                        for (int k = 0; k < kReps; k++) once();
+                        SFA_HIP(hipEventRecord(e1, stream));
+                        SFA_HIP(hipEventSynchronize(e1));
+                        float ms = 0.f;
+                        SFA_HIP(hipEventElapsedTime(&ms, e0, e1));
+                        if (rep == 0 || ms < ms_min) ms_min = ms;
+                    }
+                    SFA_HIP(hipGetLastError());
+                    const double us_per = 1000.0 * ms_min / kReps;
+                    t.us[b][i] = us_per;
+                    if (i == 0 || us_per < best_us) {
+                        best_us = us_per;
+                        chosen[b] = K;
+                    }
+                }
+            }
+            SFA_HIP(hipStreamSynchronize(stream));
+            cleanup();
+        } catch (...) {
+            (void)hipStreamSynchronize(stream);
+            cleanup();
+            throw;
+        }
+        set_spmv(chosen.data());
+        for (int b = 0; b < kSpmvBuckets; b++) t.lanes_per_row[b] = t.rows[b] ? chosen[b] : 0;
+        *out = t;
     }
 
     void remove_matrix(int id)
@@ -1043,6 +1229,22 @@ int sfa_batch_memory(sfa_batch b, long long* bytes)
     return sfa::guarded([&] {
         if (!b || !bytes) throw sfa::Error{SFA_ERR_INVALID, "null argument"};
         b->h.visit([&](auto& e) { *bytes = e.bytes_in_use(); });
+    });
+}
+
+int sfa_batch_tune_spmv(sfa_batch b, sfa_spmv_tuning* out)
+{
+    return sfa::guarded([&] {
+        if (!b || !out) throw sfa::Error{SFA_ERR_INVALID, "null argument"};
+        b->h.visit([&](auto& e) { e.tune_spmv(out); });
+    });
+}
+
+int sfa_batch_set_spmv(sfa_batch b, const int* lanes_per_row)
+{
+    return sfa::guarded([&] {
+        if (!b || !lanes_per_row) throw sfa::Error{SFA_ERR_INVALID, "null argument"};
+        b->h.visit([&](auto& e) { e.set_spmv(lanes_per_row); });
     });
 }
 

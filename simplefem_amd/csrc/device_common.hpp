@@ -207,6 +207,109 @@ __device__ inline T sell_finish(const SellViewT<T>& A, const SellChunk<T, U>& R,
     return acc;
 }
 
+// ---- K lanes per row (K = 2, 4, 8) over the same SELL-64 storage.
+// A wavefront covers rows q*64/K .. (q+1)*64/K - 1 of a slice (its part q of
+// K); lane t = lane % K of a row reads the row's entries t, t+K, t+2K, ...,
+// so one load instruction reads 64/K consecutive rows at K consecutive entry
+// positions: K contiguous runs of 64/K elements.  A lane holds at most U
+// entries per chunk, so a row of w entries is one chunk when w <= U*K.  The
+// products are added in the row's entry order, each passed from the lane
+// that formed it, so every lane of the row forms the same sum, bit for bit
+// the sum one lane forms over the row.
+
+// Row inside the slice of a lane of part q.
+template <int K>
+__device__ inline int sellk_slice_row(int q, int lane)
+{
+    return q * (kWave / K) + lane / K;
+}
+
+template <int K, typename T, int U>
+__device__ inline void sellk_prefetch(const SellViewT<T>& A, int slice, int local, int gbase,
+                                      int gw, int q, int lane, SellChunk<T, U>& R)
+{
+    int o0;
+    if (gw >= 0) {
+        o0 = gbase + local * (gw * kWave);
+        R.w = gw;
+    } else {
+        o0 = A.off[slice];
+        R.w = (A.off[slice + 1] - o0) / kWave;
+    }
+    const int t = lane % K;
+    R.e = o0 + sellk_slice_row<K>(q, lane) + t * kWave;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        if (u * K + t < R.w) {
+            R.c[u] = (unsigned)__builtin_nontemporal_load(&A.cols[R.e + u * K * kWave]);
+            R.v[u] = __builtin_nontemporal_load(&A.vals[R.e + u * K * kWave]);
+        } else {
+            R.c[u] = 0u;
+            R.v[u] = T(0);
+        }
+    }
+}
+
+template <bool SUB, typename T>
+__device__ inline T sell_acc(T acc, T x)
+{
+    return SUB ? acc - x : acc + x;
+}
+
+// acc (+|-)= the row's products, in entry order (see above).
+template <int K, bool SUB, typename T, int U, class Get>
+__device__ inline T sellk_finish(const SellViewT<T>& A, const SellChunk<T, U>& R, Get get, T acc,
+                                 int lane)
+{
+    const int t = lane % K;
+    const int base = lane - t;
+    T pr[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) pr[u] = u * K + t < R.w ? R.v[u] * get(R.c[u]) : T(0);
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+#pragma unroll
+        for (int tt = 0; tt < K; tt++)
+            if (u * K + tt < R.w) acc = sell_acc<SUB>(acc, __shfl(pr[u], base + tt, kWave));
+    }
+    constexpr int V = 8;
+    for (int k0 = U; k0 * K < R.w; k0 += V) {
+        T h[V];
+#pragma unroll
+        for (int u = 0; u < V; u++) {
+            h[u] = T(0);
+            if ((k0 + u) * K + t < R.w) {
+                const int e = R.e + (k0 + u) * K * kWave;
+                const unsigned c = (unsigned)__builtin_nontemporal_load(&A.cols[e]);
+                const T v = __builtin_nontemporal_load(&A.vals[e]);
+                h[u] = v * get(c);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < V; u++) {
+#pragma unroll
+            for (int tt = 0; tt < K; tt++)
+                if ((k0 + u) * K + tt < R.w) acc = sell_acc<SUB>(acc, __shfl(h[u], base + tt, kWave));
+        }
+    }
+    return acc;
+}
+
+// One row of a part with K lanes per row (K = 1: sell_row), offsets read
+// from the part.
+template <int K, bool SUB, typename T>
+__device__ inline T sellk_row(const SellViewT<T>& A, int slice, int q, int lane,
+                              const T* __restrict__ vec, T acc)
+{
+    if constexpr (K == 1) {
+        return sell_row<SUB>(A, slice, lane, vec, acc);
+    } else {
+        SellChunk<T, 8> R;
+        sellk_prefetch<K>(A, slice, slice, 0, -1, q, lane, R);
+        return sellk_finish<K, SUB>(A, R, [&](unsigned c) { return vec[c]; }, acc, lane);
+    }
+}
+
 // Workgroups are dispatched to the GPU's 8 XCDs round robin (workgroup i on
 // XCD i % 8), and each XCD has its own L2.  xcd_block maps workgroup i of n
 // to a logical block so that every XCD gets a contiguous range of logical
@@ -232,6 +335,18 @@ __device__ inline int group_row(const SellViewT<T>& A, int slice, int local, int
         return i < nrows ? row0 + i : -1;
     }
     return A.rows[slice * kWave + lane];
+}
+
+// Device row of row i (0..63) of a slice.
+template <typename T>
+__device__ inline int group_row_at(const SellViewT<T>& A, int slice, int local, int i, int row0,
+                                   int nrows)
+{
+    if (row0 >= 0) {
+        const int j = local * kWave + i;
+        return j < nrows ? row0 + j : -1;
+    }
+    return A.rows[slice * kWave + i];
 }
 
 

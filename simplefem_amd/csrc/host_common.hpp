@@ -69,9 +69,13 @@ struct DevPartT {
     long long entries = 0;
     long long nrows = 0;
     int zero_slot = 0;
+    // rows and stored entries of the slices in each SpMV row-length bucket
+    long long bucket_rows[kSpmvBuckets] = {};
+    long long bucket_stored[kSpmvBuckets] = {};
 
     void upload(const SellPart& p)
     {
+        spmv_bucket_counts(p, bucket_rows, bucket_stored);
         nslices = p.nslices;
         zero_slot = p.zero_slot;
         group_slice = p.group_slice;
@@ -108,6 +112,14 @@ struct DevPartT {
         vals = nullptr;
     }
     SellViewT<T> view() const { return SellViewT<T>{off, rows, cols, vals, zero_slot}; }
+    // the bucket holding most of the part's stored entries
+    int main_bucket() const
+    {
+        int b = 0;
+        for (int i = 1; i < kSpmvBuckets; i++)
+            if (bucket_stored[i] > bucket_stored[b]) b = i;
+        return b;
+    }
     int ngroups() const { return (int)group_slice.size() - 1; }
     int group_begin(int g) const { return group_slice[g]; }
     int group_count(int g) const { return group_slice[g + 1] - group_slice[g]; }

@@ -222,20 +222,51 @@ __global__ void __launch_bounds__(kWave) k_result_finish(SolveState* st, const d
 
 // ------------------------------------------------------------------ apply
 
-// MatrixCSR::apply (sparse/matcsr.cpp:216-244): dest = 0; dest += scaling*V*x.
-template <typename T>
+// MatrixCSR::apply (sparse/matcsr.cpp:216-244): dest = 0; dest += scaling*V*x,
+// entries in order.  K lanes per row: one wavefront per task (slice
+// task/K, part task%K of 64/K rows), products passed lane to lane in entry
+// order, so every K gives the same bits.
+template <typename T, int K>
 __global__ void __launch_bounds__(kSellBlock) k_apply(SellViewT<T> A, int nslices, int row0,
                                                       int nrows, const T* src, T* dest,
                                                       T scaling)
 {
-    const int slice = blockIdx.x;
+    const int task = blockIdx.x;
     const int lane = threadIdx.x;
-    if (slice >= nslices) return;
-    const int row = group_row(A, slice, slice, lane, row0, nrows);
-    if (row < 0) return;
-    const int beg = A.off[slice] + lane, end = A.off[slice + 1];
+    if (task >= nslices * K) return;
+    const int slice = task / K, part = task % K;
+    const int i = sellk_slice_row<K>(part, lane);
+    const int row = group_row_at(A, slice, slice, i, row0, nrows);
+    if (row < 0) return;  // every lane of a row agrees
     T acc = 0;
-    for (int e = beg; e < end; e += kWave) acc += scaling * A.vals[e] * src[A.cols[e]];
+    if constexpr (K == 1) {
+        const int beg = A.off[slice] + lane, end = A.off[slice + 1];
+        for (int e = beg; e < end; e += kWave) acc += scaling * A.vals[e] * src[A.cols[e]];
+    } else {
+        const int t = lane % K, base = lane - t;
+        const int o0 = A.off[slice];
+        const int w = (A.off[slice + 1] - o0) / kWave;
+        const int e0 = o0 + i + t * kWave;
+        constexpr int U = 8;
+        for (int u0 = 0; u0 * K < w; u0 += U) {
+            T pr[U];
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+                pr[u] = T(0);
+                if ((u0 + u) * K + t < w) {
+                    const int e = e0 + (u0 + u) * K * kWave;
+                    pr[u] = scaling * A.vals[e] * src[A.cols[e]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; u++) {
+#pragma unroll
+                for (int tt = 0; tt < K; tt++)
+                    if ((u0 + u) * K + tt < w) acc += __shfl(pr[u], base + tt, kWave);
+            }
+        }
+        if (t != 0) return;
+    }
     dest[row] = acc;
 }
 
@@ -521,12 +552,13 @@ __global__ void __launch_bounds__(kSellBlock) k_eisenstat_upper(SellViewT<T> P, 
 
 // Opens the iteration (z.r of the previous sweep, beta, K, exit tests), then
 // dn = src + beta*dold (solver/sparsesolver.cpp:698-706 and its siblings),
-// q = A dn and the d.Ad partials (:611-626, 1197-1212).  Each wavefront owns
-// slices b*4 + wave + k*4*gridDim.x (b: the block's logical index, see
-// xcd_block); neighbours' dn are formed from
-// src and dold where they are gathered, with the same arithmetic as the
-// owner's, so every copy has the same bits.
-template <typename T>
+// q = A dn and the d.Ad partials (:611-626, 1197-1212).  With K lanes per row
+// a slice is K tasks of 64/K rows (task i: slice i/K, part i%K); each
+// wavefront owns tasks b*4 + wave + k*4*gridDim.x (b: the block's logical
+// index, see xcd_block).  Neighbours' dn are formed from src and dold where
+// they are gathered, with the same arithmetic as the owner's, so every copy
+// has the same bits.
+template <typename T, int K>
 __global__ void __launch_bounds__(kSpmvBlock) k_spmv_dir(SellViewT<T> A, int nslices, int row0,
                                                          int nrows, int n_active,
                                                          SolveState* st, int p,
@@ -540,21 +572,26 @@ __global__ void __launch_bounds__(kSpmvBlock) k_spmv_dir(SellViewT<T> A, int nsl
     // scalar loads
     const int lane = threadIdx.x % kWave;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int first_slice = xcd_block(blockIdx.x, gridDim.x) * kSpmvWaves + wave;
+    const int ntasks = nslices * K;
+    const int first_task = xcd_block(blockIdx.x, gridDim.x) * kSpmvWaves + wave;
     const int stride = gridDim.x * kSpmvWaves;
-    // the first slice's own-row values and entries, loaded ahead of the state
+    // the lane that stores its row's results (lane 0 of the row's K lanes)
+    const bool owner = lane % K == 0;
+    // the first task's own-row values and entries, loaded ahead of the state
     int row = -1;
     T s_own = T(0), d_own = T(0);
     SellChunk<T, 8> R;  // 8: keeps every block of the grid resident
-    if (first_slice < nslices) {
-        row = group_row(A, first_slice, first_slice, lane, row0, nrows);
+    if (first_task < ntasks) {
+        const int slice = first_task / K, part = first_task % K;
+        row = group_row_at(A, slice, slice, sellk_slice_row<K>(part, lane), row0, nrows);
         const int ri = row >= 0 ? row : A.zero;
         s_own = src[ri];
         d_own = dold[ri];
-        sell_prefetch(A, first_slice, first_slice, 0, gw, lane, R);
+        if constexpr (K == 1) sell_prefetch(A, slice, slice, 0, gw, lane, R);
+        else sellk_prefetch<K>(A, slice, slice, 0, gw, part, lane, R);
     }
     const int fold = st->fold[p];
-    const int K = st->Kp[p];
+    const int Kit = st->Kp[p];
     const double zr_old = st->zrp[p];
     // Block 0 may set done while other blocks start; the vote keeps the
     // waves of a block together (a set flag means the loop has ended).
@@ -572,30 +609,34 @@ __global__ void __launch_bounds__(kSpmvBlock) k_spmv_dir(SellViewT<T> A, int nsl
         zr_new = sh;
     }
     double beta_d;
-    if (open_step(st, p, fold, zr_old, K, zr_new, &beta_d)) return;
+    if (open_step(st, p, fold, zr_old, Kit, zr_new, &beta_d)) return;
     const T beta = (T)beta_d;
     // rows past n_active (inactive rows, the zero slot) keep their d
     auto dn = [&](unsigned c) -> T {
         return fold && c < (unsigned)n_active ? src[c] + beta * dold[c] : dold[c];
     };
-    double part = 0.;
-    for (int slice = first_slice; slice < nslices; slice += stride) {
-        if (slice != first_slice) {
-            row = group_row(A, slice, slice, lane, row0, nrows);
+    double part_sum = 0.;
+    for (int task = first_task; task < ntasks; task += stride) {
+        if (task != first_task) {
+            const int slice = task / K, part = task % K;
+            row = group_row_at(A, slice, slice, sellk_slice_row<K>(part, lane), row0, nrows);
             const int ri = row >= 0 ? row : A.zero;
             s_own = src[ri];
             d_own = dold[ri];
-            sell_prefetch(A, slice, slice, 0, gw, lane, R);
+            if constexpr (K == 1) sell_prefetch(A, slice, slice, 0, gw, lane, R);
+            else sellk_prefetch<K>(A, slice, slice, 0, gw, part, lane, R);
         }
-        const T acc = sell_finish<false>(A, R, dn, T(0));
-        if (row >= 0) {
+        T acc;
+        if constexpr (K == 1) acc = sell_finish<false>(A, R, dn, T(0));
+        else acc = sellk_finish<K, false>(A, R, dn, T(0), lane);
+        if (row >= 0 && owner) {
             const T dr = fold && row < n_active ? s_own + beta * d_own : d_own;  // = dn(row)
             dnew[row] = dr;
             q[row] = acc;
-            part += (double)(dr * acc);
+            part_sum += (double)(dr * acc);
         }
     }
-    block_partial<kSpmvBlock>(part, &spart[blockIdx.x]);
+    block_partial<kSpmvBlock>(part_sum, &spart[blockIdx.x]);
 }
 
 // d.Ad step from dpart, then x += a dx; r -= a q; z.r by mode
@@ -722,10 +763,17 @@ void launch_init_state(hipStream_t s, SolveState* st, const T* b, int n, double 
 
 template <typename T>
 void launch_apply(hipStream_t s, SellViewT<T> A, int nslices, int row0, int nrows, const T* src,
-                  T* dest, T scaling)
+                  T* dest, T scaling, int lanes)
 {
-    if (nslices > 0)
-        launch(k_apply<T>, dim3(nslices), dim3(kSellBlock), s, A, nslices, row0, nrows, src, dest, scaling);
+    if (nslices <= 0) return;
+    const dim3 grid(nslices * lanes), block(kSellBlock);
+    switch (lanes) {
+    case 1: launch(k_apply<T, 1>, grid, block, s, A, nslices, row0, nrows, src, dest, scaling); break;
+    case 2: launch(k_apply<T, 2>, grid, block, s, A, nslices, row0, nrows, src, dest, scaling); break;
+    case 4: launch(k_apply<T, 4>, grid, block, s, A, nslices, row0, nrows, src, dest, scaling); break;
+    case 8: launch(k_apply<T, 8>, grid, block, s, A, nslices, row0, nrows, src, dest, scaling); break;
+    default: break;
+    }
 }
 
 template <typename T>
@@ -826,13 +874,20 @@ template <typename T>
 void launch_spmv_dir(hipStream_t s, SellViewT<T> A, int nslices, int row0, int nrows,
                      int n_active, SolveState* st, int p, const T* src, const T* dold, T* dnew,
                      T* q, double* spmv_partials, const double* fold_partials, int nfold,
-                     const double* group_sums, int ngroups, int gw)
+                     const double* group_sums, int ngroups, int gw, int lanes)
 {
-    launch(k_spmv_dir<T>, dim3(spmv_blocks(nslices)), dim3(kSpmvBlock), s, A, nslices, row0, nrows, n_active,
-                                                              st, p, src,
-                                                              dold, dnew, q, spmv_partials,
-                                                              fold_partials, nfold, group_sums,
-                                                              ngroups, gw);
+    const dim3 grid(spmv_blocks(nslices * lanes)), block(kSpmvBlock);
+#define SFA_SPMV_DIR(KK)                                                                       \
+    launch(k_spmv_dir<T, KK>, grid, block, s, A, nslices, row0, nrows, n_active, st, p, src,  \
+           dold, dnew, q, spmv_partials, fold_partials, nfold, group_sums, ngroups, gw)
+    switch (lanes) {
+    case 1: SFA_SPMV_DIR(1); break;
+    case 2: SFA_SPMV_DIR(2); break;
+    case 4: SFA_SPMV_DIR(4); break;
+    case 8: SFA_SPMV_DIR(8); break;
+    default: break;
+    }
+#undef SFA_SPMV_DIR
 }
 
 template <typename T>
@@ -869,7 +924,8 @@ void launch_add_correction(hipStream_t s, double* x, const float* e, int n)
     template void launch_scatter<T>(hipStream_t, T*, const T*, const int*, int);               \
     template void launch_init_state<T>(hipStream_t, SolveState*, const T*, int, double, double, \
                                        int, int, int, double, double*);                        \
-    template void launch_apply<T>(hipStream_t, SellViewT<T>, int, int, int, const T*, T*, T);  \
+    template void launch_apply<T>(hipStream_t, SellViewT<T>, int, int, int, const T*, T*, T,   \
+                                  int);                                                        \
     template void launch_residual<T>(hipStream_t, ResidualMode, SellViewT<T>, int, int, int,   \
                                      SolveState*, const T*, const T*, T*, T*, T*, const T*,    \
                                      double*);                                                 \
@@ -894,7 +950,7 @@ void launch_add_correction(hipStream_t s, double* x, const float* e, int n)
                                             const T*, T, const double*, int, int, int);        \
     template void launch_spmv_dir<T>(hipStream_t, SellViewT<T>, int, int, int, int,            \
                                      SolveState*, int, const T*, const T*, T*, T*, double*,    \
-                                     const double*, int, const double*, int, int);             \
+                                     const double*, int, const double*, int, int, int);        \
     template void launch_update<T>(hipStream_t, UpdateMode, SolveState*, int, int, T*, T*,     \
                                    const T*, const T*, T*, const T*, T, const double*, int,    \
                                    double*);                                                   \

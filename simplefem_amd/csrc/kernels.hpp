@@ -79,9 +79,10 @@ void launch_init_state(hipStream_t s, SolveState* st, const T* b, int n, double 
 // MatrixCSR::apply: dest[row] = sum (scaling*v)*src[col], over a part's rows.
 // (row0, nrows): a group whose rows are row0 .. row0+nrows-1 in slice order;
 // row0 = -1 makes the kernels read row ids from the part's row list.
+// lanes: lanes per row (1, 2, 4, 8); every value gives the same bits.
 template <typename T>
 void launch_apply(hipStream_t s, SellViewT<T> A, int nslices, int row0, int nrows, const T* src,
-                  T* dest, T scaling);
+                  T* dest, T scaling, int lanes = 1);
 
 // --- restart (K = 0) ---
 enum ResidualMode { RES_ONLY = 0, RES_PLAIN = 1, RES_JACOBI = 2 };
@@ -156,12 +157,13 @@ void launch_eisenstat_upper(hipStream_t s, SellViewT<T> P, int slice0, int nslic
 // (sum of group_sums[0..ngroups) then of fold_partials[0..nfold)), beta, K
 // and the loop-head tests; then dn = fold ? src + beta*dold : dold on the
 // active rows (dold elsewhere), dnew = dn, q = A dn and the d.Ad partials
-// (spmv_partials, one per block).
+// (spmv_partials, one per block: spmv_blocks(nslices * lanes) of them).
+// lanes (1, 2, 4, 8): lanes per row of the SpMV mapping.
 template <typename T>
 void launch_spmv_dir(hipStream_t s, SellViewT<T> A, int nslices, int row0, int nrows,
                      int n_active, SolveState* st, int p, const T* src, const T* dold, T* dnew,
                      T* q, double* spmv_partials, const double* fold_partials, int nfold,
-                     const double* group_sums, int ngroups, int gw);
+                     const double* group_sums, int ngroups, int gw, int lanes);
 enum UpdateMode { UPD_PLAIN = 0, UPD_JACOBI = 1, UPD_EISENSTAT = 2 };
 // Forms d.Ad from dad_partials (ndad), breakdown tests, alpha; then
 // x += a*dx; r -= a*q; z.r per mode into zr_partials (one per block).

@@ -184,6 +184,28 @@ std::vector<int> sweep_levels(int n, const int* A, const int* C, const std::vect
 
 }  // namespace
 
+void spmv_bucket_counts(const SellPart& part, long long rows[kSpmvBuckets],
+                        long long stored[kSpmvBuckets])
+{
+    for (int b = 0; b < kSpmvBuckets; b++) rows[b] = stored[b] = 0;
+    for (int g = 0; g < part.ngroups(); g++) {
+        for (int sl = part.group_slice[g]; sl < part.group_slice[g + 1]; sl++) {
+            const int n = part.slice_off[sl + 1] - part.slice_off[sl];
+            const int b = spmv_bucket(n / kSliceRows);
+            int r = 0;
+            if (part.group_row0[g] >= 0) {
+                const long long left = part.group_rows[g] -
+                                       (long long)(sl - part.group_slice[g]) * kSliceRows;
+                r = (int)std::min<long long>(kSliceRows, std::max<long long>(0, left));
+            } else {
+                for (int l = 0; l < kSliceRows; l++) r += part.slice_rows[(size_t)sl * kSliceRows + l] >= 0;
+            }
+            rows[b] += r;
+            stored[b] += n;
+        }
+    }
+}
+
 void build_full_identity(SellPart& part, int rows, int columns, const int* A, const int* C,
                          const double* V)
 {

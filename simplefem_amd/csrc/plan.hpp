@@ -64,6 +64,17 @@ struct Plan {
     bool has_lower_restart = false;
 };
 
+// SpMV row-length buckets: the widest row of a slice has <= 8, <= 16, <= 32
+// or more entries.
+constexpr int kSpmvBuckets = 4;
+inline int spmv_bucket(int width)
+{
+    return width <= 8 ? 0 : width <= 16 ? 1 : width <= 32 ? 2 : 3;
+}
+// Rows and stored entries (padding included) of a part's slices per bucket.
+void spmv_bucket_counts(const SellPart& part, long long rows[kSpmvBuckets],
+                        long long stored[kSpmvBuckets]);
+
 // Builds the plan.  aux is the diagonal (SSOR family) or precon (Jacobi), may
 // be null for plain CG; F/B/R/num_colors only for the Rainbow variants.
 // Returns "" on success, else an error message; *colouring_error is set when

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -59,6 +60,7 @@ struct SystemT {
     bool rainbow = false;
     bool fuse_first_backward = false;  // see forward_sweep
     bool merge_first_forward = false;  // see forward_sweep
+    int spmv_lanes = 1;                // lanes per row of the iteration's SpMV (1, 2, 4, 8)
 
     DevPartT<T> full, lower, upper, lower_restart;
     bool has_lower_restart = false;
@@ -214,7 +216,7 @@ struct SystemT {
         io3 = dev_alloc<T>(n);
         state = dev_alloc<SolveState>(1);
         hist = dev_alloc<double>((size_t)n + 1);
-        ps = dev_alloc<double>(std::max(spmv_blocks(full.nslices), kVecReduceBlocks));
+        ps = dev_alloc<double>(std::max(kSpmvMaxBlocks, kVecReduceBlocks));
         pu = dev_alloc<double>(std::max({full.nslices, kVecReduceBlocks, 1}));
         pb = dev_alloc<double>(std::max(sweeps ? upper.nslices : 0, 1));
         int ngr = 1;
@@ -244,7 +246,7 @@ struct SystemT {
         // vector streams per row: x r/w, d, r r/w, q, diag, m (iteration);
         // r, diag, m (restart); + z when fused
         const double per_row = mode == FWD_RESTART ? 24.0 : 64.0;
-        const int nspmv = spmv_blocks(full.nslices);
+        const int nspmv = spmv_blocks(full.nslices * spmv_lanes);
         const int G = L.ngroups();
         const int g0 = merge_first_forward ? 1 : 0;
         for (int g = g0; g < G; g++) {
@@ -346,7 +348,7 @@ struct SystemT {
         T* dold = p ? d1 : d;
         T* dnew = p ? d : d1;
         const double spmv_bytes = full.bytes_all(32.0);  // src, dold, dnew, q
-        const int nspmv = spmv_blocks(full.nslices);
+        const int nspmv = spmv_blocks(full.nslices * spmv_lanes);
         const int nvec = vec_reduce_blocks(n_active);
         switch (method) {
         case SFA_METHOD_CG:
@@ -355,7 +357,7 @@ struct SystemT {
             run(s, SFA_KERNEL_SPMV, spmv_bytes, [&] {
                 launch_spmv_dir(s, F, full.nslices, full.row0(0), full.nrows_of(0), n_active,
                                 state, p, jac ? z : r, dold, dnew, q, ps, pu, nvec,
-                                (const double*)nullptr, 0, full.width(0));
+                                (const double*)nullptr, 0, full.width(0), spmv_lanes);
             });
             run(s, SFA_KERNEL_UPDATE, (jac ? 64.0 : 48.0) * na, [&] {
                 launch_update(s, jac ? UPD_JACOBI : UPD_PLAIN, state, p, n_active, x, r, dnew, q,
@@ -374,7 +376,7 @@ struct SystemT {
                                 state, p, z, dold, dnew, q, ps,
                                 G > 0 ? pb + upper.group_begin(G - 1) : pb,
                                 G > 0 ? upper.group_count(G - 1) : 0, grp, G > 0 ? G - 1 : 0,
-                                full.width(0));
+                                full.width(0), spmv_lanes);
             });
             forward_sweep(s, lower, FWD_UPDATE, p, dnew);
             backward_sweep(s, 1, dnew);
@@ -647,7 +649,7 @@ struct SystemT {
         launch_fill(s, srcbuf + n, T(0), 1);
         T* out = device_pointers ? dest : io;
         launch_apply(s, apply_part.view(), apply_part.nslices, apply_part.row0(0), apply_part.nrows_of(0),
-                     srcbuf, out, T(scaling));
+                     srcbuf, out, T(scaling), spmv_lanes);
         if (!device_pointers)
             SFA_HIP(hipMemcpyAsync(dest, io, n * sizeof(T), hipMemcpyDeviceToHost, s));
         SFA_HIP(hipStreamSynchronize(s));
@@ -722,6 +724,107 @@ struct SystemT {
         }
     }
 
+    bool has_spmv() const
+    {
+        return method == SFA_METHOD_CG || method == SFA_METHOD_DIAGONAL ||
+               method == SFA_METHOD_SSOR || method == SFA_METHOD_RAINBOW;
+    }
+
+    void set_spmv(int lanes)
+    {
+        if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8)
+            throw Error{SFA_ERR_INVALID, "lanes per row must be 1, 2, 4 or 8"};
+        if (lanes == spmv_lanes) return;
+        spmv_lanes = lanes;
+        if (chunk_exec) SFA_HIP(hipGraphExecDestroy(chunk_exec));  // recorded with the old mapping
+        chunk_exec = nullptr;
+        chunk_iters = 0;
+    }
+
+    // Times each mapping on the iteration's SpMV kernel, as it runs in a
+    // solve (a fold, two gathers per entry), against a scratch state whose
+    // loop never ends, and installs the fastest.
+    void tune_spmv(sfa_spmv_tuning* out)
+    {
+        sfa_spmv_tuning t{};
+        for (int bk = 0; bk < kSpmvBuckets; bk++) t.rows[bk] = full.bucket_rows[bk];
+        auto report = [&](int lanes) {
+            for (int bk = 0; bk < kSpmvBuckets; bk++) t.lanes_per_row[bk] = t.rows[bk] ? lanes : 0;
+        };
+        if (!has_spmv() || full.nslices == 0) {
+            report(1);
+            *out = t;
+            return;
+        }
+        const int G = sweeps ? upper.ngroups() : 0;
+        const int nfold = G > 0 ? upper.group_count(G - 1) : vec_reduce_blocks(n_active);
+        const int ngroups = G > 0 ? G - 1 : 0;
+        const int nscratch = std::max({kSpmvMaxBlocks, nfold, ngroups, 1});
+        SolveState* ts = dev_alloc<SolveState>(1);
+        double* th = dev_alloc<double>(2);
+        double* tz = dev_alloc<double>(nscratch);  // zero partials to fold
+        double* tp = dev_alloc<double>(kSpmvMaxBlocks);
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        auto cleanup = [&] {
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            for (void* v : {(void*)ts, (void*)th, (void*)tz, (void*)tp}) dev_free(v);
+        };
+        try {
+            SolveState h{};
+            h.tol2 = -1.;  // never converged
+            h.maxK = INT_MAX;
+            h.fold[0] = 1;
+            h.zrp[0] = 1.;
+            h.hist = th;
+            SFA_HIP(hipMemcpyAsync(ts, &h, sizeof(h), hipMemcpyHostToDevice, stream));
+            launch_fill<double>(stream, tz, 0., nscratch);
+            SFA_HIP(hipEventCreate(&e0));
+            SFA_HIP(hipEventCreate(&e1));
+            const SellViewT<T> F = full.view();
+            auto once = [&](int lanes) {
+                launch_spmv_dir(stream, F, full.nslices, full.row0(0), full.nrows_of(0), n_active,
+                                ts, 0, z, d, d1, q, tp, tz, nfold, tz, ngroups, full.width(0),
+                                lanes);
+            };
+            const int cand[SFA_SPMV_MAPPINGS] = {1, 2, 4, 8};
+            const int main = full.main_bucket();
+            constexpr int kReps = 20;
+            int best = 1;
+            double best_us = 0.;
+            for (int i = 0; i < SFA_SPMV_MAPPINGS; i++) {
+                const int lanes = cand[i];
+                for (int w = 0; w < 3; w++) once(lanes);
+                float ms_min = 0.f;
+                for (int rep = 0; rep < 3; rep++) {
+                    SFA_HIP(hipEventRecord(e0, stream));
+                    for (int k = 0; k < kReps; k++) once(lanes);
+                    SFA_HIP(hipEventRecord(e1, stream));
+                    SFA_HIP(hipEventSynchronize(e1));
+                    float ms = 0.f;
+                    SFA_HIP(hipEventElapsedTime(&ms, e0, e1));
+                    if (rep == 0 || ms < ms_min) ms_min = ms;
+                }
+                SFA_HIP(hipGetLastError());
+                const double us = 1000.0 * ms_min / kReps;
+                t.us[main][i] = us;
+                if (i == 0 || us < best_us) {
+                    best = lanes;
+                    best_us = us;
+                }
+            }
+            SFA_HIP(hipStreamSynchronize(stream));
+            cleanup();
+            set_spmv(best);
+            report(best);
+            *out = t;
+        } catch (...) {
+            (void)hipStreamSynchronize(stream);
+            cleanup();
+            throw;
+        }
+    }
+
     void query(sfa_system_stats* st) const
     {
         sfa_system_stats o{};
@@ -746,6 +849,7 @@ struct SystemT {
         o.bytes_spmv = spmv;
         o.bytes_per_iteration = it;
         o.kernels_per_iteration = kernels_per_iteration();
+        o.spmv_lanes_per_row = spmv_lanes;
         *st = o;
     }
 };
@@ -973,6 +1077,28 @@ int sfa_system_profile(sfa_system sys, double* x, const double* b, double* resid
     return sfa::guarded([&] {
         if (!sys || !x || !b || !residual || !out) throw Error{SFA_ERR_INVALID, "null argument"};
         sys->sys.profile(x, b, residual, precision, device_pointers, out, info);
+    });
+}
+
+int sfa_system_tune_spmv(sfa_system sys, int fp32_inner, sfa_spmv_tuning* out)
+{
+    return sfa::guarded([&] {
+        if (!sys || !out) throw Error{SFA_ERR_INVALID, "null argument"};
+        if (fp32_inner) {
+            if (!sys->sys.inner) throw Error{SFA_ERR_INVALID, "the system has no FP32 inner system"};
+            sys->sys.inner->tune_spmv(out);
+        } else {
+            sys->sys.tune_spmv(out);
+        }
+    });
+}
+
+int sfa_system_set_spmv(sfa_system sys, int lanes_per_row)
+{
+    return sfa::guarded([&] {
+        if (!sys) throw Error{SFA_ERR_INVALID, "null system"};
+        sys->sys.set_spmv(lanes_per_row);
+        if (sys->sys.inner) sys->sys.inner->set_spmv(lanes_per_row);
     });
 }
 

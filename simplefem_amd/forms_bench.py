@@ -52,6 +52,12 @@ def hcurl_mixed(args):
     col = (rb.num_colors, rb.F, rb.B, rb.R)
     log(f"hcurl system n={s.n} nnz={s.nnz} colours={rb.num_colors} ({time.time() - t0:.1f}s)")
     sysd = capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, d, 1.0, *col, fp32_inner=True)
+    if args.spmv_lanes:
+        sysd.set_spmv(args.spmv_lanes)
+        tuning = {"lanes_per_row": args.spmv_lanes, "tuned": False}
+    else:
+        tuning = {"fp64": sysd.tune_spmv(), "fp32_inner": sysd.tune_spmv(fp32_inner=True)}
+        log(f"spmv tuning: {tuning}")
     bt = DeviceArray.from_host(b)
     x = DeviceArray(s.n)
     r = DeviceArray(s.n)
@@ -96,6 +102,7 @@ def hcurl_mixed(args):
                      "inner_iterations": mi["inner_iterations"],
                      "true_relative_residual": resm},
            "mixed_vs_fp64_solution_rel_error": err,
+           "spmv_mapping": tuning,
            "speedup_mixed_over_fp64": t64 / tmx}
     return _base(args, round(1.0 / tmx, 3), "solves/s", round(tmx * 1e3, 3), cfg, "f32+f64")
 
@@ -115,6 +122,12 @@ def batched(args):
                 s = j.system
                 mids.append(eng.add_matrix(s.n, s.A, s.C, s.V, s.diagonal(), 1.0, rb.num_colors,
                                            rb.F, rb.B, rb.R))
+            if args.spmv_lanes:
+                eng.set_spmv([args.spmv_lanes] * capi.SPMV_BUCKETS)
+                tuning = {"lanes_per_row": args.spmv_lanes, "tuned": False}
+            else:
+                tuning = eng.tune_spmv()
+                log(f"{'mixed' if mixed else 'fp64'} engine spmv tuning: {tuning}")
             synchronize()
             t1 = time.perf_counter()
             slots = []
@@ -134,21 +147,24 @@ def batched(args):
                     last = now
             synchronize()
             elapsed = time.perf_counter() - t1
-            worst = 0.0
+            worst = 0.0  # largest true relative residual over a sample of the solves
             iters = 0
             ok = 0
-            for slot, j, b in slots:
+            check_every = max(1, len(slots) // 16)
+            for k, (slot, j, b) in enumerate(slots):
                 x, info = eng.collect(slot, j.system.n)
                 iters += info["iterations"]
                 ok += info["success"]
-            # spot-check the true residual of a few solves
-            for slot, j, b in slots[:: max(1, len(slots) // 8)]:
-                pass
-            return elapsed, t_sub, iters, ok, eng.memory()
+                if k % check_every == 0:
+                    s = j.system
+                    res = b - capi.csr_apply(s.n, s.n, s.A, s.C, s.V, x)
+                    res[s.dirichlet] = 0.0
+                    worst = max(worst, float(np.linalg.norm(res) / np.linalg.norm(b)))
+            return elapsed, t_sub, iters, ok, eng.memory(), tuning, worst
 
-    e_mx, sub_mx, it_mx, ok_mx, mem = run_engine(True)
+    e_mx, sub_mx, it_mx, ok_mx, mem, tune_mx, res_mx = run_engine(True)
     log(f"batched mixed: {nrhs} solves in {e_mx:.3f}s ({ok_mx} converged)")
-    e_64, sub_64, it_64, ok_64, _ = run_engine(False)
+    e_64, sub_64, it_64, ok_64, _, tune_64, res_64 = run_engine(False)
     log(f"batched fp64: {nrhs} solves in {e_64:.3f}s ({ok_64} converged)")
     # one-at-a-time FP64 path (device-resident System per matrix), on a sample
     sample = jobs[:: max(1, len(jobs) // 32)]
@@ -159,6 +175,10 @@ def batched(args):
         rb = capi.Rainbow(s.n, s.A, s.C)
         with capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, s.diagonal(), 1.0,
                          rb.num_colors, rb.F, rb.B, rb.R) as sysd:
+            if args.spmv_lanes:
+                sysd.set_spmv(args.spmv_lanes)
+            else:
+                sysd.tune_spmv()
             bt = DeviceArray.from_host(j.rhs[0])
             x = DeviceArray(s.n)
             r = DeviceArray(s.n)
@@ -180,8 +200,11 @@ def batched(args):
                        "refinement to rtol 1e-10",
            "systems": len(jobs), "solves": nrhs, "rows_total": int(sum(j.system.n for j in jobs)),
            "mixed": {"solves_per_s": nrhs / e_mx, "converged": ok_mx, "inner_iterations": it_mx,
-                     "submit_s": sub_mx},
-           "fp64_batched": {"solves_per_s": nrhs / e_64, "converged": ok_64, "iterations": it_64},
+                     "submit_s": sub_mx, "max_sampled_true_relative_residual": res_mx,
+                     "spmv_mapping": tune_mx},
+           "fp64_batched": {"solves_per_s": nrhs / e_64, "converged": ok_64, "iterations": it_64,
+                            "max_sampled_true_relative_residual": res_64,
+                            "spmv_mapping": tune_64},
            "fp64_one_at_a_time": {"solves_per_s": seq_rate, "sample_solves": n_seq},
            "speedup_mixed_batched_over_fp64_one_at_a_time": (nrhs / e_mx) / seq_rate,
            "device_bytes": mem}
