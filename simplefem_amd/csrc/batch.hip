@@ -415,22 +415,32 @@ __device__ int next_units(BSlot<T>& S, const BMat<T>& M)
     return S.kind == PH_DONE ? 0 : units;
 }
 
+// The slot s with start[s] <= u < start[s+1] (binary search, start in LDS).
+__device__ inline int unit_owner(const int* start, int ns, int u)
+{
+    int lo = 0, hi = ns;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (start[mid] <= u) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(kUnitBlock) k_batch_step(BArgs<T> a)
 {
     __shared__ double ws[kUnitBlock / 64];
+    __shared__ int start[kMaxSlots + 1];  // the step's unit offsets, read once per block
     const BSched* sc = a.sched;
     const int p = (int)(sc->step & 1u);
     const int W = sc->W[p];
     const int ns = sc->nslots;
-    const int* start = sc->start[p];
+    if ((int)blockIdx.x >= W) return;
+    for (int i = threadIdx.x; i <= ns; i += kUnitBlock) start[i] = sc->start[p][i];
+    __syncthreads();
     for (int u = blockIdx.x; u < W; u += gridDim.x) {
-        int lo = 0, hi = ns;  // the slot s with start[s] <= u < start[s+1]
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (start[mid] <= u) lo = mid;
-            else hi = mid;
-        }
+        const int lo = unit_owner(start, ns, u);
         BSlot<T>& S = a.slots[lo];
         const int j = u - start[lo];
         const double part = block_sum(run_unit(S, a.mats[S.mat], S.kind, S.group, j), ws);
@@ -504,7 +514,7 @@ __global__ void __launch_bounds__(kPrefixBlock) k_batch_prefix(BArgs<T> a)
     }
 }
 
-// SpMV autotuning probe: the PH_SPMV units of the matrices mats[0..nm) with
+// SpMV autotuning probe (nm <= kMaxSlots): the PH_SPMV units of the matrices mats[0..nm) with
 // K lanes per row, grid-stride as k_batch_step runs them (unit u belongs to
 // the matrix m with ustart[m] <= u < ustart[m+1]); d and q are scratch
 // vectors of at least n+1 entries for every listed matrix.
@@ -515,16 +525,15 @@ __global__ void __launch_bounds__(kUnitBlock) k_batch_spmv_probe(const BMat<T>* 
                                                                  double* partials)
 {
     __shared__ double ws[kUnitBlock / 64];
+    __shared__ int start[kMaxSlots + 1];
     const int W = ustart[nm];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if ((int)blockIdx.x >= W) return;
+    for (int i = threadIdx.x; i <= nm; i += kUnitBlock) start[i] = ustart[i];
+    __syncthreads();
     for (int u = blockIdx.x; u < W; u += gridDim.x) {
-        int lo = 0, hi = nm;
-        while (hi - lo > 1) {
-            const int mid = (lo + hi) >> 1;
-            if (ustart[mid] <= u) lo = mid;
-            else hi = mid;
-        }
-        const int j = u - ustart[lo];
+        const int lo = unit_owner(start, nm, u);
+        const int j = u - start[lo];
         const double part = block_sum(spmv_task<K>(mats[lo], d, q, j * kUnitSlices + w, lane), ws);
         if (threadIdx.x == 0) partials[u] = part;
     }
@@ -606,6 +615,7 @@ struct Engine {
     BSched* h_sched = nullptr;                // pinned
     unsigned step = 0;
     hipGraphExec_t graph = nullptr;
+    std::unique_ptr<GraphRecorder> graph_rec;  // the recorded graph and its arguments
     BMat<T>* graph_mats = nullptr;            // d_mats the graph was captured with
     int spmv_table[kSpmvBuckets] = {1, 1, 1, 1};  // lanes per row of the SpMV per bucket
 
@@ -616,6 +626,7 @@ struct Engine {
         if (stream) (void)hipStreamSynchronize(stream);
         if (graph) (void)hipGraphExecDestroy(graph);
         graph = nullptr;
+        graph_rec.reset();
         for (auto& v : vecs) v.release();
         for (auto& m : mats) m.release();
         dev_free(d_mats);
@@ -822,10 +833,11 @@ struct Engine {
             SFA_HIP(hipEventCreate(&e1));
             const int cand[SFA_SPMV_MAPPINGS] = {1, 2, 4, 8};
             for (int b = 0; b < kSpmvBuckets; b++) {
-                const int nm = (int)ids[b].size();
+                // at most kMaxSlots matrices per probe (its unit table lives in LDS)
+                const int nm = std::min((int)ids[b].size(), kMaxSlots);
                 if (nm == 0) continue;
                 std::vector<BMat<T>> hm;
-                for (int id : ids[b]) hm.push_back(mats[id].bm);
+                for (int k = 0; k < nm; k++) hm.push_back(mats[ids[b][k]].bm);
                 dev_free(dm);
                 dev_free(dstart);
                 dm = nullptr;
@@ -991,15 +1003,17 @@ struct Engine {
         if (graph && graph_mats == d_mats) return;
         if (graph) SFA_HIP(hipGraphExecDestroy(graph));
         graph = nullptr;
+        graph_rec.reset();
         const BArgs<T> a = args();
-        GraphRecorder rec;
+        auto rec = std::make_unique<GraphRecorder>();
         for (int i = 0; i < kGraphSteps; i++) {
             launch(k_batch_step<T>, dim3(kStepBlocks), dim3(kUnitBlock), stream, a);
             launch(k_batch_fold<T>, dim3(max_slots), dim3(kUnitBlock), stream, a);
             launch(k_batch_prefix<T>, dim3(1), dim3(kPrefixBlock), stream, a);
         }
         hipGraphExec_t exec = nullptr;
-        const hipError_t e = rec.instantiate(&exec);
+        const hipError_t e = rec->instantiate(&exec);
+        graph_rec = std::move(rec);
         if (e != hipSuccess)
             throw Error{SFA_ERR_HIP, std::string("batch step graph: ") + hipGetErrorString(e)};
         graph = exec;

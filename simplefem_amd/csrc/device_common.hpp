@@ -209,13 +209,14 @@ __device__ inline T sell_finish(const SellViewT<T>& A, const SellChunk<T, U>& R,
 
 // ---- K lanes per row (K = 2, 4, 8) over the same SELL-64 storage.
 // A wavefront covers rows q*64/K .. (q+1)*64/K - 1 of a slice (its part q of
-// K); lane t = lane % K of a row reads the row's entries t, t+K, t+2K, ...,
-// so one load instruction reads 64/K consecutive rows at K consecutive entry
-// positions: K contiguous runs of 64/K elements.  A lane holds at most U
-// entries per chunk, so a row of w entries is one chunk when w <= U*K.  The
-// products are added in the row's entry order, each passed from the lane
-// that formed it, so every lane of the row forms the same sum, bit for bit
-// the sum one lane forms over the row.
+// K).  A row's entries are taken in chunks of K*U: in chunk c, lane t of the
+// row holds entries c*K*U + t*U + u, u < U, so one load instruction reads
+// 64/K consecutive rows at K entry positions (K contiguous runs of 64/K
+// elements).  Inside a chunk the lanes add their products into the row's sum
+// in turn (lane 0's U products, then lane 1's, ...), the running sum passed
+// from lane to lane, so the sum runs over the row's entries in order: every
+// lane of the row ends with the same bits, the sum one lane forms over the
+// row.  Per chunk: U loads per lane, K passes of the sum.
 
 // Row inside the slice of a lane of part q.
 template <int K>
@@ -224,6 +225,7 @@ __device__ inline int sellk_slice_row(int q, int lane)
     return q * (kWave / K) + lane / K;
 }
 
+// Loads chunk 0 (R.e: the lane's first entry of chunk 0).
 template <int K, typename T, int U>
 __device__ inline void sellk_prefetch(const SellViewT<T>& A, int slice, int local, int gbase,
                                       int gw, int q, int lane, SellChunk<T, U>& R)
@@ -237,12 +239,12 @@ __device__ inline void sellk_prefetch(const SellViewT<T>& A, int slice, int loca
         R.w = (A.off[slice + 1] - o0) / kWave;
     }
     const int t = lane % K;
-    R.e = o0 + sellk_slice_row<K>(q, lane) + t * kWave;
+    R.e = o0 + sellk_slice_row<K>(q, lane) + t * U * kWave;
 #pragma unroll
     for (int u = 0; u < U; u++) {
-        if (u * K + t < R.w) {
-            R.c[u] = (unsigned)__builtin_nontemporal_load(&A.cols[R.e + u * K * kWave]);
-            R.v[u] = __builtin_nontemporal_load(&A.vals[R.e + u * K * kWave]);
+        if (t * U + u < R.w) {
+            R.c[u] = (unsigned)__builtin_nontemporal_load(&A.cols[R.e + u * kWave]);
+            R.v[u] = __builtin_nontemporal_load(&A.vals[R.e + u * kWave]);
         } else {
             R.c[u] = 0u;
             R.v[u] = T(0);
@@ -256,43 +258,63 @@ __device__ inline T sell_acc(T acc, T x)
     return SUB ? acc - x : acc + x;
 }
 
-// acc (+|-)= the row's products, in entry order (see above).
+// Adds the products pr[u] of chunk c (entries c*K*U + t*U + u) into acc in
+// entry order, lane by lane.
+template <int K, bool SUB, typename T, int U>
+__device__ inline T sellk_chain(const T (&pr)[U], int c, int w, int t, int base, T acc)
+{
+#pragma unroll
+    for (int r = 0; r < K; r++) {
+        const int e0 = (c * K + r) * U;  // first entry of lane r in this chunk
+        if (e0 >= w) break;               // wave-uniform
+        if (t == r) {
+#pragma unroll
+            for (int u = 0; u < U; u++)
+                if (e0 + u < w) acc = sell_acc<SUB>(acc, pr[u]);
+        }
+        acc = __shfl(acc, base + r, kWave);
+    }
+    return acc;
+}
+
+// acc (+|-)= prod(v, c) over the row's entries, in order (see above); chunk 0
+// comes prefetched in R.
+template <int K, bool SUB, typename T, int U, class Prod>
+__device__ inline T sellk_finish_with(const SellViewT<T>& A, const SellChunk<T, U>& R, Prod prod,
+                                      T acc, int lane)
+{
+    const int t = lane % K;
+    const int base = lane - t;
+    {
+        T pr[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) pr[u] = t * U + u < R.w ? prod(R.v[u], R.c[u]) : T(0);
+        acc = sellk_chain<K, SUB>(pr, 0, R.w, t, base, acc);
+    }
+    for (int c = 1; c * K * U < R.w; c++) {
+        const int e = R.e + c * K * U * kWave;
+        const int first = (c * K + t) * U;
+        unsigned cc[U];
+        T vv[U], pr[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            if (first + u < R.w) {
+                cc[u] = (unsigned)__builtin_nontemporal_load(&A.cols[e + u * kWave]);
+                vv[u] = __builtin_nontemporal_load(&A.vals[e + u * kWave]);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) pr[u] = first + u < R.w ? prod(vv[u], cc[u]) : T(0);
+        acc = sellk_chain<K, SUB>(pr, c, R.w, t, base, acc);
+    }
+    return acc;
+}
+
 template <int K, bool SUB, typename T, int U, class Get>
 __device__ inline T sellk_finish(const SellViewT<T>& A, const SellChunk<T, U>& R, Get get, T acc,
                                  int lane)
 {
-    const int t = lane % K;
-    const int base = lane - t;
-    T pr[U];
-#pragma unroll
-    for (int u = 0; u < U; u++) pr[u] = u * K + t < R.w ? R.v[u] * get(R.c[u]) : T(0);
-#pragma unroll
-    for (int u = 0; u < U; u++) {
-#pragma unroll
-        for (int tt = 0; tt < K; tt++)
-            if (u * K + tt < R.w) acc = sell_acc<SUB>(acc, __shfl(pr[u], base + tt, kWave));
-    }
-    constexpr int V = 8;
-    for (int k0 = U; k0 * K < R.w; k0 += V) {
-        T h[V];
-#pragma unroll
-        for (int u = 0; u < V; u++) {
-            h[u] = T(0);
-            if ((k0 + u) * K + t < R.w) {
-                const int e = R.e + (k0 + u) * K * kWave;
-                const unsigned c = (unsigned)__builtin_nontemporal_load(&A.cols[e]);
-                const T v = __builtin_nontemporal_load(&A.vals[e]);
-                h[u] = v * get(c);
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < V; u++) {
-#pragma unroll
-            for (int tt = 0; tt < K; tt++)
-                if ((k0 + u) * K + tt < R.w) acc = sell_acc<SUB>(acc, __shfl(h[u], base + tt, kWave));
-        }
-    }
-    return acc;
+    return sellk_finish_with<K, SUB>(A, R, [&](T v, unsigned c) { return v * get(c); }, acc, lane);
 }
 
 // One row of a part with K lanes per row (K = 1: sell_row), offsets read

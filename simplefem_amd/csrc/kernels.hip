@@ -224,8 +224,8 @@ __global__ void __launch_bounds__(kWave) k_result_finish(SolveState* st, const d
 
 // MatrixCSR::apply (sparse/matcsr.cpp:216-244): dest = 0; dest += scaling*V*x,
 // entries in order.  K lanes per row: one wavefront per task (slice
-// task/K, part task%K of 64/K rows), products passed lane to lane in entry
-// order, so every K gives the same bits.
+// task/K, part task%K of 64/K rows), the row's sum passed lane to lane in
+// entry order, so every K gives the same bits.
 template <typename T, int K>
 __global__ void __launch_bounds__(kSellBlock) k_apply(SellViewT<T> A, int nslices, int row0,
                                                       int nrows, const T* src, T* dest,
@@ -243,29 +243,11 @@ __global__ void __launch_bounds__(kSellBlock) k_apply(SellViewT<T> A, int nslice
         const int beg = A.off[slice] + lane, end = A.off[slice + 1];
         for (int e = beg; e < end; e += kWave) acc += scaling * A.vals[e] * src[A.cols[e]];
     } else {
-        const int t = lane % K, base = lane - t;
-        const int o0 = A.off[slice];
-        const int w = (A.off[slice + 1] - o0) / kWave;
-        const int e0 = o0 + i + t * kWave;
-        constexpr int U = 8;
-        for (int u0 = 0; u0 * K < w; u0 += U) {
-            T pr[U];
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-                pr[u] = T(0);
-                if ((u0 + u) * K + t < w) {
-                    const int e = e0 + (u0 + u) * K * kWave;
-                    pr[u] = scaling * A.vals[e] * src[A.cols[e]];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < U; u++) {
-#pragma unroll
-                for (int tt = 0; tt < K; tt++)
-                    if ((u0 + u) * K + tt < w) acc += __shfl(pr[u], base + tt, kWave);
-            }
-        }
-        if (t != 0) return;
+        SellChunk<T, 8> R;
+        sellk_prefetch<K>(A, slice, slice, 0, -1, part, lane, R);
+        acc = sellk_finish_with<K, false>(
+            A, R, [&](T v, unsigned c) { return scaling * v * src[c]; }, acc, lane);
+        if (lane % K != 0) return;
     }
     dest[row] = acc;
 }

@@ -90,6 +90,7 @@ struct SystemT {
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
     hipGraphExec_t chunk_exec = nullptr;
+    std::unique_ptr<GraphRecorder> chunk_graph;  // the recorded graph and its arguments, kept with the exec
     int chunk_iters = 0;
     Profiler* prof = nullptr;  // set only while sfa_system_profile runs
     // FP32 copy of the plan for the mixed-precision solve (T = double only)
@@ -118,6 +119,7 @@ struct SystemT {
         inner.reset();
         if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
         chunk_exec = nullptr;
+        chunk_graph.reset();
         full.release();
         lower.release();
         upper.release();
@@ -423,9 +425,11 @@ struct SystemT {
         }
         // Recorded node by node (no stream capture), so other threads may use
         // the runtime meanwhile.
-        GraphRecorder rec;
+        chunk_graph.reset();
+        auto rec = std::make_unique<GraphRecorder>();
         for (int i = 0; i < iters; i++) iteration(own_stream, i & 1);
-        const hipError_t e = rec.instantiate(&chunk_exec);
+        const hipError_t e = rec->instantiate(&chunk_exec);
+        chunk_graph = std::move(rec);
         if (e != hipSuccess) {
             chunk_exec = nullptr;
             throw Error{SFA_ERR_HIP, std::string("iteration graph: ") + hipGetErrorString(e)};
@@ -738,6 +742,7 @@ struct SystemT {
         spmv_lanes = lanes;
         if (chunk_exec) SFA_HIP(hipGraphExecDestroy(chunk_exec));  // recorded with the old mapping
         chunk_exec = nullptr;
+        chunk_graph.reset();
         chunk_iters = 0;
     }
 
