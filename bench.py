@@ -97,6 +97,8 @@ def main():
     ap.add_argument("--precision", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--single-group-sweeps", action="store_true",
+                    help="one sweep group per launch (no paired colours)")
     ap.add_argument("--spmv-lanes", type=int, default=0, choices=[0, 1, 2, 4, 8],
                     help="SpMV lanes per row; 0: autotune on the system (default)")
     ap.add_argument("--config", default="poisson3d",
@@ -149,7 +151,7 @@ def main():
         f"built in {time.time() - t0:.1f}s")
 
     sysd = capi.System(capi.METHOD_RAINBOW, system.n, system.A, system.C, system.V, d,
-                       args.omega, *colouring)
+                       args.omega, *colouring, single_group_sweeps=args.single_group_sweeps)
     if args.spmv_lanes:
         sysd.set_spmv(args.spmv_lanes)
         tuning = {"lanes_per_row": args.spmv_lanes, "tuned": False}

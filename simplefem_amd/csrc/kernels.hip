@@ -403,9 +403,10 @@ __global__ void __launch_bounds__(kSellBlock) k_forward(SellViewT<T> L, int slic
 
 // Backward sweep of one colour / level, solver/sparsesolver.cpp:543-568
 // (restart) and :667-692, :1263-1303 (iteration): the scaling of m fused in.
-// With reduce, the z.r partial of each block goes to gpart[slice], and one
-// extra block (logical block nslices) sums the previous group's partials into
-// *prev_sum, so z.r is summed per group and over the groups in launch order.
+// With reduce, the z.r partial of each block goes to gpart[slice], and extra
+// blocks (logical blocks nslices, nslices+1) sum earlier groups' partials
+// into *prev_sum (and *prev_sumb), so z.r is summed per group and over the
+// groups in launch order.
 // With defer_update the group's x and r updates of the iteration (x += a d,
 // r -= a q) are made here, first (its forward rows were run merged into the
 // next group's kernel, which leaves them).
@@ -415,7 +416,9 @@ __global__ void __launch_bounds__(kSellBlock) k_backward(SellViewT<T> U, int sli
                                                          int reduce, T* r, const T* m,
                                                          T* z, const T* diag, T omega, T c2,
                                                          double* gpart, int prev0, int prevn,
-                                                         double* prev_sum, int gbase, int gw,
+                                                         double* prev_sum, int prev0b,
+                                                         int prevnb, double* prev_sumb,
+                                                         int gbase, int gw,
                                                          int defer_update, T* x, const T* d,
                                                          const T* q)
 {
@@ -448,6 +451,11 @@ __global__ void __launch_bounds__(kSellBlock) k_backward(SellViewT<T> U, int sli
         if (lane == 0) *prev_sum = t;
         return;
     }
+    if (reduce && prev_sumb && b == nslices + 1) {
+        const double t = wave_fold(gpart + prev0b, prevnb);
+        if (lane == 0) *prev_sumb = t;
+        return;
+    }
     double part = 0.;
     if (row >= 0) {
         T t = mm;
@@ -459,6 +467,291 @@ __global__ void __launch_bounds__(kSellBlock) k_backward(SellViewT<T> U, int sli
         part = (double)(zz * rr);
     }
     if (reduce && has) block_partial<kSellBlock>(part, &gpart[slice]);
+}
+
+// ------------------------------------------------------------------ sweep pairs
+
+constexpr int kPairBlock = kPairWaves * kWave;
+
+// m of row c of group A as A's wavefront forms it in the iteration
+// (k_forward, FWD_UPDATE; k_forward_pair): (r - a q - L m) * w / diag, the
+// entries in order, from the m of earlier groups.
+template <typename T>
+__device__ inline T pair_forward_m(const SellViewT<T>& L, const SweepGroup& A, unsigned c,
+                                   T alpha, const T* r, const T* q, const T* m, const T* diag,
+                                   T omega)
+{
+    const int pos = (int)(c - (unsigned)A.row0);
+    const int local = pos / kWave;
+    SellChunk<T> R;
+    sell_prefetch(L, A.slice0 + local, local, A.gbase, A.gw, pos % kWave, R);
+    const T rc = r[c] - alpha * q[c];
+    const T acc = sell_finish<true>(L, R, [&](unsigned k) { return m[k]; }, rc);
+    return acc * omega / diag[c];
+}
+
+// z of row c of group A as A's wavefront forms it (k_backward): (m*diag*c2
+// - U z) * w / diag, from the z of earlier groups.
+template <typename T>
+__device__ inline T pair_backward_z(const SellViewT<T>& U, const SweepGroup& A, unsigned c,
+                                    const T* m, const T* z, const T* diag, T omega, T c2)
+{
+    const int pos = (int)(c - (unsigned)A.row0);
+    const int local = pos / kWave;
+    SellChunk<T> R;
+    sell_prefetch(U, A.slice0 + local, local, A.gbase, A.gw, pos % kWave, R);
+    const T dg = diag[c];
+    T t = m[c];
+    t *= dg;
+    t *= c2;
+    const T acc = sell_finish<true>(U, R, [&](unsigned k) { return z[k]; }, t);
+    return acc * omega / dg;
+}
+
+// Wavefront 0 of a B block: the entries of its row that read A rows.  Their
+// columns go to acol[k][lane] (k in entry order), their count to cnt[lane];
+// returns the entries' bit mask.
+template <typename T, int U>
+__device__ inline unsigned pair_collect(const SellChunk<T, U>& R, const SweepGroup& A, int row,
+                                        int lane, unsigned (*acol)[kWave], int* cnt)
+{
+    unsigned mask = 0u;
+    int k = 0;
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        if (u < R.w && R.c[u] - (unsigned)A.row0 < (unsigned)A.nrows) {
+            mask |= 1u << u;
+            if (k < kPairRefs) acol[k][lane] = R.c[u];
+            k++;
+        }
+    }
+    cnt[lane] = row >= 0 ? min(k, kPairRefs) : 0;
+    return mask;
+}
+
+// Forward sweep of groups A and B in one launch (see launch_forward_pair).
+// Blocks [0, ceil(A.nslices/4)): one A slice per wavefront.  Then one block
+// per B slice: wavefront 0 owns the slice's rows and lists the A rows they
+// read; all four wavefronts form those rows' m into LDS; wavefront 0 then
+// runs the rows, taking A's m from LDS and the other m from memory, in entry
+// order.  A's x and r are left for A's backward kernel: B's blocks read A's r
+// here, before any update.
+template <typename T>
+__global__ void __launch_bounds__(kPairBlock) k_forward_pair(SellViewT<T> L, SweepGroup A,
+                                                             SweepGroup B, SolveState* st, T* x,
+                                                             T* r, const T* d, const T* q, T* m,
+                                                             const T* diag, T omega, T* zb, T c2,
+                                                             double* zpart)
+{
+    __shared__ unsigned acol[kPairRefs][kWave];
+    __shared__ T aval[kPairRefs][kWave];
+    __shared__ int acnt[kWave];
+    const int lane = threadIdx.x % kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int nab = (A.nslices + kPairWaves - 1) / kPairWaves;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    if (b < nab) {
+        const int local = b * kPairWaves + wave;
+        if (local >= A.nslices) return;
+        const int slice = A.slice0 + local;
+        const int row = group_row(L, slice, local, lane, A.row0, A.nrows);
+        const int ri = row >= 0 ? row : L.zero;
+        const T rr = r[ri];
+        const T qq = q[ri];
+        const T dg = diag[ri];
+        SellChunk<T> R;
+        sell_prefetch(L, slice, local, A.gbase, A.gw, lane, R);
+        if (st->done) return;
+        const T alpha = (T)st->alpha;
+        if (row < 0) return;
+        const T rn = rr - alpha * qq;
+        const T acc = sell_finish<true>(L, R, [&](unsigned k) { return m[k]; }, rn);
+        m[row] = acc * omega / dg;
+        return;
+    }
+    const int local = b - nab;
+    const int slice = B.slice0 + local;
+    int row = -1;
+    T rr = T(0), dg = T(0), xx = T(0), dd = T(0), qq = T(0);
+    SellChunk<T> R;
+    if (wave == 0) {
+        row = group_row(L, slice, local, lane, B.row0, B.nrows);
+        const int ri = row >= 0 ? row : L.zero;
+        rr = r[ri];
+        dg = diag[ri];
+        xx = x[ri];
+        dd = d[ri];
+        qq = q[ri];
+        sell_prefetch(L, slice, local, B.gbase, B.gw, lane, R);
+    }
+    if (__syncthreads_or(st->done)) return;
+    const T alpha = (T)st->alpha;
+    unsigned mask = 0u;
+    if (wave == 0) mask = pair_collect(R, A, row, lane, acol, acnt);
+    __syncthreads();
+    T g[kPairWidth];
+    if (wave == 0) {
+#pragma unroll
+        for (int u = 0; u < kPairWidth; u++)
+            g[u] = u < R.w && !(mask >> u & 1u) ? m[R.c[u]] : T(0);
+    }
+    const int cnt = acnt[lane];
+#pragma unroll
+    for (int k = wave; k < kPairRefs; k += kPairWaves)
+        if (k < cnt) aval[k][lane] = pair_forward_m(L, A, acol[k][lane], alpha, r, q, m, diag, omega);
+    __syncthreads();
+    if (wave != 0) return;
+    double part = 0.;
+    if (row >= 0) {
+        x[row] = xx + alpha * dd;
+        const T rn = rr - alpha * qq;
+        r[row] = rn;
+        int k = 0;
+#pragma unroll
+        for (int u = 0; u < kPairWidth; u++)
+            if (mask >> u & 1u) g[u] = aval[k++][lane];
+        T acc = rn;
+#pragma unroll
+        for (int u = 0; u < kPairWidth; u++)
+            if (u < R.w) acc -= R.v[u] * g[u];
+        const T mm = acc * omega / dg;
+        m[row] = mm;
+        if (zb) {
+            T t = mm;
+            t *= dg;
+            t *= c2;
+            const T zz = t * omega / dg;
+            zb[row] = zz;
+            part = (double)(zz * rn);
+        }
+    }
+    if (zpart) block_partial<kSellBlock>(part, &zpart[local]);
+}
+
+// Backward sweep of groups A and B in one launch, with the z.r partials
+// (see launch_backward_pair); the block layout of k_forward_pair, plus one
+// block that sums earlier groups' partials.
+template <typename T>
+__global__ void __launch_bounds__(kPairBlock) k_backward_pair(
+    SellViewT<T> U, SweepGroup A, SweepGroup B, SolveState* st, T* r, const T* m, T* z,
+    const T* diag, T omega, T c2, double* gpart, int nfold, int fold0a, int foldna,
+    double* fsuma, int fold0b, int foldnb, double* fsumb, int defer_a, int defer_b, T* x,
+    const T* d, const T* q)
+{
+    __shared__ unsigned acol[kPairRefs][kWave];
+    __shared__ T aval[kPairRefs][kWave];
+    __shared__ int acnt[kWave];
+    const int lane = threadIdx.x % kWave;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int nab = (A.nslices + kPairWaves - 1) / kPairWaves;
+    const int b = xcd_block(blockIdx.x, gridDim.x);
+    if (b < nab) {
+        const int local = b * kPairWaves + wave;
+        if (local >= A.nslices) return;
+        const int slice = A.slice0 + local;
+        const int row = group_row(U, slice, local, lane, A.row0, A.nrows);
+        const int ri = row >= 0 ? row : U.zero;
+        const T dg = diag[ri];
+        const T mm = m[ri];
+        T rr = r[ri];
+        T xx = T(0), dd = T(0), qq = T(0);
+        if (defer_a) {
+            xx = x[ri];
+            dd = d[ri];
+            qq = q[ri];
+        }
+        SellChunk<T> R;
+        sell_prefetch(U, slice, local, A.gbase, A.gw, lane, R);
+        if (st->done) return;
+        double part = 0.;
+        if (row >= 0) {
+            if (defer_a) {
+                const T alpha = (T)st->alpha;
+                x[row] = xx + alpha * dd;
+                rr = rr - alpha * qq;
+                r[row] = rr;
+            }
+            T t = mm;
+            t *= dg;
+            t *= c2;
+            const T acc = sell_finish<true>(U, R, [&](unsigned k) { return z[k]; }, t);
+            const T zz = acc * omega / dg;
+            z[row] = zz;
+            part = (double)(zz * rr);
+        }
+        const double ws = wave_sum(part);  // the slice's partial, as k_backward's block forms it
+        if (lane == 0) gpart[slice] = ws;
+        return;
+    }
+    if (b >= nab + B.nslices) {  // sums of earlier groups
+        if (st->done) return;
+        if (wave == 0 && nfold > 0) {
+            const double t = wave_fold(gpart + fold0a, foldna);
+            if (lane == 0) *fsuma = t;
+        } else if (wave == 1 && nfold > 1) {
+            const double t = wave_fold(gpart + fold0b, foldnb);
+            if (lane == 0) *fsumb = t;
+        }
+        return;
+    }
+    const int local = b - nab;
+    const int slice = B.slice0 + local;
+    int row = -1;
+    T dg = T(0), mm = T(0), rr = T(0), xx = T(0), dd = T(0), qq = T(0);
+    SellChunk<T> R;
+    if (wave == 0) {
+        row = group_row(U, slice, local, lane, B.row0, B.nrows);
+        const int ri = row >= 0 ? row : U.zero;
+        dg = diag[ri];
+        mm = m[ri];
+        rr = r[ri];
+        if (defer_b) {
+            xx = x[ri];
+            dd = d[ri];
+            qq = q[ri];
+        }
+        sell_prefetch(U, slice, local, B.gbase, B.gw, lane, R);
+    }
+    if (__syncthreads_or(st->done)) return;
+    unsigned mask = 0u;
+    if (wave == 0) mask = pair_collect(R, A, row, lane, acol, acnt);
+    __syncthreads();
+    T g[kPairWidth];
+    if (wave == 0) {
+#pragma unroll
+        for (int u = 0; u < kPairWidth; u++)
+            g[u] = u < R.w && !(mask >> u & 1u) ? z[R.c[u]] : T(0);
+    }
+    const int cnt = acnt[lane];
+#pragma unroll
+    for (int k = wave; k < kPairRefs; k += kPairWaves)
+        if (k < cnt) aval[k][lane] = pair_backward_z(U, A, acol[k][lane], m, z, diag, omega, c2);
+    __syncthreads();
+    if (wave != 0) return;
+    double part = 0.;
+    if (row >= 0) {
+        if (defer_b) {
+            const T alpha = (T)st->alpha;
+            x[row] = xx + alpha * dd;
+            rr = rr - alpha * qq;
+            r[row] = rr;
+        }
+        int k = 0;
+#pragma unroll
+        for (int u = 0; u < kPairWidth; u++)
+            if (mask >> u & 1u) g[u] = aval[k++][lane];
+        T t = mm;
+        t *= dg;
+        t *= c2;
+        T acc = t;
+#pragma unroll
+        for (int u = 0; u < kPairWidth; u++)
+            if (u < R.w) acc -= R.v[u] * g[u];
+        const T zz = acc * omega / dg;
+        z[row] = zz;
+        part = (double)(zz * rr);
+    }
+    block_partial<kSellBlock>(part, &gpart[slice]);
 }
 
 // Eisenstat triangular solves (solver/sparsesolver.cpp:814-828, 876-908,
@@ -547,6 +840,7 @@ __global__ void __launch_bounds__(kSpmvBlock) k_spmv_dir(SellViewT<T> A, int nsl
                                                          const T* src, const T* dold, T* dnew,
                                                          T* q, double* spart,
                                                          const double* fpart, int nfold,
+                                                         const double* fpartb, int nfoldb,
                                                          const double* gsum, int ngroups,
                                                          int gw)
 {
@@ -585,6 +879,7 @@ __global__ void __launch_bounds__(kSpmvBlock) k_spmv_dir(SellViewT<T> A, int nsl
             double t = 0.;
             for (int g = 0; g < ngroups; g++) t += gsum[g];
             t += wave_fold(fpart, nfold);
+            if (fpartb) t += wave_fold(fpartb, nfoldb);
             if (lane == 0) sh = t;
         }
         __syncthreads();
@@ -817,15 +1112,40 @@ template <typename T>
 void launch_backward(hipStream_t s, SellViewT<T> U, int slice0, int nslices, int row0, int nrows,
                      SolveState* st, int reduce, T* r, const T* m, T* z, const T* diag,
                      T omega, T c2, double* group_partials, int prev0, int prevn,
-                     double* prev_sum, int gbase, int gw, int defer_update, T* x, const T* d,
-                     const T* q)
+                     double* prev_sum, int prev0b, int prevnb, double* prev_sumb, int gbase,
+                     int gw, int defer_update, T* x, const T* d, const T* q)
 {
-    // one more block sums the previous group's partials
-    const int grid = nslices + (reduce && prev_sum ? 1 : 0);
+    // one more block per earlier group to sum
+    const int grid = nslices + (reduce && prev_sum ? 1 : 0) + (reduce && prev_sumb ? 1 : 0);
     if (grid == 0) return;
-    launch(k_backward<T>, dim3(grid), dim3(kSellBlock), s, U, slice0, nslices, row0, nrows, st, reduce, r, m, z,
-                                              diag, omega, c2, group_partials, prev0, prevn,
-                                              prev_sum, gbase, gw, defer_update, x, d, q);
+    launch(k_backward<T>, dim3(grid), dim3(kSellBlock), s, U, slice0, nslices, row0, nrows, st,
+           reduce, r, m, z, diag, omega, c2, group_partials, prev0, prevn, prev_sum, prev0b,
+           prevnb, prev_sumb, gbase, gw, defer_update, x, d, q);
+}
+
+template <typename T>
+void launch_forward_pair(hipStream_t s, SellViewT<T> L, SweepGroup A, SweepGroup B,
+                         SolveState* st, T* x, T* r, const T* d, const T* q, T* m,
+                         const T* diag, T omega, T* zb, T c2, double* zpart)
+{
+    const int grid = (A.nslices + kPairWaves - 1) / kPairWaves + B.nslices;
+    if (grid == 0) return;
+    launch(k_forward_pair<T>, dim3(grid), dim3(kPairBlock), s, L, A, B, st, x, r, d, q, m, diag,
+           omega, zb, c2, zpart);
+}
+
+template <typename T>
+void launch_backward_pair(hipStream_t s, SellViewT<T> U, SweepGroup A, SweepGroup B,
+                          SolveState* st, T* r, const T* m, T* z, const T* diag, T omega, T c2,
+                          double* group_partials, int nfold, int fold0a, int foldna,
+                          double* fold_suma, int fold0b, int foldnb, double* fold_sumb,
+                          int defer_a, int defer_b, T* x, const T* d, const T* q)
+{
+    const int grid = (A.nslices + kPairWaves - 1) / kPairWaves + B.nslices + (nfold > 0 ? 1 : 0);
+    if (grid == 0) return;
+    launch(k_backward_pair<T>, dim3(grid), dim3(kPairBlock), s, U, A, B, st, r, m, z, diag, omega,
+           c2, group_partials, nfold, fold0a, foldna, fold_suma, fold0b, foldnb, fold_sumb,
+           defer_a, defer_b, x, d, q);
 }
 
 template <typename T>
@@ -856,12 +1176,14 @@ template <typename T>
 void launch_spmv_dir(hipStream_t s, SellViewT<T> A, int nslices, int row0, int nrows,
                      int n_active, SolveState* st, int p, const T* src, const T* dold, T* dnew,
                      T* q, double* spmv_partials, const double* fold_partials, int nfold,
-                     const double* group_sums, int ngroups, int gw, int lanes)
+                     const double* fold_partials_b, int nfold_b, const double* group_sums,
+                     int ngroups, int gw, int lanes)
 {
     const dim3 grid(spmv_blocks(nslices * lanes)), block(kSpmvBlock);
 #define SFA_SPMV_DIR(KK)                                                                       \
     launch(k_spmv_dir<T, KK>, grid, block, s, A, nslices, row0, nrows, n_active, st, p, src,  \
-           dold, dnew, q, spmv_partials, fold_partials, nfold, group_sums, ngroups, gw)
+           dold, dnew, q, spmv_partials, fold_partials, nfold, fold_partials_b, nfold_b,      \
+           group_sums, ngroups, gw)
     switch (lanes) {
     case 1: SFA_SPMV_DIR(1); break;
     case 2: SFA_SPMV_DIR(2); break;
@@ -922,8 +1244,15 @@ void launch_add_correction(hipStream_t s, double* x, const float* e, int n)
                                     const double*, int, T*, T, double*, int, int, int, int);   \
     template void launch_backward<T>(hipStream_t, SellViewT<T>, int, int, int, int,            \
                                      SolveState*, int, T*, const T*, T*, const T*, T, T,       \
-                                     double*, int, int, double*, int, int, int, T*, const T*,  \
-                                     const T*);                                                \
+                                     double*, int, int, double*, int, int, double*, int, int,  \
+                                     int, T*, const T*, const T*);                             \
+    template void launch_forward_pair<T>(hipStream_t, SellViewT<T>, SweepGroup, SweepGroup,    \
+                                         SolveState*, T*, T*, const T*, const T*, T*,          \
+                                         const T*, T, T*, T, double*);                         \
+    template void launch_backward_pair<T>(hipStream_t, SellViewT<T>, SweepGroup, SweepGroup,   \
+                                          SolveState*, T*, const T*, T*, const T*, T, T,       \
+                                          double*, int, int, int, double*, int, int, double*,  \
+                                          int, int, T*, const T*, const T*);                   \
     template void launch_eisenstat_solve<T>(hipStream_t, SellViewT<T>, int, int, int, int,     \
                                             SolveState*, int, const T*, const T*, T*,          \
                                             const T*, T, T, int, int);                         \
@@ -932,7 +1261,8 @@ void launch_add_correction(hipStream_t s, double* x, const float* e, int n)
                                             const T*, T, const double*, int, int, int);        \
     template void launch_spmv_dir<T>(hipStream_t, SellViewT<T>, int, int, int, int,            \
                                      SolveState*, int, const T*, const T*, T*, T*, double*,    \
-                                     const double*, int, const double*, int, int, int);        \
+                                     const double*, int, const double*, int, const double*,    \
+                                     int, int, int);                                           \
     template void launch_update<T>(hipStream_t, UpdateMode, SolveState*, int, int, T*, T*,     \
                                    const T*, const T*, T*, const T*, T, const double*, int,    \
                                    double*);                                                   \

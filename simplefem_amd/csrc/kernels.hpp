@@ -127,14 +127,50 @@ void launch_forward(hipStream_t s, SellViewT<T> L, int slice0, int nslices, int 
                     const T* diag, T omega, const double* spmv_partials, int nspmv, T* zb, T c2,
                     double* zpart, int gbase, int gw, int m0_row0, int m0_nrows);
 // backward: t = m*diag*c2; z = (t - U z) * w / diag.  With reduce, each
-// block stores its z.r partial at group_partials[slice0 + block], and block 0
-// sums the previous group's partials (prev0, prevn) into *prev_sum.
+// block stores its z.r partial at group_partials[slice0 + block], and extra
+// blocks sum earlier groups' partials (prev0, prevn) into *prev_sum and, when
+// prev_sumb != null, (prev0b, prevnb) into *prev_sumb.
 template <typename T>
 void launch_backward(hipStream_t s, SellViewT<T> U, int slice0, int nslices, int row0, int nrows,
                      SolveState* st, int reduce, T* r, const T* m, T* z, const T* diag,
                      T omega, T c2, double* group_partials, int prev0, int prevn,
-                     double* prev_sum, int gbase, int gw, int defer_update, T* x, const T* d,
-                     const T* q);
+                     double* prev_sum, int prev0b, int prevnb, double* prev_sumb, int gbase,
+                     int gw, int defer_update, T* x, const T* d, const T* q);
+// --- two consecutive groups of a sweep in one launch (Rainbow iteration) ---
+// A group as the pair kernels see it: slices [slice0, slice0+nslices) of the
+// part, rows row0.. (consecutive; row0 < 0: read from the part's row list),
+// and (gbase, gw) as above.
+struct SweepGroup {
+    int slice0, nslices, row0, nrows, gbase, gw;
+};
+constexpr int kPairWaves = 4;  // wavefronts per block of the pair kernels
+constexpr int kPairRefs = 8;   // most entries of a B row whose column is an A row
+constexpr int kPairWidth = 16; // most entries per lane of a B slice
+// Forward sweep of groups A then B (B's rows read A's m): A's blocks run A as
+// the one-group kernel does (FWD_UPDATE) but leave A's x and r updates to
+// A's backward kernel; B's blocks form, for each A row a B row reads, that
+// row's m with the same arithmetic as A's wavefront (from r - a q and the m of
+// earlier groups), then run B (x, r updates; with zb the fused first
+// backward group as in launch_forward).  A's rows must be consecutive; every
+// B row may read at most kPairRefs A rows and B's slices at most kPairWidth
+// entries per lane.
+template <typename T>
+void launch_forward_pair(hipStream_t s, SellViewT<T> L, SweepGroup A, SweepGroup B,
+                         SolveState* st, T* x, T* r, const T* d, const T* q, T* m,
+                         const T* diag, T omega, T* zb, T c2, double* zpart);
+// Backward sweep of groups A then B (B's rows read A's z), with reduce: the
+// z.r partial of every slice at group_partials[slice]; B's blocks form the z
+// of the A rows they read as A's wavefront does.  defer_a / defer_b: that
+// group's x and r updates (x += a d, r -= a q) are made here, first.  nfold
+// (0..2) extra blocks sum earlier groups' partials: fold k sums
+// group_partials[fold0[k] .. fold0[k]+foldn[k]) into *fold_sum[k].
+template <typename T>
+void launch_backward_pair(hipStream_t s, SellViewT<T> U, SweepGroup A, SweepGroup B,
+                          SolveState* st, T* r, const T* m, T* z, const T* diag, T omega, T c2,
+                          double* group_partials, int nfold, int fold0a, int foldna,
+                          double* fold_suma, int fold0b, int foldnb, double* fold_sumb,
+                          int defer_a, int defer_b, T* x, const T* d, const T* q);
+
 // Eisenstat: out = (in - [c3*diag*t] - P out) * w / diag  (P = L or U part)
 template <typename T>
 void launch_eisenstat_solve(hipStream_t s, SellViewT<T> P, int slice0, int nslices, int row0,
@@ -154,7 +190,8 @@ void launch_eisenstat_upper(hipStream_t s, SellViewT<T> P, int slice0, int nslic
 // --- iteration ---
 // Opens an iteration of parity p for the SpMV-based variants: when
 // state.fold[p], forms the new z.r from the previous iteration's partials
-// (sum of group_sums[0..ngroups) then of fold_partials[0..nfold)), beta, K
+// (sum of group_sums[0..ngroups), then of fold_partials[0..nfold), then, when
+// fold_partials_b != null, of fold_partials_b[0..nfold_b)), beta, K
 // and the loop-head tests; then dn = fold ? src + beta*dold : dold on the
 // active rows (dold elsewhere), dnew = dn, q = A dn and the d.Ad partials
 // (spmv_partials, one per block: spmv_blocks(nslices * lanes) of them).
@@ -163,7 +200,8 @@ template <typename T>
 void launch_spmv_dir(hipStream_t s, SellViewT<T> A, int nslices, int row0, int nrows,
                      int n_active, SolveState* st, int p, const T* src, const T* dold, T* dnew,
                      T* q, double* spmv_partials, const double* fold_partials, int nfold,
-                     const double* group_sums, int ngroups, int gw, int lanes);
+                     const double* fold_partials_b, int nfold_b, const double* group_sums,
+                     int ngroups, int gw, int lanes);
 enum UpdateMode { UPD_PLAIN = 0, UPD_JACOBI = 1, UPD_EISENSTAT = 2 };
 // Forms d.Ad from dad_partials (ndad), breakdown tests, alpha; then
 // x += a*dx; r -= a*q; z.r per mode into zr_partials (one per block).

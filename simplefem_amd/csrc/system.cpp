@@ -60,6 +60,20 @@ struct SystemT {
     bool rainbow = false;
     bool fuse_first_backward = false;  // see forward_sweep
     bool merge_first_forward = false;  // see forward_sweep
+    // The iteration's sweeps (SSOR / Rainbow): one launch per entry, a group
+    // or, with pair, the group and the next one (k_forward_pair /
+    // k_backward_pair).  defer_bwd[g]: backward group g makes its rows' x and
+    // r updates (their forward ran as the first group of a pair, or merged).
+    // spmv_tail: the groups of the last backward launch, whose z.r partials
+    // the next SpMV sums itself.
+    struct SweepLaunch {
+        int g;
+        bool pair;
+    };
+    bool pair_sweeps = true;  // false: one group per launch
+    std::vector<SweepLaunch> fwd_sched, bwd_sched;
+    std::vector<int> defer_bwd;
+    std::vector<int> spmv_tail;
     int spmv_lanes = 1;                // lanes per row of the iteration's SpMV (1, 2, 4, 8)
 
     DevPartT<T> full, lower, upper, lower_restart;
@@ -152,6 +166,7 @@ struct SystemT {
         create_from_plan(plan, A, C, V);
         if (with_fp32) {
             inner.reset(new SystemT<float>());
+            inner->pair_sweeps = pair_sweeps;
             inner->create_from_plan(plan, A, C, V);
             inner->stream = stream;
         }
@@ -199,6 +214,7 @@ struct SystemT {
                                   upper.nrows_of(gu) == lower.nrows_of(0) &&
                                   upper.group_count(gu) == lower.group_count(0);
         }
+        if (sweeps && !eisen) build_sweep_schedule(plan);
         if (has_lower_restart) lower_restart.upload(plan.lower_restart);
 
         perm = dev_alloc<int>(n);
@@ -236,6 +252,163 @@ struct SystemT {
     T c2() const { return T((2. - omega) / omega); }
     T c3() const { return T(2. / omega - 1.); }
     T w() const { return T(omega); }
+
+    // Whether groups a and a+1 of part P run in one pair launch.  Possible
+    // when a's rows are consecutive, a+1's slices hold at most kPairWidth
+    // entries per lane and every row of a+1 reads at most kPairRefs rows of
+    // a; worth it when the rows of a that a wavefront of a+1 forms again are
+    // cheap: (most rows of a read by one row) x (entries per row of a) at
+    // most kPairCost.  Measured on the 64^3 Kuhn Poisson matrix, pairs up to
+    // 12 ran faster than their two launches, pairs from 24 up slower (the
+    // rows of a are gathered one per lane, uncoalesced).
+    static constexpr int kPairCost = 16;
+    static bool pairable(const SellPart& P, int a)
+    {
+        if (a + 1 >= P.ngroups() || P.group_row0[a] < 0) return false;
+        const unsigned r0 = (unsigned)P.group_row0[a], nr = (unsigned)P.group_rows[a];
+        int refs = 0;
+        for (int sl = P.group_slice[a + 1]; sl < P.group_slice[a + 2]; sl++) {
+            const size_t off = (size_t)P.slice_off[sl];
+            const int w = (P.slice_off[sl + 1] - P.slice_off[sl]) / kSliceRows;
+            if (w > kPairWidth) return false;
+            for (int l = 0; l < kSliceRows; l++) {
+                int cnt = 0;
+                for (int k = 0; k < w; k++)
+                    cnt += (unsigned)P.cols[off + (size_t)k * kSliceRows + l] - r0 < nr ? 1 : 0;
+                if (cnt > kPairRefs) return false;
+                refs = std::max(refs, cnt);
+            }
+        }
+        int wa = 0;  // entries per lane of a's slices
+        for (int sl = P.group_slice[a]; sl < P.group_slice[a + 1]; sl++)
+            wa = std::max(wa, (P.slice_off[sl + 1] - P.slice_off[sl]) / kSliceRows);
+        return refs * std::max(wa, 1) <= kPairCost;
+    }
+
+    // Backward group with the rows of forward group lg, or -1.
+    int upper_of(int lg) const
+    {
+        for (int ug = 0; ug < upper.ngroups(); ug++)
+            if (lower.row0(lg) >= 0 && upper.row0(ug) == lower.row0(lg) &&
+                upper.nrows_of(ug) == lower.nrows_of(lg))
+                return ug;
+        return -1;
+    }
+
+    void build_sweep_schedule(const Plan& plan)
+    {
+        const int GL = lower.ngroups(), GU = upper.ngroups();
+        fwd_sched.clear();
+        bwd_sched.clear();
+        defer_bwd.assign(std::max(GU, 1), 0);
+        if (merge_first_forward) defer_bwd[GU - 1] = 1;
+        const bool pairs = pair_sweeps && rainbow;
+        const int g0 = merge_first_forward ? 1 : 0;
+        for (int g = g0; g < GL;) {
+            // the first launch forms alpha and runs alone
+            const int ug = g > g0 && pairs && pairable(plan.lower, g) ? upper_of(g) : -1;
+            if (ug >= 0) {
+                fwd_sched.push_back({g, true});
+                defer_bwd[ug] = 1;
+                g += 2;
+            } else {
+                fwd_sched.push_back({g, false});
+                g += 1;
+            }
+        }
+        std::vector<int> pend;
+        if (fuse_first_backward) pend.push_back(0);
+        for (int g = fuse_first_backward ? 1 : 0; g < GU;) {
+            if (pairs && pairable(plan.upper, g)) {
+                bwd_sched.push_back({g, true});
+                pend = {g, g + 1};
+                g += 2;
+            } else {
+                bwd_sched.push_back({g, false});
+                pend = {g};
+                g += 1;
+            }
+        }
+        spmv_tail = pend;
+    }
+
+    SweepGroup sweep_group(const DevPartT<T>& P, int g) const
+    {
+        return SweepGroup{P.group_begin(g), P.group_count(g), P.row0(g),
+                          P.nrows_of(g),    P.base(g),        P.width(g)};
+    }
+
+    // The iteration's forward sweep (FWD_FIRST, then FWD_UPDATE), by the
+    // schedule; dd: the iteration's direction.
+    void forward_iteration(hipStream_t s, int p, const T* dd)
+    {
+        const int nspmv = spmv_blocks(full.nslices * spmv_lanes);
+        const int G = lower.ngroups();
+        const int g0 = merge_first_forward ? 1 : 0;
+        for (const SweepLaunch& e : fwd_sched) {
+            const int g = e.g;
+            const int last = e.pair ? g + 1 : g;
+            const bool fuse = fuse_first_backward && last == G - 1;
+            double* zpart = fuse ? pb + upper.group_begin(0) : nullptr;
+            if (e.pair) {
+                // A: r, q, diag, m; B: x r/w, d, r r/w, q, diag, m (+ z)
+                const double bytes = lower.bytes(g, 32.0) + lower.bytes(g + 1, fuse ? 72.0 : 64.0);
+                run(s, SFA_KERNEL_FORWARD, bytes, [&] {
+                    launch_forward_pair(s, lower.view(), sweep_group(lower, g),
+                                        sweep_group(lower, g + 1), state, x, r, dd, q, m, diag,
+                                        w(), fuse ? z : (T*)nullptr, c2(), zpart);
+                });
+                continue;
+            }
+            const int gm = g == g0 ? FWD_FIRST : FWD_UPDATE;
+            const bool merged = merge_first_forward && g == 1;
+            const double bytes = lower.bytes(g, 64.0 + (fuse ? 8.0 : 0.0)) +
+                                 (merged ? lower.bytes(0, 32.0) : 0.0);
+            run(s, SFA_KERNEL_FORWARD, bytes, [&] {
+                launch_forward(s, lower.view(), lower.group_begin(g), lower.group_count(g),
+                               lower.row0(g), lower.nrows_of(g), state, gm, p, x, r, dd, q, m,
+                               diag, w(), ps, nspmv, fuse ? z : (T*)nullptr, c2(), zpart,
+                               lower.base(g), lower.width(g), merged ? lower.row0(0) : 0,
+                               merged ? lower.nrows_of(0) : 0);
+            });
+        }
+    }
+
+    // The iteration's backward sweep with the z.r partials, by the schedule:
+    // every launch sums the partials of the groups of the launch before it.
+    void backward_iteration(hipStream_t s, const T* dd)
+    {
+        std::vector<int> pend;
+        if (fuse_first_backward) pend.push_back(0);
+        for (const SweepLaunch& e : bwd_sched) {
+            const int g = e.g;
+            const int fa = pend.size() > 0 ? pend[0] : -1;
+            const int fb = pend.size() > 1 ? pend[1] : -1;
+            const int f0a = fa >= 0 ? upper.group_begin(fa) : 0, fna = fa >= 0 ? upper.group_count(fa) : 0;
+            const int f0b = fb >= 0 ? upper.group_begin(fb) : 0, fnb = fb >= 0 ? upper.group_count(fb) : 0;
+            double* sa = fa >= 0 ? grp + fa : (double*)nullptr;
+            double* sb = fb >= 0 ? grp + fb : (double*)nullptr;
+            if (e.pair) {
+                const double bytes = upper.bytes(g, defer_bwd[g] ? 64.0 : 32.0) +
+                                     upper.bytes(g + 1, defer_bwd[g + 1] ? 64.0 : 32.0);
+                run(s, SFA_KERNEL_BACKWARD, bytes, [&] {
+                    launch_backward_pair(s, upper.view(), sweep_group(upper, g),
+                                         sweep_group(upper, g + 1), state, r, m, z, diag, w(), c2(),
+                                         pb, (int)pend.size(), f0a, fna, sa, f0b, fnb, sb,
+                                         defer_bwd[g], defer_bwd[g + 1], x, dd, q);
+                });
+                pend = {g, g + 1};
+                continue;
+            }
+            run(s, SFA_KERNEL_BACKWARD, upper.bytes(g, defer_bwd[g] ? 64.0 : 32.0), [&] {
+                launch_backward(s, upper.view(), upper.group_begin(g), upper.group_count(g),
+                                upper.row0(g), upper.nrows_of(g), state, 1, r, m, z, diag, w(),
+                                c2(), pb, f0a, fna, sa, f0b, fnb, sb, upper.base(g),
+                                upper.width(g), defer_bwd[g], x, dd, q);
+            });
+            pend = {g};
+        }
+    }
 
     // mode FWD_RESTART / FWD_UPDATE; the first group of an iteration runs as
     // FWD_FIRST (alpha from the SpMV's partials).
@@ -280,8 +453,9 @@ struct SystemT {
                                 upper.row0(g), upper.nrows_of(g), state, reduce, r, m, z, diag,
                                 w(), c2(), pb, fold_prev ? upper.group_begin(g - 1) : 0,
                                 fold_prev ? upper.group_count(g - 1) : 0,
-                                fold_prev ? grp + (g - 1) : (double*)nullptr, upper.base(g),
-                                upper.width(g), defer ? 1 : 0, x, dd, q);
+                                fold_prev ? grp + (g - 1) : (double*)nullptr, 0, 0,
+                                (double*)nullptr, upper.base(g), upper.width(g), defer ? 1 : 0,
+                                x, dd, q);
             });
         }
     }
@@ -359,7 +533,8 @@ struct SystemT {
             run(s, SFA_KERNEL_SPMV, spmv_bytes, [&] {
                 launch_spmv_dir(s, F, full.nslices, full.row0(0), full.nrows_of(0), n_active,
                                 state, p, jac ? z : r, dold, dnew, q, ps, pu, nvec,
-                                (const double*)nullptr, 0, full.width(0), spmv_lanes);
+                                (const double*)nullptr, 0, (const double*)nullptr, 0,
+                                full.width(0), spmv_lanes);
             });
             run(s, SFA_KERNEL_UPDATE, (jac ? 64.0 : 48.0) * na, [&] {
                 launch_update(s, jac ? UPD_JACOBI : UPD_PLAIN, state, p, n_active, x, r, dnew, q,
@@ -370,18 +545,22 @@ struct SystemT {
         }
         case SFA_METHOD_SSOR:
         case SFA_METHOD_RAINBOW: {
-            // z.r of the last backward group comes from its block partials,
-            // the other groups' sums from grp
+            // z.r of the last backward launch's groups comes from their block
+            // partials, the other groups' sums from grp
             const int G = upper.ngroups();
+            const int nt = (int)spmv_tail.size();
+            const int ta = nt > 0 ? spmv_tail[0] : -1, tb = nt > 1 ? spmv_tail[1] : -1;
             run(s, SFA_KERNEL_SPMV, spmv_bytes, [&] {
                 launch_spmv_dir(s, F, full.nslices, full.row0(0), full.nrows_of(0), n_active,
                                 state, p, z, dold, dnew, q, ps,
-                                G > 0 ? pb + upper.group_begin(G - 1) : pb,
-                                G > 0 ? upper.group_count(G - 1) : 0, grp, G > 0 ? G - 1 : 0,
+                                ta >= 0 ? pb + upper.group_begin(ta) : pb,
+                                ta >= 0 ? upper.group_count(ta) : 0,
+                                tb >= 0 ? pb + upper.group_begin(tb) : (const double*)nullptr,
+                                tb >= 0 ? upper.group_count(tb) : 0, grp, G - nt,
                                 full.width(0), spmv_lanes);
             });
-            forward_sweep(s, lower, FWD_UPDATE, p, dnew);
-            backward_sweep(s, 1, dnew);
+            forward_iteration(s, p, dnew);
+            backward_iteration(s, dnew);
             break;
         }
         default:  // Eisenstat: t (in m) = upper solve of d, q = lower solve
@@ -409,9 +588,7 @@ struct SystemT {
         case SFA_METHOD_CG:
         case SFA_METHOD_DIAGONAL: return 2;
         case SFA_METHOD_SSOR:
-        case SFA_METHOD_RAINBOW:
-            return 1 + lower.ngroups() + upper.ngroups() - (fuse_first_backward ? 1 : 0) -
-                   (merge_first_forward ? 1 : 0);
+        case SFA_METHOD_RAINBOW: return 1 + (int)fwd_sched.size() + (int)bwd_sched.size();
         default: return 2 + lower.ngroups() + upper.ngroups();
         }
     }
@@ -789,8 +966,8 @@ struct SystemT {
             const SellViewT<T> F = full.view();
             auto once = [&](int lanes) {
                 launch_spmv_dir(stream, F, full.nslices, full.row0(0), full.nrows_of(0), n_active,
-                                ts, 0, z, d, d1, q, tp, tz, nfold, tz, ngroups, full.width(0),
-                                lanes);
+                                ts, 0, z, d, d1, q, tp, tz, nfold, (const double*)nullptr, 0,
+                                tz, ngroups, full.width(0), lanes);
             };
             const int cand[SFA_SPMV_MAPPINGS] = {1, 2, 4, 8};
             const int main = full.main_bucket();
@@ -1024,7 +1201,9 @@ int sfa_system_create(int method, int N, const int* csrrows, const int* csrcolum
         *out = nullptr;
         std::unique_ptr<sfa_system_s> h(new sfa_system_s());
         const bool fp32 = (method & SFA_FLAG_FP32_INNER) != 0;
-        h->sys.create(method & ~SFA_FLAG_FP32_INNER, N, csrrows, csrcolumns, csrvalues, aux, omega,
+        h->sys.pair_sweeps = (method & SFA_FLAG_SINGLE_GROUP_SWEEPS) == 0;
+        h->sys.create(method & ~(SFA_FLAG_FP32_INNER | SFA_FLAG_SINGLE_GROUP_SWEEPS), N, csrrows,
+                      csrcolumns, csrvalues, aux, omega,
                       num_colors, F, B, R, fp32);
         *out = h.release();
     });

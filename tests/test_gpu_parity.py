@@ -332,3 +332,37 @@ def test_mixed_precision_refinement(variant, method):
         assert err <= 1e-8, err
         print(f"[mixed] {variant}: outer {info['outer']} inner {info['inner_iterations']} "
               f"rel.res {info['relative_residual']:.2e} err vs FP64 {err:.2e}")
+
+
+@pytest.mark.parametrize("maker", [lambda: S.poisson3d_p1_kuhn(12), lambda: S.poisson2d_p1(6),
+                                   lambda: S.hcurl3d_whitney_kuhn(5),
+                                   lambda: S.lagrange_pr(2, 6, 3)],
+                         ids=["kuhn12", "p1_2d_L6", "hcurl5", "p3_2d"])
+def test_paired_sweeps_are_bit_identical(maker):
+    """Two colours per launch (the second forms the first's values it reads)
+    give the same x, residual and iteration count, bit for bit, as one colour
+    per launch, with fewer launches per iteration; mixed precision too."""
+    system = maker()
+    b = S.rhs_uniform(system, 21)
+    d = system.diagonal()
+    col = oracle.rainbow(system.n, system.A, system.C)
+    out = []
+    for single in (False, True):
+        with capi.System(capi.METHOD_RAINBOW, system.n, system.A, system.C, system.V, d, 1.0,
+                         *col, fp32_inner=True, single_group_sweeps=single) as sysd:
+            x = np.zeros(system.n)
+            r = np.zeros(system.n)
+            info = {}
+            sysd.solve(x, b, r, 1e-10, info=info)
+            xm = np.zeros(system.n)
+            mi = sysd.solve_mixed(xm, b, 1e-10)
+            out.append((x, r, info["iterations"], xm, mi["inner_iterations"],
+                        sysd.stats()["kernels_per_iteration"]))
+    (xp, rp, kp, xmp, kmp, lp), (xs, rs, ks, xms, kms, ls) = out
+    print(f"[pairs] {system.name}: K={kp} launches/iteration {lp} paired vs {ls} single", flush=True)
+    assert kp == ks and np.array_equal(xp, xs) and np.array_equal(rp, rs)
+    assert kmp == kms and np.array_equal(xmp, xms)
+    assert lp <= ls
+    xo, _, io = oracle.solve("rainbow", system.A, system.C, system.V, b, aux=d, colouring=col)
+    assert abs(kp - io["iterations"]) <= max(1, int(0.02 * io["iterations"]))
+    _close(xp, xo, 1e-10)
