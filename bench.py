@@ -97,6 +97,9 @@ def main():
     ap.add_argument("--precision", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--numbering", default="reference", choices=["reference", "lexicographic"],
+                    help="vertex numbering of the 64^3 system: the reference's uniform-refinement "
+                         "order (its CSR structure and colouring; default) or x-fastest")
     ap.add_argument("--single-group-sweeps", action="store_true",
                     help="one sweep group per launch (no paired colours)")
     ap.add_argument("--spmv-lanes", type=int, default=0, choices=[0, 1, 2, 4, 8],
@@ -142,7 +145,7 @@ def main():
     from simplefem_amd.hipmem import DeviceArray, synchronize
 
     t0 = time.time()
-    system = S.poisson3d_p1_kuhn(args.cube)
+    system = S.poisson3d_p1_kuhn(args.cube, numbering=args.numbering)
     b = S.rhs_uniform(system, seed=2026)
     d = capi.get_diagonal(system.n, system.A, system.C, system.V)
     rb = capi.Rainbow(system.n, system.A, system.C)
@@ -231,8 +234,10 @@ def main():
         "dtype": "f64",
         "data": "synthetic",
         "config": {
-            "workload": f"3D P1 Poisson, {args.cube}^3 Kuhn cubes, Rainbow CG-SSOR "
+            "workload": f"3D P1 Poisson, {args.cube}^3 Kuhn cubes (UnitCube3D refined, "
+                        f"{args.numbering} vertex numbering), Rainbow CG-SSOR "
                         f"omega={args.omega}, precision {args.precision:g}, x0=0",
+            "numbering": args.numbering,
             "rows": system.n, "active_rows": stats["n_active"], "nnz": system.nnz,
             "colours": rb.num_colors, "iterations": info["iterations"],
             "kernels_per_iteration": stats["kernels_per_iteration"],

@@ -47,6 +47,10 @@ def _init_library():
                                     _P_I, _P_D, c_double, c_double]
     _lib.feecref_csr_op.restype = c_int
     _lib.feecref_result.argtypes = [_P_I, _P_I, _P_I, _P_D]
+    _lib.feecref_mesh.argtypes = [c_int, c_int, c_int]
+    _lib.feecref_mesh.restype = c_int
+    _lib.feecref_mesh_sizes.argtypes = [_P_I]
+    _lib.feecref_mesh_copy.argtypes = [_P_D, _P_I, _P_I, _P_I, _P_I, _P_D, _P_I, _P_I, _P_D]
 
 
 def _i(a):
@@ -89,6 +93,26 @@ def _task(kind, kw):
             raise ValueError("unknown csr op")
         out_rows = {2: m2[1], 3: m1[1]}.get(int(kw["op"]), m1[0])
         return _result(nnz, int(out_rows))
+    if kind == "mesh":
+        dim, levels, mats = int(kw["dim"]), int(kw["levels"]), int(kw["matrices"])
+        L.feecref_mesh(dim, levels, mats)
+        sz = np.zeros(4, dtype=np.int32)
+        L.feecref_mesh_sizes(_i(sz))
+        nv, ns, nb, na = (int(t) for t in sz)
+        coords = np.empty(nv * dim)
+        simp = np.empty(ns * (dim + 1), dtype=np.int32)
+        dirichlet = np.empty(nv, dtype=np.int32)
+        bA, aA = np.empty(nv + 1, dtype=np.int32), np.empty(nv + 1, dtype=np.int32)
+        bC, bV = np.empty(max(nb, 1), dtype=np.int32), np.empty(max(nb, 1))
+        aC, aV = np.empty(max(na, 1), dtype=np.int32), np.empty(max(na, 1))
+        L.feecref_mesh_copy(_d(coords), _i(simp), _i(dirichlet), _i(bA), _i(bC), _d(bV), _i(aA),
+                            _i(aC), _d(aV))
+        out = {"coords": coords.reshape(nv, dim), "simplices": simp.reshape(ns, dim + 1),
+               "dirichlet": dirichlet.astype(bool)}
+        if mats:
+            out["structural"] = (bA, bC[:nb].copy(), bV[:nb].copy())
+            out["compressed"] = (aA, aC[:na].copy(), aV[:na].copy())
+        return out
     A = np.ascontiguousarray(kw["A"], dtype=np.int32)
     C = np.ascontiguousarray(kw["C"], dtype=np.int32)
     V = np.ascontiguousarray(kw["V"], dtype=np.float64)
@@ -176,6 +200,14 @@ class RefRunner:
 
     def rainbow(self, A, C, V):
         return self._call("rainbow", A=A, C=C, V=V)
+
+    def mesh(self, dim, levels, matrices=True):
+        """UnitSquare2D_simple (dim 2) or UnitCube3D (dim 3) refined `levels`
+        times: {"coords", "simplices", "dirichlet"} and, with matrices, the
+        structural P1 stiffness (LagrangeCoefficientStiffnessMatrix, identity
+        weight) and the compressed one (MatrixCSR(LagrangeStiffnessMatrix) +
+        compressentries) as (A, C, V)."""
+        return self._call("mesh", dim=dim, levels=levels, matrices=1 if matrices else 0)
 
     def csr_apply(self, A, C, V, src, scaling=1.0):
         return self._call("apply", A=A, C=C, V=V, src=src, scaling=scaling)

@@ -217,6 +217,54 @@ void build_full_identity(SellPart& part, int rows, int columns, const int* A, co
     close_group(part);
 }
 
+namespace {
+
+// Breadth-first (Cuthill-McKee) order of the graph of the matrix: from row 0,
+// then from the lowest unvisited row; neighbours in column order.
+std::vector<int> bfs_order(int n, const int* A, const int* C)
+{
+    std::vector<int> order;
+    order.reserve(n);
+    std::vector<char> seen(n, 0);
+    for (int s = 0; s < n; s++) {
+        if (seen[s]) continue;
+        seen[s] = 1;
+        size_t head = order.size();
+        order.push_back(s);
+        while (head < order.size()) {
+            const int r = order[head++];
+            for (int k = A[r]; k < A[r + 1]; k++) {
+                const int c = C[k];
+                if (!seen[c]) {
+                    seen[c] = 1;
+                    order.push_back(c);
+                }
+            }
+        }
+    }
+    return order;
+}
+
+// Mean |rank[row] - rank[col]| over the entries.
+double mean_distance(int n, const int* A, const int* C, const std::vector<int>& rank)
+{
+    double sum = 0.;
+    for (int r = 0; r < n; r++)
+        for (int k = A[r]; k < A[r + 1]; k++) sum += std::abs(rank[r] - rank[C[k]]);
+    return A[n] > 0 ? sum / A[n] : 0.;
+}
+
+std::vector<int> locality_rank(int n, const int* A, const int* C)
+{
+    std::vector<int> ident(n), bfs(n);
+    for (int i = 0; i < n; i++) ident[i] = i;
+    const std::vector<int> order = bfs_order(n, A, C);
+    for (int i = 0; i < n; i++) bfs[order[i]] = i;
+    return mean_distance(n, A, C, bfs) < mean_distance(n, A, C, ident) ? bfs : ident;
+}
+
+}  // namespace
+
 std::string build_plan(Plan& plan, int method, int n, const int* A, const int* C, const double* V,
                        const double* aux, double omega, int num_colors, const int* F,
                        const int* B, const int* R, bool* colouring_error)
@@ -258,6 +306,21 @@ std::string build_plan(Plan& plan, int method, int n, const int* A, const int* C
     if (method != SFA_METHOD_CG)
         for (int i = 0; i < n; i++) active[i] = aux[i] != 0. ? 1 : 0;  // NaN counts as active
 
+    // ---- locality order ----
+    // Rows whose entries a wavefront gathers should sit close together in
+    // every vector.  The original order does that for a mesh numbered along
+    // its axes, not for one numbered in refinement order (a refined
+    // vertex's neighbours were numbered levels apart); a breadth-first
+    // (Cuthill-McKee) order over the matrix graph does it for both.  Of the
+    // two, the one with the smaller mean index distance over the entries
+    // ranks the rows: rows independent of one another (a colour or level,
+    // the SpMV) are laid out by rank.  Only where a row sits changes, not
+    // what it computes.
+    const std::vector<int> rank = locality_rank(n, A, C);
+    auto by_rank = [&](std::vector<int>& rows) {
+        std::sort(rows.begin(), rows.end(), [&](int a, int b) { return rank[a] < rank[b]; });
+    };
+
     // ---- device order and the sweep groups (as lists of original rows) ----
     std::vector<std::vector<int>> fwd_groups, bwd_groups;  // original row ids, launch order
     std::vector<int> order;                                // active rows in device order
@@ -265,6 +328,7 @@ std::string build_plan(Plan& plan, int method, int n, const int* A, const int* C
         fwd_groups.assign(num_colors, {});
         for (int r = 0; r < n; r++)
             if (active[r]) fwd_groups[F[r]].push_back(r);
+        for (auto& g : fwd_groups) by_rank(g);
         for (auto& g : fwd_groups) order.insert(order.end(), g.begin(), g.end());
         bwd_groups.assign(fwd_groups.rbegin(), fwd_groups.rend());
     } else if (sweeps) {
@@ -282,10 +346,13 @@ std::string build_plan(Plan& plan, int method, int n, const int* A, const int* C
                 fwd_groups[lf[r]].push_back(r);
                 bwd_groups[lb[r]].push_back(r);
             }
+        for (auto& g : fwd_groups) by_rank(g);
+        for (auto& g : bwd_groups) by_rank(g);
         for (auto& g : fwd_groups) order.insert(order.end(), g.begin(), g.end());
     } else {
         for (int r = 0; r < n; r++)
             if (active[r]) order.push_back(r);
+        by_rank(order);
     }
     plan.n_active = (int)order.size();
     plan.perm = order;
@@ -307,15 +374,17 @@ std::string build_plan(Plan& plan, int method, int n, const int* A, const int* C
     auto guarded = [&](int k) { return aux[C[k]] != 0.; };
 
     // ---- full part: SpMV of the iteration / residual of the restart ----
-    // Its rows are independent, so they are listed in original order, not in
+    // Its rows are independent, so they are listed in rank order, not in
     // device order: a range of slices then covers a piece of the mesh, whose
     // neighbours lie in a few short runs of each colour's rows (device order
     // groups rows by colour), and a GPU die that runs a contiguous range of
     // slices gathers them from its own cache instead of from all of each vector.
-    std::vector<int> active_dev;
-    active_dev.reserve(plan.n_active);
+    std::vector<int> active_rows;
+    active_rows.reserve(plan.n_active);
     for (int r = 0; r < n; r++)
-        if (active[r]) active_dev.push_back(iperm[r]);
+        if (active[r]) active_rows.push_back(r);
+    by_rank(active_rows);
+    std::vector<int> active_dev = to_dev(active_rows);
     start_part(plan.full);
     if (eisen)  // solver/sparsesolver.cpp:806-807,1420-1421 skip columns with diagonal 0
         append_group(plan.full, active_dev, plan.perm, iperm, n, A, C, V,

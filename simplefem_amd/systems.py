@@ -18,9 +18,15 @@ which makes the SSOR-family solvers skip that row.  Element contributions are
 summed in exact integer arithmetic (units of 1/2 in 2D, h/6 in 3D) and
 rounded once, so couplings that vanish are exact zeros.
 
-Vertices are numbered lexicographically (x fastest).  The reference numbers
-the vertices of a refined mesh in refinement order instead; the matrices are
-the same up to that symmetric permutation.
+Vertices are numbered lexicographically (x fastest) by default; with
+numbering="reference" they are numbered as the reference's uniform
+refinement numbers them (``reference_permutation``, a restatement of
+mesh/mesh.simplicial2D.cpp:2205-2750 and mesh/mesh.simplicial3D.ur.cxx), which
+gives the reference's CSR structure bit for bit: the structural pattern of
+LagrangeCoefficientStiffnessMatrix (fem/lagrangematrices.cpp:547-782) and, with
+exact zeros dropped, MatrixCSR(LagrangeStiffnessMatrix) + compressentries
+(fem/lagrangematrices.cpp:210-271, tests/sullivan2D/poissondirichletlshaped.cpp:128-131),
+whose values these match too.
 
 ``rhs_uniform`` draws the seeded U[-1,1) right-hand side (SplitMix64 stream)
 on the active rows and puts 0 on Dirichlet rows, as the reference's RHS
@@ -214,24 +220,211 @@ def _offsets_units(patch: CSRSystem, centre_xyz, dims, scale):
     return out
 
 
-def poisson2d_p1(levels: int) -> CSRSystem:
-    """P1 stiffness of UnitSquare2D_simple refined `levels` times: (2^L+1)^2 vertices."""
+def poisson2d_p1(levels: int, numbering: str = "lexicographic") -> CSRSystem:
+    """P1 stiffness of UnitSquare2D_simple refined `levels` times: (2^L+1)^2
+    vertices.  numbering "reference": the vertices in the order the
+    reference's uniform refinement numbers them (reference_permutation)."""
     m = 2 ** levels
     nx = m + 1
     patch = _element_assembled_2d(2)                  # 5x5 vertices, interior stencil at (2,2)
     units = _offsets_units(patch, (2, 2), (5, 5), 0.5)
     gx, gy = np.meshgrid(np.arange(nx), np.arange(nx), indexing="xy")
     dirichlet = ((gx == 0) | (gy == 0) | (gx == m) | (gy == m)).ravel()
-    return _structured(f"poisson2d_p1_L{levels}", (nx, nx), units, 0.5, dirichlet)
+    lex = _structured(f"poisson2d_p1_L{levels}", (nx, nx), units, 0.5, dirichlet)
+    return _numbered(lex, 2, levels, numbering)
 
 
-def poisson3d_p1_kuhn(n: int) -> CSRSystem:
-    """P1 stiffness on n^3 cubes of six Kuhn tetrahedra each: (n+1)^3 vertices."""
+def poisson3d_p1_kuhn(n: int, numbering: str = "lexicographic") -> CSRSystem:
+    """P1 stiffness on n^3 cubes of six Kuhn tetrahedra each: (n+1)^3 vertices.
+    numbering "reference" (n a power of two): UnitCube3D refined log2(n)
+    times, the vertices in the reference's refinement order."""
     nx = n + 1
     patch = _element_assembled_3d(4)                  # 5^3 vertices, interior stencil at (2,2,2)
     units = _offsets_units(patch, (2, 2, 2), (5, 5, 5), (1.0 / 4) / 6.0)
-    return _structured(f"poisson3d_p1_kuhn_{n}", (nx, nx, nx), units, (1.0 / n) / 6.0,
-                       _boundary_3d(n))
+    lex = _structured(f"poisson3d_p1_kuhn_{n}", (nx, nx, nx), units, (1.0 / n) / 6.0,
+                      _boundary_3d(n))
+    if numbering == "reference" and (n & (n - 1)):
+        raise ValueError("the reference's refinement gives 2^L cubes per axis")
+    return _numbered(lex, 3, n.bit_length() - 1, numbering)
+
+
+def _numbered(lex: CSRSystem, dim: int, levels: int, numbering: str) -> CSRSystem:
+    if numbering == "lexicographic":
+        return lex
+    if numbering != "reference":
+        raise ValueError(f"unknown numbering {numbering!r}")
+    return permute_symmetric(lex, reference_permutation(dim, levels), lex.name + "_refnum")
+
+
+# ---------------------------------------------------------------------------
+# The reference's vertex numbering: uniform refinement of the unit square /
+# unit cube as mesh/mesh.simplicial2D.cpp:2205-2750 and
+# mesh/mesh.simplicial3D.ur.cxx:134-1005 number it.  A refinement keeps the
+# old vertices and appends one vertex per old edge (vertex V+e is the midpoint
+# of edge e); the new edges are the two halves of every old edge (e: back
+# half, E+e: front half), then three per old face (2E + k*F + f) and, in 3D,
+# one per old tetrahedron (2E + 3F + t, the 02-13 diagonal).  Which edge gets
+# which index therefore depends on the local order of the edges of every
+# face and tetrahedron, which the refinement fixes child by child; the tables
+# below restate those assignments.  Level 0 comes from the mesh constructors
+# (mesh/mesh.simplicial2D.cpp:56-250, mesh/mesh.simplicial3D.cpp:70-460):
+# simplices with sorted vertices, sorted; faces and edges the sorted unique
+# lists of their sub-simplices; local edges 01 02 (03) 12 (13 23), local
+# faces 012 013 023 123.
+
+def _sorted_unique_rows(a):
+    return np.unique(a, axis=0)
+
+
+def _row_lookup(table, rows):
+    """Index of each row of `rows` in the sorted unique `table`."""
+    key_t = table @ (1 << (16 * np.arange(table.shape[1])[::-1]))
+    key_r = rows @ (1 << (16 * np.arange(rows.shape[1])[::-1]))
+    idx = np.searchsorted(key_t, key_r)
+    assert np.array_equal(table[idx], rows)
+    return idx
+
+
+def _reference_mesh_2d(levels: int):
+    """(lattice coordinates in units of 2^-levels, triangles) of
+    UnitSquare2D_simple (mesh/examples2D.hpp:194-211) refined `levels` times,
+    numbered as the reference numbers them."""
+    s = 1 << levels
+    X = np.array([[0, 0], [1, 0], [0, 1], [1, 1]], dtype=np.int64) * s
+    TV = np.sort(np.array([[0, 1, 3], [0, 2, 3]], dtype=np.int64), axis=1)
+    TV = _sorted_unique_rows(TV)
+    loc = [(0, 1), (0, 2), (1, 2)]
+    EV = _sorted_unique_rows(np.concatenate([TV[:, [a, b]] for a, b in loc]))
+    TE = np.stack([_row_lookup(EV, TV[:, [a, b]]) for a, b in loc], axis=1)
+    for _ in range(levels):
+        V, E, T = len(X), len(EV), len(TV)
+        X = np.concatenate([X, (X[EV[:, 0]] + X[EV[:, 1]]) // 2])
+        mid = V + TE                      # v01, v02, v12 of every triangle
+        e_ = np.arange(E)
+        t_ = np.arange(T)
+        EV = np.concatenate([
+            np.stack([EV[:, 0], V + e_], axis=1), np.stack([V + e_, EV[:, 1]], axis=1),
+            np.stack([mid[:, 0], mid[:, 1]], axis=1), np.stack([mid[:, 0], mid[:, 2]], axis=1),
+            np.stack([mid[:, 1], mid[:, 2]], axis=1)])
+        i01_02, i01_12, i02_12 = 2 * E + t_, 2 * E + T + t_, 2 * E + 2 * T + t_
+        TE = np.concatenate([
+            np.stack([TE[:, 0], TE[:, 1], i01_02], axis=1),
+            np.stack([E + TE[:, 0], i01_12, TE[:, 2]], axis=1),
+            np.stack([i02_12, E + TE[:, 1], E + TE[:, 2]], axis=1),
+            np.stack([i01_02, i01_12, i02_12], axis=1)])
+        v00, v11, v22 = TV[:, 0], TV[:, 1], TV[:, 2]
+        v01, v02, v12 = mid[:, 0], mid[:, 1], mid[:, 2]
+        TV = np.concatenate([np.stack([v00, v01, v02], axis=1), np.stack([v01, v11, v12], axis=1),
+                             np.stack([v02, v12, v22], axis=1), np.stack([v01, v02, v12], axis=1)])
+    return X, TV
+
+
+def _reference_mesh_3d(levels: int):
+    """(lattice coordinates in units of 2^-levels, tetrahedra) of UnitCube3D
+    (mesh/examples3D.hpp:14-37) refined `levels` times, numbered as the
+    reference numbers them."""
+    s = 1 << levels
+    X = np.array([[(b >> 2) & 1, (b >> 1) & 1, b & 1] for b in range(8)], dtype=np.int64) * s
+    TV = np.array([[0, 4, 6, 7], [0, 4, 5, 7], [0, 2, 6, 7], [0, 2, 3, 7], [0, 1, 5, 7],
+                   [0, 1, 3, 7]], dtype=np.int64)
+    TV = _sorted_unique_rows(np.sort(TV, axis=1))
+    floc = [(0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)]
+    eloc = [(0, 1), (0, 2), (0, 3), (1, 2), (1, 3), (2, 3)]
+    FV = _sorted_unique_rows(np.concatenate([TV[:, list(f)] for f in floc]))
+    EV = _sorted_unique_rows(np.concatenate([TV[:, list(e)] for e in eloc]))
+    TE = np.stack([_row_lookup(EV, TV[:, list(e)]) for e in eloc], axis=1)
+    TF = np.stack([_row_lookup(FV, TV[:, list(f)]) for f in floc], axis=1)
+    FE = np.stack([_row_lookup(EV, FV[:, [a, b]]) for a, b in [(0, 1), (0, 2), (1, 2)]], axis=1)
+    st = lambda *cols: np.stack(cols, axis=1)
+    for _ in range(levels):
+        V, E, F, T = len(X), len(EV), len(FE), len(TV)
+        X = np.concatenate([X, (X[EV[:, 0]] + X[EV[:, 1]]) // 2])
+        e_, f_, t_ = np.arange(E), np.arange(F), np.arange(T)
+        fm = V + FE                       # midpoints of a face's edges 01 02 12
+        tm = V + TE                       # midpoints of a tetrahedron's edges 01 02 03 12 13 23
+        EV = np.concatenate([
+            st(EV[:, 0], V + e_), st(V + e_, EV[:, 1]),
+            st(fm[:, 0], fm[:, 1]), st(fm[:, 0], fm[:, 2]), st(fm[:, 1], fm[:, 2]),
+            st(tm[:, 1], tm[:, 4])])
+        # edges of the new faces and tetrahedra
+        fe01_02, fe01_12, fe02_12 = 2 * E + f_, 2 * E + F + f_, 2 * E + 2 * F + f_
+        FE_out = np.concatenate([
+            st(FE[:, 0], FE[:, 1], fe01_02),
+            st(E + FE[:, 0], fe01_12, FE[:, 2]),
+            st(fe02_12, E + FE[:, 1], E + FE[:, 2]),
+            st(fe01_02, fe01_12, fe02_12)])
+        face_e = lambda k, j: 2 * E + k * F + TF[:, j]  # k-th new edge of the tet's old face j
+        e01_02, e01_12, e02_12 = face_e(0, 0), face_e(1, 0), face_e(2, 0)
+        e01_03, e01_13, e03_13 = face_e(0, 1), face_e(1, 1), face_e(2, 1)
+        e02_03, e02_23, e03_23 = face_e(0, 2), face_e(1, 2), face_e(2, 2)
+        e12_13, e12_23, e13_23 = face_e(0, 3), face_e(1, 3), face_e(2, 3)
+        e02_13 = 2 * E + 3 * F + t_
+        FE_in = np.concatenate([
+            st(e01_02, e01_03, e02_03), st(e01_12, e01_13, e12_13),
+            st(e02_12, e02_23, e12_23), st(e03_13, e03_23, e13_23),
+            st(e01_02, e01_13, e02_13), st(e02_03, e02_13, e03_13),
+            st(e02_12, e02_13, e12_13), st(e02_13, e02_23, e13_23)])
+        e00_01, e01_11 = TE[:, 0], E + TE[:, 0]
+        e00_02, e02_22 = TE[:, 1], E + TE[:, 1]
+        e00_03, e03_33 = TE[:, 2], E + TE[:, 2]
+        e11_12, e12_22 = TE[:, 3], E + TE[:, 3]
+        e11_13, e13_33 = TE[:, 4], E + TE[:, 4]
+        e22_23, e23_33 = TE[:, 5], E + TE[:, 5]
+        TE_new = np.concatenate([
+            st(e00_01, e00_02, e00_03, e01_02, e01_03, e02_03),
+            st(e01_11, e01_12, e01_13, e11_12, e11_13, e12_13),
+            st(e02_12, e02_22, e02_23, e12_22, e12_23, e22_23),
+            st(e03_13, e03_23, e03_33, e13_23, e13_33, e23_33),
+            st(e01_02, e01_03, e01_13, e02_03, e02_13, e03_13),
+            st(e01_02, e01_12, e01_13, e02_12, e02_13, e12_13),
+            st(e02_03, e02_13, e02_23, e03_13, e03_23, e13_23),
+            st(e02_12, e02_13, e02_23, e12_13, e12_23, e13_23)])
+        out_f = lambda k, j: k * F + TF[:, j]           # k-th child of the tet's old face j
+        in_f = lambda k: 4 * F + k * T + t_             # k-th interior face of the tet
+        TF_new = np.concatenate([
+            st(out_f(0, 0), out_f(0, 1), out_f(0, 2), in_f(0)),
+            st(out_f(1, 0), out_f(1, 1), in_f(1), out_f(0, 3)),
+            st(out_f(2, 0), in_f(2), out_f(1, 2), out_f(1, 3)),
+            st(in_f(3), out_f(2, 1), out_f(2, 2), out_f(2, 3)),
+            st(in_f(0), in_f(4), out_f(3, 1), in_f(5)),
+            st(out_f(3, 0), in_f(4), in_f(1), in_f(6)),
+            st(in_f(5), out_f(3, 2), in_f(7), in_f(3)),
+            st(in_f(6), in_f(2), in_f(7), out_f(3, 3))])
+        v00, v11, v22, v33 = TV[:, 0], TV[:, 1], TV[:, 2], TV[:, 3]
+        v01, v02, v03, v12, v13, v23 = (tm[:, k] for k in range(6))
+        TV = np.concatenate([
+            st(v00, v01, v02, v03), st(v01, v11, v12, v13), st(v02, v12, v22, v23),
+            st(v03, v13, v23, v33), st(v01, v02, v03, v13), st(v01, v02, v12, v13),
+            st(v02, v03, v13, v23), st(v02, v12, v13, v23)])
+        FE = np.concatenate([FE_out, FE_in])
+        TE, TF = TE_new, TF_new
+    return X, TV
+
+
+def reference_permutation(dim: int, levels: int) -> np.ndarray:
+    """perm[i] = lexicographic (x fastest) index of the reference's vertex i
+    on the (2^levels+1)^dim grid."""
+    X, _ = (_reference_mesh_2d if dim == 2 else _reference_mesh_3d)(levels)
+    nx = (1 << levels) + 1
+    strides = nx ** np.arange(dim)
+    return (X * strides).sum(axis=1)
+
+
+def permute_symmetric(system: CSRSystem, perm: np.ndarray, name: str) -> CSRSystem:
+    """P^T A P with row i of the result = row perm[i] of A (columns renumbered,
+    each row sorted by column, as MatrixCSR stores them)."""
+    n = system.n
+    inv = np.empty(n, dtype=np.int64)
+    inv[perm] = np.arange(n)
+    rows = np.repeat(np.arange(n), np.diff(system.A))
+    new_rows = inv[rows]
+    new_cols = inv[system.C]
+    order = np.lexsort((new_cols, new_rows))
+    A = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(A, new_rows + 1, 1)
+    A = np.cumsum(A)
+    return CSRSystem(name, n, A.astype(np.int32), new_cols[order].astype(np.int32),
+                     system.V[order].copy(), system.dirichlet[perm].copy())
 
 
 def splitmix64(seed: int, count: int) -> np.ndarray:

@@ -9,8 +9,11 @@ Each fixture holds inputs that are cheap to regenerate (the matrix comes from
 simplefem_amd.systems by name, the right-hand side from a seed) and the
 outputs of the compiled reference at one OpenMP thread: colourings
 (F, B, R), iteration counts, final residual norms and, for the small
-systems, x and residual in full.  For the two full-size systems only
+systems, x and residual in full.  For the full-size systems only
 SHA-256 digests of F/B/R, the colour count and the iteration counts are kept.
+The reference's refined unit square / unit cube and its P1 stiffness
+matrices are kept in full for small levels (mesh_*.npz) and as SHA-256
+digests of the CSR structure at the full-size levels (2D level 7, 3D level 6).
 """
 import hashlib
 import json
@@ -38,7 +41,14 @@ SMALL = {
 FULL = {
     "p1_2d_L7": lambda: S.poisson2d_p1(7),
     "kuhn_64": lambda: S.poisson3d_p1_kuhn(64),
+    # the reference's own vertex numbering (uniform refinement order)
+    "p1_2d_L7_refnum": lambda: S.poisson2d_p1(7, numbering="reference"),
+    "kuhn_64_refnum": lambda: S.poisson3d_p1_kuhn(64, numbering="reference"),
 }
+# (dimension, refinement levels) of the meshes whose full data are kept
+MESH_SMALL = [(2, 1), (2, 2), (2, 3), (2, 4), (3, 1), (3, 2), (3, 3)]
+# full-size meshes: SHA-256 digests of the reference's CSR structure
+MESH_FULL = {"p1_2d_L7_refnum": (2, 7), "kuhn_64_refnum": (3, 6)}
 
 
 def aux_for(variant, d):
@@ -53,9 +63,36 @@ def sha(a):
     return hashlib.sha256(np.ascontiguousarray(a, dtype=np.int32).tobytes()).hexdigest()
 
 
+def mesh_fixtures(ref, meta):
+    """The reference's refined meshes and their P1 stiffness matrices:
+    small ones in full (mesh_<d>d_L<l>.npz), the full-size ones as digests."""
+    meta["mesh"] = {}
+    for dim, lv in MESH_SMALL:
+        m = ref.mesh(dim, lv)
+        bA, bC, bV = m["structural"]
+        aA, aC, aV = m["compressed"]
+        np.savez_compressed(os.path.join(OUT, f"mesh_{dim}d_L{lv}.npz"), coords=m["coords"],
+                            simplices=m["simplices"], dirichlet=m["dirichlet"], sA=bA, sC=bC,
+                            sV=bV, cA=aA, cC=aC, cV=aV)
+        meta["mesh"][f"{dim}d_L{lv}"] = {"n": int(len(bA) - 1), "nnz": int(bA[-1]),
+                                         "compressed_nnz": int(aA[-1])}
+        print("mesh", dim, lv, flush=True)
+    for name, (dim, lv) in MESH_FULL.items():
+        m = ref.mesh(dim, lv)
+        bA, bC, _ = m["structural"]
+        aA, aC, _ = m["compressed"]
+        meta["mesh"][name] = {"dim": dim, "levels": lv, "n": int(len(bA) - 1),
+                              "nnz": int(bA[-1]), "sha256_A": sha(bA), "sha256_C": sha(bC),
+                              "compressed_nnz": int(aA[-1]), "sha256_compressed_A": sha(aA),
+                              "sha256_compressed_C": sha(aC),
+                              "dirichlet_rows": int(m["dirichlet"].sum())}
+        print("mesh", name, meta["mesh"][name]["nnz"], flush=True)
+
+
 def main():
     with RefRunner(1) as ref:
         meta = {"desired_precision": ref.info()["desired_precision"], "small": {}, "full": {}}
+        mesh_fixtures(ref, meta)
         for name, make in SMALL.items():
             s = make()
             b = S.rhs_uniform(s, 11)

@@ -40,9 +40,11 @@ def test_rainbow_bit_exact(make):
     assert np.array_equal(got.R, want[3])
 
 
-def test_rainbow_matches_reference_digest():
-    meta = json.load(open(os.path.join(GOLDEN, "golden.json")))["full"]["kuhn_64"]
-    s = S.poisson3d_p1_kuhn(64)
+@pytest.mark.parametrize("numbering", ["lexicographic", "reference"])
+def test_rainbow_matches_reference_digest(numbering):
+    name = "kuhn_64" + ("_refnum" if numbering == "reference" else "")
+    meta = json.load(open(os.path.join(GOLDEN, "golden.json")))["full"][name]
+    s = S.poisson3d_p1_kuhn(64, numbering=numbering)
     rb = capi.Rainbow(s.n, s.A, s.C)
     import hashlib
     sha = lambda a: hashlib.sha256(np.ascontiguousarray(a, dtype=np.int32).tobytes()).hexdigest()

@@ -273,23 +273,27 @@ def test_device_resident_system_matches_host_call():
         assert K * per_iter <= total <= (K + 1) * per_iter + st["groups_forward"] + st["groups_backward"]
 
 
-def test_full_size_3d_rainbow():
-    """The 64^3 Kuhn P1 Poisson system (250,047 active rows), rtol 1e-10, omega 1."""
-    system = S.poisson3d_p1_kuhn(64)
+@pytest.mark.parametrize("numbering", ["reference", "lexicographic"])
+def test_full_size_3d_rainbow(numbering):
+    """The 64^3 Kuhn P1 Poisson system (250,047 active rows), rtol 1e-10,
+    omega 1, in the reference's vertex numbering (its CSR structure and
+    10-colour Rainbow colouring) and in x-fastest numbering (9 colours)."""
+    system = S.poisson3d_p1_kuhn(64, numbering=numbering)
     b = S.rhs_uniform(system, seed=2026)
     io, ig = _check_parity("rainbow", system, b, omega=1.0)
-    print("[full] K oracle", io["iterations"], "K gpu", ig["iterations"], flush=True)
+    print("[full]", numbering, "K oracle", io["iterations"], "K gpu", ig["iterations"], flush=True)
 
 
+@pytest.mark.parametrize("numbering", ["reference", "lexicographic"])
 @pytest.mark.parametrize("omega", [1.0, 1.2])
-def test_full_size_2d_ssor(omega):
+def test_full_size_2d_ssor(omega, numbering):
     """The 129^2 P1 Poisson system (16,129 active rows) with natural-order SSOR,
     the reference's own CPU configuration: ~250 dependency levels per sweep."""
-    system = S.poisson2d_p1(7)
+    system = S.poisson2d_p1(7, numbering=numbering)
     b = S.rhs_uniform(system, seed=2026)
     io, ig = _check_parity("ssor", system, b, omega=omega)
-    print("[full 2d] omega", omega, "K oracle", io["iterations"], "K gpu", ig["iterations"],
-          flush=True)
+    print("[full 2d]", numbering, "omega", omega, "K oracle", io["iterations"], "K gpu",
+          ig["iterations"], flush=True)
 
 
 def test_cpp_dropin_callers():

@@ -68,8 +68,11 @@ def _sha(a):
     return hashlib.sha256(np.ascontiguousarray(a, dtype=np.int32).tobytes()).hexdigest()
 
 
-@pytest.mark.parametrize("name,make", [("p1_2d_L7", lambda: S.poisson2d_p1(7)),
-                                       ("kuhn_64", lambda: S.poisson3d_p1_kuhn(64))])
+@pytest.mark.parametrize("name,make", [
+    ("p1_2d_L7", lambda: S.poisson2d_p1(7)),
+    ("kuhn_64", lambda: S.poisson3d_p1_kuhn(64)),
+    ("p1_2d_L7_refnum", lambda: S.poisson2d_p1(7, numbering="reference")),
+    ("kuhn_64_refnum", lambda: S.poisson3d_p1_kuhn(64, numbering="reference"))])
 def test_golden_full_size_colouring(name, make):
     """Config-size systems: colouring digests of the reference's Rainbow."""
     s = make()
@@ -80,10 +83,11 @@ def test_golden_full_size_colouring(name, make):
     assert _sha(F) == meta["sha256_F"] and _sha(B) == meta["sha256_B"] and _sha(R) == meta["sha256_R"]
 
 
-def test_golden_full_size_2d_iterations():
+@pytest.mark.parametrize("numbering", ["lexicographic", "reference"])
+def test_golden_full_size_2d_iterations(numbering):
     """2D P1 129^2 (16,129 active rows): K of natural and Rainbow SSOR."""
-    s = S.poisson2d_p1(7)
-    meta = META["full"]["p1_2d_L7"]
+    s = S.poisson2d_p1(7, numbering=numbering)
+    meta = META["full"]["p1_2d_L7" + ("_refnum" if numbering == "reference" else "")]
     b = S.rhs_uniform(s, meta["rhs_seed"])
     d = s.diagonal()
     col = oracle.rainbow(s.n, s.A, s.C)

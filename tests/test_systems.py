@@ -142,3 +142,75 @@ def test_workloads_are_seeded():
     t = [j.arrival for j in req]
     assert t == sorted(t) and t[0] > 0
     assert all(100 <= j.system.n <= 20000 for j in req)
+
+
+# ------------------------------------------------- the reference's vertex numbering
+
+import hashlib  # noqa: E402
+import json  # noqa: E402
+import os  # noqa: E402
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+MESH_LEVELS = [(2, 1), (2, 2), (2, 3), (2, 4), (3, 1), (3, 2), (3, 3)]
+
+
+def _refnum(dim, levels):
+    if dim == 2:
+        return S.poisson2d_p1(levels, numbering="reference")
+    return S.poisson3d_p1_kuhn(1 << levels, numbering="reference")
+
+
+def _drop_zeros(s):
+    keep = s.V != 0
+    rows = np.repeat(np.arange(s.n), np.diff(s.A))
+    A = np.concatenate([[0], np.cumsum(np.bincount(rows[keep], minlength=s.n))])
+    return A.astype(np.int32), s.C[keep], s.V[keep]
+
+
+@pytest.mark.parametrize("dim,levels", MESH_LEVELS)
+def test_reference_numbering_matches_the_reference_mesh(dim, levels):
+    """The restated uniform refinement numbers vertices and simplices as the
+    reference's does (coordinates and vertex lists of mesh_*.npz, written by
+    the compiled reference); the reference-numbered P1 matrix has the CSR
+    structure of its LagrangeCoefficientStiffnessMatrix and, with exact zeros
+    dropped, equals its compressed MatrixCSR(LagrangeStiffnessMatrix)."""
+    g = np.load(os.path.join(GOLDEN, f"mesh_{dim}d_L{levels}.npz"))
+    X, TV = (S._reference_mesh_2d if dim == 2 else S._reference_mesh_3d)(levels)
+    assert np.array_equal(X / float(1 << levels), g["coords"])
+    assert np.array_equal(TV, g["simplices"])
+    s = _refnum(dim, levels)
+    assert np.array_equal(s.dirichlet, g["dirichlet"])
+    assert np.array_equal(s.A, g["sA"]) and np.array_equal(s.C, g["sC"])
+    # the reference's identity-weight quadrature scales every value by one
+    # constant: the same zero pattern
+    assert np.array_equal(s.V == 0, g["sV"] == 0)
+    A, C, V = _drop_zeros(s)
+    assert np.array_equal(A, g["cA"]) and np.array_equal(C, g["cC"])
+    # values: exact sums of exact element matrices here; the reference's
+    # floating-point element matrices (an inverse of J^T J per element) land
+    # within a few units in the last place (exactly, in 2D)
+    assert np.all(np.abs(V - g["cV"]) <= 1e-15 * np.abs(g["cV"]))
+    if dim == 2:
+        assert np.array_equal(V, g["cV"])
+
+
+def test_reference_numbering_full_size_digests():
+    """C1 (2D, level 7) and C2 (3D, level 6): CSR structure digests of the
+    reference's own assembly, structural and compressed."""
+    meta = json.load(open(os.path.join(GOLDEN, "golden.json")))["mesh"]
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a, dtype=np.int32).tobytes()).hexdigest()
+    for name in ("p1_2d_L7_refnum", "kuhn_64_refnum"):
+        m = meta[name]
+        s = _refnum(m["dim"], m["levels"])
+        assert (s.n, s.nnz) == (m["n"], m["nnz"]), name
+        assert (sha(s.A), sha(s.C)) == (m["sha256_A"], m["sha256_C"]), name
+        A, C, _ = _drop_zeros(s)
+        assert (len(C), sha(A), sha(C)) == (m["compressed_nnz"], m["sha256_compressed_A"],
+                                            m["sha256_compressed_C"]), name
+        assert int(s.dirichlet.sum()) == m["dirichlet_rows"], name
+
+
+def test_reference_permutation_is_a_bijection_of_the_grid():
+    for dim, levels in [(2, 5), (3, 4)]:
+        p = S.reference_permutation(dim, levels)
+        assert np.array_equal(np.sort(p), np.arange(((1 << levels) + 1) ** dim))
