@@ -28,12 +28,13 @@ sys.path.insert(0, ROOT)
 
 METRIC = "solves/s + HBM GB/s (3D P1 Poisson, rtol 1e-10); p50/p99 served latency"
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per launch of each kernel class on the default workload (64^3,
-# omega 1): rocprofv3 FETCH_SIZE and WRITE_SIZE runs, each scaled by what the
+# HBM bytes per launch of each kernel class on the default workload (64^3 in
+# the reference's vertex numbering, omega 1, paired sweeps): rocprofv3
+# FETCH_SIZE and WRITE_SIZE runs, each scaled by what the
 # counter reads for a device-to-device copy of known size (bench.py
 # --copy-calibration); commands and per-launch figures are kept in
 # profiles/rainbow_cg_ssor_64cube_fp64_hbm_traffic.md.
-MEASURED_TRAFFIC = {"spmv": 55312525, "forward": 6585245, "backward": 5002750}
+MEASURED_TRAFFIC = {"spmv": 56382191, "forward": 6474410, "backward": 5797894}
 
 
 def log(msg):
@@ -204,7 +205,8 @@ def main():
         dom = max(prof, key=lambda k: prof[k]["ms"])
         p = prof[dom]
         achieved = p["bytes"] / (p["ms"] * 1e-3) / 1e9
-        default_workload = args.cube == 64 and args.omega == 1.0
+        default_workload = (args.cube == 64 and args.omega == 1.0 and args.numbering == "reference"
+                            and not args.single_group_sweeps)
         traffic = MEASURED_TRAFFIC.get(dom) if default_workload else None
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
