@@ -29,12 +29,12 @@ sys.path.insert(0, ROOT)
 METRIC = "solves/s + HBM GB/s (3D P1 Poisson, rtol 1e-10); p50/p99 served latency"
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of each kernel class on the default workload (64^3 in
-# the reference's vertex numbering, omega 1, paired sweeps): rocprofv3
+# the reference's vertex numbering, omega 1, one colour per sweep launch): rocprofv3
 # FETCH_SIZE and WRITE_SIZE runs, each scaled by what the
 # counter reads for a device-to-device copy of known size (bench.py
 # --copy-calibration); commands and per-launch figures are kept in
 # profiles/rainbow_cg_ssor_64cube_fp64_hbm_traffic.md.
-MEASURED_TRAFFIC = {"spmv": 56382191, "forward": 6474410, "backward": 5797894}
+MEASURED_TRAFFIC = {"spmv": 56382191, "forward": 6474410, "backward": 5118184}
 
 
 def log(msg):
@@ -101,8 +101,8 @@ def main():
     ap.add_argument("--numbering", default="reference", choices=["reference", "lexicographic"],
                     help="vertex numbering of the 64^3 system: the reference's uniform-refinement "
                          "order (its CSR structure and colouring; default) or x-fastest")
-    ap.add_argument("--single-group-sweeps", action="store_true",
-                    help="one sweep group per launch (no paired colours)")
+    ap.add_argument("--paired-sweeps", action="store_true",
+                    help="two colours per sweep launch where that is cheap (SFA_FLAG_PAIRED_SWEEPS)")
     ap.add_argument("--spmv-lanes", type=int, default=0, choices=[0, 1, 2, 4, 8],
                     help="SpMV lanes per row; 0: autotune on the system (default)")
     ap.add_argument("--config", default="poisson3d",
@@ -155,7 +155,7 @@ def main():
         f"built in {time.time() - t0:.1f}s")
 
     sysd = capi.System(capi.METHOD_RAINBOW, system.n, system.A, system.C, system.V, d,
-                       args.omega, *colouring, single_group_sweeps=args.single_group_sweeps)
+                       args.omega, *colouring, paired_sweeps=args.paired_sweeps)
     if args.spmv_lanes:
         sysd.set_spmv(args.spmv_lanes)
         tuning = {"lanes_per_row": args.spmv_lanes, "tuned": False}
@@ -206,7 +206,7 @@ def main():
         p = prof[dom]
         achieved = p["bytes"] / (p["ms"] * 1e-3) / 1e9
         default_workload = (args.cube == 64 and args.omega == 1.0 and args.numbering == "reference"
-                            and not args.single_group_sweeps)
+                            and not args.paired_sweeps)
         traffic = MEASURED_TRAFFIC.get(dom) if default_workload else None
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
                     "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,

@@ -47,12 +47,14 @@ extern "C" {
 /* OR into the method of sfa_system_create: also keep an FP32 copy of the
  * matrix and schedules for sfa_system_solve_mixed */
 #define SFA_FLAG_FP32_INNER 0x100
-/* Or-ed into the method of sfa_system_create: run every sweep group in a
- * launch of its own.  By default two consecutive colours of a Rainbow sweep
- * share one launch where the second reads at most 8 rows of the first per
- * row (the second colour's blocks form those rows' values themselves, with
- * the same arithmetic); results are the same bit for bit either way. */
-#define SFA_FLAG_SINGLE_GROUP_SWEEPS 0x200
+/* Or-ed into the method of sfa_system_create: two consecutive colours of a
+ * Rainbow sweep share one launch where the second reads few rows of the
+ * first (the second colour's blocks form those rows' values themselves, with
+ * the same arithmetic), fewer launches per iteration; results are the same
+ * bit for bit as with one colour per launch (the default).  Measured on
+ * MI355X: +1% on the 64^3 P1 Poisson system, -13% on the 48^3 H(curl)
+ * system, so off unless asked for. */
+#define SFA_FLAG_PAIRED_SWEEPS 0x200
 
 /* solver variants (one per function of solver/sparsesolver.hpp:11-84) */
 #define SFA_METHOD_CG                  0

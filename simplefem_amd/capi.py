@@ -80,7 +80,7 @@ class SpmvTuning(ctypes.Structure):
 
 
 FLAG_FP32_INNER = 0x100
-FLAG_SINGLE_GROUP_SWEEPS = 0x200
+FLAG_PAIRED_SWEEPS = 0x200
 BATCH_FP64, BATCH_MIXED = 0, 1
 
 
@@ -359,12 +359,12 @@ class System:
     """
 
     def __init__(self, method, N, A, C, V, aux=None, omega=1.0, num_colors=0, F=None, B=None,
-                 R=None, fp32_inner=False, single_group_sweeps=False):
+                 R=None, fp32_inner=False, paired_sweeps=False):
         lib = load()
         if fp32_inner:
             method |= FLAG_FP32_INNER
-        if single_group_sweeps:
-            method |= FLAG_SINGLE_GROUP_SWEEPS
+        if paired_sweeps:
+            method |= FLAG_PAIRED_SWEEPS
         A, C, V = _csr(A, C, V)
         aux = _f64(aux, "aux", N)
         F, B, R = _i32(F, "F"), _i32(B, "B"), _i32(R, "R")

@@ -70,7 +70,7 @@ struct SystemT {
         int g;
         bool pair;
     };
-    bool pair_sweeps = true;  // false: one group per launch
+    bool pair_sweeps = false;  // true: pairs of groups where possible (SFA_FLAG_PAIRED_SWEEPS)
     std::vector<SweepLaunch> fwd_sched, bwd_sched;
     std::vector<int> defer_bwd;
     std::vector<int> spmv_tail;
@@ -1201,8 +1201,8 @@ int sfa_system_create(int method, int N, const int* csrrows, const int* csrcolum
         *out = nullptr;
         std::unique_ptr<sfa_system_s> h(new sfa_system_s());
         const bool fp32 = (method & SFA_FLAG_FP32_INNER) != 0;
-        h->sys.pair_sweeps = (method & SFA_FLAG_SINGLE_GROUP_SWEEPS) == 0;
-        h->sys.create(method & ~(SFA_FLAG_FP32_INNER | SFA_FLAG_SINGLE_GROUP_SWEEPS), N, csrrows,
+        h->sys.pair_sweeps = (method & SFA_FLAG_PAIRED_SWEEPS) != 0;
+        h->sys.create(method & ~(SFA_FLAG_FP32_INNER | SFA_FLAG_PAIRED_SWEEPS), N, csrrows,
                       csrcolumns, csrvalues, aux, omega,
                       num_colors, F, B, R, fp32);
         *out = h.release();

@@ -51,7 +51,8 @@ def hcurl_mixed(args):
     rb = capi.Rainbow(s.n, s.A, s.C)
     col = (rb.num_colors, rb.F, rb.B, rb.R)
     log(f"hcurl system n={s.n} nnz={s.nnz} colours={rb.num_colors} ({time.time() - t0:.1f}s)")
-    sysd = capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, d, 1.0, *col, fp32_inner=True)
+    sysd = capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, d, 1.0, *col, fp32_inner=True,
+                       paired_sweeps=args.paired_sweeps)
     if args.spmv_lanes:
         sysd.set_spmv(args.spmv_lanes)
         tuning = {"lanes_per_row": args.spmv_lanes, "tuned": False}

@@ -351,9 +351,9 @@ def test_paired_sweeps_are_bit_identical(maker):
     d = system.diagonal()
     col = oracle.rainbow(system.n, system.A, system.C)
     out = []
-    for single in (False, True):
+    for paired in (True, False):
         with capi.System(capi.METHOD_RAINBOW, system.n, system.A, system.C, system.V, d, 1.0,
-                         *col, fp32_inner=True, single_group_sweeps=single) as sysd:
+                         *col, fp32_inner=True, paired_sweeps=paired) as sysd:
             x = np.zeros(system.n)
             r = np.zeros(system.n)
             info = {}
