@@ -364,6 +364,15 @@ int sfa_batch_collect(sfa_batch b, int slot, double* x, sfa_batch_info* info);
 /* Device memory held by matrices and slots. */
 int sfa_batch_memory(sfa_batch b, long long* bytes);
 
+/* Steps the engine has executed since the batch was created (each step runs
+ * the current phase of every running solve: three kernels). */
+int sfa_batch_steps(sfa_batch b, long long* steps);
+/* Runs `steps` engine steps with each of the three kernels launched on its
+ * own between two HIP events and adds their summed times, in milliseconds,
+ * to ms[0] (step kernel), ms[1] (per-slot fold) and ms[2] (schedule).  The
+ * solves advance exactly as under sfa_batch_run.  No reference counterpart. */
+int sfa_batch_profile(sfa_batch b, int steps, double* ms);
+
 /* Autotunes the batch's SpMV mapping per row-length bucket: for each bucket
  * that holds matrices (a matrix's bucket is the one holding most of its
  * entries), times the SpMV phase of all those matrices under every candidate

@@ -175,6 +175,8 @@ _SIGNATURES = {
     "sfa_batch_finished": ([c_void_p, _P_I, c_int, POINTER(c_int)], c_int),
     "sfa_batch_collect": ([c_void_p, c_int, _P_D, POINTER(BatchInfo)], c_int),
     "sfa_batch_memory": ([c_void_p, POINTER(c_longlong)], c_int),
+    "sfa_batch_steps": ([c_void_p, POINTER(c_longlong)], c_int),
+    "sfa_batch_profile": ([c_void_p, c_int, _P_D], c_int),
     "sfa_batch_tune_spmv": ([c_void_p, POINTER(SpmvTuning)], c_int),
     "sfa_batch_set_spmv": ([c_void_p, _P_I], c_int),
     "sfa_batch_destroy": ([c_void_p], c_int),
@@ -525,6 +527,19 @@ class Batch:
         info = BatchInfo()
         _check(load().sfa_batch_collect(self._h, int(slot), _d(x), ctypes.byref(info)))
         return x, info.as_dict()
+
+    def profile(self, steps):
+        """Runs `steps` steps kernel by kernel between HIP events:
+        {"step_ms", "fold_ms", "prefix_ms"} summed over the steps."""
+        ms = np.zeros(3)
+        _check(load().sfa_batch_profile(self._h, int(steps), _d(ms)))
+        return {"step_ms": ms[0], "fold_ms": ms[1], "prefix_ms": ms[2]}
+
+    def steps(self):
+        """Engine steps executed so far."""
+        n = c_longlong(0)
+        _check(load().sfa_batch_steps(self._h, ctypes.byref(n)))
+        return n.value
 
     def tune_spmv(self):
         """Per-bucket SpMV autotuning over the batch's matrices (between solves)."""

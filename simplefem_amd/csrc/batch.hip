@@ -9,11 +9,12 @@
 //
 // the same phases as the single-system kernels, with the same per-row
 // arithmetic.  A "step" executes the current phase of every active slot; a
-// phase is split into work units of 256 rows (four SELL-64 slices, one per
-// wavefront of a 256-thread block).  A step is three kernels:
+// phase is split into work units of 64 rows (one SELL-64 slice, or 64
+// vector rows), each run by one wavefront.  A step is three kernels:
 //
-//   k_batch_step    every unit of the step's schedule (grid-stride); each
-//                   unit stores its partial sum in its slot's partials[j];
+//   k_batch_step    every unit of the step's schedule (each wavefront
+//                   grid-strides over units on its own); each unit stores
+//                   its partial sum in its slot's partials[j];
 //   k_batch_fold    one block per slot: sums the slot's unit partials in unit
 //                   order, runs the slot's scalar step (alpha / beta / exit
 //                   tests) and advances its program, then counts the units
@@ -49,8 +50,10 @@ namespace {
 
 using namespace dev;
 
-constexpr int kUnitBlock = 256;             // threads per block = rows per unit
-constexpr int kUnitSlices = kUnitBlock / 64;
+constexpr int kUnitBlock = 256;             // threads per block of the engine's kernels
+constexpr int kUnitRows = 64;               // rows per work unit: one wavefront, one slice
+constexpr int kStepWaves = kUnitBlock / 64; // wavefronts per block
+constexpr int kRiders = 4;                  // most adjacent slots one unit serves
 constexpr int kMaxSlots = 1024;
 
 enum Phase : int {
@@ -99,6 +102,8 @@ struct BSched {
     int active;         // slots with units in the next step
     int pad;
     int start[2][kMaxSlots + 1];  // unit offsets of the slots in the step of parity p
+    int ran[2][kMaxSlots];        // units of the phase each slot runs in that step
+    int riders[2][kMaxSlots];     // slots a leading slot's units serve (itself included), else 0
 };
 
 template <typename T>
@@ -115,17 +120,17 @@ template <typename T>
 __device__ int phase_units(const BMat<T>& M, int kind, int g)
 {
     switch (kind) {
-    case PH_RESID: return cdiv(M.full_slices, kUnitSlices);
-    case PH_SPMV: return max(1, cdiv(M.full_slices * M.spmv_lanes, kUnitSlices));
-    case PH_RES64: return max(1, cdiv(M.full_slices, kUnitSlices));
+    case PH_RESID: return M.full_slices;
+    case PH_SPMV: return max(1, M.full_slices * M.spmv_lanes);
+    case PH_RES64: return max(1, M.full_slices);
     case PH_FWD_R:
-    case PH_FWD: return cdiv(M.fslice[g + 1] - M.fslice[g], kUnitSlices);
-    case PH_BWD_R: return cdiv(M.bslice[g + 1] - M.bslice[g], kUnitSlices);
-    case PH_BWD: return max(1, cdiv(M.bslice[g + 1] - M.bslice[g], kUnitSlices));
-    case PH_DINIT: return max(1, cdiv(M.n_active, kUnitBlock));
+    case PH_FWD: return M.fslice[g + 1] - M.fslice[g];
+    case PH_BWD_R: return M.bslice[g + 1] - M.bslice[g];
+    case PH_BWD: return max(1, M.bslice[g + 1] - M.bslice[g]);
+    case PH_DINIT: return max(1, cdiv(M.n_active, kUnitRows));
     case PH_DIR:
     case PH_CONVERT:
-    case PH_CORRECT: return cdiv(M.n_active, kUnitBlock);
+    case PH_CORRECT: return cdiv(M.n_active, kUnitRows);
     default: return 0;
     }
 }
@@ -283,110 +288,241 @@ __device__ double spmv_task_any(const BMat<T>& M, const T* d, T* q, int task, in
     }
 }
 
-// One work unit: 256 rows of the slot's current phase.  Returns the block's
-// partial sum for reduction phases (valid in thread 0).
+// acc[k] (+|-)= sum_e v_e * vec[k][c_e] over one SELL row for riders k < nr,
+// each entry's column and value loaded once for all riders; every rider's
+// sum runs over the entries in order, as sell_row forms it.
+template <bool SUB, typename TV>
+__device__ inline void sell_row_multi(const SellViewT<TV>& A, int slice, int lane,
+                                      const TV* const* vec, int nr, TV* acc)
+{
+    const int beg = A.off[slice] + lane;
+    const int end = A.off[slice + 1];
+    constexpr int U = 8;
+    int e = beg;
+    for (; e + (U - 1) * kWave < end; e += U * kWave) {
+        unsigned c[U];
+        TV v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            c[u] = (unsigned)__builtin_nontemporal_load(&A.cols[e + u * kWave]);
+            v[u] = __builtin_nontemporal_load(&A.vals[e + u * kWave]);
+        }
+#pragma unroll
+        for (int k = 0; k < kRiders; k++) {
+            if (k < nr) {
+                TV g[U];
+#pragma unroll
+                for (int u = 0; u < U; u++) g[u] = vec[k][c[u]];
+#pragma unroll
+                for (int u = 0; u < U; u++) {
+                    if (SUB) acc[k] -= v[u] * g[u];
+                    else     acc[k] += v[u] * g[u];
+                }
+            }
+        }
+    }
+    for (; e < end; e += kWave) {
+        const unsigned c = (unsigned)__builtin_nontemporal_load(&A.cols[e]);
+        const TV v = __builtin_nontemporal_load(&A.vals[e]);
+#pragma unroll
+        for (int k = 0; k < kRiders; k++) {
+            if (k < nr) {
+                if (SUB) acc[k] -= v * vec[k][c];
+                else     acc[k] += v * vec[k][c];
+            }
+        }
+    }
+}
+
+// One work unit of the current phase of slots SS[0..nr) -- adjacent slots of
+// one matrix in the same phase ("riders"; nr = 1 for a slot on its own) --
+// run by one wavefront: slice j of the phase's part (SpMV task j), or vector
+// rows 64j .. 64j+63.  Matrix entries are loaded once for all riders; each
+// rider's rows get exactly the arithmetic they get alone.  part[k]: the
+// lane's share of rider k's reduction.
 template <typename T>
-__device__ double run_unit(BSlot<T>& S, const BMat<T>& M, int kind, int g, int j)
+__device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind, int g, int j,
+                         double* part)
 {
     const int lane = threadIdx.x & 63;
-    const int w = threadIdx.x >> 6;
-    double part = 0.;
+#pragma unroll
+    for (int k = 0; k < kRiders; k++) part[k] = 0.;
     switch (kind) {
     case PH_RESID: {
-        const int local = j * kUnitSlices + w;
+        const int local = j;
         if (local < M.full_slices) {
             const int row = group_row(M.full, local, local, lane, M.full_row0, M.n_active);
-            if (row >= 0) S.r[row] = sell_row<true>(M.full, local, lane, S.x, S.b[row]);
+            if (row >= 0) {
+                const T* xs[kRiders];
+                T acc[kRiders];
+#pragma unroll
+                for (int k = 0; k < kRiders; k++) {
+                    xs[k] = k < nr ? SS[k].x : SS[0].x;
+                    acc[k] = k < nr ? SS[k].b[row] : T(0);
+                }
+                sell_row_multi<true>(M.full, local, lane, xs, nr, acc);
+#pragma unroll
+                for (int k = 0; k < kRiders; k++)
+                    if (k < nr) SS[k].r[row] = acc[k];
+            }
         }
         break;
     }
     case PH_SPMV:
-        part = spmv_task_any(M, S.d, S.q, j * kUnitSlices + w, lane);
+        if (M.spmv_lanes == 1) {
+            const int local = j;
+            if (local < M.full_slices) {
+                const int row = group_row(M.full, local, local, lane, M.full_row0, M.n_active);
+                if (row >= 0) {
+                    const T* ds[kRiders];
+                    T acc[kRiders];
+#pragma unroll
+                    for (int k = 0; k < kRiders; k++) {
+                        ds[k] = k < nr ? SS[k].d : SS[0].d;
+                        acc[k] = T(0);
+                    }
+                    sell_row_multi<false>(M.full, local, lane, ds, nr, acc);
+#pragma unroll
+                    for (int k = 0; k < kRiders; k++) {
+                        if (k < nr) {
+                            SS[k].q[row] = acc[k];
+                            part[k] = (double)(SS[k].d[row] * acc[k]);
+                        }
+                    }
+                }
+            }
+        } else {
+            for (int k = 0; k < nr; k++) part[k] = spmv_task_any(M, SS[k].d, SS[k].q, j, lane);
+        }
         break;
     case PH_RES64: {
-        const int local = j * kUnitSlices + w;
+        const int local = j;
         if (local < M.full_slices) {
             const int row = group_row(M.full64, local, local, lane, M.full_row0, M.n_active);
             if (row >= 0) {
-                const double r = sell_row<true>(M.full64, local, lane, S.x64, S.b64[row]);
-                S.r64[row] = r;
-                part = r * r;
+                const double* xs[kRiders];
+                double acc[kRiders];
+#pragma unroll
+                for (int k = 0; k < kRiders; k++) {
+                    xs[k] = k < nr ? SS[k].x64 : SS[0].x64;
+                    acc[k] = k < nr ? SS[k].b64[row] : 0.;
+                }
+                sell_row_multi<true>(M.full64, local, lane, xs, nr, acc);
+#pragma unroll
+                for (int k = 0; k < kRiders; k++) {
+                    if (k < nr) {
+                        SS[k].r64[row] = acc[k];
+                        part[k] = acc[k] * acc[k];
+                    }
+                }
             }
         }
         break;
     }
     case PH_FWD_R:
     case PH_FWD: {
-        const int local = j * kUnitSlices + w;
+        const int local = j;
         const int slice = M.fslice[g] + local;
         if (slice < M.fslice[g + 1]) {
             const int row = group_row(M.lower, slice, local, lane, M.frow0[g], M.frows[g]);
             if (row >= 0) {
-                T rr;
-                if (kind == PH_FWD) {
-                    const T alpha = (T)S.alpha;
-                    S.x[row] += alpha * S.d[row];
-                    rr = S.r[row] - alpha * S.q[row];
-                    S.r[row] = rr;
-                } else {
-                    rr = S.r[row];
+                const T* ms[kRiders];
+                T acc[kRiders];
+#pragma unroll
+                for (int k = 0; k < kRiders; k++) {
+                    ms[k] = k < nr ? SS[k].m : SS[0].m;
+                    acc[k] = T(0);
+                    if (k < nr) {
+                        const BSlot<T>& S = SS[k];
+                        T rr;
+                        if (kind == PH_FWD) {
+                            const T alpha = (T)S.alpha;
+                            S.x[row] += alpha * S.d[row];
+                            rr = S.r[row] - alpha * S.q[row];
+                            S.r[row] = rr;
+                        } else {
+                            rr = S.r[row];
+                        }
+                        acc[k] = rr;
+                    }
                 }
-                const T acc = sell_row<true>(M.lower, slice, lane, S.m, rr);
-                S.m[row] = acc * M.omega / M.diag[row];
+                sell_row_multi<true>(M.lower, slice, lane, ms, nr, acc);
+                const T dg = M.diag[row];
+#pragma unroll
+                for (int k = 0; k < kRiders; k++)
+                    if (k < nr) SS[k].m[row] = acc[k] * M.omega / dg;
             }
         }
         break;
     }
     case PH_BWD_R:
     case PH_BWD: {
-        const int local = j * kUnitSlices + w;
+        const int local = j;
         const int slice = M.bslice[g] + local;
         if (slice < M.bslice[g + 1]) {
             const int row = group_row(M.upper, slice, local, lane, M.brow0[g], M.brows[g]);
             if (row >= 0) {
                 const T dg = M.diag[row];
-                T t = S.m[row];
-                t *= dg;
-                t *= M.c2;
-                const T acc = sell_row<true>(M.upper, slice, lane, S.z, t);
-                const T zz = acc * M.omega / dg;
-                S.z[row] = zz;
-                if (kind == PH_BWD) part = (double)(zz * S.r[row]);
+                const T* zs[kRiders];
+                T acc[kRiders];
+#pragma unroll
+                for (int k = 0; k < kRiders; k++) {
+                    zs[k] = k < nr ? SS[k].z : SS[0].z;
+                    acc[k] = T(0);
+                    if (k < nr) {
+                        T t = SS[k].m[row];
+                        t *= dg;
+                        t *= M.c2;
+                        acc[k] = t;
+                    }
+                }
+                sell_row_multi<true>(M.upper, slice, lane, zs, nr, acc);
+#pragma unroll
+                for (int k = 0; k < kRiders; k++) {
+                    if (k < nr) {
+                        const T zz = acc[k] * M.omega / dg;
+                        SS[k].z[row] = zz;
+                        if (kind == PH_BWD) part[k] = (double)(zz * SS[k].r[row]);
+                    }
+                }
             }
         }
         break;
     }
     case PH_DINIT: {
-        const int i = j * kUnitBlock + threadIdx.x;
-        if (i < M.n_active) {
-            const T zz = S.z[i];
-            S.d[i] = zz;
-            part = (double)(zz * S.r[i]);
-        }
+        const int i = j * kUnitRows + lane;
+        if (i < M.n_active)
+            for (int k = 0; k < nr; k++) {
+                const T zz = SS[k].z[i];
+                SS[k].d[i] = zz;
+                part[k] = (double)(zz * SS[k].r[i]);
+            }
         break;
     }
     case PH_DIR: {
-        const int i = j * kUnitBlock + threadIdx.x;
-        if (i < M.n_active) S.d[i] = S.z[i] + (T)S.beta * S.d[i];
+        const int i = j * kUnitRows + lane;
+        if (i < M.n_active)
+            for (int k = 0; k < nr; k++) SS[k].d[i] = SS[k].z[i] + (T)SS[k].beta * SS[k].d[i];
         break;
     }
     case PH_CONVERT: {
-        const int i = j * kUnitBlock + threadIdx.x;
-        if (i < M.n_active) {
-            S.b[i] = (T)S.r64[i];
-            S.x[i] = T(0);
-            S.r[i] = T(0);
-        }
+        const int i = j * kUnitRows + lane;
+        if (i < M.n_active)
+            for (int k = 0; k < nr; k++) {
+                SS[k].b[i] = (T)SS[k].r64[i];
+                SS[k].x[i] = T(0);
+                SS[k].r[i] = T(0);
+            }
         break;
     }
     case PH_CORRECT: {
-        const int i = j * kUnitBlock + threadIdx.x;
-        if (i < M.n_active) S.x64[i] += (double)S.x[i];
+        const int i = j * kUnitRows + lane;
+        if (i < M.n_active)
+            for (int k = 0; k < nr; k++) SS[k].x64[i] += (double)SS[k].x[i];
         break;
     }
     default: break;
     }
-    return part;
 }
 
 __device__ inline double block_sum(double v, double* ws)
@@ -430,21 +566,37 @@ __device__ inline int unit_owner(const int* start, int ns, int u)
 template <typename T>
 __global__ void __launch_bounds__(kUnitBlock) k_batch_step(BArgs<T> a)
 {
-    __shared__ double ws[kUnitBlock / 64];
     __shared__ int start[kMaxSlots + 1];  // the step's unit offsets, read once per block
     const BSched* sc = a.sched;
     const int p = (int)(sc->step & 1u);
     const int W = sc->W[p];
     const int ns = sc->nslots;
-    if ((int)blockIdx.x >= W) return;
+    if ((int)blockIdx.x * kStepWaves >= W) return;
     for (int i = threadIdx.x; i <= ns; i += kUnitBlock) start[i] = sc->start[p][i];
     __syncthreads();
-    for (int u = blockIdx.x; u < W; u += gridDim.x) {
-        const int lo = unit_owner(start, ns, u);
-        BSlot<T>& S = a.slots[lo];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    // Each wavefront runs its own units, grid-stride; the four wavefronts of
+    // a block take consecutive units, which mostly belong to one slot, so the
+    // slot record and matrix descriptor (wave-uniform, read by scalar loads)
+    // come from the scalar cache after the first.  No wavefront waits for
+    // another: each unit's partial is a wavefront sum.  A unit of a leading
+    // slot also serves its riders (see k_batch_prefix).
+    const int* riders = sc->riders[p];
+    for (int u = blockIdx.x * kStepWaves + wave; u < W; u += gridDim.x * kStepWaves) {
+        const int lo = __builtin_amdgcn_readfirstlane(unit_owner(start, ns, u));
+        const BSlot<T>* SS = a.slots + lo;
+        const int nr = riders[lo];
         const int j = u - start[lo];
-        const double part = block_sum(run_unit(S, a.mats[S.mat], S.kind, S.group, j), ws);
-        if (threadIdx.x == 0) S.partials[j] = part;
+        double part[kRiders];
+        run_unit(SS, nr, a.mats[SS[0].mat], SS[0].kind, SS[0].group, j, part);
+#pragma unroll
+        for (int k = 0; k < kRiders; k++) {
+            if (k < nr) {
+                const double t = wave_sum(part[k]);
+                if (lane == 0) SS[k].partials[j] = t;
+            }
+        }
     }
 }
 
@@ -462,7 +614,7 @@ __global__ void __launch_bounds__(kUnitBlock) k_batch_fold(BArgs<T> a)
     }
     const BMat<T>& M = a.mats[S.mat];
     const int p = (int)(sc->step & 1u);
-    const int ran = sc->start[p][s + 1] - sc->start[p][s];
+    const int ran = sc->ran[p][s];
     if (ran > 0) {  // the slot's phase ran in this step
         double acc = 0.;
         for (int i = threadIdx.x; i < ran; i += kUnitBlock) acc += S.partials[i];
@@ -474,38 +626,107 @@ __global__ void __launch_bounds__(kUnitBlock) k_batch_fold(BArgs<T> a)
 
 constexpr int kPrefixBlock = kMaxSlots;
 
-template <typename T>
-__global__ void __launch_bounds__(kPrefixBlock) k_batch_prefix(BArgs<T> a)
+// Inclusive scans over the block (one value per thread): sum and maximum.
+__device__ inline int block_scan_add(int v, int* wtot)
 {
-    __shared__ int wsum[kPrefixBlock / 64];
-    __shared__ int active;
-    BSched* sc = a.sched;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int ns = sc->nslots;
-    const unsigned step = sc->step;
-    const int q = (int)((step + 1u) & 1u);
-    const int u = t < ns ? a.slots[t].units : 0;
-    if (t == 0) active = 0;
-    int v = u;  // inclusive scan: wavefront, then wavefront totals
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const int w = __shfl_up(v, o, 64);
         if (lane >= o) v += w;
     }
-    if (lane == 63) wsum[wave] = v;
+    if (lane == 63) wtot[wave] = v;
     __syncthreads();
-    if (u > 0) atomicAdd(&active, 1);
     if (t == 0) {
         int acc = 0;
         for (int w = 0; w < kPrefixBlock / 64; w++) {
-            const int x = wsum[w];
-            wsum[w] = acc;
+            const int x = wtot[w];
+            wtot[w] = acc;
             acc += x;
         }
     }
     __syncthreads();
-    v += wsum[wave];
-    if (t < ns) sc->start[q][t + 1] = v;
+    v += wtot[wave];
+    __syncthreads();
+    return v;
+}
+
+__device__ inline int block_scan_max(int v, int* wtot)
+{
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int w = __shfl_up(v, o, 64);
+        if (lane >= o) v = max(v, w);
+    }
+    if (lane == 63) wtot[wave] = v;
+    __syncthreads();
+    if (t == 0) {
+        int acc = -1;
+        for (int w = 0; w < kPrefixBlock / 64; w++) {
+            const int x = wtot[w];
+            wtot[w] = acc;
+            acc = max(acc, x);
+        }
+    }
+    __syncthreads();
+    v = max(v, wtot[wave]);
+    __syncthreads();
+    return v;
+}
+
+// The next step's schedule.  Adjacent running slots of one matrix in the
+// same phase (e.g. the right-hand sides of one system, submitted one after
+// another) form runs; each run is cut into groups of up to kRiders slots,
+// and only a group's first slot (its leader) gets units, which serve the
+// whole group: the matrix is read once for all its right-hand sides.  Which
+// slots ride together changes nothing in any slot's arithmetic.
+template <typename T>
+__global__ void __launch_bounds__(kPrefixBlock) k_batch_prefix(BArgs<T> a)
+{
+    __shared__ int wtot[kPrefixBlock / 64];
+    __shared__ int kmat[kPrefixBlock], kkind[kPrefixBlock], kgrp[kPrefixBlock];
+    __shared__ int kunits[kPrefixBlock], krun[kPrefixBlock];
+    __shared__ int active;
+    BSched* sc = a.sched;
+    const int t = threadIdx.x;
+    const int ns = sc->nslots;
+    const unsigned step = sc->step;
+    const int q = (int)((step + 1u) & 1u);
+    int u = 0, mat = -1, kind = -1, grp = -1;
+    if (t < ns) {
+        const BSlot<T>& S = a.slots[t];
+        u = S.units;
+        mat = S.mat;
+        kind = S.kind;
+        grp = S.group;
+    }
+    kmat[t] = mat;
+    kkind[t] = kind;
+    kgrp[t] = grp;
+    kunits[t] = u;
+    if (t == 0) active = 0;
+    __syncthreads();
+    if (u > 0) atomicAdd(&active, 1);
+    // a run starts at a running slot whose predecessor is idle or differs
+    const bool joins = u > 0 && t > 0 && kunits[t - 1] > 0 && kmat[t - 1] == mat &&
+                       kkind[t - 1] == kind && kgrp[t - 1] == grp;
+    const int run = block_scan_max(joins ? -1 : t, wtot);  // first slot of t's run
+    krun[t] = run;
+    __syncthreads();
+    const bool leader = u > 0 && (t - run) % kRiders == 0;
+    int nr = 0;
+    if (leader) {
+        nr = 1;
+        for (int k = 1; k < kRiders && t + k < ns; k++)
+            if (kunits[t + k] > 0 && krun[t + k] == run) nr++;
+    }
+    const int v = block_scan_add(leader ? u : 0, wtot);
+    if (t < ns) {
+        sc->start[q][t + 1] = v;
+        sc->ran[q][t] = u;
+        sc->riders[q][t] = nr;
+    }
     if (t == ns - 1) sc->W[q] = v;
     if (t == 0) {
         sc->start[q][0] = 0;
@@ -524,18 +745,18 @@ __global__ void __launch_bounds__(kUnitBlock) k_batch_spmv_probe(const BMat<T>* 
                                                                  const T* d, T* q,
                                                                  double* partials)
 {
-    __shared__ double ws[kUnitBlock / 64];
     __shared__ int start[kMaxSlots + 1];
     const int W = ustart[nm];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if ((int)blockIdx.x >= W) return;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if ((int)blockIdx.x * kStepWaves >= W) return;
     for (int i = threadIdx.x; i <= nm; i += kUnitBlock) start[i] = ustart[i];
     __syncthreads();
-    for (int u = blockIdx.x; u < W; u += gridDim.x) {
-        const int lo = unit_owner(start, nm, u);
+    for (int u = blockIdx.x * kStepWaves + wave; u < W; u += gridDim.x * kStepWaves) {
+        const int lo = __builtin_amdgcn_readfirstlane(unit_owner(start, nm, u));
         const int j = u - start[lo];
-        const double part = block_sum(spmv_task<K>(mats[lo], d, q, j * kUnitSlices + w, lane), ws);
-        if (threadIdx.x == 0) partials[u] = part;
+        const double part = wave_sum(spmv_task<K>(mats[lo], d, q, j, lane));
+        if (lane == 0) partials[u] = part;
     }
 }
 
@@ -715,14 +936,13 @@ struct Engine {
         dg.push_back(T(0));
         SFA_HIP(hipMemcpy(M.diag, dg.data(), dg.size() * sizeof(T), hipMemcpyHostToDevice));
         M.gb = plan.upper.ngroups();
-        int mu = std::max(1, (plan.full.nslices + kUnitSlices - 1) / kUnitSlices);
-        mu = std::max(mu, (plan.n_active + kUnitBlock - 1) / kUnitBlock);
+        int mu = std::max(1, plan.full.nslices);
+        mu = std::max(mu, (plan.n_active + kUnitRows - 1) / kUnitRows);
         for (const SellPart* part : {&plan.lower, &plan.upper})
             for (int g = 0; g < part->ngroups(); g++)
-                mu = std::max(mu, (part->group_slice[g + 1] - part->group_slice[g] +
-                                   kUnitSlices - 1) / kUnitSlices);
+                mu = std::max(mu, part->group_slice[g + 1] - part->group_slice[g]);
         M.bucket = M.full.main_bucket();
-        mu = std::max(mu, (plan.full.nslices * 8 + kUnitSlices - 1) / kUnitSlices);  // any mapping
+        mu = std::max(mu, plan.full.nslices * 8);  // SpMV tasks of any mapping
         M.max_units = mu;
         M.bytes = (long long)(plan.full.vals.size() + plan.lower.vals.size() + plan.upper.vals.size()) *
                       (sizeof(T) + 4) + (long long)(n + 1) * sizeof(T) +
@@ -812,7 +1032,7 @@ struct Engine {
         }
         for (int b = 0; b < kSpmvBuckets; b++) {
             long long u = 0;
-            for (int id : ids[b]) u += (mats[id].full.nslices * 8LL + kUnitSlices - 1) / kUnitSlices + 1;
+            for (int id : ids[b]) u += mats[id].full.nslices * 8LL + 1;
             umax = std::max(umax, u);
         }
         if (umax > INT_MAX / 2) throw Error{SFA_ERR_INVALID, "too many SpMV units to tune"};
@@ -850,7 +1070,7 @@ struct Engine {
                     const int K = cand[i];
                     std::vector<int> us(nm + 1, 0);
                     for (int m = 0; m < nm; m++)
-                        us[m + 1] = us[m] + std::max(1, (hm[m].full_slices * K + kUnitSlices - 1) / kUnitSlices);
+                        us[m + 1] = us[m] + std::max(1, hm[m].full_slices * K);
                     SFA_HIP(hipMemcpy(dstart, us.data(), sizeof(int) * (nm + 1), hipMemcpyHostToDevice));
                     auto once = [&] {
                         switch (K) {
@@ -1039,6 +1259,47 @@ struct Engine {
         }
         read_sched();
         return running();
+    }
+
+    // Runs `steps` steps with the three kernels launched one by one, each
+    // between two HIP events; adds each kernel's summed time (milliseconds)
+    // to ms[0..3) (step, fold, prefix).
+    void profile(int steps, double* ms)
+    {
+        SFA_HIP(hipSetDevice(device));
+        if (!d_mats || steps <= 0) return;
+        const BArgs<T> a = args();
+        std::vector<hipEvent_t> ev(4 * steps, nullptr);
+        auto cleanup = [&] {
+            for (hipEvent_t e : ev)
+                if (e) (void)hipEventDestroy(e);
+        };
+        try {
+            for (auto& e : ev) SFA_HIP(hipEventCreate(&e));
+            for (int i = 0; i < steps; i++) {
+                SFA_HIP(hipEventRecord(ev[4 * i], stream));
+                launch(k_batch_step<T>, dim3(kStepBlocks), dim3(kUnitBlock), stream, a);
+                SFA_HIP(hipEventRecord(ev[4 * i + 1], stream));
+                launch(k_batch_fold<T>, dim3(max_slots), dim3(kUnitBlock), stream, a);
+                SFA_HIP(hipEventRecord(ev[4 * i + 2], stream));
+                launch(k_batch_prefix<T>, dim3(1), dim3(kPrefixBlock), stream, a);
+                SFA_HIP(hipEventRecord(ev[4 * i + 3], stream));
+            }
+            SFA_HIP(hipGetLastError());
+            SFA_HIP(hipEventSynchronize(ev.back()));
+            for (int i = 0; i < steps; i++)
+                for (int k = 0; k < 3; k++) {
+                    float t = 0.f;
+                    SFA_HIP(hipEventElapsedTime(&t, ev[4 * i + k], ev[4 * i + k + 1]));
+                    ms[k] += t;
+                }
+            cleanup();
+        } catch (...) {
+            (void)hipStreamSynchronize(stream);
+            cleanup();
+            throw;
+        }
+        read_sched();
     }
 
     int running()
@@ -1243,6 +1504,26 @@ int sfa_batch_memory(sfa_batch b, long long* bytes)
     return sfa::guarded([&] {
         if (!b || !bytes) throw sfa::Error{SFA_ERR_INVALID, "null argument"};
         b->h.visit([&](auto& e) { *bytes = e.bytes_in_use(); });
+    });
+}
+
+int sfa_batch_profile(sfa_batch b, int steps, double* ms)
+{
+    return sfa::guarded([&] {
+        if (!b || !ms) throw sfa::Error{SFA_ERR_INVALID, "null argument"};
+        if (steps < 0 || steps > 100000) throw sfa::Error{SFA_ERR_INVALID, "steps must be in 0..100000"};
+        b->h.visit([&](auto& e) { e.profile(steps, ms); });
+    });
+}
+
+int sfa_batch_steps(sfa_batch b, long long* steps)
+{
+    return sfa::guarded([&] {
+        if (!b || !steps) throw sfa::Error{SFA_ERR_INVALID, "null argument"};
+        b->h.visit([&](auto& e) {
+            e.read_sched();
+            *steps = (long long)e.h_sched->step;
+        });
     });
 }
 

@@ -130,6 +130,7 @@ def batched(args):
                 tuning = eng.tune_spmv()
                 log(f"{'mixed' if mixed else 'fp64'} engine spmv tuning: {tuning}")
             synchronize()
+            steps0 = eng.steps()
             t1 = time.perf_counter()
             slots = []
             for j, mid in zip(jobs, mids):
@@ -148,6 +149,7 @@ def batched(args):
                     last = now
             synchronize()
             elapsed = time.perf_counter() - t1
+            steps = eng.steps() - steps0
             worst = 0.0  # largest true relative residual over a sample of the solves
             iters = 0
             ok = 0
@@ -161,11 +163,11 @@ def batched(args):
                     res = b - capi.csr_apply(s.n, s.n, s.A, s.C, s.V, x)
                     res[s.dirichlet] = 0.0
                     worst = max(worst, float(np.linalg.norm(res) / np.linalg.norm(b)))
-            return elapsed, t_sub, iters, ok, eng.memory(), tuning, worst
+            return elapsed, t_sub, iters, ok, eng.memory(), tuning, worst, steps
 
-    e_mx, sub_mx, it_mx, ok_mx, mem, tune_mx, res_mx = run_engine(True)
-    log(f"batched mixed: {nrhs} solves in {e_mx:.3f}s ({ok_mx} converged)")
-    e_64, sub_64, it_64, ok_64, _, tune_64, res_64 = run_engine(False)
+    e_mx, sub_mx, it_mx, ok_mx, mem, tune_mx, res_mx, st_mx = run_engine(True)
+    log(f"batched mixed: {nrhs} solves in {e_mx:.3f}s ({ok_mx} converged, {st_mx} steps)")
+    e_64, sub_64, it_64, ok_64, _, tune_64, res_64, st_64 = run_engine(False)
     log(f"batched fp64: {nrhs} solves in {e_64:.3f}s ({ok_64} converged)")
     # one-at-a-time FP64 path (device-resident System per matrix), on a sample
     sample = jobs[:: max(1, len(jobs) // 32)]
@@ -202,9 +204,11 @@ def batched(args):
            "systems": len(jobs), "solves": nrhs, "rows_total": int(sum(j.system.n for j in jobs)),
            "mixed": {"solves_per_s": nrhs / e_mx, "converged": ok_mx, "inner_iterations": it_mx,
                      "submit_s": sub_mx, "max_sampled_true_relative_residual": res_mx,
+                     "engine_steps": st_mx, "us_per_step": e_mx / max(st_mx, 1) * 1e6,
                      "spmv_mapping": tune_mx},
            "fp64_batched": {"solves_per_s": nrhs / e_64, "converged": ok_64, "iterations": it_64,
                             "max_sampled_true_relative_residual": res_64,
+                            "engine_steps": st_64, "us_per_step": e_64 / max(st_64, 1) * 1e6,
                             "spmv_mapping": tune_64},
            "fp64_one_at_a_time": {"solves_per_s": seq_rate, "sample_solves": n_seq},
            "speedup_mixed_batched_over_fp64_one_at_a_time": (nrhs / e_mx) / seq_rate,

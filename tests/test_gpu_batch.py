@@ -151,3 +151,29 @@ def test_prepared_matrix_from_another_thread():
             xa, ia = batch.collect(a, s.n)
             xb, ib = batch.collect(b, s.n)
             assert np.array_equal(xa, xb) and ia["iterations"] == ib["iterations"], s.name
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_right_hand_sides_riding_together_match_alone(mixed):
+    """Adjacent slots of one matrix in the same phase share each unit (the
+    matrix is read once for all of them, up to four per unit); each one's x
+    and iteration count are bit-identical to a solve alone.  Six right-hand
+    sides per matrix: groups of four and two, which split as solves finish."""
+    systems = [S.poisson3d_p1_kuhn(7), S.poisson2d_p1(5), S.random_spd(250, 0.05, 3)]
+    jobs = []
+    with capi.Batch(mixed=mixed, max_slots=32) as batch:
+        for k, s in enumerate(systems):
+            mid, _ = _add(batch, s)
+            for seed in range(6):
+                b = S.rhs_uniform(s, 300 + 10 * k + seed)
+                jobs.append((batch.submit(mid, b), s, b))
+        _run_all(batch, [j[0] for j in jobs])
+        together = [batch.collect(slot, s.n) for slot, s, _ in jobs]
+    for (slot, s, b), (x_t, info_t) in zip(jobs, together):
+        with capi.Batch(mixed=mixed, max_slots=2) as alone:
+            mid, _ = _add(alone, s)
+            sl = alone.submit(mid, b)
+            _run_all(alone, [sl])
+            x_a, info_a = alone.collect(sl, s.n)
+        assert info_t["iterations"] == info_a["iterations"], (s.name, info_t, info_a)
+        assert np.array_equal(x_t, x_a), s.name
