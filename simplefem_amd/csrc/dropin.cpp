@@ -7,12 +7,16 @@
 // is turned into the reference's failure behaviour: a message and abort().
 #include <algorithm>
 #include <cmath>
+#include <cstdint>
+#include <memory>
+#include <string>
 #include <cstdlib>
 #include <numeric>
 #include <random>
 
 #include "../../include/feec/operators/floatvector.hpp"
 #include "../../include/feec/operators/linearoperator.hpp"
+#include "../../include/feec/solver/inv.hpp"
 #include "../../include/feec/solver/sparsesolver.hpp"
 #include "../../include/feec/sparse/matcsr.hpp"
 #include "../../include/feec/sparse/rainbow.hpp"
@@ -920,4 +924,130 @@ int ConjugateGradientSolverCSR_Eisenstat_Rainbow(const int N, Float* x, const Fl
                 diagonal, omega, num_colors, F, B, R, &info),
             "ConjugateGradientSolverCSR_Eisenstat_Rainbow");
     return info.iterations;
+}
+
+// ------------------------------------------------------------ InverseOperator
+
+struct InverseOperator<MatrixCSR>::Device {
+    sfa_system sys = nullptr;
+    uint64_t digest = 0;
+    ~Device()
+    {
+        if (sys) (void)sfa_system_destroy(sys);
+    }
+};
+
+namespace {
+
+// 64-bit FNV-1a digest of a CSR matrix's dimensions and arrays.
+uint64_t csr_digest(const MatrixCSR& m)
+{
+    uint64_t h = 1469598103934665603ull;
+    auto eat = [&](const void* p, size_t bytes) {
+        const unsigned char* c = static_cast<const unsigned char*>(p);
+        for (size_t i = 0; i < bytes; i++) {
+            h ^= c[i];
+            h *= 1099511628211ull;
+        }
+    };
+    const int rows = m.getdimout(), cols = m.getdimin(), nnz = m.getnumberofentries();
+    eat(&rows, sizeof rows);
+    eat(&cols, sizeof cols);
+    eat(m.getA(), sizeof(int) * (size_t)(rows + 1));
+    eat(m.getC(), sizeof(int) * (size_t)nnz);
+    eat(m.getV(), sizeof(Float) * (size_t)nnz);
+    return h;
+}
+
+}  // namespace
+
+InverseOperator<MatrixCSR>::InverseOperator(const MatrixCSR& op_, Float precision_, int print_modulo_)
+    : LinearOperator(op_.getdimout(), op_.getdimin()),
+      op(op_),
+      precision(precision_),
+      print_modulo(print_modulo_),
+      previous_sol(op_.getdimin(), 0.)
+{
+    require_true(op.getdimin() == op.getdimout(), "InverseOperator of a non-square operator");
+    if (print_modulo >= 0) LOG << "Inverse created" << nl;
+}
+
+InverseOperator<MatrixCSR>::InverseOperator(const InverseOperator& o)
+    : LinearOperator(o.getdimout(), o.getdimin()),
+      use_previous_sol(o.use_previous_sol),
+      op(o.op),
+      precision(o.precision),
+      print_modulo(o.print_modulo),
+      previous_sol(o.previous_sol)
+{
+}
+
+InverseOperator<MatrixCSR>::InverseOperator(InverseOperator&& o) noexcept
+    : LinearOperator(o.getdimout(), o.getdimin()),
+      use_previous_sol(o.use_previous_sol),
+      op(o.op),
+      precision(o.precision),
+      print_modulo(o.print_modulo),
+      previous_sol(std::move(o.previous_sol)),
+      device_(std::move(o.device_)),
+      last_iterations_(o.last_iterations_)
+{
+}
+
+InverseOperator<MatrixCSR>::~InverseOperator() noexcept = default;
+
+void InverseOperator<MatrixCSR>::check() const
+{
+    op.check();
+    previous_sol.check();
+    require_true(op.getdimin() == op.getdimout(), "InverseOperator of a non-square operator");
+    require_true(op.getdimin() == previous_sol.getdimension(), "InverseOperator dimension");
+}
+
+std::string InverseOperator<MatrixCSR>::text() const
+{
+    std::string inner = op.text(), out;
+    size_t at = 0;
+    while (at < inner.size()) {  // tab_each_line
+        const size_t nl_at = inner.find('\n', at);
+        const size_t end = nl_at == std::string::npos ? inner.size() : nl_at + 1;
+        out += "\t" + inner.substr(at, end - at);
+        at = end;
+    }
+    return "Inverse Operator " + std::to_string(getdimout()) + "x" + std::to_string(getdimin()) +
+           "\n" + out;
+}
+
+void InverseOperator<MatrixCSR>::apply(FloatVector& dest, const FloatVector& src, Float scaling) const
+{
+    check();
+    src.check();
+    dest.check();
+    require_true(getdimin() == src.getdimension(), "InverseOperator::apply: src dimension");
+    require_true(getdimout() == dest.getdimension(), "InverseOperator::apply: dest dimension");
+    if (use_previous_sol) dest.copydatafrom(previous_sol);
+    else dest.zero();
+    // the reference reads the operator at every apply: rebuild the device copy
+    // when its arrays have changed
+    const uint64_t digest = csr_digest(op);
+    if (!device_ || device_->digest != digest) {
+        device_.reset();
+        std::unique_ptr<Device> dev(new Device());
+        const FloatVector diagonal = op.getDiagonal();
+        const Rainbow rainbow(op);
+        require(sfa_system_create(SFA_METHOD_EISENSTAT_RAINBOW, op.getdimin(), op.getA(), op.getC(),
+                                  op.getV(), diagonal.raw(), 1.0, rainbow.num_colors,
+                                  rainbow.F.data(), rainbow.B.data(), rainbow.R.data(), &dev->sys),
+                "InverseOperator: sfa_system_create");
+        dev->digest = digest;
+        device_ = std::move(dev);
+    }
+    FloatVector res(dest);
+    sfa_solve_info info{};
+    require(sfa_system_solve(device_->sys, dest.raw(), src.raw(), res.raw(), precision, print_modulo,
+                             0, &info),
+            "InverseOperator: sfa_system_solve");
+    last_iterations_ = info.iterations;
+    dest.scale(scaling);
+    if (use_previous_sol) previous_sol.copydatafrom(dest);
 }

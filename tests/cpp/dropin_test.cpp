@@ -12,6 +12,7 @@
 
 #include "feec/base/include.hpp"
 #include "feec/operators/floatvector.hpp"
+#include "feec/solver/inv.hpp"
 #include "feec/solver/sparsesolver.hpp"
 #include "feec/sparse/matcsr.hpp"
 #include "feec/sparse/rainbow.hpp"
@@ -159,11 +160,55 @@ static void gpu_checks()
     }
 }
 
+// InverseOperator<MatrixCSR> (solver/inv.hpp:151-201): each apply is the
+// Eisenstat-Rainbow solve (omega 1) from the previous solution, bit for bit
+// the raw call's; a changed operator is seen at the next apply.
+static void inverse_operator_checks()
+{
+    MatrixCSR system = laplacian(16);
+    const int n = system.getdimin();
+    InverseOperator<MatrixCSR> inv(system, desired_precision);
+    const FloatVector diagonal = system.getDiagonal();
+    const Rainbow rainbow(system);
+    FloatVector prev(n, 0.);
+    for (int trial = 0; trial < 3; trial++) {
+        FloatVector rhs(n);
+        rhs.random();
+        FloatVector x(n, 0.);
+        inv.apply(x, rhs, 1.);
+        FloatVector want(prev), res(prev);
+        const int K = ConjugateGradientSolverCSR_Eisenstat_Rainbow(
+            n, want.raw(), rhs.raw(), system.getA(), system.getC(), system.getV(), res.raw(),
+            desired_precision, -1, diagonal.raw(), 1.0, rainbow.num_colors, rainbow.F.data(),
+            rainbow.B.data(), rainbow.R.data());
+        bool same = true;
+        for (int i = 0; i < n; i++) same = same && x[i] == want[i];
+        CHECK(same);
+        CHECK(inv.last_iterations() == K);
+        std::printf("InverseOperator apply %d: K=%d, same as the raw call: %s\n", trial, K,
+                    same ? "yes" : "no");
+        prev.copydatafrom(x);
+    }
+    FloatVector half(n);
+    half.random();
+    FloatVector x1(n, 0.);
+    inv.apply(x1, half, 0.5);  // dest = 0.5 * op^-1 src
+    CHECK((2. * (system * x1) - half).norm() < 1e-6 * half.norm());
+    system.scale(2.);
+    inv.use_previous_sol = false;
+    FloatVector rhs(n);
+    rhs.random();
+    FloatVector x(n, 0.);
+    inv.apply(x, rhs, 1.);
+    CHECK((system * x - rhs).norm() < 1e-6 * rhs.norm());
+}
+
 int main(int argc, char** argv)
 {
     const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
     cpu_checks();
     if (gpu) gpu_checks();
+    if (gpu) inverse_operator_checks();
     std::printf("%s checks: %d failure(s)\n", gpu ? "cpu+gpu" : "cpu", failures);
     return failures == 0 ? 0 : 1;
 }
