@@ -117,9 +117,9 @@ def main():
     ap.add_argument("--inner-rtol", type=float, default=1e-4,
                     help="mixed precision: reduction of the preconditioned residual per FP32 "
                          "inner solve")
-    ap.add_argument("--route-rows", type=int, default=100000,
+    ap.add_argument("--route-rows", type=int, default=10000,
                     help="served: requests with at least this many rows get their own System")
-    ap.add_argument("--dedicated-workers", type=int, default=6,
+    ap.add_argument("--dedicated-workers", type=int, default=12,
                     help="served: concurrent dedicated Systems (0: everything in the batch)")
     ap.add_argument("--no-served-baseline", action="store_true",
                     help="served: skip the plain FP64 pass over the same stream")
