@@ -114,9 +114,12 @@ def main():
     ap.add_argument("--chunk-steps", type=int, default=2000,
                     help="batched: engine steps per persistent launch between progress checks")
     ap.add_argument("--rate", type=float, default=20.0, help="served: arrivals per second")
-    ap.add_argument("--inner-rtol", type=float, default=1e-4,
+    ap.add_argument("--inner-rtol", type=float, default=1e-1,
                     help="mixed precision: reduction of the preconditioned residual per FP32 "
                          "inner solve")
+    ap.add_argument("--inner-restarts", action="store_true",
+                    help="mixed forms: restart every inner CG (plain iterative refinement) "
+                         "instead of continuing the previous inner solve's direction")
     ap.add_argument("--route-rows", type=int, default=10000,
                     help="served: requests with at least this many rows get their own System")
     ap.add_argument("--dedicated-workers", type=int, default=12,

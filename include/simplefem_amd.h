@@ -55,6 +55,14 @@ extern "C" {
  * MI355X: +1% on the 64^3 P1 Poisson system, -13% on the 48^3 H(curl)
  * system, so off unless asked for. */
 #define SFA_FLAG_PAIRED_SWEEPS 0x200
+/* Or-ed into the method of sfa_system_create with SFA_FLAG_FP32_INNER: every
+ * inner solve of sfa_system_solve_mixed starts its CG afresh (d = z), plain
+ * iterative refinement.  By default an inner solve after the first continues
+ * the previous one's search direction (residual replacement: the FP64
+ * residual replaces the FP32 recursive one, the Krylov space is kept), which
+ * needs fewer inner iterations in total.  CG, Jacobi, SSOR and Rainbow
+ * variants; the Eisenstat variants always restart. */
+#define SFA_FLAG_INNER_RESTARTS 0x400
 
 /* solver variants (one per function of solver/sparsesolver.hpp:11-84) */
 #define SFA_METHOD_CG                  0
@@ -188,13 +196,21 @@ int sfa_system_solve(sfa_system sys, double* x, const double* b,
  * FP32 (FP32 matrix, vectors and sweeps; FP64 dot products) until its
  * preconditioned residual has dropped by inner_rtol, x += e in FP64; stops
  * when ||b - A x||_2 <= max(rtol*||b||_2, sqrt(DBL_EPSILON)) or after
- * max_outer corrections.  Needs a system created with SFA_FLAG_FP32_INNER;
+ * max_outer corrections.  Each inner solve after the first continues the
+ * previous one's search direction d = z + (z.r / z.r_prev) d_prev unless the
+ * system was created with SFA_FLAG_INNER_RESTARTS, or the FP64 residual's
+ * z.r exceeds 4x the z.r the previous inner solve ended with (its FP32
+ * residual has drifted: the old direction no longer fits), or the previous
+ * inner solve did not converge; a continued inner solve stops after
+ * max(100, 3 K_prev) iterations (the outer loop then corrects and the next
+ * inner solve restarts).  Needs a system created with SFA_FLAG_FP32_INNER;
  * rows the variant skips keep their x. */
 typedef struct sfa_refine_info {
     int    outer;             /* FP64 corrections applied                    */
     int    inner_iterations;  /* FP32 CG iterations, summed                  */
     int    success;           /* final FP64 residual met the target          */
-    int    reserved;
+    int    continued;         /* inner solves that continued the previous
+                                 direction (the others restarted)            */
     double residual_norm;     /* ||b - A x||_2 of the returned x (FP64)      */
     double relative_residual; /* residual_norm / ||b||_2                     */
 } sfa_refine_info;
@@ -301,10 +317,15 @@ int sfa_system_destroy(sfa_system sys);
  *                    (sqrt(z.r) < max(precision*||b||, 2^-26)).
  *   SFA_BATCH_MIXED: FP32 matrix/vectors/sweeps inside FP64 iterative
  *                    refinement, stopping on ||b - A x||_2 <=
- *                    max(precision*||b||, 2^-26) in FP64.
+ *                    max(precision*||b||, 2^-26) in FP64; each inner solve
+ *                    after the first continues the previous one's search
+ *                    direction (as sfa_system_solve_mixed).
+ *   SFA_BATCH_MIXED_RESTARTS: the same with every inner CG started afresh
+ *                    (d = z), plain iterative refinement.
  * ------------------------------------------------------------------------- */
 #define SFA_BATCH_FP64  0
 #define SFA_BATCH_MIXED 1
+#define SFA_BATCH_MIXED_RESTARTS 2
 
 typedef struct sfa_batch_s* sfa_batch;
 

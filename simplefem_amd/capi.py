@@ -52,7 +52,7 @@ class SystemStats(ctypes.Structure):
 
 class RefineInfo(ctypes.Structure):
     _fields_ = [("outer", c_int), ("inner_iterations", c_int), ("success", c_int),
-                ("reserved", c_int), ("residual_norm", c_double), ("relative_residual", c_double)]
+                ("continued", c_int), ("residual_norm", c_double), ("relative_residual", c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
@@ -81,7 +81,8 @@ class SpmvTuning(ctypes.Structure):
 
 FLAG_FP32_INNER = 0x100
 FLAG_PAIRED_SWEEPS = 0x200
-BATCH_FP64, BATCH_MIXED = 0, 1
+FLAG_INNER_RESTARTS = 0x400
+BATCH_FP64, BATCH_MIXED, BATCH_MIXED_RESTARTS = 0, 1, 2
 
 
 class PreparedMatrix:
@@ -361,12 +362,14 @@ class System:
     """
 
     def __init__(self, method, N, A, C, V, aux=None, omega=1.0, num_colors=0, F=None, B=None,
-                 R=None, fp32_inner=False, paired_sweeps=False):
+                 R=None, fp32_inner=False, paired_sweeps=False, inner_restarts=False):
         lib = load()
         if fp32_inner:
             method |= FLAG_FP32_INNER
         if paired_sweeps:
             method |= FLAG_PAIRED_SWEEPS
+        if inner_restarts:
+            method |= FLAG_INNER_RESTARTS
         A, C, V = _csr(A, C, V)
         aux = _f64(aux, "aux", N)
         F, B, R = _i32(F, "F"), _i32(B, "B"), _i32(R, "R")
@@ -398,10 +401,11 @@ class System:
             info.update(out.as_dict())
         return out.iterations
 
-    def solve_mixed(self, x, b, rtol, inner_rtol=1e-4, max_outer=20):
+    def solve_mixed(self, x, b, rtol, inner_rtol=1e-1, max_outer=50):
         """FP64 refinement around FP32 inner solves (system created with
         fp32_inner=True).  Returns a dict (outer, inner_iterations, success,
-        residual_norm, relative_residual)."""
+        continued, residual_norm, relative_residual); `continued` counts the
+        inner solves that continued the previous one's direction."""
         x, px, dx = self._ptr(x, self.N, True)
         b, pb, db = self._ptr(b, self.N, False)
         if dx != db:
@@ -473,11 +477,11 @@ class Batch:
     """Many Rainbow CG-SSOR solves advanced together by one persistent kernel
     (sfa_batch_*): add matrices, submit right-hand sides, run, collect."""
 
-    def __init__(self, mixed=False, max_slots=256):
+    def __init__(self, mixed=False, max_slots=256, inner_restarts=False):
         self._h = c_void_p()
         self.mixed = mixed
-        _check(load().sfa_batch_create(BATCH_MIXED if mixed else BATCH_FP64, int(max_slots),
-                                       ctypes.byref(self._h)))
+        mode = (BATCH_MIXED_RESTARTS if inner_restarts else BATCH_MIXED) if mixed else BATCH_FP64
+        _check(load().sfa_batch_create(mode, int(max_slots), ctypes.byref(self._h)))
         self.n_of = {}
 
     def add_matrix(self, N, A, C, V, diagonal, omega, num_colors, F, B, R):
@@ -502,7 +506,7 @@ class Batch:
         _check(load().sfa_batch_remove_matrix(self._h, int(mid)))
         self.n_of.pop(mid, None)
 
-    def submit(self, mid, b, x0=None, precision=1e-10, inner_rtol=1e-4, max_outer=30):
+    def submit(self, mid, b, x0=None, precision=1e-10, inner_rtol=1e-1, max_outer=50):
         N = self.n_of[mid]
         b = _f64(b, "b", N)
         x0 = _f64(x0, "x0", N)

@@ -7,6 +7,13 @@
 //   iteration: SPMV   FWD[0..Gf)    BWD[0..Gb)    DIR          (repeat)
 //   mixed:     RES64 -> CONVERT -> (restart, iterations in FP32) -> CORRECT -> RES64 ...
 //
+// A mixed slot's inner solves after the first continue the previous one's
+// search direction (residual replacement): their DINIT forms z.r only, and a
+// DIR phase with beta = z.r / z.r_prev makes d = z + beta d_prev -- or, when
+// the new z.r exceeds 4x the recursive one the last inner solve ended with
+// (its FP32 residual drifted), beta = 0, a restart.  A continued inner solve
+// stops after max(100, 3 K_prev) iterations.
+//
 // the same phases as the single-system kernels, with the same per-row
 // arithmetic.  A "step" executes the current phase of every active slot; a
 // phase is split into work units of 64 rows (one SELL-64 slice, or 64
@@ -87,8 +94,12 @@ struct BSlot {
     int kind, group;
     int units;          // units of the phase in the step being scheduled
     int mixed;
+    int cont;           // mixed: inner solves continue the previous direction
     int K, maxK, exit_reason, outer, max_outer, inner_total;
     double zr, dAd, alpha, beta, tol2, rel2, target64, res64;
+    double zr_cont;     // > 0: z.r of the iterate that formed d, which the next inner solve continues
+    int k_cont;         // iterations of the inner solve that left that direction
+    int pad_;
     T *x, *r, *b, *d, *z, *q, *m;
     double *x64, *b64, *r64;
     double* partials;
@@ -185,16 +196,28 @@ template <typename T>
 __device__ void phase_complete(BSlot<T>& S, const BMat<T>& M, double total)
 {
     switch (S.kind) {
-    case PH_DINIT:  // solver/sparsesolver.cpp:1156-1190
+    case PH_DINIT: {  // solver/sparsesolver.cpp:1156-1190
+        const double zr_end = S.zr;  // recursive z.r the last inner solve ended with
         S.zr = total;
         if (S.rel2 > 0.) S.tol2 = S.rel2 * ref_absolute(total);
         if (ref_absolute(total) < S.tol2) {
+            S.zr_cont = 0.;
             S.exit_reason = SFA_EXIT_CONVERGED;
             finish_solve(S);
+        } else if (S.zr_cont > 0.) {  // d still holds the previous direction
+            if (total <= 1.25 * zr_end) {  // continue it
+                S.beta = total / S.zr_cont;
+                S.maxK = min(M.n, max(100, 3 * S.k_cont));
+            } else {  // the FP32 residual drifted: d = z
+                S.beta = 0.;
+            }
+            S.zr_cont = 0.;
+            S.kind = PH_DIR, S.group = 0;
         } else {
             S.kind = PH_SPMV, S.group = 0;
         }
         return;
+    }
     case PH_SPMV: {  // :1215-1231
         S.dAd = total;
         const bool num_bad = !isfinite(S.zr) || S.zr < 0.;
@@ -221,6 +244,7 @@ __device__ void phase_complete(BSlot<T>& S, const BMat<T>& M, double total)
         }
         double zr_new = 0.;
         for (int g = 0; g < M.gb; g++) zr_new += S.grp[g];
+        const double zr_old = S.zr;
         S.beta = zr_new / S.zr;
         S.zr = zr_new;
         S.K++;
@@ -230,6 +254,10 @@ __device__ void phase_complete(BSlot<T>& S, const BMat<T>& M, double total)
             finish_solve(S);
         } else if (ref_absolute(zr_new) < S.tol2) {
             S.exit_reason = SFA_EXIT_CONVERGED;
+            if (S.mixed && S.cont && zr_old > 0. && isfinite(zr_old)) {
+                S.zr_cont = zr_old;
+                S.k_cont = S.K;
+            }
             finish_solve(S);
         } else {
             S.kind = PH_DIR, S.group = 0;
@@ -246,6 +274,7 @@ __device__ void phase_complete(BSlot<T>& S, const BMat<T>& M, double total)
         return;
     case PH_CONVERT:  // a new inner solve starts from x = 0
         S.K = 0;
+        S.maxK = M.n;
         S.exit_reason = SFA_EXIT_CAP;
         advance(S, M);
         return;
@@ -489,12 +518,12 @@ __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind,
         }
         break;
     }
-    case PH_DINIT: {
+    case PH_DINIT: {  // a slot that continues its direction keeps d (its DIR phase follows)
         const int i = j * kUnitRows + lane;
         if (i < M.n_active)
             for (int k = 0; k < nr; k++) {
                 const T zz = SS[k].z[i];
-                SS[k].d[i] = zz;
+                if (!(SS[k].zr_cont > 0.)) SS[k].d[i] = zz;
                 part[k] = (double)(zz * SS[k].r[i]);
             }
         break;
@@ -822,6 +851,7 @@ int* upload_ints(const std::vector<int>& v)
 template <typename T>
 struct Engine {
     bool mixed = false;
+    bool cont = false;  // mixed: inner solves continue the previous direction
     int max_slots = 0;
     int device = 0;
     hipStream_t stream = nullptr;
@@ -869,6 +899,7 @@ struct Engine {
         ensure_device();
         SFA_HIP(hipGetDevice(&device));
         mixed = mixed_ != 0;
+        cont = mixed_ == 1;
         max_slots = max_slots_;
         SFA_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         slots.assign(max_slots, BSlot<T>{});
@@ -1179,6 +1210,7 @@ struct Engine {
         BSlot<T> sl{};
         sl.mat = id;
         sl.mixed = mixed ? 1 : 0;
+        sl.cont = cont ? 1 : 0;
         sl.maxK = n;
         sl.exit_reason = SFA_EXIT_CAP;
         sl.max_outer = max_outer;
@@ -1398,11 +1430,12 @@ int sfa_batch_create(int precision_mode, int max_slots, sfa_batch* out)
         if (precision_mode == SFA_BATCH_FP64) {
             b->h.e64.reset(new sfa::Engine<double>());
             b->h.e64->create(0, max_slots);
-        } else if (precision_mode == SFA_BATCH_MIXED) {
+        } else if (precision_mode == SFA_BATCH_MIXED || precision_mode == SFA_BATCH_MIXED_RESTARTS) {
             b->h.e32.reset(new sfa::Engine<float>());
-            b->h.e32->create(1, max_slots);
+            b->h.e32->create(precision_mode == SFA_BATCH_MIXED ? 1 : 2, max_slots);
         } else {
-            throw sfa::Error{SFA_ERR_INVALID, "precision_mode must be SFA_BATCH_FP64 or SFA_BATCH_MIXED"};
+            throw sfa::Error{SFA_ERR_INVALID,
+                             "precision_mode must be SFA_BATCH_FP64, SFA_BATCH_MIXED or SFA_BATCH_MIXED_RESTARTS"};
         }
         *out = b.release();
     });

@@ -296,6 +296,23 @@ __global__ void __launch_bounds__(kVecBlock) k_direction_init(int n, const T* z,
     block_partial<kVecBlock>(part, &partials[blockIdx.x]);
 }
 
+// Continues an earlier CG's direction after its residual was replaced (the
+// restart has just made d = z and z.r of the new residual, st->zr):
+// d = z + beta d_prev with beta = z.r / zr_prev, zr_prev the z.r of the
+// iterate that formed d_prev -- the iteration's own direction update with
+// the replaced residual in place of the recursive one.  The loop's cap
+// becomes maxK.
+template <typename T>
+__global__ void __launch_bounds__(kVecBlock) k_direction_continue(SolveState* st, int n, T* d,
+                                                                  const T* dprev, double zr_prev,
+                                                                  int maxK)
+{
+    const T beta = (T)(st->zr / zr_prev);
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->maxK = maxK;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        d[i] = d[i] + beta * dprev[i];
+}
+
 // solver/sparsesolver.cpp:830-844: z = r*diag*(2-w)/w, d = z, partials of z.r.
 template <typename T>
 __global__ void __launch_bounds__(kVecBlock) k_eisenstat_init(int n, const T* r, const T* diag,
@@ -1083,6 +1100,15 @@ void launch_direction_init(hipStream_t s, SolveState* st, int n_active, const T*
 }
 
 template <typename T>
+void launch_direction_continue(hipStream_t s, SolveState* st, int n_active, T* d,
+                               const T* dprev, double zr_prev, int maxK)
+{
+    const int grid = vec_reduce_blocks(n_active);
+    launch(k_direction_continue<T>, dim3(grid), dim3(kVecBlock), s, st, n_active, d, dprev, zr_prev,
+           maxK);
+}
+
+template <typename T>
 void launch_eisenstat_init(hipStream_t s, SolveState* st, int n_active, const T* r,
                            const T* diag, T omega, T* z, T* d, double* partials)
 {
@@ -1235,6 +1261,8 @@ void launch_add_correction(hipStream_t s, double* x, const float* e, int n)
                                      double*);                                                 \
     template void launch_residual_norm<T>(hipStream_t, SellViewT<T>, int, int, int,            \
                                           SolveState*, const T*, const T*, T*, double*);       \
+    template void launch_direction_continue<T>(hipStream_t, SolveState*, int, T*, const T*,   \
+                                               double, int);                                   \
     template void launch_direction_init<T>(hipStream_t, SolveState*, int, const T*, const T*,  \
                                            T*, double*);                                       \
     template void launch_eisenstat_init<T>(hipStream_t, SolveState*, int, const T*, const T*,  \

@@ -101,6 +101,11 @@ void launch_residual_norm(hipStream_t s, SellViewT<T> A, int nslices, int row0, 
 template <typename T>
 void launch_direction_init(hipStream_t s, SolveState* st, int n_active, const T* z, const T* r,
                            T* d, double* partials);
+// After a restart: d = z + (st->zr / zr_prev) dprev, the direction of an
+// earlier CG continued across a replaced residual; the loop's cap becomes maxK.
+template <typename T>
+void launch_direction_continue(hipStream_t s, SolveState* st, int n_active, T* d,
+                               const T* dprev, double zr_prev, int maxK);
 // Eisenstat restart: z = r*diag*(2-w)/w, d = z, z.r, restart scalar step.
 template <typename T>
 void launch_eisenstat_init(hipStream_t s, SolveState* st, int n_active, const T* r,

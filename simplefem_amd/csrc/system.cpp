@@ -71,6 +71,9 @@ struct SystemT {
         bool pair;
     };
     bool pair_sweeps = false;  // true: pairs of groups where possible (SFA_FLAG_PAIRED_SWEEPS)
+    // mixed solves: restart every inner CG from d = z (SFA_FLAG_INNER_RESTARTS)
+    // instead of continuing the previous inner solve's direction
+    bool inner_restarts = false;
     std::vector<SweepLaunch> fwd_sched, bwd_sched;
     std::vector<int> defer_bwd;
     std::vector<int> spmv_tail;
@@ -91,6 +94,7 @@ struct SystemT {
     T* io = nullptr;  // n staging entries for host <-> device copies
     T* io2 = nullptr;
     T* io3 = nullptr;
+    T* dkeep = nullptr;  // the direction an inner solve continues (mixed precision), lazily
     SolveState* state = nullptr;
     double* hist = nullptr;
     // block partials of the deferred reductions: SpMV / combine / tolerance
@@ -141,10 +145,10 @@ struct SystemT {
         apply_part.release();
         for (void* p : {(void*)perm, (void*)diag, (void*)x, (void*)r, (void*)b, (void*)d, (void*)d1,
                         (void*)z, (void*)q, (void*)m, (void*)io, (void*)io2, (void*)io3,
-                        (void*)state, (void*)hist, (void*)ps, (void*)pu, (void*)pb, (void*)grp})
+                        (void*)dkeep, (void*)state, (void*)hist, (void*)ps, (void*)pu, (void*)pb, (void*)grp})
             dev_free(p);
         perm = nullptr;
-        diag = x = r = b = d = d1 = z = q = m = io = io2 = io3 = nullptr;
+        diag = x = r = b = d = d1 = z = q = m = io = io2 = io3 = dkeep = nullptr;
         hist = ps = pu = pb = grp = nullptr;
         state = nullptr;
         if (host_state) (void)hipHostFree(host_state);
@@ -840,6 +844,17 @@ struct SystemT {
     // r = b - A x (FP64) -> e = A^-1 r in FP32 to a reduction inner_rtol of
     // the preconditioned residual -> x += e (FP64), until ||b - A x||_2 <=
     // max(rtol * ||b||_2, 2^-26) or max_outer corrections.
+    //
+    // Unless inner_restarts is set, an inner solve after the first continues
+    // the previous one's search direction instead of starting from d = z:
+    // the FP32 CG runs on as one Krylov iteration whose residual is replaced
+    // by the FP64 one at every correction (residual replacement, van der
+    // Vorst and Ye, SIAM J. Sci. Comput. 22 (2000) 1716-1728), so the
+    // corrections do not throw away the Krylov space built so far.
+    // A continued inner solve needs the restart's z.r within this factor of
+    // the z.r the previous inner solve ended with.
+    static constexpr double kContinueGap = 1.25;
+
     void solve_mixed(T* ux, const T* ub, double rtol, double inner_rtol, int max_outer,
                      int device_pointers, sfa_refine_info* info)
     {
@@ -869,9 +884,14 @@ struct SystemT {
         double target = rtol * std::sqrt(bnorm2);
         const double floor_ = 1.4901161193847656e-08;
         if (!(target >= floor_)) target = floor_;
-        int outer = 0;
+        const bool can_continue = !inner_restarts && inner->has_spmv();
+        if (can_continue && !inner->dkeep) inner->dkeep = dev_alloc<float>(n);
+        int outer = 0, continued = 0;
         long long inner_total = 0;
         double res = 0.;
+        double zr_prev = 0.;  // > 0: the last inner solve left a direction to continue
+        double zr_end = 0.;   // the recursive z.r that solve ended with
+        int k_end = 0;        // and its iterations
         for (;;) {
             launch_residual_norm(s, full.view(), full.nslices, full.row0(0), full.nrows_of(0),
                                  state, b, x, r, pu);
@@ -881,10 +901,36 @@ struct SystemT {
             launch_to_float(s, inner->b, r, n_active);
             launch_fill(s, inner->x, 0.0f, n);
             launch_fill(s, inner->r, 0.0f, n);
+            if (zr_prev > 0.) {
+                // the direction of the last inner iteration: slot K % 2 (the
+                // iteration of parity p reads slot p)
+                const float* dlast = (inner->host_state->K & 1) ? inner->d1 : inner->d;
+                SFA_HIP(hipMemcpyAsync(inner->dkeep, dlast, n * sizeof(float),
+                                       hipMemcpyDeviceToDevice, s));
+            }
             inner->start(s, 1.0, -3, 0., 0, inner_rtol * inner_rtol);
+            // Continue only where the FP64 residual agrees with the recursive
+            // one the last inner solve ended with: a drifted FP32 residual
+            // makes the old direction unfit (beta far too large).
+            if (zr_prev > 0. && !inner->host_state->done &&
+                inner->host_state->zr <= kContinueGap * zr_end) {
+                launch_direction_continue(s, inner->state, inner->n_active, inner->d,
+                                          (const float*)inner->dkeep, zr_prev,
+                                          (int)std::min<long long>(inner->n, std::max(100LL, 3LL * k_end)));
+                continued++;
+            }
             inner->iterate(s);
-            if (inner->host_state->exit_reason == SFA_EXIT_BREAKDOWN) break;
-            inner_total += inner->host_state->K;
+            const SolveState& ist = *inner->host_state;
+            if (ist.exit_reason == SFA_EXIT_BREAKDOWN) break;
+            // z.r of the iterate that formed the last direction (the stopping
+            // iteration read it from zrp[K % 2])
+            zr_prev = (can_continue && ist.exit_reason == SFA_EXIT_CONVERGED && ist.K >= 1)
+                          ? ist.zrp[ist.K & 1]
+                          : 0.;
+            if (!(zr_prev > 0.) || !std::isfinite(zr_prev)) zr_prev = 0.;
+            zr_end = ist.zr;
+            k_end = ist.K;
+            inner_total += ist.K;
             launch_add_correction(s, x, inner->x, n_active);
             outer++;
         }
@@ -899,6 +945,7 @@ struct SystemT {
             o.outer = outer;
             o.inner_iterations = (int)inner_total;
             o.success = res <= target ? 1 : 0;
+            o.continued = continued;
             o.residual_norm = res;
             o.relative_residual = bnorm2 > 0 ? res / std::sqrt(bnorm2) : res;
             *info = o;
@@ -1202,7 +1249,8 @@ int sfa_system_create(int method, int N, const int* csrrows, const int* csrcolum
         std::unique_ptr<sfa_system_s> h(new sfa_system_s());
         const bool fp32 = (method & SFA_FLAG_FP32_INNER) != 0;
         h->sys.pair_sweeps = (method & SFA_FLAG_PAIRED_SWEEPS) != 0;
-        h->sys.create(method & ~(SFA_FLAG_FP32_INNER | SFA_FLAG_PAIRED_SWEEPS), N, csrrows,
+        h->sys.inner_restarts = (method & SFA_FLAG_INNER_RESTARTS) != 0;
+        h->sys.create(method & ~(SFA_FLAG_FP32_INNER | SFA_FLAG_PAIRED_SWEEPS | SFA_FLAG_INNER_RESTARTS), N, csrrows,
                       csrcolumns, csrvalues, aux, omega,
                       num_colors, F, B, R, fp32);
         *out = h.release();

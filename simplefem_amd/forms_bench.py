@@ -52,7 +52,7 @@ def hcurl_mixed(args):
     col = (rb.num_colors, rb.F, rb.B, rb.R)
     log(f"hcurl system n={s.n} nnz={s.nnz} colours={rb.num_colors} ({time.time() - t0:.1f}s)")
     sysd = capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, d, 1.0, *col, fp32_inner=True,
-                       paired_sweeps=args.paired_sweeps)
+                       paired_sweeps=args.paired_sweeps, inner_restarts=args.inner_restarts)
     if args.spmv_lanes:
         sysd.set_spmv(args.spmv_lanes)
         tuning = {"lanes_per_row": args.spmv_lanes, "tuned": False}
@@ -72,7 +72,7 @@ def hcurl_mixed(args):
 
     def mixed():
         x.zero()
-        return sysd.solve_mixed(x, bt, 1e-10, inner_rtol=args.inner_rtol, max_outer=30)
+        return sysd.solve_mixed(x, bt, 1e-10, inner_rtol=args.inner_rtol, max_outer=50)
 
     fp64()
     synchronize()
@@ -84,8 +84,10 @@ def hcurl_mixed(args):
     x64 = x.download()
     res64 = _true_relres(s, x64, b)
     K64 = info["iterations"]
+    log(f"fp64: {1.0 / t64:.2f} solves/s, K = {K64}")
     mi = mixed()
     synchronize()
+    log(f"mixed: {mi}")
     t1 = time.perf_counter()
     for _ in range(steps):
         mi = mixed()
@@ -101,6 +103,9 @@ def hcurl_mixed(args):
            "fp64": {"solves_per_s": 1.0 / t64, "iterations": K64, "true_relative_residual": res64},
            "mixed": {"solves_per_s": 1.0 / tmx, "outer": mi["outer"],
                      "inner_iterations": mi["inner_iterations"],
+                     "continued_inner_solves": mi["continued"],
+                     "inner_rtol": args.inner_rtol,
+                     "inner_restarts": bool(args.inner_restarts),
                      "true_relative_residual": resm},
            "mixed_vs_fp64_solution_rel_error": err,
            "spmv_mapping": tuning,
@@ -117,7 +122,8 @@ def batched(args):
     cols = [capi.Rainbow(j.system.n, j.system.A, j.system.C) for j in jobs]
 
     def run_engine(mixed):
-        with capi.Batch(mixed=mixed, max_slots=min(1024, nrhs)) as eng:
+        with capi.Batch(mixed=mixed, max_slots=min(1024, nrhs),
+                        inner_restarts=args.inner_restarts) as eng:
             mids = []
             for j, rb in zip(jobs, cols):
                 s = j.system
@@ -251,17 +257,17 @@ def _served_fp64_one_at_a_time(jobs):
     return time.perf_counter() - start, np.array(lat)
 
 
-def _solve_dedicated(job, inner_rtol, start):
+def _solve_dedicated(job, inner_rtol, start, inner_restarts=False):
     """A large request on its own device-resident System and stream: colouring,
     plan, upload and the mixed-precision solve, on a worker thread while the
     batch engine runs.  Returns the completion time (seconds after start)."""
     s = job.system
     rb = capi.Rainbow(s.n, s.A, s.C)
     with capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, s.diagonal(), 1.0, rb.num_colors,
-                     rb.F, rb.B, rb.R, fp32_inner=True) as sysd:
+                     rb.F, rb.B, rb.R, fp32_inner=True, inner_restarts=inner_restarts) as sysd:
         bt = DeviceArray.from_host(job.rhs[0])
         x = DeviceArray(s.n)
-        sysd.solve_mixed(x, bt, 1e-10, inner_rtol=inner_rtol, max_outer=30)  # returns when done
+        sysd.solve_mixed(x, bt, 1e-10, inner_rtol=inner_rtol, max_outer=50)  # returns when done
         bt.free()
         x.free()
     return time.perf_counter() - start
@@ -280,7 +286,7 @@ def served(args, budget=200e9):
     log(f"{len(jobs)} requests generated ({time.time() - t0:.1f}s)")
     lat = []
     dedicated = []  # (job, future) of requests routed to their own System
-    with capi.Batch(mixed=True, max_slots=256) as eng, \
+    with capi.Batch(mixed=True, max_slots=256, inner_restarts=args.inner_restarts) as eng, \
             ThreadPoolExecutor(max_workers=args.prep_workers) as pool, \
             ThreadPoolExecutor(max_workers=max(1, args.dedicated_workers)) as big_pool:
         arrivals = list(jobs)  # not yet arrived
@@ -295,7 +301,7 @@ def served(args, budget=200e9):
                 j = arrivals.pop(0)
                 if args.dedicated_workers > 0 and j.system.n >= args.route_rows:
                     dedicated.append((j, big_pool.submit(_solve_dedicated, j, args.inner_rtol,
-                                                         start)))
+                                                         start, args.inner_restarts)))
                 else:
                     preparing.append((j, pool.submit(_prepare, j)))
             while preparing and preparing[0][1].done():  # admit in arrival order, within budget

@@ -104,15 +104,16 @@ def test_continuous_admission():
         assert np.array_equal(x, xm) and info["iterations"] == im["iterations"]
 
 
-def test_batch_mixed_precision_reaches_fp64_residual():
+def _mixed_batch_run(inner_restarts, inner_rtol):
     systems = [S.poisson2d_p1(5), S.poisson3d_p1_kuhn(8), S.fd_laplacian_2d(20)]
-    with capi.Batch(mixed=True, max_slots=16) as batch:
+    total = 0
+    with capi.Batch(mixed=True, max_slots=16, inner_restarts=inner_restarts) as batch:
         jobs = []
         for k, s in enumerate(systems):
             mid, col = _add(batch, s)
             for seed in (1, 2, 3, 4):
                 b = S.rhs_uniform(s, seed + 10 * k)
-                jobs.append((batch.submit(mid, b, precision=1e-10, inner_rtol=1e-4), s, b))
+                jobs.append((batch.submit(mid, b, precision=1e-10, inner_rtol=inner_rtol), s, b))
         _run_all(batch, [j[0] for j in jobs])
         for slot, s, b in jobs:
             x, info = batch.collect(slot, s.n)
@@ -123,6 +124,22 @@ def test_batch_mixed_precision_reaches_fp64_residual():
             target = max(1e-10 * np.linalg.norm(b), 2.0 ** -26)
             assert np.linalg.norm(res) <= 1.0000001 * target, (s.name, info)
             assert info["outer"] >= 1
+            total += info["iterations"]
+    return total
+
+
+@pytest.mark.parametrize("inner_restarts", [False, True], ids=["continued", "restarts"])
+def test_batch_mixed_precision_reaches_fp64_residual(inner_restarts):
+    _mixed_batch_run(inner_restarts, 1e-4)
+
+
+def test_batch_mixed_continued_direction_needs_fewer_iterations():
+    """Inner solves that continue the previous direction (the default) need
+    fewer FP32 iterations in total than inner solves that restart."""
+    cont = _mixed_batch_run(False, 1e-2)
+    rest = _mixed_batch_run(True, 1e-2)
+    print(f"[batch continued] inner iterations continued {cont} restarts {rest}", flush=True)
+    assert cont < 0.8 * rest
 
 
 def test_prepared_matrix_from_another_thread():

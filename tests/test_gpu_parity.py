@@ -309,19 +309,22 @@ def test_cpp_dropin_callers():
     assert proc.returncode == 0, proc.stdout[-3000:] + proc.stderr[-2000:]
 
 
+@pytest.mark.parametrize("inner_restarts", [False, True], ids=["continued", "restarts"])
 @pytest.mark.parametrize("variant,method", [("rainbow", capi.METHOD_RAINBOW),
                                             ("ssor", capi.METHOD_SSOR)])
-def test_mixed_precision_refinement(variant, method):
+def test_mixed_precision_refinement(variant, method, inner_restarts):
     """FP32 inner solves + FP64 refinement reach ||b-Ax|| <= 1e-10 ||b|| in FP64
     and agree with the FP64 solve of the same system to the accuracy that
-    residual allows (relative error bound 1e-10 * condition estimate)."""
+    residual allows (relative error bound 1e-10 * condition estimate); inner
+    solves that continue the previous direction and inner solves that
+    restart alike."""
     system = S.poisson3d_p1_kuhn(16)
     b = S.rhs_uniform(system, 8)
     d = system.diagonal()
     col = oracle.rainbow(system.n, system.A, system.C)
     colour_args = col if variant == "rainbow" else (0, None, None, None)
     with capi.System(method, system.n, system.A, system.C, system.V, d, 1.0, *colour_args,
-                     fp32_inner=True) as sysm:
+                     fp32_inner=True, inner_restarts=inner_restarts) as sysm:
         x = np.zeros(system.n)
         info = sysm.solve_mixed(x, b, 1e-10, inner_rtol=1e-4)
         assert info["success"] == 1 and info["relative_residual"] <= 1e-10, info
@@ -336,6 +339,40 @@ def test_mixed_precision_refinement(variant, method):
         assert err <= 1e-8, err
         print(f"[mixed] {variant}: outer {info['outer']} inner {info['inner_iterations']} "
               f"rel.res {info['relative_residual']:.2e} err vs FP64 {err:.2e}")
+
+
+@pytest.mark.parametrize("variant,method", [("rainbow", capi.METHOD_RAINBOW),
+                                            ("cg", capi.METHOD_CG)])
+def test_mixed_continued_direction_keeps_the_krylov_space(variant, method):
+    """With many corrections (inner_rtol 1e-2), inner solves that continue the
+    previous one's direction need far fewer FP32 iterations than inner solves
+    that restart, and about as many as one FP64 solve of the same system;
+    both reach the FP64 residual target."""
+    system = S.poisson3d_p1_kuhn(16)
+    b = S.rhs_uniform(system, 9)
+    d = system.diagonal()
+    col = oracle.rainbow(system.n, system.A, system.C) if variant == "rainbow" else (0, None, None, None)
+    aux = d if variant == "rainbow" else None
+    runs = {}
+    for restarts in (False, True):
+        with capi.System(method, system.n, system.A, system.C, system.V, aux, 1.0, *col,
+                         fp32_inner=True, inner_restarts=restarts) as sysm:
+            x = np.zeros(system.n)
+            info = sysm.solve_mixed(x, b, 1e-10, inner_rtol=1e-2)
+            assert info["success"] == 1 and info["relative_residual"] <= 1e-10, (restarts, info)
+            res = b - oracle.csr_apply(system.n, system.A, system.C, system.V, x)
+            res[system.dirichlet] = 0.0
+            assert np.linalg.norm(res) <= 1.0000001e-10 * np.linalg.norm(b)
+            runs[restarts] = info
+    _, _, i64 = oracle.solve(variant, system.A, system.C, system.V, b, aux=aux,
+                             colouring=col if variant == "rainbow" else None)
+    cont, rest = runs[False], runs[True]
+    print(f"[continued] {variant}: continued {cont['outer']} outer / {cont['inner_iterations']} "
+          f"inner; restarts {rest['outer']} / {rest['inner_iterations']}; FP64 K "
+          f"{i64['iterations']}", flush=True)
+    assert cont["outer"] >= 3 and cont["continued"] >= 1 and rest["continued"] == 0
+    assert cont["inner_iterations"] < 0.8 * rest["inner_iterations"]
+    assert cont["inner_iterations"] <= 1.3 * i64["iterations"] + 10
 
 
 @pytest.mark.parametrize("maker", [lambda: S.poisson3d_p1_kuhn(12), lambda: S.poisson2d_p1(6),
