@@ -2,6 +2,8 @@
 // error type and HIP checks, device allocation, SELL parts in device memory.
 #pragma once
 
+#include <type_traits>
+
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -98,8 +100,15 @@ struct DevPartT {
             SFA_HIP(hipMemcpy(rows, p.slice_rows.data(), p.slice_rows.size() * sizeof(int), hipMemcpyHostToDevice));
         if (!p.cols.empty()) {
             SFA_HIP(hipMemcpy(cols, p.cols.data(), p.cols.size() * sizeof(int), hipMemcpyHostToDevice));
-            std::vector<T> tv(p.vals.begin(), p.vals.end());
-            SFA_HIP(hipMemcpy(vals, tv.data(), tv.size() * sizeof(T), hipMemcpyHostToDevice));
+            if constexpr (std::is_same<T, double>::value) {
+                SFA_HIP(hipMemcpy(vals, p.vals.data(), p.vals.size() * sizeof(double), hipMemcpyHostToDevice));
+            } else {
+                HostArray<T> tv(p.vals.size());
+                const long long nv = (long long)p.vals.size();
+#pragma omp parallel for schedule(static)
+                for (long long i = 0; i < nv; i++) tv[i] = (T)p.vals[i];
+                SFA_HIP(hipMemcpy(vals, tv.data(), tv.size() * sizeof(T), hipMemcpyHostToDevice));
+            }
         }
     }
     void release()

@@ -63,80 +63,6 @@ int rainbow_colour(int n, const int* A, const int* C, int* F, int* B, int* R)
 
 namespace {
 
-// Appends the slices of one group of rows (device order) to a part.  keep(k)
-// decides whether CSR entry k of the row goes into this part.  Two passes
-// over the group's slices, each parallel over slices: the widths, then (after
-// their prefix sum fixes every slice's offset) the entries.
-template <class Keep>
-void append_group(SellPart& part, const std::vector<int>& rows_dev, const std::vector<int>& perm,
-                  const std::vector<int>& iperm, int n, const int* A, const int* C,
-                  const double* V, Keep keep)
-{
-    const int count = (int)rows_dev.size();
-    part.zero_slot = n;
-    bool consecutive = true;
-    for (int i = 1; i < count && consecutive; i++) consecutive = rows_dev[i] == rows_dev[0] + i;
-    part.pending_row0 = (count > 0 && consecutive) ? rows_dev[0] : (count == 0 ? 0 : -1);
-    const int nsl = (count + kSliceRows - 1) / kSliceRows;
-    part.pending_width = 0;
-    if (nsl == 0) return;
-    std::vector<int> width(nsl, 0);
-#pragma omp parallel for schedule(dynamic, 64)
-    for (int sl = 0; sl < nsl; sl++) {
-        const int s1 = std::min(count, (sl + 1) * kSliceRows);
-        int wmax = 0;
-        for (int i = sl * kSliceRows; i < s1; i++) {
-            const int orow = perm[rows_dev[i]];
-            int w = 0;
-            for (int k = A[orow]; k < A[orow + 1]; k++) w += keep(orow, k) ? 1 : 0;
-            wmax = std::max(wmax, w);
-        }
-        width[sl] = wmax;
-    }
-    // One width for the whole group when that pads at most 10% more entries.
-    long long wsum = 0;
-    int wmax = 0;
-    for (int w : width) wsum += w, wmax = std::max(wmax, w);
-    if (((long long)wmax * nsl - wsum) * 10 <= wsum) {
-        for (int& w : width) w = wmax;
-        part.pending_width = wmax;
-    } else {
-        part.pending_width = -1;
-    }
-    const int first = part.nslices;
-    for (int sl = 0; sl < nsl; sl++)
-        part.slice_off.push_back(part.slice_off.back() + width[sl] * kSliceRows);
-    const size_t total = (size_t)part.slice_off.back();
-    part.cols.resize(total, n);
-    part.vals.resize(total, 0.0);
-    const size_t row_base = part.slice_rows.size();
-    part.slice_rows.resize(row_base + (size_t)nsl * kSliceRows, -1);
-    long long entries = 0;
-#pragma omp parallel for schedule(dynamic, 64) reduction(+ : entries)
-    for (int sl = 0; sl < nsl; sl++) {
-        const size_t base = (size_t)part.slice_off[first + sl];
-        const int s0 = sl * kSliceRows, s1 = std::min(count, s0 + kSliceRows);
-        for (int i = s0; i < s1; i++) {
-            const int lane = i - s0;
-            const int p = rows_dev[i];
-            const int orow = perm[p];
-            part.slice_rows[row_base + (size_t)sl * kSliceRows + lane] = p;
-            int w = 0;
-            for (int k = A[orow]; k < A[orow + 1]; k++) {
-                if (!keep(orow, k)) continue;
-                const size_t at = base + (size_t)w * kSliceRows + lane;
-                part.cols[at] = iperm[C[k]];
-                part.vals[at] = V[k];
-                w++;
-            }
-            entries += w;
-        }
-    }
-    part.entries += entries;
-    part.rows += count;
-    part.nslices += nsl;
-}
-
 void start_part(SellPart& part)
 {
     part = SellPart();
@@ -144,19 +70,118 @@ void start_part(SellPart& part)
     part.group_slice.push_back(0);
 }
 
-void close_group(SellPart& part)
+// Builds a part from groups of rows (device-order ids, in launch order): each
+// group's rows fill consecutive SELL-64 slices.  keep(orow, k) decides whether
+// CSR entry k of original row orow goes into the part.  Two passes, each
+// parallel over the slices of all groups at once: the widths, then (after
+// their prefix sum fixes every slice's offset) the entries and padding.
+template <class Keep>
+void build_part(SellPart& part, const std::vector<std::vector<int>>& groups,
+                const std::vector<int>& perm, const std::vector<int>& iperm, int n, const int* A,
+                const int* C, const double* V, Keep keep)
 {
-    long long e = part.entries, r = part.rows;
-    for (long long v : part.group_entries) e -= v;
-    for (long long v : part.group_rows) r -= v;
-    part.group_entries.push_back(e);
-    part.group_rows.push_back(r);
-    part.group_row0.push_back(part.pending_row0);
-    part.group_width.push_back(part.pending_width);
-    part.pending_row0 = -1;
-    part.pending_width = -1;
-    part.group_slice.push_back(part.nslices);
+    start_part(part);
+    part.zero_slot = n;
+    const int G = (int)groups.size();
+    std::vector<int> gs(G + 1, 0);  // first slice of each group
+    for (int g = 0; g < G; g++)
+        gs[g + 1] = gs[g] + ((int)groups[g].size() + kSliceRows - 1) / kSliceRows;
+    const int ns = gs[G];
+    std::vector<int> sg(ns);  // group of each slice
+    for (int g = 0; g < G; g++)
+        for (int sl = gs[g]; sl < gs[g + 1]; sl++) sg[sl] = g;
+
+    std::vector<int> width(ns, 0);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int sl = 0; sl < ns; sl++) {
+        const std::vector<int>& rows = groups[sg[sl]];
+        const int s0 = (sl - gs[sg[sl]]) * kSliceRows;
+        const int s1 = std::min((int)rows.size(), s0 + kSliceRows);
+        int wmax = 0;
+        for (int i = s0; i < s1; i++) {
+            const int orow = perm[rows[i]];
+            int w = 0;
+            for (int k = A[orow]; k < A[orow + 1]; k++) w += keep(orow, k) ? 1 : 0;
+            wmax = std::max(wmax, w);
+        }
+        width[sl] = wmax;
+    }
+
+    // Per group: one width for all its slices when that pads at most 10% more
+    // entries; the first row when its rows are consecutive.
+    for (int g = 0; g < G; g++) {
+        const std::vector<int>& rows = groups[g];
+        const int count = (int)rows.size();
+        bool consecutive = true;
+        for (int i = 1; i < count && consecutive; i++) consecutive = rows[i] == rows[0] + i;
+        part.group_row0.push_back((count > 0 && consecutive) ? rows[0] : (count == 0 ? 0 : -1));
+        long long wsum = 0;
+        int wmax = 0;
+        for (int sl = gs[g]; sl < gs[g + 1]; sl++) wsum += width[sl], wmax = std::max(wmax, width[sl]);
+        const int nsl = gs[g + 1] - gs[g];
+        if (nsl == 0) {
+            part.group_width.push_back(0);
+        } else if (((long long)wmax * nsl - wsum) * 10 <= wsum) {
+            for (int sl = gs[g]; sl < gs[g + 1]; sl++) width[sl] = wmax;
+            part.group_width.push_back(wmax);
+        } else {
+            part.group_width.push_back(-1);
+        }
+    }
+    part.slice_off.resize((size_t)ns + 1);
+    for (int sl = 0; sl < ns; sl++) part.slice_off[sl + 1] = part.slice_off[sl] + width[sl] * kSliceRows;
+    const size_t total = (size_t)part.slice_off[ns];
+    part.cols.resize(total);
+    part.vals.resize(total);
+    part.slice_rows.resize((size_t)ns * kSliceRows);
+
+    std::vector<long long> stored(ns, 0);  // real entries per slice
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int sl = 0; sl < ns; sl++) {
+        const std::vector<int>& rows = groups[sg[sl]];
+        const int s0 = (sl - gs[sg[sl]]) * kSliceRows;
+        const int count = std::min((int)rows.size() - s0, kSliceRows);
+        const size_t base = (size_t)part.slice_off[sl];
+        const int wsl = width[sl];
+        long long e = 0;
+        for (int lane = 0; lane < kSliceRows; lane++) {
+            int w = 0;
+            if (lane < count) {
+                const int p = rows[s0 + lane];
+                const int orow = perm[p];
+                part.slice_rows[(size_t)sl * kSliceRows + lane] = p;
+                for (int k = A[orow]; k < A[orow + 1]; k++) {
+                    if (!keep(orow, k)) continue;
+                    const size_t at = base + (size_t)w * kSliceRows + lane;
+                    part.cols[at] = iperm[C[k]];
+                    part.vals[at] = V[k];
+                    w++;
+                }
+            } else {
+                part.slice_rows[(size_t)sl * kSliceRows + lane] = -1;
+            }
+            e += w;
+            for (; w < wsl; w++) {  // padding: the zero slot, value 0
+                const size_t at = base + (size_t)w * kSliceRows + lane;
+                part.cols[at] = n;
+                part.vals[at] = 0.0;
+            }
+        }
+        stored[sl] = e;
+    }
+
+    for (int g = 0; g < G; g++) {
+        long long e = 0;
+        for (int sl = gs[g]; sl < gs[g + 1]; sl++) e += stored[sl];
+        part.group_entries.push_back(e);
+        part.group_rows.push_back((long long)groups[g].size());
+        part.group_slice.push_back(gs[g + 1]);
+        part.entries += e;
+        part.rows += (long long)groups[g].size();
+    }
+    part.nslices = ns;
 }
+
 
 // Dependency levels of a triangular sweep over the active rows: rows of one
 // level read no row of the same or a later level.  dep(orow, k) tells whether
@@ -209,12 +234,11 @@ void spmv_bucket_counts(const SellPart& part, long long rows[kSpmvBuckets],
 void build_full_identity(SellPart& part, int rows, int columns, const int* A, const int* C,
                          const double* V)
 {
-    std::vector<int> perm(std::max(rows, columns) + 1), list(rows);
+    std::vector<int> perm(std::max(rows, columns) + 1);
+    std::vector<std::vector<int>> list(1, std::vector<int>(rows));
     for (int i = 0; i < (int)perm.size(); i++) perm[i] = i;
-    for (int i = 0; i < rows; i++) list[i] = i;
-    start_part(part);
-    append_group(part, list, perm, perm, columns, A, C, V, [](int, int) { return true; });
-    close_group(part);
+    for (int i = 0; i < rows; i++) list[0][i] = i;
+    build_part(part, list, perm, perm, columns, A, C, V, [](int, int) { return true; });
 }
 
 namespace {
@@ -384,15 +408,13 @@ std::string build_plan(Plan& plan, int method, int n, const int* A, const int* C
     for (int r = 0; r < n; r++)
         if (active[r]) active_rows.push_back(r);
     by_rank(active_rows);
-    std::vector<int> active_dev = to_dev(active_rows);
-    start_part(plan.full);
+    const std::vector<std::vector<int>> active_dev(1, to_dev(active_rows));
     if (eisen)  // solver/sparsesolver.cpp:806-807,1420-1421 skip columns with diagonal 0
-        append_group(plan.full, active_dev, plan.perm, iperm, n, A, C, V,
-                     [&](int, int k) { return guarded(k); });
+        build_part(plan.full, active_dev, plan.perm, iperm, n, A, C, V,
+                   [&](int, int k) { return guarded(k); });
     else
-        append_group(plan.full, active_dev, plan.perm, iperm, n, A, C, V,
-                     [](int, int) { return true; });
-    close_group(plan.full);
+        build_part(plan.full, active_dev, plan.perm, iperm, n, A, C, V,
+                   [](int, int) { return true; });
 
     if (!sweeps) return "";
 
@@ -402,40 +424,27 @@ std::string build_plan(Plan& plan, int method, int n, const int* A, const int* C
     auto lower_rb = [&](int orow, int k) { return F[C[k]] < F[orow]; };
     auto upper_rb = [&](int orow, int k) { return F[C[k]] > F[orow]; };
 
-    start_part(plan.lower);
-    start_part(plan.upper);
-    for (const auto& g : fwd_groups) {
-        std::vector<int> dev = to_dev(g);
-        if (method == SFA_METHOD_SSOR)
-            append_group(plan.lower, dev, plan.perm, iperm, n, A, C, V, lower_nat);
-        else if (method == SFA_METHOD_SSOR_EISENSTAT)  // :824,903
-            append_group(plan.lower, dev, plan.perm, iperm, n, A, C, V,
-                         [&](int o, int k) { return lower_nat(o, k) && guarded(k); });
-        else
-            append_group(plan.lower, dev, plan.perm, iperm, n, A, C, V, lower_rb);
-        close_group(plan.lower);
-    }
-    for (const auto& g : bwd_groups) {
-        std::vector<int> dev = to_dev(g);
-        if (method == SFA_METHOD_SSOR)
-            append_group(plan.upper, dev, plan.perm, iperm, n, A, C, V, upper_nat);
-        else if (method == SFA_METHOD_SSOR_EISENSTAT)  // :886
-            append_group(plan.upper, dev, plan.perm, iperm, n, A, C, V,
-                         [&](int o, int k) { return upper_nat(o, k) && guarded(k); });
-        else
-            append_group(plan.upper, dev, plan.perm, iperm, n, A, C, V, upper_rb);
-        close_group(plan.upper);
+    std::vector<std::vector<int>> fwd_dev, bwd_dev;
+    for (const auto& g : fwd_groups) fwd_dev.push_back(to_dev(g));
+    for (const auto& g : bwd_groups) bwd_dev.push_back(to_dev(g));
+    if (method == SFA_METHOD_SSOR) {
+        build_part(plan.lower, fwd_dev, plan.perm, iperm, n, A, C, V, lower_nat);
+        build_part(plan.upper, bwd_dev, plan.perm, iperm, n, A, C, V, upper_nat);
+    } else if (method == SFA_METHOD_SSOR_EISENSTAT) {  // :824,903 and :886
+        build_part(plan.lower, fwd_dev, plan.perm, iperm, n, A, C, V,
+                   [&](int o, int k) { return lower_nat(o, k) && guarded(k); });
+        build_part(plan.upper, bwd_dev, plan.perm, iperm, n, A, C, V,
+                   [&](int o, int k) { return upper_nat(o, k) && guarded(k); });
+    } else {
+        build_part(plan.lower, fwd_dev, plan.perm, iperm, n, A, C, V, lower_rb);
+        build_part(plan.upper, bwd_dev, plan.perm, iperm, n, A, C, V, upper_rb);
     }
     if (method == SFA_METHOD_EISENSTAT_RAINBOW) {
         // The restart's lower solve skips columns with diagonal 0 (:1448), the
         // iteration's does not (:1550-1552).
         plan.has_lower_restart = true;
-        start_part(plan.lower_restart);
-        for (const auto& g : fwd_groups) {
-            append_group(plan.lower_restart, to_dev(g), plan.perm, iperm, n, A, C, V,
-                         [&](int o, int k) { return lower_rb(o, k) && guarded(k); });
-            close_group(plan.lower_restart);
-        }
+        build_part(plan.lower_restart, fwd_dev, plan.perm, iperm, n, A, C, V,
+                   [&](int o, int k) { return lower_rb(o, k) && guarded(k); });
     }
     return "";
 }

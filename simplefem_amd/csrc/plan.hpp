@@ -18,19 +18,42 @@
 #pragma once
 
 #include <cstdint>
+#include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 namespace sfa {
 
 constexpr int kSliceRows = 64;  // one gfx950 wavefront
 
+// Leaves new elements uninitialised instead of zero-filling them: the SELL
+// arrays are written in full by parallel loops, which then also take the
+// first touch of their pages (a serial zero fill of a 10^7-entry part costs
+// about as much as building it).
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <class U>
+    NoInitAlloc(const NoInitAlloc<U>&) noexcept {}
+    template <class U>
+    void construct(U* p) noexcept { ::new ((void*)p) U; }
+    template <class U, class... Args>
+    void construct(U* p, Args&&... args) { ::new ((void*)p) U(std::forward<Args>(args)...); }
+};
+template <class T>
+using HostArray = std::vector<T, NoInitAlloc<T>>;
+
 struct SellPart {
     int nslices = 0;
     std::vector<int> slice_off;    // nslices+1 offsets into cols/vals (entries)
-    std::vector<int> slice_rows;   // nslices*64 device-order row ids, -1 = pad
-    std::vector<int> cols;         // device-order column ids, n = zero slot
-    std::vector<double> vals;
+    HostArray<int> slice_rows;     // nslices*64 device-order row ids, -1 = pad
+    HostArray<int> cols;           // device-order column ids, n = zero slot
+    HostArray<double> vals;
     std::vector<int> group_slice;  // ngroups+1 slice ranges, in launch order
     std::vector<long long> group_entries;  // real entries per group
     std::vector<long long> group_rows;     // real rows per group
@@ -40,8 +63,6 @@ struct SellPart {
     // entries per lane of every slice of a group when they are all padded to
     // the same width (the kernels then compute slice offsets), else -1
     std::vector<int> group_width;
-    int pending_row0 = -1;
-    int pending_width = -1;
     int zero_slot = 0;             // padding column = the vectors' zero slot
     long long entries = 0;         // real (unpadded) entries
     long long rows = 0;            // real rows

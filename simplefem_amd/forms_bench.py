@@ -260,17 +260,20 @@ def _served_fp64_one_at_a_time(jobs):
 def _solve_dedicated(job, inner_rtol, start, inner_restarts=False):
     """A large request on its own device-resident System and stream: colouring,
     plan, upload and the mixed-precision solve, on a worker thread while the
-    batch engine runs.  Returns the completion time (seconds after start)."""
+    batch engine runs.  Returns the times (seconds after start) at which the
+    worker took it, the System was ready, and the solve had finished."""
+    t_take = time.perf_counter() - start
     s = job.system
     rb = capi.Rainbow(s.n, s.A, s.C)
     with capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, s.diagonal(), 1.0, rb.num_colors,
                      rb.F, rb.B, rb.R, fp32_inner=True, inner_restarts=inner_restarts) as sysd:
+        t_ready = time.perf_counter() - start
         bt = DeviceArray.from_host(job.rhs[0])
         x = DeviceArray(s.n)
         sysd.solve_mixed(x, bt, 1e-10, inner_rtol=inner_rtol, max_outer=50)  # returns when done
         bt.free()
         x.free()
-    return time.perf_counter() - start
+    return t_take, t_ready, time.perf_counter() - start
 
 
 def served(args, budget=200e9):
@@ -332,8 +335,11 @@ def served(args, budget=200e9):
                 lat.append(time.perf_counter() - start - j.arrival)
                 eng.remove_matrix(mid)
                 done += 1
+        phases = []  # per large request: wait for a worker, create (colour, plan, upload), solve
         for j, fut in dedicated:  # the batch is drained; wait for the large requests
-            lat.append(fut.result() - j.arrival)
+            t_take, t_ready, t_done = fut.result()
+            lat.append(t_done - j.arrival)
+            phases.append((t_take - j.arrival, t_ready - t_take, t_done - t_ready))
             done += 1
         total = time.perf_counter() - start
     lat = np.array(lat)
@@ -352,7 +358,13 @@ def served(args, budget=200e9):
            "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
            "makespan_s": total, "prep_workers": args.prep_workers,
            "dedicated": {"rows_at_least": args.route_rows, "workers": args.dedicated_workers,
-                         "requests": len(dedicated)},
+                         "requests": len(dedicated),
+                         "mean_s": dict(zip(("wait", "create", "solve"),
+                                            np.mean(phases, axis=0).round(3).tolist()))
+                         if phases else {},
+                         "max_s": dict(zip(("wait", "create", "solve"),
+                                           np.max(phases, axis=0).round(3).tolist()))
+                         if phases else {}},
            "fp64_one_at_a_time": {"solves_per_s": len(jobs) / total64,
                                   "p50_latency_ms": float(np.percentile(lat64, 50) * 1e3),
                                   "p99_latency_ms": float(np.percentile(lat64, 99) * 1e3),
