@@ -408,6 +408,43 @@ int sfa_batch_set_spmv(sfa_batch b, const int lanes_per_row[SFA_SPMV_BUCKETS]);
 
 int sfa_batch_destroy(sfa_batch b);
 
+/* ---------------------------------------------------------------------------
+ * Block (Hodge) systems.  HodgeConjugateResidualSolverCSR_SSOR,
+ * solver/systemsparsesolver.hpp:58-72 (solver/systemsparsesolver.cpp:426-797):
+ * conjugate residuals on the Schur complement (B A^-1 Bt + C) x = b, every
+ * A^-1 an inner natural-order CG-SSOR solve (omega 1, the diagonal of A) of
+ * precision inneriteration_precision.  A is L x L, B is N x L, Bt is L x N,
+ * C is N x N, all CSR as MatrixCSR::getA/getC/getV hand them over (C may
+ * have no entries).  K and the final Md.r come back in the sfa_solve_info
+ * (zr = Md.r; success iff sqrt(Md.r) < tolerance).
+ * ------------------------------------------------------------------------- */
+int sfa_HodgeConjugateResidualSolverCSR_SSOR(
+    int N, int L, double* x, const double* b,
+    const int* Arows, const int* Acolumns, const double* Avalues,
+    const int* Brows, const int* Bcolumns, const double* Bvalues,
+    const int* Btrows, const int* Btcolumns, const double* Btvalues,
+    const int* Crows, const int* Ccolumns, const double* Cvalues,
+    double* residual, double precision, int print_modulo,
+    double inneriteration_precision, int inneriteration_print_modulo,
+    sfa_solve_info* info);
+
+/* The same solver with the four matrices resident on the device between
+ * solves (A as an sfa_system with its SSOR schedules): for callers that
+ * solve the same block system for many right-hand sides.  No reference
+ * counterpart.  device_pointers = 1: x, b and residual are device buffers. */
+typedef struct sfa_hodge_s* sfa_hodge;
+int sfa_hodge_create(int N, int L,
+                     const int* Arows, const int* Acolumns, const double* Avalues,
+                     const int* Brows, const int* Bcolumns, const double* Bvalues,
+                     const int* Btrows, const int* Btcolumns, const double* Btvalues,
+                     const int* Crows, const int* Ccolumns, const double* Cvalues,
+                     sfa_hodge* out);
+int sfa_hodge_solve(sfa_hodge h, double* x, const double* b, double* residual,
+                    double precision, int print_modulo,
+                    double inneriteration_precision, int inneriteration_print_modulo,
+                    int device_pointers, sfa_solve_info* info);
+int sfa_hodge_destroy(sfa_hodge h);
+
 #ifdef __cplusplus
 }
 #endif

@@ -611,3 +611,102 @@ int orc_cg_eisenstat_rainbow(int N, double* x, const double* b, const int* A,
     free(direction); free(zirconium); free(technical); free(auxiliary);
     return K;
 }
+
+/* HodgeConjugateResidualSolverCSR_SSOR, solver/systemsparsesolver.cpp:426-797:
+ * conjugate residuals on the Schur complement (B A^-1 Bt + C) x = b (with
+ * expected_sign_of_A = 1, :8), every A^-1 an inner ConjugateGradientSolverCSR_SSOR
+ * (omega 1, the diagonal of A).  The two inner solves of the restart start
+ * from 0, the iteration's from the previous inner solution.  info->zr holds
+ * the final Md_r (the reference's restart flags are both false, :11-12, so
+ * only K == 0 restarts). */
+int orc_hodge_cr_ssor(int N, int L, double* x, const double* b,
+                      const int* Arows, const int* Acols, const double* Avals,
+                      const int* Brows, const int* Bcols, const double* Bvals,
+                      const int* Btrows, const int* Btcols, const double* Btvals,
+                      const int* Crows, const int* Ccols, const double* Cvals,
+                      double* res, double precision, int print_modulo,
+                      double inner_precision, int inner_print_modulo, orc_info* info)
+{
+    const double tol = orc_tolerance(N, b, precision);           /* :457-461 */
+    double* dir  = orc_fill(N, 0.);
+    double* Mdir = orc_fill(N, 0.);
+    double* Mres = orc_fill(N, 0.);
+    double* vil  = orc_fill(N, 0.);
+    double* aux1 = orc_fill(L, 0.);
+    double* aux2 = orc_fill(L, 0.);
+    double* auxR = orc_fill(L, 0.);
+    double* diagonal = orc_fill(L, 0.);
+    for (int c = 0; c < L; c++)                                   /* :498-511 */
+        for (int d = Arows[c]; d < Arows[c + 1]; d++)
+            if (Acols[d] == c) diagonal[c] += Avals[d];
+    double Md_r = NAN, Md_Md = NAN;
+    int K = 0, reason = ORC_EXIT_CAP;
+    while (K < N) {
+        if (K == 0) {
+            for (int c = 0; c < L; c++) {                         /* :533-542 */
+                aux1[c] = 0.;
+                for (int d = Btrows[c]; d < Btrows[c + 1]; d++) aux1[c] += Btvals[d] * x[Btcols[d]];
+            }
+            for (int c = 0; c < L; c++) aux2[c] = 0.;            /* :544-554 */
+            orc_cg_ssor(L, aux2, aux1, Arows, Acols, Avals, auxR, inner_precision,
+                        inner_print_modulo, diagonal, 1., NULL);
+            for (int c = 0; c < N; c++) {                         /* :556-570 */
+                res[c] = b[c];
+                for (int d = Brows[c]; d < Brows[c + 1]; d++) res[c] -= 1. * Bvals[d] * aux2[Bcols[d]];
+                for (int d = Crows[c]; d < Crows[c + 1]; d++) res[c] -= Cvals[d] * x[Ccols[d]];
+                dir[c] = res[c];
+            }
+            Md_r = 0.;
+            Md_Md = 0.;
+            for (int c = 0; c < L; c++) {                         /* :576-586 */
+                aux1[c] = 0.;
+                for (int d = Btrows[c]; d < Btrows[c + 1]; d++) aux1[c] += Btvals[d] * dir[Btcols[d]];
+            }
+            for (int c = 0; c < L; c++) aux2[c] = 0.;            /* :588-598 */
+            orc_cg_ssor(L, aux2, aux1, Arows, Acols, Avals, auxR, inner_precision,
+                        inner_print_modulo, diagonal, 1., NULL);
+            for (int c = 0; c < N; c++) {                         /* :600-619 */
+                Mdir[c] = 0.;
+                for (int d = Brows[c]; d < Brows[c + 1]; d++) Mdir[c] += 1. * Bvals[d] * aux2[Bcols[d]];
+                for (int d = Crows[c]; d < Crows[c + 1]; d++) Mdir[c] += Cvals[d] * dir[Ccols[d]];
+                Mres[c] = Mdir[c];
+                Md_r += Mdir[c] * res[c];
+                Md_Md += Mdir[c] * Mdir[c];
+            }
+        }
+        if (!isfinite(Md_r) || Md_r < 0.) { reason = ORC_EXIT_BREAKDOWN; break; }  /* :632-637 */
+        if (orc_absolute(Md_r) < tol * tol) { reason = ORC_EXIT_CONVERGED; break; } /* :639-642 */
+        if (!isfinite(Md_Md) || Md_Md < 0.) { reason = ORC_EXIT_BREAKDOWN; break; } /* :646-652 */
+        if (sqrt(orc_absolute(Md_Md)) < ORC_MACHINE_EPSILON) { reason = ORC_EXIT_SMALL; break; } /* :654-658 */
+        const double alpha = Md_r / Md_Md;                        /* :663 */
+        double new_Mr_r = 0.;
+        for (int c = 0; c < L; c++) {                             /* :667-677 */
+            aux1[c] = 0.;
+            for (int d = Btrows[c]; d < Btrows[c + 1]; d++) aux1[c] += Btvals[d] * Mdir[Btcols[d]];
+        }
+        orc_cg_ssor(L, aux2, aux1, Arows, Acols, Avals, auxR, inner_precision,  /* :679-689 */
+                    inner_print_modulo, diagonal, 1., NULL);
+        for (int c = 0; c < N; c++) {                             /* :691-721 */
+            vil[c] = 0.;
+            for (int d = Brows[c]; d < Brows[c + 1]; d++) vil[c] += 1. * Bvals[d] * aux2[Bcols[d]];
+            for (int d = Crows[c]; d < Crows[c + 1]; d++) vil[c] += Cvals[d] * Mdir[Ccols[d]];
+            x[c] = x[c] + alpha * dir[c];
+            res[c] = res[c] - alpha * Mdir[c];
+            Mres[c] = Mres[c] - alpha * vil[c];
+            new_Mr_r += Mres[c] * res[c];
+        }
+        const double beta = new_Mr_r / Md_r;                      /* :723 */
+        Md_Md = 0.;
+        for (int c = 0; c < N; c++) {                             /* :725-739 */
+            dir[c] = res[c] + beta * dir[c];
+            Mdir[c] = Mres[c] + beta * Mdir[c];
+            Md_Md += Mdir[c] * Mdir[c];
+        }
+        Md_r = new_Mr_r;
+        K++;
+    }
+    orc_finish(info, K, Md_r, tol, reason);
+    free(dir); free(Mdir); free(Mres); free(vil);
+    free(aux1); free(aux2); free(auxR); free(diagonal);
+    return K;
+}

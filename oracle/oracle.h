@@ -22,6 +22,9 @@
  *                                       dense Cholesky solve, natural SSOR ==
  *                                       Rainbow SSOR with one colour per row,
  *                                       oracle/_ref bit-exact (K, x, residual)
+ *   orc_hodge_cr_ssor                 -- dense Schur-complement solve,
+ *                                       B = identity KAT (x = (A^-1 + C)^-1 b),
+ *                                       oracle/_ref bit-exact (K, x, residual)
  */
 #ifndef SFA_ORACLE_H
 #define SFA_ORACLE_H
@@ -80,6 +83,17 @@ int orc_cg_eisenstat_rainbow(int N, double* x, const double* b, const int* A,
                              const double* diagonal, double omega,
                              int num_colors, const int* F, const int* B,
                              const int* R, orc_info* info);
+
+/* HodgeConjugateResidualSolverCSR_SSOR (solver/systemsparsesolver.cpp:426-797):
+ * B is N x L, Bt is L x N, C is N x N, A is L x L. */
+int orc_hodge_cr_ssor(int N, int L, double* x, const double* b,
+                      const int* Arows, const int* Acols, const double* Avals,
+                      const int* Brows, const int* Bcols, const double* Bvals,
+                      const int* Btrows, const int* Btcols, const double* Btvals,
+                      const int* Crows, const int* Ccols, const double* Cvals,
+                      double* res, double precision, int print_modulo,
+                      double inner_precision, int inner_print_modulo,
+                      orc_info* info);
 
 #ifdef __cplusplus
 }

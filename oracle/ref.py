@@ -47,6 +47,9 @@ def _init_library():
                                     _P_I, _P_D, c_double, c_double]
     _lib.feecref_csr_op.restype = c_int
     _lib.feecref_result.argtypes = [_P_I, _P_I, _P_I, _P_D]
+    _lib.feecref_hodge_cr_ssor.argtypes = [c_int, c_int, _P_D, _P_D] + [_P_I, _P_I, _P_D] * 4 + \
+        [_P_D, c_double, c_int, c_double, c_int]
+    _lib.feecref_hodge_cr_ssor.restype = c_int
     _lib.feecref_mesh.argtypes = [c_int, c_int, c_int]
     _lib.feecref_mesh.restype = c_int
     _lib.feecref_mesh_sizes.argtypes = [_P_I]
@@ -113,6 +116,21 @@ def _task(kind, kw):
             out["structural"] = (bA, bC[:nb].copy(), bV[:nb].copy())
             out["compressed"] = (aA, aC[:na].copy(), aV[:na].copy())
         return out
+    if kind == "hodge":
+        N, Ldim = int(kw["N"]), int(kw["L"])
+        mats = []
+        for m in ("A", "B", "Bt", "C"):
+            a, c, v = kw[m]
+            mats += [np.ascontiguousarray(a, dtype=np.int32), np.ascontiguousarray(c, dtype=np.int32),
+                     np.ascontiguousarray(v, dtype=np.float64)]
+        x = np.array(kw["x0"], dtype=np.float64)
+        b = np.ascontiguousarray(kw["b"], dtype=np.float64)
+        res = np.zeros(N)
+        ptrs = [(_i if k % 3 < 2 else _d)(m) for k, m in enumerate(mats)]
+        K = L.feecref_hodge_cr_ssor(N, Ldim, _d(x), _d(b), *ptrs, _d(res), float(kw["precision"]),
+                                    int(kw["print_modulo"]), float(kw["inner_precision"]),
+                                    int(kw["inner_print_modulo"]))
+        return x, res, K
     A = np.ascontiguousarray(kw["A"], dtype=np.int32)
     C = np.ascontiguousarray(kw["C"], dtype=np.int32)
     V = np.ascontiguousarray(kw["V"], dtype=np.float64)
@@ -224,6 +242,15 @@ class RefRunner:
         return self._call("solve", variant=variant, A=A, C=C, V=V, b=b, x0=x0,
                           residual0=residual0, precision=precision, print_modulo=print_modulo,
                           aux=aux, omega=omega, colouring=colouring, repeat=repeat)
+
+    def hodge_cr_ssor(self, N, L, A, B, Bt, C, b, x0=None, precision=1e-10, print_modulo=-3,
+                      inner_precision=1e-10, inner_print_modulo=-3):
+        """HodgeConjugateResidualSolverCSR_SSOR; matrices are (rows, cols, vals)
+        CSR triples.  Returns (x, residual, K)."""
+        x0 = np.zeros(N) if x0 is None else x0
+        return self._call("hodge", N=N, L=L, A=A, B=B, Bt=Bt, C=C, b=b, x0=x0,
+                          precision=precision, print_modulo=print_modulo,
+                          inner_precision=inner_precision, inner_print_modulo=inner_print_modulo)
 
     def close(self):
         if self._proc.poll() is None:

@@ -16,6 +16,7 @@
 
 #include "base/include.hpp"
 #include "solver/sparsesolver.hpp"
+#include "solver/systemsparsesolver.hpp"
 #include "sparse/matcsr.hpp"
 #include "sparse/rainbow.hpp"
 #include "sparse/sparsematrix.hpp"
@@ -185,6 +186,18 @@ int feecref_cg(int variant, int N, double* x, const double* b, const int* A, con
     }
     if (seconds) *seconds = seconds_since(t0);
     return K;
+}
+
+// HodgeConjugateResidualSolverCSR_SSOR (solver/systemsparsesolver.cpp:426-797).
+int feecref_hodge_cr_ssor(int N, int L, double* x, const double* b, const int* Ar, const int* Ac,
+                          const double* Av, const int* Br, const int* Bc, const double* Bv,
+                          const int* Btr, const int* Btc, const double* Btv, const int* Cr,
+                          const int* Cc, const double* Cv, double* res, double precision,
+                          int print_modulo, double inner_precision, int inner_print_modulo)
+{
+    return HodgeConjugateResidualSolverCSR_SSOR(N, L, x, b, Ar, Ac, Av, Br, Bc, Bv, Btr, Btc, Btv,
+                                                Cr, Cc, Cv, res, precision, print_modulo,
+                                                inner_precision, inner_print_modulo);
 }
 
 }  // extern "C"

@@ -32,6 +32,7 @@ DEVICE = ["--offload-arch=" + ARCH, "-Xclang", "-target-feature", "-Xclang", "-p
 SOURCES = [  # (source, compile as HIP)
     ("kernels.hip", True),
     ("batch.hip", True),
+    ("hodge.hip", True),
     ("system.cpp", True),
     ("plan.cpp", False),
 ]

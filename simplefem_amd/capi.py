@@ -181,6 +181,13 @@ _SIGNATURES = {
     "sfa_batch_tune_spmv": ([c_void_p, POINTER(SpmvTuning)], c_int),
     "sfa_batch_set_spmv": ([c_void_p, _P_I], c_int),
     "sfa_batch_destroy": ([c_void_p], c_int),
+    "sfa_HodgeConjugateResidualSolverCSR_SSOR":
+        ([c_int, c_int, _P_D, _P_D] + [_P_I, _P_I, _P_D] * 4 +
+         [_P_D, c_double, c_int, c_double, c_int, POINTER(SolveInfo)], c_int),
+    "sfa_hodge_create": ([c_int, c_int] + [_P_I, _P_I, _P_D] * 4 + [POINTER(c_void_p)], c_int),
+    "sfa_hodge_solve": ([c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int, c_double, c_int,
+                         c_int, POINTER(SolveInfo)], c_int),
+    "sfa_hodge_destroy": ([c_void_p], c_int),
 }
 
 _lib = None
@@ -578,3 +585,79 @@ class Batch:
 
     def __exit__(self, *exc):
         self.close()
+
+
+# --------------------------------------------------------------------------- block systems
+
+def _blocks(N, L, A, B, Bt, C):
+    out = []
+    for (r, c, v), rows in ((A, L), (B, N), (Bt, L), (C, N)):
+        r, c, v = _csr(r, c, v)
+        if r.shape[0] != rows + 1:
+            raise ValueError(f"a block has {r.shape[0] - 1} rows, needs {rows}")
+        out += [r, c, v]
+    return out
+
+
+def HodgeConjugateResidualSolverCSR_SSOR(N, L, x, b, A, B, Bt, C, residual, precision,
+                                         print_modulo, inneriteration_precision,
+                                         inneriteration_print_modulo, info=None):
+    """solver/systemsparsesolver.hpp:58-72: (B A^-1 Bt + C) x = b by conjugate
+    residuals with inner CG-SSOR solves of A.  A, B, Bt, C are
+    (rows, cols, vals) CSR triples.  Returns K."""
+    lib = load()
+    x = _f64(x, "x", N, writable=True)
+    residual = _f64(residual, "residual", N, writable=True)
+    b = _f64(b, "b", N)
+    mats = _blocks(N, L, A, B, Bt, C)
+    ptrs = [(_i if k % 3 < 2 else _d)(m) for k, m in enumerate(mats)]
+    out = SolveInfo()
+    _check(lib.sfa_HodgeConjugateResidualSolverCSR_SSOR(
+        int(N), int(L), _d(x), _d(b), *ptrs, _d(residual), float(precision), int(print_modulo),
+        float(inneriteration_precision), int(inneriteration_print_modulo), ctypes.byref(out)))
+    if info is not None:
+        info.update(out.as_dict())
+    return out.iterations
+
+
+class Hodge:
+    """A block system kept on the device between solves (sfa_hodge_*)."""
+
+    def __init__(self, N, L, A, B, Bt, C):
+        self._h = c_void_p()
+        self.N, self.L = int(N), int(L)
+        mats = _blocks(self.N, self.L, A, B, Bt, C)
+        ptrs = [(_i if k % 3 < 2 else _d)(m) for k, m in enumerate(mats)]
+        _check(load().sfa_hodge_create(self.N, self.L, *ptrs, ctypes.byref(self._h)))
+
+    def solve(self, x, b, residual, precision=1e-10, print_modulo=-3,
+              inneriteration_precision=1e-10, inneriteration_print_modulo=-3):
+        """x, b, residual: all host float64 arrays or all device buffers
+        (simplefem_amd.hipmem.DeviceArray).  Returns the info dict."""
+        x, px, dx = System._ptr(x, self.N, True)
+        b, pb, db = System._ptr(b, self.N, False)
+        residual, pr, dr = System._ptr(residual, self.N, True)
+        if not (dx == db == dr):
+            raise TypeError("x, b and residual must all be host arrays or all device buffers")
+        out = SolveInfo()
+        _check(load().sfa_hodge_solve(self._h, px, pb, pr, float(precision), int(print_modulo),
+                                      float(inneriteration_precision),
+                                      int(inneriteration_print_modulo), dx, ctypes.byref(out)))
+        return out.as_dict()
+
+    def close(self):
+        if self._h:
+            load().sfa_hodge_destroy(self._h)
+            self._h = c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

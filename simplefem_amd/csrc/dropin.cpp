@@ -18,6 +18,7 @@
 #include "../../include/feec/operators/linearoperator.hpp"
 #include "../../include/feec/solver/inv.hpp"
 #include "../../include/feec/solver/sparsesolver.hpp"
+#include "../../include/feec/solver/systemsparsesolver.hpp"
 #include "../../include/feec/sparse/matcsr.hpp"
 #include "../../include/feec/sparse/rainbow.hpp"
 #include "../../include/feec/sparse/sparsematrix.hpp"
@@ -879,6 +880,22 @@ int ConjugateGradientSolverCSR_SSOR(const int N, Float* x, const Float* b, const
     require(sfa_ConjugateGradientSolverCSR_SSOR(N, x, b, csrrows, csrcolumns, csrvalues, residual,
                                                 precision, print_modulo, diagonal, omega, &info),
             "ConjugateGradientSolverCSR_SSOR");
+    return info.iterations;
+}
+
+int HodgeConjugateResidualSolverCSR_SSOR(
+    const int N, const int L, Float* x, const Float* b, const int* Arows, const int* Acolumns,
+    const Float* Avalues, const int* Brows, const int* Bcolumns, const Float* Bvalues,
+    const int* Btrows, const int* Btcolumns, const Float* Btvalues, const int* Crows,
+    const int* Ccolumns, const Float* Cvalues, Float* residual, Float precision, int print_modulo,
+    Float inneriteration_precision, int inneriteration_print_modulo)
+{
+    sfa_solve_info info{};
+    require(sfa_HodgeConjugateResidualSolverCSR_SSOR(
+                N, L, x, b, Arows, Acolumns, Avalues, Brows, Bcolumns, Bvalues, Btrows, Btcolumns,
+                Btvalues, Crows, Ccolumns, Cvalues, residual, precision, print_modulo,
+                inneriteration_precision, inneriteration_print_modulo, &info),
+            "HodgeConjugateResidualSolverCSR_SSOR");
     return info.iterations;
 }
 

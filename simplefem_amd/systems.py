@@ -708,3 +708,68 @@ def lagrange_pr(dim: int, n: int, r: int) -> CSRSystem:
                     (row + offsets[k]).astype(np.int32), V, dirichlet)
     _PR_CACHE[cache_key] = out
     return out
+
+
+@dataclass
+class BlockSystem:
+    """Blocks of a saddle-point system [A Bt; B -C] for the Hodge solver
+    (solver/systemsparsesolver.hpp:58-72): A is L x L (SPD), B is N x L, Bt
+    its transpose, C is N x N; each a (rows, cols, vals) CSR triple."""
+    name: str
+    N: int
+    L: int
+    A: tuple
+    B: tuple
+    Bt: tuple
+    C: tuple
+
+    def dense(self):
+        """(A, B, Bt, C) as dense arrays."""
+        def d(m, r, c):
+            M = np.zeros((r, c))
+            rows = np.repeat(np.arange(r), np.diff(m[0]))
+            np.add.at(M, (rows, m[1]), m[2])
+            return M
+        return (d(self.A, self.L, self.L), d(self.B, self.N, self.L),
+                d(self.Bt, self.L, self.N), d(self.C, self.N, self.N))
+
+
+def _rect_csr(nrows, ncols, rows, cols, vals):
+    import scipy.sparse as sp
+
+    m = sp.coo_matrix((vals, (np.asarray(rows, dtype=np.int64), np.asarray(cols, dtype=np.int64))),
+                      shape=(nrows, ncols)).tocsr()
+    m.sum_duplicates()
+    m.sort_indices()
+    return m.indptr.astype(np.int32), m.indices.astype(np.int32), m.data.astype(np.float64)
+
+
+def schur_blocks(m: int, coarse: int = 2, c_weight: float = 0.0, seed: int = 0) -> BlockSystem:
+    """A model block system on an m x m grid: A = the 5-point stencil
+    (4, -1, -1, -1, -1) plus the identity, B = averages over coarse x coarse
+    cells of grid points with seeded weights in [1, 2) (disjoint supports,
+    full row rank), Bt = B^T, C = c_weight times the 5-point stencil on the
+    coarse grid (no entries at all when c_weight is 0, as the reference's
+    Darcy tests pass a zero matrix)."""
+    assert m % coarse == 0
+    lap = fd_laplacian_2d(m)
+    L = m * m
+    rows = np.repeat(np.arange(L), np.diff(lap.A))
+    h2 = 1.0 / (m + 1) ** 2
+    A = _rect_csr(L, L, np.concatenate([rows, np.arange(L)]), np.concatenate([lap.C, np.arange(L)]),
+                  np.concatenate([lap.V * h2, np.ones(L)]))
+    mc = m // coarse
+    N = mc * mc
+    w = 1.0 + 0.5 * (uniform_pm1(seed, L) + 1.0)
+    fine = np.arange(L)
+    fx, fy = divmod(fine, m)
+    cell = (fx // coarse) * mc + (fy // coarse)
+    B = _rect_csr(N, L, cell, fine, w / (coarse * coarse))
+    Bt = _rect_csr(L, N, fine, cell, w / (coarse * coarse))
+    if c_weight > 0:
+        cl = fd_laplacian_2d(mc)
+        crows = np.repeat(np.arange(N), np.diff(cl.A))
+        C = _rect_csr(N, N, crows, cl.C, c_weight * cl.V / (mc + 1) ** 2)
+    else:
+        C = (np.zeros(N + 1, dtype=np.int32), np.zeros(0, dtype=np.int32), np.zeros(0))
+    return BlockSystem(f"schur_m{m}_c{coarse}_w{c_weight:g}", N, L, A, B, Bt, C)

@@ -4,7 +4,8 @@
 // include paths differ from a reference test.  Exit status 0 = all checks hold.
 //
 //   dropin_test cpu   host-side classes only (no GPU needed)
-//   dropin_test gpu   products and all six CG variants on the GPU
+//   dropin_test gpu   products, all six CG variants, InverseOperator and the
+//                     Hodge block solver on the GPU
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -14,6 +15,7 @@
 #include "feec/operators/floatvector.hpp"
 #include "feec/solver/inv.hpp"
 #include "feec/solver/sparsesolver.hpp"
+#include "feec/solver/systemsparsesolver.hpp"
 #include "feec/sparse/matcsr.hpp"
 #include "feec/sparse/rainbow.hpp"
 
@@ -203,12 +205,47 @@ static void inverse_operator_checks()
     CHECK((system * x - rhs).norm() < 1e-6 * rhs.norm());
 }
 
+// The call shape of tests/sullivan2D/darcyessential2D.cpp:207-224 with B = Bt =
+// the identity and C = the zero matrix: the Schur complement is A^-1 (A the
+// Laplacian plus the identity), so x = A b.
+static void hodge_checks()
+{
+    MatrixCSR lap = laplacian(12);
+    const int n = lap.getdimin();
+    const Float h2 = Float(1.) / (13 * 13);
+    std::vector<int> Ar(1, 0), Ac, Ir(1, 0), Ic, Zr(n + 1, 0);
+    std::vector<Float> Av, Iv;
+    for (int r = 0; r < n; r++) {
+        for (int k = lap.getA()[r]; k < lap.getA()[r + 1]; k++) {
+            Ac.push_back(lap.getC()[k]);
+            Av.push_back(lap.getV()[k] * h2 + (lap.getC()[k] == r ? 1. : 0.));
+        }
+        Ar.push_back((int)Ac.size());
+        Ic.push_back(r);
+        Iv.push_back(1.);
+        Ir.push_back(r + 1);
+    }
+    const MatrixCSR A(n, n, Ar, Ac, Av), I(n, n, Ir, Ic, Iv);
+    const MatrixCSR Z(n, n, Zr, std::vector<int>(), std::vector<Float>());
+    FloatVector b(n), x(n, 0.), res(n, 0.);
+    b.random();
+    const int K = HodgeConjugateResidualSolverCSR_SSOR(
+        n, n, x.raw(), b.raw(), A.getA(), A.getC(), A.getV(), I.getA(), I.getC(), I.getV(),
+        I.getA(), I.getC(), I.getV(), Z.getA(), Z.getC(), Z.getV(), res.raw(), desired_precision,
+        -1, desired_precision, -3);
+    const FloatVector want = A * b;
+    CHECK((x - want).norm() < 1e-6 * want.norm());
+    std::printf("HodgeConjugateResidualSolverCSR_SSOR: K=%d, |x - A b| / |A b| = %.2e\n", K,
+                (x - want).norm() / want.norm());
+}
+
 int main(int argc, char** argv)
 {
     const bool gpu = argc > 1 && std::strcmp(argv[1], "gpu") == 0;
     cpu_checks();
     if (gpu) gpu_checks();
     if (gpu) inverse_operator_checks();
+    if (gpu) hodge_checks();
     std::printf("%s checks: %d failure(s)\n", gpu ? "cpu+gpu" : "cpu", failures);
     return failures == 0 ? 0 : 1;
 }

@@ -14,6 +14,9 @@ SHA-256 digests of F/B/R, the colour count and the iteration counts are kept.
 The reference's refined unit square / unit cube and its P1 stiffness
 matrices are kept in full for small levels (mesh_*.npz) and as SHA-256
 digests of the CSR structure at the full-size levels (2D level 7, 3D level 6).
+Block-system solves of HodgeConjugateResidualSolverCSR_SSOR keep K, x and
+residual in full (hodge_*.npz); `--hodge-only` adds just those to an existing
+golden.json.
 """
 import hashlib
 import json
@@ -89,7 +92,41 @@ def mesh_fixtures(ref, meta):
         print("mesh", name, meta["mesh"][name]["nnz"], flush=True)
 
 
+# HodgeConjugateResidualSolverCSR_SSOR cases: schur_blocks(m, coarse, c_weight, seed)
+HODGE = {
+    "schur_m16_w0": (16, 2, 0.0, 3),
+    "schur_m16_w0.5": (16, 2, 0.5, 3),
+    "schur_m32_w0": (32, 2, 0.0, 3),
+    "schur_m40_w1": (40, 2, 1.0, 3),
+}
+HODGE_RHS_SEED = 5
+
+
+def hodge_fixtures(ref, meta):
+    """Block-system solves (HodgeConjugateResidualSolverCSR_SSOR) at
+    precision 1e-10, inner precision 1e-12: K, x and residual in full."""
+    meta["hodge"] = {}
+    for name, (m, coarse, cw, seed) in HODGE.items():
+        bs = S.schur_blocks(m, coarse, cw, seed)
+        b = S.uniform_pm1(HODGE_RHS_SEED, bs.N)
+        x, r, K = ref.hodge_cr_ssor(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, b, precision=1e-10,
+                                    inner_precision=1e-12)
+        np.savez_compressed(os.path.join(OUT, "hodge_" + name + ".npz"), x=x, res=r)
+        meta["hodge"][name] = {"m": m, "coarse": coarse, "c_weight": cw, "seed": seed,
+                               "rhs_seed": HODGE_RHS_SEED, "precision": 1e-10,
+                               "inner_precision": 1e-12, "iterations": int(K)}
+        print("hodge", name, K, flush=True)
+
+
 def main():
+    if "--hodge-only" in sys.argv:  # adds the block-system fixtures to golden.json
+        path = os.path.join(OUT, "golden.json")
+        meta = json.load(open(path))
+        with RefRunner(1) as ref:
+            hodge_fixtures(ref, meta)
+        with open(path, "w") as f:
+            json.dump(meta, f, indent=1, sort_keys=True)
+        return
     with RefRunner(1) as ref:
         meta = {"desired_precision": ref.info()["desired_precision"], "small": {}, "full": {}}
         mesh_fixtures(ref, meta)
@@ -132,6 +169,7 @@ def main():
                     entry.setdefault("x_norm", {})[f"{v}_w{omega}"] = float(np.linalg.norm(x))
                     print(name, v, omega, K, f"{t[0]:.2f}s", flush=True)
             meta["full"][name] = entry
+        hodge_fixtures(ref, meta)
     with open(os.path.join(OUT, "golden.json"), "w") as f:
         json.dump(meta, f, indent=1, sort_keys=True)
 

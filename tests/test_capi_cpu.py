@@ -76,6 +76,20 @@ def test_invalid_csr_is_rejected_before_the_device():
     assert "column" in str(e.value)
 
 
+def test_invalid_blocks_are_rejected_before_the_device():
+    """sfa_HodgeConjugateResidualSolverCSR_SSOR checks all four blocks on the
+    host: B's columns must lie in [0, L), Bt's in [0, N)."""
+    bs = S.schur_blocks(8, 2, 0.0, 1)
+    bad_B = (bs.B[0], bs.B[1].copy(), bs.B[2])
+    bad_B[1][3] = bs.L + 5
+    for B, Bt, word in ((bad_B, bs.Bt, "B"), (bs.B, (bs.Bt[0], bs.Bt[1] + bs.N, bs.Bt[2]), "Bt")):
+        with pytest.raises(capi.SfaError) as e:
+            capi.HodgeConjugateResidualSolverCSR_SSOR(bs.N, bs.L, np.zeros(bs.N), np.ones(bs.N),
+                                                      bs.A, B, Bt, bs.C, np.zeros(bs.N), 1e-10,
+                                                      -3, 1e-10, -3)
+        assert e.value.code == -1 and f": {word}: column" in str(e.value), str(e.value)
+
+
 def test_inconsistent_colouring_is_rejected_before_the_device():
     s = S.poisson2d_p1(3)
     nc, F, B, R = oracle.rainbow(s.n, s.A, s.C)

@@ -1,0 +1,81 @@
+"""Pins the oracle's HodgeConjugateResidualSolverCSR_SSOR
+(solver/systemsparsesolver.cpp:426-797) before it checks the GPU path.
+
+  * golden fixtures from the compiled reference (K, x and residual bit for bit);
+  * mathematics: the dense Schur complement solve, and with B = Bt = I and
+    C = 0 the closed form x = A b (the Schur complement is A^-1);
+  * the compiled reference itself, when oracle/_ref exists.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from simplefem_amd import systems as S
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+META = json.load(open(os.path.join(GOLDEN, "golden.json")))["hodge"]
+
+
+def _case(name):
+    e = META[name]
+    bs = S.schur_blocks(e["m"], e["coarse"], e["c_weight"], e["seed"])
+    b = S.uniform_pm1(e["rhs_seed"], bs.N)
+    return e, bs, b
+
+
+@pytest.mark.parametrize("name", sorted(META))
+def test_oracle_matches_reference_fixtures(name):
+    e, bs, b = _case(name)
+    x, res, info = oracle.hodge_cr_ssor(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, b,
+                                        precision=e["precision"],
+                                        inner_precision=e["inner_precision"])
+    g = np.load(os.path.join(GOLDEN, f"hodge_{name}.npz"))
+    assert info["iterations"] == e["iterations"]
+    assert np.array_equal(x, g["x"]) and np.array_equal(res, g["res"])
+
+
+@pytest.mark.parametrize("name", sorted(META))
+def test_oracle_solves_the_schur_complement(name):
+    e, bs, b = _case(name)
+    x, res, info = oracle.hodge_cr_ssor(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, b,
+                                        precision=e["precision"],
+                                        inner_precision=e["inner_precision"])
+    A, B, Bt, C = bs.dense()
+    schur = B @ np.linalg.solve(A, Bt) + C
+    xd = np.linalg.solve(schur, b)
+    assert np.linalg.norm(schur @ x - b) <= 2e-6 * np.linalg.norm(b), info
+    assert np.abs(x - xd).max() <= 2e-6 * np.abs(xd).max(), info
+    # res is b - S x as the iteration tracked it
+    assert np.linalg.norm(res - (b - schur @ x)) <= 1e-6 * np.linalg.norm(b)
+
+
+def test_oracle_identity_coupling_gives_a_times_b():
+    """B = Bt = I, C = 0: the Schur complement is A^-1, so x = A b."""
+    bs = S.schur_blocks(12, 2, 0.0, 1)
+    L = bs.L
+    eye = (np.arange(L + 1, dtype=np.int32), np.arange(L, dtype=np.int32), np.ones(L))
+    empty = (np.zeros(L + 1, dtype=np.int32), np.zeros(0, dtype=np.int32), np.zeros(0))
+    b = S.uniform_pm1(2, L)
+    x, _, info = oracle.hodge_cr_ssor(L, L, bs.A, eye, eye, empty, b, precision=1e-12,
+                                      inner_precision=1e-14)
+    want = oracle.csr_apply(L, bs.A[0], bs.A[1], bs.A[2], b)
+    # each inner solve stops at the reference's absolute floor sqrt(z.r) < 2^-26,
+    # which bounds the accuracy of every A^-1 application at ~1e-8
+    assert np.abs(x - want).max() <= 1e-6 * np.abs(want).max(), info
+
+
+@pytest.mark.skipif(not oracle.ref_available(), reason="oracle/_ref not built")
+def test_oracle_bitexact_with_compiled_reference():
+    from oracle.ref import RefRunner
+
+    with RefRunner(1) as ref:
+        for name in sorted(META):
+            e, bs, b = _case(name)
+            x, res, info = oracle.hodge_cr_ssor(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, b,
+                                                precision=1e-9, inner_precision=1e-11)
+            xr, rr, K = ref.hodge_cr_ssor(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, b, precision=1e-9,
+                                          inner_precision=1e-11)
+            assert K == info["iterations"] and np.array_equal(x, xr) and np.array_equal(res, rr)
