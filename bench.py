@@ -74,6 +74,55 @@ def cpu_baseline(system, b, d, colouring, precision, omega, budget_s=25.0):
             "sample": f"1 full solve (K={info['iterations']}), {dt:.3f} s"}
 
 
+def forms_cpu_baseline(args, budget_s=25.0):
+    """The reference's FP64 ConjugateGradientSolverCSR_Rainbow (it has no
+    mixed precision, batching or serving) on the host cores, over a bounded
+    sample of the form's own systems and right-hand sides, taken in workload
+    order until budget_s of solving has passed."""
+    import oracle
+    from oracle.ref import RefRunner
+    from simplefem_amd import capi
+    from simplefem_amd import systems as S
+    from simplefem_amd import workloads as W
+
+    if not oracle.ref_available():
+        return None
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    cores = max(1, min(cores, 16))
+    if args.config == "hcurl-mixed":
+        s = S.hcurl3d_whitney_kuhn(48)
+        sample = iter([(s, S.rhs_uniform(s, 2026))])
+        what = "the same H(curl) system and right-hand side"
+    elif args.config == "batched":
+        sample = ((j.system, j.rhs[0]) for j in W.batched_systems(args.systems, 4))
+        what = f"the first systems of the {args.systems}-system workload, first right-hand side each"
+    else:
+        sample = ((j.system, j.rhs[0]) for j in
+                  W.served_requests(args.requests, args.rate, dofs=(1e3, 2e6)))
+        what = "the first requests of the served stream, in arrival order"
+    spent, count, rows, iters = 0.0, 0, [], []
+    with RefRunner(cores) as ref:
+        for s, b in sample:
+            d = capi.get_diagonal(s.n, s.A, s.C, s.V)
+            rb = capi.Rainbow(s.n, s.A, s.C)
+            _, _, K, times = ref.solve("rainbow", s.A, s.C, s.V, b, aux=d, omega=1.0,
+                                       colouring=(rb.num_colors, rb.F, rb.B, rb.R),
+                                       precision=1e-10)
+            spent += times[0]
+            count += 1
+            rows.append(s.n)
+            iters.append(K)
+            log(f"cpu baseline: {s.name} K={K} {times[0]:.2f}s")
+            if spent >= budget_s:
+                break
+    return {"value": count / spent, "unit": "solves/s", "cores": cores, "kind": "reference",
+            "sample": f"{count} FP64 solves at rtol 1e-10 of {what} (rows {rows}, K {iters}), "
+                      f"{spent:.1f} s of solver time, OMP_NUM_THREADS={cores}"}
+
+
 def copy_calibration(mib):
     """A plain copy of known size: its counters read (bytes read, bytes written)."""
     from simplefem_amd.hipmem import DeviceArray, synchronize
@@ -141,6 +190,8 @@ def main():
     if args.config != "poisson3d":
         from simplefem_amd import forms_bench
         out = forms_bench.run(args)
+        if not args.no_cpu_baseline:
+            out["cpu_baseline"] = forms_cpu_baseline(args)
         print(json.dumps(out), flush=True)
         return 0
 
