@@ -369,7 +369,7 @@ __device__ inline void sell_row_multi(const SellViewT<TV>& A, int slice, int lan
 // rows 64j .. 64j+63.  Matrix entries are loaded once for all riders; each
 // rider's rows get exactly the arithmetic they get alone.  part[k]: the
 // lane's share of rider k's reduction.
-template <typename T>
+template <typename T, bool WIDE>
 __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind, int g, int j,
                          double* part)
 {
@@ -398,7 +398,7 @@ __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind,
         break;
     }
     case PH_SPMV:
-        if (M.spmv_lanes == 1) {
+        if (!WIDE || M.spmv_lanes == 1) {
             const int local = j;
             if (local < M.full_slices) {
                 const int row = group_row(M.full, local, local, lane, M.full_row0, M.n_active);
@@ -420,7 +420,7 @@ __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind,
                     }
                 }
             }
-        } else {
+        } else if constexpr (WIDE) {
             for (int k = 0; k < nr; k++) part[k] = spmv_task_any(M, SS[k].d, SS[k].q, j, lane);
         }
         break;
@@ -592,7 +592,11 @@ __device__ inline int unit_owner(const int* start, int ns, int u)
     return lo;
 }
 
-template <typename T>
+// WIDE: the SpMV phase may run a matrix with 2, 4 or 8 lanes per row.  Those
+// code paths need about twice the registers of everything else (117 vs 66
+// VGPRs in FP64: 4 vs 7 wavefronts per SIMD), so the engine runs the variant
+// without them whenever every bucket uses one lane per row.
+template <typename T, bool WIDE>
 __global__ void __launch_bounds__(kUnitBlock) k_batch_step(BArgs<T> a)
 {
     __shared__ int start[kMaxSlots + 1];  // the step's unit offsets, read once per block
@@ -618,7 +622,7 @@ __global__ void __launch_bounds__(kUnitBlock) k_batch_step(BArgs<T> a)
         const int nr = riders[lo];
         const int j = u - start[lo];
         double part[kRiders];
-        run_unit(SS, nr, a.mats[SS[0].mat], SS[0].kind, SS[0].group, j, part);
+        run_unit<T, WIDE>(SS, nr, a.mats[SS[0].mat], SS[0].kind, SS[0].group, j, part);
 #pragma unroll
         for (int k = 0; k < kRiders; k++) {
             if (k < nr) {
@@ -868,6 +872,7 @@ struct Engine {
     hipGraphExec_t graph = nullptr;
     std::unique_ptr<GraphRecorder> graph_rec;  // the recorded graph and its arguments
     BMat<T>* graph_mats = nullptr;            // d_mats the graph was captured with
+    bool graph_wide = false;                  // the graph's step kernel variant
     int spmv_table[kSpmvBuckets] = {1, 1, 1, 1};  // lanes per row of the SpMV per bucket
 
     ~Engine() { release(); }
@@ -1248,18 +1253,35 @@ struct Engine {
         step = h_sched->step;
     }
 
+    // Whether some bucket runs its SpMV with more than one lane per row (the
+    // step kernel then needs those code paths).
+    bool wide() const
+    {
+        for (int b = 0; b < kSpmvBuckets; b++)
+            if (spmv_table[b] != 1) return true;
+        return false;
+    }
+
+    void launch_step(const BArgs<T>& a, bool w)
+    {
+        if (w) launch(k_batch_step<T, true>, dim3(kStepBlocks), dim3(kUnitBlock), stream, a);
+        else launch(k_batch_step<T, false>, dim3(kStepBlocks), dim3(kUnitBlock), stream, a);
+    }
+
     // Records kGraphSteps steps (k_batch_step, k_batch_fold, k_batch_prefix)
-    // into a graph, node by node; rebuilt when the device matrix table has moved.
+    // into a graph, node by node; rebuilt when the device matrix table has
+    // moved or the step kernel variant changed.
     void build_graph()
     {
-        if (graph && graph_mats == d_mats) return;
+        const bool w = wide();
+        if (graph && graph_mats == d_mats && graph_wide == w) return;
         if (graph) SFA_HIP(hipGraphExecDestroy(graph));
         graph = nullptr;
         graph_rec.reset();
         const BArgs<T> a = args();
         auto rec = std::make_unique<GraphRecorder>();
         for (int i = 0; i < kGraphSteps; i++) {
-            launch(k_batch_step<T>, dim3(kStepBlocks), dim3(kUnitBlock), stream, a);
+            launch_step(a, w);
             launch(k_batch_fold<T>, dim3(max_slots), dim3(kUnitBlock), stream, a);
             launch(k_batch_prefix<T>, dim3(1), dim3(kPrefixBlock), stream, a);
         }
@@ -1270,6 +1292,7 @@ struct Engine {
             throw Error{SFA_ERR_HIP, std::string("batch step graph: ") + hipGetErrorString(e)};
         graph = exec;
         graph_mats = d_mats;
+        graph_wide = w;
     }
 
     // Runs about max_steps steps (whole graph replays, at least one; 0: until
@@ -1310,7 +1333,7 @@ struct Engine {
             for (auto& e : ev) SFA_HIP(hipEventCreate(&e));
             for (int i = 0; i < steps; i++) {
                 SFA_HIP(hipEventRecord(ev[4 * i], stream));
-                launch(k_batch_step<T>, dim3(kStepBlocks), dim3(kUnitBlock), stream, a);
+                launch_step(a, wide());
                 SFA_HIP(hipEventRecord(ev[4 * i + 1], stream));
                 launch(k_batch_fold<T>, dim3(max_slots), dim3(kUnitBlock), stream, a);
                 SFA_HIP(hipEventRecord(ev[4 * i + 2], stream));
