@@ -243,6 +243,8 @@ void build_full_identity(SellPart& part, int rows, int columns, const int* A, co
 
 namespace {
 
+constexpr size_t kSortWindow = 512;  // rows per length-sorting window (8 slices)
+
 // Breadth-first (Cuthill-McKee) order of the graph of the matrix: from row 0,
 // then from the lowest unvisited row; neighbours in column order.
 std::vector<int> bfs_order(int n, const int* A, const int* C)
@@ -403,11 +405,24 @@ std::string build_plan(Plan& plan, int method, int n, const int* A, const int* C
     // neighbours lie in a few short runs of each colour's rows (device order
     // groups rows by colour), and a GPU die that runs a contiguous range of
     // slices gathers them from its own cache instead of from all of each vector.
+    //
+    // Inside windows of kSortWindow rank-ordered rows, its rows are then
+    // ordered by decreasing length (stable), so that a 64-row slice holds rows
+    // of similar length and is padded little (SELL-C-sigma): for degree-2/3
+    // Lagrange and Whitney matrices, whose vertex, edge and interior rows
+    // differ in length, this cuts the stored entries of this part from
+    // 1.16-1.73x to 1.00-1.07x the real ones; rows of equal length keep their
+    // rank order.  (The colour groups are left in rank order: their parts are
+    // padded little anyway, about 1.01x in 2D.)
     std::vector<int> active_rows;
     active_rows.reserve(plan.n_active);
     for (int r = 0; r < n; r++)
         if (active[r]) active_rows.push_back(r);
     by_rank(active_rows);
+    for (size_t w = 0; w < active_rows.size(); w += kSortWindow)
+        std::stable_sort(active_rows.begin() + w,
+                         active_rows.begin() + std::min(active_rows.size(), w + kSortWindow),
+                         [&](int a, int b) { return A[a + 1] - A[a] > A[b + 1] - A[b]; });
     const std::vector<std::vector<int>> active_dev(1, to_dev(active_rows));
     if (eisen)  // solver/sparsesolver.cpp:806-807,1420-1421 skip columns with diagonal 0
         build_part(plan.full, active_dev, plan.perm, iperm, n, A, C, V,
