@@ -166,6 +166,9 @@ def main():
     ap.add_argument("--inner-rtol", type=float, default=1e-1,
                     help="mixed precision: reduction of the preconditioned residual per FP32 "
                          "inner solve")
+    ap.add_argument("--engine-profile", action="store_true",
+                    help="batched: only time the FP64 engine's kernels step by step (HIP events, "
+                         "chunks of 2000 steps) and print that table")
     ap.add_argument("--inner-restarts", action="store_true",
                     help="mixed forms: restart every inner CG (plain iterative refinement) "
                          "instead of continuing the previous inner solve's direction")
@@ -190,6 +193,9 @@ def main():
     if args.config != "poisson3d":
         from simplefem_amd import forms_bench
         out = forms_bench.run(args)
+        if "engine_profile_fp64" in out:
+            print(json.dumps(out), flush=True)
+            return 0
         if not args.no_cpu_baseline:
             out["cpu_baseline"] = forms_cpu_baseline(args)
         print(json.dumps(out), flush=True)

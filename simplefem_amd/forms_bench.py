@@ -113,6 +113,35 @@ def hcurl_mixed(args):
     return _base(args, round(1.0 / tmx, 3), "solves/s", round(tmx * 1e3, 3), cfg, "f32+f64")
 
 
+def engine_profile(jobs, cols, mixed, chunk=2000):
+    """The batched workload on one engine with every kernel of every step
+    launched between HIP events (sfa_batch_profile), in chunks of `chunk`
+    steps: per chunk the solves still running and the mean step, fold and
+    schedule kernel times."""
+    rows = []
+    with capi.Batch(mixed=mixed, max_slots=min(1024, sum(len(j.rhs) for j in jobs))) as eng:
+        mids = [eng.add_matrix(j.system.n, j.system.A, j.system.C, j.system.V, j.system.diagonal(),
+                               1.0, rb.num_colors, rb.F, rb.B, rb.R) for j, rb in zip(jobs, cols)]
+        nslots = 0
+        for j, mid in zip(jobs, mids):
+            for b in j.rhs:
+                eng.submit(mid, b, precision=1e-10)
+                nslots += 1
+        total = 0
+        while True:
+            p = eng.profile(chunk)
+            total += chunk
+            running = nslots - len(eng.finished(2048))
+            rows.append({"steps": total, "running": running,
+                         "step_us": round(p["step_ms"] / chunk * 1e3, 1),
+                         "fold_us": round(p["fold_ms"] / chunk * 1e3, 1),
+                         "schedule_us": round(p["prefix_ms"] / chunk * 1e3, 1)})
+            log(f"engine profile: {rows[-1]}")
+            if running == 0 or total > 400000:
+                break
+    return rows
+
+
 def batched(args):
     t0 = time.time()
     jobs = W.batched_systems(args.systems, 4)
@@ -171,6 +200,8 @@ def batched(args):
                     worst = max(worst, float(np.linalg.norm(res) / np.linalg.norm(b)))
             return elapsed, t_sub, iters, ok, eng.memory(), tuning, worst, steps
 
+    if args.engine_profile:
+        return {"engine_profile_fp64": engine_profile(jobs, cols, False)}
     e_mx, sub_mx, it_mx, ok_mx, mem, tune_mx, res_mx, st_mx = run_engine(True)
     log(f"batched mixed: {nrhs} solves in {e_mx:.3f}s ({ok_mx} converged, {st_mx} steps)")
     e_64, sub_64, it_64, ok_64, _, tune_64, res_64, st_64 = run_engine(False)
