@@ -302,6 +302,17 @@ int sfa_system_tune_spmv(sfa_system sys, int fp32_inner, sfa_spmv_tuning* out);
  * system and its FP32 inner system. */
 int sfa_system_set_spmv(sfa_system sys, int lanes_per_row);
 
+/* Microseconds per launch of the iteration's forward (SFA_KERNEL_FORWARD) or
+ * backward (SFA_KERNEL_BACKWARD) sweep kernels as they run back to back in a
+ * recorded graph: the last right-hand side's solve is restarted from x = 0
+ * and run one iteration, then reps copies of one iteration's launches of
+ * that class are replayed between two HIP events.  SSOR and Rainbow systems,
+ * after at least one solve.  No reference counterpart: the launch time a
+ * roofline is priced with, free of the per-launch events of
+ * sfa_system_profile.  The next solve restarts, so nothing of it remains. */
+int sfa_system_time_sweeps(sfa_system sys, int kernel_class, int reps,
+                           double* us_per_launch);
+
 int sfa_system_destroy(sfa_system sys);
 
 /* ---------------------------------------------------------------------------

@@ -160,6 +160,7 @@ _SIGNATURES = {
     "sfa_system_profile": ([c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int,
                             POINTER(KernelProfile), POINTER(SolveInfo)], c_int),
     "sfa_system_tune_spmv": ([c_void_p, c_int, POINTER(SpmvTuning)], c_int),
+    "sfa_system_time_sweeps": ([c_void_p, c_int, c_int, POINTER(c_double)], c_int),
     "sfa_system_set_spmv": ([c_void_p, c_int], c_int),
     "sfa_system_destroy": ([c_void_p], c_int),
     "sfa_batch_create": ([c_int, c_int, POINTER(c_void_p)], c_int),
@@ -421,6 +422,15 @@ class System:
         _check(load().sfa_system_solve_mixed(self._h, px, pb, float(rtol), float(inner_rtol),
                                              int(max_outer), dx, ctypes.byref(out)))
         return out.as_dict()
+
+    def time_sweeps(self, kernel_class, reps=200):
+        """Microseconds per launch of the forward (KERNEL_CLASS "forward") or
+        backward sweep kernels, back to back in a recorded graph
+        (sfa_system_time_sweeps); after a solve."""
+        klass = KERNEL_NAMES.index(kernel_class) if isinstance(kernel_class, str) else kernel_class
+        us = c_double(0.0)
+        _check(load().sfa_system_time_sweeps(self._h, int(klass), int(reps), ctypes.byref(us)))
+        return us.value
 
     def profile(self, x, b, residual, precision):
         """One solve with every kernel timed by HIP events; per-class totals."""

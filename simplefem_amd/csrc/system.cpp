@@ -760,6 +760,60 @@ struct SystemT {
         }
     }
 
+    // Microseconds per launch of the iteration's forward (or backward) sweep
+    // kernels as they run back to back inside a recorded graph: the solve of
+    // the last right-hand side is restarted from x = 0 and run one iteration,
+    // then `reps` copies of one iteration's launches of that class are
+    // recorded and the graph is replayed between two HIP events (after one
+    // replay to warm up).  The kernels repeat their own arithmetic on
+    // evolving values; the next solve restarts, so nothing of this remains.
+    double time_sweeps(int klass, int reps)
+    {
+        if (method != SFA_METHOD_SSOR && method != SFA_METHOD_RAINBOW)
+            throw Error{SFA_ERR_INVALID, "sweep timing needs an SSOR or Rainbow system"};
+        if (klass != SFA_KERNEL_FORWARD && klass != SFA_KERNEL_BACKWARD)
+            throw Error{SFA_ERR_INVALID, "kernel class must be SFA_KERNEL_FORWARD or _BACKWARD"};
+        if (reps < 1 || reps > 100000) throw Error{SFA_ERR_INVALID, "reps must be 1..100000"};
+        SFA_HIP(hipSetDevice(device));
+        hipStream_t s = stream;
+        launch_fill(s, x, T(0), n);
+        start(s, 1e-10, -3, 0., 0, 0.);
+        if (host_state->done) throw Error{SFA_ERR_INVALID, "the restart already converged"};
+        iteration(s, 0);
+        const size_t per = klass == SFA_KERNEL_FORWARD ? fwd_sched.size() : bwd_sched.size();
+        hipGraphExec_t exec = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        std::unique_ptr<GraphRecorder> rec;
+        auto cleanup = [&] {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+        };
+        try {
+            rec = std::make_unique<GraphRecorder>();
+            for (int i = 0; i < reps; i++) {  // the launches of the iteration of parity 1
+                if (klass == SFA_KERNEL_FORWARD) forward_iteration(s, 1, d);
+                else backward_iteration(s, d);
+            }
+            const hipError_t e = rec->instantiate(&exec);
+            if (e != hipSuccess) throw Error{SFA_ERR_HIP, std::string("sweep graph: ") + hipGetErrorString(e)};
+            SFA_HIP(hipEventCreate(&e0));
+            SFA_HIP(hipEventCreate(&e1));
+            SFA_HIP(hipGraphLaunch(exec, s));
+            SFA_HIP(hipEventRecord(e0, s));
+            SFA_HIP(hipGraphLaunch(exec, s));
+            SFA_HIP(hipEventRecord(e1, s));
+            SFA_HIP(hipEventSynchronize(e1));
+            float ms = 0.f;
+            SFA_HIP(hipEventElapsedTime(&ms, e0, e1));
+            cleanup();
+            return ms * 1e3 / ((double)reps * (double)per);
+        } catch (...) {
+            cleanup();
+            throw;
+        }
+    }
+
     int predict_chunks(int G)
     {
         // Geometric model of the last few z.r values: remaining iterations
@@ -1309,6 +1363,14 @@ int sfa_system_profile(sfa_system sys, double* x, const double* b, double* resid
     return sfa::guarded([&] {
         if (!sys || !x || !b || !residual || !out) throw Error{SFA_ERR_INVALID, "null argument"};
         sys->sys.profile(x, b, residual, precision, device_pointers, out, info);
+    });
+}
+
+int sfa_system_time_sweeps(sfa_system sys, int kernel_class, int reps, double* us_per_launch)
+{
+    return sfa::guarded([&] {
+        if (!sys || !us_per_launch) throw Error{SFA_ERR_INVALID, "null argument"};
+        *us_per_launch = sys->sys.time_sweeps(kernel_class, reps);
     });
 }
 

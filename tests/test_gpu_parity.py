@@ -407,3 +407,32 @@ def test_paired_sweeps_are_bit_identical(maker):
     xo, _, io = oracle.solve("rainbow", system.A, system.C, system.V, b, aux=d, colouring=col)
     assert abs(kp - io["iterations"]) <= max(1, int(0.02 * io["iterations"]))
     _close(xp, xo, 1e-10)
+
+
+def test_sweep_timing_leaves_later_solves_intact():
+    """sfa_system_time_sweeps times the forward and backward sweep launches in
+    a recorded graph (a few microseconds each); the next solve restarts and
+    still matches the oracle; plain CG systems have no sweeps to time."""
+    s = S.poisson3d_p1_kuhn(12)
+    b = S.rhs_uniform(s, 4)
+    d = s.diagonal()
+    col = oracle.rainbow(s.n, s.A, s.C)
+    with capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, d, 1.0, *col) as sysd:
+        x = np.zeros(s.n)
+        sysd.solve(x, b, np.zeros(s.n), 1e-10)
+        for klass in ("forward", "backward"):
+            us = sysd.time_sweeps(klass, 50)
+            print(f"[sweep timing] {klass} {us:.2f} us per launch", flush=True)
+            assert 0.5 < us < 200.0, (klass, us)
+        x = np.zeros(s.n)
+        info = {}
+        sysd.solve(x, b, np.zeros(s.n), 1e-10, info=info)
+        _check_against_oracle = oracle.solve("rainbow", s.A, s.C, s.V, b, aux=d, colouring=col)
+        xo, _, io = _check_against_oracle
+        assert abs(info["iterations"] - io["iterations"]) <= 1, (info, io)
+        assert np.abs(x - xo).max() <= 1e-10 * np.abs(xo).max()
+    sr = S.random_spd(100, 0.1, 2)
+    with capi.System(capi.METHOD_CG, sr.n, sr.A, sr.C, sr.V) as plain:
+        plain.solve(np.zeros(sr.n), S.uniform_pm1(1, sr.n), np.zeros(sr.n), 1e-10)
+        with pytest.raises(capi.SfaError):
+            plain.time_sweeps("forward", 10)
