@@ -10,9 +10,11 @@ seeded U[-1,1) right-hand side on the active rows.
 
 A step is one complete solve with the matrix, colouring and right-hand side
 already resident in HBM.  Prints one JSON line with solves/s, the roofline of
-the dominant kernel (per-launch HIP events in a profiled solve of the same
-system) and the CPU baseline (the reference's own solver, compiled from its
-sources, timed on this host on a bounded sample).
+the dominant kernel class (chosen by per-launch HIP events in a profiled solve
+of the same system, its launch time measured by replaying 200 iterations'
+launches of that class in a recorded graph between two HIP events) and the CPU
+baseline (the reference's own solver, compiled from its sources, timed on this
+host on a bounded sample).
 """
 import argparse
 import json
