@@ -90,3 +90,26 @@ def test_hodge_identity_coupling_gives_a_times_b():
     # each inner solve stops at the reference's absolute floor sqrt(z.r) < 2^-26,
     # which bounds the accuracy of every A^-1 application at ~1e-8
     assert np.abs(x - want).max() <= 1e-6 * np.abs(want).max()
+
+
+def test_hodge_edge_cases_match_oracle():
+    """b = 0 (the restart meets the tolerance floor: K = 0, x unchanged), a
+    one-unknown Schur complement, and a nonzero initial guess: K, exit and x
+    as the oracle's."""
+    bs = S.schur_blocks(8, 2, 0.5, 4)
+    cases = [("zero rhs", bs, np.zeros(bs.N), np.zeros(bs.N)),
+             ("x0", bs, S.uniform_pm1(3, bs.N), S.uniform_pm1(4, bs.N) * 0.1)]
+    one = S.schur_blocks(2, 2, 0.0, 1)  # N = 1, L = 4
+    cases.append(("N=1", one, np.array([0.7]), np.zeros(1)))
+    for name, blk, b, x0 in cases:
+        xo, ro, io = oracle.hodge_cr_ssor(blk.N, blk.L, blk.A, blk.B, blk.Bt, blk.C, b, x0=x0,
+                                          precision=1e-10, inner_precision=1e-12)
+        x = x0.copy()
+        info = {}
+        K = capi.HodgeConjugateResidualSolverCSR_SSOR(blk.N, blk.L, x, b, blk.A, blk.B, blk.Bt,
+                                                      blk.C, np.zeros(blk.N), 1e-10, -3, 1e-12,
+                                                      -3, info=info)
+        print(f"[hodge edge] {name}: K gpu {K} oracle {io['iterations']}", flush=True)
+        assert abs(K - io["iterations"]) <= 1 and info["exit_reason"] == io["exit_reason"], name
+        assert np.abs(x - xo).max() <= 1e-6 * max(np.abs(xo).max(), 1e-300) or \
+            np.array_equal(x, xo), name

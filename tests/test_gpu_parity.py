@@ -436,3 +436,27 @@ def test_sweep_timing_leaves_later_solves_intact():
         plain.solve(np.zeros(sr.n), S.uniform_pm1(1, sr.n), np.zeros(sr.n), 1e-10)
         with pytest.raises(capi.SfaError):
             plain.time_sweeps("forward", 10)
+
+
+def test_mixed_refinement_edge_cases():
+    """max_outer = 0 returns x0 untouched with its residual; b = 0 needs no
+    correction; a nonzero x0 close to the solution needs fewer corrections."""
+    s = S.poisson3d_p1_kuhn(10)
+    d = s.diagonal()
+    col = oracle.rainbow(s.n, s.A, s.C)
+    b = S.rhs_uniform(s, 12)
+    with capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, d, 1.0, *col,
+                     fp32_inner=True) as sysm:
+        x = np.zeros(s.n)
+        info = sysm.solve_mixed(x, b, 1e-10, max_outer=0)
+        assert info["outer"] == 0 and info["inner_iterations"] == 0 and not x.any()
+        assert info["success"] == 0
+        x = np.zeros(s.n)
+        info = sysm.solve_mixed(x, np.zeros(s.n), 1e-10)
+        assert info["outer"] == 0 and info["success"] == 1 and not x.any()
+        x = np.zeros(s.n)
+        full = sysm.solve_mixed(x, b, 1e-10)
+        x2 = x + 1e-7 * S.uniform_pm1(3, s.n) * np.abs(x).max()
+        x2[s.dirichlet] = 0.0
+        near = sysm.solve_mixed(x2, b, 1e-10)
+        assert near["success"] == 1 and near["outer"] < full["outer"], (near, full)
