@@ -61,6 +61,7 @@ constexpr int kUnitBlock = 256;             // threads per block of the engine's
 constexpr int kUnitRows = 64;               // rows per work unit: one wavefront, one slice
 constexpr int kStepWaves = kUnitBlock / 64; // wavefronts per block
 constexpr int kRiders = 4;                  // most adjacent slots one unit serves
+constexpr int kXcdChunk = 32;               // consecutive units one XCD takes at a time
 constexpr int kMaxSlots = 1024;
 
 enum Phase : int {
@@ -604,19 +605,31 @@ __global__ void __launch_bounds__(kUnitBlock) k_batch_step(BArgs<T> a)
     const int p = (int)(sc->step & 1u);
     const int W = sc->W[p];
     const int ns = sc->nslots;
-    if ((int)blockIdx.x * kStepWaves >= W) return;
+    // Workgroups reach the XCDs round robin (block i on XCD i % 8), each XCD
+    // with its own L2: the step's units are dealt to the XCDs in chunks of
+    // kXcdChunk consecutive units (chunk c to XCD c % 8), so neighbouring
+    // slices of one matrix, which gather the same vector lines, mostly share
+    // an L2, while heavy and light slots still spread over all XCDs.  Which
+    // wavefront runs a unit changes nothing it computes.
+    const int xcd = blockIdx.x % kXcds, kb = blockIdx.x / kXcds;
+    const int per_xcd = gridDim.x / kXcds;
+    // the v-th unit of this XCD (increasing in v)
+    auto unit_of = [&](int v) { return ((v / kXcdChunk) * kXcds + xcd) * kXcdChunk + v % kXcdChunk; };
+    if (unit_of(kb * kStepWaves) >= W) return;
     for (int i = threadIdx.x; i <= ns; i += kUnitBlock) start[i] = sc->start[p][i];
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    // Each wavefront runs its own units, grid-stride; the four wavefronts of
-    // a block take consecutive units, which mostly belong to one slot, so the
-    // slot record and matrix descriptor (wave-uniform, read by scalar loads)
-    // come from the scalar cache after the first.  No wavefront waits for
-    // another: each unit's partial is a wavefront sum.  A unit of a leading
-    // slot also serves its riders (see k_batch_prefix).
+    // Each wavefront runs its own units, strided over its XCD's range; the
+    // four wavefronts of a block take consecutive units, which mostly belong
+    // to one slot, so the slot record and matrix descriptor (wave-uniform,
+    // read by scalar loads) come from the scalar cache after the first.  No
+    // wavefront waits for another: each unit's partial is a wavefront sum.
+    // A unit of a leading slot also serves its riders (see k_batch_prefix).
     const int* riders = sc->riders[p];
-    for (int u = blockIdx.x * kStepWaves + wave; u < W; u += gridDim.x * kStepWaves) {
+    for (int v = kb * kStepWaves + wave;; v += per_xcd * kStepWaves) {
+        const int u = unit_of(v);
+        if (u >= W) break;
         const int lo = __builtin_amdgcn_readfirstlane(unit_owner(start, ns, u));
         const BSlot<T>* SS = a.slots + lo;
         const int nr = riders[lo];
