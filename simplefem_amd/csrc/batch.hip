@@ -795,10 +795,16 @@ __global__ void __launch_bounds__(kUnitBlock) k_batch_spmv_probe(const BMat<T>* 
     const int W = ustart[nm];
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    if ((int)blockIdx.x * kStepWaves >= W) return;
+    // the step kernel's unit-to-XCD chunks
+    const int xcd = blockIdx.x % kXcds, kb = blockIdx.x / kXcds;
+    const int per_xcd = gridDim.x / kXcds;
+    auto unit_of = [&](int v) { return ((v / kXcdChunk) * kXcds + xcd) * kXcdChunk + v % kXcdChunk; };
+    if (unit_of(kb * kStepWaves) >= W) return;
     for (int i = threadIdx.x; i <= nm; i += kUnitBlock) start[i] = ustart[i];
     __syncthreads();
-    for (int u = blockIdx.x * kStepWaves + wave; u < W; u += gridDim.x * kStepWaves) {
+    for (int v = kb * kStepWaves + wave;; v += per_xcd * kStepWaves) {
+        const int u = unit_of(v);
+        if (u >= W) break;
         const int lo = __builtin_amdgcn_readfirstlane(unit_owner(start, nm, u));
         const int j = u - start[lo];
         const double part = wave_sum(spmv_task<K>(mats[lo], d, q, j, lane));
