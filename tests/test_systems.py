@@ -23,7 +23,9 @@ def test_p1_3d_structured_equals_element_assembly(n):
 
 
 def test_config_sizes():
-    """SURVEY §8 counts: N = (2^L+1)^2, nnz = N + 2E (every element edge kept)."""
+    """Sizes of the two benchmark meshes: N = (2^L+1)^2 vertex rows (129^2 and
+    65^3) and nnz = N + 2E over the E mesh edges (every element edge kept as
+    an entry, zeros included); 16,129 and 250,047 interior rows."""
     s2 = S.poisson2d_p1(7)
     assert (s2.n, s2.nnz, int((~s2.dirichlet).sum())) == (16641, 115457, 16129)
     s3 = S.poisson3d_p1_kuhn(64)
