@@ -54,6 +54,10 @@ def _init_library():
     _lib.feecref_mesh.restype = c_int
     _lib.feecref_mesh_sizes.argtypes = [_P_I]
     _lib.feecref_mesh_copy.argtypes = [_P_D, _P_I, _P_I, _P_I, _P_I, _P_D, _P_I, _P_I, _P_D]
+    _lib.feecref_whitney1.argtypes = [c_int]
+    _lib.feecref_whitney1.restype = c_int
+    _lib.feecref_whitney1_sizes.argtypes = [_P_I]
+    _lib.feecref_whitney1_copy.argtypes = [_P_I, _P_I, _P_I, _P_I, _P_D, _P_I, _P_I, _P_D]
 
 
 def _i(a):
@@ -116,6 +120,21 @@ def _task(kind, kw):
             out["structural"] = (bA, bC[:nb].copy(), bV[:nb].copy())
             out["compressed"] = (aA, aC[:na].copy(), aV[:na].copy())
         return out
+    if kind == "whitney1":
+        L.feecref_whitney1(int(kw["levels"]))
+        sz = np.zeros(3, dtype=np.int32)
+        L.feecref_whitney1_sizes(_i(sz))
+        ne, nk, nm = (int(t) for t in sz)
+        edges = np.empty(2 * ne, dtype=np.int32)
+        dirichlet = np.empty(ne, dtype=np.int32)
+        kA, mA = np.empty(ne + 1, dtype=np.int32), np.empty(ne + 1, dtype=np.int32)
+        kC, kV = np.empty(max(nk, 1), dtype=np.int32), np.empty(max(nk, 1))
+        mC, mV = np.empty(max(nm, 1), dtype=np.int32), np.empty(max(nm, 1))
+        L.feecref_whitney1_copy(_i(edges), _i(dirichlet), _i(kA), _i(kC), _d(kV), _i(mA), _i(mC),
+                                _d(mV))
+        return {"edges": edges.reshape(ne, 2), "dirichlet": dirichlet.astype(bool),
+                "curlcurl": (kA, kC[:nk].copy(), kV[:nk].copy()),
+                "mass": (mA, mC[:nm].copy(), mV[:nm].copy())}
     if kind == "hodge":
         N, Ldim = int(kw["N"]), int(kw["L"])
         mats = []
@@ -226,6 +245,13 @@ class RefRunner:
         weight) and the compressed one (MatrixCSR(LagrangeStiffnessMatrix) +
         compressentries) as (A, C, V)."""
         return self._call("mesh", dim=dim, levels=levels, matrices=1 if matrices else 0)
+
+    def whitney1(self, levels):
+        """UnitCube3D refined `levels` times: {"edges" (vertex pairs, the
+        mesh's order), "dirichlet" (per edge), "curlcurl", "mass"} with the
+        Whitney 1-form matrices as (A, C, V), duplicates kept as
+        Conjugation leaves them."""
+        return self._call("whitney1", levels=levels)
 
     def csr_apply(self, A, C, V, src, scaling=1.0):
         return self._call("apply", A=A, C=C, V=V, src=src, scaling=scaling)

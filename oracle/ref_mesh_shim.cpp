@@ -7,12 +7,18 @@
 // simplices, CSR structure and values).  The shim adds nothing to the
 // algorithms: it builds the reference's unit square / unit cube, refines it,
 // sets the Dirichlet flags and assembles the P1 stiffness matrices the
-// reference's tests use.
+// reference's tests use, and the lowest-order Whitney 1-form curl-curl and
+// mass matrices as its 3D curl-curl tests form them
+// (tests/sullivan3D/curlcurl3D.cpp:241-312, with the Whitney inclusion of
+// tests/fem/condition3Ddense.cpp:220-223 in place of the Sullivan one).
 #include <cstring>
 #include <vector>
 
 #include "base/include.hpp"
 #include "dense/densematrix.hpp"
+#include "fem/global.diffmatrix.hpp"
+#include "fem/global.inclwhitney.hpp"
+#include "fem/global.massmatrix.hpp"
 #include "fem/lagrangematrices.hpp"
 #include "mesh/examples2D.hpp"
 #include "mesh/examples3D.hpp"
@@ -76,6 +82,14 @@ void collect(M& mesh, int dim, int with_matrices)
     keep_csr(a, r.aA, r.aC, r.aV);
 }
 
+struct WhitneyResult {
+    int ne = 0;
+    std::vector<int> edges;      // ne x 2, the mesh's own vertex order
+    std::vector<int> dirichlet;  // per edge
+    std::vector<int> kA, kC, mA, mC;
+    std::vector<double> kV, mV;
+} g_whitney;
+
 }  // namespace
 
 extern "C" {
@@ -121,6 +135,58 @@ void feecref_mesh_copy(double* coords, int* simplices, int* dirichlet, int* bA, 
         std::memcpy(aC, r.aC.data(), r.aC.size() * sizeof(int));
         std::memcpy(aV, r.aV.data(), r.aV.size() * sizeof(double));
     }
+}
+
+// UnitCube3D refined `levels` times: curl-curl K = Conjugation(M_2, D & I)
+// and mass M = Conjugation(M_1, I) of the Whitney 1-forms, where I is
+// FEECWhitneyInclusionMatrix(., 3, 1, 1) (Whitney forms into broken
+// P_1 Lambda^1), D = FEECBrokenDiffMatrix(., 3, 1, 1) and M_k the broken mass
+// matrices of P_1 Lambda^1 and P_0 Lambda^2.  Keeps the edges' vertices and
+// Dirichlet flags (automatic_dirichlet_flags); returns the edge count.
+int feecref_whitney1(int levels)
+{
+    MeshSimplicial3D M = UnitCube3D();
+    for (int l = 0; l < levels; l++) M.uniformrefinement();
+    M.automatic_dirichlet_flags();
+    WhitneyResult& r = g_whitney;
+    r = WhitneyResult();
+    r.ne = M.count_simplices(1);
+    r.edges.resize((size_t)r.ne * 2);
+    r.dirichlet.resize(r.ne);
+    for (int e = 0; e < r.ne; e++) {
+        r.edges[2 * e] = M.get_subsimplex(1, 0, e, 0);
+        r.edges[2 * e + 1] = M.get_subsimplex(1, 0, e, 1);
+        r.dirichlet[e] = M.get_flag(1, e) == SimplexFlag::SimplexFlagDirichlet ? 1 : 0;
+    }
+    const MatrixCSR incl(FEECWhitneyInclusionMatrix(M, 3, 1, 1));
+    const MatrixCSR diff(FEECBrokenDiffMatrix(M, 3, 1, 1));
+    const MatrixCSR mass2(FEECBrokenMassMatrix(M, 3, 2, 0));
+    const MatrixCSR mass1(FEECBrokenMassMatrix(M, 3, 1, 1));
+    keep_csr(Conjugation(mass2, diff & incl), r.kA, r.kC, r.kV);
+    keep_csr(Conjugation(mass1, incl), r.mA, r.mC, r.mV);
+    return r.ne;
+}
+
+// sizes: edges, entries of K, entries of M
+void feecref_whitney1_sizes(int* out)
+{
+    out[0] = g_whitney.ne;
+    out[1] = g_whitney.kA.empty() ? 0 : g_whitney.kA.back();
+    out[2] = g_whitney.mA.empty() ? 0 : g_whitney.mA.back();
+}
+
+void feecref_whitney1_copy(int* edges, int* dirichlet, int* kA, int* kC, double* kV, int* mA,
+                           int* mC, double* mV)
+{
+    const WhitneyResult& r = g_whitney;
+    std::memcpy(edges, r.edges.data(), r.edges.size() * sizeof(int));
+    std::memcpy(dirichlet, r.dirichlet.data(), r.dirichlet.size() * sizeof(int));
+    std::memcpy(kA, r.kA.data(), r.kA.size() * sizeof(int));
+    std::memcpy(kC, r.kC.data(), r.kC.size() * sizeof(int));
+    std::memcpy(kV, r.kV.data(), r.kV.size() * sizeof(double));
+    std::memcpy(mA, r.mA.data(), r.mA.size() * sizeof(int));
+    std::memcpy(mC, r.mC.data(), r.mC.size() * sizeof(int));
+    std::memcpy(mV, r.mV.data(), r.mV.size() * sizeof(double));
 }
 
 }  // extern "C"

@@ -16,7 +16,10 @@ matrices are kept in full for small levels (mesh_*.npz) and as SHA-256
 digests of the CSR structure at the full-size levels (2D level 7, 3D level 6).
 Block-system solves of HodgeConjugateResidualSolverCSR_SSOR keep K, x and
 residual in full (hodge_*.npz); `--hodge-only` adds just those to an existing
-golden.json.
+golden.json.  The reference's lowest-order Whitney 1-form curl-curl and mass
+matrices on the refined unit cube (FEEC inclusion, broken exterior derivative
+and broken mass matrices, tangential Dirichlet edges) are kept in full with the
+edges' vertices (whitney1_L*.npz); `--whitney-only` writes just those.
 """
 import hashlib
 import json
@@ -118,7 +121,27 @@ def hodge_fixtures(ref, meta):
         print("hodge", name, K, flush=True)
 
 
+# refinement levels of UnitCube3D whose Whitney 1-form matrices are kept
+WHITNEY_LEVELS = (1, 2)
+
+
+def whitney_fixtures(ref):
+    """The reference's Whitney 1-form curl-curl and mass matrices as its FEEC
+    assembly forms them, CSR as Conjugation leaves it (duplicates kept)."""
+    for lv in WHITNEY_LEVELS:
+        w = ref.whitney1(lv)
+        kA, kC, kV = w["curlcurl"]
+        mA, mC, mV = w["mass"]
+        np.savez_compressed(os.path.join(OUT, f"whitney1_L{lv}.npz"), edges=w["edges"],
+                            dirichlet=w["dirichlet"], kA=kA, kC=kC, kV=kV, mA=mA, mC=mC, mV=mV)
+        print("whitney1", lv, len(w["edges"]), flush=True)
+
+
 def main():
+    if "--whitney-only" in sys.argv:
+        with RefRunner(1) as ref:
+            whitney_fixtures(ref)
+        return
     if "--hodge-only" in sys.argv:  # adds the block-system fixtures to golden.json
         path = os.path.join(OUT, "golden.json")
         meta = json.load(open(path))
@@ -130,6 +153,7 @@ def main():
     with RefRunner(1) as ref:
         meta = {"desired_precision": ref.info()["desired_precision"], "small": {}, "full": {}}
         mesh_fixtures(ref, meta)
+        whitney_fixtures(ref)
         for name, make in SMALL.items():
             s = make()
             b = S.rhs_uniform(s, 11)

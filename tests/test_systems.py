@@ -216,3 +216,39 @@ def test_reference_permutation_is_a_bijection_of_the_grid():
     for dim, levels in [(2, 5), (3, 4)]:
         p = S.reference_permutation(dim, levels)
         assert np.array_equal(np.sort(p), np.arange(((1 << levels) + 1) ** dim))
+
+
+@pytest.mark.parametrize("levels", [1, 2])
+def test_whitney_matches_the_reference_feec_assembly(levels):
+    """The H(curl) generator is the reference's lowest-order Whitney 1-form
+    operator: with each reference edge mapped to the build's edge through the
+    reference vertex numbering, the Dirichlet edges are the same, no edge
+    changes orientation (both run from the lower to the higher lexicographic
+    vertex), and curl-curl and mass equal the reference's
+    Conjugation(M_2, D & I) and Conjugation(M_1, I) (whitney1_L*.npz, written by
+    the compiled reference's FEEC assembly) to a unit in the last place."""
+    g = np.load(os.path.join(GOLDEN, f"whitney1_L{levels}.npz"))
+    n = 1 << levels
+    perm = S.reference_permutation(3, levels)
+    curl = S.hcurl3d_whitney_kuhn(n, curl_weight=1.0, mass_weight=0.0)
+    mass = S.hcurl3d_whitney_kuhn(n, curl_weight=0.0, mass_weight=1.0)
+    ne = curl.n
+    assert len(g["edges"]) == ne
+    a, b = perm[g["edges"][:, 0]], perm[g["edges"][:, 1]]
+    assert np.all(a < b)
+    ours = {(int(p), int(q)): i for i, (p, q) in enumerate(curl.edges)}
+    idx = np.array([ours[(int(p), int(q))] for p, q in zip(a, b)])
+    assert np.array_equal(np.sort(idx), np.arange(ne))
+    assert np.array_equal(curl.dirichlet[idx], g["dirichlet"])
+
+    def ref_dense(A, C, V):
+        M = np.zeros((ne, ne))
+        np.add.at(M, (np.repeat(np.arange(ne), np.diff(A)), C), V)
+        return M
+
+    for s, key in ((curl, "k"), (mass, "m")):
+        R = ref_dense(g[key + "A"], g[key + "C"], g[key + "V"])
+        O = _csr(s).toarray()[np.ix_(idx, idx)]
+        assert np.abs(R - O).max() <= 4e-16 * np.abs(R).max(), key
+        # the build's pattern holds every nonzero of the reference's
+        assert not np.any((R != 0) & (O == 0))
