@@ -252,3 +252,61 @@ def test_whitney_matches_the_reference_feec_assembly(levels):
         assert np.abs(R - O).max() <= 4e-16 * np.abs(R).max(), key
         # the build's pattern holds every nonzero of the reference's
         assert not np.any((R != 0) & (O == 0))
+
+
+def _lagrange_node_weights(dim, r):
+    """(barycentric multi-index, integral of its nodal basis function over a
+    simplex / the simplex's volume) for degree-r Lagrange on the principal
+    lattice, from a Vandermonde solve over monomials integrated exactly on
+    the unit simplex (integral of x^a = prod a_i! / (|a| + dim)!)."""
+    from itertools import product
+    from math import factorial
+
+    alphas = [a for a in product(range(r + 1), repeat=dim) if sum(a) <= r]
+    nodes = np.array(alphas, dtype=float) / r
+    V = np.array([[np.prod(x ** np.array(b)) for b in alphas] for x in nodes])
+    coef = np.linalg.solve(V, np.eye(len(alphas)))  # column j: monomial coefficients of phi_j
+    mono = np.array([np.prod([factorial(t) for t in b]) / factorial(sum(b) + dim) for b in alphas])
+    w = (mono @ coef) * factorial(dim)
+    # Cartesian multi-index a of the unit simplex -> barycentric (r - |a|, a)
+    return [((r - sum(a),) + tuple(a), w[k]) for k, a in enumerate(alphas)]
+
+
+@pytest.mark.parametrize("dim,n,r", [(2, 4, 2), (2, 4, 3), (3, 3, 2), (3, 3, 3)])
+def test_lagrange_pr_is_exact_on_quadratics(dim, n, r):
+    """Degree-r Lagrange stiffness against the Galerkin identity for
+    f = sum x_k^2 (degree 2 <= r, so f is in the space): on a row whose
+    neighbours are all interior, (K f)_i = a(f, phi_i) = -(Laplacian f) *
+    integral(phi_i) = -2 dim * integral(phi_i), with integral(phi_i) summed
+    from the exact per-simplex node weights of every simplex around node i.
+    A wrong gradient, node placement or cell scaling in the element matrices
+    breaks this even where affine functions are still reproduced."""
+    from math import factorial
+
+    s = S.lagrange_pr(dim, n, r)
+    m = r * n + 1
+    h = 1.0 / n
+    grid = np.stack(np.unravel_index(np.arange(m ** dim), (m,) * dim)[::-1], axis=1)
+    f = ((grid / (r * n)) ** 2).sum(axis=1)
+    if dim == 2:
+        simplices = [[(0, 0), (1, 0), (1, 1)], [(0, 0), (0, 1), (1, 1)]]
+    else:
+        simplices = [[((bb >> 2) & 1, (bb >> 1) & 1, bb & 1) for bb in (0,) + path]
+                     for path in S._KUHN]
+    vol = h ** dim / factorial(dim)
+    weights = _lagrange_node_weights(dim, r)
+    w = np.zeros(m ** dim)
+    cells = np.stack(np.unravel_index(np.arange(n ** dim), (n,) * dim)[::-1], axis=1)
+    strides = m ** np.arange(dim)
+    for simp in simplices:
+        Vs = np.array(simp)
+        for bary, wk in weights:
+            local = (np.array(bary)[:, None] * Vs).sum(axis=0)  # lattice offset in the cell
+            np.add.at(w, ((r * cells + local) * strides).sum(axis=1), vol * wk)
+    assert abs(w.sum() - 1.0) <= 1e-12  # partition of unity: integral of 1
+    Kf = _csr(s) @ f
+    act = ~s.dirichlet
+    rows = [i for i in np.nonzero(act)[0] if not s.dirichlet[s.C[s.A[i]:s.A[i + 1]]].any()]
+    assert len(rows) > 0
+    expect = -2.0 * dim * w[rows]
+    assert np.abs(Kf[rows] - expect).max() <= 1e-11 * max(np.abs(expect).max(), h ** dim)
