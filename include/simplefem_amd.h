@@ -439,10 +439,42 @@ int sfa_HodgeConjugateResidualSolverCSR_SSOR(
     double inneriteration_precision, int inneriteration_print_modulo,
     sfa_solve_info* info);
 
-/* The same solver with the four matrices resident on the device between
- * solves (A as an sfa_system with its SSOR schedules): for callers that
- * solve the same block system for many right-hand sides.  No reference
- * counterpart.  device_pointers = 1: x, b and residual are device buffers. */
+/* HodgeConjugateResidualSolverCSR, solver/systemsparsesolver.hpp:10-24: the
+ * reference's plain variant calls the SSOR one (solver/systemsparsesolver.cpp:18-49),
+ * and so does this. */
+int sfa_HodgeConjugateResidualSolverCSR(
+    int N, int L, double* x, const double* b,
+    const int* Arows, const int* Acolumns, const double* Avalues,
+    const int* Brows, const int* Bcolumns, const double* Bvalues,
+    const int* Btrows, const int* Btcolumns, const double* Btvalues,
+    const int* Crows, const int* Ccolumns, const double* Cvalues,
+    double* residual, double precision, int print_modulo,
+    double inneriteration_precision, int inneriteration_print_modulo,
+    sfa_solve_info* info);
+
+/* HodgeConjugateResidualSolverCSR_diagonal, solver/systemsparsesolver.hpp:26-40
+ * (solver/systemsparsesolver.cpp:52-424): the same conjugate residual loop
+ * with every A^-1 an inner Jacobi CG solve (precon = 1 / the diagonal of A,
+ * not guarded against a zero diagonal, as in the reference's NDEBUG build). */
+int sfa_HodgeConjugateResidualSolverCSR_diagonal(
+    int N, int L, double* x, const double* b,
+    const int* Arows, const int* Acolumns, const double* Avalues,
+    const int* Brows, const int* Bcolumns, const double* Bvalues,
+    const int* Btrows, const int* Btcolumns, const double* Btvalues,
+    const int* Crows, const int* Ccolumns, const double* Cvalues,
+    double* residual, double precision, int print_modulo,
+    double inneriteration_precision, int inneriteration_print_modulo,
+    sfa_solve_info* info);
+
+/* The same solvers with the four matrices resident on the device between
+ * solves (A as an sfa_system with its SSOR schedules, or its Jacobi
+ * preconditioner): for callers that solve the same block system for many
+ * right-hand sides.  No reference counterpart.  sfa_hodge_create keeps the
+ * SSOR inner solver; sfa_hodge_create_inner takes SFA_HODGE_INNER_SSOR or
+ * SFA_HODGE_INNER_DIAGONAL.  device_pointers = 1: x, b and residual are
+ * device buffers. */
+#define SFA_HODGE_INNER_SSOR 0
+#define SFA_HODGE_INNER_DIAGONAL 1
 typedef struct sfa_hodge_s* sfa_hodge;
 int sfa_hodge_create(int N, int L,
                      const int* Arows, const int* Acolumns, const double* Avalues,
@@ -450,6 +482,12 @@ int sfa_hodge_create(int N, int L,
                      const int* Btrows, const int* Btcolumns, const double* Btvalues,
                      const int* Crows, const int* Ccolumns, const double* Cvalues,
                      sfa_hodge* out);
+int sfa_hodge_create_inner(int N, int L,
+                           const int* Arows, const int* Acolumns, const double* Avalues,
+                           const int* Brows, const int* Bcolumns, const double* Bvalues,
+                           const int* Btrows, const int* Btcolumns, const double* Btvalues,
+                           const int* Crows, const int* Ccolumns, const double* Cvalues,
+                           int inner, sfa_hodge* out);
 int sfa_hodge_solve(sfa_hodge h, double* x, const double* b, double* residual,
                     double precision, int print_modulo,
                     double inneriteration_precision, int inneriteration_print_modulo,

@@ -50,6 +50,8 @@ _SIG = {
         (_SOLVER + [_P_D, c_double, c_int, _P_I, _P_I, _P_I, POINTER(OrcInfo)], c_int),
     "orc_hodge_cr_ssor": ([c_int, c_int, _P_D, _P_D] + [_P_I, _P_I, _P_D] * 4 +
                           [_P_D, c_double, c_int, c_double, c_int, POINTER(OrcInfo)], c_int),
+    "orc_hodge_cr_diagonal": ([c_int, c_int, _P_D, _P_D] + [_P_I, _P_I, _P_D] * 4 +
+                              [_P_D, c_double, c_int, c_double, c_int, POINTER(OrcInfo)], c_int),
 }
 
 _lib = None
@@ -144,10 +146,14 @@ def solve(variant, A, C, V, b, x0=None, residual0=None, precision=1e-10, print_m
 
 
 def hodge_cr_ssor(N, L, A, B, Bt, C, b, x0=None, precision=1e-10, print_modulo=-3,
-                  inner_precision=1e-10, inner_print_modulo=-3):
+                  inner_precision=1e-10, inner_print_modulo=-3, inner="ssor"):
     """HodgeConjugateResidualSolverCSR_SSOR (solver/systemsparsesolver.cpp:426-797)
     on (B A^-1 Bt + C) x = b; A is L x L, B is N x L, Bt is L x N, C is N x N,
-    each a (rows, cols, vals) CSR triple.  Returns (x, residual, info dict)."""
+    each a (rows, cols, vals) CSR triple.  inner="diagonal" runs
+    HodgeConjugateResidualSolverCSR_diagonal (:52-424, inner Jacobi CG).
+    Returns (x, residual, info dict)."""
+    if inner not in ("ssor", "diagonal"):
+        raise ValueError(inner)
     mats = []
     for m in (A, B, Bt, C):
         mats += [_arr_i(m[0]), _arr_i(m[1]), _arr_d(m[2])]
@@ -156,9 +162,9 @@ def hodge_cr_ssor(N, L, A, B, Bt, C, b, x0=None, precision=1e-10, print_modulo=-
     res = np.zeros(N)
     info = OrcInfo()
     ptrs = [(_i if k % 3 < 2 else _d)(m) for k, m in enumerate(mats)]
-    lib().orc_hodge_cr_ssor(int(N), int(L), _d(x), _d(b), *ptrs, _d(res), float(precision),
-                            int(print_modulo), float(inner_precision), int(inner_print_modulo),
-                            ctypes.byref(info))
+    fn = lib().orc_hodge_cr_ssor if inner == "ssor" else lib().orc_hodge_cr_diagonal
+    fn(int(N), int(L), _d(x), _d(b), *ptrs, _d(res), float(precision), int(print_modulo),
+       float(inner_precision), int(inner_print_modulo), ctypes.byref(info))
     return x, res, info.as_dict()
 
 

@@ -618,14 +618,30 @@ int orc_cg_eisenstat_rainbow(int N, double* x, const double* b, const int* A,
  * (omega 1, the diagonal of A).  The two inner solves of the restart start
  * from 0, the iteration's from the previous inner solution.  info->zr holds
  * the final Md_r (the reference's restart flags are both false, :11-12, so
- * only K == 0 restarts). */
-int orc_hodge_cr_ssor(int N, int L, double* x, const double* b,
-                      const int* Arows, const int* Acols, const double* Avals,
-                      const int* Brows, const int* Bcols, const double* Bvals,
-                      const int* Btrows, const int* Btcols, const double* Btvals,
-                      const int* Crows, const int* Ccols, const double* Cvals,
-                      double* res, double precision, int print_modulo,
-                      double inner_precision, int inner_print_modulo, orc_info* info)
+ * only K == 0 restarts).  HodgeConjugateResidualSolverCSR_diagonal
+ * (:52-424) is the same loop with every inner solve a
+ * ConjugateGradientSolverCSR_DiagonalPreconditioner with precon = 1 / the
+ * diagonal of A (:76-89, :123-131, :167-175, :259-267): jacobi = 1. */
+static void orc_hodge_inner(int jacobi, int L, double* aux2, const double* aux1,
+                            const int* Arows, const int* Acols, const double* Avals,
+                            double* auxR, double inner_precision, int inner_print_modulo,
+                            const double* diag_or_precon)
+{
+    if (jacobi)
+        orc_cg_jacobi(L, aux2, aux1, Arows, Acols, Avals, auxR, inner_precision,
+                      inner_print_modulo, diag_or_precon, NULL);
+    else
+        orc_cg_ssor(L, aux2, aux1, Arows, Acols, Avals, auxR, inner_precision,
+                    inner_print_modulo, diag_or_precon, 1., NULL);
+}
+
+static int orc_hodge_cr(int jacobi, int N, int L, double* x, const double* b,
+                        const int* Arows, const int* Acols, const double* Avals,
+                        const int* Brows, const int* Bcols, const double* Bvals,
+                        const int* Btrows, const int* Btcols, const double* Btvals,
+                        const int* Crows, const int* Ccols, const double* Cvals,
+                        double* res, double precision, int print_modulo,
+                        double inner_precision, int inner_print_modulo, orc_info* info)
 {
     const double tol = orc_tolerance(N, b, precision);           /* :457-461 */
     double* dir  = orc_fill(N, 0.);
@@ -639,6 +655,8 @@ int orc_hodge_cr_ssor(int N, int L, double* x, const double* b,
     for (int c = 0; c < L; c++)                                   /* :498-511 */
         for (int d = Arows[c]; d < Arows[c + 1]; d++)
             if (Acols[d] == c) diagonal[c] += Avals[d];
+    if (jacobi)                                                   /* :85 */
+        for (int c = 0; c < L; c++) diagonal[c] = 1. / diagonal[c];
     double Md_r = NAN, Md_Md = NAN;
     int K = 0, reason = ORC_EXIT_CAP;
     while (K < N) {
@@ -648,8 +666,8 @@ int orc_hodge_cr_ssor(int N, int L, double* x, const double* b,
                 for (int d = Btrows[c]; d < Btrows[c + 1]; d++) aux1[c] += Btvals[d] * x[Btcols[d]];
             }
             for (int c = 0; c < L; c++) aux2[c] = 0.;            /* :544-554 */
-            orc_cg_ssor(L, aux2, aux1, Arows, Acols, Avals, auxR, inner_precision,
-                        inner_print_modulo, diagonal, 1., NULL);
+            orc_hodge_inner(jacobi, L, aux2, aux1, Arows, Acols, Avals, auxR, inner_precision,
+                            inner_print_modulo, diagonal);
             for (int c = 0; c < N; c++) {                         /* :556-570 */
                 res[c] = b[c];
                 for (int d = Brows[c]; d < Brows[c + 1]; d++) res[c] -= 1. * Bvals[d] * aux2[Bcols[d]];
@@ -663,8 +681,8 @@ int orc_hodge_cr_ssor(int N, int L, double* x, const double* b,
                 for (int d = Btrows[c]; d < Btrows[c + 1]; d++) aux1[c] += Btvals[d] * dir[Btcols[d]];
             }
             for (int c = 0; c < L; c++) aux2[c] = 0.;            /* :588-598 */
-            orc_cg_ssor(L, aux2, aux1, Arows, Acols, Avals, auxR, inner_precision,
-                        inner_print_modulo, diagonal, 1., NULL);
+            orc_hodge_inner(jacobi, L, aux2, aux1, Arows, Acols, Avals, auxR, inner_precision,
+                            inner_print_modulo, diagonal);
             for (int c = 0; c < N; c++) {                         /* :600-619 */
                 Mdir[c] = 0.;
                 for (int d = Brows[c]; d < Brows[c + 1]; d++) Mdir[c] += 1. * Bvals[d] * aux2[Bcols[d]];
@@ -684,8 +702,8 @@ int orc_hodge_cr_ssor(int N, int L, double* x, const double* b,
             aux1[c] = 0.;
             for (int d = Btrows[c]; d < Btrows[c + 1]; d++) aux1[c] += Btvals[d] * Mdir[Btcols[d]];
         }
-        orc_cg_ssor(L, aux2, aux1, Arows, Acols, Avals, auxR, inner_precision,  /* :679-689 */
-                    inner_print_modulo, diagonal, 1., NULL);
+        orc_hodge_inner(jacobi, L, aux2, aux1, Arows, Acols, Avals, auxR,  /* :679-689 */
+                        inner_precision, inner_print_modulo, diagonal);
         for (int c = 0; c < N; c++) {                             /* :691-721 */
             vil[c] = 0.;
             for (int d = Brows[c]; d < Brows[c + 1]; d++) vil[c] += 1. * Bvals[d] * aux2[Bcols[d]];
@@ -709,4 +727,30 @@ int orc_hodge_cr_ssor(int N, int L, double* x, const double* b,
     free(dir); free(Mdir); free(Mres); free(vil);
     free(aux1); free(aux2); free(auxR); free(diagonal);
     return K;
+}
+
+int orc_hodge_cr_ssor(int N, int L, double* x, const double* b,
+                      const int* Arows, const int* Acols, const double* Avals,
+                      const int* Brows, const int* Bcols, const double* Bvals,
+                      const int* Btrows, const int* Btcols, const double* Btvals,
+                      const int* Crows, const int* Ccols, const double* Cvals,
+                      double* res, double precision, int print_modulo,
+                      double inner_precision, int inner_print_modulo, orc_info* info)
+{
+    return orc_hodge_cr(0, N, L, x, b, Arows, Acols, Avals, Brows, Bcols, Bvals, Btrows, Btcols,
+                        Btvals, Crows, Ccols, Cvals, res, precision, print_modulo, inner_precision,
+                        inner_print_modulo, info);
+}
+
+int orc_hodge_cr_diagonal(int N, int L, double* x, const double* b,
+                          const int* Arows, const int* Acols, const double* Avals,
+                          const int* Brows, const int* Bcols, const double* Bvals,
+                          const int* Btrows, const int* Btcols, const double* Btvals,
+                          const int* Crows, const int* Ccols, const double* Cvals,
+                          double* res, double precision, int print_modulo,
+                          double inner_precision, int inner_print_modulo, orc_info* info)
+{
+    return orc_hodge_cr(1, N, L, x, b, Arows, Acols, Avals, Brows, Bcols, Bvals, Btrows, Btcols,
+                        Btvals, Crows, Ccols, Cvals, res, precision, print_modulo, inner_precision,
+                        inner_print_modulo, info);
 }

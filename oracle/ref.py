@@ -50,6 +50,8 @@ def _init_library():
     _lib.feecref_hodge_cr_ssor.argtypes = [c_int, c_int, _P_D, _P_D] + [_P_I, _P_I, _P_D] * 4 + \
         [_P_D, c_double, c_int, c_double, c_int]
     _lib.feecref_hodge_cr_ssor.restype = c_int
+    _lib.feecref_hodge_cr_diagonal.argtypes = _lib.feecref_hodge_cr_ssor.argtypes
+    _lib.feecref_hodge_cr_diagonal.restype = c_int
     _lib.feecref_mesh.argtypes = [c_int, c_int, c_int]
     _lib.feecref_mesh.restype = c_int
     _lib.feecref_mesh_sizes.argtypes = [_P_I]
@@ -146,9 +148,9 @@ def _task(kind, kw):
         b = np.ascontiguousarray(kw["b"], dtype=np.float64)
         res = np.zeros(N)
         ptrs = [(_i if k % 3 < 2 else _d)(m) for k, m in enumerate(mats)]
-        K = L.feecref_hodge_cr_ssor(N, Ldim, _d(x), _d(b), *ptrs, _d(res), float(kw["precision"]),
-                                    int(kw["print_modulo"]), float(kw["inner_precision"]),
-                                    int(kw["inner_print_modulo"]))
+        fn = L.feecref_hodge_cr_diagonal if kw.get("inner") == "diagonal" else L.feecref_hodge_cr_ssor
+        K = fn(N, Ldim, _d(x), _d(b), *ptrs, _d(res), float(kw["precision"]),
+               int(kw["print_modulo"]), float(kw["inner_precision"]), int(kw["inner_print_modulo"]))
         return x, res, K
     A = np.ascontiguousarray(kw["A"], dtype=np.int32)
     C = np.ascontiguousarray(kw["C"], dtype=np.int32)
@@ -270,13 +272,17 @@ class RefRunner:
                           aux=aux, omega=omega, colouring=colouring, repeat=repeat)
 
     def hodge_cr_ssor(self, N, L, A, B, Bt, C, b, x0=None, precision=1e-10, print_modulo=-3,
-                      inner_precision=1e-10, inner_print_modulo=-3):
-        """HodgeConjugateResidualSolverCSR_SSOR; matrices are (rows, cols, vals)
-        CSR triples.  Returns (x, residual, K)."""
+                      inner_precision=1e-10, inner_print_modulo=-3, inner="ssor"):
+        """HodgeConjugateResidualSolverCSR_SSOR (inner="diagonal":
+        HodgeConjugateResidualSolverCSR_diagonal); matrices are (rows, cols,
+        vals) CSR triples.  Returns (x, residual, K)."""
+        if inner not in ("ssor", "diagonal"):
+            raise ValueError(inner)
         x0 = np.zeros(N) if x0 is None else x0
         return self._call("hodge", N=N, L=L, A=A, B=B, Bt=Bt, C=C, b=b, x0=x0,
                           precision=precision, print_modulo=print_modulo,
-                          inner_precision=inner_precision, inner_print_modulo=inner_print_modulo)
+                          inner_precision=inner_precision, inner_print_modulo=inner_print_modulo,
+                          inner=inner)
 
     def close(self):
         if self._proc.poll() is None:

@@ -185,7 +185,15 @@ _SIGNATURES = {
     "sfa_HodgeConjugateResidualSolverCSR_SSOR":
         ([c_int, c_int, _P_D, _P_D] + [_P_I, _P_I, _P_D] * 4 +
          [_P_D, c_double, c_int, c_double, c_int, POINTER(SolveInfo)], c_int),
+    "sfa_HodgeConjugateResidualSolverCSR":
+        ([c_int, c_int, _P_D, _P_D] + [_P_I, _P_I, _P_D] * 4 +
+         [_P_D, c_double, c_int, c_double, c_int, POINTER(SolveInfo)], c_int),
+    "sfa_HodgeConjugateResidualSolverCSR_diagonal":
+        ([c_int, c_int, _P_D, _P_D] + [_P_I, _P_I, _P_D] * 4 +
+         [_P_D, c_double, c_int, c_double, c_int, POINTER(SolveInfo)], c_int),
     "sfa_hodge_create": ([c_int, c_int] + [_P_I, _P_I, _P_D] * 4 + [POINTER(c_void_p)], c_int),
+    "sfa_hodge_create_inner": ([c_int, c_int] + [_P_I, _P_I, _P_D] * 4 + [c_int, POINTER(c_void_p)],
+                               c_int),
     "sfa_hodge_solve": ([c_void_p, c_void_p, c_void_p, c_void_p, c_double, c_int, c_double, c_int,
                          c_int, POINTER(SolveInfo)], c_int),
     "sfa_hodge_destroy": ([c_void_p], c_int),
@@ -609,12 +617,12 @@ def _blocks(N, L, A, B, Bt, C):
     return out
 
 
-def HodgeConjugateResidualSolverCSR_SSOR(N, L, x, b, A, B, Bt, C, residual, precision,
-                                         print_modulo, inneriteration_precision,
-                                         inneriteration_print_modulo, info=None):
-    """solver/systemsparsesolver.hpp:58-72: (B A^-1 Bt + C) x = b by conjugate
-    residuals with inner CG-SSOR solves of A.  A, B, Bt, C are
-    (rows, cols, vals) CSR triples.  Returns K."""
+HODGE_INNER_SSOR = 0
+HODGE_INNER_DIAGONAL = 1
+
+
+def _hodge_once(entry, N, L, x, b, A, B, Bt, C, residual, precision, print_modulo,
+                inneriteration_precision, inneriteration_print_modulo, info):
     lib = load()
     x = _f64(x, "x", N, writable=True)
     residual = _f64(residual, "residual", N, writable=True)
@@ -622,7 +630,7 @@ def HodgeConjugateResidualSolverCSR_SSOR(N, L, x, b, A, B, Bt, C, residual, prec
     mats = _blocks(N, L, A, B, Bt, C)
     ptrs = [(_i if k % 3 < 2 else _d)(m) for k, m in enumerate(mats)]
     out = SolveInfo()
-    _check(lib.sfa_HodgeConjugateResidualSolverCSR_SSOR(
+    _check(getattr(lib, entry)(
         int(N), int(L), _d(x), _d(b), *ptrs, _d(residual), float(precision), int(print_modulo),
         float(inneriteration_precision), int(inneriteration_print_modulo), ctypes.byref(out)))
     if info is not None:
@@ -630,15 +638,51 @@ def HodgeConjugateResidualSolverCSR_SSOR(N, L, x, b, A, B, Bt, C, residual, prec
     return out.iterations
 
 
-class Hodge:
-    """A block system kept on the device between solves (sfa_hodge_*)."""
+def HodgeConjugateResidualSolverCSR_SSOR(N, L, x, b, A, B, Bt, C, residual, precision,
+                                         print_modulo, inneriteration_precision,
+                                         inneriteration_print_modulo, info=None):
+    """solver/systemsparsesolver.hpp:58-72: (B A^-1 Bt + C) x = b by conjugate
+    residuals with inner CG-SSOR solves of A.  A, B, Bt, C are
+    (rows, cols, vals) CSR triples.  Returns K."""
+    return _hodge_once("sfa_HodgeConjugateResidualSolverCSR_SSOR", N, L, x, b, A, B, Bt, C,
+                       residual, precision, print_modulo, inneriteration_precision,
+                       inneriteration_print_modulo, info)
 
-    def __init__(self, N, L, A, B, Bt, C):
+
+def HodgeConjugateResidualSolverCSR(N, L, x, b, A, B, Bt, C, residual, precision, print_modulo,
+                                    inneriteration_precision, inneriteration_print_modulo,
+                                    info=None):
+    """solver/systemsparsesolver.hpp:10-24: the SSOR variant, as the reference's
+    plain variant calls it (solver/systemsparsesolver.cpp:18-49).  Returns K."""
+    return _hodge_once("sfa_HodgeConjugateResidualSolverCSR", N, L, x, b, A, B, Bt, C, residual,
+                       precision, print_modulo, inneriteration_precision,
+                       inneriteration_print_modulo, info)
+
+
+def HodgeConjugateResidualSolverCSR_diagonal(N, L, x, b, A, B, Bt, C, residual, precision,
+                                             print_modulo, inneriteration_precision,
+                                             inneriteration_print_modulo, info=None):
+    """solver/systemsparsesolver.hpp:26-40: the same conjugate residual loop
+    with inner Jacobi CG solves of A (precon = 1 / diag A).  Returns K."""
+    return _hodge_once("sfa_HodgeConjugateResidualSolverCSR_diagonal", N, L, x, b, A, B, Bt, C,
+                       residual, precision, print_modulo, inneriteration_precision,
+                       inneriteration_print_modulo, info)
+
+
+class Hodge:
+    """A block system kept on the device between solves (sfa_hodge_*);
+    inner="diagonal" solves with inner Jacobi CG as
+    HodgeConjugateResidualSolverCSR_diagonal does."""
+
+    def __init__(self, N, L, A, B, Bt, C, inner="ssor"):
+        if inner not in ("ssor", "diagonal"):
+            raise ValueError(f"inner must be 'ssor' or 'diagonal', not {inner!r}")
         self._h = c_void_p()
         self.N, self.L = int(N), int(L)
         mats = _blocks(self.N, self.L, A, B, Bt, C)
         ptrs = [(_i if k % 3 < 2 else _d)(m) for k, m in enumerate(mats)]
-        _check(load().sfa_hodge_create(self.N, self.L, *ptrs, ctypes.byref(self._h)))
+        kind = HODGE_INNER_SSOR if inner == "ssor" else HODGE_INNER_DIAGONAL
+        _check(load().sfa_hodge_create_inner(self.N, self.L, *ptrs, kind, ctypes.byref(self._h)))
 
     def solve(self, x, b, residual, precision=1e-10, print_modulo=-3,
               inneriteration_precision=1e-10, inneriteration_print_modulo=-3):

@@ -1,9 +1,12 @@
 // hodge.hip -- HodgeConjugateResidualSolverCSR_SSOR on the GPU
-// (solver/systemsparsesolver.cpp:426-797).
+// (solver/systemsparsesolver.cpp:426-797), and the same loop as
+// HodgeConjugateResidualSolverCSR_diagonal (:52-424).
 //
 // Conjugate residuals on the Schur complement (B A^-1 Bt + C) x = b of a
 // block system [A Bt; B -C], every A^-1 an inner ConjugateGradientSolverCSR_SSOR
-// (natural-order SSOR, omega 1, the diagonal of A).  A stays resident as an
+// (natural-order SSOR, omega 1, the diagonal of A) or, for the diagonal
+// variant, an inner ConjugateGradientSolverCSR_DiagonalPreconditioner with
+// precon = 1 / the diagonal of A (:76-89).  A stays resident as an
 // sfa_system (its level schedules and SELL parts built once); B, Bt and C are
 // SELL-64 parts in their own row order.  Each outer iteration is one product
 // Bt Mdir, one inner solve, and two fused row kernels:
@@ -131,7 +134,8 @@ struct HodgeSolver {
     int N = 0, L = 0;
     int device = 0;
     hipStream_t stream = nullptr;
-    sfa_system inner = nullptr;       // A, natural-order SSOR, resident
+    sfa_system inner = nullptr;       // A, natural-order SSOR or Jacobi CG, resident
+    bool diagonal_variant = false;    // inner Jacobi CG (the _diagonal variant)
     DevPart Bp, Btp, Cp;              // B (N x L), Bt (L x N), C (N x N)
     double *x = nullptr, *b = nullptr, *res = nullptr, *dir = nullptr, *Mdir = nullptr,
            *Mres = nullptr, *aux1 = nullptr, *aux2 = nullptr, *auxR = nullptr;
@@ -161,8 +165,11 @@ struct HodgeSolver {
 
     void create(int N_, int L_, const int* Ar, const int* Ac, const double* Av, const int* Br,
                 const int* Bc, const double* Bv, const int* Btr, const int* Btc,
-                const double* Btv, const int* Cr, const int* Cc, const double* Cv)
+                const double* Btv, const int* Cr, const int* Cc, const double* Cv,
+                int inner_kind)
     {
+        if (inner_kind != SFA_HODGE_INNER_SSOR && inner_kind != SFA_HODGE_INNER_DIAGONAL)
+            throw Error{SFA_ERR_INVALID, "inner must be SFA_HODGE_INNER_SSOR or _DIAGONAL"};
         if (N_ <= 0 || L_ <= 0) throw Error{SFA_ERR_INVALID, "N and L must be positive"};
         // A block without entries (the zero matrix C of the reference's Darcy
         // tests) may come with null column and value arrays.
@@ -184,16 +191,20 @@ struct HodgeSolver {
         block(N_, N_, Cr, Cc, Cv, "C");
         N = N_;
         L = L_;
-        // the diagonal of A as the reference sums it (:498-511)
+        diagonal_variant = inner_kind == SFA_HODGE_INNER_DIAGONAL;
+        // the diagonal of A as the reference sums it (:498-511, :76-83); the
+        // diagonal variant inverts it (:85, no guard: the reference asserts > 0)
         std::vector<double> diag(L, 0.);
-        for (int c = 0; c < L; c++)
+        for (int c = 0; c < L; c++) {
             for (int d = Ar[c]; d < Ar[c + 1]; d++)
                 if (Ac[d] == c) diag[c] += Av[d];
+            if (diagonal_variant) diag[c] = 1. / diag[c];
+        }
         ensure_device();
         SFA_HIP(hipGetDevice(&device));
         SFA_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        check(sfa_system_create(SFA_METHOD_SSOR, L, Ar, Ac, Av, diag.data(), 1.0, 0, nullptr,
-                                nullptr, nullptr, &inner));
+        check(sfa_system_create(diagonal_variant ? SFA_METHOD_DIAGONAL : SFA_METHOD_SSOR, L, Ar, Ac,
+                                Av, diag.data(), 1.0, 0, nullptr, nullptr, nullptr, &inner));
         check(sfa_system_set_stream(inner, stream));
         SellPart part;
         build_full_identity(part, N, L, Br, Bc, Bv);
@@ -223,7 +234,7 @@ struct HodgeSolver {
         if (status != SFA_OK) throw Error{status, sfa_last_error()};
     }
 
-    // aux2 = A^-1 (Bt src) by the inner CG-SSOR, from aux2 (zero it first
+    // aux2 = A^-1 (Bt src) by the inner CG, from aux2 (zero it first
     // for a cold start).
     void inner_solve(const double* src, bool cold, double precision, int print_modulo)
     {
@@ -275,8 +286,15 @@ struct HodgeSolver {
         tol = precision * std::sqrt(tol);
         if (!(tol >= kDesiredPrecision) && tol <= kDesiredPrecision) tol = kDesiredPrecision;
 
+        // the diagonal variant gates its breakdown, warning and finish lines
+        // by print_modulo (:91-96, :280-286, :375); the SSOR one prints the
+        // first two always and the last from print_modulo >= 0 (:465-470, :746)
+        const bool dv = diagonal_variant;
+        const bool print_hard = !dv || print_modulo >= -2;
+        const bool print_finish = dv ? print_modulo >= -1 : print_modulo >= 0;
         if (print_modulo >= -1)
-            std::printf("(%d/%d)     BEGIN: Hodge Conjugate Residual (SSOR) CSR\n", 0, N);
+            std::printf("(%d/%d)     BEGIN: Hodge Conjugate Residual (%s) CSR\n", 0, N,
+                        dv ? "diag" : "SSOR");
         double Md_r = NAN, Md_Md = NAN;
         int K = 0, reason = SFA_EXIT_CAP;
         const double tol2 = tol * tol;
@@ -306,16 +324,19 @@ struct HodgeSolver {
                 break;
             }
             if (!std::isfinite(Md_Md) || Md_Md < 0.) {  // :646-652
-                std::printf("(%d/%d) BREAKDOWN: Gradient double energy is unreasonable with %.9le\n",
-                            K, N, Md_Md);
+                if (print_hard)
+                    std::printf("(%d/%d) BREAKDOWN: Gradient double energy is unreasonable with %.9le\n",
+                                K, N, Md_Md);
                 reason = SFA_EXIT_BREAKDOWN;
                 break;
             }
             if (std::sqrt(std::fabs(Md_Md)) < kMachineEpsilon) {  // :654-658
-                std::printf("(%d/%d)   WARNING: Residual norm is %.9le < %.9le\n", K, N,
-                            std::sqrt(Md_r), tol);
-                std::printf("(%d/%d)   WARNING: Gradient double energy is small with %.9le\n", K,
-                            N, Md_Md);
+                if (print_hard) {
+                    std::printf("(%d/%d)   WARNING: Residual norm is %.9le < %.9le\n", K, N,
+                                std::sqrt(Md_r), tol);
+                    std::printf("(%d/%d)   WARNING: Gradient double energy is small with %.9le\n",
+                                K, N, Md_Md);
+                }
                 reason = SFA_EXIT_SMALL;
                 break;
             }
@@ -331,7 +352,7 @@ struct HodgeSolver {
             Md_r = new_Mr_r;
             K++;
         }
-        if (print_modulo >= 0)
+        if (print_finish)
             std::printf("(%d/%d) %9s: Residual norm is %.9le < %.9le\n", K, N,
                         std::sqrt(Md_r) < tol ? "SUCCESS" : "FAILED", std::sqrt(Md_r), tol);
         std::fflush(stdout);
@@ -359,21 +380,50 @@ struct sfa_hodge_s {
     sfa::HodgeSolver h;
 };
 
+static int hodge_once(int inner, int N, int L, double* x, const double* b, const int* Arows,
+               const int* Acolumns, const double* Avalues, const int* Brows, const int* Bcolumns,
+               const double* Bvalues, const int* Btrows, const int* Btcolumns,
+               const double* Btvalues, const int* Crows, const int* Ccolumns,
+               const double* Cvalues, double* residual, double precision, int print_modulo,
+               double inneriteration_precision, int inneriteration_print_modulo,
+               sfa_solve_info* info)
+{
+    return sfa::guarded([&] {
+        if (!x || !b || !residual) throw sfa::Error{SFA_ERR_INVALID, "null x, b or residual"};
+        sfa::HodgeSolver h;
+        h.create(N, L, Arows, Acolumns, Avalues, Brows, Bcolumns, Bvalues, Btrows, Btcolumns,
+                 Btvalues, Crows, Ccolumns, Cvalues, inner);
+        h.solve(x, b, residual, precision, print_modulo, inneriteration_precision,
+                inneriteration_print_modulo, 0, info);
+    });
+}
+
 extern "C" {
 
-int sfa_hodge_create(int N, int L, const int* Arows, const int* Acolumns, const double* Avalues,
-                     const int* Brows, const int* Bcolumns, const double* Bvalues,
-                     const int* Btrows, const int* Btcolumns, const double* Btvalues,
-                     const int* Crows, const int* Ccolumns, const double* Cvalues, sfa_hodge* out)
+int sfa_hodge_create_inner(int N, int L, const int* Arows, const int* Acolumns,
+                           const double* Avalues, const int* Brows, const int* Bcolumns,
+                           const double* Bvalues, const int* Btrows, const int* Btcolumns,
+                           const double* Btvalues, const int* Crows, const int* Ccolumns,
+                           const double* Cvalues, int inner, sfa_hodge* out)
 {
     return sfa::guarded([&] {
         if (!out) throw sfa::Error{SFA_ERR_INVALID, "null output handle"};
         *out = nullptr;
         std::unique_ptr<sfa_hodge_s> h(new sfa_hodge_s());
         h->h.create(N, L, Arows, Acolumns, Avalues, Brows, Bcolumns, Bvalues, Btrows, Btcolumns,
-                    Btvalues, Crows, Ccolumns, Cvalues);
+                    Btvalues, Crows, Ccolumns, Cvalues, inner);
         *out = h.release();
     });
+}
+
+int sfa_hodge_create(int N, int L, const int* Arows, const int* Acolumns, const double* Avalues,
+                     const int* Brows, const int* Bcolumns, const double* Bvalues,
+                     const int* Btrows, const int* Btcolumns, const double* Btvalues,
+                     const int* Crows, const int* Ccolumns, const double* Cvalues, sfa_hodge* out)
+{
+    return sfa_hodge_create_inner(N, L, Arows, Acolumns, Avalues, Brows, Bcolumns, Bvalues,
+                                  Btrows, Btcolumns, Btvalues, Crows, Ccolumns, Cvalues,
+                                  SFA_HODGE_INNER_SSOR, out);
 }
 
 int sfa_hodge_solve(sfa_hodge h, double* x, const double* b, double* residual, double precision,
@@ -393,6 +443,7 @@ int sfa_hodge_destroy(sfa_hodge h)
     return SFA_OK;
 }
 
+
 int sfa_HodgeConjugateResidualSolverCSR_SSOR(
     int N, int L, double* x, const double* b, const int* Arows, const int* Acolumns,
     const double* Avalues, const int* Brows, const int* Bcolumns, const double* Bvalues,
@@ -401,14 +452,39 @@ int sfa_HodgeConjugateResidualSolverCSR_SSOR(
     int print_modulo, double inneriteration_precision, int inneriteration_print_modulo,
     sfa_solve_info* info)
 {
-    return sfa::guarded([&] {
-        if (!x || !b || !residual) throw sfa::Error{SFA_ERR_INVALID, "null x, b or residual"};
-        sfa::HodgeSolver h;
-        h.create(N, L, Arows, Acolumns, Avalues, Brows, Bcolumns, Bvalues, Btrows, Btcolumns,
-                 Btvalues, Crows, Ccolumns, Cvalues);
-        h.solve(x, b, residual, precision, print_modulo, inneriteration_precision,
-                inneriteration_print_modulo, 0, info);
-    });
+    return hodge_once(SFA_HODGE_INNER_SSOR, N, L, x, b, Arows, Acolumns, Avalues, Brows,
+                      Bcolumns, Bvalues, Btrows, Btcolumns, Btvalues, Crows, Ccolumns, Cvalues,
+                      residual, precision, print_modulo, inneriteration_precision,
+                      inneriteration_print_modulo, info);
+}
+
+int sfa_HodgeConjugateResidualSolverCSR(
+    int N, int L, double* x, const double* b, const int* Arows, const int* Acolumns,
+    const double* Avalues, const int* Brows, const int* Bcolumns, const double* Bvalues,
+    const int* Btrows, const int* Btcolumns, const double* Btvalues, const int* Crows,
+    const int* Ccolumns, const double* Cvalues, double* residual, double precision,
+    int print_modulo, double inneriteration_precision, int inneriteration_print_modulo,
+    sfa_solve_info* info)
+{
+    // the reference's plain variant is the SSOR one (solver/systemsparsesolver.cpp:18-49)
+    return sfa_HodgeConjugateResidualSolverCSR_SSOR(
+        N, L, x, b, Arows, Acolumns, Avalues, Brows, Bcolumns, Bvalues, Btrows, Btcolumns,
+        Btvalues, Crows, Ccolumns, Cvalues, residual, precision, print_modulo,
+        inneriteration_precision, inneriteration_print_modulo, info);
+}
+
+int sfa_HodgeConjugateResidualSolverCSR_diagonal(
+    int N, int L, double* x, const double* b, const int* Arows, const int* Acolumns,
+    const double* Avalues, const int* Brows, const int* Bcolumns, const double* Bvalues,
+    const int* Btrows, const int* Btcolumns, const double* Btvalues, const int* Crows,
+    const int* Ccolumns, const double* Cvalues, double* residual, double precision,
+    int print_modulo, double inneriteration_precision, int inneriteration_print_modulo,
+    sfa_solve_info* info)
+{
+    return hodge_once(SFA_HODGE_INNER_DIAGONAL, N, L, x, b, Arows, Acolumns, Avalues, Brows,
+                      Bcolumns, Bvalues, Btrows, Btcolumns, Btvalues, Crows, Ccolumns, Cvalues,
+                      residual, precision, print_modulo, inneriteration_precision,
+                      inneriteration_print_modulo, info);
 }
 
 }  // extern "C"

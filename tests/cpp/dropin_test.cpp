@@ -237,6 +237,20 @@ static void hodge_checks()
     CHECK((x - want).norm() < 1e-6 * want.norm());
     std::printf("HodgeConjugateResidualSolverCSR_SSOR: K=%d, |x - A b| / |A b| = %.2e\n", K,
                 (x - want).norm() / want.norm());
+    // the plain variant is the SSOR one; the diagonal one has inner Jacobi CG
+    FloatVector xp(n, 0.), xd(n, 0.), rp(n, 0.), rd(n, 0.);
+    const int Kp = HodgeConjugateResidualSolverCSR(
+        n, n, xp.raw(), b.raw(), A.getA(), A.getC(), A.getV(), I.getA(), I.getC(), I.getV(),
+        I.getA(), I.getC(), I.getV(), Z.getA(), Z.getC(), Z.getV(), rp.raw(), desired_precision,
+        -3, desired_precision, -3);
+    CHECK(Kp == K && (xp - x).norm() == 0.);
+    const int Kd = HodgeConjugateResidualSolverCSR_diagonal(
+        n, n, xd.raw(), b.raw(), A.getA(), A.getC(), A.getV(), I.getA(), I.getC(), I.getV(),
+        I.getA(), I.getC(), I.getV(), Z.getA(), Z.getC(), Z.getV(), rd.raw(), desired_precision,
+        -1, desired_precision, -3);
+    CHECK((xd - want).norm() < 1e-6 * want.norm());
+    std::printf("HodgeConjugateResidualSolverCSR_diagonal: K=%d, |x - A b| / |A b| = %.2e\n", Kd,
+                (xd - want).norm() / want.norm());
 }
 
 int main(int argc, char** argv)

@@ -88,6 +88,24 @@ def test_invalid_blocks_are_rejected_before_the_device():
                                                       bs.A, B, Bt, bs.C, np.zeros(bs.N), 1e-10,
                                                       -3, 1e-10, -3)
         assert e.value.code == -1 and f": {word}: column" in str(e.value), str(e.value)
+    # the plain and diagonal variants run the same host checks
+    for fn in (capi.HodgeConjugateResidualSolverCSR, capi.HodgeConjugateResidualSolverCSR_diagonal):
+        with pytest.raises(capi.SfaError) as e:
+            fn(bs.N, bs.L, np.zeros(bs.N), np.ones(bs.N), bs.A, bad_B, bs.Bt, bs.C,
+               np.zeros(bs.N), 1e-10, -3, 1e-10, -3)
+        assert e.value.code == -1 and ": B: column" in str(e.value), str(e.value)
+
+
+def test_unknown_hodge_inner_solver_is_rejected_before_the_device():
+    bs = S.schur_blocks(8, 2, 0.0, 1)
+    with pytest.raises(ValueError):
+        capi.Hodge(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, inner="jacobi")
+    mats = capi._blocks(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C)
+    ptrs = [(capi._i if k % 3 < 2 else capi._d)(m) for k, m in enumerate(mats)]
+    h = ctypes.c_void_p()
+    rc = capi.load().sfa_hodge_create_inner(bs.N, bs.L, *ptrs, 7, ctypes.byref(h))
+    assert rc == -1 and not h.value
+    assert "inner must be" in capi.load().sfa_last_error().decode()
 
 
 def test_inconsistent_colouring_is_rejected_before_the_device():

@@ -113,3 +113,52 @@ def test_hodge_edge_cases_match_oracle():
         assert abs(K - io["iterations"]) <= 1 and info["exit_reason"] == io["exit_reason"], name
         assert np.abs(x - xo).max() <= 1e-6 * max(np.abs(xo).max(), 1e-300) or \
             np.array_equal(x, xo), name
+
+
+# HodgeConjugateResidualSolverCSR_diagonal (solver/systemsparsesolver.cpp:52-424, inner
+# Jacobi CG) and HodgeConjugateResidualSolverCSR (:18-49, the SSOR variant), same bars.
+
+@pytest.mark.parametrize("case", CASES, ids=[f"m{c[0]}_c{c[1]}_w{c[2]}" for c in CASES])
+def test_hodge_diagonal_one_shot_matches_oracle(case):
+    bs = S.schur_blocks(*case)
+    b = S.uniform_pm1(5, bs.N)
+    print(f"[hodge diagonal] {bs.name} N={bs.N} L={bs.L}", flush=True)
+    xo, ro, io = oracle.hodge_cr_ssor(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, b, precision=1e-10,
+                                      inner_precision=1e-12, inner="diagonal")
+    x = np.zeros(bs.N)
+    r = np.zeros(bs.N)
+    info = {}
+    K = capi.HodgeConjugateResidualSolverCSR_diagonal(bs.N, bs.L, x, b, bs.A, bs.B, bs.Bt, bs.C,
+                                                      r, 1e-10, -3, 1e-12, -3, info=info)
+    assert K == info["iterations"]
+    print(f"[hodge diagonal] K gpu {K} oracle {io['iterations']}", flush=True)
+    _check(bs, b, x, info, xo, io)
+
+
+def test_hodge_plain_variant_is_the_ssor_one():
+    bs = S.schur_blocks(16, 2, 0.5, 3)
+    b = S.uniform_pm1(5, bs.N)
+    xs, xp = np.zeros(bs.N), np.zeros(bs.N)
+    rs, rp = np.zeros(bs.N), np.zeros(bs.N)
+    Ks = capi.HodgeConjugateResidualSolverCSR_SSOR(bs.N, bs.L, xs, b, bs.A, bs.B, bs.Bt, bs.C, rs,
+                                                   1e-10, -3, 1e-12, -3)
+    Kp = capi.HodgeConjugateResidualSolverCSR(bs.N, bs.L, xp, b, bs.A, bs.B, bs.Bt, bs.C, rp,
+                                              1e-10, -3, 1e-12, -3)
+    assert Ks == Kp and np.array_equal(xs, xp) and np.array_equal(rs, rp)
+
+
+def test_hodge_diagonal_handle_matches_one_shot():
+    bs = S.schur_blocks(32, 2, 0.25, 11)
+    b = S.uniform_pm1(8, bs.N)
+    x1, r1 = np.zeros(bs.N), np.zeros(bs.N)
+    K1 = capi.HodgeConjugateResidualSolverCSR_diagonal(bs.N, bs.L, x1, b, bs.A, bs.B, bs.Bt, bs.C,
+                                                       r1, 1e-10, -3, 1e-12, -3)
+    xo, _, io = oracle.hodge_cr_ssor(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, b, precision=1e-10,
+                                     inner_precision=1e-12, inner="diagonal")
+    with capi.Hodge(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, inner="diagonal") as h:
+        for _ in range(2):
+            x = np.zeros(bs.N)
+            r = np.zeros(bs.N)
+            info = h.solve(x, b, r, 1e-10, -3, 1e-12, -3)
+            assert info["iterations"] == K1 and np.array_equal(x, x1) and np.array_equal(r, r1)
+            _check(bs, b, x, info, xo, io)

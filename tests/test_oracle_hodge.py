@@ -1,5 +1,6 @@
 """Pins the oracle's HodgeConjugateResidualSolverCSR_SSOR
-(solver/systemsparsesolver.cpp:426-797) before it checks the GPU path.
+(solver/systemsparsesolver.cpp:426-797) and HodgeConjugateResidualSolverCSR_diagonal
+(:52-424, inner Jacobi CG) before they check the GPU path.
 
   * golden fixtures from the compiled reference (K, x and residual bit for bit);
   * mathematics: the dense Schur complement solve, and with B = Bt = I and
@@ -26,23 +27,29 @@ def _case(name):
     return e, bs, b
 
 
+INNER = ["ssor", "diagonal"]
+
+
+@pytest.mark.parametrize("inner", INNER)
 @pytest.mark.parametrize("name", sorted(META))
-def test_oracle_matches_reference_fixtures(name):
+def test_oracle_matches_reference_fixtures(name, inner):
     e, bs, b = _case(name)
     x, res, info = oracle.hodge_cr_ssor(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, b,
                                         precision=e["precision"],
-                                        inner_precision=e["inner_precision"])
-    g = np.load(os.path.join(GOLDEN, f"hodge_{name}.npz"))
-    assert info["iterations"] == e["iterations"]
+                                        inner_precision=e["inner_precision"], inner=inner)
+    tag = "" if inner == "ssor" else "diagonal_"
+    g = np.load(os.path.join(GOLDEN, f"hodge_{tag}{name}.npz"))
+    assert info["iterations"] == e["iterations" if inner == "ssor" else "iterations_diagonal"]
     assert np.array_equal(x, g["x"]) and np.array_equal(res, g["res"])
 
 
+@pytest.mark.parametrize("inner", INNER)
 @pytest.mark.parametrize("name", sorted(META))
-def test_oracle_solves_the_schur_complement(name):
+def test_oracle_solves_the_schur_complement(name, inner):
     e, bs, b = _case(name)
     x, res, info = oracle.hodge_cr_ssor(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, b,
                                         precision=e["precision"],
-                                        inner_precision=e["inner_precision"])
+                                        inner_precision=e["inner_precision"], inner=inner)
     A, B, Bt, C = bs.dense()
     schur = B @ np.linalg.solve(A, Bt) + C
     xd = np.linalg.solve(schur, b)
@@ -52,7 +59,8 @@ def test_oracle_solves_the_schur_complement(name):
     assert np.linalg.norm(res - (b - schur @ x)) <= 1e-6 * np.linalg.norm(b)
 
 
-def test_oracle_identity_coupling_gives_a_times_b():
+@pytest.mark.parametrize("inner", INNER)
+def test_oracle_identity_coupling_gives_a_times_b(inner):
     """B = Bt = I, C = 0: the Schur complement is A^-1, so x = A b."""
     bs = S.schur_blocks(12, 2, 0.0, 1)
     L = bs.L
@@ -60,7 +68,7 @@ def test_oracle_identity_coupling_gives_a_times_b():
     empty = (np.zeros(L + 1, dtype=np.int32), np.zeros(0, dtype=np.int32), np.zeros(0))
     b = S.uniform_pm1(2, L)
     x, _, info = oracle.hodge_cr_ssor(L, L, bs.A, eye, eye, empty, b, precision=1e-12,
-                                      inner_precision=1e-14)
+                                      inner_precision=1e-14, inner=inner)
     want = oracle.csr_apply(L, bs.A[0], bs.A[1], bs.A[2], b)
     # each inner solve stops at the reference's absolute floor sqrt(z.r) < 2^-26,
     # which bounds the accuracy of every A^-1 application at ~1e-8
@@ -74,8 +82,10 @@ def test_oracle_bitexact_with_compiled_reference():
     with RefRunner(1) as ref:
         for name in sorted(META):
             e, bs, b = _case(name)
-            x, res, info = oracle.hodge_cr_ssor(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, b,
-                                                precision=1e-9, inner_precision=1e-11)
-            xr, rr, K = ref.hodge_cr_ssor(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, b, precision=1e-9,
-                                          inner_precision=1e-11)
-            assert K == info["iterations"] and np.array_equal(x, xr) and np.array_equal(res, rr)
+            for inner in INNER:
+                x, res, info = oracle.hodge_cr_ssor(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, b,
+                                                    precision=1e-9, inner_precision=1e-11,
+                                                    inner=inner)
+                xr, rr, K = ref.hodge_cr_ssor(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, b,
+                                              precision=1e-9, inner_precision=1e-11, inner=inner)
+                assert K == info["iterations"] and np.array_equal(x, xr) and np.array_equal(res, rr)
