@@ -466,15 +466,29 @@ int sfa_HodgeConjugateResidualSolverCSR_diagonal(
     double inneriteration_precision, int inneriteration_print_modulo,
     sfa_solve_info* info);
 
+/* HodgeConjugateResidualSolverCSR_textbook, solver/systemsparsesolver.hpp:42-56
+ * (solver/systemsparsesolver.cpp:798-1121): the same loop with every A^-1 an
+ * inner plain CG solve. */
+int sfa_HodgeConjugateResidualSolverCSR_textbook(
+    int N, int L, double* x, const double* b,
+    const int* Arows, const int* Acolumns, const double* Avalues,
+    const int* Brows, const int* Bcolumns, const double* Bvalues,
+    const int* Btrows, const int* Btcolumns, const double* Btvalues,
+    const int* Crows, const int* Ccolumns, const double* Cvalues,
+    double* residual, double precision, int print_modulo,
+    double inneriteration_precision, int inneriteration_print_modulo,
+    sfa_solve_info* info);
+
 /* The same solvers with the four matrices resident on the device between
- * solves (A as an sfa_system with its SSOR schedules, or its Jacobi
- * preconditioner): for callers that solve the same block system for many
- * right-hand sides.  No reference counterpart.  sfa_hodge_create keeps the
- * SSOR inner solver; sfa_hodge_create_inner takes SFA_HODGE_INNER_SSOR or
- * SFA_HODGE_INNER_DIAGONAL.  device_pointers = 1: x, b and residual are
+ * solves (A as an sfa_system with its SSOR schedules, its Jacobi
+ * preconditioner or none): for callers that solve the same block system for
+ * many right-hand sides.  No reference counterpart.  sfa_hodge_create keeps
+ * the SSOR inner solver; sfa_hodge_create_inner takes SFA_HODGE_INNER_SSOR,
+ * _DIAGONAL or _TEXTBOOK.  device_pointers = 1: x, b and residual are
  * device buffers. */
 #define SFA_HODGE_INNER_SSOR 0
 #define SFA_HODGE_INNER_DIAGONAL 1
+#define SFA_HODGE_INNER_TEXTBOOK 2
 typedef struct sfa_hodge_s* sfa_hodge;
 int sfa_hodge_create(int N, int L,
                      const int* Arows, const int* Acolumns, const double* Avalues,

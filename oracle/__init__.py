@@ -52,6 +52,8 @@ _SIG = {
                           [_P_D, c_double, c_int, c_double, c_int, POINTER(OrcInfo)], c_int),
     "orc_hodge_cr_diagonal": ([c_int, c_int, _P_D, _P_D] + [_P_I, _P_I, _P_D] * 4 +
                               [_P_D, c_double, c_int, c_double, c_int, POINTER(OrcInfo)], c_int),
+    "orc_hodge_cr_textbook": ([c_int, c_int, _P_D, _P_D] + [_P_I, _P_I, _P_D] * 4 +
+                              [_P_D, c_double, c_int, c_double, c_int, POINTER(OrcInfo)], c_int),
 }
 
 _lib = None
@@ -150,9 +152,10 @@ def hodge_cr_ssor(N, L, A, B, Bt, C, b, x0=None, precision=1e-10, print_modulo=-
     """HodgeConjugateResidualSolverCSR_SSOR (solver/systemsparsesolver.cpp:426-797)
     on (B A^-1 Bt + C) x = b; A is L x L, B is N x L, Bt is L x N, C is N x N,
     each a (rows, cols, vals) CSR triple.  inner="diagonal" runs
-    HodgeConjugateResidualSolverCSR_diagonal (:52-424, inner Jacobi CG).
-    Returns (x, residual, info dict)."""
-    if inner not in ("ssor", "diagonal"):
+    HodgeConjugateResidualSolverCSR_diagonal (:52-424, inner Jacobi CG),
+    inner="textbook" HodgeConjugateResidualSolverCSR_textbook (:798-1121, inner
+    plain CG).  Returns (x, residual, info dict)."""
+    if inner not in ("ssor", "diagonal", "textbook"):
         raise ValueError(inner)
     mats = []
     for m in (A, B, Bt, C):
@@ -162,7 +165,7 @@ def hodge_cr_ssor(N, L, A, B, Bt, C, b, x0=None, precision=1e-10, print_modulo=-
     res = np.zeros(N)
     info = OrcInfo()
     ptrs = [(_i if k % 3 < 2 else _d)(m) for k, m in enumerate(mats)]
-    fn = lib().orc_hodge_cr_ssor if inner == "ssor" else lib().orc_hodge_cr_diagonal
+    fn = getattr(lib(), "orc_hodge_cr_" + inner)
     fn(int(N), int(L), _d(x), _d(b), *ptrs, _d(res), float(precision), int(print_modulo),
        float(inner_precision), int(inner_print_modulo), ctypes.byref(info))
     return x, res, info.as_dict()

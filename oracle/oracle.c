@@ -621,13 +621,20 @@ int orc_cg_eisenstat_rainbow(int N, double* x, const double* b, const int* A,
  * only K == 0 restarts).  HodgeConjugateResidualSolverCSR_diagonal
  * (:52-424) is the same loop with every inner solve a
  * ConjugateGradientSolverCSR_DiagonalPreconditioner with precon = 1 / the
- * diagonal of A (:76-89, :123-131, :167-175, :259-267): jacobi = 1. */
-static void orc_hodge_inner(int jacobi, int L, double* aux2, const double* aux1,
+ * diagonal of A (:76-89, :123-131, :167-175, :259-267): inner = 1.
+ * HodgeConjugateResidualSolverCSR_textbook (:798-1121) is the same loop with
+ * every inner solve a plain ConjugateGradientSolverCSR (:895-903, :938-946,
+ * :1030-1038): inner = 2 (its Mr_r is the Md_r here: at the restart
+ * Mres = Mdir, :975-977, and both are updated alike after). */
+static void orc_hodge_inner(int inner, int L, double* aux2, const double* aux1,
                             const int* Arows, const int* Acols, const double* Avals,
                             double* auxR, double inner_precision, int inner_print_modulo,
                             const double* diag_or_precon)
 {
-    if (jacobi)
+    if (inner == 2)
+        orc_cg(L, aux2, aux1, Arows, Acols, Avals, auxR, inner_precision, inner_print_modulo,
+               NULL);
+    else if (inner == 1)
         orc_cg_jacobi(L, aux2, aux1, Arows, Acols, Avals, auxR, inner_precision,
                       inner_print_modulo, diag_or_precon, NULL);
     else
@@ -635,7 +642,7 @@ static void orc_hodge_inner(int jacobi, int L, double* aux2, const double* aux1,
                     inner_print_modulo, diag_or_precon, 1., NULL);
 }
 
-static int orc_hodge_cr(int jacobi, int N, int L, double* x, const double* b,
+static int orc_hodge_cr(int inner, int N, int L, double* x, const double* b,
                         const int* Arows, const int* Acols, const double* Avals,
                         const int* Brows, const int* Bcols, const double* Bvals,
                         const int* Btrows, const int* Btcols, const double* Btvals,
@@ -655,7 +662,7 @@ static int orc_hodge_cr(int jacobi, int N, int L, double* x, const double* b,
     for (int c = 0; c < L; c++)                                   /* :498-511 */
         for (int d = Arows[c]; d < Arows[c + 1]; d++)
             if (Acols[d] == c) diagonal[c] += Avals[d];
-    if (jacobi)                                                   /* :85 */
+    if (inner == 1)                                               /* :85 */
         for (int c = 0; c < L; c++) diagonal[c] = 1. / diagonal[c];
     double Md_r = NAN, Md_Md = NAN;
     int K = 0, reason = ORC_EXIT_CAP;
@@ -666,7 +673,7 @@ static int orc_hodge_cr(int jacobi, int N, int L, double* x, const double* b,
                 for (int d = Btrows[c]; d < Btrows[c + 1]; d++) aux1[c] += Btvals[d] * x[Btcols[d]];
             }
             for (int c = 0; c < L; c++) aux2[c] = 0.;            /* :544-554 */
-            orc_hodge_inner(jacobi, L, aux2, aux1, Arows, Acols, Avals, auxR, inner_precision,
+            orc_hodge_inner(inner, L, aux2, aux1, Arows, Acols, Avals, auxR, inner_precision,
                             inner_print_modulo, diagonal);
             for (int c = 0; c < N; c++) {                         /* :556-570 */
                 res[c] = b[c];
@@ -681,7 +688,7 @@ static int orc_hodge_cr(int jacobi, int N, int L, double* x, const double* b,
                 for (int d = Btrows[c]; d < Btrows[c + 1]; d++) aux1[c] += Btvals[d] * dir[Btcols[d]];
             }
             for (int c = 0; c < L; c++) aux2[c] = 0.;            /* :588-598 */
-            orc_hodge_inner(jacobi, L, aux2, aux1, Arows, Acols, Avals, auxR, inner_precision,
+            orc_hodge_inner(inner, L, aux2, aux1, Arows, Acols, Avals, auxR, inner_precision,
                             inner_print_modulo, diagonal);
             for (int c = 0; c < N; c++) {                         /* :600-619 */
                 Mdir[c] = 0.;
@@ -702,7 +709,7 @@ static int orc_hodge_cr(int jacobi, int N, int L, double* x, const double* b,
             aux1[c] = 0.;
             for (int d = Btrows[c]; d < Btrows[c + 1]; d++) aux1[c] += Btvals[d] * Mdir[Btcols[d]];
         }
-        orc_hodge_inner(jacobi, L, aux2, aux1, Arows, Acols, Avals, auxR,  /* :679-689 */
+        orc_hodge_inner(inner, L, aux2, aux1, Arows, Acols, Avals, auxR,  /* :679-689 */
                         inner_precision, inner_print_modulo, diagonal);
         for (int c = 0; c < N; c++) {                             /* :691-721 */
             vil[c] = 0.;
@@ -738,6 +745,19 @@ int orc_hodge_cr_ssor(int N, int L, double* x, const double* b,
                       double inner_precision, int inner_print_modulo, orc_info* info)
 {
     return orc_hodge_cr(0, N, L, x, b, Arows, Acols, Avals, Brows, Bcols, Bvals, Btrows, Btcols,
+                        Btvals, Crows, Ccols, Cvals, res, precision, print_modulo, inner_precision,
+                        inner_print_modulo, info);
+}
+
+int orc_hodge_cr_textbook(int N, int L, double* x, const double* b,
+                          const int* Arows, const int* Acols, const double* Avals,
+                          const int* Brows, const int* Bcols, const double* Bvals,
+                          const int* Btrows, const int* Btcols, const double* Btvals,
+                          const int* Crows, const int* Ccols, const double* Cvals,
+                          double* res, double precision, int print_modulo,
+                          double inner_precision, int inner_print_modulo, orc_info* info)
+{
+    return orc_hodge_cr(2, N, L, x, b, Arows, Acols, Avals, Brows, Bcols, Bvals, Btrows, Btcols,
                         Btvals, Crows, Ccols, Cvals, res, precision, print_modulo, inner_precision,
                         inner_print_modulo, info);
 }

@@ -22,7 +22,7 @@
  *                                       dense Cholesky solve, natural SSOR ==
  *                                       Rainbow SSOR with one colour per row,
  *                                       oracle/_ref bit-exact (K, x, residual)
- *   orc_hodge_cr_ssor, _diagonal      -- dense Schur-complement solve,
+ *   orc_hodge_cr_ssor/_diagonal/_textbook -- dense Schur-complement solve,
  *                                       B = identity KAT (x = (A^-1 + C)^-1 b),
  *                                       oracle/_ref bit-exact (K, x, residual)
  */
@@ -94,6 +94,17 @@ int orc_hodge_cr_ssor(int N, int L, double* x, const double* b,
                       double* res, double precision, int print_modulo,
                       double inner_precision, int inner_print_modulo,
                       orc_info* info);
+
+/* HodgeConjugateResidualSolverCSR_textbook (solver/systemsparsesolver.cpp:798-1121):
+ * the same loop with inner plain CG solves. */
+int orc_hodge_cr_textbook(int N, int L, double* x, const double* b,
+                          const int* Arows, const int* Acols, const double* Avals,
+                          const int* Brows, const int* Bcols, const double* Bvals,
+                          const int* Btrows, const int* Btcols, const double* Btvals,
+                          const int* Crows, const int* Ccols, const double* Cvals,
+                          double* res, double precision, int print_modulo,
+                          double inner_precision, int inner_print_modulo,
+                          orc_info* info);
 
 /* HodgeConjugateResidualSolverCSR_diagonal (solver/systemsparsesolver.cpp:52-424):
  * the same loop with inner Jacobi CG solves, precon = 1 / diag(A). */

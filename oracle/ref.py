@@ -52,6 +52,8 @@ def _init_library():
     _lib.feecref_hodge_cr_ssor.restype = c_int
     _lib.feecref_hodge_cr_diagonal.argtypes = _lib.feecref_hodge_cr_ssor.argtypes
     _lib.feecref_hodge_cr_diagonal.restype = c_int
+    _lib.feecref_hodge_cr_textbook.argtypes = _lib.feecref_hodge_cr_ssor.argtypes
+    _lib.feecref_hodge_cr_textbook.restype = c_int
     _lib.feecref_mesh.argtypes = [c_int, c_int, c_int]
     _lib.feecref_mesh.restype = c_int
     _lib.feecref_mesh_sizes.argtypes = [_P_I]
@@ -148,7 +150,7 @@ def _task(kind, kw):
         b = np.ascontiguousarray(kw["b"], dtype=np.float64)
         res = np.zeros(N)
         ptrs = [(_i if k % 3 < 2 else _d)(m) for k, m in enumerate(mats)]
-        fn = L.feecref_hodge_cr_diagonal if kw.get("inner") == "diagonal" else L.feecref_hodge_cr_ssor
+        fn = getattr(L, "feecref_hodge_cr_" + kw.get("inner", "ssor"))
         K = fn(N, Ldim, _d(x), _d(b), *ptrs, _d(res), float(kw["precision"]),
                int(kw["print_modulo"]), float(kw["inner_precision"]), int(kw["inner_print_modulo"]))
         return x, res, K
@@ -274,9 +276,10 @@ class RefRunner:
     def hodge_cr_ssor(self, N, L, A, B, Bt, C, b, x0=None, precision=1e-10, print_modulo=-3,
                       inner_precision=1e-10, inner_print_modulo=-3, inner="ssor"):
         """HodgeConjugateResidualSolverCSR_SSOR (inner="diagonal":
-        HodgeConjugateResidualSolverCSR_diagonal); matrices are (rows, cols,
+        HodgeConjugateResidualSolverCSR_diagonal, inner="textbook":
+        HodgeConjugateResidualSolverCSR_textbook); matrices are (rows, cols,
         vals) CSR triples.  Returns (x, residual, K)."""
-        if inner not in ("ssor", "diagonal"):
+        if inner not in ("ssor", "diagonal", "textbook"):
             raise ValueError(inner)
         x0 = np.zeros(N) if x0 is None else x0
         return self._call("hodge", N=N, L=L, A=A, B=B, Bt=Bt, C=C, b=b, x0=x0,

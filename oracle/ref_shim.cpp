@@ -200,6 +200,19 @@ int feecref_hodge_cr_ssor(int N, int L, double* x, const double* b, const int* A
                                                 inner_precision, inner_print_modulo);
 }
 
+// HodgeConjugateResidualSolverCSR_textbook (solver/systemsparsesolver.cpp:798-1121).
+int feecref_hodge_cr_textbook(int N, int L, double* x, const double* b, const int* Ar,
+                              const int* Ac, const double* Av, const int* Br, const int* Bc,
+                              const double* Bv, const int* Btr, const int* Btc, const double* Btv,
+                              const int* Cr, const int* Cc, const double* Cv, double* res,
+                              double precision, int print_modulo, double inner_precision,
+                              int inner_print_modulo)
+{
+    return HodgeConjugateResidualSolverCSR_textbook(N, L, x, b, Ar, Ac, Av, Br, Bc, Bv, Btr, Btc,
+                                                    Btv, Cr, Cc, Cv, res, precision, print_modulo,
+                                                    inner_precision, inner_print_modulo);
+}
+
 // HodgeConjugateResidualSolverCSR_diagonal (solver/systemsparsesolver.cpp:52-424).
 int feecref_hodge_cr_diagonal(int N, int L, double* x, const double* b, const int* Ar,
                               const int* Ac, const double* Av, const int* Br, const int* Bc,

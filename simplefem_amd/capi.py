@@ -191,6 +191,9 @@ _SIGNATURES = {
     "sfa_HodgeConjugateResidualSolverCSR_diagonal":
         ([c_int, c_int, _P_D, _P_D] + [_P_I, _P_I, _P_D] * 4 +
          [_P_D, c_double, c_int, c_double, c_int, POINTER(SolveInfo)], c_int),
+    "sfa_HodgeConjugateResidualSolverCSR_textbook":
+        ([c_int, c_int, _P_D, _P_D] + [_P_I, _P_I, _P_D] * 4 +
+         [_P_D, c_double, c_int, c_double, c_int, POINTER(SolveInfo)], c_int),
     "sfa_hodge_create": ([c_int, c_int] + [_P_I, _P_I, _P_D] * 4 + [POINTER(c_void_p)], c_int),
     "sfa_hodge_create_inner": ([c_int, c_int] + [_P_I, _P_I, _P_D] * 4 + [c_int, POINTER(c_void_p)],
                                c_int),
@@ -619,6 +622,7 @@ def _blocks(N, L, A, B, Bt, C):
 
 HODGE_INNER_SSOR = 0
 HODGE_INNER_DIAGONAL = 1
+HODGE_INNER_TEXTBOOK = 2
 
 
 def _hodge_once(entry, N, L, x, b, A, B, Bt, C, residual, precision, print_modulo,
@@ -669,19 +673,32 @@ def HodgeConjugateResidualSolverCSR_diagonal(N, L, x, b, A, B, Bt, C, residual, 
                        inneriteration_print_modulo, info)
 
 
+def HodgeConjugateResidualSolverCSR_textbook(N, L, x, b, A, B, Bt, C, residual, precision,
+                                             print_modulo, inneriteration_precision,
+                                             inneriteration_print_modulo, info=None):
+    """solver/systemsparsesolver.hpp:42-56: the same conjugate residual loop
+    with inner plain CG solves of A.  Returns K."""
+    return _hodge_once("sfa_HodgeConjugateResidualSolverCSR_textbook", N, L, x, b, A, B, Bt, C,
+                       residual, precision, print_modulo, inneriteration_precision,
+                       inneriteration_print_modulo, info)
+
+
 class Hodge:
     """A block system kept on the device between solves (sfa_hodge_*);
     inner="diagonal" solves with inner Jacobi CG as
-    HodgeConjugateResidualSolverCSR_diagonal does."""
+    HodgeConjugateResidualSolverCSR_diagonal does, inner="textbook" with inner
+    plain CG as HodgeConjugateResidualSolverCSR_textbook does."""
+
+    _INNER = {"ssor": 0, "diagonal": 1, "textbook": 2}
 
     def __init__(self, N, L, A, B, Bt, C, inner="ssor"):
-        if inner not in ("ssor", "diagonal"):
-            raise ValueError(f"inner must be 'ssor' or 'diagonal', not {inner!r}")
+        if inner not in self._INNER:
+            raise ValueError(f"inner must be 'ssor', 'diagonal' or 'textbook', not {inner!r}")
         self._h = c_void_p()
         self.N, self.L = int(N), int(L)
         mats = _blocks(self.N, self.L, A, B, Bt, C)
         ptrs = [(_i if k % 3 < 2 else _d)(m) for k, m in enumerate(mats)]
-        kind = HODGE_INNER_SSOR if inner == "ssor" else HODGE_INNER_DIAGONAL
+        kind = self._INNER[inner]
         _check(load().sfa_hodge_create_inner(self.N, self.L, *ptrs, kind, ctypes.byref(self._h)))
 
     def solve(self, x, b, residual, precision=1e-10, print_modulo=-3,

@@ -931,6 +931,22 @@ int HodgeConjugateResidualSolverCSR_diagonal(
     return info.iterations;
 }
 
+int HodgeConjugateResidualSolverCSR_textbook(
+    const int N, const int L, Float* x, const Float* b, const int* Arows, const int* Acolumns,
+    const Float* Avalues, const int* Brows, const int* Bcolumns, const Float* Bvalues,
+    const int* Btrows, const int* Btcolumns, const Float* Btvalues, const int* Crows,
+    const int* Ccolumns, const Float* Cvalues, Float* residual, Float precision, int print_modulo,
+    Float inneriteration_precision, int inneriteration_print_modulo)
+{
+    sfa_solve_info info{};
+    require(sfa_HodgeConjugateResidualSolverCSR_textbook(
+                N, L, x, b, Arows, Acolumns, Avalues, Brows, Bcolumns, Bvalues, Btrows, Btcolumns,
+                Btvalues, Crows, Ccolumns, Cvalues, residual, precision, print_modulo,
+                inneriteration_precision, inneriteration_print_modulo, &info),
+            "HodgeConjugateResidualSolverCSR_textbook");
+    return info.iterations;
+}
+
 int ConjugateGradientSolverCSR_SSOR_Eisenstat(const int N, Float* x, const Float* b,
                                               const int* csrrows, const int* csrcolumns,
                                               const Float* csrvalues, Float* residual,

@@ -1,12 +1,15 @@
 // hodge.hip -- HodgeConjugateResidualSolverCSR_SSOR on the GPU
 // (solver/systemsparsesolver.cpp:426-797), and the same loop as
-// HodgeConjugateResidualSolverCSR_diagonal (:52-424).
+// HodgeConjugateResidualSolverCSR_diagonal (:52-424) and
+// HodgeConjugateResidualSolverCSR_textbook (:798-1121).
 //
 // Conjugate residuals on the Schur complement (B A^-1 Bt + C) x = b of a
 // block system [A Bt; B -C], every A^-1 an inner ConjugateGradientSolverCSR_SSOR
 // (natural-order SSOR, omega 1, the diagonal of A) or, for the diagonal
 // variant, an inner ConjugateGradientSolverCSR_DiagonalPreconditioner with
-// precon = 1 / the diagonal of A (:76-89).  A stays resident as an
+// precon = 1 / the diagonal of A (:76-89), or, for the textbook variant, a
+// plain ConjugateGradientSolverCSR (its Mr.r equals the Md.r here: Mres =
+// Mdir at the restart, :975-977, and both update alike).  A stays resident as an
 // sfa_system (its level schedules and SELL parts built once); B, Bt and C are
 // SELL-64 parts in their own row order.  Each outer iteration is one product
 // Bt Mdir, one inner solve, and two fused row kernels:
@@ -135,7 +138,7 @@ struct HodgeSolver {
     int device = 0;
     hipStream_t stream = nullptr;
     sfa_system inner = nullptr;       // A, natural-order SSOR or Jacobi CG, resident
-    bool diagonal_variant = false;    // inner Jacobi CG (the _diagonal variant)
+    int inner_kind = SFA_HODGE_INNER_SSOR;
     DevPart Bp, Btp, Cp;              // B (N x L), Bt (L x N), C (N x N)
     double *x = nullptr, *b = nullptr, *res = nullptr, *dir = nullptr, *Mdir = nullptr,
            *Mres = nullptr, *aux1 = nullptr, *aux2 = nullptr, *auxR = nullptr;
@@ -166,10 +169,12 @@ struct HodgeSolver {
     void create(int N_, int L_, const int* Ar, const int* Ac, const double* Av, const int* Br,
                 const int* Bc, const double* Bv, const int* Btr, const int* Btc,
                 const double* Btv, const int* Cr, const int* Cc, const double* Cv,
-                int inner_kind)
+                int inner_choice)
     {
-        if (inner_kind != SFA_HODGE_INNER_SSOR && inner_kind != SFA_HODGE_INNER_DIAGONAL)
-            throw Error{SFA_ERR_INVALID, "inner must be SFA_HODGE_INNER_SSOR or _DIAGONAL"};
+        if (inner_choice != SFA_HODGE_INNER_SSOR && inner_choice != SFA_HODGE_INNER_DIAGONAL &&
+            inner_choice != SFA_HODGE_INNER_TEXTBOOK)
+            throw Error{SFA_ERR_INVALID,
+                        "inner must be SFA_HODGE_INNER_SSOR, _DIAGONAL or _TEXTBOOK"};
         if (N_ <= 0 || L_ <= 0) throw Error{SFA_ERR_INVALID, "N and L must be positive"};
         // A block without entries (the zero matrix C of the reference's Darcy
         // tests) may come with null column and value arrays.
@@ -191,7 +196,8 @@ struct HodgeSolver {
         block(N_, N_, Cr, Cc, Cv, "C");
         N = N_;
         L = L_;
-        diagonal_variant = inner_kind == SFA_HODGE_INNER_DIAGONAL;
+        inner_kind = inner_choice;
+        const bool diagonal_variant = inner_choice == SFA_HODGE_INNER_DIAGONAL;
         // the diagonal of A as the reference sums it (:498-511, :76-83); the
         // diagonal variant inverts it (:85, no guard: the reference asserts > 0)
         std::vector<double> diag(L, 0.);
@@ -203,8 +209,12 @@ struct HodgeSolver {
         ensure_device();
         SFA_HIP(hipGetDevice(&device));
         SFA_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
-        check(sfa_system_create(diagonal_variant ? SFA_METHOD_DIAGONAL : SFA_METHOD_SSOR, L, Ar, Ac,
-                                Av, diag.data(), 1.0, 0, nullptr, nullptr, nullptr, &inner));
+        const int method = inner_choice == SFA_HODGE_INNER_SSOR       ? SFA_METHOD_SSOR
+                           : inner_choice == SFA_HODGE_INNER_DIAGONAL ? SFA_METHOD_DIAGONAL
+                                                               : SFA_METHOD_CG;
+        check(sfa_system_create(method, L, Ar, Ac, Av,
+                                method == SFA_METHOD_CG ? nullptr : diag.data(), 1.0, 0, nullptr,
+                                nullptr, nullptr, &inner));
         check(sfa_system_set_stream(inner, stream));
         SellPart part;
         build_full_identity(part, N, L, Br, Bc, Bv);
@@ -286,15 +296,18 @@ struct HodgeSolver {
         tol = precision * std::sqrt(tol);
         if (!(tol >= kDesiredPrecision) && tol <= kDesiredPrecision) tol = kDesiredPrecision;
 
-        // the diagonal variant gates its breakdown, warning and finish lines
-        // by print_modulo (:91-96, :280-286, :375); the SSOR one prints the
-        // first two always and the last from print_modulo >= 0 (:465-470, :746)
-        const bool dv = diagonal_variant;
+        // the diagonal and textbook variants gate their breakdown, warning and
+        // finish lines by print_modulo (:91-96, :280-286, :375); the SSOR one
+        // prints the first two always and the last from print_modulo >= 0
+        // (:465-470, :746)
+        const bool dv = inner_kind != SFA_HODGE_INNER_SSOR;
         const bool print_hard = !dv || print_modulo >= -2;
         const bool print_finish = dv ? print_modulo >= -1 : print_modulo >= 0;
         if (print_modulo >= -1)
             std::printf("(%d/%d)     BEGIN: Hodge Conjugate Residual (%s) CSR\n", 0, N,
-                        dv ? "diag" : "SSOR");
+                        inner_kind == SFA_HODGE_INNER_SSOR       ? "SSOR"
+                        : inner_kind == SFA_HODGE_INNER_DIAGONAL ? "diag"
+                                                                 : "textbook");
         double Md_r = NAN, Md_Md = NAN;
         int K = 0, reason = SFA_EXIT_CAP;
         const double tol2 = tol * tol;
@@ -471,6 +484,20 @@ int sfa_HodgeConjugateResidualSolverCSR(
         N, L, x, b, Arows, Acolumns, Avalues, Brows, Bcolumns, Bvalues, Btrows, Btcolumns,
         Btvalues, Crows, Ccolumns, Cvalues, residual, precision, print_modulo,
         inneriteration_precision, inneriteration_print_modulo, info);
+}
+
+int sfa_HodgeConjugateResidualSolverCSR_textbook(
+    int N, int L, double* x, const double* b, const int* Arows, const int* Acolumns,
+    const double* Avalues, const int* Brows, const int* Bcolumns, const double* Bvalues,
+    const int* Btrows, const int* Btcolumns, const double* Btvalues, const int* Crows,
+    const int* Ccolumns, const double* Cvalues, double* residual, double precision,
+    int print_modulo, double inneriteration_precision, int inneriteration_print_modulo,
+    sfa_solve_info* info)
+{
+    return hodge_once(SFA_HODGE_INNER_TEXTBOOK, N, L, x, b, Arows, Acolumns, Avalues, Brows,
+                      Bcolumns, Bvalues, Btrows, Btcolumns, Btvalues, Crows, Ccolumns, Cvalues,
+                      residual, precision, print_modulo, inneriteration_precision,
+                      inneriteration_print_modulo, info);
 }
 
 int sfa_HodgeConjugateResidualSolverCSR_diagonal(

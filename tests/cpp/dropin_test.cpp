@@ -251,6 +251,14 @@ static void hodge_checks()
     CHECK((xd - want).norm() < 1e-6 * want.norm());
     std::printf("HodgeConjugateResidualSolverCSR_diagonal: K=%d, |x - A b| / |A b| = %.2e\n", Kd,
                 (xd - want).norm() / want.norm());
+    FloatVector xt(n, 0.), rt(n, 0.);
+    const int Kt = HodgeConjugateResidualSolverCSR_textbook(
+        n, n, xt.raw(), b.raw(), A.getA(), A.getC(), A.getV(), I.getA(), I.getC(), I.getV(),
+        I.getA(), I.getC(), I.getV(), Z.getA(), Z.getC(), Z.getV(), rt.raw(), desired_precision,
+        -1, desired_precision, -3);
+    CHECK((xt - want).norm() < 1e-6 * want.norm());
+    std::printf("HodgeConjugateResidualSolverCSR_textbook: K=%d, |x - A b| / |A b| = %.2e\n", Kt,
+                (xt - want).norm() / want.norm());
 }
 
 int main(int argc, char** argv)

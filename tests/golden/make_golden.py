@@ -15,8 +15,8 @@ The reference's refined unit square / unit cube and its P1 stiffness
 matrices are kept in full for small levels (mesh_*.npz) and as SHA-256
 digests of the CSR structure at the full-size levels (2D level 7, 3D level 6).
 Block-system solves of HodgeConjugateResidualSolverCSR_SSOR keep K, x and
-residual in full (hodge_*.npz, and hodge_diagonal_*.npz for
-HodgeConjugateResidualSolverCSR_diagonal); `--hodge-only` adds just those to an existing
+residual in full (hodge_*.npz, and hodge_{diagonal,textbook}_*.npz for
+HodgeConjugateResidualSolverCSR_diagonal and _textbook); `--hodge-only` adds just those to an existing
 golden.json.  The reference's lowest-order Whitney 1-form curl-curl and mass
 matrices on the refined unit cube (FEEC inclusion, broken exterior derivative
 and broken mass matrices, tangential Dirichlet edges) are kept in full with the
@@ -116,15 +116,19 @@ def hodge_fixtures(ref, meta):
         x, r, K = ref.hodge_cr_ssor(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, b, precision=1e-10,
                                     inner_precision=1e-12)
         np.savez_compressed(os.path.join(OUT, "hodge_" + name + ".npz"), x=x, res=r)
-        # HodgeConjugateResidualSolverCSR_diagonal on the same system
-        xd, rd, Kd = ref.hodge_cr_ssor(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, b, precision=1e-10,
-                                       inner_precision=1e-12, inner="diagonal")
-        np.savez_compressed(os.path.join(OUT, "hodge_diagonal_" + name + ".npz"), x=xd, res=rd)
+        # the _diagonal and _textbook variants on the same system
+        ks = {}
+        for inner in ("diagonal", "textbook"):
+            xv, rv, ks[inner] = ref.hodge_cr_ssor(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, b,
+                                                  precision=1e-10, inner_precision=1e-12,
+                                                  inner=inner)
+            np.savez_compressed(os.path.join(OUT, f"hodge_{inner}_{name}.npz"), x=xv, res=rv)
         meta["hodge"][name] = {"m": m, "coarse": coarse, "c_weight": cw, "seed": seed,
                                "rhs_seed": HODGE_RHS_SEED, "precision": 1e-10,
                                "inner_precision": 1e-12, "iterations": int(K),
-                               "iterations_diagonal": int(Kd)}
-        print("hodge", name, K, Kd, flush=True)
+                               "iterations_diagonal": int(ks["diagonal"]),
+                               "iterations_textbook": int(ks["textbook"])}
+        print("hodge", name, K, ks, flush=True)
 
 
 # refinement levels of UnitCube3D whose Whitney 1-form matrices are kept

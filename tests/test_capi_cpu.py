@@ -89,7 +89,8 @@ def test_invalid_blocks_are_rejected_before_the_device():
                                                       -3, 1e-10, -3)
         assert e.value.code == -1 and f": {word}: column" in str(e.value), str(e.value)
     # the plain and diagonal variants run the same host checks
-    for fn in (capi.HodgeConjugateResidualSolverCSR, capi.HodgeConjugateResidualSolverCSR_diagonal):
+    for fn in (capi.HodgeConjugateResidualSolverCSR, capi.HodgeConjugateResidualSolverCSR_diagonal,
+               capi.HodgeConjugateResidualSolverCSR_textbook):
         with pytest.raises(capi.SfaError) as e:
             fn(bs.N, bs.L, np.zeros(bs.N), np.ones(bs.N), bs.A, bad_B, bs.Bt, bs.C,
                np.zeros(bs.N), 1e-10, -3, 1e-10, -3)

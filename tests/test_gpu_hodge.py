@@ -116,22 +116,24 @@ def test_hodge_edge_cases_match_oracle():
 
 
 # HodgeConjugateResidualSolverCSR_diagonal (solver/systemsparsesolver.cpp:52-424, inner
-# Jacobi CG) and HodgeConjugateResidualSolverCSR (:18-49, the SSOR variant), same bars.
+# Jacobi CG), HodgeConjugateResidualSolverCSR_textbook (:798-1121, inner plain CG) and
+# HodgeConjugateResidualSolverCSR (:18-49, the SSOR variant), same bars.
 
+@pytest.mark.parametrize("inner", ["diagonal", "textbook"])
 @pytest.mark.parametrize("case", CASES, ids=[f"m{c[0]}_c{c[1]}_w{c[2]}" for c in CASES])
-def test_hodge_diagonal_one_shot_matches_oracle(case):
+def test_hodge_variant_one_shot_matches_oracle(case, inner):
     bs = S.schur_blocks(*case)
     b = S.uniform_pm1(5, bs.N)
-    print(f"[hodge diagonal] {bs.name} N={bs.N} L={bs.L}", flush=True)
+    print(f"[hodge {inner}] {bs.name} N={bs.N} L={bs.L}", flush=True)
     xo, ro, io = oracle.hodge_cr_ssor(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, b, precision=1e-10,
-                                      inner_precision=1e-12, inner="diagonal")
+                                      inner_precision=1e-12, inner=inner)
     x = np.zeros(bs.N)
     r = np.zeros(bs.N)
     info = {}
-    K = capi.HodgeConjugateResidualSolverCSR_diagonal(bs.N, bs.L, x, b, bs.A, bs.B, bs.Bt, bs.C,
-                                                      r, 1e-10, -3, 1e-12, -3, info=info)
+    fn = getattr(capi, "HodgeConjugateResidualSolverCSR_" + inner)
+    K = fn(bs.N, bs.L, x, b, bs.A, bs.B, bs.Bt, bs.C, r, 1e-10, -3, 1e-12, -3, info=info)
     assert K == info["iterations"]
-    print(f"[hodge diagonal] K gpu {K} oracle {io['iterations']}", flush=True)
+    print(f"[hodge {inner}] K gpu {K} oracle {io['iterations']}", flush=True)
     _check(bs, b, x, info, xo, io)
 
 
@@ -147,15 +149,16 @@ def test_hodge_plain_variant_is_the_ssor_one():
     assert Ks == Kp and np.array_equal(xs, xp) and np.array_equal(rs, rp)
 
 
-def test_hodge_diagonal_handle_matches_one_shot():
+@pytest.mark.parametrize("inner", ["diagonal", "textbook"])
+def test_hodge_variant_handle_matches_one_shot(inner):
     bs = S.schur_blocks(32, 2, 0.25, 11)
     b = S.uniform_pm1(8, bs.N)
     x1, r1 = np.zeros(bs.N), np.zeros(bs.N)
-    K1 = capi.HodgeConjugateResidualSolverCSR_diagonal(bs.N, bs.L, x1, b, bs.A, bs.B, bs.Bt, bs.C,
-                                                       r1, 1e-10, -3, 1e-12, -3)
+    K1 = getattr(capi, "HodgeConjugateResidualSolverCSR_" + inner)(
+        bs.N, bs.L, x1, b, bs.A, bs.B, bs.Bt, bs.C, r1, 1e-10, -3, 1e-12, -3)
     xo, _, io = oracle.hodge_cr_ssor(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, b, precision=1e-10,
-                                     inner_precision=1e-12, inner="diagonal")
-    with capi.Hodge(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, inner="diagonal") as h:
+                                     inner_precision=1e-12, inner=inner)
+    with capi.Hodge(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, inner=inner) as h:
         for _ in range(2):
             x = np.zeros(bs.N)
             r = np.zeros(bs.N)

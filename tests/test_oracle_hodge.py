@@ -1,6 +1,7 @@
 """Pins the oracle's HodgeConjugateResidualSolverCSR_SSOR
 (solver/systemsparsesolver.cpp:426-797) and HodgeConjugateResidualSolverCSR_diagonal
-(:52-424, inner Jacobi CG) before they check the GPU path.
+(:52-424, inner Jacobi CG) and HodgeConjugateResidualSolverCSR_textbook
+(:798-1121, inner plain CG) before they check the GPU path.
 
   * golden fixtures from the compiled reference (K, x and residual bit for bit);
   * mathematics: the dense Schur complement solve, and with B = Bt = I and
@@ -27,7 +28,7 @@ def _case(name):
     return e, bs, b
 
 
-INNER = ["ssor", "diagonal"]
+INNER = ["ssor", "diagonal", "textbook"]
 
 
 @pytest.mark.parametrize("inner", INNER)
@@ -37,9 +38,9 @@ def test_oracle_matches_reference_fixtures(name, inner):
     x, res, info = oracle.hodge_cr_ssor(bs.N, bs.L, bs.A, bs.B, bs.Bt, bs.C, b,
                                         precision=e["precision"],
                                         inner_precision=e["inner_precision"], inner=inner)
-    tag = "" if inner == "ssor" else "diagonal_"
+    tag = "" if inner == "ssor" else inner + "_"
     g = np.load(os.path.join(GOLDEN, f"hodge_{tag}{name}.npz"))
-    assert info["iterations"] == e["iterations" if inner == "ssor" else "iterations_diagonal"]
+    assert info["iterations"] == e["iterations" if inner == "ssor" else "iterations_" + inner]
     assert np.array_equal(x, g["x"]) and np.array_equal(res, g["res"])
 
 
