@@ -142,6 +142,37 @@ int sfa_ConjugateGradientSolverCSR_Eisenstat_Rainbow(
     sfa_solve_info* info);
 
 /* ---------------------------------------------------------------------------
+ * No-allocation solve.  No counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The arguments of
+ * ConjugateGradientSolverCSR_Rainbow, all arrays DEVICE pointers on the
+ * current GPU (x, residual in/out as in the reference; B is also read by the
+ * host).  The solve runs on the caller's CSR arrays in their own row order
+ * on the null stream and returns when it has finished.  Its Krylov vectors,
+ * z.r history, reduction partials and scalars live in one static device
+ * array of SFA_NOALLOC_SCRATCH_BYTES that each GPU holds from the first call
+ * until the process ends; no hipMalloc, hipFree or other allocating routine
+ * is called.  Calls on one GPU share that array: calls that overlap on one
+ * GPU race and must not be made.  N > SFA_NOALLOC_MAX_ROWS or num_colors >
+ * SFA_NOALLOC_MAX_COLORS is rejected with SFA_ERR_INVALID; the CSR structure
+ * and the colouring are checked on the device before any value is read
+ * (SFA_ERR_INVALID / SFA_ERR_COLOURING).  Same K, x and residual as
+ * sfa_ConjugateGradientSolverCSR_Rainbow up to the summation order of the
+ * dot products.
+ * ------------------------------------------------------------------------- */
+#define SFA_NOALLOC_MAX_ROWS    1048576
+#define SFA_NOALLOC_MAX_COLORS  1024
+#define SFA_NOALLOC_SCRATCH_BYTES (42u << 20)  /* 42 MiB per GPU */
+
+int sfa_ConjugateGradientSolverCSR_Rainbow_noalloc(
+    int N, double* x, const double* b,
+    const int* csrrows, const int* csrcolumns, const double* csrvalues,
+    double* residual, double precision, int print_modulo,
+    const double* diagonal, double omega,
+    int num_colors, const int* F, const int* B, const int* R,
+    sfa_solve_info* info);
+
+/* ---------------------------------------------------------------------------
  * CSR operations
  * ------------------------------------------------------------------------- */
 

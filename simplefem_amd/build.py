@@ -33,6 +33,7 @@ SOURCES = [  # (source, compile as HIP)
     ("kernels.hip", True),
     ("batch.hip", True),
     ("hodge.hip", True),
+    ("noalloc.hip", True),
     ("system.cpp", True),
     ("plan.cpp", False),
 ]

@@ -145,6 +145,9 @@ _SIGNATURES = {
         (_SOLVER_ARGS + [_P_D, c_double, c_int, _P_I, _P_I, _P_I, POINTER(SolveInfo)], c_int),
     "sfa_ConjugateGradientSolverCSR_Eisenstat_Rainbow":
         (_SOLVER_ARGS + [_P_D, c_double, c_int, _P_I, _P_I, _P_I, POINTER(SolveInfo)], c_int),
+    "sfa_ConjugateGradientSolverCSR_Rainbow_noalloc":
+        ([c_int] + [c_void_p] * 5 + [c_void_p, c_double, c_int, c_void_p, c_double, c_int]
+         + [c_void_p] * 3 + [POINTER(SolveInfo)], c_int),
     "sfa_MatrixCSR_apply": ([c_int, c_int, _P_I, _P_I, _P_D, _P_D, _P_D, c_double], c_int),
     "sfa_MatrixCSR_getDiagonal": ([c_int, _P_I, _P_I, _P_D, _P_D], c_int),
     "sfa_Rainbow": ([c_int, _P_I, _P_I, _P_I, _P_I, _P_I, _P_I], c_int),
@@ -334,6 +337,31 @@ def ConjugateGradientSolverCSR_Eisenstat_Rainbow(N, x, b, csrrows, csrcolumns, c
     return _solve("sfa_ConjugateGradientSolverCSR_Eisenstat_Rainbow", N, x, b, csrrows,
                   csrcolumns, csrvalues, residual, precision, print_modulo,
                   [_d(diagonal), float(omega), int(num_colors), _i(F), _i(B), _i(R)], info)
+
+
+NOALLOC_MAX_ROWS = 1048576
+NOALLOC_MAX_COLORS = 1024
+
+
+def ConjugateGradientSolverCSR_Rainbow_noalloc(N, x, b, csrrows, csrcolumns, csrvalues, residual,
+                                               precision, print_modulo, diagonal, omega,
+                                               num_colors, F, B, R, info=None):
+    """sfa_ConjugateGradientSolverCSR_Rainbow_noalloc: the arguments of
+    ConjugateGradientSolverCSR_Rainbow (solver/sparsesolver.hpp:59-70) as device
+    buffers (objects with a ``ptr`` attribute, e.g. hipmem.DeviceArray: float64
+    x, b, csrvalues, residual, diagonal; int32 csrrows, csrcolumns, F, B, R).
+    The library allocates no device memory for it (its workspace is a static
+    device array).  Returns K; x and residual are updated in place."""
+    out = SolveInfo()
+    lib = load()
+    p = lambda a: a.ptr if a is not None else None  # noqa: E731
+    _check(lib.sfa_ConjugateGradientSolverCSR_Rainbow_noalloc(
+        int(N), p(x), p(b), p(csrrows), p(csrcolumns), p(csrvalues), p(residual),
+        float(precision), int(print_modulo), p(diagonal), float(omega), int(num_colors),
+        p(F), p(B), p(R), ctypes.byref(out)))
+    if info is not None:
+        info.update(out.as_dict())
+    return out.iterations
 
 
 # --------------------------------------------------------------------------- CSR operations
