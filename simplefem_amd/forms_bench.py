@@ -254,14 +254,6 @@ def batched(args):
                  "f32+f64")
 
 
-def _prepare(job):
-    """Colouring and device-order plan of a request's matrix (host only)."""
-    s = job.system
-    rb = capi.Rainbow(s.n, s.A, s.C)
-    return capi.PreparedMatrix(s.n, s.A, s.C, s.V, s.diagonal(), 1.0, rb.num_colors, rb.F, rb.B,
-                               rb.R)
-
-
 def _served_fp64_one_at_a_time(jobs):
     """The plain FP64 path on the same stream: requests in arrival order, one
     device-resident System each (colouring, plan, upload, solve)."""
@@ -288,27 +280,11 @@ def _served_fp64_one_at_a_time(jobs):
     return time.perf_counter() - start, np.array(lat)
 
 
-def _solve_dedicated(job, inner_rtol, start, inner_restarts=False):
-    """A large request on its own device-resident System and stream: colouring,
-    plan, upload and the mixed-precision solve, on a worker thread while the
-    batch engine runs.  Returns the times (seconds after start) at which the
-    worker took it, the System was ready, and the solve had finished."""
-    t_take = time.perf_counter() - start
-    s = job.system
-    rb = capi.Rainbow(s.n, s.A, s.C)
-    with capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, s.diagonal(), 1.0, rb.num_colors,
-                     rb.F, rb.B, rb.R, fp32_inner=True, inner_restarts=inner_restarts) as sysd:
-        t_ready = time.perf_counter() - start
-        bt = DeviceArray.from_host(job.rhs[0])
-        x = DeviceArray(s.n)
-        sysd.solve_mixed(x, bt, 1e-10, inner_rtol=inner_rtol, max_outer=50)  # returns when done
-        bt.free()
-        x.free()
-    return t_take, t_ready, time.perf_counter() - start
-
-
 def served(args, budget=200e9):
-    from concurrent.futures import ThreadPoolExecutor
+    """The served stream through serving.SolveServer (admission under one HBM
+    budget for both routes, continuous batching, every result checked),
+    against the plain FP64 path on the same stream."""
+    from . import serving
 
     t0 = time.time()
 
@@ -318,91 +294,48 @@ def served(args, budget=200e9):
 
     jobs = W.served_requests(args.requests, args.rate, dofs=(1e3, 2e6), progress=progress)
     log(f"{len(jobs)} requests generated ({time.time() - t0:.1f}s)")
-    lat = []
-    dedicated = []  # (job, future) of requests routed to their own System
-    with capi.Batch(mixed=True, max_slots=256, inner_restarts=args.inner_restarts) as eng, \
-            ThreadPoolExecutor(max_workers=args.prep_workers) as pool, \
-            ThreadPoolExecutor(max_workers=max(1, args.dedicated_workers)) as big_pool:
-        arrivals = list(jobs)  # not yet arrived
-        preparing = []         # (job, future), arrival order
-        inflight = {}
-        start = time.perf_counter()
-        last_log = 0.0
-        done = 0
-        while arrivals or preparing or inflight:
-            now = time.perf_counter() - start
-            while arrivals and arrivals[0].arrival <= now:  # a request is seen once it arrives
-                j = arrivals.pop(0)
-                if args.dedicated_workers > 0 and j.system.n >= args.route_rows:
-                    dedicated.append((j, big_pool.submit(_solve_dedicated, j, args.inner_rtol,
-                                                         start, args.inner_restarts)))
-                else:
-                    preparing.append((j, pool.submit(_prepare, j)))
-            while preparing and preparing[0][1].done():  # admit in arrival order, within budget
-                j, fut = preparing[0]
-                s = j.system
-                need = 40.0 * s.nnz + 80.0 * s.n
-                if inflight and eng.memory() + need > budget:
-                    break
-                prep = fut.result()
-                mid = eng.add_prepared(prep)
-                prep.close()
-                slot = eng.submit(mid, j.rhs[0], precision=1e-10, inner_rtol=args.inner_rtol)
-                inflight[slot] = (j, mid)
-                preparing.pop(0)
-            if inflight:
-                eng.run(128)
-                if time.perf_counter() - start - last_log > 5.0:
-                    last_log = time.perf_counter() - start
-                    log(f"served {done} of {len(jobs)} after {last_log:.1f}s, "
-                        f"{len(inflight)} in flight, {len(preparing)} being prepared")
-            elif preparing:
-                time.sleep(0.0005)
-            elif arrivals:
-                time.sleep(max(0.0, arrivals[0].arrival - (time.perf_counter() - start)))
-            for slot in eng.finished():
-                j, mid = inflight.pop(slot)
-                x, info = eng.collect(slot, j.system.n)
-                lat.append(time.perf_counter() - start - j.arrival)
-                eng.remove_matrix(mid)
-                done += 1
-        phases = []  # per large request: wait for a worker, create (colour, plan, upload), solve
-        for j, fut in dedicated:  # the batch is drained; wait for the large requests
-            t_take, t_ready, t_done = fut.result()
-            lat.append(t_done - j.arrival)
-            phases.append((t_take - j.arrival, t_ready - t_take, t_done - t_ready))
-            done += 1
-        total = time.perf_counter() - start
-    lat = np.array(lat)
-    log(f"served {done} in {total:.2f}s")
+    reqs = [serving.Request(j.system, j.rhs[0], j.arrival, k) for k, j in enumerate(jobs)]
+    srv = serving.SolveServer(budget_bytes=budget, max_slots=256, route_rows=args.route_rows,
+                              dedicated_workers=args.dedicated_workers,
+                              prep_workers=args.prep_workers, precision=1e-10,
+                              inner_rtol=args.inner_rtol, inner_restarts=args.inner_restarts)
+    results = srv.serve(reqs, log=log)
+    summ = serving.summary(results, srv.stats.makespan)
+    log(f"served: {summ}")
     if args.no_served_baseline:
         total64, lat64 = float("nan"), np.array([np.nan])
     else:
         log("plain FP64 path on the same stream ...")
         total64, lat64 = _served_fp64_one_at_a_time(jobs)
+    phases = np.array(srv.stats.dedicated_phases) if srv.stats.dedicated_phases else None
+    worst = max((r.residual_norm / r.target for r in results if r.status == "ok"), default=None)
     cfg = {"workload": f"{len(jobs)} requests, Poisson arrivals at {args.rate}/s, 2D/3D, "
                        "log-uniform 1e3..2e6 unknowns, Lagrange degree 1..3, one RHS each, "
-                       "mixed precision to rtol 1e-10, 200 GB HBM admission budget; colouring "
-                       "and plan built on host worker threads after arrival; requests of at "
-                       f"least {args.route_rows} rows on their own System and stream",
-           "requests": len(jobs), "p50_latency_ms": float(np.percentile(lat, 50) * 1e3),
-           "p99_latency_ms": float(np.percentile(lat, 99) * 1e3),
-           "makespan_s": total, "prep_workers": args.prep_workers,
+                       "mixed precision to rtol 1e-10, 200 GB HBM admission budget over both "
+                       "routes (simplefem_amd.serving.SolveServer); colouring and plan built on "
+                       "host worker threads after arrival; requests of at least "
+                       f"{args.route_rows} rows on their own System and stream",
+           "requests": len(jobs), "ok": summ["ok"], "failed": summ["failed"],
+           "rejected": summ["rejected"],
+           "max_residual_over_target": worst,
+           "p50_latency_ms": summ["p50_latency_ms"], "p99_latency_ms": summ["p99_latency_ms"],
+           "makespan_s": summ["makespan_s"], "prep_workers": args.prep_workers,
+           "peak_admitted_bytes": srv.stats.peak_bytes,
            "dedicated": {"rows_at_least": args.route_rows, "workers": args.dedicated_workers,
-                         "requests": len(dedicated),
+                         "requests": srv.stats.dedicated,
                          "mean_s": dict(zip(("wait", "create", "solve"),
-                                            np.mean(phases, axis=0).round(3).tolist()))
-                         if phases else {},
+                                            phases.mean(axis=0).round(3).tolist()))
+                         if phases is not None else {},
                          "max_s": dict(zip(("wait", "create", "solve"),
-                                           np.max(phases, axis=0).round(3).tolist()))
-                         if phases else {}},
+                                           phases.max(axis=0).round(3).tolist()))
+                         if phases is not None else {}},
            "fp64_one_at_a_time": {"solves_per_s": len(jobs) / total64,
                                   "p50_latency_ms": float(np.percentile(lat64, 50) * 1e3),
                                   "p99_latency_ms": float(np.percentile(lat64, 99) * 1e3),
                                   "makespan_s": total64},
-           "speedup_over_fp64_one_at_a_time": (done / total) / (len(jobs) / total64)}
-    return _base(args, round(done / total, 3), "solves/s", round(total / done * 1e3, 3), cfg,
-                 "f32+f64")
+           "speedup_over_fp64_one_at_a_time": summ["solves_per_s"] / (len(jobs) / total64)}
+    return _base(args, round(summ["solves_per_s"], 3), "solves/s",
+                 round(summ["makespan_s"] / max(1, len(jobs)) * 1e3, 3), cfg, "f32+f64")
 
 
 def run(args):
