@@ -7,10 +7,10 @@
 // inverse many times.  Here the matrix, its colouring and the solve's
 // schedules stay on the GPU between applications; each apply copies src in
 // and the result out.  The operator is read at every apply as the reference
-// does: when its entries have changed since the last one (a 64-bit digest of
-// the CSR arrays), the device copy is rebuilt.  Only MatrixCSR operators
-// have this inverse; the reference's generic one (a conjugate residual
-// method on any LinearOperator) is not part of this path.
+// does: when its arrays differ from the ones the device copy was built from
+// (a host copy of them, compared byte for byte), the device copy is rebuilt.
+// Only MatrixCSR operators have this inverse; the reference's generic one (a
+// conjugate residual method on any LinearOperator) is not part of this path.
 #pragma once
 
 #include <cstdint>

@@ -11,6 +11,7 @@
 #include <memory>
 #include <string>
 #include <cstdlib>
+#include <cstring>
 #include <numeric>
 #include <random>
 
@@ -995,36 +996,33 @@ int ConjugateGradientSolverCSR_Eisenstat_Rainbow(const int N, Float* x, const Fl
 
 struct InverseOperator<MatrixCSR>::Device {
     sfa_system sys = nullptr;
-    uint64_t digest = 0;
+    // the arrays the device copy was built from, to see whether the operator
+    // has changed since (compared with memcmp: exact, at memory bandwidth)
+    int rows = -1, cols = -1;
+    std::vector<int> A, C;
+    std::vector<Float> V;
     ~Device()
     {
         if (sys) (void)sfa_system_destroy(sys);
     }
+    bool same_as(const MatrixCSR& m) const
+    {
+        const int r = m.getdimout(), c = m.getdimin(), nnz = m.getnumberofentries();
+        return r == rows && c == cols && (size_t)nnz == C.size() &&
+               std::memcmp(A.data(), m.getA(), sizeof(int) * (size_t)(r + 1)) == 0 &&
+               std::memcmp(C.data(), m.getC(), sizeof(int) * (size_t)nnz) == 0 &&
+               std::memcmp(V.data(), m.getV(), sizeof(Float) * (size_t)nnz) == 0;
+    }
+    void keep(const MatrixCSR& m)
+    {
+        rows = m.getdimout();
+        cols = m.getdimin();
+        const int nnz = m.getnumberofentries();
+        A.assign(m.getA(), m.getA() + rows + 1);
+        C.assign(m.getC(), m.getC() + nnz);
+        V.assign(m.getV(), m.getV() + nnz);
+    }
 };
-
-namespace {
-
-// 64-bit FNV-1a digest of a CSR matrix's dimensions and arrays.
-uint64_t csr_digest(const MatrixCSR& m)
-{
-    uint64_t h = 1469598103934665603ull;
-    auto eat = [&](const void* p, size_t bytes) {
-        const unsigned char* c = static_cast<const unsigned char*>(p);
-        for (size_t i = 0; i < bytes; i++) {
-            h ^= c[i];
-            h *= 1099511628211ull;
-        }
-    };
-    const int rows = m.getdimout(), cols = m.getdimin(), nnz = m.getnumberofentries();
-    eat(&rows, sizeof rows);
-    eat(&cols, sizeof cols);
-    eat(m.getA(), sizeof(int) * (size_t)(rows + 1));
-    eat(m.getC(), sizeof(int) * (size_t)nnz);
-    eat(m.getV(), sizeof(Float) * (size_t)nnz);
-    return h;
-}
-
-}  // namespace
 
 InverseOperator<MatrixCSR>::InverseOperator(const MatrixCSR& op_, Float precision_, int print_modulo_)
     : LinearOperator(op_.getdimout(), op_.getdimin()),
@@ -1094,8 +1092,7 @@ void InverseOperator<MatrixCSR>::apply(FloatVector& dest, const FloatVector& src
     else dest.zero();
     // the reference reads the operator at every apply: rebuild the device copy
     // when its arrays have changed
-    const uint64_t digest = csr_digest(op);
-    if (!device_ || device_->digest != digest) {
+    if (!device_ || !device_->same_as(op)) {
         device_.reset();
         std::unique_ptr<Device> dev(new Device());
         const FloatVector diagonal = op.getDiagonal();
@@ -1104,7 +1101,7 @@ void InverseOperator<MatrixCSR>::apply(FloatVector& dest, const FloatVector& src
                                   op.getV(), diagonal.raw(), 1.0, rainbow.num_colors,
                                   rainbow.F.data(), rainbow.B.data(), rainbow.R.data(), &dev->sys),
                 "InverseOperator: sfa_system_create");
-        dev->digest = digest;
+        dev->keep(op);
         device_ = std::move(dev);
     }
     FloatVector res(dest);

@@ -216,6 +216,9 @@ struct HodgeSolver {
                                 method == SFA_METHOD_CG ? nullptr : diag.data(), 1.0, 0, nullptr,
                                 nullptr, nullptr, &inner));
         check(sfa_system_set_stream(inner, stream));
+        // each inner tolerance from ||Bt v|| summed in row order, as the
+        // reference's inner call sums it (solver/sparsesolver.cpp:469-472)
+        system_row_order_tolerance(inner, true);
         SellPart part;
         build_full_identity(part, N, L, Br, Bc, Bv);
         Bp.upload(part);

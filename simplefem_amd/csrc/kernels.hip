@@ -173,6 +173,34 @@ __global__ void __launch_bounds__(kVecBlock) k_sumsq(const T* b, int n, double* 
     block_partial<kVecBlock>(s, &partials[blockIdx.x]);
 }
 
+// sum b^2 over b[0..n) in row order (solver/sparsesolver.cpp:469-472 exactly,
+// so the tolerance has the reference's bits): the block stages 2048 squares
+// in LDS, then one lane adds them in order.  One block; for callers whose
+// vectors are short (the inner solves of the block solvers).
+template <typename T>
+__global__ void __launch_bounds__(kVecBlock) k_sumsq_rows(const T* __restrict__ b, int n,
+                                                         double* out)
+{
+    constexpr int kChunk = 8 * kVecBlock;
+    __shared__ double sq[kChunk];
+    double t = 0.;
+    for (int i0 = 0; i0 < n; i0 += kChunk) {
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int k = u * kVecBlock + threadIdx.x;
+            const double v = i0 + k < n ? (double)b[i0 + k] : 0.;
+            sq[k] = v * v;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            const int cnt = min(kChunk, n - i0);
+            for (int k = 0; k < cnt; k++) t += sq[k];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = t;
+}
+
 // State reset and tolerance: tol = max(precision * sqrt(sum b^2), desired_precision).
 __global__ void __launch_bounds__(kWave) k_init_state(SolveState* st, const double* partials,
                                                       int np, double precision, double host_tol,
@@ -1047,11 +1075,16 @@ void launch_scatter(hipStream_t s, T* dst, const T* src, const int* perm, int n)
 template <typename T>
 void launch_init_state(hipStream_t s, SolveState* st, const T* b, int n, double precision,
                        double host_tol, int use_host_tol, int maxK, int small_breaks, double rel2,
-                       double* partials)
+                       double* partials, const T* rows_b)
 {
-    const int grid = vec_reduce_blocks(n);
-    if (!use_host_tol) launch(k_sumsq<T>, dim3(grid), dim3(kVecBlock), s, b, n, partials);
-    launch(k_init_state, dim3(1), dim3(kWave), s, st, partials, grid, precision, host_tol, use_host_tol, maxK,
+    int np = vec_reduce_blocks(n);
+    if (!use_host_tol && rows_b) {  // the reference's row order
+        launch(k_sumsq_rows<T>, dim3(1), dim3(kVecBlock), s, rows_b, n, partials);
+        np = 1;
+    } else if (!use_host_tol) {
+        launch(k_sumsq<T>, dim3(np), dim3(kVecBlock), s, b, n, partials);
+    }
+    launch(k_init_state, dim3(1), dim3(kWave), s, st, partials, np, precision, host_tol, use_host_tol, maxK,
                                      small_breaks, rel2);
 }
 
@@ -1253,7 +1286,7 @@ void launch_add_correction(hipStream_t s, double* x, const float* e, int n)
     template void launch_gather<T>(hipStream_t, T*, const T*, const int*, int);                \
     template void launch_scatter<T>(hipStream_t, T*, const T*, const int*, int);               \
     template void launch_init_state<T>(hipStream_t, SolveState*, const T*, int, double, double, \
-                                       int, int, int, double, double*);                        \
+                                       int, int, int, double, double*, const T*);              \
     template void launch_apply<T>(hipStream_t, SellViewT<T>, int, int, int, const T*, T*, T,   \
                                   int);                                                        \
     template void launch_residual<T>(hipStream_t, ResidualMode, SellViewT<T>, int, int, int,   \

@@ -70,11 +70,12 @@ template <typename T> void launch_fill(hipStream_t s, T* v, T value, int n);
 template <typename T> void launch_gather(hipStream_t s, T* dst, const T* src, const int* perm, int n);
 template <typename T> void launch_scatter(hipStream_t s, T* dst, const T* src, const int* perm, int n);
 // Resets the state for a solve: tol = max(precision*||b||, 2^-26) (or host_tol),
-// Kp = {0, 0}, fold = {0, 0}.
+// Kp = {0, 0}, fold = {0, 0}.  ||b|| from block partials over b, or, when
+// rows_b is given, summed in row order over rows_b (b in its caller's order).
 template <typename T>
 void launch_init_state(hipStream_t s, SolveState* st, const T* b, int n, double precision,
                        double host_tol, int use_host_tol, int maxK, int small_breaks, double rel2,
-                       double* partials);
+                       double* partials, const T* rows_b = nullptr);
 
 // MatrixCSR::apply: dest[row] = sum (scaling*v)*src[col], over a part's rows.
 // (row0, nrows): a group whose rows are row0 .. row0+nrows-1 in slice order;

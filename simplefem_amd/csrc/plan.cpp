@@ -291,6 +291,35 @@ std::vector<int> locality_rank(int n, const int* A, const int* C)
 
 }  // namespace
 
+std::string check_arguments(int method, int n, const int* A, const int* C, const double* V,
+                            const double* aux, int num_colors, const int* F, const int* B,
+                            const int* R, bool* colouring_error)
+{
+    *colouring_error = false;
+    if (method < SFA_METHOD_CG || method > SFA_METHOD_EISENSTAT_RAINBOW) return "unknown method";
+    std::string err = check_csr(n, A, C);
+    if (!err.empty()) return err;
+    if (!V) return "null csrvalues";
+    if (method != SFA_METHOD_CG && !aux) return "null diagonal/precon";
+    if (method == SFA_METHOD_RAINBOW || method == SFA_METHOD_EISENSTAT_RAINBOW) {
+        *colouring_error = true;
+        if (num_colors < 0 || !F || !B || !R) return "num_colors < 0 or null F/B/R";
+        if (B[0] != 0 || B[num_colors] != n) return "B must run from 0 to N";
+        for (int c = 0; c < num_colors; c++)
+            if (B[c + 1] < B[c]) return "B must be non-decreasing";
+        std::vector<char> seen(n, 0);
+        for (int c = 0; c < num_colors; c++)
+            for (int i = B[c]; i < B[c + 1]; i++) {
+                const int r = R[i];
+                if (r < 0 || r >= n || seen[r]) return "R must be a permutation of the rows";
+                if (F[r] != c) return "F disagrees with B/R";
+                seen[r] = 1;
+            }
+        *colouring_error = false;
+    }
+    return "";
+}
+
 std::string build_plan(Plan& plan, int method, int n, const int* A, const int* C, const double* V,
                        const double* aux, double omega, int num_colors, const int* F,
                        const int* B, const int* R, bool* colouring_error)

@@ -100,6 +100,12 @@ void spmv_bucket_counts(const SellPart& part, long long rows[kSpmvBuckets],
 // be null for plain CG; F/B/R/num_colors only for the Rainbow variants.
 // Returns "" on success, else an error message; *colouring_error is set when
 // the message is about F/B/R.
+// The host checks of a solver call's arguments (CSR structure, null arrays,
+// the colouring's consistency), without building anything; "" when valid.
+std::string check_arguments(int method, int n, const int* A, const int* C, const double* V,
+                            const double* aux, int num_colors, const int* F, const int* B,
+                            const int* R, bool* colouring_error);
+
 std::string build_plan(Plan& plan, int method, int n, const int* A, const int* C,
                        const double* V, const double* aux, double omega, int num_colors,
                        const int* F, const int* B, const int* R, bool* colouring_error);

@@ -13,7 +13,9 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <cstdlib>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -71,6 +73,9 @@ struct SystemT {
         bool pair;
     };
     bool pair_sweeps = false;  // true: pairs of groups where possible (SFA_FLAG_PAIRED_SWEEPS)
+    // device-pointer solves: sum ||b|| for the tolerance in row order, one
+    // lane (the reference's bits; for short vectors) instead of by blocks
+    bool row_order_tol = false;
     // mixed solves: restart every inner CG from d = z (SFA_FLAG_INNER_RESTARTS)
     // instead of continuing the previous inner solve's direction
     bool inner_restarts = false;
@@ -666,18 +671,20 @@ struct SystemT {
         launch_gather(s, x, src_x, perm, n);
         launch_gather(s, b, src_b, perm, n);
         launch_gather(s, r, src_r, perm, n);
-        start(s, precision, print_modulo, host_tol, use_host_tol, rel2);
+        // device b: ||b|| in the reference's row order when asked for
+        const T* rows_b = device_pointers && row_order_tol ? src_b : nullptr;
+        start(s, precision, print_modulo, host_tol, use_host_tol, rel2, rows_b);
         return s;
     }
 
     // State, scratch and restart for x, b, r already in device order.
     void start(hipStream_t s, double precision, int print_modulo, double host_tol,
-               int use_host_tol, double rel2)
+               int use_host_tol, double rel2, const T* rows_b = nullptr)
     {
         init_scratch(s);
         const int small_breaks = (method == SFA_METHOD_SSOR_EISENSTAT) ? (print_modulo >= -2) : 1;
         launch_init_state(s, state, b, n, precision, host_tol, use_host_tol, n, small_breaks,
-                          rel2, ps);
+                          rel2, ps, rows_b);
         restart(s);
         read_state();
     }
@@ -722,12 +729,15 @@ struct SystemT {
         // how many chunks to queue before the next look at the state.
         const int G = 4;
         build_chunk_graph(G);
-        int queued = 4;
+        // at most kMaxQueuedDispatches kernels queued between looks at the
+        // state, however many launches an iteration takes
+        const int max_chunks = std::max(1, kMaxQueuedDispatches / (G * kernels_per_iteration()));
+        int queued = std::min(4, max_chunks);
         while (!host_state->done) {
             for (int i = 0; i < queued; i++) SFA_HIP(hipGraphLaunch(chunk_exec, s));
             read_state();
             if (host_state->done) break;
-            queued = predict_chunks(G);
+            queued = std::min(predict_chunks(G), max_chunks);
         }
     }
 
@@ -1150,6 +1160,138 @@ struct sfa_system_s {
 
 namespace {
 
+// The reference's callers make the same raw call many times with one matrix
+// (a call per right-hand side, per time step, per inner solve).  The last
+// few Systems built by drop-in calls are kept: a call whose arguments equal
+// a kept System's -- device, method, N, omega, and the CSR arrays, the
+// diagonal (or precon) and the colouring byte for byte, compared with
+// memcmp against host copies -- reuses its plan, device arrays and
+// recorded graphs; anything else builds a new System and the least recently
+// used one is dropped.  Reuse changes no result (every solve re-reads x, b
+// and residual and resets its state).  SFA_DROPIN_CACHE sets how many are
+// kept (default 4, 0: none).  A System is used by one call at a time; a
+// call that finds its match busy builds another.
+struct DropinCache {
+    struct Entry {
+        int device = 0, method = 0, n = 0, nc = 0;
+        double omega = 0.;
+        std::vector<double> aux;
+        std::vector<int> F, B, R;
+        std::unique_ptr<System> sys;
+        unsigned long long last_use = 0;
+        bool busy = false;
+    };
+    std::mutex mu;
+    std::vector<std::unique_ptr<Entry>> entries;
+    unsigned long long clock = 0;
+    int capacity = 4;
+
+    DropinCache()
+    {
+        if (const char* e = std::getenv("SFA_DROPIN_CACHE")) capacity = std::max(0, std::atoi(e));
+    }
+
+    template <class T>
+    static bool same(const std::vector<T>& kept, const T* p, size_t count)
+    {
+        if (kept.size() != count) return false;
+        return count == 0 || (p && std::memcmp(kept.data(), p, count * sizeof(T)) == 0);
+    }
+
+    static bool matches(const Entry& e, int device, int method, int n, const int* A,
+                        const int* C, const double* V, const double* aux, double omega, int nc,
+                        const int* F, const int* B, const int* R)
+    {
+        const System& s = *e.sys;
+        if (e.device != device || e.method != method || e.n != n || e.nc != nc) return false;
+        if (std::memcmp(&e.omega, &omega, sizeof omega) != 0) return false;
+        if (!same(s.host_A, A, (size_t)n + 1)) return false;
+        const size_t nnz = (size_t)A[n];
+        return same(s.host_C, C, nnz) && same(s.host_V, V, nnz) &&
+               same(e.aux, aux, aux ? (size_t)n : 0) && same(e.F, F, F ? (size_t)n : 0) &&
+               same(e.B, B, B ? (size_t)nc + 1 : 0) && same(e.R, R, R ? (size_t)n : 0);
+    }
+
+    // A System for the call: a kept one (marked busy) or a new one.
+    Entry* acquire(int method, int n, const int* A, const int* C, const double* V,
+                   const double* aux, double omega, int nc, const int* F, const int* B,
+                   const int* R, std::unique_ptr<Entry>* fresh)
+    {
+        int device = 0;
+        SFA_HIP(hipGetDevice(&device));
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            for (auto& e : entries)
+                if (!e->busy && matches(*e, device, method, n, A, C, V, aux, omega, nc, F, B, R)) {
+                    e->busy = true;
+                    return e.get();
+                }
+        }
+        std::unique_ptr<Entry> e(new Entry());
+        e->sys.reset(new System());
+        e->sys->create(method, n, A, C, V, aux, omega, nc, F, B, R);
+        e->device = device;
+        e->method = method;
+        e->n = n;
+        e->nc = nc;
+        e->omega = omega;
+        if (aux) e->aux.assign(aux, aux + n);
+        if (F) e->F.assign(F, F + n);
+        if (B) e->B.assign(B, B + nc + 1);
+        if (R) e->R.assign(R, R + n);
+        e->busy = true;
+        *fresh = std::move(e);
+        return fresh->get();
+    }
+
+    // Ends a call: a new System is kept (dropping the least recently used
+    // idle one beyond the capacity), a kept one is marked idle.  A System
+    // whose solve failed is not kept.
+    void release(Entry* e, std::unique_ptr<Entry> fresh, bool ok)
+    {
+        std::vector<std::unique_ptr<Entry>> drop;
+        {
+            std::lock_guard<std::mutex> lock(mu);
+            if (fresh) {
+                if (ok && capacity > 0) {
+                    fresh->busy = false;
+                    fresh->last_use = ++clock;
+                    entries.push_back(std::move(fresh));
+                }
+            } else {
+                for (size_t i = 0; i < entries.size(); i++)
+                    if (entries[i].get() == e) {
+                        if (ok) {
+                            e->busy = false;
+                            e->last_use = ++clock;
+                        } else {
+                            drop.push_back(std::move(entries[i]));
+                            entries.erase(entries.begin() + i);
+                        }
+                        break;
+                    }
+            }
+            while ((int)entries.size() > capacity) {
+                int victim = -1;
+                for (size_t i = 0; i < entries.size(); i++)
+                    if (!entries[i]->busy &&
+                        (victim < 0 || entries[i]->last_use < entries[victim]->last_use))
+                        victim = (int)i;
+                if (victim < 0) break;
+                drop.push_back(std::move(entries[victim]));
+                entries.erase(entries.begin() + victim);
+            }
+        }
+        // fresh (if not kept) and dropped Systems are destroyed here, unlocked
+    }
+};
+
+DropinCache& dropin_cache()
+{
+    static DropinCache* cache = new DropinCache();  // kept until the process ends
+    return *cache;
+}
+
 int solve_host(int method, int N, double* x, const double* b, const int* A, const int* C,
                const double* V, double* residual, double precision, int print_modulo,
                const double* aux, double omega, int nc, const int* F, const int* B, const int* R,
@@ -1157,9 +1299,24 @@ int solve_host(int method, int N, double* x, const double* b, const int* A, cons
 {
     return sfa::guarded([&] {
         if (!x || !b || !residual) throw Error{SFA_ERR_INVALID, "null x, b or residual"};
-        System sys;
-        sys.create(method, N, A, C, V, aux, omega, nc, F, B, R);
-        sys.solve(x, b, residual, precision, print_modulo, 0, info);
+        // every argument is checked on the host before the device or a kept
+        // System is touched (a kept one is then compared with valid arrays)
+        bool colouring_error = false;
+        const std::string err = sfa::check_arguments(method, N, A, C, V, aux, nc, F, B, R,
+                                                     &colouring_error);
+        if (!err.empty()) throw Error{colouring_error ? SFA_ERR_COLOURING : SFA_ERR_INVALID, err};
+        sfa::ensure_device();
+        DropinCache& cache = dropin_cache();
+        std::unique_ptr<DropinCache::Entry> fresh;
+        DropinCache::Entry* e =
+            cache.acquire(method, N, A, C, V, aux, omega, nc, F, B, R, &fresh);
+        try {
+            e->sys->solve(x, b, residual, precision, print_modulo, 0, info);
+        } catch (...) {
+            cache.release(e, std::move(fresh), false);
+            throw;
+        }
+        cache.release(e, std::move(fresh), true);
     });
 }
 
@@ -1347,6 +1504,17 @@ int sfa_system_set_stream(sfa_system sys, void* hip_stream)
         sys->sys.stream = hip_stream ? (hipStream_t)hip_stream : sys->sys.own_stream;
     });
 }
+
+}  // extern "C"
+
+namespace sfa {
+void system_row_order_tolerance(sfa_system sys, bool on)
+{
+    if (sys) sys->sys.row_order_tol = on;
+}
+}  // namespace sfa
+
+extern "C" {
 
 int sfa_system_query(sfa_system sys, sfa_system_stats* stats)
 {

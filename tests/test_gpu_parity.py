@@ -460,3 +460,32 @@ def test_mixed_refinement_edge_cases():
         x2[s.dirichlet] = 0.0
         near = sysm.solve_mixed(x2, b, 1e-10)
         assert near["success"] == 1 and near["outer"] < full["outer"], (near, full)
+
+
+def test_repeated_dropin_calls_reuse_the_plan_and_see_changes():
+    """Drop-in calls on host arrays keep the last few Systems: a repeated call
+    with the same arrays reuses one (same bits as the first call and the
+    oracle's bar), a call whose values changed in place -- the same arrays,
+    new contents -- is solved with the new values, and calls that alternate
+    between matrices give each matrix's own result."""
+    s1, s2 = S.poisson3d_p1_kuhn(8), S.poisson2d_p1(5)
+    b1, b2 = S.rhs_uniform(s1, 3), S.rhs_uniform(s2, 4)
+    first = _gpu_solve("rainbow", s1, b1, s1.diagonal(), 1.0,
+                       oracle.rainbow(s1.n, s1.A, s1.C))
+    for _ in range(2):
+        again = _gpu_solve("rainbow", s1, b1, s1.diagonal(), 1.0,
+                           oracle.rainbow(s1.n, s1.A, s1.C))
+        assert again[2]["iterations"] == first[2]["iterations"]
+        assert np.array_equal(again[0], first[0]) and np.array_equal(again[1], first[1])
+        _check_parity("ssor", s2, b2)  # another matrix in between
+    _check_parity("rainbow", s1, b1)
+    # the same arrays, scaled in place: x halves
+    V = s1.V
+    V *= 2.0
+    d = s1.diagonal()
+    x2, _, i2 = _gpu_solve("rainbow", s1, b1, d, 1.0, oracle.rainbow(s1.n, s1.A, s1.C))
+    V *= 0.5
+    xo, _, io = oracle.solve("rainbow", s1.A, s1.C, s1.V * 2.0, b1, aux=d, omega=1.0,
+                             colouring=oracle.rainbow(s1.n, s1.A, s1.C))
+    assert abs(i2["iterations"] - io["iterations"]) <= 1
+    _close(x2, xo)
