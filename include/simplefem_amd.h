@@ -346,6 +346,18 @@ int sfa_system_time_sweeps(sfa_system sys, int kernel_class, int reps,
 
 int sfa_system_destroy(sfa_system sys);
 
+/* Microseconds per round of a grid-wide barrier inside one resident launch
+ * of `blocks` workgroups of `threads` (a cooperative launch; mode 0: one
+ * counter, mode 1: a counter per group of workgroups blockIdx % 8, then one
+ * top counter), or of a kernel boundary (mode 2: one launch per round,
+ * replayed from a graph), timed over `rounds` rounds between two HIP
+ * events.  Each round every workgroup stores a value and reads its
+ * neighbour's after the barrier; *errors gets bit 0 for a wait that gave
+ * up (every wait is bounded) and bit 1 for a stale value seen.  No reference
+ * counterpart: the measurement behind one launch per colour in the sweeps. */
+int sfa_grid_barrier_cost(int blocks, int threads, int rounds, int mode,
+                          double* us_per_round, int* errors);
+
 /* ---------------------------------------------------------------------------
  * Batched and served solves.  No reference counterpart (the reference solves
  * one system per call).  A batch holds up to max_slots (<= 1024) solves of

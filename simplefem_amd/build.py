@@ -34,6 +34,7 @@ SOURCES = [  # (source, compile as HIP)
     ("batch.hip", True),
     ("hodge.hip", True),
     ("noalloc.hip", True),
+    ("barrier.hip", True),
     ("system.cpp", True),
     ("plan.cpp", False),
 ]

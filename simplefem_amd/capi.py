@@ -166,6 +166,8 @@ _SIGNATURES = {
     "sfa_system_time_sweeps": ([c_void_p, c_int, c_int, POINTER(c_double)], c_int),
     "sfa_system_set_spmv": ([c_void_p, c_int], c_int),
     "sfa_system_destroy": ([c_void_p], c_int),
+    "sfa_grid_barrier_cost": ([c_int, c_int, c_int, c_int, POINTER(c_double), POINTER(c_int)],
+                              c_int),
     "sfa_batch_create": ([c_int, c_int, POINTER(c_void_p)], c_int),
     "sfa_batch_add_matrix": ([c_void_p, c_int, _P_I, _P_I, _P_D, _P_D, c_double, c_int, _P_I,
                               _P_I, _P_I, POINTER(c_int)], c_int),
@@ -382,6 +384,15 @@ def get_diagonal(n, A, C, V):
     diag = np.empty(n, dtype=np.float64)
     _check(load().sfa_MatrixCSR_getDiagonal(n, _i(A), _i(C), _d(V), _d(diag)))
     return diag
+
+
+def grid_barrier_cost(blocks, threads=256, rounds=2000, mode=1):
+    """sfa_grid_barrier_cost: (microseconds per round, error bits)."""
+    us = c_double(0.0)
+    err = c_int(0)
+    _check(load().sfa_grid_barrier_cost(int(blocks), int(threads), int(rounds), int(mode),
+                                        ctypes.byref(us), ctypes.byref(err)))
+    return us.value, err.value
 
 
 class Rainbow:
