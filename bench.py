@@ -30,43 +30,114 @@ sys.path.insert(0, ROOT)
 
 METRIC = "solves/s + HBM GB/s (3D P1 Poisson, rtol 1e-10); p50/p99 served latency"
 PEAK_HBM_GBS = 8000.0
-# HBM bytes per launch of each kernel class on the default workload (64^3 in
-# the reference's vertex numbering, omega 1, one colour per sweep launch): rocprofv3
-# FETCH_SIZE and WRITE_SIZE runs, each scaled by what the
-# counter reads for a device-to-device copy of known size (bench.py
-# --copy-calibration); commands and per-launch figures are kept in
-# profiles/rainbow_cg_ssor_64cube_fp64_hbm_traffic.md.
-MEASURED_TRAFFIC = {"spmv": 56382191, "forward": 6474410, "backward": 5118184}
+# The kernel trace and the HBM counters of the default workload (64^3 in the
+# reference's vertex numbering, omega 1) are taken by rocprofv3 runs of this
+# script and summarised in a profile whose "roofline-stamp:" line carries, per
+# kernel class, the traced mean duration of a launch and its HBM bytes, plus
+# a digest of the sources that make those kernels and launches.  The
+# roofline uses them only while the digest matches the sources built now.
+STAMPED_PROFILE = os.path.join(ROOT, "profiles", "rainbow_cg_ssor_64cube_fp64_kernels.md")
+KERNEL_SOURCES = ["kernels.hip", "kernels.hpp", "device_common.hpp", "launch.hpp",
+                  "system.cpp", "plan.cpp", "plan.hpp", "host_common.hpp"]
+
+
+def kernel_source_digest():
+    """sha256 over the sources of the single-system kernels and their launches."""
+    import hashlib
+
+    h = hashlib.sha256()
+    for name in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, "simplefem_amd", "csrc", name), "rb") as f:
+            h.update(f.read())
+    return h.hexdigest()[:16]
+
+
+def stamped_profile():
+    """The profile's stamp if it was taken from the sources built now, else None."""
+    try:
+        with open(STAMPED_PROFILE) as f:
+            for line in f:
+                if line.startswith("roofline-stamp:"):
+                    stamp = json.loads(line.split(":", 1)[1])
+                    return stamp if stamp.get("source_digest") == kernel_source_digest() else None
+    except (OSError, ValueError):
+        pass
+    return None
 
 
 def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(system, b, d, colouring, precision, omega, budget_s=25.0):
-    """The reference's ConjugateGradientSolverCSR_Rainbow on the host cores."""
-    import oracle
-    from oracle.ref import RefRunner
+def host_cpu():
+    """The host's CPU (lscpu) and the threads a CPU baseline runs with: the
+    CPUs this process may use, capped by OMP_NUM_THREADS where the
+    environment sets it (the CPU share of one GPU on a shared node)."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+
+    def num(key):
+        try:
+            return int(info.get(key, "0").split()[0])
+        except ValueError:
+            return 0
 
     try:
-        cores = len(os.sched_getaffinity(0))
+        avail = len(os.sched_getaffinity(0))
     except AttributeError:
-        cores = os.cpu_count() or 1
-    cores = max(1, min(cores, 16))
+        avail = os.cpu_count() or 1
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    threads = min(avail, int(env)) if env.isdigit() and int(env) > 0 else avail
+    return {"model": info.get("Model name"), "sockets": num("Socket(s)"),
+            "physical_cores": num("Socket(s)") * num("Core(s) per socket"),
+            "threads_per_core": num("Thread(s) per core"), "cpus_available": avail,
+            "omp_num_threads_env": env or None, "threads": max(1, threads),
+            "binding": "OMP_PROC_BIND=close OMP_PLACES=cores"}
+
+
+def _ref_median(threads, variant, system, b, d, colouring, precision, omega, budget_s):
+    """Median seconds of the reference's solver over up to 5 solves within
+    budget_s, at `threads` OpenMP threads, bound one per core."""
+    from oracle.ref import RefRunner
+
+    with RefRunner(threads, bind=True) as ref:
+        t0 = time.time()
+        _, _, K, times = ref.solve(variant, system.A, system.C, system.V, b, aux=d, omega=omega,
+                                   colouring=colouring, precision=precision)
+        reps = max(0, min(4, int((budget_s - (time.time() - t0)) / max(times[0], 1e-3)) - 1))
+        if reps:
+            times += ref.solve(variant, system.A, system.C, system.V, b, aux=d, omega=omega,
+                               colouring=colouring, precision=precision, repeat=reps)[3]
+    return float(np.median(times)), K, len(times)
+
+
+def cpu_baseline(system, b, d, colouring, precision, omega, budget_s=25.0):
+    """The reference's ConjugateGradientSolverCSR_Rainbow on the host cores
+    (all threads of this process's CPU share, and one thread)."""
+    import oracle
+
+    cpu = host_cpu()
+    cores = cpu["threads"]
     if oracle.ref_available():
-        with RefRunner(cores) as ref:
-            t0 = time.time()
-            x, r, K, times = ref.solve("rainbow", system.A, system.C, system.V, b, aux=d,
-                                       omega=omega, colouring=colouring, precision=precision)
-            reps = max(0, min(4, int((budget_s - (time.time() - t0)) / max(times[0], 1e-3)) - 1))
-            if reps:
-                times += ref.solve("rainbow", system.A, system.C, system.V, b, aux=d,
-                                   omega=omega, colouring=colouring, precision=precision,
-                                   repeat=reps)[3]
-        med = float(np.median(times))
+        med, K, n = _ref_median(cores, "rainbow", system, b, d, colouring, precision, omega,
+                                0.4 * budget_s)
+        med1, K1, n1 = _ref_median(1, "rainbow", system, b, d, colouring, precision, omega,
+                                   0.6 * budget_s)
         return {"value": 1.0 / med, "unit": "solves/s", "cores": cores, "kind": "reference",
-                "sample": f"{len(times)} full solves of the same system (K={K}), median "
-                          f"{med:.3f} s, OMP_NUM_THREADS={cores}"}
+                "sample": f"{n} full solves of the same system (K={K}), median {med:.3f} s, "
+                          f"OMP_NUM_THREADS={cores}, OMP_PROC_BIND=close, OMP_PLACES=cores",
+                "one_thread": {"value": 1.0 / med1, "unit": "solves/s", "cores": 1,
+                               "sample": f"{n1} full solves (K={K1}), median {med1:.3f} s"},
+                "host": cpu,
+                "build": "oracle/Makefile: g++ -O3 -march=x86-64-v3 -fopenmp -DNDEBUG "
+                         "-ffp-contract=off (built with the library and shipped with it, so "
+                         "for a portable x86-64 level rather than -march=native)"}
     # no reference build: the oracle's C restatement, one thread
     t0 = time.time()
     x, r, info = oracle.solve("rainbow", system.A, system.C, system.V, b, aux=d, omega=omega,
@@ -74,6 +145,89 @@ def cpu_baseline(system, b, d, colouring, precision, omega, budget_s=25.0):
     dt = time.time() - t0
     return {"value": 1.0 / dt, "unit": "solves/s", "cores": 1, "kind": "port",
             "sample": f"1 full solve (K={info['iterations']}), {dt:.3f} s"}
+
+
+def _per_call(fn, reps):
+    """Seconds per call of fn(): the median of reps calls after one warm-up."""
+    fn()
+    times = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        times.append(time.perf_counter() - t0)
+    return float(np.median(times))
+
+
+def dropin_per_call(system, b, d, colouring, omega, precision, reps=3):
+    """The raw drop-in call on host arrays (sfa_ConjugateGradientSolverCSR_Rainbow:
+    argument checks, device-order plan, upload, solve, download), as the
+    reference's callers make it, seconds per call."""
+    from simplefem_amd import capi
+
+    info = {}
+
+    def call():
+        x = np.zeros(system.n)
+        r = np.zeros(system.n)
+        capi.ConjugateGradientSolverCSR_Rainbow(system.n, x, b, system.A, system.C, system.V, r,
+                                                precision, -3, d, omega, *colouring, info=info)
+
+    s = _per_call(call, reps)
+    return {"seconds_per_call": round(s, 5), "calls_per_s": round(1.0 / s, 3),
+            "iterations": info["iterations"], "calls": reps}
+
+
+def c1_natural_ssor(precision=1e-10, omega=1.0, budget_s=12.0, with_cpu=True):
+    """2D P1 Poisson, 129^2 vertices in the reference's numbering,
+    ConjugateGradientSolverCSR_SSOR (natural-order SSOR) omega 1: the GPU
+    with the system resident, the raw drop-in call on host arrays, and the
+    reference's solver at one and all threads of the host."""
+    import oracle
+    from simplefem_amd import capi
+    from simplefem_amd import systems as S
+    from simplefem_amd.hipmem import DeviceArray, synchronize
+
+    s = S.poisson2d_p1(7, numbering="reference")
+    b = S.rhs_uniform(s, seed=2026)
+    d = capi.get_diagonal(s.n, s.A, s.C, s.V)
+    out = {"workload": "2D P1 Poisson, 129^2 vertices (UnitSquare2D_simple refined 7 times, "
+                       "reference numbering), ConjugateGradientSolverCSR_SSOR omega 1, "
+                       f"precision {precision:g}, x0 = 0", "rows": s.n, "nnz": s.nnz}
+    with capi.System(capi.METHOD_SSOR, s.n, s.A, s.C, s.V, d, omega) as sysd:
+        st = sysd.stats()
+        bt, x, r = DeviceArray.from_host(b), DeviceArray(s.n), DeviceArray(s.n)
+        info = {}
+
+        def solve():
+            x.zero()
+            r.zero()
+            sysd.solve(x, bt, r, precision, info=info)
+            synchronize()
+
+        t = _per_call(solve, 3)
+        out["gpu_resident"] = {"solves_per_s": round(1.0 / t, 3), "seconds": round(t, 5),
+                               "iterations": info["iterations"],
+                               "levels_per_sweep": st["groups_forward"],
+                               "kernels_per_iteration": st["kernels_per_iteration"]}
+    info = {}
+
+    def call():
+        xh, rh = np.zeros(s.n), np.zeros(s.n)
+        capi.ConjugateGradientSolverCSR_SSOR(s.n, xh, b, s.A, s.C, s.V, rh, precision, -3, d,
+                                             omega, info=info)
+
+    t = _per_call(call, 3)
+    out["gpu_dropin_call"] = {"seconds_per_call": round(t, 5), "calls_per_s": round(1.0 / t, 3),
+                              "iterations": info["iterations"]}
+    if with_cpu and oracle.ref_available():
+        cpu = host_cpu()
+        for name, threads in (("cpu_reference_all_threads", cpu["threads"]),
+                              ("cpu_reference_one_thread", 1)):
+            med, K, n = _ref_median(threads, "ssor", s, b, d, None, precision, omega,
+                                    budget_s / 2)
+            out[name] = {"solves_per_s": round(1.0 / med, 3), "seconds": round(med, 5),
+                         "iterations": K, "threads": threads, "solves": n}
+    return out
 
 
 def forms_cpu_baseline(args, budget_s=25.0):
@@ -139,6 +293,27 @@ def copy_calibration(mib):
     return 0
 
 
+def barrier_cost(rounds=2000):
+    """Grid-wide barrier inside one resident launch against a kernel boundary
+    (sfa_grid_barrier_cost) on grids of 256-1024 workgroups of 256 threads:
+    one JSON line per grid and mode."""
+    from simplefem_amd import capi
+
+    names = {0: "one counter", 1: "8 group counters + top", 2: "kernel boundary (graph replay)"}
+    for blocks in (256, 512, 1024):
+        for mode in (0, 1, 2):
+            try:
+                us, err = capi.grid_barrier_cost(blocks, 256, rounds, mode)
+                row = {"blocks": blocks, "threads": 256, "mode": names[mode], "rounds": rounds,
+                       "us_per_round": round(us, 3), "errors": err}
+            except capi.SfaError as e:
+                row = {"blocks": blocks, "threads": 256, "mode": names[mode], "error": str(e)}
+            print(json.dumps(row), flush=True)
+            if row.get("errors"):
+                return 1
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -149,6 +324,8 @@ def main():
     ap.add_argument("--precision", type=float, default=1e-10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the per-call drop-in timing and the 2D natural-order SSOR lines")
     ap.add_argument("--numbering", default="reference", choices=["reference", "lexicographic"],
                     help="vertex numbering of the 64^3 system: the reference's uniform-refinement "
                          "order (its CSR structure and colouring; default) or x-fastest")
@@ -185,6 +362,9 @@ def main():
     ap.add_argument("--copy-calibration", type=float, default=0.0, metavar="MIB",
                     help="only copy MIB MiB device to device 5 times (hipMemcpy), to scale "
                          "the FETCH_SIZE / WRITE_SIZE counters of a kernel of known traffic")
+    ap.add_argument("--barrier-cost", action="store_true",
+                    help="only time a grid barrier inside one resident launch against a kernel "
+                         "boundary (sfa_grid_barrier_cost) and print one line per grid")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU: this build has no multi-GPU path "
@@ -192,6 +372,8 @@ def main():
         return 2
     if args.copy_calibration > 0:
         return copy_calibration(args.copy_calibration)
+    if args.barrier_cost:
+        return barrier_cost()
     if args.config != "poisson3d":
         from simplefem_amd import forms_bench
         out = forms_bench.run(args)
@@ -269,20 +451,27 @@ def main():
         bytes_per_launch = p["bytes"] / max(p["launches"], 1)
         event_us = p["ms"] * 1e3 / max(p["launches"], 1)  # each launch between two events
         # the class's launches back to back in a recorded graph, timed by two
-        # events around the replay: the duration the kernel trace reports
-        launch_us = sysd.time_sweeps(dom, 200) if dom in ("forward", "backward") else event_us
-        achieved = bytes_per_launch / (launch_us * 1e-6) / 1e9
+        # events around the replay (live, this run)
+        replay_us = sysd.time_sweeps(dom, 200) if dom in ("forward", "backward") else event_us
         default_workload = (args.cube == 64 and args.omega == 1.0 and args.numbering == "reference"
                             and not args.paired_sweeps)
-        traffic = MEASURED_TRAFFIC.get(dom) if default_workload else None
+        stamp = stamped_profile() if default_workload else None
+        traced = stamp["classes"].get(dom) if stamp else None
+        if traced:  # the kernel trace's mean duration of a launch of the class
+            launch_us, source = traced["traced_us"], "rocprofv3 kernel trace (" + stamp["profile"] + ")"
+        else:
+            launch_us, source = replay_us, "graph replay of 200 iterations' launches (no current trace)"
+        achieved = bytes_per_launch / (launch_us * 1e-6) / 1e9
         roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS,
-                    "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                    "unit": "GB/s", "frac": round(achieved / PEAK_HBM_GBS, 4),
+                    "traffic": traced["hbm_bytes"] if traced else None,
                     "kernel": dom, "launches_per_solve": p["launches"],
-                    "avg_launch_us": round(launch_us, 3),
-                    "avg_launch_us_timing": "graph replay of 200 iterations' launches of the class"
-                                            if dom in ("forward", "backward") else "per-launch events",
+                    "bytes_per_launch": round(bytes_per_launch),
+                    "avg_launch_us": round(launch_us, 3), "avg_launch_us_source": source,
+                    "replay_launch_us": round(replay_us, 3),
+                    "replay_frac": round(bytes_per_launch / (replay_us * 1e-6) / 1e9 / PEAK_HBM_GBS, 4),
                     "event_bracketed_launch_us": round(event_us, 3),
-                    "bytes_per_launch": round(bytes_per_launch)}
+                    "stamp_source_digest": kernel_source_digest()}
         log("profile: " + ", ".join(f"{k} {v['launches']}x {v['ms']:.2f}ms" for k, v in
                                      prof.items() if v["launches"]))
 
@@ -323,8 +512,18 @@ def main():
     }
     if prof is not None:
         out["kernel_profile"] = {k: v for k, v in prof.items() if v["launches"]}
-    print(json.dumps(out), flush=True)
     sysd.close()
+    if not args.no_extras:
+        # what the reference's callers see: one raw call on host arrays
+        log("drop-in call on host arrays ...")
+        out["dropin_per_call"] = dropin_per_call(system, b, d, colouring, args.omega,
+                                                 args.precision)
+        log(f"drop-in call: {out['dropin_per_call']}")
+        log("2D natural-order SSOR (129^2) ...")
+        out["c1_ssor_2d"] = c1_natural_ssor(args.precision, args.omega,
+                                            with_cpu=not args.no_cpu_baseline)
+        log(f"c1: {out['c1_ssor_2d']}")
+    print(json.dumps(out), flush=True)
     return 0
 
 

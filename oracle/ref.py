@@ -200,12 +200,16 @@ def _task(kind, kw):
 class RefRunner:
     """One child process running the reference library at `threads` threads."""
 
-    def __init__(self, threads: int = 1, timeout: float = 600.0):
+    def __init__(self, threads: int = 1, timeout: float = 600.0, bind: bool = False):
+        """bind: OMP_PROC_BIND=close, OMP_PLACES=cores (one thread per core of
+        the process's CPU set), as the timed CPU baseline runs."""
         if not os.path.exists(REF_LIB_PATH):
             raise FileNotFoundError(REF_LIB_PATH)
         self.threads = threads
         self.timeout = timeout
         env = dict(os.environ, OMP_NUM_THREADS=str(threads))
+        if bind:
+            env.update(OMP_PROC_BIND="close", OMP_PLACES="cores")
         self._proc = subprocess.Popen([sys.executable, os.path.join(HERE, "_refworker.py"),
                                        str(threads)], stdin=subprocess.PIPE,
                                       stdout=subprocess.PIPE, stderr=subprocess.DEVNULL,
