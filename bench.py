@@ -347,7 +347,9 @@ def main():
                          "inner solve")
     ap.add_argument("--engine-profile", action="store_true",
                     help="batched: only time the FP64 engine's kernels step by step (HIP events, "
-                         "chunks of 2000 steps) and print that table")
+                         "chunks of 2000 steps) and print that table; hcurl-mixed: only run one "
+                         "FP64 solve with every kernel launched on its own (no graphs) between "
+                         "HIP events and print the per-class table")
     ap.add_argument("--inner-restarts", action="store_true",
                     help="mixed forms: restart every inner CG (plain iterative refinement) "
                          "instead of continuing the previous inner solve's direction")
@@ -377,7 +379,7 @@ def main():
     if args.config != "poisson3d":
         from simplefem_amd import forms_bench
         out = forms_bench.run(args)
-        if "engine_profile_fp64" in out:
+        if "engine_profile_fp64" in out or "hcurl_kernel_profile_fp64" in out:
             print(json.dumps(out), flush=True)
             return 0
         if not args.no_cpu_baseline:

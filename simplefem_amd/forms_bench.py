@@ -64,6 +64,16 @@ def hcurl_mixed(args):
     r = DeviceArray(s.n)
     info = {}
     steps = max(1, args.steps // 4)
+    if args.engine_profile:
+        # one FP64 solve with every kernel launched on its own between two
+        # events (no graphs): per-class launches, event time and bytes
+        x.zero()
+        r.zero()
+        prof, pinfo = sysd.profile(x, bt, r, 1e-10)
+        sysd.close()
+        return {"hcurl_kernel_profile_fp64": {k: v for k, v in prof.items() if v["launches"]},
+                "iterations": pinfo["iterations"], "rows": s.n, "nnz": s.nnz,
+                "colours": rb.num_colors}
 
     def fp64():
         x.zero()
