@@ -62,6 +62,10 @@ def _init_library():
     _lib.feecref_whitney1.restype = c_int
     _lib.feecref_whitney1_sizes.argtypes = [_P_I]
     _lib.feecref_whitney1_copy.argtypes = [_P_I, _P_I, _P_I, _P_I, _P_D, _P_I, _P_I, _P_D]
+    _lib.feecref_sullivan2d.argtypes = [c_int, c_int]
+    _lib.feecref_sullivan2d.restype = c_int
+    _lib.feecref_sullivan2d_sizes.argtypes = [_P_I]
+    _lib.feecref_sullivan2d_copy.argtypes = [_P_I] * 7 + [_P_D]
 
 
 def _i(a):
@@ -139,6 +143,22 @@ def _task(kind, kw):
         return {"edges": edges.reshape(ne, 2), "dirichlet": dirichlet.astype(bool),
                 "curlcurl": (kA, kC[:nk].copy(), kV[:nk].copy()),
                 "mass": (mA, mC[:nm].copy(), mV[:nm].copy())}
+    if kind == "sullivan2d":
+        L.feecref_sullivan2d(int(kw["levels"]), int(kw["r"]))
+        sz = np.zeros(5, dtype=np.int32)
+        L.feecref_sullivan2d_sizes(_i(sz))
+        nv, ne, nt, n, nnz = (int(t) for t in sz)
+        edges = np.empty(2 * ne, dtype=np.int32)
+        vflag, eflag = np.empty(nv, dtype=np.int32), np.empty(ne, dtype=np.int32)
+        tri, tri_e = np.empty(3 * nt, dtype=np.int32), np.empty(3 * nt, dtype=np.int32)
+        A, C, V = np.empty(n + 1, dtype=np.int32), np.empty(max(nnz, 1), dtype=np.int32), \
+            np.empty(max(nnz, 1))
+        L.feecref_sullivan2d_copy(_i(edges), _i(vflag), _i(eflag), _i(tri), _i(tri_e), _i(A),
+                                  _i(C), _d(V))
+        return {"edges": edges.reshape(ne, 2), "vertex_dirichlet": vflag.astype(bool),
+                "edge_dirichlet": eflag.astype(bool), "triangles": tri.reshape(nt, 3),
+                "triangle_edges": tri_e.reshape(nt, 3),
+                "stiffness": (A, C[:nnz].copy(), V[:nnz].copy())}
     if kind == "hodge":
         N, Ldim = int(kw["N"]), int(kw["L"])
         mats = []
@@ -260,6 +280,13 @@ class RefRunner:
         Whitney 1-form matrices as (A, C, V), duplicates kept as
         Conjugation leaves them."""
         return self._call("whitney1", levels=levels)
+
+    def sullivan2d(self, levels, r):
+        """UnitSquare2D_simple refined `levels` times: {"edges", "vertex_dirichlet",
+        "edge_dirichlet", "triangles", "triangle_edges", "stiffness" (A, C, V)}
+        with the degree-r stiffness in the Sullivan basis, composed as
+        tests/sullivan2D/poissondirichlet2D.cpp composes it."""
+        return self._call("sullivan2d", levels=levels, r=r)
 
     def csr_apply(self, A, C, V, src, scaling=1.0):
         return self._call("apply", A=A, C=C, V=V, src=src, scaling=scaling)

@@ -17,6 +17,7 @@
 #include "base/include.hpp"
 #include "dense/densematrix.hpp"
 #include "fem/global.diffmatrix.hpp"
+#include "fem/global.inclsullivan.hpp"
 #include "fem/global.inclwhitney.hpp"
 #include "fem/global.massmatrix.hpp"
 #include "fem/lagrangematrices.hpp"
@@ -165,6 +166,80 @@ int feecref_whitney1(int levels)
     keep_csr(Conjugation(mass2, diff & incl), r.kA, r.kC, r.kV);
     keep_csr(Conjugation(mass1, incl), r.mA, r.mC, r.mV);
     return r.ne;
+}
+
+// Degree-r Lagrange stiffness in FEEC's Sullivan basis on UnitSquare2D_simple
+// refined `levels` times with automatic Dirichlet flags, composed as
+// tests/sullivan2D/poissondirichlet2D.cpp:120-151 composes it:
+// (D I)^T M_1 (D I) with I = FEECSullivanInclusionMatrix(M, 2, 0, r),
+// D = FEECBrokenDiffMatrix(M, 2, 0, r), M_1 = FEECBrokenMassMatrix(M, 2, 1, r-1),
+// sortentries, MatrixCSR.  Also the mesh's edges (vertex pairs) and the
+// Dirichlet flags of vertices and edges.  Returns the number of unknowns.
+struct SullivanResult {
+    int nv = 0, ne = 0, nt = 0, n = 0;
+    std::vector<int> edges, vflag, eflag, triangles, tri_edges;
+    std::vector<int> A, C;
+    std::vector<double> V;
+} g_sullivan;
+
+int feecref_sullivan2d(int levels, int r)
+{
+    MeshSimplicial2D M = UnitSquare2D_simple();
+    for (int l = 0; l < levels; l++) M.uniformrefinement();
+    M.automatic_dirichlet_flags();
+    SullivanResult& res = g_sullivan;
+    res = SullivanResult();
+    res.nv = M.count_simplices(0);
+    res.ne = M.count_simplices(1);
+    res.nt = M.count_simplices(2);
+    for (int e = 0; e < res.ne; e++) {
+        res.edges.push_back(M.get_subsimplex(1, 0, e, 0));
+        res.edges.push_back(M.get_subsimplex(1, 0, e, 1));
+        res.eflag.push_back(M.get_flag(1, e) == SimplexFlag::SimplexFlagDirichlet ? 1 : 0);
+    }
+    for (int v = 0; v < res.nv; v++)
+        res.vflag.push_back(M.get_flag(0, v) == SimplexFlag::SimplexFlagDirichlet ? 1 : 0);
+    for (int t = 0; t < res.nt; t++)
+        for (int i = 0; i < 3; i++) {
+            res.triangles.push_back(M.get_subsimplex(2, 0, t, i));
+            res.tri_edges.push_back(M.get_subsimplex(2, 1, t, i));
+        }
+    SparseMatrix vector_massmatrix = FEECBrokenMassMatrix(M, 2, 1, r - 1);
+    SparseMatrix diffmatrix = FEECBrokenDiffMatrix(M, 2, 0, r);
+    SparseMatrix incmatrix = FEECSullivanInclusionMatrix(M, 2, 0, r);
+    auto opr = diffmatrix & incmatrix;
+    auto opl = opr.getTranspose();
+    auto stiffness = opl & (vector_massmatrix & opr);
+    stiffness.sortentries();
+    const MatrixCSR csr(stiffness);
+    keep_csr(csr, res.A, res.C, res.V);
+    res.n = csr.getdimout();
+    return res.n;
+}
+
+// sizes: vertices, edges, triangles, unknowns, entries
+void feecref_sullivan2d_sizes(int* out)
+{
+    const SullivanResult& r = g_sullivan;
+    out[0] = r.nv;
+    out[1] = r.ne;
+    out[2] = r.nt;
+    out[3] = r.n;
+    out[4] = r.A.empty() ? 0 : r.A.back();
+}
+
+void feecref_sullivan2d_copy(int* edges, int* vflag, int* eflag, int* triangles, int* tri_edges,
+                             int* A, int* C, double* V)
+{
+    const SullivanResult& r = g_sullivan;
+    std::memcpy(edges, r.edges.data(), r.edges.size() * sizeof(int));
+    std::memcpy(vflag, r.vflag.data(), r.vflag.size() * sizeof(int));
+    std::memcpy(eflag, r.eflag.data(), r.eflag.size() * sizeof(int));
+    std::memcpy(triangles, r.triangles.data(), r.triangles.size() * sizeof(int));
+    std::memcpy(tri_edges, r.tri_edges.data(), r.tri_edges.size() * sizeof(int));
+    std::memcpy(A, r.A.data(), r.A.size() * sizeof(int));
+    std::memcpy(C, r.C.data(), r.C.size() * sizeof(int));
+    std::memcpy(V, r.V.data(), r.V.size() * sizeof(double));
 }
 
 // sizes: edges, entries of K, entries of M

@@ -285,10 +285,11 @@ def _row_lookup(table, rows):
     return idx
 
 
-def _reference_mesh_2d(levels: int):
+def _reference_mesh_2d(levels: int, edges: bool = False):
     """(lattice coordinates in units of 2^-levels, triangles) of
     UnitSquare2D_simple (mesh/examples2D.hpp:194-211) refined `levels` times,
-    numbered as the reference numbers them."""
+    numbered as the reference numbers them; with edges=True also (edge
+    vertex pairs, the three edges of every triangle: 01, 02, 12)."""
     s = 1 << levels
     X = np.array([[0, 0], [1, 0], [0, 1], [1, 1]], dtype=np.int64) * s
     TV = np.sort(np.array([[0, 1, 3], [0, 2, 3]], dtype=np.int64), axis=1)
@@ -316,6 +317,8 @@ def _reference_mesh_2d(levels: int):
         v01, v02, v12 = mid[:, 0], mid[:, 1], mid[:, 2]
         TV = np.concatenate([np.stack([v00, v01, v02], axis=1), np.stack([v01, v11, v12], axis=1),
                              np.stack([v02, v12, v22], axis=1), np.stack([v01, v02, v12], axis=1)])
+    if edges:
+        return X, TV, EV, TE
     return X, TV
 
 
@@ -707,6 +710,183 @@ def lagrange_pr(dim: int, n: int, r: int) -> CSRSystem:
     out = CSRSystem(f"lagrange{dim}d_n{n}_r{r}", N, np.cumsum(A).astype(np.int32),
                     (row + offsets[k]).astype(np.int32), V, dirichlet)
     _PR_CACHE[cache_key] = out
+    return out
+
+
+def _multi_indices(m: int, degree: int):
+    """generateMultiIndices over m entries of total `degree`, in the
+    reference's order (combinatorics/generatemultiindices.cpp:17-72): the
+    candidates 0, 1, 2, ... written in base degree+1 with entry 0 as the least
+    significant digit, those whose digits sum to `degree`."""
+    base = degree + 1
+    out = []
+    for cand in range(degree, base ** m):
+        digits, c = [], cand
+        for _ in range(m):
+            digits.append(c % base)
+            c //= base
+        if sum(digits) == degree:
+            out.append(tuple(digits))
+    return out
+
+
+def _seq_segment_sum(keys, vals):
+    """Unique keys (ascending) and, per key, the sum of its values added one
+    after another in the order given (a sequential left-to-right sum)."""
+    order = np.argsort(keys, kind="stable")
+    k, v = keys[order], vals[order]
+    uk, start, count = np.unique(k, return_index=True, return_counts=True)
+    acc = v[start].copy()
+    for t in range(1, int(count.max()) if len(count) else 0):
+        has = count > t
+        acc[has] += v[start[has] + t]
+    return uk, acc, count
+
+
+_SULLIVAN_CACHE = {}
+
+
+def sullivan_pr_2d(levels: int, r: int) -> CSRSystem:
+    """The reference's degree-r Lagrange stiffness in FEEC's Sullivan basis on
+    UnitSquare2D_simple refined `levels` times with automatic Dirichlet flags,
+    as tests/sullivan2D/poissondirichlet2D.cpp:120-151 composes it:
+    K = (D I)^T M_1 (D I) with the inclusion I = FEECSullivanInclusionMatrix
+    (fem/global.inclsullivan.cpp:19-196), the broken differential
+    D = FEECBrokenDiffMatrix (fem/global.diffmatrix.cpp:15-115) and the broken
+    mass matrix of degree r-1 one-forms M_1 = FEECBrokenMassMatrix
+    (fem/global.massmatrix.cpp:20-289), each product a SparseMatrixMultiplication
+    (sparse/sparsematrix.cpp:609-722: zeros dropped from both factors, products
+    of the shared index summed, zeros dropped again from the factors of the
+    next product), then MatrixCSR.
+
+    Unknowns: the vertices, then (r-1) per edge, then (r-1)(r-2)/2 per
+    triangle, each in the mesh's own numbering; a basis function is a
+    barycentric monomial lambda^alpha supported on the face alpha spans.  Rows
+    and columns of Dirichlet unknowns have no entries (the inclusion's entries
+    for Dirichlet faces are zero), so their diagonal is 0.  Sums of several
+    products run in the order of the shared index (the reference's merge
+    order; its final std::sort may reorder equal keys, which moves a value by
+    a few units in the last place at most)."""
+    key = (levels, r)
+    if key in _SULLIVAN_CACHE:
+        return _SULLIVAN_CACHE[key]
+    if not 1 <= r <= 6:
+        raise ValueError("degree r must be 1..6")
+    X, TV, EV, TE = _reference_mesh_2d(levels, edges=True)
+    nv, ne, nt = len(X), len(EV), len(TV)
+    # automatic_dirichlet_flags (mesh/mesh.cpp:340-363): edges of one triangle,
+    # and their vertices
+    edge_dir = np.bincount(TE.ravel(), minlength=ne) == 1
+    vert_dir = np.zeros(nv, dtype=bool)
+    vert_dir[EV[edge_dir].ravel()] = True
+
+    # continuous unknowns: ListOfSullivanIndices(d, 0, r) (fem/indexfunctions.cpp:15-84)
+    lists = [[a for a in _multi_indices(d + 1, r) if min(a) >= 1] for d in range(3)]
+    offset = [0, nv * len(lists[0]), nv * len(lists[0]) + ne * len(lists[1])]
+    n = offset[2] + nt * len(lists[2])
+
+    # the local broken space of degree r and the one-forms of degree r-1
+    broken = _multi_indices(3, r)
+    bidx = {a: i for i, a in enumerate(broken)}
+    low = _multi_indices(3, r - 1)
+    lidx = {a: i for i, a in enumerate(low)}
+    nb, nf = len(broken), 3 * len(low)
+
+    # inclusion: local broken function -> continuous unknown (-1: Dirichlet)
+    glob = np.full((nt, nb), -1, dtype=np.int64)
+    s_idx = np.arange(nt)
+    for fi in range(3):  # vertices
+        vol = [0, 0, 0]
+        vol[fi] = r
+        glob[:, bidx[tuple(vol)]] = np.where(vert_dir[TV[:, fi]], -1, offset[0] + TV[:, fi])
+    for fi, (a, b) in enumerate(((0, 1), (0, 2), (1, 2))):  # edges: TE[:, fi] joins a, b
+        e = TE[:, fi]
+        for ai, alpha in enumerate(lists[1]):
+            vol = [0, 0, 0]
+            vol[a], vol[b] = alpha
+            glob[:, bidx[tuple(vol)]] = np.where(edge_dir[e], -1,
+                                                 offset[1] + e * len(lists[1]) + ai)
+    for ai, alpha in enumerate(lists[2]):  # the triangle
+        glob[:, bidx[alpha]] = offset[2] + s_idx * len(lists[2]) + ai
+
+    # broken differential: d lambda^alpha = sum_p alpha_p lambda^(alpha - e_p) d lambda_p
+    Dloc = np.zeros((nf, nb))
+    for j, a in enumerate(broken):
+        for p in range(3):
+            if a[p] > 0:
+                new = list(a)
+                new[p] -= 1
+                Dloc[lidx[tuple(new)] * 3 + p, j] = a[p]
+
+    # per triangle: gradient products of the barycentric coordinates and area
+    # (mesh/mesh.cpp:525-542,971-1001), exact on this dyadic mesh
+    h = 1.0 / (1 << levels)
+    P = X[TV].astype(np.float64) * h                       # (nt, 3 vertices, 2)
+    J = np.stack([P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]], axis=2)  # columns v1-v0, v2-v0
+    JtJ = np.einsum("tki,tkj->tij", J, J)
+    det = JtJ[:, 0, 0] * JtJ[:, 1, 1] - JtJ[:, 0, 1] * JtJ[:, 1, 0]
+    inv = np.stack([np.stack([JtJ[:, 1, 1], -JtJ[:, 0, 1]], 1),
+                    np.stack([-JtJ[:, 1, 0], JtJ[:, 0, 0]], 1)], 1) / det[:, None, None]
+    Mul = np.array([[-1.0, 1.0, 0.0], [-1.0, 0.0, 1.0]])
+    G = np.einsum("ia,tij,jb->tab", Mul, inv, Mul)         # (nt, 3, 3)
+    area = np.abs(J[:, 0, 0] * J[:, 1, 1] - J[:, 0, 1] * J[:, 1, 0]) / 2.0
+
+    # polynomialmassmatrix(2, r-1) (fem/polynomialmassmatrix.cpp:15-60)
+    def fact(k):
+        out = 1.0
+        for i in range(2, k + 1):
+            out *= i
+        return out
+
+    polyMM = np.array([[(2.0 * np.prod([fact(x + y) for x, y in zip(a, b)])) /
+                        fact(2 + 2 * (r - 1)) for b in low] for a in low])
+    # M_1 = (polyMM (x) G) * area, entry (ip*3+if, jp*3+jf)
+    M1 = (polyMM[None, :, None, :, None] * G[:, None, :, None, :]).reshape(nt, nf, nf)
+    M1 = M1 * area[:, None, None]
+
+    # opr = D I: broken one-form row i, continuous column glob[j], value D[i, j]
+    # (one inclusion entry per broken function, so nothing is summed)
+    valid = glob >= 0                                      # (nt, nb)
+    # X = M_1 & opr: X[i, col(j)] = sum_k M_1[i, k] * D[k, j] over k ascending,
+    # nonzero factors only; `has` marks entries with at least one product
+    Xv = np.zeros((nt, nf, nb))
+    has = np.zeros((nt, nf, nb), dtype=bool)
+    for k in range(nf):
+        dk = Dloc[k]                                       # (nb,)
+        mk = M1[:, :, k]                                   # (nt, nf)
+        use = (mk != 0.0)[:, :, None] & (dk != 0.0)[None, None, :] & valid[:, None, :]
+        Xv = np.where(use, Xv + mk[:, :, None] * dk[None, None, :], Xv)
+        has |= use
+    keep = has & (Xv != 0.0)                               # dropped again as a factor
+
+    # K = opr^T & X: K[col(j), col(l)] = sum over one-form rows i (triangle,
+    # then local row ascending) of D[i, j] * X[i, l]
+    rows_l, cols_l, vals_l, ord_l = [], [], [], []
+    for i in range(nf):
+        for j in range(nb):
+            if Dloc[i, j] == 0.0:
+                continue
+            sel = valid[:, j][:, None] & keep[:, i, :]     # (nt, nb)
+            t, l = np.nonzero(sel)
+            rows_l.append(glob[t, j])
+            cols_l.append(glob[t, l])
+            vals_l.append(Dloc[i, j] * Xv[t, i, l])
+            ord_l.append(t * nf + i)                       # the shared index
+    rows = np.concatenate(rows_l)
+    cols = np.concatenate(cols_l)
+    vals = np.concatenate(vals_l)
+    order = np.argsort(np.concatenate(ord_l), kind="stable")
+    rows, cols, vals = rows[order], cols[order], vals[order]
+    keys, sums, _ = _seq_segment_sum(rows * n + cols, vals)
+    Rk, Ck = keys // n, keys % n
+    A = np.zeros(n + 1, dtype=np.int64)
+    np.add.at(A, Rk + 1, 1)
+    dirichlet = np.zeros(n, dtype=bool)
+    dirichlet[offset[0]:offset[1]] = vert_dir
+    dirichlet[offset[1]:offset[2]] = np.repeat(edge_dir, len(lists[1]))
+    out = CSRSystem(f"sullivan2d_L{levels}_r{r}", n, np.cumsum(A).astype(np.int32),
+                    Ck.astype(np.int32), sums, dirichlet)
+    _SULLIVAN_CACHE[key] = out
     return out
 
 

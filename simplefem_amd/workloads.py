@@ -1,12 +1,18 @@
 """Workloads of the mixed-precision, batched and served forms.
 
 * ``batched_systems``  256 independent 2D Poisson systems: per system a seeded
-  refinement level L in 4..8 (n = 2^L cells per axis) and Lagrange degree
-  r in 1..3, (r*2^L+1)^2 unknowns, each with 4 seeded U[-1,1) right-hand sides.
+  refinement level L in 4..8 (UnitSquare2D_simple refined L times) and
+  Lagrange degree r in 1..3, (r*2^L+1)^2 unknowns, each with 4 seeded
+  U[-1,1) right-hand sides.  The matrices are the reference's: its degree-r
+  stiffness in FEEC's Sullivan basis on its own refined mesh
+  (``systems.sullivan_pr_2d``).
 * ``served_requests``  a Poisson arrival stream; each request draws the
   dimension (2 or 3), a log-uniform unknown count in [lo, hi] and a degree
   1..3, and carries one seeded right-hand side.  Requests of equal shape share
-  their generated matrix (the generator is deterministic).
+  their generated matrix (the generator is deterministic).  Their meshes have
+  any number of cells per axis, which the reference's refinement (2^L only)
+  cannot produce, so they use this build's own nodal Lagrange assembly
+  (``systems.lagrange_pr``).
 """
 from __future__ import annotations
 
@@ -34,7 +40,7 @@ def batched_systems(count=256, nrhs=4, seed=2026, levels=(4, 8), degrees=(1, 3))
     for k in range(count):
         L = int(rng.integers(levels[0], levels[1] + 1))
         r = int(rng.integers(degrees[0], degrees[1] + 1))
-        s = S.lagrange_pr(2, 2 ** L, r)
+        s = S.sullivan_pr_2d(L, r)
         rhs = [S.rhs_uniform(s, seed * 100003 + k * 17 + j) for j in range(nrhs)]
         jobs.append(Job(s, rhs))
     return jobs

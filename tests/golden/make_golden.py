@@ -21,6 +21,10 @@ golden.json.  The reference's lowest-order Whitney 1-form curl-curl and mass
 matrices on the refined unit cube (FEEC inclusion, broken exterior derivative
 and broken mass matrices, tangential Dirichlet edges) are kept in full with the
 edges' vertices (whitney1_L*.npz); `--whitney-only` writes just those.
+The degree-r (1, 2, 3) Lagrange stiffness in FEEC's Sullivan basis on the
+refined unit square is kept in full at levels 1-3 (sullivan2d_L*_r*.npz) and
+as structure digests with sampled values at levels 5 and 6
+(sullivan2d_digests.json); `--sullivan-only` writes just those.
 """
 import hashlib
 import json
@@ -147,10 +151,49 @@ def whitney_fixtures(ref):
         print("whitney1", lv, len(w["edges"]), flush=True)
 
 
+# (level, degrees) of UnitSquare2D_simple whose Sullivan-basis stiffness is
+# kept in full; SULLIVAN_DIGEST_LEVELS keep SHA-256 digests of the structure
+SULLIVAN_LEVELS = (1, 2, 3)
+SULLIVAN_DIGEST_LEVELS = (5, 6)
+
+
+def sullivan_fixtures(ref):
+    """The reference's degree-r (1, 2, 3) stiffness in the Sullivan basis on the
+    refined unit square, composed as tests/sullivan2D/poissondirichlet2D.cpp
+    composes it: in full (sullivan2d_L*_r*.npz, with the mesh's edges and the
+    Dirichlet flags) at small levels; structure digests, a sample of values
+    and the largest magnitude at larger ones (sullivan2d_digests.json)."""
+    for lv in SULLIVAN_LEVELS:
+        for r in (1, 2, 3):
+            o = ref.sullivan2d(lv, r)
+            A, C, V = o["stiffness"]
+            np.savez_compressed(os.path.join(OUT, f"sullivan2d_L{lv}_r{r}.npz"), A=A, C=C, V=V,
+                                edges=o["edges"], vertex_dirichlet=o["vertex_dirichlet"],
+                                edge_dirichlet=o["edge_dirichlet"], triangles=o["triangles"],
+                                triangle_edges=o["triangle_edges"])
+            print("sullivan2d", lv, r, len(A) - 1, len(V), flush=True)
+    digests = {}
+    for lv in SULLIVAN_DIGEST_LEVELS:
+        for r in (1, 2, 3):
+            A, C, V = ref.sullivan2d(lv, r)["stiffness"]
+            pick = np.linspace(0, len(V) - 1, 64).astype(np.int64)
+            digests[f"L{lv}_r{r}"] = {"n": len(A) - 1, "nnz": len(V), "sha256_A": sha(A),
+                                      "sha256_C": sha(C), "sample_index": pick.tolist(),
+                                      "sample_value": V[pick].tolist(),
+                                      "max_abs": float(np.abs(V).max())}
+            print("sullivan2d digest", lv, r, flush=True)
+    with open(os.path.join(OUT, "sullivan2d_digests.json"), "w") as f:
+        json.dump(digests, f, indent=1, sort_keys=True)
+
+
 def main():
     if "--whitney-only" in sys.argv:
         with RefRunner(1) as ref:
             whitney_fixtures(ref)
+        return
+    if "--sullivan-only" in sys.argv:
+        with RefRunner(1) as ref:
+            sullivan_fixtures(ref)
         return
     if "--hodge-only" in sys.argv:  # adds the block-system fixtures to golden.json
         path = os.path.join(OUT, "golden.json")
@@ -164,6 +207,7 @@ def main():
         meta = {"desired_precision": ref.info()["desired_precision"], "small": {}, "full": {}}
         mesh_fixtures(ref, meta)
         whitney_fixtures(ref)
+        sullivan_fixtures(ref)
         for name, make in SMALL.items():
             s = make()
             b = S.rhs_uniform(s, 11)

@@ -142,9 +142,12 @@ def test_hcurl_mixed_precision_48_cube_against_fp64_gpu_solve():
 # ---------------------------------------------------------------- batch engine, degree 1..3
 
 def _lagrange_mix():
-    return [S.lagrange_pr(2, 4, 2), S.lagrange_pr(2, 8, 2), S.lagrange_pr(2, 4, 3),
-            S.lagrange_pr(2, 8, 3), S.lagrange_pr(2, 16, 3), S.lagrange_pr(2, 16, 1),
-            S.lagrange_pr(3, 3, 2), S.lagrange_pr(3, 4, 3)]
+    # the batched configuration's systems: the reference's degree-r stiffness
+    # in the Sullivan basis on its refined unit square (rows of Dirichlet
+    # unknowns empty), and nodal degree-r systems in 2D and 3D
+    return [S.sullivan_pr_2d(2, 2), S.sullivan_pr_2d(3, 2), S.sullivan_pr_2d(2, 3),
+            S.sullivan_pr_2d(3, 3), S.sullivan_pr_2d(4, 3), S.sullivan_pr_2d(4, 1),
+            S.lagrange_pr(2, 8, 3), S.lagrange_pr(3, 3, 2), S.lagrange_pr(3, 4, 3)]
 
 
 def _run(batch, slots):

@@ -310,3 +310,89 @@ def test_lagrange_pr_is_exact_on_quadratics(dim, n, r):
     assert len(rows) > 0
     expect = -2.0 * dim * w[rows]
     assert np.abs(Kf[rows] - expect).max() <= 1e-11 * max(np.abs(expect).max(), h ** dim)
+
+
+# ---------------------------------------------------------------- Sullivan basis, degree r
+
+def _close_to_reference(s, A, C, V):
+    """Structure bit for bit; every value within 2 units in the last place of
+    itself plus one of its row's largest magnitude (a sum of products whose
+    terms cancel lands within rounding of 0 in either summation order)."""
+    assert np.array_equal(s.A, A) and np.array_equal(s.C, C), s.name
+    rows = np.repeat(np.arange(s.n), np.diff(A))
+    rowmax = np.zeros(s.n)
+    np.maximum.at(rowmax, rows, np.abs(V))
+    tol = 2 * np.spacing(np.abs(V)) + np.spacing(rowmax[rows])
+    assert np.all(np.abs(s.V - V) <= tol), s.name
+
+
+@pytest.mark.parametrize("levels", [1, 2, 3])
+@pytest.mark.parametrize("r", [1, 2, 3])
+def test_sullivan_matches_the_reference_assembly(levels, r):
+    """The degree-r stiffness in FEEC's Sullivan basis as the reference's
+    FEECSullivanInclusionMatrix, FEECBrokenDiffMatrix and FEECBrokenMassMatrix
+    compose it (tests/sullivan2D/poissondirichlet2D.cpp:120-151; fixtures
+    sullivan2d_L*_r*.npz written from the compiled reference), with the
+    mesh's edges, triangle edges and Dirichlet flags."""
+    g = np.load(os.path.join(GOLDEN, f"sullivan2d_L{levels}_r{r}.npz"))
+    X, TV, EV, TE = S._reference_mesh_2d(levels, edges=True)
+    assert np.array_equal(EV, g["edges"]) and np.array_equal(TE, g["triangle_edges"])
+    assert np.array_equal(TV, g["triangles"])
+    s = S.sullivan_pr_2d(levels, r)
+    _close_to_reference(s, g["A"], g["C"], g["V"])
+    nv, ne = len(X), len(EV)
+    assert np.array_equal(s.dirichlet[:nv], g["vertex_dirichlet"])
+    assert np.array_equal(s.dirichlet[nv:nv + ne * (r - 1)],
+                          np.repeat(g["edge_dirichlet"], r - 1))
+    assert np.all(s.diagonal()[s.dirichlet] == 0) and np.all(s.diagonal()[~s.dirichlet] > 0)
+
+
+def test_sullivan_digests_at_larger_levels():
+    """Levels 5 and 6: SHA-256 of the reference's CSR structure and a sample
+    of its values (sullivan2d_digests.json)."""
+    import hashlib
+    import json
+
+    digests = json.load(open(os.path.join(GOLDEN, "sullivan2d_digests.json")))
+    sha = lambda a: hashlib.sha256(np.ascontiguousarray(a, dtype=np.int32).tobytes()).hexdigest()  # noqa: E731
+    for key, d in digests.items():
+        lv, r = (int(t[1:]) for t in key.split("_"))
+        s = S.sullivan_pr_2d(lv, r)
+        assert (s.n, s.nnz) == (d["n"], d["nnz"]), key
+        assert sha(s.A) == d["sha256_A"] and sha(s.C) == d["sha256_C"], key
+        got = s.V[np.array(d["sample_index"])]
+        want = np.array(d["sample_value"])
+        assert np.all(np.abs(got - want) <= 2 * np.spacing(np.abs(want))
+                      + np.spacing(d["max_abs"])), key
+
+
+def test_sullivan_degree_one_is_the_p1_stiffness():
+    """For r = 1 the Sullivan basis is the hat-function basis: the interior
+    block equals the reference-numbered P1 stiffness (its values are exact)."""
+    s = S.sullivan_pr_2d(4, 1)
+    p = S.poisson2d_p1(4, numbering="reference")
+    act = ~p.dirichlet
+    assert np.array_equal(s.dirichlet, p.dirichlet)
+    a = _csr(s)[act][:, act].toarray()
+    b = _csr(p)[act][:, act].toarray()
+    assert np.array_equal(a, b)
+
+
+@pytest.mark.parametrize("r", [2, 3])
+def test_sullivan_is_symmetric_positive_definite_on_interior_unknowns(r):
+    s = S.sullivan_pr_2d(3, r)
+    act = ~s.dirichlet
+    M = _csr(s)[act][:, act].toarray()
+    assert np.allclose(M, M.T, rtol=0, atol=1e-14 * np.abs(M).max())
+    assert np.linalg.eigvalsh((M + M.T) / 2).min() > 0
+
+
+def test_sullivan_live_against_the_compiled_reference():
+    import oracle
+    if not oracle.ref_available():
+        pytest.skip("oracle/_ref not built (no reference sources)")
+    from oracle.ref import RefRunner
+
+    with RefRunner(1) as ref:
+        for r in (1, 2, 3):
+            _close_to_reference(S.sullivan_pr_2d(4, r), *ref.sullivan2d(4, r)["stiffness"])
