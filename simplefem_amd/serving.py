@@ -15,11 +15,14 @@ system per call and has no server; this is the request loop around that call.
   ``dedicated_workers`` host threads (its own stream, overlapping the batch
   on the GPU).  The colouring and device-order plan of a batched request are
   built on ``prep_workers`` host threads right after it arrives.
-* Admission: requests reach the device in arrival order under ONE budget of
-  device bytes covering both routes -- the batch engine's resident bytes
+* Admission: requests reach the device under ONE budget of device bytes
+  covering both routes -- the batch engine's resident bytes
   (``sfa_batch_memory``) plus an estimate for every dedicated System in
-  flight (``dedicated_bytes``).  A request that would exceed the budget waits
-  until earlier ones finish; one larger than the whole budget is rejected.
+  flight (``dedicated_bytes``) -- in arrival order among the requests that
+  are ready (a batched request whose host plan is still being built lets
+  later ones pass).  A request that would exceed the budget waits, and so do
+  all after it, until earlier ones finish; one larger than the whole budget
+  is rejected.
 * Matrices, colourings and Krylov state stay on the device while a request
   runs and are freed when its result is collected.
 * Every result is checked: status "ok" only if the solve reports success and
@@ -224,9 +227,13 @@ class SolveServer:
                     r = arrivals.pop(0)
                     fut = prep_pool.submit(_prepare, r) if self.route_of(r) == "batch" else None
                     pending.append((r, fut))
-                # admission, in arrival order, under one budget for both routes
-                while pending:
-                    r, fut = pending[0]
+                # admission under one budget for both routes, in arrival order
+                # among the requests that are ready (a batched request is
+                # ready once its host plan is built; one still being planned
+                # lets later ones pass rather than hold them back)
+                i = 0
+                while i < len(pending):
+                    r, fut = pending[i]
                     need = self.bytes_of(r)
                     b = np.ascontiguousarray(r.rhs, dtype=np.float64)
                     target = _target(b, self.precision)
@@ -236,7 +243,10 @@ class SolveServer:
                         results[r.id] = Result(r.id, "none", "rejected", target=target,
                                                arrival=r.arrival, finished=now)
                         st.rejected += 1
-                        pending.pop(0)
+                        pending.pop(i)
+                        continue
+                    if fut is not None and not fut.done():
+                        i += 1
                         continue
                     used = (eng.memory() if in_batch else 0.0) + ded_bytes
                     if used + need > self.budget and (in_batch or in_dedicated):
@@ -248,8 +258,6 @@ class SolveServer:
                         ded_bytes += need
                         st.dedicated += 1
                     else:
-                        if not fut.done():
-                            break
                         prep = fut.result()
                         mid = eng.add_prepared(prep)
                         prep.close()
@@ -260,7 +268,7 @@ class SolveServer:
                     # what is now held: the engine's actual bytes, the dedicated estimates
                     st.peak_bytes = max(st.peak_bytes,
                                         (eng.memory() if in_batch else 0.0) + ded_bytes)
-                    pending.pop(0)
+                    pending.pop(i)
                 if in_batch:
                     eng.run(self.engine_steps)
                 for slot in eng.finished():
