@@ -1323,12 +1323,7 @@ struct Engine {
         build_graph();
         if (max_steps > 0) {
             const int reps = std::max(1, max_steps / kGraphSteps);
-            // at most kMaxQueuedDispatches kernels queued at once
-            const int group = std::max(1, kMaxQueuedDispatches / (3 * kGraphSteps));
-            for (int i = 0; i < reps; i++) {
-                if (i > 0 && i % group == 0) SFA_HIP(hipStreamSynchronize(stream));
-                SFA_HIP(hipGraphLaunch(graph, stream));
-            }
+            for (int i = 0; i < reps; i++) SFA_HIP(hipGraphLaunch(graph, stream));
         } else {
             // the first replay also schedules slots admitted since the last one
             do {

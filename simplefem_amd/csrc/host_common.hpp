@@ -18,11 +18,6 @@ namespace sfa {
 // message of the calling thread's last failed C-ABI call
 std::string& last_error();
 
-// Kernels a host loop may queue on a stream (as replayed graphs) before it
-// waits for them: bounds the work in flight (and what a profiler that
-// intercepts every dispatch has to track) to ~1,000 dispatches.
-constexpr int kMaxQueuedDispatches = 1024;
-
 // A system's device-pointer solves sum ||b|| for their tolerance in row
 // order (the reference's bits) instead of by blocks; for the inner solves of
 // the block solvers, whose right-hand sides are short.

@@ -729,15 +729,12 @@ struct SystemT {
         // how many chunks to queue before the next look at the state.
         const int G = 4;
         build_chunk_graph(G);
-        // at most kMaxQueuedDispatches kernels queued between looks at the
-        // state, however many launches an iteration takes
-        const int max_chunks = std::max(1, kMaxQueuedDispatches / (G * kernels_per_iteration()));
-        int queued = std::min(4, max_chunks);
+        int queued = 4;
         while (!host_state->done) {
             for (int i = 0; i < queued; i++) SFA_HIP(hipGraphLaunch(chunk_exec, s));
             read_state();
             if (host_state->done) break;
-            queued = std::min(predict_chunks(G), max_chunks);
+            queued = predict_chunks(G);
         }
     }
 
