@@ -11,10 +11,14 @@ seeded U[-1,1) right-hand side on the active rows.
 A step is one complete solve with the matrix, colouring and right-hand side
 already resident in HBM.  Prints one JSON line with solves/s, the roofline of
 the dominant kernel class (chosen by per-launch HIP events in a profiled solve
-of the same system, its launch time measured by replaying 200 iterations'
-launches of that class in a recorded graph between two HIP events) and the CPU
-baseline (the reference's own solver, compiled from its sources, timed on this
-host on a bounded sample).
+of the same system; its launch duration is the kernel trace's, from the
+committed profile whose source digest matches the sources built, with the
+live graph-replay time beside it) and the CPU baseline (the reference's own
+solver, compiled from its sources, timed on this host on a bounded sample, at
+all threads of the host's share and at one).  The same line carries the
+host-array drop-in call's cost per call, the 2D natural-order SSOR
+configuration (GPU and the reference at one and all threads) and, under
+"forms", the mixed-precision, batched and served configurations.
 """
 import argparse
 import json
@@ -326,6 +330,9 @@ def main():
     ap.add_argument("--no-profile", action="store_true")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the per-call drop-in timing and the 2D natural-order SSOR lines")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the precision, batched and served configurations (the 'forms' "
+                         "key of the default line)")
     ap.add_argument("--numbering", default="reference", choices=["reference", "lexicographic"],
                     help="vertex numbering of the 64^3 system: the reference's uniform-refinement "
                          "order (its CSR structure and colouring; default) or x-fastest")
@@ -525,8 +532,36 @@ def main():
         out["c1_ssor_2d"] = c1_natural_ssor(args.precision, args.omega,
                                             with_cpu=not args.no_cpu_baseline)
         log(f"c1: {out['c1_ssor_2d']}")
+    if not args.no_forms:
+        out["forms"] = forms_summary(args)
     print(json.dumps(out), flush=True)
     return 0
+
+
+def forms_summary(args):
+    """The precision, batched and served configurations, each run as
+    `bench.py --config <name> --no-cpu-baseline` runs it, in one line each
+    (per-mapping tuning tables left out)."""
+    from simplefem_amd import forms_bench
+
+    def strip(d):
+        if isinstance(d, dict):
+            return {k: strip(v) for k, v in d.items() if k != "spmv_mapping"}
+        return d
+
+    out = {}
+    for name in ("hcurl-mixed", "batched", "served"):
+        fargs = argparse.Namespace(**vars(args))
+        fargs.config = name
+        fargs.no_cpu_baseline = True
+        log(f"form {name} ...")
+        t0 = time.perf_counter()
+        res = forms_bench.run(fargs)
+        out[name] = {"value": res["value"], "unit": res["unit"], "dtype": res["dtype"],
+                     "ms_per_solve": res["ms_per_step"], "seconds": round(time.perf_counter() - t0, 1),
+                     "config": strip(res["config"])}
+        log(f"form {name}: {res['value']} {res['unit']}")
+    return out
 
 
 if __name__ == "__main__":
