@@ -216,7 +216,11 @@ int sfa_system_create(int method, int N,
 /* Solve with the system's matrix.  x, b, residual are N-vectors with the
  * same meaning as in the drop-in call; device_pointers = 1 means they are
  * device (HBM) addresses on the system's GPU, 0 that they are host arrays.
- * Returns once the solve has finished. */
+ * Device inputs are read after all work queued on the null stream before the
+ * call (the system's own stream waits for it; on a stream set with
+ * sfa_system_set_stream the caller orders them).  The same holds for
+ * sfa_system_solve_mixed, sfa_system_apply and the Hodge solve with device
+ * pointers.  Returns once the solve has finished. */
 int sfa_system_solve(sfa_system sys, double* x, const double* b,
                      double* residual, double precision, int print_modulo,
                      int device_pointers, sfa_solve_info* info);
@@ -424,6 +428,18 @@ int sfa_batch_submit(sfa_batch b, int matrix_id, const double* rhs,
                      const double* x0, double precision, double inner_rtol,
                      int max_outer, int* slot);
 
+/* Queues nrhs (1..4) solves of matrix_id at once: right-hand side k at
+ * rhs + k N, initial guess at x0 + k N (x0 NULL: all 0).  They take nrhs
+ * consecutive slots (slots[0..nrhs)) whose vectors are stored interleaved,
+ * row i of solve k at i nrhs + k, so the solves riding together gather a
+ * column's values from one cache line (four of them with one vector load);
+ * each solve's arithmetic, and so its result, is that of sfa_batch_submit.
+ * The block is freed with the last of its slots.  No reference counterpart
+ * (the reference solves one right-hand side per call). */
+int sfa_batch_submit_group(sfa_batch b, int matrix_id, int nrhs, const double* rhs,
+                           const double* x0, double precision, double inner_rtol,
+                           int max_outer, int* slots);
+
 /* Runs max_steps steps rounded down to whole graph replays of 32 steps, at
  * least one replay (0: until no solve is running); *running receives the
  * number of solves still running. */
@@ -440,11 +456,11 @@ int sfa_batch_collect(sfa_batch b, int slot, double* x, sfa_batch_info* info);
 int sfa_batch_memory(sfa_batch b, long long* bytes);
 
 /* Steps the engine has executed since the batch was created (each step runs
- * the current phase of every running solve: three kernels). */
+ * the current phase of every running solve: four kernels). */
 int sfa_batch_steps(sfa_batch b, long long* steps);
-/* Runs `steps` engine steps with each of the three kernels launched on its
- * own between two HIP events and adds their summed times, in milliseconds,
- * to ms[0] (step kernel), ms[1] (per-slot fold) and ms[2] (schedule).  The
+/* Runs `steps` engine steps with the kernels launched on their own between
+ * HIP events and adds their summed times, in milliseconds, to ms[0] (the two
+ * step kernels), ms[1] (per-slot fold) and ms[2] (schedule).  The
  * solves advance exactly as under sfa_batch_run.  No reference counterpart. */
 int sfa_batch_profile(sfa_batch b, int steps, double* ms);
 

@@ -288,6 +288,16 @@ struct HodgeSolver {
         if (!(precision > 0)) throw Error{SFA_ERR_INVALID, "precision must be positive"};
         SFA_HIP(hipSetDevice(device));
         const hipMemcpyKind in = device_pointers ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+        if (device_pointers) {
+            // inputs written on the null stream, which this non-blocking
+            // stream does not wait for by itself
+            hipEvent_t ev = nullptr;
+            SFA_HIP(hipEventCreateWithFlags(&ev, hipEventDisableTiming));
+            const hipError_t e1 = hipEventRecord(ev, nullptr);
+            const hipError_t e2 = e1 == hipSuccess ? hipStreamWaitEvent(stream, ev, 0) : e1;
+            (void)hipEventDestroy(ev);
+            SFA_HIP(e2);
+        }
         SFA_HIP(hipMemcpyAsync(x, ux, N * sizeof(double), in, stream));
         SFA_HIP(hipMemcpyAsync(b, ub, N * sizeof(double), in, stream));
         // tolerance from ||b|| (:457-461), summed in row order on the host

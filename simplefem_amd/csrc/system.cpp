@@ -112,6 +112,7 @@ struct SystemT {
 
     hipStream_t own_stream = nullptr;
     hipStream_t stream = nullptr;
+    hipEvent_t null_done = nullptr;  // see order_after_null_stream
     hipGraphExec_t chunk_exec = nullptr;
     std::unique_ptr<GraphRecorder> chunk_graph;  // the recorded graph and its arguments, kept with the exec
     int chunk_iters = 0;
@@ -160,6 +161,21 @@ struct SystemT {
         host_state = nullptr;
         if (own_stream) (void)hipStreamDestroy(own_stream);
         own_stream = stream = nullptr;
+        if (null_done) (void)hipEventDestroy(null_done);
+        null_done = nullptr;
+    }
+
+    // Device-pointer inputs may have been written by work queued on the null
+    // stream (hipMemset or hipMemcpy there, a framework's default stream),
+    // which the system's own stream, created non-blocking, does not wait for:
+    // it waits for everything queued there so far.  A stream the caller set
+    // (sfa_system_set_stream) is ordered by the caller.
+    void order_after_null_stream(hipStream_t s)
+    {
+        if (s != own_stream) return;
+        if (!null_done) SFA_HIP(hipEventCreateWithFlags(&null_done, hipEventDisableTiming));
+        SFA_HIP(hipEventRecord(null_done, nullptr));
+        SFA_HIP(hipStreamWaitEvent(s, null_done, 0));
     }
 
     void create(int method_, int n_, const int* A, const int* C, const double* V, const double* aux,
@@ -667,6 +683,8 @@ struct SystemT {
             src_x = io;
             src_b = io2;
             src_r = io3;
+        } else {
+            order_after_null_stream(s);
         }
         launch_gather(s, x, src_x, perm, n);
         launch_gather(s, b, src_b, perm, n);
@@ -888,9 +906,10 @@ struct SystemT {
         hipStream_t s = stream;
         // source with a zero slot at n: use the x-sized buffer m
         T* srcbuf = q;
-        if (device_pointers)
+        if (device_pointers) {
+            order_after_null_stream(s);
             SFA_HIP(hipMemcpyAsync(srcbuf, src, n * sizeof(T), hipMemcpyDeviceToDevice, s));
-        else
+        } else
             SFA_HIP(hipMemcpyAsync(srcbuf, src, n * sizeof(T), hipMemcpyHostToDevice, s));
         launch_fill(s, srcbuf + n, T(0), 1);
         T* out = device_pointers ? dest : io;
@@ -934,6 +953,8 @@ struct SystemT {
             SFA_HIP(hipMemcpyAsync(io2, ub, n * sizeof(T), hipMemcpyHostToDevice, s));
             src_x = io;
             src_b = io2;
+        } else {
+            order_after_null_stream(s);
         }
         launch_gather(s, x, src_x, perm, n);
         launch_gather(s, b, src_b, perm, n);
