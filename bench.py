@@ -357,6 +357,9 @@ def main():
                          "chunks of 2000 steps) and print that table; hcurl-mixed: only run one "
                          "FP64 solve with every kernel launched on its own (no graphs) between "
                          "HIP events and print the per-class table")
+    ap.add_argument("--rhs-layout", default="grouped", choices=["grouped", "separate"],
+                    help="batched: a system's right-hand sides as one interleaved slot group "
+                         "(sfa_batch_submit_group; default) or as separate slots")
     ap.add_argument("--inner-restarts", action="store_true",
                     help="mixed forms: restart every inner CG (plain iterative refinement) "
                          "instead of continuing the previous inner solve's direction")

@@ -178,6 +178,8 @@ _SIGNATURES = {
     "sfa_prepared_destroy": ([c_void_p], c_int),
     "sfa_batch_submit": ([c_void_p, c_int, _P_D, _P_D, c_double, c_double, c_int,
                           POINTER(c_int)], c_int),
+    "sfa_batch_submit_group": ([c_void_p, c_int, c_int, _P_D, _P_D, c_double, c_double, c_int,
+                                _P_I], c_int),
     "sfa_batch_run": ([c_void_p, c_int, POINTER(c_int)], c_int),
     "sfa_batch_finished": ([c_void_p, _P_I, c_int, POINTER(c_int)], c_int),
     "sfa_batch_collect": ([c_void_p, c_int, _P_D, POINTER(BatchInfo)], c_int),
@@ -581,6 +583,27 @@ class Batch:
         _check(load().sfa_batch_submit(self._h, int(mid), _d(b), _d(x0), float(precision),
                                        float(inner_rtol), int(max_outer), ctypes.byref(slot)))
         return slot.value
+
+    def submit_group(self, mid, bs, x0s=None, precision=1e-10, inner_rtol=1e-1, max_outer=50):
+        """Up to four right-hand sides of one matrix in consecutive slots
+        whose vectors are interleaved (sfa_batch_submit_group); returns their
+        slots in order.  Each result equals that of submit()."""
+        N = self.n_of[mid]
+        bs = [_f64(b, "b", N)[:N] for b in bs]
+        k = len(bs)
+        if not 1 <= k <= 4:
+            raise ValueError("a group holds 1 to 4 right-hand sides")
+        flat = np.ascontiguousarray(np.concatenate(bs))
+        x0 = None
+        if x0s is not None:
+            if len(x0s) != k:
+                raise ValueError("one initial guess per right-hand side")
+            x0 = np.ascontiguousarray(np.concatenate([_f64(x, "x0", N)[:N] for x in x0s]))
+        out = np.full(k, -1, dtype=np.int32)
+        _check(load().sfa_batch_submit_group(self._h, int(mid), k, _d(flat), _d(x0),
+                                             float(precision), float(inner_rtol),
+                                             int(max_outer), _i(out)))
+        return out.tolist()
 
     def run(self, max_steps=0):
         running = c_int(0)

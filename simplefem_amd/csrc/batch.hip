@@ -17,11 +17,13 @@
 // the same phases as the single-system kernels, with the same per-row
 // arithmetic.  A "step" executes the current phase of every active slot; a
 // phase is split into work units of 64 rows (one SELL-64 slice, or 64
-// vector rows), each run by one wavefront.  A step is three kernels:
+// vector rows), each run by one wavefront.  A step is four kernels:
 //
-//   k_batch_step    every unit of the step's schedule (each wavefront
-//                   grid-strides over units on its own); each unit stores
-//                   its partial sum in its slot's partials[j];
+//   k_batch_step    (two launches: the units of complete interleaved groups
+//                   of four right-hand sides, then all others) every unit of
+//                   the step's schedule (each wavefront grid-strides over
+//                   units on its own); each unit stores its partial sum in
+//                   its slot's partials[j];
 //   k_batch_fold    one block per slot: sums the slot's unit partials in unit
 //                   order, runs the slot's scalar step (alpha / beta / exit
 //                   tests) and advances its program, then counts the units
@@ -29,7 +31,7 @@
 //   k_batch_prefix  one block: prefix sums of the unit counts -> the next
 //                   step's schedule, and the step counter.
 //
-// Kernel boundaries order the three, so no kernel waits on another block:
+// Kernel boundaries order the four, so no kernel waits on another block:
 // there are no atomics, fences or spin loops.  The host replays a captured
 // graph of a fixed number of steps; a step with no scheduled units is a
 // no-op, so replays past the end change nothing.  Between replays the host
@@ -100,7 +102,11 @@ struct BSlot {
     double zr, dAd, alpha, beta, tol2, rel2, target64, res64;
     double zr_cont;     // > 0: z.r of the iterate that formed d, which the next inner solve continues
     int k_cont;         // iterations of the inner solve that left that direction
-    int pad_;
+    // The slot's vectors are lane `lane` of a group of `ls` slots submitted
+    // together, stored interleaved: element i of each vector at v[i * ls]
+    // (v points at the lane's first element).  A slot submitted alone has
+    // ls = 1.  vgroup: the group's first slot.
+    int ls, lane, vgroup;
     T *x, *r, *b, *d, *z, *q, *m;
     double *x64, *b64, *r64;
     double* partials;
@@ -110,10 +116,12 @@ struct BSlot {
 struct BSched {
     unsigned step;      // steps completed
     int nslots;
-    int W[2];           // units of the step of parity p
+    int W[2];           // units of the step of parity p: slots outside complete groups
+    int WG[2];          // and leaders of complete interleaved groups (see k_batch_prefix)
     int active;         // slots with units in the next step
     int pad;
     int start[2][kMaxSlots + 1];  // unit offsets of the slots in the step of parity p
+    int startG[2][kMaxSlots + 1]; // the same for the complete groups' units
     int ran[2][kMaxSlots];        // units of the phase each slot runs in that step
     int riders[2][kMaxSlots];     // slots a leading slot's units serve (itself included), else 0
 };
@@ -289,42 +297,68 @@ __device__ void phase_complete(BSlot<T>& S, const BMat<T>& M, double total)
     }
 }
 
-// q = A d over the rows of SpMV task `task` of a matrix whose SpMV runs K
-// lanes per row (task i: slice i/K, part i%K of 64/K rows); returns the
-// lane's share of d.Ad (the row's first lane holds the row's product).
-template <int K, typename T>
-__device__ double spmv_task(const BMat<T>& M, const T* __restrict__ d, T* __restrict__ q,
-                            int task, int lane)
+// acc[k] (+|-)= sum_e v_e * g[k][c_e] over one SELL row for the kRiders
+// lanes of one interleaved group (lane k's element i at base[i * kRiders +
+// k]): the four values of a column come in one vector load.  Each rider's sum runs over the entries in order, as sell_row forms
+// it (the unroll groups loads only).
+template <bool SUB, typename TV>
+__device__ inline void sell_row_group(const SellViewT<TV>& A, int slice, int lane,
+                                      const TV* __restrict__ base, TV* acc)
 {
-    const int slice = task / K;
-    if (slice >= M.full_slices) return 0.;
-    const int row = group_row_at(M.full, slice, slice, sellk_slice_row<K>(task % K, lane),
-                                 M.full_row0, M.n_active);
-    if (row < 0) return 0.;  // every lane of a row agrees
-    const T acc = sellk_row<K, false>(M.full, slice, task % K, lane, d, T(0));
-    if (lane % K != 0) return 0.;
-    q[row] = acc;
-    return (double)(d[row] * acc);
-}
-
-template <typename T>
-__device__ double spmv_task_any(const BMat<T>& M, const T* d, T* q, int task, int lane)
-{
-    switch (M.spmv_lanes) {
-    case 2: return spmv_task<2>(M, d, q, task, lane);
-    case 4: return spmv_task<4>(M, d, q, task, lane);
-    case 8: return spmv_task<8>(M, d, q, task, lane);
-    default: return spmv_task<1>(M, d, q, task, lane);
+    typedef TV V4 __attribute__((ext_vector_type(kRiders)));
+    const V4* g4 = reinterpret_cast<const V4*>(base);
+    const int beg = A.off[slice] + lane;
+    const int end = A.off[slice + 1];
+    // entries per unrolled step: the gathered values held are U * kRiders
+    // (FP64 U = 2 and FP32 U = 4 keep the group kernel at 68 / 57 VGPRs)
+    constexpr int U = sizeof(TV) == 8 ? 2 : 4;
+    int e = beg;
+    for (; e + (U - 1) * kWave < end; e += U * kWave) {
+        unsigned c[U];
+        TV v[U];
+        V4 g[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            c[u] = (unsigned)__builtin_nontemporal_load(&A.cols[e + u * kWave]);
+            v[u] = __builtin_nontemporal_load(&A.vals[e + u * kWave]);
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) g[u] = g4[c[u]];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+#pragma unroll
+            for (int k = 0; k < kRiders; k++) {
+                if (SUB) acc[k] -= v[u] * g[u][k];
+                else     acc[k] += v[u] * g[u][k];
+            }
+        }
+    }
+    for (; e < end; e += kWave) {
+        const unsigned c = (unsigned)__builtin_nontemporal_load(&A.cols[e]);
+        const TV v = __builtin_nontemporal_load(&A.vals[e]);
+        const V4 g = g4[c];
+#pragma unroll
+        for (int k = 0; k < kRiders; k++) {
+            if (SUB) acc[k] -= v * g[k];
+            else     acc[k] += v * g[k];
+        }
     }
 }
 
-// acc[k] (+|-)= sum_e v_e * vec[k][c_e] over one SELL row for riders k < nr,
-// each entry's column and value loaded once for all riders; every rider's
-// sum runs over the entries in order, as sell_row forms it.
-template <bool SUB, typename TV>
+// acc[k] (+|-)= sum_e v_e * vec[k][c_e * ls] over one SELL row for riders
+// k < nr whose vectors have stride ls, each entry's column and value loaded
+// once for all riders; every rider's sum runs over the entries in order, as
+// sell_row forms it.  G: acc[] is indexed by the lanes of an interleaved
+// group whose lane 0 is vec[0] (see run_unit), which takes sell_row_group.
+template <bool SUB, bool G, typename TV>
 __device__ inline void sell_row_multi(const SellViewT<TV>& A, int slice, int lane,
-                                      const TV* const* vec, int nr, TV* acc)
+                                      const TV* const* vec, int nr, int ls, TV* acc)
 {
+    if constexpr (G) {
+        sell_row_group<SUB>(A, slice, lane, vec[0], acc);
+        return;
+    }
+    const unsigned st = (unsigned)ls;
     const int beg = A.off[slice] + lane;
     const int end = A.off[slice + 1];
     constexpr int U = 8;
@@ -334,7 +368,7 @@ __device__ inline void sell_row_multi(const SellViewT<TV>& A, int slice, int lan
         TV v[U];
 #pragma unroll
         for (int u = 0; u < U; u++) {
-            c[u] = (unsigned)__builtin_nontemporal_load(&A.cols[e + u * kWave]);
+            c[u] = (unsigned)__builtin_nontemporal_load(&A.cols[e + u * kWave]) * st;
             v[u] = __builtin_nontemporal_load(&A.vals[e + u * kWave]);
         }
 #pragma unroll
@@ -352,7 +386,7 @@ __device__ inline void sell_row_multi(const SellViewT<TV>& A, int slice, int lan
         }
     }
     for (; e < end; e += kWave) {
-        const unsigned c = (unsigned)__builtin_nontemporal_load(&A.cols[e]);
+        const unsigned c = (unsigned)__builtin_nontemporal_load(&A.cols[e]) * st;
         const TV v = __builtin_nontemporal_load(&A.vals[e]);
 #pragma unroll
         for (int k = 0; k < kRiders; k++) {
@@ -364,17 +398,124 @@ __device__ inline void sell_row_multi(const SellViewT<TV>& A, int slice, int lan
     }
 }
 
+// q = A d over the rows of SpMV task `task` of a matrix whose SpMV runs K
+// lanes per row (task i: slice i/K, part i%K of 64/K rows), for vectors
+// stored with stride ls; returns the lane's share of d.Ad (the row's first
+// lane holds the row's product).
+template <int K, typename T>
+__device__ double spmv_task(const BMat<T>& M, const T* __restrict__ d, T* __restrict__ q,
+                            int task, int lane, int ls)
+{
+    const int slice = task / K;
+    if (slice >= M.full_slices) return 0.;
+    const int row = group_row_at(M.full, slice, slice, sellk_slice_row<K>(task % K, lane),
+                                 M.full_row0, M.n_active);
+    if (row < 0) return 0.;  // every lane of a row agrees
+    T acc;
+    if constexpr (K == 1) {  // the step kernel's one-lane SpMV code, for one rider
+        const T* ds[kRiders] = {d, d, d, d};
+        T a[kRiders] = {T(0), T(0), T(0), T(0)};
+        sell_row_multi<false, false>(M.full, slice, lane, ds, 1, ls, a);
+        acc = a[0];
+    } else {
+        SellChunk<T, 8> R;
+        sellk_prefetch<K>(M.full, slice, slice, 0, -1, task % K, lane, R);
+        acc = sellk_finish<K, false>(M.full, R, [&](unsigned c) { return d[c * (unsigned)ls]; },
+                                     T(0), lane);
+    }
+    if (lane % K != 0) return 0.;
+    const unsigned i = (unsigned)row * (unsigned)ls;
+    q[i] = acc;
+    return (double)(d[i] * acc);
+}
+
+template <typename T>
+__device__ double spmv_task_any(const BMat<T>& M, const T* d, T* q, int task, int lane, int ls)
+{
+    switch (M.spmv_lanes) {
+    case 2: return spmv_task<2>(M, d, q, task, lane, ls);
+    case 4: return spmv_task<4>(M, d, q, task, lane, ls);
+    case 8: return spmv_task<8>(M, d, q, task, lane, ls);
+    default: return spmv_task<1>(M, d, q, task, lane, ls);
+    }
+}
+
+// Row `row` of the vector that member vp of each rider points to (T or
+// double), for riders k < nr with vectors of stride ls.  G: SS[0] is lane 0
+// of an interleaved group of kRiders lanes, and the four lanes' values come
+// with one vector access (lanes that do not ride are read and ignored).
+template <bool G, typename V, typename T>
+__device__ inline void row_load(const BSlot<T>* SS, int nr, int ls, V* BSlot<T>::*vp,
+                                int row, V* out)
+{
+    if constexpr (G) {
+        typedef V V4 __attribute__((ext_vector_type(kRiders)));
+        const V4 v = *reinterpret_cast<const V4*>(SS[0].*vp + (unsigned)row * kRiders);
+#pragma unroll
+        for (int k = 0; k < kRiders; k++) out[k] = v[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < kRiders; k++)
+            if (k < nr) out[k] = (SS[k].*vp)[(unsigned)row * (unsigned)ls];
+    }
+}
+
+// Stores in[k] for the riders (G: lanes) lo <= k < hi.  G with all four
+// lanes riding: one vector store.
+template <bool G, typename V, typename T>
+__device__ inline void row_store(const BSlot<T>* SS, int lo, int hi, int ls, V* BSlot<T>::*vp,
+                                 int row, const V* in)
+{
+    if constexpr (G) {
+        V* base = SS[0].*vp + (unsigned)row * kRiders;
+        if (lo == 0 && hi == kRiders) {
+            typedef V V4 __attribute__((ext_vector_type(kRiders)));
+            V4 v;
+#pragma unroll
+            for (int k = 0; k < kRiders; k++) v[k] = in[k];
+            *reinterpret_cast<V4*>(base) = v;
+        } else {
+#pragma unroll
+            for (int k = 0; k < kRiders; k++)
+                if (k >= lo && k < hi) base[k] = in[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kRiders; k++)
+            if (k < hi) (SS[k].*vp)[(unsigned)row * (unsigned)ls] = in[k];
+    }
+}
+
 // One work unit of the current phase of slots SS[0..nr) -- adjacent slots of
-// one matrix in the same phase ("riders"; nr = 1 for a slot on its own) --
-// run by one wavefront: slice j of the phase's part (SpMV task j), or vector
-// rows 64j .. 64j+63.  Matrix entries are loaded once for all riders; each
-// rider's rows get exactly the arithmetic they get alone.  part[k]: the
-// lane's share of rider k's reduction.
-template <typename T, bool WIDE>
+// one matrix in the same phase ("riders"; nr = 1 for a slot on its own),
+// whose vectors all have the stride ls -- run by one wavefront: slice j of
+// the phase's part (SpMV task j), or vector rows 64j .. 64j+63.  Matrix
+// entries are loaded once for all riders; each rider's rows get exactly the
+// arithmetic they get alone.  part[k]: the lane's share of rider k's
+// reduction.
+//
+// G: the riders are lanes k0 .. k0+nr-1 of an interleaved group of kRiders
+// (k0 = SS[0].lane).  The unit then works in lane space: SS is moved to the
+// group's lane 0, every loaded row and gathered column brings all four
+// lanes in one vector access, the arithmetic runs for all four (a lane
+// that does not ride computes values nobody stores), and only the riding
+// lanes' values are stored; part[] is indexed by lane.  Never with a
+// multi-lane SpMV (k_batch_prefix leaves those to G = false).
+template <typename T, bool WIDE, bool G>
 __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind, int g, int j,
                          double* part)
 {
+    using S_ = BSlot<T>;
     const int lane = threadIdx.x & 63;
+    int lo = 0, hi = nr;  // riders (G: lanes) whose values are stored
+    int ls = SS[0].ls;
+    if constexpr (G) {
+        lo = SS[0].lane;
+        hi = lo + nr;
+        SS -= lo;
+        nr = kRiders;
+        ls = kRiders;
+    }
 #pragma unroll
     for (int k = 0; k < kRiders; k++) part[k] = 0.;
     switch (kind) {
@@ -388,44 +529,43 @@ __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind,
 #pragma unroll
                 for (int k = 0; k < kRiders; k++) {
                     xs[k] = k < nr ? SS[k].x : SS[0].x;
-                    acc[k] = k < nr ? SS[k].b[row] : T(0);
+                    acc[k] = T(0);
                 }
-                sell_row_multi<true>(M.full, local, lane, xs, nr, acc);
-#pragma unroll
-                for (int k = 0; k < kRiders; k++)
-                    if (k < nr) SS[k].r[row] = acc[k];
+                row_load<G>(SS, nr, ls, &S_::b, row, acc);
+                sell_row_multi<true, G>(M.full, local, lane, xs, nr, ls, acc);
+                row_store<G>(SS, lo, hi, ls, &S_::r, row, acc);
             }
         }
         break;
     }
     case PH_SPMV:
-        if (!WIDE || M.spmv_lanes == 1) {
+        if (G || !WIDE || M.spmv_lanes == 1) {
             const int local = j;
             if (local < M.full_slices) {
                 const int row = group_row(M.full, local, local, lane, M.full_row0, M.n_active);
                 if (row >= 0) {
                     const T* ds[kRiders];
-                    T acc[kRiders];
+                    T acc[kRiders], dr[kRiders];
 #pragma unroll
                     for (int k = 0; k < kRiders; k++) {
                         ds[k] = k < nr ? SS[k].d : SS[0].d;
                         acc[k] = T(0);
+                        dr[k] = T(0);
                     }
-                    sell_row_multi<false>(M.full, local, lane, ds, nr, acc);
+                    sell_row_multi<false, G>(M.full, local, lane, ds, nr, ls, acc);
+                    row_store<G>(SS, lo, hi, ls, &S_::q, row, acc);
+                    row_load<G>(SS, nr, ls, &S_::d, row, dr);
 #pragma unroll
-                    for (int k = 0; k < kRiders; k++) {
-                        if (k < nr) {
-                            SS[k].q[row] = acc[k];
-                            part[k] = (double)(SS[k].d[row] * acc[k]);
-                        }
-                    }
+                    for (int k = 0; k < kRiders; k++)
+                        if (k < nr) part[k] = (double)(dr[k] * acc[k]);
                 }
             }
-        } else if constexpr (WIDE) {
-            for (int k = 0; k < nr; k++) part[k] = spmv_task_any(M, SS[k].d, SS[k].q, j, lane);
+        } else if constexpr (WIDE && !G) {
+            for (int k = 0; k < nr; k++) part[k] = spmv_task_any(M, SS[k].d, SS[k].q, j, lane, ls);
         }
         break;
     case PH_RES64: {
+        if constexpr (G) break;  // never scheduled here (k_batch_prefix)
         const int local = j;
         if (local < M.full_slices) {
             const int row = group_row(M.full64, local, local, lane, M.full_row0, M.n_active);
@@ -435,16 +575,14 @@ __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind,
 #pragma unroll
                 for (int k = 0; k < kRiders; k++) {
                     xs[k] = k < nr ? SS[k].x64 : SS[0].x64;
-                    acc[k] = k < nr ? SS[k].b64[row] : 0.;
+                    acc[k] = 0.;
                 }
-                sell_row_multi<true>(M.full64, local, lane, xs, nr, acc);
+                row_load<G>(SS, nr, ls, &S_::b64, row, acc);
+                sell_row_multi<true, G>(M.full64, local, lane, xs, nr, ls, acc);
+                row_store<G>(SS, lo, hi, ls, &S_::r64, row, acc);
 #pragma unroll
-                for (int k = 0; k < kRiders; k++) {
-                    if (k < nr) {
-                        SS[k].r64[row] = acc[k];
-                        part[k] = acc[k] * acc[k];
-                    }
-                }
+                for (int k = 0; k < kRiders; k++)
+                    if (k < nr) part[k] = acc[k] * acc[k];
             }
         }
         break;
@@ -462,25 +600,31 @@ __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind,
                 for (int k = 0; k < kRiders; k++) {
                     ms[k] = k < nr ? SS[k].m : SS[0].m;
                     acc[k] = T(0);
-                    if (k < nr) {
-                        const BSlot<T>& S = SS[k];
-                        T rr;
-                        if (kind == PH_FWD) {
-                            const T alpha = (T)S.alpha;
-                            S.x[row] += alpha * S.d[row];
-                            rr = S.r[row] - alpha * S.q[row];
-                            S.r[row] = rr;
-                        } else {
-                            rr = S.r[row];
-                        }
-                        acc[k] = rr;
-                    }
                 }
-                sell_row_multi<true>(M.lower, slice, lane, ms, nr, acc);
+                row_load<G>(SS, nr, ls, &S_::r, row, acc);
+                if (kind == PH_FWD) {
+                    T xv[kRiders], dv[kRiders], qv[kRiders];
+#pragma unroll
+                    for (int k = 0; k < kRiders; k++) xv[k] = dv[k] = qv[k] = T(0);
+                    row_load<G>(SS, nr, ls, &S_::x, row, xv);
+                    row_load<G>(SS, nr, ls, &S_::d, row, dv);
+                    row_load<G>(SS, nr, ls, &S_::q, row, qv);
+#pragma unroll
+                    for (int k = 0; k < kRiders; k++) {
+                        if (k < nr) {
+                            const T alpha = (T)SS[k].alpha;
+                            xv[k] += alpha * dv[k];
+                            acc[k] = acc[k] - alpha * qv[k];
+                        }
+                    }
+                    row_store<G>(SS, lo, hi, ls, &S_::x, row, xv);
+                    row_store<G>(SS, lo, hi, ls, &S_::r, row, acc);
+                }
+                sell_row_multi<true, G>(M.lower, slice, lane, ms, nr, ls, acc);
                 const T dg = M.diag[row];
 #pragma unroll
-                for (int k = 0; k < kRiders; k++)
-                    if (k < nr) SS[k].m[row] = acc[k] * M.omega / dg;
+                for (int k = 0; k < kRiders; k++) acc[k] = acc[k] * M.omega / dg;
+                row_store<G>(SS, lo, hi, ls, &S_::m, row, acc);
             }
         }
         break;
@@ -499,56 +643,77 @@ __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind,
                 for (int k = 0; k < kRiders; k++) {
                     zs[k] = k < nr ? SS[k].z : SS[0].z;
                     acc[k] = T(0);
-                    if (k < nr) {
-                        T t = SS[k].m[row];
-                        t *= dg;
-                        t *= M.c2;
-                        acc[k] = t;
-                    }
                 }
-                sell_row_multi<true>(M.upper, slice, lane, zs, nr, acc);
+                row_load<G>(SS, nr, ls, &S_::m, row, acc);
 #pragma unroll
                 for (int k = 0; k < kRiders; k++) {
-                    if (k < nr) {
-                        const T zz = acc[k] * M.omega / dg;
-                        SS[k].z[row] = zz;
-                        if (kind == PH_BWD) part[k] = (double)(zz * SS[k].r[row]);
-                    }
+                    T t = acc[k];
+                    t *= dg;
+                    t *= M.c2;
+                    acc[k] = t;
+                }
+                sell_row_multi<true, G>(M.upper, slice, lane, zs, nr, ls, acc);
+#pragma unroll
+                for (int k = 0; k < kRiders; k++) acc[k] = acc[k] * M.omega / dg;
+                row_store<G>(SS, lo, hi, ls, &S_::z, row, acc);
+                if (kind == PH_BWD) {
+                    T rv[kRiders];
+#pragma unroll
+                    for (int k = 0; k < kRiders; k++) rv[k] = T(0);
+                    row_load<G>(SS, nr, ls, &S_::r, row, rv);
+#pragma unroll
+                    for (int k = 0; k < kRiders; k++)
+                        if (k < nr) part[k] = (double)(acc[k] * rv[k]);
                 }
             }
         }
         break;
     }
     case PH_DINIT: {  // a slot that continues its direction keeps d (its DIR phase follows)
-        const int i = j * kUnitRows + lane;
-        if (i < M.n_active)
-            for (int k = 0; k < nr; k++) {
+        const int row = j * kUnitRows + lane;
+        if (row < M.n_active) {
+            const unsigned i = (unsigned)row * (unsigned)ls;
+            for (int k = lo; k < hi; k++) {
                 const T zz = SS[k].z[i];
                 if (!(SS[k].zr_cont > 0.)) SS[k].d[i] = zz;
                 part[k] = (double)(zz * SS[k].r[i]);
             }
+        }
         break;
     }
     case PH_DIR: {
-        const int i = j * kUnitRows + lane;
-        if (i < M.n_active)
-            for (int k = 0; k < nr; k++) SS[k].d[i] = SS[k].z[i] + (T)SS[k].beta * SS[k].d[i];
+        const int row = j * kUnitRows + lane;
+        if (row < M.n_active) {
+            T zv[kRiders], dv[kRiders];
+#pragma unroll
+            for (int k = 0; k < kRiders; k++) zv[k] = dv[k] = T(0);
+            row_load<G>(SS, nr, ls, &S_::z, row, zv);
+            row_load<G>(SS, nr, ls, &S_::d, row, dv);
+#pragma unroll
+            for (int k = 0; k < kRiders; k++)
+                if (k < nr) dv[k] = zv[k] + (T)SS[k].beta * dv[k];
+            row_store<G>(SS, lo, hi, ls, &S_::d, row, dv);
+        }
         break;
     }
     case PH_CONVERT: {
-        const int i = j * kUnitRows + lane;
-        if (i < M.n_active)
-            for (int k = 0; k < nr; k++) {
+        const int row = j * kUnitRows + lane;
+        if (row < M.n_active) {
+            const unsigned i = (unsigned)row * (unsigned)ls;
+            for (int k = lo; k < hi; k++) {
                 SS[k].b[i] = (T)SS[k].r64[i];
                 SS[k].x[i] = T(0);
                 SS[k].r[i] = T(0);
             }
+        }
         break;
     }
     case PH_CORRECT: {
-        const int i = j * kUnitRows + lane;
-        if (i < M.n_active)
-            for (int k = 0; k < nr; k++) SS[k].x64[i] += (double)SS[k].x[i];
+        const int row = j * kUnitRows + lane;
+        if (row < M.n_active) {
+            const unsigned i = (unsigned)row * (unsigned)ls;
+            for (int k = lo; k < hi; k++) SS[k].x64[i] += (double)SS[k].x[i];
+        }
         break;
     }
     default: break;
@@ -597,13 +762,19 @@ __device__ inline int unit_owner(const int* start, int ns, int u)
 // code paths need about twice the registers of everything else (117 vs 66
 // VGPRs in FP64: 4 vs 7 wavefronts per SIMD), so the engine runs the variant
 // without them whenever every bucket uses one lane per row.
-template <typename T, bool WIDE>
+// G: this launch runs the units of riders in interleaved groups of four
+// (their own unit table, startG / WG), with the group code paths only;
+// G = false runs every other unit.  Each variant's register count then follows its own
+// paths (FP64: the group paths in the same kernel took 70 -> 96 VGPRs, 7 -> 5
+// wavefronts per SIMD, for all units).
+template <typename T, bool WIDE, bool G>
 __global__ void __launch_bounds__(kUnitBlock) k_batch_step(BArgs<T> a)
 {
     __shared__ int start[kMaxSlots + 1];  // the step's unit offsets, read once per block
     const BSched* sc = a.sched;
     const int p = (int)(sc->step & 1u);
-    const int W = sc->W[p];
+    const int W = G ? sc->WG[p] : sc->W[p];
+    const int* ustart = G ? sc->startG[p] : sc->start[p];
     const int ns = sc->nslots;
     // Workgroups reach the XCDs round robin (block i on XCD i % 8), each XCD
     // with its own L2: the step's units are dealt to the XCDs in chunks of
@@ -616,7 +787,7 @@ __global__ void __launch_bounds__(kUnitBlock) k_batch_step(BArgs<T> a)
     // the v-th unit of this XCD (increasing in v)
     auto unit_of = [&](int v) { return ((v / kXcdChunk) * kXcds + xcd) * kXcdChunk + v % kXcdChunk; };
     if (unit_of(kb * kStepWaves) >= W) return;
-    for (int i = threadIdx.x; i <= ns; i += kUnitBlock) start[i] = sc->start[p][i];
+    for (int i = threadIdx.x; i <= ns; i += kUnitBlock) start[i] = ustart[i];
     __syncthreads();
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -635,12 +806,14 @@ __global__ void __launch_bounds__(kUnitBlock) k_batch_step(BArgs<T> a)
         const int nr = riders[lo];
         const int j = u - start[lo];
         double part[kRiders];
-        run_unit<T, WIDE>(SS, nr, a.mats[SS[0].mat], SS[0].kind, SS[0].group, j, part);
+        run_unit<T, WIDE && !G, G>(SS, nr, a.mats[SS[0].mat], SS[0].kind, SS[0].group, j, part);
+        // G: part[] is indexed by lane, the riders are lanes k0 .. k0+nr-1
+        const int k0 = G ? SS[0].lane : 0;
 #pragma unroll
         for (int k = 0; k < kRiders; k++) {
-            if (k < nr) {
+            if (k >= k0 && k < k0 + nr) {
                 const double t = wave_sum(part[k]);
-                if (lane == 0) SS[k].partials[j] = t;
+                if (lane == 0) SS[k - k0].partials[j] = t;
             }
         }
     }
@@ -722,41 +895,68 @@ __device__ inline int block_scan_max(int v, int* wtot)
 }
 
 // The next step's schedule.  Adjacent running slots of one matrix in the
-// same phase (e.g. the right-hand sides of one system, submitted one after
-// another) form runs; each run is cut into groups of up to kRiders slots,
-// and only a group's first slot (its leader) gets units, which serve the
-// whole group: the matrix is read once for all its right-hand sides.  Which
-// slots ride together changes nothing in any slot's arithmetic.
+// same phase form runs -- slots submitted alone (e.g. the right-hand sides
+// of one system, submitted one after another) or lanes of one interleaved
+// group, never a mix of the two; each run is cut into groups of up to
+// kRiders slots, and only a group's first slot (its leader) gets units,
+// which serve the whole group: the matrix is read once for all its
+// right-hand sides.  Which slots ride together changes nothing in any slot's
+// arithmetic.  Leaders of riders in interleaved groups of four go to the
+// second unit table (startG, WG), run by the group variant of the step
+// kernel -- except in an SpMV phase with more than
+// one lane per row, which only the other variant has, and the FP64 residual
+// phase of mixed solves (once per correction; its FP64 group gathers would
+// set the group variant's register count).
 template <typename T>
 __global__ void __launch_bounds__(kPrefixBlock) k_batch_prefix(BArgs<T> a)
 {
     __shared__ int wtot[kPrefixBlock / 64];
     __shared__ int kmat[kPrefixBlock], kkind[kPrefixBlock], kgrp[kPrefixBlock];
-    __shared__ int kunits[kPrefixBlock], krun[kPrefixBlock];
+    __shared__ int kunits[kPrefixBlock], krun[kPrefixBlock], kvg[kPrefixBlock];
     __shared__ int active;
     BSched* sc = a.sched;
     const int t = threadIdx.x;
     const int ns = sc->nslots;
     const unsigned step = sc->step;
     const int q = (int)((step + 1u) & 1u);
-    int u = 0, mat = -1, kind = -1, grp = -1;
+    int u = 0, mat = -1, kind = -1, grp = -1, vg = -1, ls = 1;
     if (t < ns) {
         const BSlot<T>& S = a.slots[t];
+        ls = S.ls;
         u = S.units;
         mat = S.mat;
         kind = S.kind;
         grp = S.group;
+        vg = S.ls > 1 ? S.vgroup : -1;  // -1: a slot submitted alone
     }
     kmat[t] = mat;
+    kvg[t] = vg;
     kkind[t] = kind;
     kgrp[t] = grp;
     kunits[t] = u;
     if (t == 0) active = 0;
     __syncthreads();
-    if (u > 0) atomicAdd(&active, 1);
+    // A lane of an interleaved group about to start an inner solve (mixed
+    // engine, PH_CONVERT) waits while another lane of its group is still in
+    // its previous one, so the group's inner solves start, and mostly run,
+    // together.  Waiting changes when a slot's phases run, never what they
+    // compute; the lanes waited for are running and reach PH_CONVERT or
+    // PH_DONE, so every wait ends.
+    bool wait = false;
+    if (u > 0 && ls > 1 && kind == PH_CONVERT)
+        for (int l = vg; l < vg + ls; l++)
+            if (l != t && kvg[l] == vg && kkind[l] != PH_CONVERT && kkind[l] != PH_DONE)
+                wait = true;
+    __syncthreads();
+    if (wait) {
+        kunits[t] = 0;
+        u = 0;
+    }
+    if (u > 0 || wait) atomicAdd(&active, 1);
+    __syncthreads();
     // a run starts at a running slot whose predecessor is idle or differs
     const bool joins = u > 0 && t > 0 && kunits[t - 1] > 0 && kmat[t - 1] == mat &&
-                       kkind[t - 1] == kind && kgrp[t - 1] == grp;
+                       kkind[t - 1] == kind && kgrp[t - 1] == grp && kvg[t - 1] == vg;
     const int run = block_scan_max(joins ? -1 : t, wtot);  // first slot of t's run
     krun[t] = run;
     __syncthreads();
@@ -767,15 +967,23 @@ __global__ void __launch_bounds__(kPrefixBlock) k_batch_prefix(BArgs<T> a)
         for (int k = 1; k < kRiders && t + k < ns; k++)
             if (kunits[t + k] > 0 && krun[t + k] == run) nr++;
     }
-    const int v = block_scan_add(leader ? u : 0, wtot);
+    const bool group = leader && ls == kRiders && kind != PH_RES64 &&
+                       !(kind == PH_SPMV && a.mats[mat].spmv_lanes > 1);
+    const int v = block_scan_add(leader && !group ? u : 0, wtot);
+    const int vG = block_scan_add(group ? u : 0, wtot);
     if (t < ns) {
         sc->start[q][t + 1] = v;
+        sc->startG[q][t + 1] = vG;
         sc->ran[q][t] = u;
         sc->riders[q][t] = nr;
     }
-    if (t == ns - 1) sc->W[q] = v;
+    if (t == ns - 1) {
+        sc->W[q] = v;
+        sc->WG[q] = vG;
+    }
     if (t == 0) {
         sc->start[q][0] = 0;
+        sc->startG[q][0] = 0;
         sc->active = active;
         sc->step = step + 1u;
     }
@@ -807,7 +1015,7 @@ __global__ void __launch_bounds__(kUnitBlock) k_batch_spmv_probe(const BMat<T>* 
         if (u >= W) break;
         const int lo = __builtin_amdgcn_readfirstlane(unit_owner(start, nm, u));
         const int j = u - start[lo];
-        const double part = wave_sum(spmv_task<K>(mats[lo], d, q, j, lane));
+        const double part = wave_sum(spmv_task<K>(mats[lo], d, q, j, lane, 1));
         if (lane == 0) partials[u] = part;
     }
 }
@@ -848,18 +1056,40 @@ struct HostMat {
     }
 };
 
-// Vectors of one slot (device order, n+1 entries each).
+// The Krylov vectors of a group of `lanes` slots submitted together (device
+// order, n+1 rows), interleaved: row i of lane k at v[i * lanes + k].  Freed
+// with the group's last slot.
 template <typename T>
-struct SlotVectors {
+struct VecBlock {
     T *x = nullptr, *r = nullptr, *b = nullptr, *d = nullptr, *z = nullptr, *q = nullptr,
       *m = nullptr;
-    double *x64 = nullptr, *b64 = nullptr, *r64 = nullptr, *partials = nullptr, *grp = nullptr;
+    double *x64 = nullptr, *b64 = nullptr, *r64 = nullptr;
+    int lanes = 1;
     long long bytes = 0;
-    void release()
+    VecBlock() = default;
+    VecBlock(const VecBlock&) = delete;
+    VecBlock& operator=(const VecBlock&) = delete;
+    ~VecBlock()
     {
         for (void* p : {(void*)x, (void*)r, (void*)b, (void*)d, (void*)z, (void*)q, (void*)m,
-                        (void*)x64, (void*)b64, (void*)r64, (void*)partials, (void*)grp})
+                        (void*)x64, (void*)b64, (void*)r64})
             dev_free(p);
+    }
+};
+
+// One slot's storage: its lane of a vector block, and its own unit partials
+// and per-colour sums.
+template <typename T>
+struct SlotVectors {
+    std::shared_ptr<VecBlock<T>> block;
+    int lane = 0;
+    double *partials = nullptr, *grp = nullptr;
+    long long bytes = 0;  // partials and grp
+    void release()
+    {
+        block.reset();
+        dev_free(partials);
+        dev_free(grp);
         *this = SlotVectors();
     }
 };
@@ -1183,86 +1413,145 @@ struct Engine {
         long long b = 0;
         for (const auto& m : mats)
             if (m.live) b += m.bytes;
-        for (const auto& v : vecs) b += v.bytes;
+        std::vector<const VecBlock<T>*> seen;
+        for (const auto& v : vecs) {
+            b += v.bytes;
+            const VecBlock<T>* vb = v.block.get();
+            if (vb && std::find(seen.begin(), seen.end(), vb) == seen.end()) {
+                seen.push_back(vb);
+                b += vb->bytes;
+            }
+        }
         return b;
     }
 
     int submit(int id, const double* bvec, const double* x0, double precision, double inner_rtol,
                int max_outer)
     {
+        int s = -1;
+        submit_group(id, 1, bvec, x0, precision, inner_rtol, max_outer, &s);
+        return s;
+    }
+
+    // nrhs right-hand sides of matrix id (rhs k at bvec + k n, initial guess
+    // at x0 + k n, or zero) in nrhs consecutive slots whose vectors are one
+    // interleaved block; slot ids to out[0..nrhs).
+    void submit_group(int id, int nrhs, const double* bvec, const double* x0, double precision,
+                      double inner_rtol, int max_outer, int* out)
+    {
         if (id < 0 || id >= (int)mats.size() || !mats[id].live)
             throw Error{SFA_ERR_INVALID, "no such matrix"};
+        if (nrhs < 1 || nrhs > kRiders)
+            throw Error{SFA_ERR_INVALID, "a group holds 1 to 4 right-hand sides"};
         if (!bvec) throw Error{SFA_ERR_INVALID, "null right-hand side"};
         if (!(precision > 0)) throw Error{SFA_ERR_INVALID, "precision must be positive"};
         if (mixed && !(inner_rtol > 0 && inner_rtol < 1))
             throw Error{SFA_ERR_INVALID, "inner_rtol must be in (0, 1)"};
-        int s = -1;
-        for (int i = 0; i < max_slots; i++)
-            if (!slot_used[i]) {
-                s = i;
-                break;
-            }
-        if (s < 0) throw Error{SFA_ERR_INVALID, "no free slot"};
         HostMat<T>& M = mats[id];
         const int n = M.n;
+        if ((long long)(n + 1) * nrhs > INT_MAX)
+            throw Error{SFA_ERR_INVALID, "group vectors exceed 2^31 entries"};
+        int s0 = -1;  // first of nrhs consecutive free slots
+        for (int i = 0, run = 0; i < max_slots; i++) {
+            run = slot_used[i] ? 0 : run + 1;
+            if (run == nrhs) {
+                s0 = i - nrhs + 1;
+                break;
+            }
+        }
+        if (s0 < 0) throw Error{SFA_ERR_INVALID, "no free slot"};
         SFA_HIP(hipSetDevice(device));
-        SlotVectors<T>& v = vecs[s];
-        v.release();
-        T** tv[] = {&v.x, &v.r, &v.b, &v.d, &v.z, &v.q, &v.m};
-        for (T** p : tv) *p = dev_alloc<T>(n + 1);
-        v.bytes = 7LL * (n + 1) * sizeof(T);
-        if (mixed) {
-            v.x64 = dev_alloc<double>(n + 1);
-            v.b64 = dev_alloc<double>(n + 1);
-            v.r64 = dev_alloc<double>(n + 1);
-            v.bytes += 3LL * (n + 1) * 8;
+        const size_t len = (size_t)(n + 1) * nrhs;
+        auto blk = std::make_shared<VecBlock<T>>();
+        blk->lanes = nrhs;
+        T** tv[] = {&blk->x, &blk->r, &blk->b, &blk->d, &blk->z, &blk->q, &blk->m};
+        for (T** p : tv) {
+            *p = dev_alloc<T>(len);
+            SFA_HIP(hipMemsetAsync(*p, 0, len * sizeof(T), stream));
         }
-        v.partials = dev_alloc<double>(M.max_units);
-        v.grp = dev_alloc<double>(std::max(1, M.gb));
-        for (T* p : {v.x, v.r, v.b, v.d, v.z, v.q, v.m})
-            SFA_HIP(hipMemsetAsync(p, 0, (n + 1) * sizeof(T), stream));
-        // inputs in device order; tolerance as the reference computes it
-        std::vector<double> bd(n), xd(n, 0.0);
-        double bb = 0.;
-        for (int i = 0; i < n; i++) bb += bvec[i] * bvec[i];
-        for (int p = 0; p < n; p++) bd[p] = bvec[M.perm[p]];
-        if (x0)
-            for (int p = 0; p < n; p++) xd[p] = x0[M.perm[p]];
-        double tol = precision * std::sqrt(bb);
-        const double floor_ = 1.4901161193847656e-08;
-        if (!(tol >= floor_) && tol <= floor_) tol = floor_;
-        BSlot<T> sl{};
-        sl.mat = id;
-        sl.mixed = mixed ? 1 : 0;
-        sl.cont = cont ? 1 : 0;
-        sl.maxK = n;
-        sl.exit_reason = SFA_EXIT_CAP;
-        sl.max_outer = max_outer;
-        sl.x = v.x; sl.r = v.r; sl.b = v.b; sl.d = v.d; sl.z = v.z; sl.q = v.q; sl.m = v.m;
-        sl.x64 = v.x64; sl.b64 = v.b64; sl.r64 = v.r64;
-        sl.partials = v.partials;
-        sl.grp = v.grp;
+        blk->bytes = 7LL * (long long)len * sizeof(T);
         if (mixed) {
-            SFA_HIP(hipMemsetAsync(v.r64, 0, (n + 1) * sizeof(double), stream));
-            SFA_HIP(hipMemcpyAsync(v.b64, bd.data(), n * sizeof(double), hipMemcpyHostToDevice, stream));
-            SFA_HIP(hipMemcpyAsync(v.x64, xd.data(), n * sizeof(double), hipMemcpyHostToDevice, stream));
-            SFA_HIP(hipMemsetAsync(v.x64 + n, 0, sizeof(double), stream));
-            sl.kind = PH_RES64;
-            sl.target64 = tol;
-            sl.rel2 = inner_rtol * inner_rtol;
+            for (double** p : {&blk->x64, &blk->b64, &blk->r64}) {
+                *p = dev_alloc<double>(len);
+                SFA_HIP(hipMemsetAsync(*p, 0, len * sizeof(double), stream));
+            }
+            blk->bytes += 3LL * (long long)len * 8;
+        }
+        // inputs in device order, lanes interleaved (row n, the padding row,
+        // stays zero); each tolerance as the reference computes it
+        std::vector<double> bI((size_t)n * nrhs), xI((size_t)n * nrhs, 0.0);
+        std::vector<double> tol(nrhs);
+        for (int k = 0; k < nrhs; k++) {
+            const double* bk = bvec + (size_t)k * n;
+            double bb = 0.;
+            for (int i = 0; i < n; i++) bb += bk[i] * bk[i];
+            for (int p = 0; p < n; p++) bI[(size_t)p * nrhs + k] = bk[M.perm[p]];
+            if (x0) {
+                const double* xk = x0 + (size_t)k * n;
+                for (int p = 0; p < n; p++) xI[(size_t)p * nrhs + k] = xk[M.perm[p]];
+            }
+            double t = precision * std::sqrt(bb);
+            const double floor_ = 1.4901161193847656e-08;
+            if (!(t >= floor_) && t <= floor_) t = floor_;
+            tol[k] = t;
+        }
+        const size_t m = (size_t)n * nrhs;
+        std::vector<T> bt, xt;
+        if (mixed) {
+            SFA_HIP(hipMemcpyAsync(blk->b64, bI.data(), m * sizeof(double), hipMemcpyHostToDevice, stream));
+            SFA_HIP(hipMemcpyAsync(blk->x64, xI.data(), m * sizeof(double), hipMemcpyHostToDevice, stream));
         } else {
-            std::vector<T> bt(bd.begin(), bd.end()), xt(xd.begin(), xd.end());
-            SFA_HIP(hipMemcpyAsync(v.b, bt.data(), n * sizeof(T), hipMemcpyHostToDevice, stream));
-            SFA_HIP(hipMemcpyAsync(v.x, xt.data(), n * sizeof(T), hipMemcpyHostToDevice, stream));
-            sl.kind = PH_RESID;
-            sl.tol2 = tol * tol;
+            bt.assign(bI.begin(), bI.end());
+            xt.assign(xI.begin(), xI.end());
+            SFA_HIP(hipMemcpyAsync(blk->b, bt.data(), m * sizeof(T), hipMemcpyHostToDevice, stream));
+            SFA_HIP(hipMemcpyAsync(blk->x, xt.data(), m * sizeof(T), hipMemcpyHostToDevice, stream));
         }
-        SFA_HIP(hipMemcpyAsync(d_slots + s, &sl, sizeof(BSlot<T>), hipMemcpyHostToDevice, stream));
+        std::vector<BSlot<T>> sls(nrhs);
+        for (int k = 0; k < nrhs; k++) {
+            const int s = s0 + k;
+            SlotVectors<T>& v = vecs[s];
+            v.release();
+            v.block = blk;
+            v.lane = k;
+            v.partials = dev_alloc<double>(M.max_units);
+            v.grp = dev_alloc<double>(std::max(1, M.gb));
+            v.bytes = (long long)M.max_units * 8 + (long long)std::max(1, M.gb) * 8;
+            BSlot<T> sl{};
+            sl.mat = id;
+            sl.mixed = mixed ? 1 : 0;
+            sl.cont = cont ? 1 : 0;
+            sl.maxK = n;
+            sl.exit_reason = SFA_EXIT_CAP;
+            sl.max_outer = max_outer;
+            sl.ls = nrhs;
+            sl.lane = k;
+            sl.vgroup = s0;
+            sl.x = blk->x + k; sl.r = blk->r + k; sl.b = blk->b + k; sl.d = blk->d + k;
+            sl.z = blk->z + k; sl.q = blk->q + k; sl.m = blk->m + k;
+            sl.x64 = mixed ? blk->x64 + k : nullptr;
+            sl.b64 = mixed ? blk->b64 + k : nullptr;
+            sl.r64 = mixed ? blk->r64 + k : nullptr;
+            sl.partials = v.partials;
+            sl.grp = v.grp;
+            if (mixed) {
+                sl.kind = PH_RES64;
+                sl.target64 = tol[k];
+                sl.rel2 = inner_rtol * inner_rtol;
+            } else {
+                sl.kind = PH_RESID;
+                sl.tol2 = tol[k] * tol[k];
+            }
+            sls[k] = sl;
+        }
+        SFA_HIP(hipMemcpyAsync(d_slots + s0, sls.data(), sizeof(BSlot<T>) * nrhs,
+                               hipMemcpyHostToDevice, stream));
         SFA_HIP(hipStreamSynchronize(stream));  // host vectors go out of scope
-        slots[s] = sl;
-        slot_used[s] = 1;
-        M.slots_in_use++;
-        return s;
+        for (int k = 0; k < nrhs; k++) {
+            slots[s0 + k] = sls[k];
+            slot_used[s0 + k] = 1;
+            out[k] = s0 + k;
+        }
+        M.slots_in_use += nrhs;
     }
 
     void read_sched()
@@ -1281,13 +1570,15 @@ struct Engine {
         return false;
     }
 
+    // The two step kernels: complete interleaved groups, then every other unit.
     void launch_step(const BArgs<T>& a, bool w)
     {
-        if (w) launch(k_batch_step<T, true>, dim3(kStepBlocks), dim3(kUnitBlock), stream, a);
-        else launch(k_batch_step<T, false>, dim3(kStepBlocks), dim3(kUnitBlock), stream, a);
+        launch(k_batch_step<T, false, true>, dim3(kStepBlocks), dim3(kUnitBlock), stream, a);
+        if (w) launch(k_batch_step<T, true, false>, dim3(kStepBlocks), dim3(kUnitBlock), stream, a);
+        else launch(k_batch_step<T, false, false>, dim3(kStepBlocks), dim3(kUnitBlock), stream, a);
     }
 
-    // Records kGraphSteps steps (k_batch_step, k_batch_fold, k_batch_prefix)
+    // Records kGraphSteps steps (k_batch_step twice, k_batch_fold, k_batch_prefix)
     // into a graph, node by node; rebuilt when the device matrix table has
     // moved or the step kernel variant changed.
     void build_graph()
@@ -1335,9 +1626,9 @@ struct Engine {
         return running();
     }
 
-    // Runs `steps` steps with the three kernels launched one by one, each
+    // Runs `steps` steps with the kernels launched one by one, each
     // between two HIP events; adds each kernel's summed time (milliseconds)
-    // to ms[0..3) (step, fold, prefix).
+    // to ms[0..3) (both step launches, fold, prefix).
     void profile(int steps, double* ms)
     {
         SFA_HIP(hipSetDevice(device));
@@ -1405,14 +1696,19 @@ struct Engine {
         if (sl.kind != PH_DONE) throw Error{SFA_ERR_INVALID, "slot has not finished"};
         HostMat<T>& M = mats[sl.mat];
         const int n = M.n;
-        if (x) {
+        if (x) {  // the slot's lane of its group's block
+            const VecBlock<T>& blk = *vecs[s].block;
+            const int L = blk.lanes, k = vecs[s].lane;
+            const size_t m = (size_t)n * L;
             std::vector<double> xd(n);
             if (mixed) {
-                SFA_HIP(hipMemcpy(xd.data(), sl.x64, n * sizeof(double), hipMemcpyDeviceToHost));
+                std::vector<double> xa(m);
+                SFA_HIP(hipMemcpy(xa.data(), blk.x64, m * sizeof(double), hipMemcpyDeviceToHost));
+                for (int i = 0; i < n; i++) xd[i] = xa[(size_t)i * L + k];
             } else {
-                std::vector<T> xt(n);
-                SFA_HIP(hipMemcpy(xt.data(), sl.x, n * sizeof(T), hipMemcpyDeviceToHost));
-                for (int i = 0; i < n; i++) xd[i] = (double)xt[i];
+                std::vector<T> xa(m);
+                SFA_HIP(hipMemcpy(xa.data(), blk.x, m * sizeof(T), hipMemcpyDeviceToHost));
+                for (int i = 0; i < n; i++) xd[i] = (double)xa[(size_t)i * L + k];
             }
             for (int p = 0; p < n; p++) x[M.perm[p]] = xd[p];
         }
@@ -1543,6 +1839,18 @@ int sfa_batch_submit(sfa_batch b, int matrix_id, const double* rhs, const double
         if (!b || !slot) throw sfa::Error{SFA_ERR_INVALID, "null argument"};
         b->h.visit([&](auto& e) {
             *slot = e.submit(matrix_id, rhs, x0, precision, inner_rtol, max_outer);
+        });
+    });
+}
+
+int sfa_batch_submit_group(sfa_batch b, int matrix_id, int nrhs, const double* rhs,
+                           const double* x0, double precision, double inner_rtol,
+                           int max_outer, int* slots)
+{
+    return sfa::guarded([&] {
+        if (!b || !slots) throw sfa::Error{SFA_ERR_INVALID, "null argument"};
+        b->h.visit([&](auto& e) {
+            e.submit_group(matrix_id, nrhs, rhs, x0, precision, inner_rtol, max_outer, slots);
         });
     });
 }

@@ -194,3 +194,48 @@ def test_right_hand_sides_riding_together_match_alone(mixed):
             x_a, info_a = alone.collect(sl, s.n)
         assert info_t["iterations"] == info_a["iterations"], (s.name, info_t, info_a)
         assert np.array_equal(x_t, x_a), s.name
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_interleaved_groups_match_separate_slots(mixed):
+    """Right-hand sides submitted as one interleaved group
+    (sfa_batch_submit_group: vectors stored row-interleaved, a complete group
+    of four gathering with one vector load) give the x and iteration counts of
+    the same right-hand sides submitted one slot at a time, bit for bit.
+    Groups of 4, 3, 2 and 1 over matrices whose solves converge at different
+    iteration counts, so groups split as their solves finish; initial guesses
+    nonzero for one group."""
+    systems = [S.poisson3d_p1_kuhn(7), S.poisson2d_p1(5), S.random_spd(250, 0.05, 3)]
+    sizes = [4, 3, 2, 1]
+    with capi.Batch(mixed=mixed, max_slots=64) as grouped, \
+            capi.Batch(mixed=mixed, max_slots=64) as separate:
+        jobs = []
+        for k, s in enumerate(systems):
+            mg, _ = _add(grouped, s)
+            ms, _ = _add(separate, s)
+            for gi, g in enumerate(sizes):
+                bs = [S.rhs_uniform(s, 500 + 40 * k + 10 * gi + i) for i in range(g)]
+                x0s = None
+                if gi == 1:
+                    rng = np.random.default_rng(9 + k)
+                    x0s = [rng.uniform(-1, 1, s.n) for _ in range(g)]
+                gslots = grouped.submit_group(mg, bs, x0s)
+                assert gslots == list(range(gslots[0], gslots[0] + g))
+                sslots = [separate.submit(ms, b, None if x0s is None else x0s[i])
+                          for i, b in enumerate(bs)]
+                jobs += [(gs, ss, s) for gs, ss in zip(gslots, sslots)]
+        print(f"case: interleaved groups {'mixed' if mixed else 'fp64'}, {len(jobs)} solves",
+              flush=True)
+        _run_all(grouped, [j[0] for j in jobs])
+        _run_all(separate, [j[1] for j in jobs])
+        iters = set()
+        for gs, ss, s in jobs:
+            xg, ig = grouped.collect(gs, s.n)
+            xs, is_ = separate.collect(ss, s.n)
+            assert ig["iterations"] == is_["iterations"], (s.name, ig, is_)
+            assert ig["exit_reason"] == is_["exit_reason"] and ig["success"] == 1
+            assert np.array_equal(xg, xs), s.name
+            iters.add((s.name, ig["iterations"]))
+        # the groups did split: one matrix's solves end at different counts
+        assert len(iters) > len(systems), iters
+        assert grouped.memory() > 0
