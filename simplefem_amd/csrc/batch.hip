@@ -349,7 +349,8 @@ __device__ inline void sell_row_group(const SellViewT<TV>& A, int slice, int lan
 // k < nr whose vectors have stride ls, each entry's column and value loaded
 // once for all riders; every rider's sum runs over the entries in order, as
 // sell_row forms it.  G: acc[] is indexed by the lanes of an interleaved
-// group whose lane 0 is vec[0] (see run_unit), which takes sell_row_group.
+// group whose base address (lane 0 of row 0) is vec[0] (see run_unit),
+// which takes sell_row_group.
 template <bool SUB, bool G, typename TV>
 __device__ inline void sell_row_multi(const SellViewT<TV>& A, int slice, int lane,
                                       const TV* const* vec, int nr, int ls, TV* acc)
@@ -441,16 +442,20 @@ __device__ double spmv_task_any(const BMat<T>& M, const T* d, T* q, int task, in
 }
 
 // Row `row` of the vector that member vp of each rider points to (T or
-// double), for riders k < nr with vectors of stride ls.  G: SS[0] is lane 0
-// of an interleaved group of kRiders lanes, and the four lanes' values come
-// with one vector access (lanes that do not ride are read and ignored).
+// double), for riders k < nr with vectors of stride ls.  G: the riders are
+// lanes lo .. lo+nr-1 of an interleaved group of kRiders lanes, SS[0] the
+// one in lane lo; the group's four values of the row come in one vector
+// access, indexed by lane (lanes that do not ride are read and ignored).
+// The group's base address is taken from SS[0]'s own pointer, never from
+// the slot of lane 0, which may have been collected and reused.
 template <bool G, typename V, typename T>
-__device__ inline void row_load(const BSlot<T>* SS, int nr, int ls, V* BSlot<T>::*vp,
+__device__ inline void row_load(const BSlot<T>* SS, int lo, int nr, int ls, V* BSlot<T>::*vp,
                                 int row, V* out)
 {
     if constexpr (G) {
         typedef V V4 __attribute__((ext_vector_type(kRiders)));
-        const V4 v = *reinterpret_cast<const V4*>(SS[0].*vp + (unsigned)row * kRiders);
+        const V* base = SS[0].*vp - lo;
+        const V4 v = *reinterpret_cast<const V4*>(base + (unsigned)row * kRiders);
 #pragma unroll
         for (int k = 0; k < kRiders; k++) out[k] = v[k];
     } else {
@@ -467,7 +472,7 @@ __device__ inline void row_store(const BSlot<T>* SS, int lo, int hi, int ls, V* 
                                  int row, const V* in)
 {
     if constexpr (G) {
-        V* base = SS[0].*vp + (unsigned)row * kRiders;
+        V* base = SS[0].*vp - lo + (unsigned)row * kRiders;
         if (lo == 0 && hi == kRiders) {
             typedef V V4 __attribute__((ext_vector_type(kRiders)));
             V4 v;
@@ -494,28 +499,32 @@ __device__ inline void row_store(const BSlot<T>* SS, int lo, int hi, int ls, V* 
 // arithmetic they get alone.  part[k]: the lane's share of rider k's
 // reduction.
 //
-// G: the riders are lanes k0 .. k0+nr-1 of an interleaved group of kRiders
-// (k0 = SS[0].lane).  The unit then works in lane space: SS is moved to the
-// group's lane 0, every loaded row and gathered column brings all four
-// lanes in one vector access, the arithmetic runs for all four (a lane
-// that does not ride computes values nobody stores), and only the riding
-// lanes' values are stored; part[] is indexed by lane.  Never with a
-// multi-lane SpMV (k_batch_prefix leaves those to G = false).
+// G: the riders are lanes lo .. lo+nr-1 of an interleaved group of kRiders
+// (lo = SS[0].lane).  The unit then works in lane space: every loaded row and
+// gathered column brings all four lanes in one vector access, the
+// arithmetic runs for all four (a lane that does not ride computes values
+// nobody stores, from the scalars of SS[0]), only the riding lanes' values
+// are stored, and part[] is indexed by lane.  Never with a multi-lane SpMV
+// (k_batch_prefix leaves those to G = false).
 template <typename T, bool WIDE, bool G>
 __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind, int g, int j,
                          double* part)
 {
     using S_ = BSlot<T>;
     const int lane = threadIdx.x & 63;
-    int lo = 0, hi = nr;  // riders (G: lanes) whose values are stored
-    int ls = SS[0].ls;
-    if constexpr (G) {
-        lo = SS[0].lane;
-        hi = lo + nr;
-        SS -= lo;
-        nr = kRiders;
-        ls = kRiders;
-    }
+    const int lo = G ? SS[0].lane : 0;  // riders (G: lanes) lo .. hi-1 are stored
+    const int hi = lo + nr;
+    const int ls = G ? kRiders : SS[0].ls;
+    const int nl = G ? kRiders : nr;    // riders (G: lanes) computed
+    // the slot of rider (G: lane) k, for its scalars; SS[0] for a lane that
+    // does not ride
+    auto slot = [&](int k) -> const S_& { return SS[(k >= lo && k < hi) ? k - lo : 0]; };
+    // the vectors gathered by sell_row_multi: per rider, or (G) the group's base
+    auto gathered = [&](auto vp, auto* out) {
+#pragma unroll
+        for (int k = 0; k < kRiders; k++)
+            out[k] = G ? SS[0].*vp - lo : (k < nr ? SS[k].*vp : SS[0].*vp);
+    };
 #pragma unroll
     for (int k = 0; k < kRiders; k++) part[k] = 0.;
     switch (kind) {
@@ -526,12 +535,10 @@ __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind,
             if (row >= 0) {
                 const T* xs[kRiders];
                 T acc[kRiders];
+                gathered(&S_::x, xs);
 #pragma unroll
-                for (int k = 0; k < kRiders; k++) {
-                    xs[k] = k < nr ? SS[k].x : SS[0].x;
-                    acc[k] = T(0);
-                }
-                row_load<G>(SS, nr, ls, &S_::b, row, acc);
+                for (int k = 0; k < kRiders; k++) acc[k] = T(0);
+                row_load<G>(SS, lo, nr, ls, &S_::b, row, acc);
                 sell_row_multi<true, G>(M.full, local, lane, xs, nr, ls, acc);
                 row_store<G>(SS, lo, hi, ls, &S_::r, row, acc);
             }
@@ -546,18 +553,15 @@ __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind,
                 if (row >= 0) {
                     const T* ds[kRiders];
                     T acc[kRiders], dr[kRiders];
+                    gathered(&S_::d, ds);
 #pragma unroll
-                    for (int k = 0; k < kRiders; k++) {
-                        ds[k] = k < nr ? SS[k].d : SS[0].d;
-                        acc[k] = T(0);
-                        dr[k] = T(0);
-                    }
+                    for (int k = 0; k < kRiders; k++) acc[k] = dr[k] = T(0);
                     sell_row_multi<false, G>(M.full, local, lane, ds, nr, ls, acc);
                     row_store<G>(SS, lo, hi, ls, &S_::q, row, acc);
-                    row_load<G>(SS, nr, ls, &S_::d, row, dr);
+                    row_load<G>(SS, lo, nr, ls, &S_::d, row, dr);
 #pragma unroll
                     for (int k = 0; k < kRiders; k++)
-                        if (k < nr) part[k] = (double)(dr[k] * acc[k]);
+                        if (k < nl) part[k] = (double)(dr[k] * acc[k]);
                 }
             }
         } else if constexpr (WIDE && !G) {
@@ -572,17 +576,15 @@ __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind,
             if (row >= 0) {
                 const double* xs[kRiders];
                 double acc[kRiders];
+                gathered(&S_::x64, xs);
 #pragma unroll
-                for (int k = 0; k < kRiders; k++) {
-                    xs[k] = k < nr ? SS[k].x64 : SS[0].x64;
-                    acc[k] = 0.;
-                }
-                row_load<G>(SS, nr, ls, &S_::b64, row, acc);
+                for (int k = 0; k < kRiders; k++) acc[k] = 0.;
+                row_load<G>(SS, lo, nr, ls, &S_::b64, row, acc);
                 sell_row_multi<true, G>(M.full64, local, lane, xs, nr, ls, acc);
                 row_store<G>(SS, lo, hi, ls, &S_::r64, row, acc);
 #pragma unroll
                 for (int k = 0; k < kRiders; k++)
-                    if (k < nr) part[k] = acc[k] * acc[k];
+                    if (k < nl) part[k] = acc[k] * acc[k];
             }
         }
         break;
@@ -596,23 +598,21 @@ __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind,
             if (row >= 0) {
                 const T* ms[kRiders];
                 T acc[kRiders];
+                gathered(&S_::m, ms);
 #pragma unroll
-                for (int k = 0; k < kRiders; k++) {
-                    ms[k] = k < nr ? SS[k].m : SS[0].m;
-                    acc[k] = T(0);
-                }
-                row_load<G>(SS, nr, ls, &S_::r, row, acc);
+                for (int k = 0; k < kRiders; k++) acc[k] = T(0);
+                row_load<G>(SS, lo, nr, ls, &S_::r, row, acc);
                 if (kind == PH_FWD) {
                     T xv[kRiders], dv[kRiders], qv[kRiders];
 #pragma unroll
                     for (int k = 0; k < kRiders; k++) xv[k] = dv[k] = qv[k] = T(0);
-                    row_load<G>(SS, nr, ls, &S_::x, row, xv);
-                    row_load<G>(SS, nr, ls, &S_::d, row, dv);
-                    row_load<G>(SS, nr, ls, &S_::q, row, qv);
+                    row_load<G>(SS, lo, nr, ls, &S_::x, row, xv);
+                    row_load<G>(SS, lo, nr, ls, &S_::d, row, dv);
+                    row_load<G>(SS, lo, nr, ls, &S_::q, row, qv);
 #pragma unroll
                     for (int k = 0; k < kRiders; k++) {
-                        if (k < nr) {
-                            const T alpha = (T)SS[k].alpha;
+                        if (k < nl) {
+                            const T alpha = (T)slot(k).alpha;
                             xv[k] += alpha * dv[k];
                             acc[k] = acc[k] - alpha * qv[k];
                         }
@@ -639,12 +639,10 @@ __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind,
                 const T dg = M.diag[row];
                 const T* zs[kRiders];
                 T acc[kRiders];
+                gathered(&S_::z, zs);
 #pragma unroll
-                for (int k = 0; k < kRiders; k++) {
-                    zs[k] = k < nr ? SS[k].z : SS[0].z;
-                    acc[k] = T(0);
-                }
-                row_load<G>(SS, nr, ls, &S_::m, row, acc);
+                for (int k = 0; k < kRiders; k++) acc[k] = T(0);
+                row_load<G>(SS, lo, nr, ls, &S_::m, row, acc);
 #pragma unroll
                 for (int k = 0; k < kRiders; k++) {
                     T t = acc[k];
@@ -660,10 +658,10 @@ __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind,
                     T rv[kRiders];
 #pragma unroll
                     for (int k = 0; k < kRiders; k++) rv[k] = T(0);
-                    row_load<G>(SS, nr, ls, &S_::r, row, rv);
+                    row_load<G>(SS, lo, nr, ls, &S_::r, row, rv);
 #pragma unroll
                     for (int k = 0; k < kRiders; k++)
-                        if (k < nr) part[k] = (double)(acc[k] * rv[k]);
+                        if (k < nl) part[k] = (double)(acc[k] * rv[k]);
                 }
             }
         }
@@ -674,9 +672,10 @@ __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind,
         if (row < M.n_active) {
             const unsigned i = (unsigned)row * (unsigned)ls;
             for (int k = lo; k < hi; k++) {
-                const T zz = SS[k].z[i];
-                if (!(SS[k].zr_cont > 0.)) SS[k].d[i] = zz;
-                part[k] = (double)(zz * SS[k].r[i]);
+                const S_& S = SS[k - lo];
+                const T zz = S.z[i];
+                if (!(S.zr_cont > 0.)) S.d[i] = zz;
+                part[k] = (double)(zz * S.r[i]);
             }
         }
         break;
@@ -687,11 +686,11 @@ __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind,
             T zv[kRiders], dv[kRiders];
 #pragma unroll
             for (int k = 0; k < kRiders; k++) zv[k] = dv[k] = T(0);
-            row_load<G>(SS, nr, ls, &S_::z, row, zv);
-            row_load<G>(SS, nr, ls, &S_::d, row, dv);
+            row_load<G>(SS, lo, nr, ls, &S_::z, row, zv);
+            row_load<G>(SS, lo, nr, ls, &S_::d, row, dv);
 #pragma unroll
             for (int k = 0; k < kRiders; k++)
-                if (k < nr) dv[k] = zv[k] + (T)SS[k].beta * dv[k];
+                if (k < nl) dv[k] = zv[k] + (T)slot(k).beta * dv[k];
             row_store<G>(SS, lo, hi, ls, &S_::d, row, dv);
         }
         break;
@@ -701,9 +700,10 @@ __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind,
         if (row < M.n_active) {
             const unsigned i = (unsigned)row * (unsigned)ls;
             for (int k = lo; k < hi; k++) {
-                SS[k].b[i] = (T)SS[k].r64[i];
-                SS[k].x[i] = T(0);
-                SS[k].r[i] = T(0);
+                const S_& S = SS[k - lo];
+                S.b[i] = (T)S.r64[i];
+                S.x[i] = T(0);
+                S.r[i] = T(0);
             }
         }
         break;
@@ -712,7 +712,10 @@ __device__ void run_unit(const BSlot<T>* SS, int nr, const BMat<T>& M, int kind,
         const int row = j * kUnitRows + lane;
         if (row < M.n_active) {
             const unsigned i = (unsigned)row * (unsigned)ls;
-            for (int k = lo; k < hi; k++) SS[k].x64[i] += (double)SS[k].x[i];
+            for (int k = lo; k < hi; k++) {
+                const S_& S = SS[k - lo];
+                S.x64[i] += (double)S.x[i];
+            }
         }
         break;
     }

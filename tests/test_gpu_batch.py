@@ -239,3 +239,42 @@ def test_interleaved_groups_match_separate_slots(mixed):
         # the groups did split: one matrix's solves end at different counts
         assert len(iters) > len(systems), iters
         assert grouped.memory() > 0
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_group_lanes_run_on_after_lane_zero_slot_is_reused(mixed):
+    """A group's lane 0 finishes first (a right-hand side of 1e-12 scale is
+    within the 2^-26 floor at once) and is collected; its slot is then taken
+    by a single solve of a larger matrix while lanes 1..3 still run.  The
+    lanes keep using their own group's vectors: every result equals the
+    same solve in separate slots, bit for bit."""
+    sg, sb = S.poisson3d_p1_kuhn(6), S.poisson2d_p1(6)
+    bs = [1e-12 * S.rhs_uniform(sg, 700)] + [S.rhs_uniform(sg, 701 + i) for i in range(3)]
+    bb = S.rhs_uniform(sb, 710)
+    with capi.Batch(mixed=mixed, max_slots=8) as batch:
+        mg, _ = _add(batch, sg)
+        mb, _ = _add(batch, sb)
+        slots = batch.submit_group(mg, bs)
+        print(f"case: lane 0 reuse {'mixed' if mixed else 'fp64'}, group slots {slots}", flush=True)
+        for _ in range(50):
+            batch.run(32)
+            done = set(batch.finished())
+            if slots[0] in done:
+                break
+        assert slots[0] in done and not (set(slots[1:]) & done), done
+        x0, i0 = batch.collect(slots[0], sg.n)
+        single = batch.submit(mb, bb)
+        assert single == slots[0]  # the freed slot of lane 0
+        _run_all(batch, slots[1:] + [single])
+        got = [batch.collect(s, sg.n) for s in slots[1:]] + [batch.collect(single, sb.n)]
+    want = []
+    with capi.Batch(mixed=mixed, max_slots=8) as sep:
+        ms, _ = _add(sep, sg)
+        mbs, _ = _add(sep, sb)
+        ss = [sep.submit(ms, b) for b in bs] + [sep.submit(mbs, bb)]
+        _run_all(sep, ss)
+        want = [sep.collect(s, sg.n) for s in ss[:4]] + [sep.collect(ss[4], sb.n)]
+    assert np.array_equal(x0, want[0][0]) and i0["iterations"] == want[0][1]["iterations"]
+    for (xg, ig), (xw, iw) in zip(got, want[1:]):
+        assert ig["iterations"] == iw["iterations"] and ig["exit_reason"] == iw["exit_reason"]
+        assert np.array_equal(xg, xw)
