@@ -358,6 +358,11 @@ def main():
                          "chunks of 2000 steps) and print that table; hcurl-mixed: only run one "
                          "FP64 solve with every kernel launched on its own (no graphs) between "
                          "HIP events and print the per-class table")
+    ap.add_argument("--only-system", default="",
+                    help="batched: keep only the workload's systems of this name (e.g. "
+                         "sullivan2d_L8_r3, the long tail), for profiling")
+    ap.add_argument("--profile-steps", type=int, default=400000,
+                    help="batched --engine-profile: stop after this many steps")
     ap.add_argument("--rhs-layout", default="grouped", choices=["grouped", "separate"],
                     help="batched: a system's right-hand sides as one interleaved slot group "
                          "(sfa_batch_submit_group; default) or as separate slots")

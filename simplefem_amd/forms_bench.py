@@ -170,11 +170,11 @@ def _submit_all(eng, jobs, mids, grouped, **kw):
     return out
 
 
-def engine_profile(jobs, cols, mixed, chunk=2000, grouped=True):
+def engine_profile(jobs, cols, mixed, chunk=2000, grouped=True, max_steps=400000):
     """The batched workload on one engine with every kernel of every step
     launched between HIP events (sfa_batch_profile), in chunks of `chunk`
-    steps: per chunk the solves still running and the mean step, fold and
-    schedule kernel times."""
+    steps (up to max_steps): per chunk the solves still running and the mean
+    step, fold and schedule kernel times."""
     rows = []
     with capi.Batch(mixed=mixed, max_slots=min(1024, sum(len(j.rhs) for j in jobs))) as eng:
         mids = [eng.add_matrix(j.system.n, j.system.A, j.system.C, j.system.V, j.system.diagonal(),
@@ -182,6 +182,7 @@ def engine_profile(jobs, cols, mixed, chunk=2000, grouped=True):
         nslots = len(_submit_all(eng, jobs, mids, grouped, precision=1e-10))
         total = 0
         while True:
+            chunk = min(chunk, max_steps - total)
             p = eng.profile(chunk)
             total += chunk
             running = nslots - len(eng.finished(2048))
@@ -190,7 +191,7 @@ def engine_profile(jobs, cols, mixed, chunk=2000, grouped=True):
                          "fold_us": round(p["fold_ms"] / chunk * 1e3, 1),
                          "schedule_us": round(p["prefix_ms"] / chunk * 1e3, 1)})
             log(f"engine profile: {rows[-1]}")
-            if running == 0 or total > 400000:
+            if running == 0 or total >= max_steps:
                 break
     return rows
 
@@ -198,6 +199,10 @@ def engine_profile(jobs, cols, mixed, chunk=2000, grouped=True):
 def batched(args):
     t0 = time.time()
     jobs = W.batched_systems(args.systems, 4)
+    if args.only_system:  # the workload's systems of that name only (profiling)
+        jobs = [j for j in jobs if j.system.name == args.only_system]
+        if not jobs:
+            raise SystemExit(f"no system named {args.only_system} in the batched workload")
     nrhs = sum(len(j.rhs) for j in jobs)
     log(f"{len(jobs)} systems, {nrhs} right-hand sides, "
         f"{sum(j.system.n for j in jobs)} rows in total ({time.time() - t0:.1f}s)")
@@ -259,7 +264,8 @@ def batched(args):
 
     if args.engine_profile:
         return {"engine_profile_fp64": engine_profile(jobs, cols, False,
-                                                      grouped=args.rhs_layout == "grouped")}
+                                                      grouped=args.rhs_layout == "grouped",
+                                                      max_steps=args.profile_steps)}
     e_mx, sub_mx, it_mx, ok_mx, mem, tune_mx, res_mx, st_mx, ab_mx = run_engine(True)
     log(f"batched mixed: {nrhs} solves in {e_mx:.3f}s ({ok_mx} converged, {st_mx} steps)")
     e_64, sub_64, it_64, ok_64, _, tune_64, res_64, st_64, ab_64 = run_engine(False)
