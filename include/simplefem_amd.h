@@ -195,6 +195,31 @@ int sfa_Rainbow(int n, const int* A, const int* C,
                 int* F, int* B, int* R, int* num_colors);
 
 /* ---------------------------------------------------------------------------
+ * CSR assembly operations on the GPU, bit for bit with the reference's host
+ * code: the results are held in an sfa_csr handle (host arrays) until copied
+ * out.  Both end with the reference's row-wise sort-and-compress
+ * (sparse/matcsr.cpp:585-687: entries sorted by column, duplicates summed in
+ * the reference's order, zero entries dropped), which runs one row per GPU
+ * thread in the reference's loop order.
+ * ------------------------------------------------------------------------- */
+typedef struct sfa_csr_s* sfa_csr;
+
+/* MatrixCSRMultiplication, sparse/matcsr.hpp:126 / sparse/matcsr.cpp:767-849:
+ * m1 (rows1 x cols1) times m2 (rows2 x cols2, rows2 == cols1). */
+int sfa_MatrixCSRMultiplication(int rows1, int cols1, const int* A1, const int* C1,
+                                const double* V1, int rows2, int cols2, const int* A2,
+                                const int* C2, const double* V2, sfa_csr* out);
+
+/* MatrixCSR::getTranspose, sparse/matcsr.hpp:84 / sparse/matcsr.cpp:291-350. */
+int sfa_MatrixCSR_getTranspose(int rows, int cols, const int* A, const int* C, const double* V,
+                               sfa_csr* out);
+
+/* Shape and entries of a result: A needs rows + 1 ints, C and V nnz each. */
+int sfa_csr_sizes(sfa_csr m, int* rows, int* cols, int* nnz);
+int sfa_csr_copy(sfa_csr m, int* A, int* C, double* V);
+int sfa_csr_destroy(sfa_csr m);
+
+/* ---------------------------------------------------------------------------
  * Device-resident systems.  No reference counterpart: the reference
  * rebuilds nothing between calls because it keeps everything in host memory;
  * these keep the matrix, the colour/level schedules and the Krylov

@@ -35,6 +35,7 @@ SOURCES = [  # (source, compile as HIP)
     ("hodge.hip", True),
     ("noalloc.hip", True),
     ("barrier.hip", True),
+    ("assembly.hip", True),
     ("system.cpp", True),
     ("plan.cpp", False),
 ]

@@ -181,6 +181,12 @@ _SIGNATURES = {
     "sfa_batch_submit_group": ([c_void_p, c_int, c_int, _P_D, _P_D, c_double, c_double, c_int,
                                 _P_I], c_int),
     "sfa_batch_run": ([c_void_p, c_int, POINTER(c_int)], c_int),
+    "sfa_MatrixCSRMultiplication": ([c_int, c_int, _P_I, _P_I, _P_D, c_int, c_int, _P_I, _P_I,
+                                     _P_D, POINTER(c_void_p)], c_int),
+    "sfa_MatrixCSR_getTranspose": ([c_int, c_int, _P_I, _P_I, _P_D, POINTER(c_void_p)], c_int),
+    "sfa_csr_sizes": ([c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int)], c_int),
+    "sfa_csr_copy": ([c_void_p, _P_I, _P_I, _P_D], c_int),
+    "sfa_csr_destroy": ([c_void_p], c_int),
     "sfa_batch_finished": ([c_void_p, _P_I, c_int, POINTER(c_int)], c_int),
     "sfa_batch_collect": ([c_void_p, c_int, _P_D, POINTER(BatchInfo)], c_int),
     "sfa_batch_memory": ([c_void_p, POINTER(c_longlong)], c_int),
@@ -378,6 +384,43 @@ def csr_apply(rows, columns, A, C, V, src, scaling=1.0):
     _check(load().sfa_MatrixCSR_apply(rows, columns, _i(A), _i(C), _d(V), _d(dest), _d(src),
                                       float(scaling)))
     return dest
+
+
+def _take_csr(h):
+    """(rows, cols, A, C, V) of an sfa_csr result; frees it."""
+    try:
+        rows, cols, nnz = c_int(0), c_int(0), c_int(0)
+        _check(load().sfa_csr_sizes(h, ctypes.byref(rows), ctypes.byref(cols), ctypes.byref(nnz)))
+        A = np.empty(rows.value + 1, dtype=np.int32)
+        C = np.empty(nnz.value, dtype=np.int32)
+        V = np.empty(nnz.value, dtype=np.float64)
+        _check(load().sfa_csr_copy(h, _i(A), _i(C), _d(V)))
+        return rows.value, cols.value, A, C, V
+    finally:
+        load().sfa_csr_destroy(h)
+
+
+def csr_multiply(m1, m2):
+    """MatrixCSRMultiplication (sparse/matcsr.cpp:767-849) on the GPU, bit for
+    bit with the reference; m1, m2 and the result are (rows, cols, A, C, V)."""
+    r1, c1, A1, C1, V1 = m1
+    r2, c2, A2, C2, V2 = m2
+    A1, C1, V1 = _csr(A1, C1, V1)
+    A2, C2, V2 = _csr(A2, C2, V2)
+    h = c_void_p()
+    _check(load().sfa_MatrixCSRMultiplication(int(r1), int(c1), _i(A1), _i(C1), _d(V1), int(r2),
+                                              int(c2), _i(A2), _i(C2), _d(V2), ctypes.byref(h)))
+    return _take_csr(h)
+
+
+def csr_transpose(m):
+    """MatrixCSR::getTranspose (sparse/matcsr.cpp:291-350) with its compression
+    on the GPU, bit for bit with the reference; (rows, cols, A, C, V)."""
+    r, c, A, C, V = m
+    A, C, V = _csr(A, C, V)
+    h = c_void_p()
+    _check(load().sfa_MatrixCSR_getTranspose(int(r), int(c), _i(A), _i(C), _d(V), ctypes.byref(h)))
+    return _take_csr(h)
 
 
 def get_diagonal(n, A, C, V):

@@ -581,6 +581,25 @@ MatrixCSR MatrixCSRMultiplication(const MatrixCSR& m1, const MatrixCSR& m2)
     const int rows = m1.getdimout();
     const int *A1 = m1.getA(), *C1 = m1.getC(), *A2 = m2.getA(), *C2 = m2.getC();
     const Float *V1 = m1.getV(), *V2 = m2.getV();
+    long long products = 0;
+    for (int i = 0; i < A1[rows]; i++) products += A2[C1[i] + 1] - A2[C1[i]];
+    // Large products go to the GPU (sfa_MatrixCSRMultiplication): the same
+    // products and the same compression, each row's run by one GPU thread,
+    // so the result is the same array for array.
+    constexpr long long kDeviceProducts = 1LL << 19;
+    if (products >= kDeviceProducts) {
+        sfa_csr h = nullptr;
+        require(sfa_MatrixCSRMultiplication(rows, m1.getdimin(), A1, C1, V1, m2.getdimout(),
+                                            m2.getdimin(), A2, C2, V2, &h),
+                "MatrixCSRMultiplication");
+        int r = 0, c = 0, nnz = 0;
+        require(sfa_csr_sizes(h, &r, &c, &nnz), "MatrixCSRMultiplication");
+        std::vector<int> A(r + 1), C(nnz);
+        std::vector<Float> V(nnz);
+        require(sfa_csr_copy(h, A.data(), C.data(), V.data()), "MatrixCSRMultiplication");
+        sfa_csr_destroy(h);
+        return MatrixCSR(r, c, std::move(A), std::move(C), std::move(V));
+    }
     std::vector<int> A(rows + 1, 0);
     for (int r = 0; r < rows; r++) {
         int count = 0;
