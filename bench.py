@@ -369,7 +369,7 @@ def main():
     ap.add_argument("--inner-restarts", action="store_true",
                     help="mixed forms: restart every inner CG (plain iterative refinement) "
                          "instead of continuing the previous inner solve's direction")
-    ap.add_argument("--route-rows", type=int, default=10000,
+    ap.add_argument("--route-rows", type=int, default=30000,
                     help="served: requests with at least this many rows get their own System")
     ap.add_argument("--dedicated-workers", type=int, default=12,
                     help="served: concurrent dedicated Systems (0: everything in the batch)")
