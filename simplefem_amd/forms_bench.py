@@ -333,7 +333,9 @@ def _served_fp64_one_at_a_time(jobs):
             time.sleep(j.arrival - now)
         s = j.system
         rb = capi.Rainbow(s.n, s.A, s.C)
-        with capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, s.diagonal(), 1.0,
+        # the diagonal formed as the server forms it (MatrixCSR::getDiagonal)
+        d = capi.get_diagonal(s.n, s.A, s.C, s.V)
+        with capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, d, 1.0,
                          rb.num_colors, rb.F, rb.B, rb.R) as sysd:
             bt = DeviceArray.from_host(j.rhs[0])
             x = DeviceArray(s.n)

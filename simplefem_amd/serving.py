@@ -63,9 +63,10 @@ def dedicated_bytes(n: int, nnz: int) -> float:
 
 @dataclass
 class Request:
-    """One solve: a square CSR system (``system.n``, ``.A``, ``.C``, ``.V``,
-    ``.diagonal()``) and a right-hand side; ``arrival`` in seconds after the
-    stream starts."""
+    """One solve: a square CSR system (``system.n``, ``.A``, ``.C``, ``.V``)
+    and a right-hand side; ``arrival`` in seconds after the stream starts.
+    The SSOR diagonal is the matrix's own (MatrixCSR::getDiagonal, formed by
+    the library on the serving side)."""
     system: object
     rhs: np.ndarray
     arrival: float = 0.0
@@ -103,11 +104,17 @@ def _status(success, residual_norm, target):
     return "ok" if ok else "failed"
 
 
+def _diagonal(s):
+    """MatrixCSR::getDiagonal of the request's matrix (sfa_MatrixCSR_getDiagonal:
+    C++, without the GIL, so worker threads form theirs side by side)."""
+    return capi.get_diagonal(s.n, s.A, s.C, s.V)
+
+
 def _prepare(req: Request):
     """Colouring and device-order plan of a batched request (host only)."""
     s = req.system
     rb = capi.Rainbow(s.n, s.A, s.C)
-    return capi.PreparedMatrix(s.n, s.A, s.C, s.V, s.diagonal(), 1.0, rb.num_colors, rb.F, rb.B,
+    return capi.PreparedMatrix(s.n, s.A, s.C, s.V, _diagonal(s), 1.0, rb.num_colors, rb.F, rb.B,
                                rb.R)
 
 
@@ -164,7 +171,7 @@ class SolveServer:
         t_take = time.perf_counter() - start
         s = req.system
         rb = capi.Rainbow(s.n, s.A, s.C)
-        with capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, s.diagonal(), 1.0,
+        with capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, _diagonal(s), 1.0,
                          rb.num_colors, rb.F, rb.B, rb.R, fp32_inner=True,
                          inner_restarts=self.inner_restarts) as sysd:
             t_ready = time.perf_counter() - start
