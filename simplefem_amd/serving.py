@@ -131,9 +131,9 @@ class ServerStats:
 
 class SolveServer:
     # route_rows: on the 48-request stream (1e3..2e6 unknowns, 20/s, MI355X),
-    # 30,000 served 1.92-2.05x the plain FP64 path's solves/s against
-    # 1.85-1.89x at 10,000 (three runs each), at a p50 of 1.13-1.38 s instead
-    # of 0.92-1.07 s.
+    # 30,000 served ~6% more solves/s than 10,000 (three runs each: 1.92-2.05x
+    # against 1.85-1.89x the plain FP64 path's rate), at a p50 of 1.13-1.38 s
+    # instead of 0.92-1.07 s; 100,000 served less.
     def __init__(self, budget_bytes=200e9, max_slots=256, route_rows=30000, dedicated_workers=12,
                  prep_workers=8, precision=1e-10, inner_rtol=1e-1, max_outer=50,
                  inner_restarts=False, engine_steps=128):
