@@ -284,7 +284,10 @@ int sfa_system_apply(sfa_system sys, double* dest, const double* src,
                      double scaling, int device_pointers);
 
 /* Run the solve on this HIP stream (a hipStream_t); NULL restores the
- * system's own stream. */
+ * system's own stream.  Switching waits for the work queued on the previous
+ * stream, and sfa_system_destroy waits for the work on the current one, so a
+ * stream set here must stay valid until it is replaced or the system is
+ * destroyed. */
 int sfa_system_set_stream(sfa_system sys, void* hip_stream);
 
 /* Plan facts: groups per sweep, stored entries, algorithmic bytes. */
@@ -299,6 +302,9 @@ typedef struct sfa_system_stats {
     double bytes_spmv;        /* algorithmic bytes of one SpMV launch        */
     int    kernels_per_iteration;
     int    spmv_lanes_per_row; /* SpMV mapping in use (sfa_system_tune_spmv) */
+    double setup_ms[3];       /* sfa_system_create's wall time (ms): checks and
+                                 host plan; FP64 device set-up (allocation,
+                                 uploads); the FP32 copy (0 without one)     */
 } sfa_system_stats;
 
 int sfa_system_query(sfa_system sys, sfa_system_stats* stats);

@@ -38,6 +38,7 @@ SOURCES = [  # (source, compile as HIP)
     ("assembly.hip", True),
     ("system.cpp", True),
     ("plan.cpp", False),
+    ("pool.cpp", True),
 ]
 CXX_SOURCES = ["dropin.cpp"]
 DROPIN_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "dropin_test.cpp")

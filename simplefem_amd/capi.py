@@ -47,7 +47,7 @@ class SystemStats(ctypes.Structure):
                 ("groups_forward", c_int), ("groups_backward", c_int),
                 ("stored_entries", c_longlong), ("bytes_per_iteration", c_double),
                 ("bytes_spmv", c_double), ("kernels_per_iteration", c_int),
-                ("spmv_lanes_per_row", c_int)]
+                ("spmv_lanes_per_row", c_int), ("setup_ms", c_double * 3)]
 
 
 class RefineInfo(ctypes.Structure):
@@ -565,7 +565,9 @@ class System:
     def stats(self):
         st = SystemStats()
         _check(load().sfa_system_query(self._h, ctypes.byref(st)))
-        return {k: getattr(st, k) for k, _ in st._fields_ if k != "reserved"}
+        out = {k: getattr(st, k) for k, _ in st._fields_ if k != "reserved"}
+        out["setup_ms"] = list(out["setup_ms"])
+        return out
 
     def close(self):
         if self._h:

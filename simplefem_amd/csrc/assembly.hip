@@ -170,6 +170,9 @@ struct Buffers {
     }
     ~Buffers()
     {
+        // the kernels run on the null stream; on an error path one may still
+        // be queued when the buffers go back to the cache
+        (void)hipStreamSynchronize(nullptr);
         for (void* p : held) dev_free(p);
     }
 };

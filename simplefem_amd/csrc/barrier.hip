@@ -149,6 +149,7 @@ double run(int blocks, int threads, int rounds, int mode, int* errors)
         if (exec) (void)hipGraphExecDestroy(exec);
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
+        if (s) (void)hipStreamSynchronize(s);  // before the buffers go back to the cache
         if (s) (void)hipStreamDestroy(s);
         dev_free(a.counters);
         dev_free(a.data);

@@ -1140,7 +1140,7 @@ struct Engine {
         dev_free(d_mats);
         dev_free(d_slots);
         dev_free(d_sched);
-        if (h_sched) (void)hipHostFree(h_sched);
+        pinned_free(h_sched);
         if (stream) (void)hipStreamDestroy(stream);
         d_mats = nullptr;
         d_slots = nullptr;
@@ -1169,7 +1169,7 @@ struct Engine {
         d_slots = dev_alloc<BSlot<T>>(max_slots);
         SFA_HIP(hipMemcpy(d_slots, slots.data(), sizeof(BSlot<T>) * max_slots, hipMemcpyHostToDevice));
         d_sched = dev_alloc<BSched>(1);
-        SFA_HIP(hipHostMalloc((void**)&h_sched, sizeof(BSched), hipHostMallocDefault));
+        h_sched = pinned_alloc<BSched>(1);
         std::memset(h_sched, 0, sizeof(BSched));
         h_sched->nslots = max_slots;
         SFA_HIP(hipMemcpy(d_sched, h_sched, sizeof(BSched), hipMemcpyHostToDevice));

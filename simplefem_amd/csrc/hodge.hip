@@ -160,7 +160,7 @@ struct HodgeSolver {
                         (void*)aux1, (void*)aux2, (void*)auxR, (void*)part0, (void*)part1})
             dev_free(p);
         x = b = res = dir = Mdir = Mres = aux1 = aux2 = auxR = part0 = part1 = nullptr;
-        if (hpart) (void)hipHostFree(hpart);
+        pinned_free(hpart);
         hpart = nullptr;
         if (stream) (void)hipStreamDestroy(stream);
         stream = nullptr;
@@ -236,7 +236,7 @@ struct HodgeSolver {
         nblocks = cdiv(cdiv(N, kWave), kHodgeWaves);
         part0 = dev_alloc<double>(nblocks);
         part1 = dev_alloc<double>(nblocks);
-        SFA_HIP(hipHostMalloc((void**)&hpart, 2 * nblocks * sizeof(double), hipHostMallocDefault));
+        hpart = pinned_alloc<double>(2 * (size_t)nblocks);
         for (double* v : {x, dir, Mdir}) launch_fill(stream, v, 0.0, N + 1);
         launch_fill(stream, aux2, 0.0, L + 1);
         SFA_HIP(hipStreamSynchronize(stream));

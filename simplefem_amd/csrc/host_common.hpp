@@ -2,6 +2,7 @@
 // error type and HIP checks, device allocation, SELL parts in device memory.
 #pragma once
 
+#include <algorithm>
 #include <type_traits>
 
 #include <hip/hip_runtime.h>
@@ -38,19 +39,32 @@ struct Error {
         }                                                                              \
     } while (0)
 
+// Cached blocks (pool.cpp): device memory on the current device, pinned host
+// memory.  release_block returns a block to the cache without waiting for
+// the device, so its caller must first have synchronised the stream(s) whose
+// queued work uses it.
+void* device_block(size_t bytes);
+void* pinned_block(size_t bytes);
+void release_block(void* p);
+
 template <class T>
 T* dev_alloc(size_t count)
 {
-    void* p = nullptr;
     if (count == 0) count = 1;
-    SFA_HIP(hipMalloc(&p, count * sizeof(T)));
-    return static_cast<T*>(p);
+    return static_cast<T*>(device_block(count * sizeof(T)));
 }
 
-inline void dev_free(void* p)
+// No queued GPU work may still use p (see release_block).
+inline void dev_free(void* p) { release_block(p); }
+
+template <class T>
+T* pinned_alloc(size_t count)
 {
-    if (p) (void)hipFree(p);
+    if (count == 0) count = 1;
+    return static_cast<T*>(pinned_block(count * sizeof(T)));
 }
+
+inline void pinned_free(void* p) { release_block(p); }
 
 inline void ensure_device()
 {
@@ -79,6 +93,7 @@ struct DevPartT {
     // rows and stored entries of the slices in each SpMV row-length bucket
     long long bucket_rows[kSpmvBuckets] = {};
     long long bucket_stored[kSpmvBuckets] = {};
+    bool shares_structure = false;  // off, rows, cols belong to another part (share_from)
 
     void upload(const SellPart& p)
     {
@@ -116,14 +131,48 @@ struct DevPartT {
             }
         }
     }
+    // The part of src (same slices, groups and padding) in precision T: the
+    // offsets, row ids and columns are src's arrays, which must outlive this
+    // part; the values are src's, converted on the device on stream s.
+    void share_from(const DevPartT<double>& src, hipStream_t s)
+    {
+        static_assert(std::is_same<T, float>::value, "an FP32 copy of an FP64 part");
+        nslices = src.nslices;
+        zero_slot = src.zero_slot;
+        group_slice = src.group_slice;
+        group_entries = src.group_entries;
+        group_rows = src.group_rows;
+        group_row0 = src.group_row0;
+        group_width = src.group_width;
+        group_base = src.group_base;
+        stored = src.stored;
+        entries = src.entries;
+        nrows = src.nrows;
+        for (int b = 0; b < kSpmvBuckets; b++) {
+            bucket_rows[b] = src.bucket_rows[b];
+            bucket_stored[b] = src.bucket_stored[b];
+        }
+        off = src.off;
+        rows = src.rows;
+        cols = src.cols;
+        shares_structure = true;
+        vals = dev_alloc<T>((size_t)stored);
+        constexpr long long kChunk = 1LL << 30;  // the conversion kernel counts in int
+        for (long long i = 0; i < stored; i += kChunk)
+            launch_to_float(s, vals + i, src.vals + i, (int)std::min(kChunk, stored - i));
+        SFA_HIP(hipGetLastError());
+    }
     void release()
     {
-        dev_free(off);
-        dev_free(rows);
-        dev_free(cols);
+        if (!shares_structure) {
+            dev_free(off);
+            dev_free(rows);
+            dev_free(cols);
+        }
         dev_free(vals);
         off = rows = cols = nullptr;
         vals = nullptr;
+        shares_structure = false;
     }
     SellViewT<T> view() const { return SellViewT<T>{off, rows, cols, vals, zero_slot}; }
     // the bucket holding most of the part's stored entries

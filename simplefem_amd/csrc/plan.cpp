@@ -1,7 +1,10 @@
 // plan.cpp -- device order, sweep schedules and SELL-64 parts (host code).
 #include "plan.hpp"
 
+#include <omp.h>
+
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 
 #include "../../include/simplefem_amd.h"
@@ -63,6 +66,23 @@ int rainbow_colour(int n, const int* A, const int* C, int* F, int* B, int* R)
 
 namespace {
 
+// Parts built at this moment, over all host threads.  Each OpenMP parallel
+// loop called from another host thread gets a team of its own; when several
+// threads build plans side by side (a server's workers), full-size teams
+// would put many times more spinning threads than cores on the host.  So a
+// build takes its share of the threads: all of them when it is alone.
+std::atomic<int> g_builds{0};
+
+struct BuildShare {
+    int threads;
+    BuildShare()
+    {
+        const int k = ++g_builds;
+        threads = std::max(1, omp_get_max_threads() / k);
+    }
+    ~BuildShare() { --g_builds; }
+};
+
 void start_part(SellPart& part)
 {
     part = SellPart();
@@ -91,8 +111,9 @@ void build_part(SellPart& part, const std::vector<std::vector<int>>& groups,
     for (int g = 0; g < G; g++)
         for (int sl = gs[g]; sl < gs[g + 1]; sl++) sg[sl] = g;
 
+    const BuildShare share;
     std::vector<int> width(ns, 0);
-#pragma omp parallel for schedule(dynamic, 16)
+#pragma omp parallel for schedule(dynamic, 16) num_threads(share.threads)
     for (int sl = 0; sl < ns; sl++) {
         const std::vector<int>& rows = groups[sg[sl]];
         const int s0 = (sl - gs[sg[sl]]) * kSliceRows;
@@ -136,7 +157,7 @@ void build_part(SellPart& part, const std::vector<std::vector<int>>& groups,
     part.slice_rows.resize((size_t)ns * kSliceRows);
 
     std::vector<long long> stored(ns, 0);  // real entries per slice
-#pragma omp parallel for schedule(dynamic, 16)
+#pragma omp parallel for schedule(dynamic, 16) num_threads(share.threads)
     for (int sl = 0; sl < ns; sl++) {
         const std::vector<int>& rows = groups[sg[sl]];
         const int s0 = (sl - gs[sg[sl]]) * kSliceRows;

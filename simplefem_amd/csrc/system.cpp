@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -90,8 +91,10 @@ struct SystemT {
     bool has_apply_part = false;
     std::vector<int> host_A, host_C;  // kept only to build apply_part later
     std::vector<double> host_V;
+    double setup_ms[3] = {0., 0., 0.};  // create(): host plan, FP64 device set-up, FP32 copy
 
     int* perm = nullptr;
+    bool shares_perm = false;  // perm belongs to the FP64 system (FP32 copy)
     T* diag = nullptr;
     // d and d1: the search direction, double-buffered by iteration parity
     T *x = nullptr, *r = nullptr, *b = nullptr, *d = nullptr, *d1 = nullptr, *z = nullptr,
@@ -140,6 +143,9 @@ struct SystemT {
 
     void release()
     {
+        // The buffers go back to the block cache, which does not wait for the
+        // device: nothing queued on the system's stream may still use them.
+        if (stream) (void)hipStreamSynchronize(stream);
         inner.reset();
         if (chunk_exec) (void)hipGraphExecDestroy(chunk_exec);
         chunk_exec = nullptr;
@@ -149,7 +155,9 @@ struct SystemT {
         upper.release();
         lower_restart.release();
         apply_part.release();
-        for (void* p : {(void*)perm, (void*)diag, (void*)x, (void*)r, (void*)b, (void*)d, (void*)d1,
+        if (!shares_perm) dev_free(perm);
+        shares_perm = false;
+        for (void* p : {(void*)diag, (void*)x, (void*)r, (void*)b, (void*)d, (void*)d1,
                         (void*)z, (void*)q, (void*)m, (void*)io, (void*)io2, (void*)io3,
                         (void*)dkeep, (void*)state, (void*)hist, (void*)ps, (void*)pu, (void*)pb, (void*)grp})
             dev_free(p);
@@ -157,7 +165,7 @@ struct SystemT {
         diag = x = r = b = d = d1 = z = q = m = io = io2 = io3 = dkeep = nullptr;
         hist = ps = pu = pb = grp = nullptr;
         state = nullptr;
-        if (host_state) (void)hipHostFree(host_state);
+        pinned_free(host_state);
         host_state = nullptr;
         if (own_stream) (void)hipStreamDestroy(own_stream);
         own_stream = stream = nullptr;
@@ -183,21 +191,37 @@ struct SystemT {
                 bool with_fp32 = false)
     {
         // Arguments are checked on the host before the device is touched.
+        using clock = std::chrono::steady_clock;
+        auto ms_since = [](clock::time_point t0) {
+            return std::chrono::duration<double, std::milli>(clock::now() - t0).count();
+        };
+        auto t0 = clock::now();
         Plan plan;
         bool colouring_error = false;
         std::string err = build_plan(plan, method_, n_, A, C, V, aux, omega_, nc, F, B, R,
                                      &colouring_error);
         if (!err.empty()) throw Error{colouring_error ? SFA_ERR_COLOURING : SFA_ERR_INVALID, err};
+        setup_ms[0] = ms_since(t0);
+        t0 = clock::now();
         create_from_plan(plan, A, C, V);
+        setup_ms[1] = ms_since(t0);
         if (with_fp32) {
+            t0 = clock::now();
             inner.reset(new SystemT<float>());
             inner->pair_sweeps = pair_sweeps;
-            inner->create_from_plan(plan, A, C, V);
+            inner->create_from_plan(plan, A, C, V, this);
             inner->stream = stream;
+            setup_ms[2] = ms_since(t0);
         }
     }
 
-    void create_from_plan(const Plan& plan, const int* A, const int* C, const double* V)
+    // share: for the FP32 copy of an FP64 system, that system.  The copy then
+    // takes its SELL offsets, row ids, columns and permutation (which the
+    // FP64 system owns and outlives the copy with), converts its values on
+    // the device instead of uploading them, and keeps no host CSR (it is
+    // never applied on its own).
+    void create_from_plan(const Plan& plan, const int* A, const int* C, const double* V,
+                          const SystemT<double>* share = nullptr)
     {
         const int method_ = plan.method, n_ = plan.n;
         const double omega_ = plan.omega;
@@ -211,17 +235,28 @@ struct SystemT {
         rainbow = method == SFA_METHOD_RAINBOW || method == SFA_METHOD_EISENSTAT_RAINBOW;
         eisen = method == SFA_METHOD_SSOR_EISENSTAT || method == SFA_METHOD_EISENSTAT_RAINBOW;
         sweeps = method >= SFA_METHOD_SSOR;
-        host_A.assign(A, A + n + 1);
-        host_C.assign(C, C + nnz);
-        host_V.assign(V, V + nnz);
+        if (!share) {
+            host_A.assign(A, A + n + 1);
+            host_C.assign(C, C + nnz);
+            host_V.assign(V, V + nnz);
+        }
 
         SFA_HIP(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
         stream = own_stream;
 
-        full.upload(plan.full);
+        auto part = [&](DevPartT<T>& dst, const SellPart& src, const DevPartT<double>* from) {
+            if constexpr (std::is_same<T, float>::value) {
+                if (from) {
+                    dst.share_from(*from, own_stream);
+                    return;
+                }
+            }
+            dst.upload(src);
+        };
+        part(full, plan.full, share ? &share->full : nullptr);
         if (sweeps) {
-            lower.upload(plan.lower);
-            upper.upload(plan.upper);
+            part(lower, plan.lower, share ? &share->lower : nullptr);
+            part(upper, plan.upper, share ? &share->upper : nullptr);
         }
         has_lower_restart = plan.has_lower_restart;
         if (sweeps && !eisen && lower.ngroups() > 0 && upper.ngroups() > 0) {
@@ -240,10 +275,17 @@ struct SystemT {
                                   upper.group_count(gu) == lower.group_count(0);
         }
         if (sweeps && !eisen) build_sweep_schedule(plan);
-        if (has_lower_restart) lower_restart.upload(plan.lower_restart);
+        if (has_lower_restart)
+            part(lower_restart, plan.lower_restart, share ? &share->lower_restart : nullptr);
 
-        perm = dev_alloc<int>(n);
-        SFA_HIP(hipMemcpy(perm, plan.perm.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        if (share) {
+            perm = share->perm;
+            shares_perm = true;
+            SFA_HIP(hipStreamSynchronize(own_stream));  // the converted values
+        } else {
+            perm = dev_alloc<int>(n);
+            SFA_HIP(hipMemcpy(perm, plan.perm.data(), n * sizeof(int), hipMemcpyHostToDevice));
+        }
         diag = dev_alloc<T>(n + 1);
         std::vector<T> dg(plan.diag.begin(), plan.diag.end());
         dg.push_back(T(0));
@@ -269,7 +311,7 @@ struct SystemT {
         init.hist = hist;
         init.maxK = n;
         SFA_HIP(hipMemcpy(state, &init, sizeof(SolveState), hipMemcpyHostToDevice));
-        SFA_HIP(hipHostMalloc((void**)&host_state, sizeof(SolveState), hipHostMallocDefault));
+        host_state = pinned_alloc<SolveState>(1);
     }
 
     // ---------------------------------------------------------------- phases
@@ -898,6 +940,7 @@ struct SystemT {
     {
         SFA_HIP(hipSetDevice(device));
         if (!has_apply_part) {
+            if (host_A.empty()) throw Error{SFA_ERR_INVALID, "no matrix kept for apply"};
             SellPart p;
             build_full_identity(p, n, n, host_A.data(), host_C.data(), host_V.data());
             apply_part.upload(p);
@@ -1161,6 +1204,7 @@ struct SystemT {
         o.bytes_per_iteration = it;
         o.kernels_per_iteration = kernels_per_iteration();
         o.spmv_lanes_per_row = spmv_lanes;
+        for (int i = 0; i < 3; i++) o.setup_ms[i] = setup_ms[i];
         *st = o;
     }
 };
@@ -1432,6 +1476,7 @@ int sfa_MatrixCSR_apply(int rows, int columns, const int* A, const int* C, const
             SFA_HIP(hipGetLastError());
             SFA_HIP(hipMemcpy(dest, ddst, rows * sizeof(double), hipMemcpyDeviceToHost));
         } catch (...) {
+            (void)hipStreamSynchronize(nullptr);  // the kernel may still be queued
             dp.release();
             sfa::dev_free(dsrc);
             sfa::dev_free(ddst);
@@ -1519,7 +1564,12 @@ int sfa_system_set_stream(sfa_system sys, void* hip_stream)
 {
     return sfa::guarded([&] {
         if (!sys) throw Error{SFA_ERR_INVALID, "null system"};
-        sys->sys.stream = hip_stream ? (hipStream_t)hip_stream : sys->sys.own_stream;
+        hipStream_t next = hip_stream ? (hipStream_t)hip_stream : sys->sys.own_stream;
+        // what is queued on the old stream finishes first: the system's
+        // buffers are released after a wait on the stream it runs on
+        if (sys->sys.stream && sys->sys.stream != next)
+            SFA_HIP(hipStreamSynchronize(sys->sys.stream));
+        sys->sys.stream = next;
     });
 }
 

@@ -60,19 +60,25 @@ def synchronize():
 
 
 class DeviceArray:
-    """A float64 (or int32) vector in HBM."""
+    """A float64 (or int32) vector in HBM, zero-filled unless zero=False.
 
-    def __init__(self, n: int, dtype=np.float64):
+    The solvers take x as their initial guess, as the reference's do, so a
+    vector that starts as whatever hipMalloc returned (another solve's data
+    after a free, or NaN) would change the solve it is handed to."""
+
+    def __init__(self, n: int, dtype=np.float64, zero: bool = True):
         self.n = int(n)
         self.dtype = np.dtype(dtype)
         self.nbytes = self.n * self.dtype.itemsize
         self.ptr = c_void_p()
         check(hip().hipMalloc(ctypes.byref(self.ptr), max(self.nbytes, 1)))
+        if zero:
+            self.zero()
 
     @classmethod
     def from_host(cls, a):
         a = np.ascontiguousarray(a)
-        d = cls(a.shape[0], a.dtype)
+        d = cls(a.shape[0], a.dtype, zero=False)
         d.upload(a)
         return d
 

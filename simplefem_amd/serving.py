@@ -41,7 +41,6 @@ from typing import Optional
 import numpy as np
 
 from . import capi
-from .hipmem import DeviceArray
 
 FLOOR = 2.0 ** -26  # desired_precision, base/base.hpp:121-128
 
@@ -56,9 +55,10 @@ def batch_bytes(n: int, nnz: int) -> float:
 
 def dedicated_bytes(n: int, nnz: int) -> float:
     """Estimated device bytes of a dedicated System with its FP32 inner system:
-    FP64 SpMV and triangular parts (24 B per entry), their FP32 copies (16 B),
-    SELL padding, ~15 FP64 and ~10 FP32 vectors."""
-    return 48.0 * nnz + 192.0 * n
+    FP64 SpMV and triangular parts (24 B per entry), their values in FP32 (8 B;
+    the FP32 system shares their columns), SELL padding, ~15 FP64 and ~10 FP32
+    vectors."""
+    return 40.0 * nnz + 192.0 * n
 
 
 @dataclass
@@ -126,7 +126,9 @@ class ServerStats:
     dedicated: int = 0
     rejected: int = 0
     makespan: float = 0.0
-    dedicated_phases: list = field(default_factory=list)  # (wait, create, solve) seconds
+    # per dedicated request, seconds: (wait for a worker, colouring + System
+    # creation, solve + download, freeing the System and its buffers)
+    dedicated_phases: list = field(default_factory=list)
 
 
 class SolveServer:
@@ -167,7 +169,8 @@ class SolveServer:
     def _solve_dedicated(self, req: Request, start: float):
         """Colouring, plan, upload and mixed-precision solve of one request on
         its own System (runs on a worker thread).  Returns (x, refine info,
-        time taken, time ready, time done) in seconds after start."""
+        time taken, time ready, time solved, time done) in seconds after
+        start."""
         t_take = time.perf_counter() - start
         s = req.system
         rb = capi.Rainbow(s.n, s.A, s.C)
@@ -175,17 +178,15 @@ class SolveServer:
                          rb.num_colors, rb.F, rb.B, rb.R, fp32_inner=True,
                          inner_restarts=self.inner_restarts) as sysd:
             t_ready = time.perf_counter() - start
-            bt = DeviceArray.from_host(req.rhs)
-            x = DeviceArray(s.n)
-            x.zero()
-            try:
-                info = sysd.solve_mixed(x, bt, self.precision, inner_rtol=self.inner_rtol,
-                                        max_outer=self.max_outer)
-                xh = x.download()
-            finally:
-                bt.free()
-                x.free()
-        return xh, info, t_take, t_ready, time.perf_counter() - start
+            # host arrays: the System copies them through its own buffers,
+            # where a hipMalloc'd vector would end in a hipFree that waits
+            # for every other stream's work
+            xh = np.zeros(s.n)
+            info = sysd.solve_mixed(xh, np.ascontiguousarray(req.rhs, dtype=np.float64),
+                                    self.precision, inner_rtol=self.inner_rtol,
+                                    max_outer=self.max_outer)
+            t_solved = time.perf_counter() - start
+        return xh, info, t_take, t_ready, t_solved, time.perf_counter() - start
 
     def solve_alone(self, req: Request) -> Result:
         """The request on its own, by the route the server gives it: a batch
@@ -297,10 +298,10 @@ class SolveServer:
                     if not fut.done():
                         still.append(item)
                         continue
-                    x, info, t_take, t_ready, t_done = fut.result()
+                    x, info, t_take, t_ready, t_solved, t_done = fut.result()
                     ded_bytes -= need
                     st.dedicated_phases.append((t_take - r.arrival, t_ready - t_take,
-                                                t_done - t_ready))
+                                                t_solved - t_ready, t_done - t_solved))
                     results[r.id] = Result(r.id, "dedicated",
                                            _status(info["success"], info["residual_norm"], target),
                                            x, info["inner_iterations"], info["outer"],
