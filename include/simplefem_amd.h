@@ -63,6 +63,11 @@ extern "C" {
  * needs fewer inner iterations in total.  CG, Jacobi, SSOR and Rainbow
  * variants; the Eisenstat variants always restart. */
 #define SFA_FLAG_INNER_RESTARTS 0x400
+/* Or-ed into the method of sfa_system_create: the system keeps pointers to
+ * csrrows, csrcolumns and csrvalues instead of a host copy of them (12 B per
+ * entry, read only by sfa_system_apply); the caller keeps the three arrays
+ * alive and unchanged until sfa_system_destroy. */
+#define SFA_FLAG_BORROW_CSR 0x800
 
 /* solver variants (one per function of solver/sparsesolver.hpp:11-84) */
 #define SFA_METHOD_CG                  0
@@ -302,9 +307,11 @@ typedef struct sfa_system_stats {
     double bytes_spmv;        /* algorithmic bytes of one SpMV launch        */
     int    kernels_per_iteration;
     int    spmv_lanes_per_row; /* SpMV mapping in use (sfa_system_tune_spmv) */
-    double setup_ms[3];       /* sfa_system_create's wall time (ms): checks and
-                                 host plan; FP64 device set-up (allocation,
-                                 uploads); the FP32 copy (0 without one)     */
+    double setup_ms[4];       /* sfa_system_create's wall time (ms): checks and
+                                 host plan; the host copy of the CSR arrays
+                                 kept for sfa_system_apply; FP64 device
+                                 set-up (allocation, uploads); the FP32 copy
+                                 (0 without one)                             */
 } sfa_system_stats;
 
 int sfa_system_query(sfa_system sys, sfa_system_stats* stats);

@@ -47,7 +47,7 @@ class SystemStats(ctypes.Structure):
                 ("groups_forward", c_int), ("groups_backward", c_int),
                 ("stored_entries", c_longlong), ("bytes_per_iteration", c_double),
                 ("bytes_spmv", c_double), ("kernels_per_iteration", c_int),
-                ("spmv_lanes_per_row", c_int), ("setup_ms", c_double * 3)]
+                ("spmv_lanes_per_row", c_int), ("setup_ms", c_double * 4)]
 
 
 class RefineInfo(ctypes.Structure):
@@ -82,6 +82,7 @@ class SpmvTuning(ctypes.Structure):
 FLAG_FP32_INNER = 0x100
 FLAG_PAIRED_SWEEPS = 0x200
 FLAG_INNER_RESTARTS = 0x400
+FLAG_BORROW_CSR = 0x800
 BATCH_FP64, BATCH_MIXED, BATCH_MIXED_RESTARTS = 0, 1, 2
 
 
@@ -465,7 +466,8 @@ class System:
     """
 
     def __init__(self, method, N, A, C, V, aux=None, omega=1.0, num_colors=0, F=None, B=None,
-                 R=None, fp32_inner=False, paired_sweeps=False, inner_restarts=False):
+                 R=None, fp32_inner=False, paired_sweeps=False, inner_restarts=False,
+                 borrow_csr=False):
         lib = load()
         if fp32_inner:
             method |= FLAG_FP32_INNER
@@ -474,6 +476,13 @@ class System:
         if inner_restarts:
             method |= FLAG_INNER_RESTARTS
         A, C, V = _csr(A, C, V)
+        # borrow_csr: the library keeps pointers to the CSR arrays instead of
+        # a copy; this object holds them (as passed to it, or their converted
+        # copies) until it is closed, and they must not be changed meanwhile
+        self._csr = None
+        if borrow_csr:
+            method |= FLAG_BORROW_CSR
+            self._csr = (A, C, V)
         aux = _f64(aux, "aux", N)
         F, B, R = _i32(F, "F"), _i32(B, "B"), _i32(R, "R")
         self._h = c_void_p()
@@ -573,6 +582,7 @@ class System:
         if self._h:
             load().sfa_system_destroy(self._h)
             self._h = c_void_p()
+        self._csr = None
 
     def __del__(self):
         try:

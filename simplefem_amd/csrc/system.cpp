@@ -91,7 +91,13 @@ struct SystemT {
     bool has_apply_part = false;
     std::vector<int> host_A, host_C;  // kept only to build apply_part later
     std::vector<double> host_V;
-    double setup_ms[3] = {0., 0., 0.};  // create(): host plan, FP64 device set-up, FP32 copy
+    // the CSR arrays apply_part is built from: the copies above, or the
+    // caller's arrays (SFA_FLAG_BORROW_CSR); null for an FP32 copy
+    const int* csr_A = nullptr;
+    const int* csr_C = nullptr;
+    const double* csr_V = nullptr;
+    // create(): host plan, host CSR copy, FP64 device set-up, FP32 copy
+    double setup_ms[4] = {0., 0., 0., 0.};
 
     int* perm = nullptr;
     bool shares_perm = false;  // perm belongs to the FP64 system (FP32 copy)
@@ -186,9 +192,11 @@ struct SystemT {
         SFA_HIP(hipStreamWaitEvent(s, null_done, 0));
     }
 
+    // borrow: keep A, C, V as the caller's pointers (which stay valid and
+    // unchanged until the system is destroyed) instead of a copy.
     void create(int method_, int n_, const int* A, const int* C, const double* V, const double* aux,
                 double omega_, int nc, const int* F, const int* B, const int* R,
-                bool with_fp32 = false)
+                bool with_fp32 = false, bool borrow = false)
     {
         // Arguments are checked on the host before the device is touched.
         using clock = std::chrono::steady_clock;
@@ -203,15 +211,29 @@ struct SystemT {
         if (!err.empty()) throw Error{colouring_error ? SFA_ERR_COLOURING : SFA_ERR_INVALID, err};
         setup_ms[0] = ms_since(t0);
         t0 = clock::now();
-        create_from_plan(plan, A, C, V);
+        if (borrow) {
+            csr_A = A;
+            csr_C = C;
+            csr_V = V;
+        } else {
+            host_A.assign(A, A + n_ + 1);
+            host_C.assign(C, C + A[n_]);
+            host_V.assign(V, V + A[n_]);
+            csr_A = host_A.data();
+            csr_C = host_C.data();
+            csr_V = host_V.data();
+        }
         setup_ms[1] = ms_since(t0);
+        t0 = clock::now();
+        create_from_plan(plan, A, C, V);
+        setup_ms[2] = ms_since(t0);
         if (with_fp32) {
             t0 = clock::now();
             inner.reset(new SystemT<float>());
             inner->pair_sweeps = pair_sweeps;
             inner->create_from_plan(plan, A, C, V, this);
             inner->stream = stream;
-            setup_ms[2] = ms_since(t0);
+            setup_ms[3] = ms_since(t0);
         }
     }
 
@@ -235,12 +257,6 @@ struct SystemT {
         rainbow = method == SFA_METHOD_RAINBOW || method == SFA_METHOD_EISENSTAT_RAINBOW;
         eisen = method == SFA_METHOD_SSOR_EISENSTAT || method == SFA_METHOD_EISENSTAT_RAINBOW;
         sweeps = method >= SFA_METHOD_SSOR;
-        if (!share) {
-            host_A.assign(A, A + n + 1);
-            host_C.assign(C, C + nnz);
-            host_V.assign(V, V + nnz);
-        }
-
         SFA_HIP(hipStreamCreateWithFlags(&own_stream, hipStreamNonBlocking));
         stream = own_stream;
 
@@ -940,9 +956,9 @@ struct SystemT {
     {
         SFA_HIP(hipSetDevice(device));
         if (!has_apply_part) {
-            if (host_A.empty()) throw Error{SFA_ERR_INVALID, "no matrix kept for apply"};
+            if (!csr_A) throw Error{SFA_ERR_INVALID, "no matrix kept for apply"};
             SellPart p;
-            build_full_identity(p, n, n, host_A.data(), host_C.data(), host_V.data());
+            build_full_identity(p, n, n, csr_A, csr_C, csr_V);
             apply_part.upload(p);
             has_apply_part = true;
         }
@@ -1204,7 +1220,7 @@ struct SystemT {
         o.bytes_per_iteration = it;
         o.kernels_per_iteration = kernels_per_iteration();
         o.spmv_lanes_per_row = spmv_lanes;
-        for (int i = 0; i < 3; i++) o.setup_ms[i] = setup_ms[i];
+        for (int i = 0; i < 4; i++) o.setup_ms[i] = setup_ms[i];
         *st = o;
     }
 };
@@ -1524,9 +1540,11 @@ int sfa_system_create(int method, int N, const int* csrrows, const int* csrcolum
         const bool fp32 = (method & SFA_FLAG_FP32_INNER) != 0;
         h->sys.pair_sweeps = (method & SFA_FLAG_PAIRED_SWEEPS) != 0;
         h->sys.inner_restarts = (method & SFA_FLAG_INNER_RESTARTS) != 0;
-        h->sys.create(method & ~(SFA_FLAG_FP32_INNER | SFA_FLAG_PAIRED_SWEEPS | SFA_FLAG_INNER_RESTARTS), N, csrrows,
-                      csrcolumns, csrvalues, aux, omega,
-                      num_colors, F, B, R, fp32);
+        const bool borrow = (method & SFA_FLAG_BORROW_CSR) != 0;
+        h->sys.create(method & ~(SFA_FLAG_FP32_INNER | SFA_FLAG_PAIRED_SWEEPS |
+                                 SFA_FLAG_INNER_RESTARTS | SFA_FLAG_BORROW_CSR),
+                      N, csrrows, csrcolumns, csrvalues, aux, omega, num_colors, F, B, R, fp32,
+                      borrow);
         *out = h.release();
     });
 }

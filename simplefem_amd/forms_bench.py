@@ -390,8 +390,9 @@ def _served_fp64_one_at_a_time(jobs, precisions=None):
         # the diagonal formed as the server forms it (MatrixCSR::getDiagonal)
         d = capi.get_diagonal(s.n, s.A, s.C, s.V)
         t1 = time.perf_counter()
+        # the request's arrays outlive the System, as on the served route
         with capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, d, 1.0,
-                         rb.num_colors, rb.F, rb.B, rb.R) as sysd:
+                         rb.num_colors, rb.F, rb.B, rb.R, borrow_csr=True) as sysd:
             t2 = time.perf_counter()
             bt = DeviceArray.from_host(j.rhs[0])
             x = DeviceArray(s.n)

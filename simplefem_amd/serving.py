@@ -176,7 +176,7 @@ class SolveServer:
         rb = capi.Rainbow(s.n, s.A, s.C)
         with capi.System(capi.METHOD_RAINBOW, s.n, s.A, s.C, s.V, _diagonal(s), 1.0,
                          rb.num_colors, rb.F, rb.B, rb.R, fp32_inner=True,
-                         inner_restarts=self.inner_restarts) as sysd:
+                         inner_restarts=self.inner_restarts, borrow_csr=True) as sysd:
             t_ready = time.perf_counter() - start
             # host arrays: the System copies them through its own buffers,
             # where a hipMalloc'd vector would end in a hipFree that waits

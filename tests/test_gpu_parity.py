@@ -273,6 +273,35 @@ def test_device_resident_system_matches_host_call():
         assert K * per_iter <= total <= (K + 1) * per_iter + st["groups_forward"] + st["groups_backward"]
 
 
+def test_borrowed_csr_system_matches_copied():
+    """A System that borrows the caller's CSR arrays (SFA_FLAG_BORROW_CSR)
+    solves, solves mixed and applies bit for bit as one that copies them;
+    the mixed System's FP32 copy shares the FP64 system's structure."""
+    system = S.poisson3d_p1_kuhn(9)
+    b = S.rhs_uniform(system, 12)
+    d = system.diagonal()
+    col = oracle.rainbow(system.n, system.A, system.C)
+    out = []
+    for borrow in (False, True):
+        A, C, V = system.A.copy(), system.C.copy(), system.V.copy()
+        print(f"case: borrow_csr={borrow}, n={system.n}", flush=True)
+        with capi.System(capi.METHOD_RAINBOW, system.n, A, C, V, d, 1.0, *col, fp32_inner=True,
+                         borrow_csr=borrow) as sysd:
+            x, r, info = np.zeros(system.n), np.zeros(system.n), {}
+            sysd.solve(x, b, r, 1e-10, info=info)
+            xm = np.zeros(system.n)
+            mi = sysd.solve_mixed(xm, b, 1e-10)
+            q = np.zeros(system.n)
+            sysd.apply(q, b)
+            st = sysd.stats()
+            assert len(st["setup_ms"]) == 4 and min(st["setup_ms"]) >= 0
+            assert mi["success"]
+            out.append((x, r, info["iterations"], xm, mi["inner_iterations"], q))
+    for mine, theirs in zip(*out):
+        assert np.array_equal(np.asarray(mine), np.asarray(theirs))
+    assert np.array_equal(out[1][5], oracle.csr_apply(system.n, system.A, system.C, system.V, b))
+
+
 @pytest.mark.parametrize("numbering", ["reference", "lexicographic"])
 def test_full_size_3d_rainbow(numbering):
     """The 64^3 Kuhn P1 Poisson system (250,047 active rows), rtol 1e-10,
