@@ -292,22 +292,35 @@ std::vector<int> bfs_order(int n, const int* A, const int* C)
     return order;
 }
 
-// Mean |rank[row] - rank[col]| over the entries.
-double mean_distance(int n, const int* A, const int* C, const std::vector<int>& rank)
-{
-    double sum = 0.;
-    for (int r = 0; r < n; r++)
-        for (int k = A[r]; k < A[r + 1]; k++) sum += std::abs(rank[r] - rank[C[k]]);
-    return A[n] > 0 ? sum / A[n] : 0.;
-}
+// A locality order of the rows: rank[row] is its place, order[place] the row.
+struct Locality {
+    std::vector<int> rank, order;
+};
 
-std::vector<int> locality_rank(int n, const int* A, const int* C)
+// The breadth-first order if its summed |rank[row] - rank[col]| over the
+// entries is smaller than the original order's, else the original order.
+// The sums are exact integers, so the threads may add them in any order.
+Locality locality_rank(int n, const int* A, const int* C)
 {
-    std::vector<int> ident(n), bfs(n);
-    for (int i = 0; i < n; i++) ident[i] = i;
-    const std::vector<int> order = bfs_order(n, A, C);
-    for (int i = 0; i < n; i++) bfs[order[i]] = i;
-    return mean_distance(n, A, C, bfs) < mean_distance(n, A, C, ident) ? bfs : ident;
+    Locality bfs;
+    bfs.order = bfs_order(n, A, C);
+    bfs.rank.resize(n);
+    for (int i = 0; i < n; i++) bfs.rank[bfs.order[i]] = i;
+    const BuildShare share;
+    long long sum_bfs = 0, sum_ident = 0;
+    const int* rk = bfs.rank.data();
+#pragma omp parallel for schedule(static) reduction(+ : sum_bfs, sum_ident) num_threads(share.threads)
+    for (int r = 0; r < n; r++)
+        for (int k = A[r]; k < A[r + 1]; k++) {
+            sum_bfs += std::abs(rk[r] - rk[C[k]]);
+            sum_ident += std::abs(r - C[k]);
+        }
+    if (sum_bfs < sum_ident) return bfs;
+    Locality ident;
+    ident.rank.resize(n);
+    for (int i = 0; i < n; i++) ident.rank[i] = i;
+    ident.order = ident.rank;
+    return ident;
 }
 
 }  // namespace
@@ -392,19 +405,17 @@ std::string build_plan(Plan& plan, int method, int n, const int* A, const int* C
     // ranks the rows: rows independent of one another (a colour or level,
     // the SpMV) are laid out by rank.  Only where a row sits changes, not
     // what it computes.
-    const std::vector<int> rank = locality_rank(n, A, C);
-    auto by_rank = [&](std::vector<int>& rows) {
-        std::sort(rows.begin(), rows.end(), [&](int a, int b) { return rank[a] < rank[b]; });
-    };
+    // Lists filled by walking the rows in rank order are in rank order.
+    const Locality loc = locality_rank(n, A, C);
+    const std::vector<int>& by_rank = loc.order;
 
     // ---- device order and the sweep groups (as lists of original rows) ----
     std::vector<std::vector<int>> fwd_groups, bwd_groups;  // original row ids, launch order
     std::vector<int> order;                                // active rows in device order
     if (rainbow) {
         fwd_groups.assign(num_colors, {});
-        for (int r = 0; r < n; r++)
+        for (int r : by_rank)
             if (active[r]) fwd_groups[F[r]].push_back(r);
-        for (auto& g : fwd_groups) by_rank(g);
         for (auto& g : fwd_groups) order.insert(order.end(), g.begin(), g.end());
         bwd_groups.assign(fwd_groups.rbegin(), fwd_groups.rend());
     } else if (sweeps) {
@@ -417,18 +428,15 @@ std::string build_plan(Plan& plan, int method, int n, const int* A, const int* C
         std::vector<int> lb = sweep_levels(n, A, C, active, desc, is_upper, &nb);
         fwd_groups.assign(nf, {});
         bwd_groups.assign(nb, {});
-        for (int r = 0; r < n; r++)
+        for (int r : by_rank)
             if (active[r]) {
                 fwd_groups[lf[r]].push_back(r);
                 bwd_groups[lb[r]].push_back(r);
             }
-        for (auto& g : fwd_groups) by_rank(g);
-        for (auto& g : bwd_groups) by_rank(g);
         for (auto& g : fwd_groups) order.insert(order.end(), g.begin(), g.end());
     } else {
-        for (int r = 0; r < n; r++)
+        for (int r : by_rank)
             if (active[r]) order.push_back(r);
-        by_rank(order);
     }
     plan.n_active = (int)order.size();
     plan.perm = order;
@@ -466,9 +474,8 @@ std::string build_plan(Plan& plan, int method, int n, const int* A, const int* C
     // padded little anyway, about 1.01x in 2D.)
     std::vector<int> active_rows;
     active_rows.reserve(plan.n_active);
-    for (int r = 0; r < n; r++)
+    for (int r : by_rank)
         if (active[r]) active_rows.push_back(r);
-    by_rank(active_rows);
     for (size_t w = 0; w < active_rows.size(); w += kSortWindow)
         std::stable_sort(active_rows.begin() + w,
                          active_rows.begin() + std::min(active_rows.size(), w + kSortWindow),
