@@ -6,21 +6,80 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <cstring>
 
 #include "../../include/simplefem_amd.h"
 
 namespace sfa {
+
+namespace {
+
+// Parallel passes over host arrays (a plan's parts, argument checks, the
+// comparison with a kept plan) at this moment, over all host threads.  Each
+// OpenMP parallel loop called from another host thread gets a team of its
+// own; when several threads do this side by side (a server's workers),
+// full-size teams would put many times more spinning threads than cores on
+// the host.  So each takes its share of the threads: all of them when alone.
+std::atomic<int> g_builds{0};
+
+struct BuildShare {
+    int threads;
+    BuildShare()
+    {
+        const int k = ++g_builds;
+        threads = std::max(1, omp_get_max_threads() / k);
+    }
+    ~BuildShare() { --g_builds; }
+};
+
+// Below this many elements a check runs on the calling thread alone.
+constexpr long long kParallelMin = 1LL << 18;
+
+// Whether pred(i) holds for any i in [0, count).
+template <class Pred>
+bool any_index(long long count, Pred pred)
+{
+    if (count < kParallelMin) {
+        for (long long i = 0; i < count; i++)
+            if (pred(i)) return true;
+        return false;
+    }
+    const BuildShare share;
+    int hit = 0;
+#pragma omp parallel for schedule(static) reduction(| : hit) num_threads(share.threads)
+    for (long long i = 0; i < count; i++) hit |= pred(i) ? 1 : 0;
+    return hit != 0;
+}
+
+}  // namespace
+
+bool bytes_equal(const void* a, const void* b, size_t bytes)
+{
+    constexpr size_t kChunk = size_t(1) << 20;
+    const char* x = static_cast<const char*>(a);
+    const char* y = static_cast<const char*>(b);
+    if (bytes <= 4 * kChunk) return std::memcmp(x, y, bytes) == 0;
+    const long long chunks = (long long)((bytes + kChunk - 1) / kChunk);
+    const BuildShare share;
+    int differ = 0;
+#pragma omp parallel for schedule(static) reduction(| : differ) num_threads(share.threads)
+    for (long long c = 0; c < chunks; c++) {
+        const size_t off = (size_t)c * kChunk;
+        differ |= std::memcmp(x + off, y + off, std::min(kChunk, bytes - off)) != 0;
+    }
+    return differ == 0;
+}
 
 std::string check_csr_rect(int rows, int columns, const int* A, const int* C)
 {
     if (rows <= 0 || columns <= 0) return "matrix dimensions must be positive";
     if (!A || !C) return "null CSR pointer";
     if (A[0] != 0) return "csrrows[0] must be 0";
-    for (int i = 0; i < rows; i++)
-        if (A[i + 1] < A[i]) return "csrrows must be non-decreasing";
+    if (any_index(rows, [&](long long i) { return A[i + 1] < A[i]; }))
+        return "csrrows must be non-decreasing";
     const int nnz = A[rows];
-    for (int k = 0; k < nnz; k++)
-        if (C[k] < 0 || C[k] >= columns) return "column index out of range";
+    if (any_index(nnz, [&](long long k) { return C[k] < 0 || C[k] >= columns; }))
+        return "column index out of range";
     return "";
 }
 
@@ -65,23 +124,6 @@ int rainbow_colour(int n, const int* A, const int* C, int* F, int* B, int* R)
 }
 
 namespace {
-
-// Parts built at this moment, over all host threads.  Each OpenMP parallel
-// loop called from another host thread gets a team of its own; when several
-// threads build plans side by side (a server's workers), full-size teams
-// would put many times more spinning threads than cores on the host.  So a
-// build takes its share of the threads: all of them when it is alone.
-std::atomic<int> g_builds{0};
-
-struct BuildShare {
-    int threads;
-    BuildShare()
-    {
-        const int k = ++g_builds;
-        threads = std::max(1, omp_get_max_threads() / k);
-    }
-    ~BuildShare() { --g_builds; }
-};
 
 void start_part(SellPart& part)
 {
@@ -341,7 +383,27 @@ std::string check_arguments(int method, int n, const int* A, const int* C, const
         if (B[0] != 0 || B[num_colors] != n) return "B must run from 0 to N";
         for (int c = 0; c < num_colors; c++)
             if (B[c + 1] < B[c]) return "B must be non-decreasing";
-        std::vector<char> seen(n, 0);
+        std::vector<unsigned char> seen(n, 0);
+        if (n >= kParallelMin) {
+            // one parallel pass says whether all is well; if not, the serial
+            // pass below names the first problem in row order
+            const BuildShare share;
+            int bad = 0;
+            unsigned char* sp = seen.data();
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : bad) num_threads(share.threads)
+            for (int c = 0; c < num_colors; c++)
+                for (int i = B[c]; i < B[c + 1]; i++) {
+                    const int r = R[i];
+                    if (r < 0 || r >= n || F[r] != c ||
+                        __atomic_exchange_n(&sp[r], (unsigned char)1, __ATOMIC_RELAXED))
+                        bad = 1;
+                }
+            if (!bad) {
+                *colouring_error = false;
+                return "";
+            }
+            std::fill(seen.begin(), seen.end(), (unsigned char)0);
+        }
         for (int c = 0; c < num_colors; c++)
             for (int i = B[c]; i < B[c + 1]; i++) {
                 const int r = R[i];
@@ -358,32 +420,12 @@ std::string build_plan(Plan& plan, int method, int n, const int* A, const int* C
                        const double* aux, double omega, int num_colors, const int* F,
                        const int* B, const int* R, bool* colouring_error)
 {
-    *colouring_error = false;
-    if (method < SFA_METHOD_CG || method > SFA_METHOD_EISENSTAT_RAINBOW) return "unknown method";
-    std::string err = check_csr(n, A, C);
+    const std::string err = check_arguments(method, n, A, C, V, aux, num_colors, F, B, R,
+                                            colouring_error);
     if (!err.empty()) return err;
-    if (!V) return "null csrvalues";
-    if (method != SFA_METHOD_CG && !aux) return "null diagonal/precon";
     const bool rainbow = method == SFA_METHOD_RAINBOW || method == SFA_METHOD_EISENSTAT_RAINBOW;
     const bool eisen = method == SFA_METHOD_SSOR_EISENSTAT || method == SFA_METHOD_EISENSTAT_RAINBOW;
     const bool sweeps = method >= SFA_METHOD_SSOR;
-
-    if (rainbow) {
-        *colouring_error = true;
-        if (num_colors < 0 || !F || !B || !R) return "num_colors < 0 or null F/B/R";
-        if (B[0] != 0 || B[num_colors] != n) return "B must run from 0 to N";
-        for (int c = 0; c < num_colors; c++)
-            if (B[c + 1] < B[c]) return "B must be non-decreasing";
-        std::vector<char> seen(n, 0);
-        for (int c = 0; c < num_colors; c++)
-            for (int i = B[c]; i < B[c + 1]; i++) {
-                const int r = R[i];
-                if (r < 0 || r >= n || seen[r]) return "R must be a permutation of the rows";
-                if (F[r] != c) return "F disagrees with B/R";
-                seen[r] = 1;
-            }
-        *colouring_error = false;
-    }
 
     plan = Plan();
     plan.method = method;

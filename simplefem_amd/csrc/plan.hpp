@@ -116,6 +116,10 @@ std::string build_plan(Plan& plan, int method, int n, const int* A, const int* C
 void build_full_identity(SellPart& part, int rows, int columns, const int* A, const int* C,
                          const double* V);
 
+// Whether two host arrays hold the same bytes (compared by several threads
+// when large).
+bool bytes_equal(const void* a, const void* b, size_t bytes);
+
 // Checks a rows x columns CSR matrix; returns "" when it is well formed.
 std::string check_csr_rect(int rows, int columns, const int* A, const int* C);
 

@@ -1273,7 +1273,7 @@ struct DropinCache {
     static bool same(const std::vector<T>& kept, const T* p, size_t count)
     {
         if (kept.size() != count) return false;
-        return count == 0 || (p && std::memcmp(kept.data(), p, count * sizeof(T)) == 0);
+        return count == 0 || (p && sfa::bytes_equal(kept.data(), p, count * sizeof(T)));
     }
 
     static bool matches(const Entry& e, int device, int method, int n, const int* A,

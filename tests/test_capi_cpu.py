@@ -122,6 +122,49 @@ def test_inconsistent_colouring_is_rejected_before_the_device():
     assert e.value.code == -5
 
 
+def test_large_inputs_are_checked_with_the_same_messages():
+    """At 2^18 rows and more the host checks run a parallel pass first and,
+    when it finds a problem, the serial pass for its message: the first
+    problem in row order, as below that size."""
+    s = S.poisson3d_p1_kuhn(64)  # 274,625 rows
+    n = s.n
+    rb = capi.Rainbow(n, s.A, s.C)
+    d = s.diagonal()
+
+    def call(A=s.A, C=s.C, F=rb.F, R=rb.R):
+        return capi.ConjugateGradientSolverCSR_Rainbow(n, np.zeros(n), np.ones(n), A, C, s.V,
+                                                      np.zeros(n), 1e-10, -3, d, 1.0,
+                                                      rb.num_colors, F, rb.B, R)
+
+    c0, c1 = int(rb.B[3]), int(rb.B[4])  # colour 3's positions in R
+    # a row listed twice early in colour 3, and a wrong colour for a row later on
+    R2 = rb.R.copy()
+    R2[c0 + 10] = R2[c0 + 9]
+    F2 = rb.F.copy()
+    r_late = int(rb.R[c1 - 5])
+    F2[r_late] = (F2[r_late] + 1) % rb.num_colors
+    for F, R, code, text in ((F2, R2, -5, "R must be a permutation of the rows"),
+                             (F2, rb.R, -5, "F disagrees with B/R")):
+        with pytest.raises(capi.SfaError) as e:
+            call(F=F, R=R)
+        assert e.value.code == code and text in str(e.value), str(e.value)
+    C2 = s.C.copy()
+    C2[-1] = n
+    with pytest.raises(capi.SfaError) as e:
+        call(C=C2)
+    assert e.value.code == -1 and "column index out of range" in str(e.value)
+    A2 = s.A.copy()
+    A2[n - 7] = A2[n - 6] + 1
+    with pytest.raises(capi.SfaError) as e:
+        call(A=A2)
+    assert e.value.code == -1 and "csrrows must be non-decreasing" in str(e.value)
+    # valid arrays pass every check (here without a GPU the call then stops there)
+    try:
+        call()
+    except capi.SfaError as e:
+        assert e.code == -3, str(e)
+
+
 def test_null_arguments():
     lib = capi.load()
     info = capi.SolveInfo()
