@@ -385,7 +385,7 @@ def main():
                          "boundary (sfa_grid_barrier_cost) and print one line per grid")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU: this build has no multi-GPU path "
+        print(f"bench.py runs on one GPU: the library has no multi-GPU path "
               f"(--gpus {args.gpus} requested).", file=sys.stderr)
         return 2
     if args.copy_calibration > 0:

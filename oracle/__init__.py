@@ -1,8 +1,8 @@
 """Python binding of the oracle (CPU restatement of FEEC++'s CG/SSOR path).
 
 TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and the
-cpu_baseline leg of bench.py may import this package: it is the checker the
-HIP path is compared against, never part of the product.
+cpu_baseline leg of bench.py may import this package: it is what the tests
+compare the HIP path against, never part of the product.
 
 * ``liboracle.so`` (oracle.c) -- the restatement, in-process via ctypes;
 * ``oracle.ref`` -- the reference's own code compiled from its sources into

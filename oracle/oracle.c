@@ -1,7 +1,7 @@
 /*
  * oracle.c -- CPU restatement of FEEC++'s CSR CG / SSOR / Rainbow-SSOR path.
  *
- * TEST INFRASTRUCTURE ONLY: the checker for the HIP path, never the product.
+ * TEST INFRASTRUCTURE ONLY: the tests compare the HIP path with it; never the product.
  * Every loop below follows the reference routine named in its comment, with
  * the same operation order; reductions run in sequential row order.
  * Paths are relative to the reference repository (martinlicht/simplefem).

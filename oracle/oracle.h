@@ -1,8 +1,8 @@
 /*
  * oracle.h -- CPU restatement of FEEC++'s sparse CG/SSOR solve path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker that the HIP path
- * is compared against: only tests/, __graft_entry__.smoke() and the
+ * TEST INFRASTRUCTURE ONLY.  The tests compare the HIP path with this
+ * library: only tests/, __graft_entry__.smoke() and the
  * cpu_baseline leg of bench.py may load it.  Nothing in simplefem_amd/
  * links, loads or calls it.
  *

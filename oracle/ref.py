@@ -1,6 +1,6 @@
 """Calls into the reference's own compiled code (oracle/_ref/libfeecref.so).
 
-TEST INFRASTRUCTURE ONLY (checker and CPU baseline).  Each RefRunner owns
+TEST INFRASTRUCTURE ONLY (test comparisons and the CPU baseline).  Each RefRunner owns
 one child process (oracle/_refworker.py) that loads the library with
 OMP_NUM_THREADS fixed, so the reference's start-up code (banner, flush-to-zero
 on every OpenMP thread, base/base.cpp:38-292) stays out of the calling

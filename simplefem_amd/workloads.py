@@ -11,7 +11,7 @@
   1..3, and carries one seeded right-hand side.  Requests of equal shape share
   their generated matrix (the generator is deterministic).  Their meshes have
   any number of cells per axis, which the reference's refinement (2^L only)
-  cannot produce, so they use this build's own nodal Lagrange assembly
+  cannot produce, so they use the package's own nodal Lagrange assembly
   (``systems.lagrange_pr``).
 """
 from __future__ import annotations

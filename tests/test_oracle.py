@@ -1,4 +1,4 @@
-"""Pins the oracle (CPU restatement) before it is trusted as the checker.
+"""Pins the oracle (CPU restatement) before the GPU tests compare against it.
 
 Checks, in order of independence from the oracle's own code:
   * golden fixtures written from the compiled reference (tests/golden/);
