@@ -521,6 +521,9 @@ def main():
             "colours": rb.num_colors, "iterations": info["iterations"],
             "kernels_per_iteration": stats["kernels_per_iteration"],
             "bytes_per_iteration": stats["bytes_per_iteration"],
+            # the System's creation (a drop-in call's first-call cost), ms
+            "setup_ms": dict(zip(("host_plan", "host_copy", "device", "fp32_copy"),
+                                 [round(v, 1) for v in stats["setup_ms"]])),
             "spmv_mapping": tuning,
             "true_relative_residual": rel_res,
         },
