@@ -518,3 +518,26 @@ def test_repeated_dropin_calls_reuse_the_plan_and_see_changes():
                              colouring=oracle.rainbow(s1.n, s1.A, s1.C))
     assert abs(i2["iterations"] - io["iterations"]) <= 1
     _close(x2, xo)
+
+
+def test_dropin_sees_a_change_in_the_last_entries_of_large_arrays():
+    """The drop-in's comparison of large arrays with a kept plan (above 4 MB
+    it compares chunks on several host threads) finds a single changed value
+    near the end: the diagonal of the last active row of the 48^3 system,
+    scaled in place, gives the modified matrix's solution."""
+    s = S.poisson3d_p1_kuhn(48)  # 117,649 rows, ~1.7 M entries: 13.7 MB of values
+    b = S.rhs_uniform(s, 21)
+    col = oracle.rainbow(s.n, s.A, s.C)
+    print(f"case: drop-in change detection, n={s.n} nnz={s.nnz}", flush=True)
+    x1, _, i1 = _gpu_solve("rainbow", s, b, s.diagonal(), 1.0, col)
+    r = int(np.nonzero(s.diagonal())[0][-1])
+    k = int(s.A[r] + np.nonzero(s.C[s.A[r]:s.A[r + 1]] == r)[0][0])
+    assert k > 0.98 * s.nnz  # the change lies among the last chunks compared
+    s.V[k] *= 1.5
+    d = s.diagonal()
+    x2, _, i2 = _gpu_solve("rainbow", s, b, d, 1.0, col)
+    xo, _, io = oracle.solve("rainbow", s.A, s.C, s.V, b, aux=d, omega=1.0, colouring=col)
+    s.V[k] /= 1.5
+    assert not np.array_equal(x1, x2)
+    assert abs(i2["iterations"] - io["iterations"]) <= 1
+    _close(x2, xo)
