@@ -26,7 +26,7 @@
 extern "C" {
 #endif
 
-#define SFA_API_VERSION 1
+#define SFA_API_VERSION 2
 
 /* status codes */
 #define SFA_OK                 0

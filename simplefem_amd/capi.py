@@ -226,6 +226,10 @@ def header_symbols(path: str = HEADER):
     return sorted(set(re.findall(r"\b(sfa_\w+)\s*\(", text)))
 
 
+# SFA_API_VERSION of include/simplefem_amd.h that the structures here follow
+API_VERSION = 2
+
+
 def load(path: str = LIB_PATH):
     """Loads the HIP library (raises ImportError when it has not been built)."""
     global _lib
@@ -238,6 +242,9 @@ def load(path: str = LIB_PATH):
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = res
+    if lib.sfa_api_version() != API_VERSION:
+        raise ImportError(f"{path} has API version {lib.sfa_api_version()}, this binding "
+                          f"needs {API_VERSION}: rebuild it with `python -m simplefem_amd.build`")
     if path == LIB_PATH:
         _lib = lib
     return lib

@@ -21,7 +21,7 @@ def test_library_exports_every_header_symbol():
     assert len(names) >= 17
     for name in names:
         assert hasattr(lib, name), name
-    assert lib.sfa_api_version() == 1
+    assert lib.sfa_api_version() == capi.API_VERSION == 2
 
 
 def test_python_signatures_cover_header():
