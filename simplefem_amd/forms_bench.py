@@ -509,7 +509,8 @@ def served(args, budget=200e9):
 # (measured on MI355X; the batched form's lockstep right-hand-side groups wait
 # for each other at every correction, so fewer, longer inner solves pay there:
 # 1e-1 4.8 s, 1e-3 4.24 s, 1e-4 4.01 s, 1e-5 4.26 s; on H(curl) 1e-1 is best:
-# 1e-3 and 1e-2 take 0.85x and 0.68x its solves/s).
+# 1e-3 and 1e-2 take 0.85x and 0.68x its solves/s; 0.05, 0.2 and 0.3 gave
+# 5.13, 5.10 and 5.15 solves/s against 5.41).
 INNER_RTOL = {"hcurl-mixed": 1e-1, "batched": 1e-4, "served": 1e-1}
 
 
