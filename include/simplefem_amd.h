@@ -202,7 +202,7 @@ int sfa_Rainbow(int n, const int* A, const int* C,
 /* ---------------------------------------------------------------------------
  * CSR assembly operations on the GPU, bit for bit with the reference's host
  * code: the results are held in an sfa_csr handle (host arrays) until copied
- * out.  Both end with the reference's row-wise sort-and-compress
+ * out.  All end with the reference's row-wise sort-and-compress
  * (sparse/matcsr.cpp:585-687: entries sorted by column, duplicates summed in
  * the reference's order, zero entries dropped), which runs one row per GPU
  * thread in the reference's loop order.
@@ -214,6 +214,14 @@ typedef struct sfa_csr_s* sfa_csr;
 int sfa_MatrixCSRMultiplication(int rows1, int cols1, const int* A1, const int* C1,
                                 const double* V1, int rows2, int cols2, const int* A2,
                                 const int* C2, const double* V2, sfa_csr* out);
+
+/* MatrixCSRMultiplication_reduced, sparse/matcsr.hpp:128 /
+ * sparse/matcsr.cpp:857-962: only pairs of nonzero values form products;
+ * each row's distinct columns in increasing order, each value 0 plus its
+ * products in the order they are formed, then the same compression. */
+int sfa_MatrixCSRMultiplication_reduced(int rows1, int cols1, const int* A1, const int* C1,
+                                        const double* V1, int rows2, int cols2, const int* A2,
+                                        const int* C2, const double* V2, sfa_csr* out);
 
 /* MatrixCSR::getTranspose, sparse/matcsr.hpp:84 / sparse/matcsr.cpp:291-350. */
 int sfa_MatrixCSR_getTranspose(int rows, int cols, const int* A, const int* C, const double* V,

@@ -184,6 +184,8 @@ _SIGNATURES = {
     "sfa_batch_run": ([c_void_p, c_int, POINTER(c_int)], c_int),
     "sfa_MatrixCSRMultiplication": ([c_int, c_int, _P_I, _P_I, _P_D, c_int, c_int, _P_I, _P_I,
                                      _P_D, POINTER(c_void_p)], c_int),
+    "sfa_MatrixCSRMultiplication_reduced": ([c_int, c_int, _P_I, _P_I, _P_D, c_int, c_int, _P_I,
+                                             _P_I, _P_D, POINTER(c_void_p)], c_int),
     "sfa_MatrixCSR_getTranspose": ([c_int, c_int, _P_I, _P_I, _P_D, POINTER(c_void_p)], c_int),
     "sfa_csr_sizes": ([c_void_p, POINTER(c_int), POINTER(c_int), POINTER(c_int)], c_int),
     "sfa_csr_copy": ([c_void_p, _P_I, _P_I, _P_D], c_int),
@@ -408,16 +410,19 @@ def _take_csr(h):
         load().sfa_csr_destroy(h)
 
 
-def csr_multiply(m1, m2):
-    """MatrixCSRMultiplication (sparse/matcsr.cpp:767-849) on the GPU, bit for
-    bit with the reference; m1, m2 and the result are (rows, cols, A, C, V)."""
+def csr_multiply(m1, m2, reduced=False):
+    """MatrixCSRMultiplication (sparse/matcsr.cpp:767-849), or with reduced=True
+    MatrixCSRMultiplication_reduced (:857-962), on the GPU, bit for bit with
+    the reference; m1, m2 and the result are (rows, cols, A, C, V)."""
     r1, c1, A1, C1, V1 = m1
     r2, c2, A2, C2, V2 = m2
     A1, C1, V1 = _csr(A1, C1, V1)
     A2, C2, V2 = _csr(A2, C2, V2)
     h = c_void_p()
-    _check(load().sfa_MatrixCSRMultiplication(int(r1), int(c1), _i(A1), _i(C1), _d(V1), int(r2),
-                                              int(c2), _i(A2), _i(C2), _d(V2), ctypes.byref(h)))
+    fn = (load().sfa_MatrixCSRMultiplication_reduced if reduced
+          else load().sfa_MatrixCSRMultiplication)
+    _check(fn(int(r1), int(c1), _i(A1), _i(C1), _d(V1), int(r2), int(c2), _i(A2), _i(C2), _d(V2),
+              ctypes.byref(h)))
     return _take_csr(h)
 
 

@@ -1,8 +1,8 @@
-// assembly.hip -- the reference's CSR product and transpose with their
+// assembly.hip -- the reference's CSR products and transpose with their
 // row-wise sort-and-compress on the GPU, bit for bit:
-// MatrixCSRMultiplication (sparse/matcsr.cpp:767-849) and
-// MatrixCSR::getTranspose (:291-350), both ending in the compression of
-// :585-687.
+// MatrixCSRMultiplication (sparse/matcsr.cpp:767-849),
+// MatrixCSRMultiplication_reduced (:857-962) and MatrixCSR::getTranspose
+// (:291-350), all ending in the compression of :585-687.
 //
 // The compression is a per-row selection sort that merges an entry into the
 // first one of its column as it meets it and then moves zeros to the end; the
@@ -80,6 +80,101 @@ __global__ void __launch_bounds__(kRowBlock) k_product_fill(int rows, const int*
         }
         at += len;
     }
+}
+
+// Products per row of the reduced product: pairs whose two values are both
+// nonzero (sparse/matcsr.cpp:889-892).
+__global__ void __launch_bounds__(kRowBlock) k_product_counts_nz(int rows, const int* __restrict__ A1,
+                                                                 const int* __restrict__ C1,
+                                                                 const double* __restrict__ V1,
+                                                                 const int* __restrict__ A2,
+                                                                 const double* __restrict__ V2,
+                                                                 long long* __restrict__ counts)
+{
+    const int r = blockIdx.x * kRowBlock + threadIdx.x;
+    if (r >= rows) return;
+    long long n = 0;
+    for (int i = A1[r]; i < A1[r + 1]; i++) {
+        if (V1[i] == 0.) continue;
+        for (int k = A2[C1[i]]; k < A2[C1[i] + 1]; k++) n += V2[k] != 0. ? 1 : 0;
+    }
+    counts[r] = n;
+}
+
+// Row r's products with both values nonzero, from position P[r], in the
+// reference's (entry of m1, entry of m2) order.  One wavefront per row: its
+// lanes take one row of m2 at a time and pack the kept products with a
+// ballot, lanes in order.
+__global__ void __launch_bounds__(kRowBlock) k_product_fill_nz(int rows, const int* __restrict__ A1,
+                                                               const int* __restrict__ C1,
+                                                               const double* __restrict__ V1,
+                                                               const int* __restrict__ A2,
+                                                               const int* __restrict__ C2,
+                                                               const double* __restrict__ V2,
+                                                               const long long* __restrict__ P,
+                                                               int* __restrict__ C,
+                                                               double* __restrict__ V)
+{
+    const int lane = threadIdx.x & 63;
+    const int r = blockIdx.x * kWaveRows + (threadIdx.x >> 6);
+    if (r >= rows) return;
+    const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+    long long at = P[r];
+    for (int i = A1[r]; i < A1[r + 1]; i++) {
+        const double v1 = V1[i];
+        if (v1 == 0.) continue;  // wave-uniform
+        const int c1 = C1[i];
+        const int k0 = A2[c1], len = A2[c1 + 1] - k0;
+        for (int t0 = 0; t0 < len; t0 += 64) {
+            const int t = t0 + lane;
+            const double v2 = t < len ? V2[k0 + t] : 0.;
+            const bool keep = t < len && v2 != 0.;
+            const unsigned long long mask = __ballot(keep);
+            if (keep) {
+                const long long to = at + __popcll(mask & below);
+                C[to] = C2[k0 + t];
+                V[to] = v1 * v2;
+            }
+            at += __popcll(mask);
+        }
+    }
+}
+
+// The reduced product's row before its compression (sparse/matcsr.cpp:
+// 886-950): the distinct columns in increasing order, each value 0 plus its
+// products in the order they were formed.  The row's products are sorted by
+// column with an insertion sort, which keeps equal columns in their order,
+// and each run is summed from 0; the row's distinct entries end up at its
+// start and their count in distinct[r].  One thread per row.
+__global__ void __launch_bounds__(kRowBlock) k_merge_columns(int rows, const long long* __restrict__ P,
+                                                             int* __restrict__ C,
+                                                             double* __restrict__ V,
+                                                             int* __restrict__ distinct)
+{
+    const int r = blockIdx.x * kRowBlock + threadIdx.x;
+    if (r >= rows) return;
+    const long long b = P[r], e = P[r + 1];
+    for (long long i = b + 1; i < e; i++) {
+        const int c = C[i];
+        const double v = V[i];
+        long long j = i;
+        for (; j > b && C[j - 1] > c; j--) {
+            C[j] = C[j - 1];
+            V[j] = V[j - 1];
+        }
+        C[j] = c;
+        V[j] = v;
+    }
+    long long out = b;
+    for (long long i = b; i < e;) {
+        const int c = C[i];
+        double s = 0.;
+        for (; i < e && C[i] == c; i++) s += V[i];
+        C[out] = c;
+        V[out] = s;
+        out++;
+    }
+    distinct[r] = (int)(out - b);
 }
 
 // The reference's compression of one row, positions b .. e-1
@@ -269,6 +364,77 @@ int sfa_MatrixCSRMultiplication(int rows1, int cols1, const int* A1, const int* 
                         (const long long*)dP, dC, dV);
         }
         sfa::compress_rows(rows1, cols2, P, dC, dV, buf, res.get());
+        SFA_HIP(hipDeviceSynchronize());
+        *out = res.release();
+    });
+}
+
+int sfa_MatrixCSRMultiplication_reduced(int rows1, int cols1, const int* A1, const int* C1,
+                                        const double* V1, int rows2, int cols2, const int* A2,
+                                        const int* C2, const double* V2, sfa_csr* out)
+{
+    return sfa::guarded([&] {
+        if (!out) throw sfa::Error{SFA_ERR_INVALID, "null output handle"};
+        *out = nullptr;
+        if (cols1 != rows2) throw sfa::Error{SFA_ERR_INVALID, "columns of m1 != rows of m2"};
+        std::string err = sfa::check_csr_rect(rows1, cols1, A1, C1);
+        if (err.empty()) err = sfa::check_csr_rect(rows2, cols2, A2, C2);
+        if (!err.empty()) throw sfa::Error{SFA_ERR_INVALID, err};
+        if ((A1[rows1] > 0 && !V1) || (A2[rows2] > 0 && !V2))
+            throw sfa::Error{SFA_ERR_INVALID, "null values"};
+        sfa::ensure_device();
+        std::unique_ptr<sfa_csr_s> res(new sfa_csr_s());
+        sfa::Buffers buf;
+        const int nnz1 = A1[rows1], nnz2 = A2[rows2];
+        int* dA1 = sfa::upload(buf, A1, rows1 + 1);
+        int* dC1 = sfa::upload(buf, C1, nnz1);
+        double* dV1 = sfa::upload(buf, V1, nnz1);
+        int* dA2 = sfa::upload(buf, A2, rows2 + 1);
+        int* dC2 = sfa::upload(buf, C2, nnz2);
+        double* dV2 = sfa::upload(buf, V2, nnz2);
+        long long* dcount = buf.get<long long>(rows1);
+        if (rows1 > 0)
+            sfa::launch(sfa::k_product_counts_nz, dim3(sfa::blocks_for(rows1, sfa::kRowBlock)),
+                        dim3(sfa::kRowBlock), nullptr, rows1, (const int*)dA1, (const int*)dC1,
+                        (const double*)dV1, (const int*)dA2, (const double*)dV2, dcount);
+        std::vector<long long> P(rows1 + 1, 0);
+        if (rows1 > 0)
+            SFA_HIP(hipMemcpy(P.data() + 1, dcount, rows1 * sizeof(long long), hipMemcpyDeviceToHost));
+        for (int r = 0; r < rows1; r++) P[r + 1] += P[r];
+        const long long np = P[rows1];
+        int* dC = buf.get<int>(np);
+        double* dV = buf.get<double>(np);
+        int* ddistinct = buf.get<int>(rows1);
+        std::vector<long long> Q(rows1 + 1, 0);  // offsets of the distinct entries
+        if (rows1 > 0 && np > 0) {
+            long long* dP = sfa::upload(buf, P.data(), P.size());
+            sfa::launch(sfa::k_product_fill_nz, dim3(sfa::blocks_for(rows1, sfa::kWaveRows)),
+                        dim3(sfa::kRowBlock), nullptr, rows1, (const int*)dA1, (const int*)dC1,
+                        (const double*)dV1, (const int*)dA2, (const int*)dC2, (const double*)dV2,
+                        (const long long*)dP, dC, dV);
+            sfa::launch(sfa::k_merge_columns, dim3(sfa::blocks_for(rows1, sfa::kRowBlock)),
+                        dim3(sfa::kRowBlock), nullptr, rows1, (const long long*)dP, dC, dV,
+                        ddistinct);
+            std::vector<int> distinct(rows1);
+            SFA_HIP(hipMemcpy(distinct.data(), ddistinct, rows1 * sizeof(int),
+                              hipMemcpyDeviceToHost));
+            for (int r = 0; r < rows1; r++) Q[r + 1] = Q[r] + distinct[r];
+        }
+        // the reference stores the distinct entries with int offsets
+        if (Q[rows1] > INT_MAX) throw sfa::Error{SFA_ERR_INVALID, "more than 2^31 - 1 entries"};
+        const long long nd = Q[rows1];
+        int* dCd = buf.get<int>(nd);
+        double* dVd = buf.get<double>(nd);
+        if (nd > 0) {
+            std::vector<int> Qi(Q.begin(), Q.end());
+            long long* dP = sfa::upload(buf, P.data(), P.size());
+            int* dQ = sfa::upload(buf, Qi.data(), Qi.size());
+            sfa::launch(sfa::k_compact, dim3(sfa::blocks_for(rows1, sfa::kWaveRows)),
+                        dim3(sfa::kRowBlock), nullptr, rows1, (const long long*)dP,
+                        (const int*)dQ, (const int*)dC, (const double*)dV, dCd, dVd);
+        }
+        // then the compression of sparse/matcsr.cpp:959 (:690-693 calls :585-687)
+        sfa::compress_rows(rows1, cols2, Q, dCd, dVd, buf, res.get());
         SFA_HIP(hipDeviceSynchronize());
         *out = res.release();
     });

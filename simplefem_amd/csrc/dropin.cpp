@@ -572,6 +572,37 @@ MatrixCSR MatrixCSRAddition(const MatrixCSR& m1, const MatrixCSR& m2, Float s1, 
     return MatrixCSR(rows, m1.getdimin(), std::move(A), std::move(C), std::move(V));
 }
 
+// Products of at least this many pairs of entries run on the GPU
+// (sfa_MatrixCSRMultiplication, sfa_MatrixCSRMultiplication_reduced): the
+// same products and the same compression, each row's run by one GPU thread,
+// so the result is the same array for array.
+constexpr long long kDeviceProducts = 1LL << 19;
+
+static long long product_pairs(const MatrixCSR& m1, const MatrixCSR& m2)
+{
+    const int *A1 = m1.getA(), *C1 = m1.getC(), *A2 = m2.getA();
+    long long products = 0;
+    for (int i = 0; i < A1[m1.getdimout()]; i++) products += A2[C1[i] + 1] - A2[C1[i]];
+    return products;
+}
+
+static MatrixCSR device_product(const MatrixCSR& m1, const MatrixCSR& m2, bool reduced,
+                                const char* what)
+{
+    auto fn = reduced ? sfa_MatrixCSRMultiplication_reduced : sfa_MatrixCSRMultiplication;
+    sfa_csr h = nullptr;
+    require(fn(m1.getdimout(), m1.getdimin(), m1.getA(), m1.getC(), m1.getV(), m2.getdimout(),
+               m2.getdimin(), m2.getA(), m2.getC(), m2.getV(), &h),
+            what);
+    int r = 0, c = 0, nnz = 0;
+    require(sfa_csr_sizes(h, &r, &c, &nnz), what);
+    std::vector<int> A(r + 1), C(nnz);
+    std::vector<Float> V(nnz);
+    require(sfa_csr_copy(h, A.data(), C.data(), V.data()), what);
+    sfa_csr_destroy(h);
+    return MatrixCSR(r, c, std::move(A), std::move(C), std::move(V));
+}
+
 // mat1 * mat2 (sparse/matcsr.cpp:767-849): row r collects, for each entry
 // (r, c1) of mat1 and each entry (c1, c2) of mat2 in order, the product
 // v1 * v2 at column c2; sort_and_compress then merges and drops zeros.
@@ -581,25 +612,8 @@ MatrixCSR MatrixCSRMultiplication(const MatrixCSR& m1, const MatrixCSR& m2)
     const int rows = m1.getdimout();
     const int *A1 = m1.getA(), *C1 = m1.getC(), *A2 = m2.getA(), *C2 = m2.getC();
     const Float *V1 = m1.getV(), *V2 = m2.getV();
-    long long products = 0;
-    for (int i = 0; i < A1[rows]; i++) products += A2[C1[i] + 1] - A2[C1[i]];
-    // Large products go to the GPU (sfa_MatrixCSRMultiplication): the same
-    // products and the same compression, each row's run by one GPU thread,
-    // so the result is the same array for array.
-    constexpr long long kDeviceProducts = 1LL << 19;
-    if (products >= kDeviceProducts) {
-        sfa_csr h = nullptr;
-        require(sfa_MatrixCSRMultiplication(rows, m1.getdimin(), A1, C1, V1, m2.getdimout(),
-                                            m2.getdimin(), A2, C2, V2, &h),
-                "MatrixCSRMultiplication");
-        int r = 0, c = 0, nnz = 0;
-        require(sfa_csr_sizes(h, &r, &c, &nnz), "MatrixCSRMultiplication");
-        std::vector<int> A(r + 1), C(nnz);
-        std::vector<Float> V(nnz);
-        require(sfa_csr_copy(h, A.data(), C.data(), V.data()), "MatrixCSRMultiplication");
-        sfa_csr_destroy(h);
-        return MatrixCSR(r, c, std::move(A), std::move(C), std::move(V));
-    }
+    if (product_pairs(m1, m2) >= kDeviceProducts)
+        return device_product(m1, m2, false, "MatrixCSRMultiplication");
     std::vector<int> A(rows + 1, 0);
     for (int r = 0; r < rows; r++) {
         int count = 0;
@@ -626,6 +640,8 @@ MatrixCSR MatrixCSRMultiplication(const MatrixCSR& m1, const MatrixCSR& m2)
 MatrixCSR MatrixCSRMultiplication_reduced(const MatrixCSR& m1, const MatrixCSR& m2)
 {
     require_true(m1.getdimin() == m2.getdimout(), "MatrixCSRMultiplication_reduced dimensions");
+    if (product_pairs(m1, m2) >= kDeviceProducts)
+        return device_product(m1, m2, true, "MatrixCSRMultiplication_reduced");
     const int rows = m1.getdimout();
     const int *A1 = m1.getA(), *C1 = m1.getC(), *A2 = m2.getA(), *C2 = m2.getC();
     const Float *V1 = m1.getV(), *V2 = m2.getV();

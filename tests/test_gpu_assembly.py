@@ -1,7 +1,8 @@
-"""CSR product and transpose on the GPU (sfa_MatrixCSRMultiplication,
-sfa_MatrixCSR_getTranspose) against the compiled reference's
-MatrixCSRMultiplication and MatrixCSR::getTranspose (sparse/matcsr.cpp),
-bit for bit: the A, C and V arrays are equal.  The inputs hold what makes the
+"""CSR products and transpose on the GPU (sfa_MatrixCSRMultiplication,
+sfa_MatrixCSRMultiplication_reduced, sfa_MatrixCSR_getTranspose) against the
+compiled reference's MatrixCSRMultiplication, MatrixCSRMultiplication_reduced
+and MatrixCSR::getTranspose (sparse/matcsr.cpp), bit for bit: the A, C and V
+arrays are equal.  The inputs hold what makes the
 reference's compression order matter -- repeated columns within a row,
 unsorted rows, exact zeros, small integer values whose products cancel to
 exactly zero (dropped), irregular values whose duplicate sums round by
@@ -60,16 +61,23 @@ def test_product_and_transpose_match_reference_bitwise():
                 what = ("mul", rows, inner, cols, unsorted)
                 print(f"case: {what}", flush=True)
                 _same(capi.csr_multiply(m1, m2), ref.csr_op("mul", m1, m2), what)
+                what = ("mul_reduced", rows, inner, cols, unsorted)
+                _same(capi.csr_multiply(m1, m2, reduced=True), ref.csr_op("mul_reduced", m1, m2),
+                      what)
                 what = ("transpose", rows, inner, unsorted)
                 _same(capi.csr_transpose(m1), ref.csr_op("transpose", m1), what)
         # no entries at all, and a product with no surviving entry
         empty = (4, 3, np.zeros(5, np.int32), np.zeros(0, np.int32), np.zeros(0))
         m = _random_csr(rng, 3, 5, 4, True)
         _same(capi.csr_multiply(empty, m), ref.csr_op("mul", empty, m), "empty product")
+        _same(capi.csr_multiply(empty, m, reduced=True), ref.csr_op("mul_reduced", empty, m),
+              "empty reduced product")
         _same(capi.csr_transpose(empty), ref.csr_op("transpose", empty), "empty transpose")
         zero = (3, 5, m[2], m[3], np.zeros_like(m[4]))
         m2 = _random_csr(rng, 5, 4, 3, False)
         _same(capi.csr_multiply(zero, m2), ref.csr_op("mul", zero, m2), "all-zero values")
+        _same(capi.csr_multiply(zero, m2, reduced=True), ref.csr_op("mul_reduced", zero, m2),
+              "all-zero values, reduced")
 
 
 def test_stiffness_products_match_reference_and_time():
@@ -92,13 +100,17 @@ def test_stiffness_products_match_reference_and_time():
                   f"trip) {(t2 - t1) * 1e3:.1f} ms", flush=True)
             _same(mine, theirs, ("stiffness product", n))
             _same(capi.csr_transpose(m), ref.csr_op("transpose", m), ("stiffness transpose", n))
+            if n <= 16:
+                _same(capi.csr_multiply(m, m, reduced=True), ref.csr_op("mul_reduced", m, m),
+                      ("stiffness reduced product", n))
 
 
 def test_dropin_product_takes_the_gpu_for_large_products():
-    """The C++ drop-in's MatrixCSRMultiplication hands products of at least
-    2^19 entries to sfa_MatrixCSRMultiplication: on the 16^3 stiffness
-    squared (~1.1 M products) its arrays are still the reference's, bit for
-    bit, as they are below the threshold (8^3)."""
+    """The C++ drop-in's MatrixCSRMultiplication and
+    MatrixCSRMultiplication_reduced hand products of at least 2^19 pairs to
+    the GPU: on the 16^3 stiffness squared (~1.1 M products) their arrays are
+    still the reference's, bit for bit, as they are below the threshold
+    (8^3)."""
     from oracle.ref import RefRunner
     from test_dropin_cpu import dropin_op
 
@@ -109,3 +121,5 @@ def test_dropin_product_takes_the_gpu_for_large_products():
             products = int(sum(np.diff(s.A)[s.C]))
             print(f"case: drop-in product 3D n={n}, {products} products", flush=True)
             _same(dropin_op("mul", m, m), ref.csr_op("mul", m, m), ("drop-in product", n))
+            _same(dropin_op("mul_reduced", m, m), ref.csr_op("mul_reduced", m, m),
+                  ("drop-in reduced product", n))
