@@ -1249,6 +1249,85 @@ namespace {
 // and residual and resets its state).  SFA_DROPIN_CACHE sets how many are
 // kept (default 4, 0: none).  A System is used by one call at a time; a
 // call that finds its match busy builds another.
+// The device parts of the last few matrices applied through
+// sfa_MatrixCSR_apply (MatrixCSR::apply of the drop-in), each with host copies
+// of its arrays: a call whose arrays hold the same bytes as a kept entry's
+// reuses its part, so repeated applies of one matrix skip the SELL build and
+// the upload.  A call holds the lock for its whole apply; the apply waits for
+// its result, so nothing is queued on a part when it is evicted.
+struct ApplyCache {
+    struct Entry {
+        int device = 0, rows = 0, cols = 0;
+        std::vector<int> A, C;
+        std::vector<double> V;
+        sfa::DevPart part;
+        unsigned long long last_use = 0;
+        ~Entry() { part.release(); }
+    };
+    std::mutex mu;
+    std::vector<std::unique_ptr<Entry>> entries;
+    unsigned long long clock = 0;
+    int capacity = 4;
+
+    ApplyCache()
+    {
+        if (const char* e = std::getenv("SFA_DROPIN_CACHE")) capacity = std::max(0, std::atoi(e));
+    }
+
+    static bool matches(const Entry& e, int device, int rows, int cols, const int* A,
+                        const int* C, const double* V)
+    {
+        if (e.device != device || e.rows != rows || e.cols != cols) return false;
+        const size_t nnz = (size_t)A[rows];
+        return e.A.size() == (size_t)rows + 1 && e.C.size() == nnz &&
+               sfa::bytes_equal(e.A.data(), A, sizeof(int) * ((size_t)rows + 1)) &&
+               (nnz == 0 || (sfa::bytes_equal(e.C.data(), C, sizeof(int) * nnz) &&
+                             sfa::bytes_equal(e.V.data(), V, sizeof(double) * nnz)));
+    }
+
+    // The part for these arrays: a kept one, or a new one kept in place of
+    // the least recently used (held by `fresh` alone when nothing is kept).
+    const sfa::DevPart& part(int device, int rows, int cols, const int* A, const int* C,
+                             const double* V, std::unique_ptr<Entry>* fresh)
+    {
+        for (auto& e : entries)
+            if (matches(*e, device, rows, cols, A, C, V)) {
+                e->last_use = ++clock;
+                return e->part;
+            }
+        std::unique_ptr<Entry> e(new Entry());
+        e->device = device;
+        e->rows = rows;
+        e->cols = cols;
+        sfa::SellPart p;
+        sfa::build_full_identity(p, rows, cols, A, C, V);
+        e->part.upload(p);
+        e->last_use = ++clock;
+        if (capacity == 0) {
+            *fresh = std::move(e);
+            return (*fresh)->part;
+        }
+        const size_t nnz = (size_t)A[rows];
+        e->A.assign(A, A + rows + 1);
+        e->C.assign(C, C + nnz);
+        e->V.assign(V, V + nnz);
+        if ((int)entries.size() >= capacity) {
+            auto lru = std::min_element(entries.begin(), entries.end(), [](const auto& a, const auto& b) {
+                return a->last_use < b->last_use;
+            });
+            entries.erase(lru);
+        }
+        entries.push_back(std::move(e));
+        return entries.back()->part;
+    }
+};
+
+ApplyCache& apply_cache()
+{
+    static ApplyCache* c = new ApplyCache();  // lives to the end of the process
+    return *c;
+}
+
 struct DropinCache {
     struct Entry {
         int device = 0, method = 0, n = 0, nc = 0;
@@ -1477,12 +1556,14 @@ int sfa_MatrixCSR_apply(int rows, int columns, const int* A, const int* C, const
         if (!err.empty()) throw Error{SFA_ERR_INVALID, err};
         if (!V || !dest || !src) throw Error{SFA_ERR_INVALID, "null pointer"};
         sfa::ensure_device();
-        sfa::SellPart p;
-        sfa::build_full_identity(p, rows, columns, A, C, V);
-        sfa::DevPart dp;
+        int device = 0;
+        SFA_HIP(hipGetDevice(&device));
+        ApplyCache& cache = apply_cache();
+        std::lock_guard<std::mutex> lock(cache.mu);
+        std::unique_ptr<ApplyCache::Entry> fresh;
+        const sfa::DevPart& dp = cache.part(device, rows, columns, A, C, V, &fresh);
         double *dsrc = nullptr, *ddst = nullptr;
         try {
-            dp.upload(p);
             dsrc = sfa::dev_alloc<double>(columns + 1);
             ddst = sfa::dev_alloc<double>(rows);
             SFA_HIP(hipMemcpy(dsrc, src, columns * sizeof(double), hipMemcpyHostToDevice));
@@ -1493,12 +1574,10 @@ int sfa_MatrixCSR_apply(int rows, int columns, const int* A, const int* C, const
             SFA_HIP(hipMemcpy(dest, ddst, rows * sizeof(double), hipMemcpyDeviceToHost));
         } catch (...) {
             (void)hipStreamSynchronize(nullptr);  // the kernel may still be queued
-            dp.release();
             sfa::dev_free(dsrc);
             sfa::dev_free(ddst);
             throw;
         }
-        dp.release();
         sfa::dev_free(dsrc);
         sfa::dev_free(ddst);
     });

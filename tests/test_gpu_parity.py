@@ -121,6 +121,29 @@ def test_apply_bitexact():
             assert np.array_equal(got, want), system.name
 
 
+def test_apply_keeps_parts_and_sees_changes():
+    """sfa_MatrixCSR_apply keeps the device parts of recent matrices: applies
+    that alternate between matrices give each one's product with the oracle's
+    bits, and a matrix changed in place (one value near the end of 13.7 MB of
+    values, compared by several host threads) gives the new matrix's."""
+    small, large = S.random_spd(300, 0.05, 3), S.poisson3d_p1_kuhn(48)
+    xs, xl = S.uniform_pm1(1, small.n), S.uniform_pm1(2, large.n)
+    print(f"case: apply cache, n={small.n} and n={large.n}", flush=True)
+    for _ in range(3):
+        for s, x in ((small, xs), (large, xl)):
+            got = capi.csr_apply(s.n, s.n, s.A, s.C, s.V, x)
+            assert np.array_equal(got, oracle.csr_apply(s.n, s.A, s.C, s.V, x)), s.name
+    for s, x in ((small, xs), (large, xl)):
+        s.V[-3] *= 2.0
+        try:
+            got = capi.csr_apply(s.n, s.n, s.A, s.C, s.V, x)
+            assert np.array_equal(got, oracle.csr_apply(s.n, s.A, s.C, s.V, x)), s.name
+        finally:
+            s.V[-3] /= 2.0
+        assert np.array_equal(capi.csr_apply(s.n, s.n, s.A, s.C, s.V, x),
+                              oracle.csr_apply(s.n, s.A, s.C, s.V, x)), s.name
+
+
 def test_apply_rectangular_kat():
     """tests/sparse/sparsematrix.cpp:51-67: [[1,2,3],[4,5,6]] x [13,17,19] = [104, 251]."""
     A = np.array([0, 3, 6], dtype=np.int32)
