@@ -182,47 +182,131 @@ __global__ void __launch_bounds__(kRowBlock) k_merge_columns(int rows, const lon
 // one of its own column adds that entry's value to itself and zeroes it;
 // then zeros are moved to the end, other entries keeping their order; the
 // row keeps its leading nonzero entries.  One thread per row.
+//
+// Each wavefront's 64 rows are first copied to scratch interleaved by lane
+// (entry k of lane l's row at wofs + 64 k + l), so that the sort's accesses,
+// which all lanes make at the same step, are one cache line per 8 lanes
+// instead of one per lane.  Step j of the sort's inner loop writes position j
+// only, so the next kAhead positions are loaded together and then taken in
+// order: the same additions in the same order.  The reference moves the
+// zeros with bubble passes that never let two nonzero entries cross, and
+// runs enough of them to finish; the entries it keeps are therefore the
+// row's nonzero entries in order, which one pass gathers back into the row.
 __global__ void __launch_bounds__(kRowBlock) k_sort_compress(int rows, const long long* __restrict__ P,
+                                                             const long long* __restrict__ wofs,
                                                              int* __restrict__ C,
                                                              double* __restrict__ V,
+                                                             int* __restrict__ sC,
+                                                             double* __restrict__ sV,
                                                              int* __restrict__ kept)
 {
+    constexpr int kAhead = 8;
+    const int r = blockIdx.x * kRowBlock + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const long long w = (long long)blockIdx.x * (kRowBlock / 64) + (threadIdx.x >> 6);
+    if (r >= rows) return;
+    const long long b = P[r];
+    const int n = (int)(P[r + 1] - b);
+    int* c = sC + wofs[w] + lane;
+    double* v = sV + wofs[w] + lane;
+    for (int k = 0; k < n; k++) {
+        c[64 * (long long)k] = C[b + k];
+        v[64 * (long long)k] = V[b + k];
+    }
+    for (int i = 0; i < n; i++) {
+        int ci = c[64 * (long long)i];
+        double vi = v[64 * (long long)i];
+        for (int j0 = i + 1; j0 < n; j0 += kAhead) {
+            int cj[kAhead];
+            double vj[kAhead];
+#pragma unroll
+            for (int u = 0; u < kAhead; u++) {
+                if (j0 + u < n) {
+                    cj[u] = c[64 * (long long)(j0 + u)];
+                    vj[u] = v[64 * (long long)(j0 + u)];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < kAhead; u++) {
+                if (j0 + u >= n) break;
+                const long long at = 64 * (long long)(j0 + u);
+                if (ci == cj[u]) {
+                    vi += vj[u];
+                    v[at] = 0.;
+                } else if (ci > cj[u]) {
+                    c[at] = ci;
+                    v[at] = vi;
+                    ci = cj[u];
+                    vi = vj[u];
+                }
+            }
+        }
+        c[64 * (long long)i] = ci;
+        v[64 * (long long)i] = vi;
+    }
+    int k = 0;
+    for (int i = 0; i < n; i++) {
+        const double x = v[64 * (long long)i];
+        if (x != 0.) {
+            C[b + k] = c[64 * (long long)i];
+            V[b + k] = x;
+            k++;
+        }
+    }
+    kept[r] = k;
+}
+
+// The same compression in place, for rows too uneven in length to share a
+// wavefront's scratch (see compress_rows).
+__global__ void __launch_bounds__(kRowBlock) k_sort_compress_in_place(int rows,
+                                                                      const long long* __restrict__ P,
+                                                                      int* __restrict__ C,
+                                                                      double* __restrict__ V,
+                                                                      int* __restrict__ kept)
+{
+    constexpr int kAhead = 8;
     const int r = blockIdx.x * kRowBlock + threadIdx.x;
     if (r >= rows) return;
     const long long b = P[r], e = P[r + 1];
     for (long long i = b; i < e; i++) {
         int ci = C[i];
         double vi = V[i];
-        for (long long j = i + 1; j < e; j++) {
-            const int cj = C[j];
-            if (ci == cj) {
-                vi += V[j];
-                V[j] = 0.;
-                continue;
+        for (long long j0 = i + 1; j0 < e; j0 += kAhead) {
+            int cj[kAhead];
+            double vj[kAhead];
+#pragma unroll
+            for (int u = 0; u < kAhead; u++) {
+                if (j0 + u < e) {
+                    cj[u] = C[j0 + u];
+                    vj[u] = V[j0 + u];
+                }
             }
-            if (ci > cj) {
-                C[j] = ci;
-                const double vj = V[j];
-                V[j] = vi;
-                ci = cj;
-                vi = vj;
+#pragma unroll
+            for (int u = 0; u < kAhead; u++) {
+                if (j0 + u >= e) break;
+                if (ci == cj[u]) {
+                    vi += vj[u];
+                    V[j0 + u] = 0.;
+                } else if (ci > cj[u]) {
+                    C[j0 + u] = ci;
+                    V[j0 + u] = vi;
+                    ci = cj[u];
+                    vi = vj[u];
+                }
             }
         }
         C[i] = ci;
         V[i] = vi;
     }
-    for (long long i = b; i < e; i++)
-        for (long long j = b + 1; j < e; j++)
-            if (V[j - 1] == 0. && V[j] != 0.) {
-                const int c = C[j - 1];
-                C[j - 1] = C[j];
-                C[j] = c;
-                const double v = V[j - 1];
-                V[j - 1] = V[j];
-                V[j] = v;
-            }
     long long k = b;
-    while (k < e && V[k] != 0.) k++;
+    for (long long i = b; i < e; i++) {
+        const double x = V[i];
+        if (x != 0.) {
+            C[k] = C[i];
+            V[k] = x;
+            k++;
+        }
+    }
     kept[r] = (int)(k - b);
 }
 
@@ -288,9 +372,29 @@ void compress_rows(int rows, int cols, const std::vector<long long>& hP, int* dC
 {
     long long* dP = upload(buf, hP.data(), hP.size());
     int* dkept = buf.get<int>(rows);
-    if (rows > 0)
-        launch(k_sort_compress, dim3(blocks_for(rows, kRowBlock)), dim3(kRowBlock), nullptr, rows,
-               (const long long*)dP, dC, dV, dkept);
+    if (rows > 0) {
+        // scratch for each wavefront's 64 rows, interleaved: 64 x its longest row
+        const long long nwaves = ((long long)rows + 63) / 64;
+        std::vector<long long> wofs(nwaves + 1, 0);
+        for (long long w = 0; w < nwaves; w++) {
+            long long widest = 0;
+            for (long long r = w * 64; r < std::min<long long>(rows, (w + 1) * 64); r++)
+                widest = std::max(widest, hP[r + 1] - hP[r]);
+            wofs[w + 1] = wofs[w] + 64 * widest;
+        }
+        // rows of very different lengths in one wavefront would make the
+        // scratch many times the entries: those are compressed in place
+        if (wofs[nwaves] <= 4 * hP[rows] + 64 * 1024) {
+            long long* dwofs = upload(buf, wofs.data(), wofs.size());
+            int* sC = buf.get<int>((size_t)wofs[nwaves]);
+            double* sV = buf.get<double>((size_t)wofs[nwaves]);
+            launch(k_sort_compress, dim3(blocks_for(rows, kRowBlock)), dim3(kRowBlock), nullptr,
+                   rows, (const long long*)dP, (const long long*)dwofs, dC, dV, sC, sV, dkept);
+        } else {
+            launch(k_sort_compress_in_place, dim3(blocks_for(rows, kRowBlock)), dim3(kRowBlock),
+                   nullptr, rows, (const long long*)dP, dC, dV, dkept);
+        }
+    }
     std::vector<int> kept(rows);
     if (rows > 0) SFA_HIP(hipMemcpy(kept.data(), dkept, rows * sizeof(int), hipMemcpyDeviceToHost));
     out->rows = rows;

@@ -66,6 +66,20 @@ def test_product_and_transpose_match_reference_bitwise():
                       what)
                 what = ("transpose", rows, inner, unsorted)
                 _same(capi.csr_transpose(m1), ref.csr_op("transpose", m1), what)
+        # one long row among short ones: too uneven to share a wavefront's
+        # interleaved scratch, so its rows are compressed in place
+        cols_l = rng.integers(0, 150, 600)
+        A_l = np.concatenate([[0], 600 + np.arange(200)]).astype(np.int32)
+        C_l = np.concatenate([cols_l, rng.integers(0, 150, 199)]).astype(np.int32)
+        V_l = rng.standard_normal(C_l.size)
+        V_l[rng.random(C_l.size) < 0.3] = rng.integers(-2, 3, 1)[0]
+        uneven = (200, 150, A_l, C_l, V_l)
+        m2 = _random_csr(rng, 150, 180, 12, True)
+        for reduced, op in ((False, "mul"), (True, "mul_reduced")):
+            _same(capi.csr_multiply(uneven, m2, reduced=reduced), ref.csr_op(op, uneven, m2),
+                  ("uneven rows", op))
+        _same(capi.csr_transpose(capi.csr_multiply(uneven, m2)),
+              ref.csr_op("transpose", ref.csr_op("mul", uneven, m2)), "uneven rows, transpose")
         # no entries at all, and a product with no surviving entry
         empty = (4, 3, np.zeros(5, np.int32), np.zeros(0, np.int32), np.zeros(0))
         m = _random_csr(rng, 3, 5, 4, True)
