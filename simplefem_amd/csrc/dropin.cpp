@@ -575,15 +575,28 @@ MatrixCSR MatrixCSRAddition(const MatrixCSR& m1, const MatrixCSR& m2, Float s1, 
 // Products of at least this many pairs of entries run on the GPU
 // (sfa_MatrixCSRMultiplication, sfa_MatrixCSRMultiplication_reduced): the
 // same products and the same compression, each row's run by one GPU thread,
-// so the result is the same array for array.
+// so the result is the same array for array.  The compression's work grows
+// with the square of a row's product count and one GPU thread takes a step
+// many times slower than a host core, so the GPU pays only when that work is
+// spread over many rows: the sum over rows of (products)^2 must be at least
+// kSpread times the largest row's.
 constexpr long long kDeviceProducts = 1LL << 19;
+constexpr double kSpread = 64.;
 
-static long long product_pairs(const MatrixCSR& m1, const MatrixCSR& m2)
+static bool product_on_device(const MatrixCSR& m1, const MatrixCSR& m2)
 {
     const int *A1 = m1.getA(), *C1 = m1.getC(), *A2 = m2.getA();
     long long products = 0;
-    for (int i = 0; i < A1[m1.getdimout()]; i++) products += A2[C1[i] + 1] - A2[C1[i]];
-    return products;
+    double sum_sq = 0., max_sq = 0.;
+    for (int r = 0; r < m1.getdimout(); r++) {
+        long long row = 0;
+        for (int i = A1[r]; i < A1[r + 1]; i++) row += A2[C1[i] + 1] - A2[C1[i]];
+        products += row;
+        const double sq = (double)row * (double)row;
+        sum_sq += sq;
+        max_sq = std::max(max_sq, sq);
+    }
+    return products >= kDeviceProducts && sum_sq >= kSpread * max_sq;
 }
 
 static MatrixCSR device_product(const MatrixCSR& m1, const MatrixCSR& m2, bool reduced,
@@ -612,8 +625,7 @@ MatrixCSR MatrixCSRMultiplication(const MatrixCSR& m1, const MatrixCSR& m2)
     const int rows = m1.getdimout();
     const int *A1 = m1.getA(), *C1 = m1.getC(), *A2 = m2.getA(), *C2 = m2.getC();
     const Float *V1 = m1.getV(), *V2 = m2.getV();
-    if (product_pairs(m1, m2) >= kDeviceProducts)
-        return device_product(m1, m2, false, "MatrixCSRMultiplication");
+    if (product_on_device(m1, m2)) return device_product(m1, m2, false, "MatrixCSRMultiplication");
     std::vector<int> A(rows + 1, 0);
     for (int r = 0; r < rows; r++) {
         int count = 0;
@@ -640,7 +652,7 @@ MatrixCSR MatrixCSRMultiplication(const MatrixCSR& m1, const MatrixCSR& m2)
 MatrixCSR MatrixCSRMultiplication_reduced(const MatrixCSR& m1, const MatrixCSR& m2)
 {
     require_true(m1.getdimin() == m2.getdimout(), "MatrixCSRMultiplication_reduced dimensions");
-    if (product_pairs(m1, m2) >= kDeviceProducts)
+    if (product_on_device(m1, m2))
         return device_product(m1, m2, true, "MatrixCSRMultiplication_reduced");
     const int rows = m1.getdimout();
     const int *A1 = m1.getA(), *C1 = m1.getC(), *A2 = m2.getA(), *C2 = m2.getC();
